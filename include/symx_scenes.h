@@ -1,0 +1,117 @@
+/*
+ * symx_scenes.h - C ABI of the example problems built on the C++ front end
+ * (the counterpart of SymX's examples/NeoHookean.cpp, examples/rubber_extrusion.cpp
+ * and examples/dynamic_elasticity_with_contact.cpp). Used by bench.py and the
+ * parity tests: each scene defines its potentials through
+ * GlobalPotential::add_potential and solves with NewtonsMethod, so every call
+ * here exercises the same path a SymX user would.
+ */
+#ifndef SYMX_SCENES_H
+#define SYMX_SCENES_H
+
+#include <stdint.h>
+
+#include "symx_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct symx_scene_s* symx_scene;
+
+/* Tet4 elastic block with backward-Euler inertia and a penalty Dirichlet face. */
+typedef struct symx_tet4_block_params {
+    int32_t cells[3];        /* hex cells per axis (6 tets each) */
+    double size[3];          /* extent; the block spans [0, size] */
+    double youngs_modulus;
+    double poissons_ratio;
+    double density;
+    double gravity[3];
+    double dt;               /* <= 0: quasistatic (no inertia potential) */
+    double noise;            /* uniform perturbation amplitude of free vertices */
+    uint64_t seed;
+    double bc_stiffness;     /* penalty 0.5 k |x - X|^2 on the fixed face */
+    int32_t fixed_axis;      /* 0,1,2: vertices with coordinate == 0 on this axis are fixed; -1: none */
+    int32_t material;        /* 0: Neo-Hookean, 1: stable Neo-Hookean */
+    int32_t log_mu_field;    /* 1: per-tet mu = exp(log_mu[e]) parameters (see symx_scene_set_log_mu) */
+} symx_tet4_block_params;
+
+/* Newton settings exposed to callers (subset of NewtonSettings, solver_utils.h:173-258). */
+typedef struct symx_newton_params {
+    int32_t precision;               /* SYMX_HIP_FP64 or SYMX_HIP_MIXED */
+    int32_t projection_mode;         /* 0 Newton, 1 ProjectedNewton, 2 ProjectOnDemand */
+    int32_t use_mirroring;
+    int32_t max_iterations;
+    double residual_tolerance_abs;
+    double residual_tolerance_rel;
+    double cg_rel_tolerance;
+    double cg_abs_tolerance;
+    int32_t cg_max_iterations;
+    double projection_eps;
+    double step_tolerance;
+} symx_newton_params;
+
+typedef struct symx_solve_stats {
+    int32_t result;              /* symx::SolverReturn */
+    int32_t newton_iterations;
+    int32_t cg_iterations;
+    int32_t ls_bt_iterations;
+    int32_t n_refinements;
+    double final_energy;
+    double final_residual;
+    double first_residual;
+    double seconds;
+    double seconds_eval;         /* element evaluation (PGH) */
+    double seconds_project;
+    double seconds_assembly;
+    double seconds_linear;
+    double seconds_energy;       /* line-search energies */
+} symx_solve_stats;
+
+void symx_newton_params_default(symx_newton_params* p);
+void symx_tet4_block_params_default(symx_tet4_block_params* p);
+
+int symx_scene_tet4_block_create(const symx_tet4_block_params* p, const symx_newton_params* np, symx_scene* out);
+void symx_scene_destroy(symx_scene s);
+const char* symx_scene_last_error(void);
+
+int symx_scene_n_vertices(symx_scene s, int32_t* n);
+int symx_scene_n_elements(symx_scene s, int32_t* n);
+/* connectivity [n_elem][4] */
+int symx_scene_connectivity(symx_scene s, int32_t* conn);
+/* positions (DoFs), previous positions x0, velocities v0, rest positions, masses */
+int symx_scene_get_array(symx_scene s, const char* name, double* out);
+int symx_scene_set_array(symx_scene s, const char* name, const double* in);
+/* per-element log(mu) parameters (scenes created with log_mu_field = 1) */
+int symx_scene_set_log_mu(symx_scene s, const double* log_mu);
+int symx_scene_set_newton_params(symx_scene s, const symx_newton_params* np);
+/* Newton solve of the current time step (does not advance time). */
+int symx_scene_solve(symx_scene s, symx_solve_stats* st);
+/* Solve then advance: v = (x - x0)/dt, x0 = x. */
+int symx_scene_step(symx_scene s, symx_solve_stats* st);
+/* Device problem of the scene, for direct symx_hip_* calls. */
+symx_hip_problem symx_scene_problem(symx_scene s);
+/* Generated HIP source of potential i (NUL-terminated copy into buf). */
+int symx_scene_source(symx_scene s, int32_t i, char* buf, int64_t cap, int64_t* len);
+/* SSA operation counts of potential i: P, PG, PGH kernels. */
+int symx_scene_op_counts(symx_scene s, int32_t i, int64_t* ops3);
+double symx_scene_compile_seconds(symx_scene s);
+
+/* ---- host-only helpers (no GPU): symbolic path checks ----------------------- */
+/* Evaluate E, gradient and Hessian of the Neo-Hookean Tet4 element exactly as a
+ * scene builds it (fem_integrator + neohookean energy density), with the host
+ * reference evaluator of the expression graph. material: 0 NH, 1 stable NH
+ * (then lambda, mu are read as E and nu). */
+int symx_host_tet4_element(int32_t material, const double* x12, const double* X12, double lambda, double mu,
+                           double* E, double* g12, double* H144);
+/* Point-triangle signed distance (sign * (p - a).n) and its gradient and Hessian
+ * w.r.t. (a, b, c, p): in[13] = a, b, c, p, sign; out[157] = value, grad[12], H[144]. */
+int symx_host_point_triangle(const double* in13, double* out157);
+/* Generate the HIP module of the Tet4 block scene's elastic potential and
+ * compile it with hiprtc for gfx950 (no GPU needed); returns code size. */
+int symx_host_jit_tet4(const char* cache_dir, int32_t material, int64_t* code_size, int64_t* n_ops_pgh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

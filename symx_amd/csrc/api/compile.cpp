@@ -1,0 +1,352 @@
+#include "compile.h"
+
+#include <cinttypes>
+#include <cstdio>
+#include <cstring>
+#include <sstream>
+#include <stdexcept>
+
+namespace symx
+{
+    // ---- Element --------------------------------------------------------------------
+    Index Element::operator[](int i) const
+    {
+        if (i < 0 || i >= size) throw std::runtime_error("symx: Element index out of range");
+        return Index{ i };
+    }
+    Index Element::operator[](const std::string& label) const
+    {
+        for (size_t i = 0; i < labels.size(); i++) {
+            if (labels[i] == label) return Index{ (int)i };
+        }
+        throw std::runtime_error("symx: Element has no label '" + label + "'");
+    }
+    std::vector<Index> Element::slice(int begin, int end) const
+    {
+        std::vector<Index> r;
+        for (int i = begin; i < end; i++) r.push_back((*this)[i]);
+        return r;
+    }
+    std::vector<Index> Element::all() const { return slice(0, size); }
+
+    // ---- MappedWorkspace --------------------------------------------------------------
+    Scalar MappedWorkspace<double>::make_scalar(std::function<std::uintptr_t()> id, std::function<const double*()> data)
+    {
+        maps.push_back(DataMap{ id, data, []() { return 1; }, 1, -1, ws.get_n_symbols() });
+        return ws.make_scalar();
+    }
+    Scalar MappedWorkspace<double>::make_scalar(const double& v)
+    {
+        return make_scalar([&v]() { return reinterpret_cast<std::uintptr_t>(&v); }, [&v]() { return &v; });
+    }
+    Vector MappedWorkspace<double>::make_vector(std::function<std::uintptr_t()> id, std::function<const double*()> data, int32_t stride)
+    {
+        maps.push_back(DataMap{ id, data, []() { return 1; }, stride, -1, ws.get_n_symbols() });
+        return ws.make_vector(stride);
+    }
+    Scalar MappedWorkspace<double>::make_scalar(std::function<std::uintptr_t()> id, std::function<const double*()> data,
+                                                std::function<int32_t()> n, const Index& idx)
+    {
+        if (idx.idx < 0 || idx.idx >= conn.stride) throw std::runtime_error("symx: connectivity index out of range");
+        maps.push_back(DataMap{ id, data, n, 1, idx.idx, ws.get_n_symbols() });
+        return ws.make_scalar();
+    }
+    Vector MappedWorkspace<double>::make_vector(std::function<std::uintptr_t()> id, std::function<const double*()> data,
+                                                std::function<int32_t()> n, int32_t stride, const Index& idx)
+    {
+        if (idx.idx < 0 || idx.idx >= conn.stride) throw std::runtime_error("symx: connectivity index out of range");
+        maps.push_back(DataMap{ id, data, n, stride, idx.idx, ws.get_n_symbols() });
+        return ws.make_vector(stride);
+    }
+    Vector MappedWorkspace<double>::make_summation_vector(const double* data, int32_t stride, int32_t n_iterations)
+    {
+        if (summation.n_iterations > 0) throw std::runtime_error("symx: only one summation per potential is supported");
+        if (stride <= 0 || n_iterations <= 0) throw std::runtime_error("symx: empty summation table");
+        summation.data.assign(data, data + (size_t)stride * n_iterations);
+        summation.stride = stride;
+        summation.n_iterations = n_iterations;
+        summation.first_symbol_idx = ws.get_n_symbols();
+        return ws.make_vector(stride);
+    }
+    std::vector<int32_t> MappedWorkspace<double>::maps_of_container(std::uintptr_t id) const
+    {
+        std::vector<int32_t> r;
+        for (size_t m = 0; m < maps.size(); m++) {
+            if (maps[m].id() == id) r.push_back((int32_t)m);
+        }
+        return r;
+    }
+
+    // ---- emitter ----------------------------------------------------------------------
+    namespace
+    {
+        std::string literal(double v)
+        {
+            if (v == 0.0) return "0.0";
+            char buf[64];
+            std::snprintf(buf, sizeof(buf), "%a", v);
+            return std::string("(") + buf + ")";
+        }
+
+        // Mark nodes reachable from the roots.
+        std::vector<uint8_t> reachable(const Graph& g, const std::vector<int32_t>& roots)
+        {
+            std::vector<uint8_t> mark(g.size(), 0);
+            std::vector<int32_t> stack(roots.begin(), roots.end());
+            while (!stack.empty()) {
+                const int32_t id = stack.back();
+                stack.pop_back();
+                if (id < 0 || mark[id]) continue;
+                mark[id] = 1;
+                const Node& n = g.node(id);
+                switch (n.op) {
+                case Op::Zero: case Op::One: case Op::Const: case Op::Symbol: break;
+                case Op::PowN: stack.push_back(n.a); break;
+                case Op::Branch: stack.push_back(n.a); stack.push_back(n.b); stack.push_back(n.c); break;
+                default:
+                    stack.push_back(n.a);
+                    if (n.b >= 0) stack.push_back(n.b);
+                }
+            }
+            return mark;
+        }
+
+        std::string ref(const Graph& g, int32_t id)
+        {
+            const Node& n = g.node(id);
+            switch (n.op) {
+            case Op::Zero: return "0.0";
+            case Op::One: return "1.0";
+            case Op::Const: return literal(n.value);
+            case Op::Symbol: return "s" + std::to_string(n.a);
+            default: return "t" + std::to_string(id);
+            }
+        }
+
+        std::string expr_of(const Graph& g, int32_t id)
+        {
+            const Node& n = g.node(id);
+            const std::string a = n.a >= 0 ? ref(g, n.a) : "";
+            switch (n.op) {
+            case Op::Add: return a + " + " + ref(g, n.b);
+            case Op::Sub: return a + " - " + ref(g, n.b);
+            case Op::Mul: return a + " * " + ref(g, n.b);
+            case Op::Recip: return "1.0 / " + a;
+            case Op::PowN:
+                if (n.b == 2) return a + " * " + a;
+                if (n.b == 3) return a + " * " + a + " * " + a;
+                if (n.b == -1) return "1.0 / " + a;
+                return "symx_powi(" + a + ", " + std::to_string(n.b) + ")";
+            case Op::PowF: return "pow(" + a + ", " + ref(g, n.b) + ")";
+            case Op::Sqrt: return "sqrt(" + a + ")";
+            case Op::Ln: return "symx_ln(" + a + ")";
+            case Op::Log10: return "symx_log10(" + a + ")";
+            case Op::Exp: return "exp(" + a + ")";
+            case Op::Sin: return "sin(" + a + ")";
+            case Op::Cos: return "cos(" + a + ")";
+            case Op::Tan: return "tan(" + a + ")";
+            case Op::ASin: return "asin(" + a + ")";
+            case Op::ACos: return "acos(" + a + ")";
+            case Op::ATan: return "atan(" + a + ")";
+            case Op::Branch: return "(" + ref(g, n.c) + " > 0.0) ? " + a + " : " + ref(g, n.b);
+            default: throw std::runtime_error("symx: emitter met a non-operation node");
+            }
+        }
+
+        bool is_op(const Node& n) { return !(n.op == Op::Zero || n.op == Op::One || n.op == Op::Const || n.op == Op::Symbol); }
+
+        // Statements for all reachable operation nodes, in topological (id) order.
+        void emit_body(std::ostringstream& os, const Graph& g, const std::vector<uint8_t>& mark, const char* indent)
+        {
+            for (int32_t id = 0; id < g.size(); id++) {
+                if (!mark[id]) continue;
+                const Node& n = g.node(id);
+                if (!is_op(n)) continue;
+                os << indent << "const double t" << id << " = " << expr_of(g, id) << ";\n";
+            }
+        }
+
+        struct SymbolUse
+        {
+            std::vector<uint8_t> used;   // per symbol
+            bool uses_summation = false;
+        };
+
+        SymbolUse symbol_use(const Graph& g, const std::vector<uint8_t>& mark, const SummationMap& sum)
+        {
+            SymbolUse u;
+            u.used.assign(g.n_symbols(), 0);
+            for (int32_t id = 0; id < g.size(); id++) {
+                if (mark[id] && g.node(id).op == Op::Symbol) {
+                    const int32_t s = g.node(id).a;
+                    u.used[s] = 1;
+                    if (sum.n_iterations > 0 && s >= sum.first_symbol_idx && s < sum.first_symbol_idx + sum.stride) u.uses_summation = true;
+                }
+            }
+            return u;
+        }
+
+        void emit_gather(std::ostringstream& os, const Graph& g, const std::vector<DataMap>& maps, const SymbolUse& use,
+                         const SummationMap& sum)
+        {
+            std::vector<uint8_t> covered(g.n_symbols(), 0);
+            std::vector<uint8_t> col_loaded(64, 0);
+            for (size_t m = 0; m < maps.size(); m++) {
+                const DataMap& dm = maps[m];
+                bool any = false;
+                for (int k = 0; k < dm.stride; k++) {
+                    covered[dm.first_symbol_idx + k] = 1;
+                    any = any || use.used[dm.first_symbol_idx + k];
+                }
+                if (!any) continue;
+                std::string base;
+                if (dm.connectivity_index >= 0) {
+                    const int c = dm.connectivity_index;
+                    if (c < 64 && !col_loaded[c]) {
+                        os << "        const long long i" << c << " = (long long)p.conn[" << c << " * (long long)p.n_elem + e];\n";
+                        col_loaded[c] = 1;
+                    }
+                    base = "i" + std::to_string(c) + " * " + std::to_string(dm.stride) + " + ";
+                }
+                for (int k = 0; k < dm.stride; k++) {
+                    const int s = dm.first_symbol_idx + k;
+                    if (!use.used[s]) continue;
+                    os << "        const double s" << s << " = p.maps[" << m << "][" << base << k << "];\n";
+                }
+            }
+            if (sum.n_iterations > 0)
+                for (int k = 0; k < sum.stride; k++) covered[sum.first_symbol_idx + k] = 1;
+            for (int32_t s = 0; s < g.n_symbols(); s++) {
+                if (use.used[s] && !covered[s]) throw std::runtime_error("symx: expression uses a symbol that is not bound to data");
+            }
+        }
+    }
+
+    std::string emit_hip_module(const std::string& module_name, Graph& g, const std::vector<DataMap>& maps, const SummationMap& sum,
+                                const std::vector<KernelSpec>& kernels, const std::string& prelude)
+    {
+        if (maps.size() > 48) throw std::runtime_error("symx: too many data maps in one potential (max 48)");
+        std::ostringstream os;
+        os << prelude << "\n// ---- generated module: " << module_name << " ----\n";
+        const bool multi_q = sum.n_iterations > 1;
+        if (multi_q) {
+            os << "__constant__ double " << module_name << "_sum[" << sum.data.size() << "] = {";
+            for (size_t i = 0; i < sum.data.size(); i++) os << (i ? ", " : "") << literal(sum.data[i]);
+            os << "};\n";
+        }
+        for (const KernelSpec& ks : kernels) {
+            std::vector<int32_t> roots;
+            for (const Scalar& s : ks.outputs) roots.push_back(s.id);
+            const std::vector<uint8_t> mark = reachable(g, roots);
+            const SymbolUse use = symbol_use(g, mark, sum);
+            const int n = ks.n_dofs;
+            const int nb = n / 3;
+            const int n_ub = nb * (nb + 1) / 2;
+
+            os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.name << "(const symx_hip_eval_params p)\n{\n";
+            os << "    const int e = blockIdx.x * 256 + threadIdx.x;\n";
+            if (ks.kind != KernelKind::C) os << "    double e_out = 0.0;\n";
+            os << "    if (e < p.n_elem) {\n";
+            if (ks.kind != KernelKind::C) os << "        const bool act = (p.active == nullptr) || (p.active[e] != 0);\n";
+            emit_gather(os, g, maps, use, sum);
+            const std::string eg = "(long long)e * " + std::to_string(n);
+            const std::string eh = "(long long)e * " + std::to_string(n_ub * 9);
+            // zero outputs (inactive elements, and accumulation start for multi-point sums)
+            auto emit_zero = [&](const char* indent) {
+                if (ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH)
+                    for (int i = 0; i < n; i++) os << indent << "p.G[" << eg << " + " << i << "] = 0.0;\n";
+                if (ks.kind == KernelKind::PGH)
+                    for (int i = 0; i < n_ub * 9; i++) os << indent << "p.H[" << eh << " + " << i << "] = 0.0;\n";
+            };
+            if (ks.kind == KernelKind::C) {
+                if (use.uses_summation) throw std::runtime_error("symx: activity conditions may not depend on summation symbols");
+                emit_body(os, g, mark, "        ");
+                os << "        p.C_out[e] = (" << ref(g, ks.outputs[0].id) << " > 0.0) ? 1 : 0;\n";
+                os << "    }\n}\n";
+                continue;
+            }
+            os << "        if (act) {\n";
+            const char* in = "            ";
+            std::string acc = "=";
+            if (use.uses_summation) {
+                if (multi_q) {
+                    emit_zero(in);
+                    acc = "+=";
+                    os << in << "for (int q = 0; q < " << sum.n_iterations << "; ++q) {\n";
+                    for (int k = 0; k < sum.stride; k++) {
+                        const int s = sum.first_symbol_idx + k;
+                        if (use.used[s]) os << in << "    const double s" << s << " = " << module_name << "_sum[q * " << sum.stride << " + " << k << "];\n";
+                    }
+                }
+                else {
+                    for (int k = 0; k < sum.stride; k++) {
+                        const int s = sum.first_symbol_idx + k;
+                        if (use.used[s]) os << in << "const double s" << s << " = " << literal(sum.data[k]) << ";\n";
+                    }
+                }
+            }
+            const std::string body_in = (use.uses_summation && multi_q) ? "                " : in;
+            emit_body(os, g, mark, body_in.c_str());
+            os << body_in << "e_out += " << ref(g, ks.outputs[0].id) << ";\n";
+            if (ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH) {
+                for (int i = 0; i < n; i++) os << body_in << "p.G[" << eg << " + " << i << "] " << acc << " " << ref(g, ks.outputs[1 + i].id) << ";\n";
+            }
+            if (ks.kind == KernelKind::PGH) {
+                int ub = 0;
+                for (int a = 0; a < nb; a++) {
+                    for (int b = a; b < nb; b++) {
+                        for (int i = 0; i < 3; i++) {
+                            for (int j = 0; j < 3; j++) {
+                                const int row = 3 * a + i, col = 3 * b + j;
+                                os << body_in << "p.H[" << eh << " + " << (ub * 9 + 3 * i + j) << "] " << acc << " "
+                                   << ref(g, ks.outputs[1 + n + row * n + col].id) << ";\n";
+                            }
+                        }
+                        ub++;
+                    }
+                }
+            }
+            if (use.uses_summation && multi_q) os << in << "}\n";
+            os << "        }\n        else {\n";
+            emit_zero("            ");
+            os << "        }\n    }\n    symx_store_energy(e_out, p.E_partial);\n}\n";
+        }
+        return os.str();
+    }
+
+    std::string emit_hip_batch_function(const std::string& name, Graph& g, const std::vector<Scalar>& outputs, int32_t n_inputs)
+    {
+        std::vector<int32_t> roots;
+        for (const Scalar& s : outputs) roots.push_back(s.id);
+        const std::vector<uint8_t> mark = reachable(g, roots);
+        std::ostringstream os;
+        os << "__device__ __forceinline__ double symx_ln(double x) { return (x <= 0.0) ? -__builtin_inf() : log(x); }\n"
+           << "__device__ __forceinline__ double symx_log10(double x) { return (x <= 0.0) ? -__builtin_inf() : log10(x); }\n"
+           << "__device__ __forceinline__ double symx_powi(double x, int n) { const bool ng = n < 0; unsigned m = ng ? -n : n; double r = 1.0, b = x;"
+              " while (m) { if (m & 1u) r *= b; b *= b; m >>= 1; } return ng ? 1.0 / r : r; }\n";
+        os << "extern \"C\" __global__ void __launch_bounds__(256) " << name << "(const double* in, double* out, int n)\n{\n";
+        os << "    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;\n    if (i >= n) return;\n";
+        for (int32_t id = 0; id < g.size(); id++) {
+            if (mark[id] && g.node(id).op == Op::Symbol) {
+                const int s = g.node(id).a;
+                if (s >= n_inputs) throw std::runtime_error("symx: symbol outside the input range");
+                os << "    const double s" << s << " = in[i * " << n_inputs << " + " << s << "];\n";
+            }
+        }
+        emit_body(os, g, mark, "    ");
+        for (size_t k = 0; k < outputs.size(); k++) os << "    out[i * " << outputs.size() << " + " << k << "] = " << ref(g, outputs[k].id) << ";\n";
+        os << "}\n";
+        return os.str();
+    }
+
+    int64_t count_operations(Graph& g, const std::vector<Scalar>& outputs)
+    {
+        std::vector<int32_t> roots;
+        for (const Scalar& s : outputs) roots.push_back(s.id);
+        const std::vector<uint8_t> mark = reachable(g, roots);
+        int64_t c = 0;
+        for (int32_t id = 0; id < g.size(); id++)
+            if (mark[id] && is_op(g.node(id))) c++;
+        return c;
+    }
+}

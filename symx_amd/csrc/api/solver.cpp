@@ -1,0 +1,827 @@
+#include "solver.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+
+namespace symx
+{
+    namespace
+    {
+        void check(int rc, symx_hip_ctx ctx, const char* what)
+        {
+            if (rc != SYMX_HIP_OK) throw std::runtime_error(std::string("symx_hip: ") + what + ": " + symx_hip_last_error(ctx));
+        }
+        std::string identifier(const std::string& s)
+        {
+            std::string r;
+            for (char c : s) r += (std::isalnum((unsigned char)c) ? c : '_');
+            if (r.empty() || std::isdigit((unsigned char)r[0])) r = "p_" + r;
+            return r;
+        }
+    }
+
+    // ---- Logger / OutputSink ------------------------------------------------------------
+    Logger::Stats Logger::get_stats(const std::string& key) const
+    {
+        Stats s;
+        auto it = series.find(key);
+        if (it == series.end() || it->second.empty()) return s;
+        s.min = it->second[0];
+        s.max = it->second[0];
+        for (double v : it->second) {
+            s.total += v;
+            s.min = std::min(s.min, v);
+            s.max = std::max(s.max, v);
+        }
+        s.count = (int64_t)it->second.size();
+        s.avg = s.total / (double)s.count;
+        return s;
+    }
+    std::vector<std::string> Logger::get_timer_labels() const
+    {
+        std::vector<std::string> r;
+        for (auto& kv : timers) r.push_back(kv.first);
+        return r;
+    }
+    double Logger::get_timer_total(const std::string& label) const
+    {
+        auto it = timers.find(label);
+        return it == timers.end() ? 0.0 : it->second;
+    }
+    void OutputSink::print(const std::string& s, Verbosity v)
+    {
+        if ((int)v <= (int)verbosity && verbosity != Verbosity::None) std::cout << s << std::flush;
+    }
+    void OutputSink::print_with_new_line(const std::string& s, Verbosity v) { print("\n" + s, v); }
+
+    // ---- Context --------------------------------------------------------------------------
+    symx_hip_ctx Context::hip()
+    {
+        if (!hip_ctx) check(symx_hip_ctx_create(device, &hip_ctx), nullptr, "creating the HIP context");
+        return hip_ctx;
+    }
+    Context::~Context()
+    {
+        if (hip_ctx) symx_hip_ctx_destroy(hip_ctx);
+    }
+    std::string get_codegen_dir()
+    {
+        if (const char* d = std::getenv("SYMX_AMD_CODEGEN_DIR")) return d;
+        if (const char* h = std::getenv("HOME")) return std::string(h) + "/.cache/symx_amd";
+        return "/tmp/symx_amd_codegen";
+    }
+
+    const char* to_string(SolverReturn r)
+    {
+        switch (r) {
+        case SolverReturn::Successful: return "Successful";
+        case SolverReturn::Running: return "Running";
+        case SolverReturn::InvalidInitialState: return "InvalidInitialState";
+        case SolverReturn::TooManyIterations: return "TooManyIterations";
+        case SolverReturn::TooManyArmijoIterations: return "TooManyArmijoIterations";
+        case SolverReturn::LinearSystemSolveFailure: return "LinearSystemSolveFailure";
+        case SolverReturn::TooManyInvalidIntermediateIterations: return "TooManyInvalidIntermediateIterations";
+        case SolverReturn::StepDoesNotDescend: return "StepDoesNotDescend";
+        case SolverReturn::InvalidConvergedState: return "InvalidConvergedState";
+        }
+        return "?";
+    }
+
+    // ---- GlobalPotential (GlobalPotential.cpp) -------------------------------------------
+    void GlobalPotential::add_potential(const std::string& name, spMWS<double> mws, Element& elem, DefFunc f)
+    {
+        auto p = std::make_unique<Potential>();
+        p->name = name;
+        p->mws = mws;
+        p->expression = f(*mws, elem);
+        potentials.push_back(std::move(p));
+    }
+    void GlobalPotential::add_potential(const std::string& name, spMWS<double> mws, Element& elem, DefWCondFunc f)
+    {
+        auto p = std::make_unique<Potential>();
+        p->name = name;
+        p->mws = mws;
+        auto pc = f(*mws, elem);
+        p->expression = pc.first;
+        p->condition = pc.second;
+        p->has_condition = true;
+        potentials.push_back(std::move(p));
+    }
+    void GlobalPotential::add_potential(const std::string& name, std::function<const int32_t*()> data, std::function<int32_t()> n_elements,
+                                        int32_t stride, DefFunc f)
+    {
+        auto [mws, elem] = MappedWorkspace<double>::create(ConnectivityMap(data, n_elements, stride));
+        add_potential(name, mws, elem, f);
+    }
+    void GlobalPotential::add_potential(const std::string& name, const std::vector<int32_t>& arr, int32_t n, DefFunc f)
+    {
+        auto [mws, elem] = MappedWorkspace<double>::create(arr, n);
+        add_potential(name, mws, elem, f);
+    }
+    void GlobalPotential::add_potential(const std::string& name, const std::vector<int32_t>& arr, int32_t n, DefWCondFunc f)
+    {
+        auto [mws, elem] = MappedWorkspace<double>::create(arr, n);
+        add_potential(name, mws, elem, f);
+    }
+    void GlobalPotential::add_dof(std::function<std::uintptr_t()> id, std::function<double*()> data, std::function<int32_t()> flat_size,
+                                  const std::string& name)
+    {
+        dof_maps.push_back(DofSet{ id, data, flat_size, name });
+    }
+    const std::vector<int32_t>& GlobalPotential::get_dofs_offsets()
+    {
+        dof_offsets.assign(dof_maps.size() + 1, 0);
+        for (size_t i = 0; i < dof_maps.size(); i++) dof_offsets[i + 1] = dof_offsets[i] + dof_maps[i].flat_size();
+        return dof_offsets;
+    }
+    int32_t GlobalPotential::get_total_n_dofs() const
+    {
+        int32_t n = 0;
+        for (auto& d : dof_maps) n += d.flat_size();
+        return n;
+    }
+    void GlobalPotential::get_dofs(double* u) const
+    {
+        int32_t off = 0;
+        for (auto& d : dof_maps) {
+            const int32_t n = d.flat_size();
+            std::copy(d.data(), d.data() + n, u + off);
+            off += n;
+        }
+    }
+    void GlobalPotential::set_dofs(const double* u)
+    {
+        int32_t off = 0;
+        for (auto& d : dof_maps) {
+            const int32_t n = d.flat_size();
+            std::copy(u + off, u + off + n, d.data());
+            off += n;
+        }
+    }
+    void GlobalPotential::apply_dof_increment(const double* du)
+    {
+        int32_t off = 0;
+        for (auto& d : dof_maps) {
+            const int32_t n = d.flat_size();
+            double* x = d.data();
+            for (int32_t i = 0; i < n; i++) x[i] += du[off + i];
+            off += n;
+        }
+    }
+
+    // ---- DeviceGlobal (SecondOrderCompiledGlobal + SecondOrderCompiledPotential) ---------
+    DeviceGlobal::DeviceGlobal(spGlobalPotential G_, spContext ctx_) : G(G_), ctx(ctx_)
+    {
+        symx_hip_ctx hip = ctx->hip();
+        ndofs = G->get_total_n_dofs();
+        if (ndofs <= 0) throw std::runtime_error("symx: no degrees of freedom");
+        if (ndofs % 3 != 0) throw std::runtime_error("symx: ndofs must be divisible by 3");
+        const std::vector<int32_t> offsets = G->get_dofs_offsets();
+        check(symx_hip_problem_create(hip, ndofs, &prob), hip, "problem_create");
+        const std::string cache = ctx->compilation_directory.empty() ? get_codegen_dir() : ctx->compilation_directory;
+        const auto t0 = std::chrono::steady_clock::now();
+        int32_t pot_index = 0;
+        for (const auto& pp : G->get_potentials()) {
+            const Potential& P = *pp;
+            MappedWorkspace<double>& mws = *P.mws;
+            Graph* g = mws.workspace().graph();
+            const auto& maps = mws.get_maps();
+            // DoF identification: maps whose container is a DoF set, in set order then
+            // map order (SecondOrderCompiledPotential.cpp:10-33).
+            std::vector<Scalar> dofs;
+            std::vector<int32_t> dof_col, dof_off;
+            std::vector<int32_t> map_dof_set(maps.size(), -1);
+            for (int32_t s = 0; s < G->get_n_dof_sets(); s++) {
+                const std::uintptr_t id = G->get_dof_sets()[s].id();
+                for (int32_t m : mws.maps_of_container(id)) {
+                    const DataMap& dm = maps[m];
+                    if (dm.stride != 3)
+                        throw std::runtime_error("symx: expected DoF stride of 3, got " + std::to_string(dm.stride) + " in potential " + P.name);
+                    if (dm.connectivity_index < 0) throw std::runtime_error("symx: DoF data must be indexed by the connectivity (" + P.name + ")");
+                    for (int k = 0; k < 3; k++) dofs.push_back(Scalar(g, g->symbol_node(dm.first_symbol_idx + k)));
+                    dof_col.push_back(dm.connectivity_index);
+                    dof_off.push_back(offsets[s]);
+                    map_dof_set[m] = s;
+                }
+            }
+            if (dofs.empty()) throw std::runtime_error("symx: potential with zero DoFs: " + P.name);
+            const int32_t n = (int32_t)dofs.size();
+
+            DiffCache dc;
+            const Vector grad = gradient(P.expression, dofs, dc);
+            const Matrix hess = gradient(grad, dofs, true, dc);
+            const std::string base = identifier(P.name);
+            std::vector<KernelSpec> specs(3);
+            specs[0] = KernelSpec{ base + "_P", KernelKind::P, { P.expression }, n };
+            specs[1] = KernelSpec{ base + "_P__dP_du", KernelKind::PG, collect_scalars({ { P.expression }, grad.values() }), n };
+            specs[2] = KernelSpec{ base + "_P__dP_du__d2P_du2", KernelKind::PGH, collect_scalars({ { P.expression }, grad.values(), hess.values() }), n };
+            if (P.has_condition) specs.push_back(KernelSpec{ base + "_C", KernelKind::C, { P.condition }, n });
+            const std::string src = emit_hip_module(base, *g, maps, mws.get_summation(), specs, symx_hip_eval_prelude());
+
+            PotentialInfo info;
+            info.name = P.name;
+            info.n_dofs = n;
+            info.ops_P = count_operations(*g, specs[0].outputs);
+            info.ops_PG = count_operations(*g, specs[1].outputs);
+            info.ops_PGH = count_operations(*g, specs[2].outputs);
+            info.source = src;
+
+            symx_hip_module mod = nullptr;
+            char key[80];
+            int was_cached = 0;
+            size_t code_size = 0;
+            check(symx_hip_jit_compile(src.c_str(), "gfx950", cache.c_str(), key, &was_cached, &code_size), nullptr, ("JIT of " + P.name).c_str());
+            n_cached += was_cached;
+            check(symx_hip_module_load(hip, src.c_str(), cache.c_str(), &mod), hip, ("loading module of " + P.name).c_str());
+            modules.push_back(mod);
+
+            const int32_t n_elem = mws.conn.n_elements();
+            info.n_elem = n_elem;
+            std::vector<symx_hip_map_desc> mdesc(maps.size());
+            for (size_t m = 0; m < maps.size(); m++) {
+                const DataMap& dm = maps[m];
+                if (map_dof_set[m] >= 0) {
+                    mdesc[m].is_dof = 1;
+                    mdesc[m].dof_offset = offsets[map_dof_set[m]];
+                    mdesc[m].host = nullptr;
+                    mdesc[m].n_values = 0;
+                }
+                else {
+                    mdesc[m].is_dof = 0;
+                    mdesc[m].dof_offset = 0;
+                    mdesc[m].host = dm.data();
+                    mdesc[m].n_values = (int64_t)dm.n_elements() * dm.stride;
+                    host_maps.push_back(HostMap{ pot_index, (int32_t)m, dm.data, mdesc[m].n_values });
+                }
+            }
+            symx_hip_potential_desc d{};
+            d.module = mod;
+            d.kernel_P = specs[0].name.c_str();
+            d.kernel_PG = specs[1].name.c_str();
+            d.kernel_PGH = specs[2].name.c_str();
+            d.kernel_C = P.has_condition ? specs[3].name.c_str() : nullptr;
+            d.n_elem = n_elem;
+            d.conn_stride = mws.conn.stride;
+            d.conn = mws.conn.data();
+            d.n_dof_blocks = n / 3;
+            d.dof_conn_col = dof_col.data();
+            d.dof_set_offset = dof_off.data();
+            d.n_maps = (int32_t)mdesc.size();
+            d.maps = mdesc.data();
+            int32_t pid = -1;
+            check(symx_hip_problem_add_potential(prob, &d, &pid), hip, ("adding potential " + P.name).c_str());
+            infos.push_back(info);
+            pot_index++;
+        }
+        check(symx_hip_problem_finalize(prob), hip, "problem_finalize");
+        compile_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    DeviceGlobal::~DeviceGlobal()
+    {
+        if (prob) symx_hip_problem_destroy(prob);
+        for (auto m : modules) symx_hip_module_unload(m);
+    }
+
+    void DeviceGlobal::refresh_data()
+    {
+        for (auto& hm : host_maps) check(symx_hip_problem_set_map_data(prob, hm.pot, hm.map, hm.data(), hm.n), ctx->hip(), "refreshing data");
+    }
+
+    bool DeviceGlobal::matches_topology() const
+    {
+        if (G->get_total_n_dofs() != ndofs) return false;
+        const auto& pots = G->get_potentials();
+        if (pots.size() != infos.size()) return false;
+        for (size_t i = 0; i < pots.size(); i++)
+            if (pots[i]->mws->conn.n_elements() != infos[i].n_elem) return false;
+        return true;
+    }
+
+    // ---- NewtonsMethod (NewtonsMethod.cpp:12-641) ----------------------------------------
+    NewtonsMethod::NewtonsMethod(spGlobalPotential G_, spContext context_, spSolverCallbacks cb) : callbacks(cb), G(G_), context(context_)
+    {
+        if (!callbacks) callbacks = std::make_shared<SolverCallbacks>(context);
+        dev = std::make_unique<DeviceGlobal>(G, context);
+    }
+
+    void NewtonsMethod::sync_dofs_to_host()
+    {
+        host_buf.resize(dev->n_dofs());
+        check(symx_hip_problem_get_dofs(dev->problem(), host_buf.data()), context->hip(), "get_dofs");
+        G->set_dofs(host_buf.data());
+    }
+
+    double NewtonsMethod::compute_residual()
+    {
+        if (callbacks->has_custom_residual) {
+            std::vector<double> g(dev->n_dofs());
+            check(symx_hip_problem_download(dev->problem(), SYMX_HIP_VEC_GRAD, g.data()), context->hip(), "download gradient");
+            return callbacks->residual(g);
+        }
+        double r = 0.0;
+        check(symx_hip_problem_max_abs(dev->problem(), SYMX_HIP_VEC_GRAD, &r), context->hip(), "max_abs");
+        return r;
+    }
+
+    SolverReturn NewtonsMethod::solve()
+    {
+        const int ndofs = G->get_total_n_dofs();
+        if (ndofs <= 0) throw std::runtime_error("symx error NewtonsMethod::solve(): No degrees of freedom.");
+        if (ndofs % 3 != 0) throw std::runtime_error("symx error NewtonsMethod::solve(): ndofs must be divisible by 3.");
+        if (settings.linear_solver != LinearSolver::BDPCG)
+            throw std::runtime_error("symx: the HIP backend solves with BDPCG only (DirectLLT is not available on the device)");
+        if (settings.projection_mode == ProjectionToPD::Progressive)
+            throw std::runtime_error("symx: ProjectionToPD::Progressive is not implemented by the HIP backend yet");
+        if (!dev->matches_topology()) dev = std::make_unique<DeviceGlobal>(G, context);
+        symx_hip_ctx hip = context->hip();
+        symx_hip_problem prob = dev->problem();
+        auto& logger = context->logger;
+        auto& out = context->output;
+
+        dev->refresh_data();
+        host_buf.resize(ndofs);
+        G->get_dofs(host_buf.data());
+        check(symx_hip_problem_set_dofs(prob, host_buf.data()), hip, "set_dofs");
+
+        stats = SolveStats();
+        double E0 = 0.0;
+        double du_dot_grad = 0.0;
+        double res_0 = std::numeric_limits<double>::max();
+        SolverReturn result = SolverReturn::Running;
+        int pdn_countdown = 0;
+        double ppn_threshold = -1.0;
+        const bool precision_mixed = settings.precision == Precision::Mixed;
+        int64_t n_hessians = 0;
+        symx_hip_problem_n_element_hessians(prob, &n_hessians);
+
+        auto run_bool = [](bool def, const std::vector<std::function<bool()>>& fs) {
+            bool v = def;
+            for (auto& f : fs) v = v && f();
+            return v;
+        };
+
+        if (!run_bool(true, callbacks->is_initial_state_valid)) {
+            out->print_with_new_line("Newton failure: Invalid initial state.", Verbosity::Medium);
+            result = SolverReturn::InvalidInitialState;
+        }
+
+        int newton_iteration = -1;
+        while (result == SolverReturn::Running) {
+            newton_iteration++;
+            if (newton_iteration == settings.max_iterations) {
+                out->print_with_new_line("Newton failure: Too many iterations.", Verbosity::Medium);
+                result = settings.max_iterations_as_success ? SolverReturn::Successful : SolverReturn::TooManyIterations;
+                break;
+            }
+            if (!callbacks->before_energy_evaluation.empty()) {
+                sync_dofs_to_host();
+                for (auto& f : callbacks->before_energy_evaluation) f();
+                dev->refresh_data();
+            }
+            {
+                auto t = logger->time("evaluate_P__dP_du__local_d2P_du2");
+                check(symx_hip_problem_eval(prob, SYMX_HIP_EVAL_PGH, &E0), hip, "evaluation");
+            }
+            const double residual_norm = compute_residual();
+            stats.energies.push_back(E0);
+            stats.residuals.push_back(residual_norm);
+            if (newton_iteration == 0) res_0 = residual_norm;
+            if (residual_norm < settings.bailout_residual) {
+                result = SolverReturn::Successful;
+                break;
+            }
+            if (newton_iteration >= settings.min_iterations) {
+                if (residual_norm < settings.residual_tolerance_abs) {
+                    result = SolverReturn::Successful;
+                    break;
+                }
+                if (newton_iteration > 0 && residual_norm / res_0 < settings.residual_tolerance_rel) {
+                    result = SolverReturn::Successful;
+                    break;
+                }
+            }
+
+            const int initial_cg = stats.cg_iterations;
+            bool assembled = false;
+            bool ok = false;
+            while (!ok) {
+                // Project element Hessians and assemble (NewtonsMethod.cpp:254-352).
+                bool all_projected = false;
+                {
+                    auto t = logger->time("project_to_PD");
+                    bool project = false;
+                    if (settings.projection_mode == ProjectionToPD::ProjectedNewton) project = true;
+                    else if (settings.projection_mode == ProjectionToPD::ProjectOnDemand) project = pdn_countdown > 0;
+                    if (project) {
+                        int64_t changed = 0;
+                        check(symx_hip_problem_project(prob, settings.projection_eps, settings.project_to_pd_use_mirroring ? 1 : 0, &changed), hip,
+                              "projection");
+                        all_projected = true;
+                    }
+                }
+                {
+                    auto t = logger->time("assembly");
+                    check(symx_hip_problem_assemble(prob, precision_mixed ? SYMX_HIP_MIXED : SYMX_HIP_FP64), hip, "assembly");
+                    assembled = true;
+                }
+                ok = solve_linear_system(residual_norm);
+                const bool can_project_more = settings.projection_mode != ProjectionToPD::Newton && !all_projected;
+                if (!ok && !can_project_more) {
+                    out->print("Linear system failed. ", Verbosity::Summary);
+                    result = SolverReturn::LinearSystemSolveFailure;
+                    break;
+                }
+                bool descends = false;
+                if (ok) {
+                    check(symx_hip_problem_dot(prob, SYMX_HIP_VEC_DU, SYMX_HIP_VEC_GRAD, &du_dot_grad), hip, "dot");
+                    descends = du_dot_grad < 0.0;
+                    if (!descends && !can_project_more) {
+                        out->print("Step does not descend. ", Verbosity::Summary);
+                        result = SolverReturn::StepDoesNotDescend;
+                        break;
+                    }
+                }
+                if (ok && descends) break;
+                ok = false;
+                // _increase_projection
+                if (settings.projection_mode == ProjectionToPD::ProjectOnDemand) pdn_countdown = settings.project_on_demand_countdown;
+            }
+            (void)assembled;
+            if (result != SolverReturn::Running) {
+                out->print_with_new_line("Newton failure: Could not solve the linear system or find a descend direction.", Verbosity::Summary);
+                break;
+            }
+            stats.cg_per_iteration.push_back(stats.cg_iterations - initial_cg);
+            // _decrease_projection
+            if (settings.projection_mode == ProjectionToPD::ProjectOnDemand) pdn_countdown--;
+            const bool projected_now = settings.projection_mode == ProjectionToPD::ProjectedNewton ||
+                                       (settings.projection_mode == ProjectionToPD::ProjectOnDemand && pdn_countdown >= 0 && pdn_countdown + 1 > 0 &&
+                                        false);
+            (void)projected_now;
+            stats.n_hessians += (uint64_t)n_hessians;
+            if (settings.projection_mode == ProjectionToPD::ProjectedNewton) stats.n_projected_hessians += (uint64_t)n_hessians;
+            logger->add_and_append("cg_iterations", last_cg_iterations);
+
+            double du_max = 0.0;
+            check(symx_hip_problem_max_abs(prob, SYMX_HIP_VEC_DU, &du_max), hip, "max_abs");
+            if (newton_iteration >= settings.min_iterations && du_max < settings.step_tolerance) {
+                result = SolverReturn::Successful;
+                break;
+            }
+            result = line_search(E0, du_dot_grad, du_max);
+            if (newton_iteration >= settings.min_iterations && !callbacks->is_converged.empty()) {
+                sync_dofs_to_host();
+                if (run_bool(false, callbacks->is_converged)) {
+                    result = SolverReturn::Successful;
+                    break;
+                }
+            }
+            if (result != SolverReturn::Running) break;
+        }
+
+        sync_dofs_to_host();
+        if (result == SolverReturn::Successful && !run_bool(true, callbacks->is_converged_state_valid)) {
+            out->print_with_new_line("Newton failure: Invalid converged state.", Verbosity::Medium);
+            result = SolverReturn::InvalidConvergedState;
+        }
+        stats.newton_iterations = newton_iteration;
+        if (stats.n_hessians > 0) stats.projected_hessians_ratio = (double)stats.n_projected_hessians / (double)stats.n_hessians;
+        stats.final_energy = stats.energies.empty() ? 0.0 : stats.energies.back();
+        stats.final_residual = stats.residuals.empty() ? 0.0 : stats.residuals.back();
+        logger->add_and_append("newton_iterations", stats.newton_iterations);
+        return result;
+    }
+
+    bool NewtonsMethod::solve_linear_system(double residual_norm)
+    {
+        auto t = context->logger->time("linear_system_solve");
+        symx_hip_ctx hip = context->hip();
+        symx_hip_problem prob = dev->problem();
+        check(symx_hip_problem_scale(prob, SYMX_HIP_VEC_RHS, SYMX_HIP_VEC_GRAD, -1.0), hip, "rhs");
+        // Forcing sequence (NewtonsMethod.cpp:423-424).
+        const double forcing = std::min(1e-2, residual_norm * std::min(0.5, std::sqrt(residual_norm)));
+        const double abs_tol = std::max(forcing, settings.cg_abs_tolerance);
+        symx_hip_pcg_info info{};
+        check(symx_hip_problem_solve(prob, settings.precision == Precision::Mixed ? SYMX_HIP_MIXED : SYMX_HIP_FP64, abs_tol,
+                                     settings.cg_rel_tolerance, settings.cg_max_iterations, settings.cg_stop_on_indefiniteness ? 1 : 0, &info),
+              hip, "linear solve");
+        stats.cg_iterations += info.n_iterations;
+        stats.n_refinements += info.n_refinements;
+        last_cg_iterations = info.n_iterations;
+        return info.converged != 0;
+    }
+
+    SolverReturn NewtonsMethod::line_search(double E0, double du_dot_grad, double du_max)
+    {
+        symx_hip_ctx hip = context->hip();
+        symx_hip_problem prob = dev->problem();
+        auto& out = context->output;
+        // dofs_before_ls = dofs
+        check(symx_hip_problem_scale(prob, SYMX_HIP_VEC_DOFS_LS, SYMX_HIP_VEC_DOFS, 1.0), hip, "save dofs");
+        auto apply = [&](double s) {
+            check(symx_hip_problem_axpy(prob, SYMX_HIP_VEC_DOFS, SYMX_HIP_VEC_DOFS_LS, s, SYMX_HIP_VEC_DU), hip, "apply step");
+        };
+        double retraction = 1.0;
+        if (du_max > settings.step_cap) {
+            retraction *= settings.step_cap / du_max;
+            check(symx_hip_problem_scale(prob, SYMX_HIP_VEC_DU, SYMX_HIP_VEC_DU, retraction), hip, "cap");
+            du_max = settings.step_cap;
+            stats.ls_cap_iterations++;
+        }
+        double max_step = 1.0;
+        if (!callbacks->max_allowed_step.empty()) {
+            sync_dofs_to_host();
+            for (auto& f : callbacks->max_allowed_step) max_step = std::min(max_step, f());
+        }
+        if (max_step < 1.0) {
+            retraction *= max_step;
+            check(symx_hip_problem_scale(prob, SYMX_HIP_VEC_DU, SYMX_HIP_VEC_DU, max_step), hip, "max step");
+            du_max *= max_step;
+            stats.ls_max_iterations++;
+        }
+        const double shrink = 0.5;
+        double step = 1.0;
+        apply(step);
+        int ls_inv_it = 0;
+        for (; ls_inv_it < settings.max_backtracking_invalid_state_iterations; ++ls_inv_it) {
+            bool valid = true;
+            if (!callbacks->is_intermediate_state_valid.empty()) {
+                sync_dofs_to_host();
+                for (auto& f : callbacks->is_intermediate_state_valid) valid = valid && f();
+            }
+            if (valid) break;
+            step *= shrink;
+            apply(step);
+            stats.ls_inv_iterations++;
+        }
+        if (ls_inv_it == settings.max_backtracking_invalid_state_iterations) {
+            out->print_with_new_line("Newton failure: Too many invalid intermediate state iterations.", Verbosity::Medium);
+            for (auto& f : callbacks->on_intermediate_state_invalid) f();
+            return SolverReturn::TooManyInvalidIntermediateIterations;
+        }
+        if (!settings.enable_armijo_backtracking) {
+            stats.step_sizes.push_back(step * retraction);
+            return SolverReturn::Running;
+        }
+        const double expected_decrease = settings.line_search_armijo_beta * du_dot_grad * retraction;
+        double E_threshold = E0 + expected_decrease * step;
+        int armijo = 0;
+        for (; armijo < settings.max_backtracking_armijo_iterations; ++armijo) {
+            if (!callbacks->before_energy_evaluation.empty()) {
+                sync_dofs_to_host();
+                for (auto& f : callbacks->before_energy_evaluation) f();
+                dev->refresh_data();
+            }
+            double E1 = 0.0;
+            {
+                auto t = context->logger->time("evaluate_P");
+                check(symx_hip_problem_eval(prob, SYMX_HIP_EVAL_P, &E1), hip, "energy");
+            }
+            if (E1 < E_threshold) break;
+            step *= shrink;
+            apply(step);
+            E_threshold = E0 + expected_decrease * step;
+            stats.ls_bt_iterations++;
+        }
+        context->logger->add_and_append("ls_bt", armijo);
+        if (armijo == settings.max_backtracking_armijo_iterations) {
+            out->print_with_new_line("Newton failure: Too many armijo iterations.", Verbosity::Medium);
+            for (auto& f : callbacks->on_armijo_fail) f();
+            return SolverReturn::TooManyArmijoIterations;
+        }
+        stats.step_sizes.push_back(step * retraction);
+        return SolverReturn::Running;
+    }
+
+    void NewtonsMethod::print_summary(double total_time) const
+    {
+        const auto& L = *context->logger;
+        std::printf("\n  %-24s %10s\n", "Solve", "Total");
+        std::printf("  Newton iterations        %10d\n", stats.newton_iterations);
+        std::printf("  CG iterations            %10d\n", stats.cg_iterations);
+        std::printf("  Line search bt           %10d\n", stats.ls_bt_iterations);
+        if (total_time <= 0.0) {
+            total_time = 0.0;
+            for (auto& l : L.get_timer_labels()) total_time += L.get_timer_total(l);
+        }
+        std::printf("\n  %-40s %10s  %6s\n", "Runtime", "Time (s)", "%");
+        for (auto& l : L.get_timer_labels()) {
+            const double t = L.get_timer_total(l);
+            std::printf("  %-40s %10.6f  %5.1f%%\n", l.c_str(), t, total_time > 0 ? 100.0 * t / total_time : 0.0);
+        }
+    }
+
+    // ---- Compiled<double> -----------------------------------------------------------------
+    namespace
+    {
+        symx_hip_ctx default_ctx()
+        {
+            static symx_hip_ctx c = nullptr;
+            if (!c) check(symx_hip_ctx_create(0, &c), nullptr, "creating the HIP context");
+            return c;
+        }
+    }
+    Compiled<double>::Compiled(const std::vector<Scalar>& exprs, const std::string& name_, const std::string& folder, const std::string& cache_id)
+        : name(identifier(name_))
+    {
+        (void)cache_id;
+        if (exprs.empty()) throw std::runtime_error("symx: Compiled<double> without expressions");
+        Graph* g = exprs[0].g;
+        n_in = g->n_symbols();
+        n_out = (int32_t)exprs.size();
+        in.assign(n_in, 0.0);
+        out.assign(n_out, 0.0);
+        const std::string src = emit_hip_batch_function(name, *g, exprs, n_in);
+        ctx = default_ctx();
+        char key[80];
+        int wc = 0;
+        size_t cs = 0;
+        const std::string dir = folder.empty() ? get_codegen_dir() : folder;
+        check(symx_hip_jit_compile(src.c_str(), "gfx950", dir.c_str(), key, &wc, &cs), nullptr, "JIT");
+        cached = wc != 0;
+        check(symx_hip_module_load(ctx, src.c_str(), dir.c_str(), &mod), ctx, "module load");
+    }
+    Compiled<double>::~Compiled()
+    {
+        if (mod) symx_hip_module_unload(mod);
+    }
+    void Compiled<double>::set(const Scalar& s, double v)
+    {
+        if (!s.is_symbol()) throw std::runtime_error("symx: Compiled::set on a non-symbol");
+        in[s.get_symbol_idx()] = v;
+    }
+    void Compiled<double>::set(const Vector& v, const double* values)
+    {
+        for (int32_t i = 0; i < v.size(); i++) set(v[i], values[i]);
+    }
+    const std::vector<double>& Compiled<double>::run()
+    {
+        check(symx_hip_batch_eval(ctx, mod, name.c_str(), in.data(), 1, n_in, out.data(), n_out), ctx, "run");
+        for (double x : out) {
+            if (std::isnan(x)) throw std::runtime_error("symx: Compiled::run() produced NaN");
+        }
+        return out;
+    }
+    void Compiled<double>::run_batch(const double* inputs, int64_t n, double* outputs)
+    {
+        check(symx_hip_batch_eval(ctx, mod, name.c_str(), inputs, n, n_in, outputs, n_out), ctx, "run_batch");
+    }
+
+    // ---- FEM helpers (fem_integrators.cpp) -------------------------------------------
+    std::string get_name(const FEM_Element& e)
+    {
+        switch (e) {
+        case FEM_Element::Tet4: return "Tet4";
+        case FEM_Element::Tet10: return "Tet10";
+        case FEM_Element::Hex8: return "Hex8";
+        case FEM_Element::Hex27: return "Hex27";
+        }
+        return "?";
+    }
+    int get_stride(const FEM_Element& e)
+    {
+        switch (e) {
+        case FEM_Element::Tet4: return 4;
+        case FEM_Element::Tet10: return 10;
+        case FEM_Element::Hex8: return 8;
+        case FEM_Element::Hex27: return 27;
+        }
+        return 0;
+    }
+    std::vector<std::array<double, 4>> get_integration_rule(const FEM_Element& e)
+    {
+        // Rules as tabulated by fem_integrators.cpp:52-120 (weight, xi0, xi1, xi2). The
+        // truncated decimal constants are kept so that results match the reference.
+        if (e == FEM_Element::Tet4) return { { 1.0 / 6.0, 0.25, 0.25, 0.25 } };
+        if (e == FEM_Element::Tet10) {
+            const double a = 0.58541020, b = 0.13819660, w = 0.041666667;
+            return { { w, a, b, b }, { w, b, a, b }, { w, b, b, a }, { w, b, b, b } };
+        }
+        if (e == FEM_Element::Hex8) {
+            const double p = 1.0 / std::sqrt(3.0);
+            return { { 1.0, -p, -p, -p }, { 1.0, p, -p, -p }, { 1.0, p, p, -p }, { 1.0, -p, p, -p },
+                     { 1.0, -p, -p, p }, { 1.0, p, -p, p }, { 1.0, p, p, p }, { 1.0, -p, p, p } };
+        }
+        // Hex27: 3-point Gauss-Legendre per axis with 10-digit truncated constants.
+        const double p = 0.7745966692;
+        const double wc = 0.1714677641, we = 0.2743484225, wf = 0.4389574760, wm = 0.7023319616;
+        return { { wc, -p, -p, -p }, { wc, p, -p, -p }, { wc, -p, p, -p }, { wc, p, p, -p },
+                 { wc, -p, -p, p }, { wc, p, -p, p }, { wc, -p, p, p }, { wc, p, p, p },
+                 { we, 0.0, -p, -p }, { we, -p, 0.0, -p }, { we, p, 0.0, -p }, { we, 0.0, p, -p },
+                 { we, -p, -p, 0.0 }, { we, p, -p, 0.0 }, { we, -p, p, 0.0 }, { we, p, p, 0.0 },
+                 { we, 0.0, -p, p }, { we, -p, 0.0, p }, { we, p, 0.0, p }, { we, 0.0, p, p },
+                 { wf, 0.0, 0.0, -p }, { wf, 0.0, -p, 0.0 }, { wf, -p, 0.0, 0.0 }, { wf, p, 0.0, 0.0 },
+                 { wf, 0.0, p, 0.0 }, { wf, 0.0, 0.0, p }, { wm, 0.0, 0.0, 0.0 } };
+    }
+
+    static std::vector<Scalar> shape_functions(const FEM_Element& e, const Vector& xi)
+    {
+        const Scalar& u = xi[0];
+        const Scalar& v = xi[1];
+        const Scalar& w = xi[2];
+        if (e == FEM_Element::Tet4) return { 1.0 - u - v - w, u, v, w };
+        if (e == FEM_Element::Hex8) {
+            // corners in lexicographic order (a + 2b + 4c), a,b,c in {0,1} <-> xi in {-1,+1}
+            std::vector<Scalar> N;
+            for (int c = 0; c < 2; c++)
+                for (int b = 0; b < 2; b++)
+                    for (int a = 0; a < 2; a++) {
+                        const double sa = a ? 1.0 : -1.0, sb = b ? 1.0 : -1.0, sc = c ? 1.0 : -1.0;
+                        N.push_back(0.125 * (1.0 + sa * u) * (1.0 + sb * v) * (1.0 + sc * w));
+                    }
+            return N;
+        }
+        if (e == FEM_Element::Hex27) {
+            // quadratic Lagrange on nodes {-1, 0, 1}; node index a + 3b + 9c
+            auto L = [](const Scalar& x, int i) -> Scalar {
+                if (i == 0) return 0.5 * x * (x - 1.0);
+                if (i == 1) return 1.0 - x * x;
+                return 0.5 * x * (x + 1.0);
+            };
+            std::vector<Scalar> N;
+            for (int c = 0; c < 3; c++)
+                for (int b = 0; b < 3; b++)
+                    for (int a = 0; a < 3; a++) N.push_back(L(u, a) * L(v, b) * L(w, c));
+            return N;
+        }
+        throw std::runtime_error("symx: shape functions for " + get_name(e) + " are not implemented");
+    }
+
+    Vector fem_interpolation(const FEM_Element& e, const std::vector<Vector>& xh, const Vector& xi)
+    {
+        const std::vector<Scalar> N = shape_functions(e, xi);
+        if (N.size() != xh.size()) throw std::runtime_error("symx: fem_interpolation node count mismatch");
+        Vector x = xh[0] * N[0];
+        for (size_t i = 1; i < N.size(); i++) x = x + xh[i] * N[i];
+        return x;
+    }
+    Matrix fem_jacobian(const FEM_Element& e, const std::vector<Vector>& xh, const Vector& xi)
+    {
+        const Vector x = fem_interpolation(e, xh, xi);
+        std::vector<Scalar> vals;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) vals.push_back(diff(x[i], xi[j]));
+        return Matrix(vals, { 3, 3 });
+    }
+    Scalar fem_integrator(MappedWorkspace<double>& mws, const FEM_Element& e, std::function<Scalar(Scalar& w, Vector& xi)> summand)
+    {
+        const std::vector<std::array<double, 4>> rule = get_integration_rule(e);
+        return mws.add_for_each(rule, [&](Vector& ip) {
+            Scalar w = ip[0];
+            Vector xi({ ip[1], ip[2], ip[3] });
+            return summand(w, xi);
+        });
+    }
+
+    // ---- meshes ------------------------------------------------------------------------
+    FEMMesh generate_cuboid_Tet4_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& n, const std::array<double, 3>& origin)
+    {
+        FEMMesh m;
+        m.element_type = FEM_Element::Tet4;
+        m.connectivity_stride = 4;
+        const double d[3] = { size[0] / n[0], size[1] / n[1], size[2] / n[2] };
+        for (int i = 0; i <= n[0]; i++)
+            for (int j = 0; j <= n[1]; j++)
+                for (int k = 0; k <= n[2]; k++) m.vertices.push_back({ origin[0] + i * d[0], origin[1] + j * d[1], origin[2] + k * d[2] });
+        auto idx = [&](int i, int j, int k) { return (int32_t)(i * (n[1] + 1) * (n[2] + 1) + j * (n[2] + 1) + k); };
+        m.connectivity.reserve((size_t)n[0] * n[1] * n[2] * 24);
+        for (int i = 0; i < n[0]; i++)
+            for (int j = 0; j < n[1]; j++)
+                for (int k = 0; k < n[2]; k++) {
+                    const int32_t v0 = idx(i, j, k), v1 = idx(i + 1, j, k), v2 = idx(i + 1, j + 1, k), v3 = idx(i, j + 1, k);
+                    const int32_t v4 = idx(i, j, k + 1), v5 = idx(i + 1, j, k + 1), v6 = idx(i + 1, j + 1, k + 1), v7 = idx(i, j + 1, k + 1);
+                    const int32_t t[6][4] = { { v0, v1, v6, v5 }, { v0, v1, v2, v6 }, { v0, v2, v3, v6 },
+                                              { v0, v5, v6, v4 }, { v0, v6, v3, v7 }, { v0, v6, v7, v4 } };
+                    for (auto& tt : t) m.connectivity.insert(m.connectivity.end(), tt, tt + 4);
+                }
+        m.n_elements = (int32_t)(m.connectivity.size() / 4);
+        return m;
+    }
+
+    FEMMesh generate_cuboid_Hex27_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& n, const std::array<double, 3>& origin)
+    {
+        FEMMesh m;
+        m.element_type = FEM_Element::Hex27;
+        m.connectivity_stride = 27;
+        const int N[3] = { 2 * n[0] + 1, 2 * n[1] + 1, 2 * n[2] + 1 };
+        const double d[3] = { size[0] / (2.0 * n[0]), size[1] / (2.0 * n[1]), size[2] / (2.0 * n[2]) };
+        for (int i = 0; i < N[0]; i++)
+            for (int j = 0; j < N[1]; j++)
+                for (int k = 0; k < N[2]; k++) m.vertices.push_back({ origin[0] + i * d[0], origin[1] + j * d[1], origin[2] + k * d[2] });
+        auto idx = [&](int i, int j, int k) { return (int32_t)(i * N[1] * N[2] + j * N[2] + k); };
+        for (int i = 0; i < n[0]; i++)
+            for (int j = 0; j < n[1]; j++)
+                for (int k = 0; k < n[2]; k++)
+                    for (int c = 0; c < 3; c++)
+                        for (int b = 0; b < 3; b++)
+                            for (int a = 0; a < 3; a++) m.connectivity.push_back(idx(2 * i + a, 2 * j + b, 2 * k + c));
+        m.n_elements = (int32_t)(m.connectivity.size() / 27);
+        return m;
+    }
+}

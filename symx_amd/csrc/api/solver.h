@@ -1,0 +1,399 @@
+// Solver front end mirroring SymX's symx/src/solver/{GlobalPotential,NewtonsMethod,
+// solver_utils,Context,fem_integrators,mesh_generation}.h. The control flow of
+// NewtonsMethod::solve is the reference's (NewtonsMethod.cpp:28-641); every
+// vector and matrix operation runs on the GPU through include/symx_hip.h.
+#pragma once
+#include <array>
+#include <chrono>
+#include <cstdint>
+#include <functional>
+#include <limits>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/symx_hip.h"
+#include "compile.h"
+#include "symbol.h"
+
+namespace symx
+{
+    // ---- logging (Logger.h subset: timers and per-solve series) ----------------------
+    class Logger
+    {
+    public:
+        struct Stats { double total = 0, avg = 0, min = 0, max = 0; int64_t count = 0; };
+        class ScopedTimer
+        {
+        public:
+            ScopedTimer(Logger* l, std::string label) : l(l), label(std::move(label)), t0(std::chrono::steady_clock::now()) {}
+            ~ScopedTimer() { l->timers[label] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+        private:
+            Logger* l;
+            std::string label;
+            std::chrono::steady_clock::time_point t0;
+        };
+        ScopedTimer time(const std::string& label) { return ScopedTimer(this, label); }
+        void add_and_append(const std::string& key, double v) { series[key].push_back(v); }
+        Stats get_stats(const std::string& key) const;
+        std::vector<std::string> get_timer_labels() const;
+        double get_timer_total(const std::string& label) const;
+        void clear() { timers.clear(); series.clear(); }
+        std::map<std::string, double> timers;
+        std::map<std::string, std::vector<double>> series;
+    };
+    using spLogger = std::shared_ptr<Logger>;
+
+    enum class Verbosity { None = 0, Summary = 1, Medium = 2, Full = 3 };
+    class OutputSink
+    {
+    public:
+        void print(const std::string& s, Verbosity v = Verbosity::Medium);
+        void print_with_new_line(const std::string& s, Verbosity v = Verbosity::Medium);
+        void print_new_line(Verbosity v = Verbosity::Medium) { print("\n", v); }
+        void set_console_verbosity(Verbosity v) { verbosity = v; }
+        Verbosity get_console_verbosity() const { return verbosity; }
+    private:
+        Verbosity verbosity = Verbosity::None;
+    };
+    using spOutputSink = std::shared_ptr<OutputSink>;
+
+    // ---- context (Context.h) -----------------------------------------------------------
+    struct Context
+    {
+        int n_threads = 1;
+        std::string compilation_directory;  // code-object cache
+        int device = 0;
+        spOutputSink output = std::make_shared<OutputSink>();
+        spLogger logger = std::make_shared<Logger>();
+        static std::shared_ptr<Context> create() { return std::make_shared<Context>(); }
+        // Lazily created HIP context shared by everything built from this Context.
+        symx_hip_ctx hip();
+        ~Context();
+    private:
+        symx_hip_ctx hip_ctx = nullptr;
+    };
+    using spContext = std::shared_ptr<Context>;
+    std::string get_codegen_dir();
+
+    // ---- settings, callbacks, return codes (solver_utils.h:15-258) --------------------
+    enum class SolverReturn
+    {
+        Successful,
+        Running,
+        InvalidInitialState,
+        TooManyIterations,
+        TooManyArmijoIterations,
+        LinearSystemSolveFailure,
+        TooManyInvalidIntermediateIterations,
+        StepDoesNotDescend,
+        InvalidConvergedState,
+    };
+    const char* to_string(SolverReturn r);
+
+    enum class LinearSolver { DirectLLT, BDPCG };
+    enum class ProjectionToPD { Newton, ProjectedNewton, ProjectOnDemand, Progressive };
+    // Additive backend knob: FP64 everywhere, or FP32 element Hessians / block matrix /
+    // PCG inside FP64 iterative refinement.
+    enum class Precision { FP64, Mixed };
+
+    struct SolverSettings
+    {
+        int max_iterations = std::numeric_limits<int>::max();
+        int min_iterations = 0;
+        double residual_tolerance_abs = 1e-6;
+        double residual_tolerance_rel = 0.0;
+        double step_tolerance = 0.0;
+        bool max_iterations_as_success = false;
+        double step_cap = std::numeric_limits<double>::infinity();
+        bool enable_armijo_backtracking = true;
+        double line_search_armijo_beta = 1e-4;
+        int max_backtracking_armijo_iterations = 20;
+        int max_backtracking_invalid_state_iterations = 8;
+        bool print_line_search_upon_failure = false;
+    };
+
+    struct NewtonSettings : public SolverSettings
+    {
+        ProjectionToPD projection_mode = ProjectionToPD::ProjectedNewton;
+        double projection_eps = 1e-10;
+        bool project_to_pd_use_mirroring = false;
+        int project_on_demand_countdown = 4;
+        double ppn_tightening_factor = 0.5;
+        double ppn_release_factor = 2.0;
+        LinearSolver linear_solver = LinearSolver::BDPCG;
+        int cg_max_iterations = 10000;
+        double cg_abs_tolerance = 1e-12;
+        double cg_rel_tolerance = 1e-4;
+        bool cg_stop_on_indefiniteness = true;
+        double bailout_residual = 1e-10;
+        Precision precision = Precision::FP64;
+    };
+
+    // Residual callbacks receive the gradient downloaded to the host.
+    class SolverCallbacks
+    {
+    public:
+        explicit SolverCallbacks(spContext c) : context(c) {}
+        static std::shared_ptr<SolverCallbacks> create(spContext c) { return std::make_shared<SolverCallbacks>(c); }
+        void set_residual(std::function<double(std::vector<double>&)> f) { residual = f; has_custom_residual = true; }
+        void add_before_energy_evaluation(std::function<void()> f) { before_energy_evaluation.push_back(f); }
+        void add_is_initial_state_valid(std::function<bool()> f) { is_initial_state_valid.push_back(f); }
+        void add_is_intermediate_state_valid(std::function<bool()> f) { is_intermediate_state_valid.push_back(f); }
+        void add_on_intermediate_state_invalid(std::function<void()> f) { on_intermediate_state_invalid.push_back(f); }
+        void add_on_armijo_fail(std::function<void()> f) { on_armijo_fail.push_back(f); }
+        void add_is_converged(std::function<bool()> f) { is_converged.push_back(f); }
+        void add_is_converged_state_valid(std::function<bool()> f) { is_converged_state_valid.push_back(f); }
+        void add_max_allowed_step(std::function<double()> f) { max_allowed_step.push_back(f); }
+
+        std::vector<std::function<void()>> before_energy_evaluation;
+        std::vector<std::function<bool()>> is_initial_state_valid;
+        std::vector<std::function<bool()>> is_intermediate_state_valid;
+        std::vector<std::function<void()>> on_intermediate_state_invalid;
+        std::vector<std::function<void()>> on_armijo_fail;
+        std::vector<std::function<bool()>> is_converged;
+        std::vector<std::function<bool()>> is_converged_state_valid;
+        std::vector<std::function<double()>> max_allowed_step;
+        std::function<double(std::vector<double>&)> residual;
+        bool has_custom_residual = false;
+        spContext context;
+    };
+    using spSolverCallbacks = std::shared_ptr<SolverCallbacks>;
+
+    // ---- potentials (Potential.h, GlobalPotential.h) ------------------------------------
+    class Potential
+    {
+    public:
+        std::string name;
+        spMWS<double> mws;
+        Scalar expression;
+        Scalar condition;
+        bool has_condition = false;
+    };
+
+    class GlobalPotential
+    {
+    public:
+        using DefFunc = std::function<Scalar(MappedWorkspace<double>&, Element&)>;
+        using DefWCondFunc = std::function<std::pair<Scalar, Scalar>(MappedWorkspace<double>&, Element&)>;
+
+        static std::shared_ptr<GlobalPotential> create() { return std::make_shared<GlobalPotential>(); }
+
+        void add_potential(const std::string& name, spMWS<double> mws, Element& elem, DefFunc f);
+        void add_potential(const std::string& name, spMWS<double> mws, Element& elem, DefWCondFunc f);
+        void add_potential(const std::string& name, std::function<const int32_t*()> data, std::function<int32_t()> n_elements,
+                           int32_t stride, DefFunc f);
+        void add_potential(const std::string& name, const std::vector<int32_t>& arr, int32_t n_items_per_element, DefFunc f);
+        void add_potential(const std::string& name, const std::vector<int32_t>& arr, int32_t n_items_per_element, DefWCondFunc f);
+        template <std::size_t N>
+        void add_potential(const std::string& name, const std::vector<std::array<int32_t, N>>& arr, DefFunc f)
+        {
+            auto [mws, elem] = MappedWorkspace<double>::create(arr);
+            add_potential(name, mws, elem, f);
+        }
+        template <std::size_t N>
+        void add_potential(const std::string& name, const std::vector<std::array<int32_t, N>>& arr, DefWCondFunc f)
+        {
+            auto [mws, elem] = MappedWorkspace<double>::create(arr);
+            add_potential(name, mws, elem, f);
+        }
+        template <std::size_t N>
+        void add_potential(const std::string& name, const LabelledConnectivity<N>& conn, DefFunc f)
+        {
+            auto [mws, elem] = MappedWorkspace<double>::create(conn);
+            add_potential(name, mws, elem, f);
+        }
+        template <std::size_t N>
+        void add_potential(const std::string& name, const LabelledConnectivity<N>& conn, DefWCondFunc f)
+        {
+            auto [mws, elem] = MappedWorkspace<double>::create(conn);
+            add_potential(name, mws, elem, f);
+        }
+
+        void add_dof(std::function<std::uintptr_t()> id, std::function<double*()> data, std::function<int32_t()> flat_size,
+                     const std::string& name = "");
+        template <typename STATIC_VECTOR>
+        void add_dof(std::vector<STATIC_VECTOR>& arr, const std::string& name = "")
+        {
+            constexpr int32_t stride = (int32_t)(sizeof(STATIC_VECTOR) / sizeof(double));
+            add_dof([&arr]() { return reinterpret_cast<std::uintptr_t>(&arr); }, [&arr]() { return reinterpret_cast<double*>(arr.data()); },
+                    [&arr]() { return (int32_t)(arr.size() * stride); }, name);
+        }
+
+        const std::vector<int32_t>& get_dofs_offsets();
+        int32_t get_n_dof_sets() const { return (int32_t)dof_maps.size(); }
+        int32_t get_n_dofs(int32_t set_i) const { return dof_maps[set_i].flat_size(); }
+        int32_t get_total_n_dofs() const;
+        void get_dofs(double* u) const;
+        void set_dofs(const double* u);
+        void apply_dof_increment(const double* du);
+        const std::vector<std::unique_ptr<Potential>>& get_potentials() const { return potentials; }
+        int32_t get_n_potentials() const { return (int32_t)potentials.size(); }
+
+        struct DofSet
+        {
+            std::function<std::uintptr_t()> id;
+            std::function<double*()> data;
+            std::function<int32_t()> flat_size;
+            std::string name;
+        };
+        const std::vector<DofSet>& get_dof_sets() const { return dof_maps; }
+
+    private:
+        std::vector<std::unique_ptr<Potential>> potentials;
+        std::vector<DofSet> dof_maps;
+        std::vector<int32_t> dof_offsets;
+    };
+    using spGlobalPotential = std::shared_ptr<GlobalPotential>;
+
+    // ---- device-compiled global potential (SecondOrderCompiledGlobal) ------------------
+    // Owns the generated modules and the device problem.
+    class DeviceGlobal
+    {
+    public:
+        DeviceGlobal(spGlobalPotential G, spContext ctx);
+        ~DeviceGlobal();
+        symx_hip_problem problem() { return prob; }
+        // Re-upload non-DoF data maps from the user arrays (they may have changed).
+        void refresh_data();
+        // Whether the potentials' element counts / DoF counts still match the device problem.
+        bool matches_topology() const;
+        int32_t n_dofs() const { return ndofs; }
+        struct PotentialInfo
+        {
+            std::string name;
+            int32_t n_elem = 0;
+            int32_t n_dofs = 0;
+            int64_t ops_P = 0, ops_PG = 0, ops_PGH = 0;
+            std::string source;
+        };
+        const std::vector<PotentialInfo>& info() const { return infos; }
+        double compile_seconds = 0.0;
+        int n_cached = 0;
+
+    private:
+        spGlobalPotential G;
+        spContext ctx;
+        symx_hip_problem prob = nullptr;
+        std::vector<symx_hip_module> modules;
+        std::vector<PotentialInfo> infos;
+        int32_t ndofs = 0;
+        struct HostMap { int32_t pot, map; std::function<const double*()> data; int64_t n; };
+        std::vector<HostMap> host_maps;
+    };
+
+    class NewtonsMethod
+    {
+    public:
+        struct SolveStats
+        {
+            int newton_iterations = 0;
+            int cg_iterations = 0;
+            int ls_cap_iterations = 0;
+            int ls_max_iterations = 0;
+            int ls_inv_iterations = 0;
+            int ls_bt_iterations = 0;
+            uint64_t n_hessians = 0;
+            uint64_t n_projected_hessians = 0;
+            double projected_hessians_ratio = 0.0;
+            int n_refinements = 0;
+            double final_residual = 0.0;
+            double final_energy = 0.0;
+            std::vector<double> energies;   // E at each Newton iteration
+            std::vector<double> residuals;  // residual at each Newton iteration
+            std::vector<int> cg_per_iteration;
+            std::vector<double> step_sizes;
+        };
+
+        NewtonsMethod(spGlobalPotential G, spContext context, spSolverCallbacks callbacks = nullptr);
+        static std::shared_ptr<NewtonsMethod> create(spGlobalPotential G, spContext context, spSolverCallbacks callbacks = nullptr)
+        {
+            return std::make_shared<NewtonsMethod>(G, context, callbacks);
+        }
+        SolverReturn solve();
+        const SolveStats& get_last_solve_stats() const { return stats; }
+        void print_summary(double total_time = -1.0) const;
+        DeviceGlobal& device() { return *dev; }
+
+        spSolverCallbacks callbacks;
+        NewtonSettings settings;
+
+    private:
+        bool project_and_assemble(bool& assembled, int& pdn_countdown, double& ppn_threshold);
+        bool solve_linear_system(double residual_norm);
+        SolverReturn line_search(double E0, double du_dot_grad, double du_max);
+        void sync_dofs_to_host();
+        double compute_residual();
+        double energy();
+
+        spGlobalPotential G;
+        spContext context;
+        std::unique_ptr<DeviceGlobal> dev;
+        SolveStats stats;
+        std::vector<double> host_buf;
+        int last_cg_iterations = 0;
+    };
+    using spNewtonsMethod = std::shared_ptr<NewtonsMethod>;
+
+    // ---- single-expression runtime (Compiled.h) ----------------------------------------
+    // Evaluates a list of expressions on the GPU for one or many input vectors.
+    template <typename T>
+    class Compiled;
+
+    template <>
+    class Compiled<double>
+    {
+    public:
+        Compiled(const std::vector<Scalar>& expressions, const std::string& name, const std::string& folder = "", const std::string& cache_id = "");
+        ~Compiled();
+        void set(const Scalar& symbol, double value);
+        void set(const Vector& symbol, const double* values);
+        template <typename V>
+        void set(const Vector& symbol, const V& values) { set(symbol, values.data()); }
+        // Evaluate at the values set (one launch); returns the outputs.
+        const std::vector<double>& run();
+        // Evaluate n input vectors ([n][n_inputs] -> [n][n_outputs]).
+        void run_batch(const double* inputs, int64_t n, double* outputs);
+        int32_t get_n_inputs() const { return n_in; }
+        int32_t get_n_outputs() const { return n_out; }
+        bool was_cached() const { return cached; }
+
+    private:
+        std::shared_ptr<Graph> g_keep;
+        std::string name;
+        int32_t n_in = 0, n_out = 0;
+        std::vector<double> in, out;
+        symx_hip_ctx ctx = nullptr;
+        symx_hip_module mod = nullptr;
+        bool cached = false;
+        double* d_in = nullptr;
+        double* d_out = nullptr;
+        int64_t cap = 0;
+    };
+
+    // ---- FEM helpers (fem_integrators.h) and meshes (mesh_generation.h) ----------------
+    enum class FEM_Element { Tet4, Tet10, Hex8, Hex27 };
+    std::string get_name(const FEM_Element& e);
+    int get_stride(const FEM_Element& e);
+    std::vector<std::array<double, 4>> get_integration_rule(const FEM_Element& e);
+    Vector fem_interpolation(const FEM_Element& e, const std::vector<Vector>& xh, const Vector& xi);
+    Matrix fem_jacobian(const FEM_Element& e, const std::vector<Vector>& xh, const Vector& xi);
+    Scalar fem_integrator(MappedWorkspace<double>& mws, const FEM_Element& e, std::function<Scalar(Scalar& w, Vector& xi)> summand);
+
+    struct FEMMesh
+    {
+        FEM_Element element_type = FEM_Element::Tet4;
+        std::vector<std::array<double, 3>> vertices;
+        std::vector<int32_t> connectivity;
+        int32_t connectivity_stride = 4;
+        int32_t n_elements = 0;
+    };
+    // Cuboid of `size` with its minimum corner at `origin`, split into Tet4 with the
+    // 6-tetrahedra pattern of mesh_generation.cpp:78-83; vertices ordered i-major.
+    FEMMesh generate_cuboid_Tet4_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
+                                      const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
+    // Cuboid of Hex27 elements (quadratic Lagrange, nodes on a (2n+1)^3 lattice).
+    FEMMesh generate_cuboid_Hex27_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
+                                       const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
+}

@@ -1,0 +1,175 @@
+// Internal state of the C ABI (include/symx_hip.h). Host-side only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/symx_hip.h"
+
+namespace symx_hip
+{
+    // Device scalars of one PCG run; lives in device memory, mirrored to pinned host memory.
+    struct PcgState
+    {
+        double bb;      // ||b||^2
+        double rz;      // r.z of the current iterate
+        double pAp;
+        double alpha;
+        double beta;
+        double rr;      // ||r||^2
+        double err;     // ||r|| / ||b||
+        double err0;
+        double abs_tol;
+        double rel_tol;
+        int32_t it;
+        int32_t max_iter;
+        int32_t done;
+        int32_t converged;
+        int32_t indef;
+        int32_t stop_on_indef;
+    };
+
+    // Sliced block-CSR (SELL-64 of 3x3 blocks): rows are grouped in slices of 64
+    // block rows; slice s stores width_s block columns for each of its rows, the
+    // diagonal block first. Slot (s, j, lane) = slice_ptr[s] + 64 j + lane; value k
+    // of that slot lives at ((slice_ptr[s] / 64 + j) * 9 + k) * 64 + lane.
+    struct SellPattern
+    {
+        int32_t n_rows = 0;
+        int32_t n_slices = 0;
+        int64_t n_slots = 0;
+        int64_t nnzb = 0;                 // real (non padding) blocks
+        std::vector<int64_t> h_slice_ptr; // [n_slices + 1], multiples of 64
+        std::vector<int32_t> h_col;       // [n_slots]
+        std::vector<int32_t> h_row_len;   // [n_rows]
+        int64_t* d_slice_ptr = nullptr;
+        int32_t* d_col = nullptr;
+    };
+
+    void set_thread_error(const std::string& msg);
+}
+
+struct symx_hip_ctx_s
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // Reduction scratch: per-workgroup partials and the last-block ticket.
+    double* d_partials = nullptr;
+    size_t partials_cap = 0;
+    unsigned int* d_ticket = nullptr;
+    double* d_scalars = nullptr;   // small device scalar area
+    double* h_pinned = nullptr;    // pinned host scalars (64 doubles)
+    symx_hip::PcgState* d_pcg = nullptr;
+    symx_hip::PcgState* h_pcg = nullptr;
+};
+
+struct symx_hip_module_s
+{
+    symx_hip_ctx ctx = nullptr;
+    hipModule_t mod = nullptr;
+    std::unordered_map<std::string, hipFunction_t> fns;
+};
+
+namespace symx_hip
+{
+    struct DevMap
+    {
+        int32_t is_dof = 0;
+        int32_t dof_offset = 0;
+        double* d_data = nullptr;  // owned copy of non-DoF data
+        int64_t n_values = 0;
+    };
+
+    struct DevPotential
+    {
+        symx_hip_module module = nullptr;
+        std::string kP, kPG, kPGH, kC;
+        int32_t n_elem = 0;
+        int32_t stride = 0;
+        std::vector<int32_t> h_conn;        // row-major copy (topology)
+        int32_t* d_conn = nullptr;          // [stride][n_elem]
+        int32_t nb = 0;                     // DoF blocks per element
+        int32_t n_ub = 0;                   // upper 3x3 blocks per element
+        std::vector<int32_t> dof_col, dof_off;
+        std::vector<DevMap> maps;
+        uint8_t* d_active = nullptr;
+        int64_t g_base = 0;                 // first gradient block (3 doubles)
+        int64_t h_base = 0;                 // first Hessian block (9 doubles)
+        int64_t part_base = 0;              // first energy partial
+        int32_t n_groups = 0;               // workgroups of the element kernel
+    };
+}
+
+struct symx_hip_problem_s
+{
+    symx_hip_ctx ctx = nullptr;
+    int32_t n_dofs = 0;
+    int32_t nb = 0;
+    double* d_vec[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
+    std::vector<symx_hip::DevPotential> pots;
+    bool finalized = false;
+
+    // element outputs
+    int64_t total_g = 0, total_h = 0, total_parts = 0;
+    double* d_G = nullptr;
+    double* d_H = nullptr;
+    double* d_Epart = nullptr;
+    bool h_valid = false;
+
+    // gradient scatter (CSR by DoF block of gradient-block ids)
+    uint32_t* d_goff = nullptr;
+    uint32_t* d_glist = nullptr;
+
+    // block matrix
+    symx_hip::SellPattern sell;
+    uint32_t* d_coff = nullptr;   // [n_slots + 1]
+    uint32_t* d_clist = nullptr;  // Hessian block id * 2 + transpose flag
+    double* d_val64 = nullptr;
+    float* d_val32 = nullptr;
+    double* d_dinv64 = nullptr;
+    float* d_dinv32 = nullptr;
+    int32_t assembled_precision = -1;
+
+    // PCG work vectors
+    double *d_x = nullptr, *d_r = nullptr, *d_z = nullptr, *d_p0 = nullptr, *d_p1 = nullptr, *d_q = nullptr;
+    float *f_x = nullptr, *f_r = nullptr, *f_z = nullptr, *f_p0 = nullptr, *f_p1 = nullptr, *f_q = nullptr, *f_b = nullptr;
+};
+
+// Launch wrappers implemented in kernels.hip ------------------------------------
+namespace symx_hip
+{
+    // Fixed-order sum of n doubles into *d_out (single workgroup).
+    void launch_sum(hipStream_t s, const double* d_in, int64_t n, double* d_out);
+    void launch_grad_assemble(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* G, double* grad, int32_t nb);
+    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, double skip_tol,
+                          unsigned long long* d_count);
+    int launch_project_dense(hipStream_t s, double* mats, int64_t n_mats, int32_t n, double eps, int32_t mirror, uint8_t* changed);
+    void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* coff, const uint32_t* clist, const double* H,
+                             double* v64, float* v32);
+    void launch_dinv(hipStream_t s, const SellPattern& P, const double* v64, const float* v32, double* dinv64, float* dinv32);
+    // PCG on the sell matrix. T = double: FP64; T = float: FP32 (FP32 matrix).
+    void launch_pcg_init64(hipStream_t s, const SellPattern& P, const double* dinv, const double* b, double* x, double* r, double* z,
+                           double* p0, symx_hip::PcgState* st, double* partials, unsigned int* ticket);
+    void launch_pcg_iter64(hipStream_t s, const SellPattern& P, const double* val, const double* dinv, double* x, double* r, double* z,
+                           const double* p_old, double* p_new, double* q, symx_hip::PcgState* st, double* partials, unsigned int* ticket);
+    void launch_pcg_init32(hipStream_t s, const SellPattern& P, const float* dinv, const float* b, float* x, float* r, float* z,
+                           float* p0, symx_hip::PcgState* st, double* partials, unsigned int* ticket);
+    void launch_pcg_iter32(hipStream_t s, const SellPattern& P, const float* val, const float* dinv, float* x, float* r, float* z,
+                           const float* p_old, float* p_new, float* q, symx_hip::PcgState* st, double* partials, unsigned int* ticket);
+    // Mixed-precision refinement helpers.
+    // r = b - A x (FP64 vectors, FP32 matrix, FP64 arithmetic); partial ||r||^2 -> out (device scalar, fixed order).
+    void launch_residual_mixed(hipStream_t s, const SellPattern& P, const float* val, const double* b, const double* x, double* r,
+                               float* r32, double* rr_out, double* partials, unsigned int* ticket);
+    void launch_x_update_mixed(hipStream_t s, int64_t n, double* x, const float* d);
+    // Vector ops (FP64).
+    void launch_axpy(hipStream_t s, int64_t n, double* dst, const double* a, double sc, const double* b);
+    void launch_scale(hipStream_t s, int64_t n, double* dst, const double* a, double sc);
+    void launch_dot(hipStream_t s, int64_t n, const double* a, const double* b, double* out, double* partials, unsigned int* ticket);
+    void launch_max_abs(hipStream_t s, int64_t n, const double* a, double* out, double* partials, unsigned int* ticket);
+    void launch_fill_u8(hipStream_t s, int64_t n, uint8_t* p, uint8_t v);
+    int reduction_groups(int64_t n);
+}

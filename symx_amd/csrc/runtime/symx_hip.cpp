@@ -1,0 +1,1065 @@
+// Host side of the C ABI declared in include/symx_hip.h.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <vector>
+
+#include "internal.h"
+#include "sha256.h"
+
+using namespace symx_hip;
+
+namespace
+{
+    thread_local std::string g_thread_error;
+
+    int fail(symx_hip_ctx ctx, int code, const std::string& msg)
+    {
+        if (ctx) ctx->err = msg;
+        g_thread_error = msg;
+        return code;
+    }
+
+#define HIP_TRY(ctx, expr)                                                                                              \
+    do {                                                                                                                \
+        hipError_t e_ = (expr);                                                                                         \
+        if (e_ != hipSuccess) return fail((ctx), SYMX_HIP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+    template <typename T>
+    int dalloc(symx_hip_ctx ctx, T** p, size_t count)
+    {
+        *p = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return fail(ctx, SYMX_HIP_ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes: " + hipGetErrorString(e));
+        }
+        return SYMX_HIP_OK;
+    }
+    template <typename T>
+    void dfree(T*& p)
+    {
+        if (p) (void)hipFree((void*)p);
+        p = nullptr;
+    }
+
+    int ensure_partials(symx_hip_ctx ctx, size_t n)
+    {
+        if (n <= ctx->partials_cap) return SYMX_HIP_OK;
+        dfree(ctx->d_partials);
+        const size_t cap = std::max<size_t>(n, 4096);
+        int rc = dalloc(ctx, &ctx->d_partials, cap);
+        if (rc) return rc;
+        ctx->partials_cap = cap;
+        return SYMX_HIP_OK;
+    }
+
+    const char* kPrelude = R"PRELUDE(// ---- symx_hip element-kernel prelude ----
+typedef int symx_i32;
+typedef unsigned char symx_u8;
+#define SYMX_HIP_MAX_MAPS 48
+struct symx_hip_eval_params {
+    const symx_i32* conn;
+    const symx_u8* active;
+    const double* maps[SYMX_HIP_MAX_MAPS];
+    double* E_partial;
+    double* G;
+    double* H;
+    symx_u8* C_out;
+    symx_i32 n_elem;
+    symx_i32 pad_;
+};
+static_assert(sizeof(symx_hip_eval_params) == 440, "symx_hip_eval_params layout");
+__device__ __forceinline__ double symx_ln(double x) { return (x <= 0.0) ? -__builtin_inf() : log(x); }
+__device__ __forceinline__ double symx_log10(double x) { return (x <= 0.0) ? -__builtin_inf() : log10(x); }
+__device__ __forceinline__ double symx_powi(double x, int n)
+{
+    const bool neg = n < 0;
+    unsigned int m = neg ? (unsigned int)(-n) : (unsigned int)n;
+    double r = 1.0, b = x;
+    while (m) { if (m & 1u) r *= b; b *= b; m >>= 1; }
+    return neg ? 1.0 / r : r;
+}
+// Fixed-order workgroup sum of the per-element energies (256 threads).
+__device__ __forceinline__ void symx_store_energy(double v, double* E_partial)
+{
+    __shared__ double lds[4];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) E_partial[blockIdx.x] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+// ---- end of prelude ----
+)PRELUDE";
+
+    std::string hiprtc_log(hiprtcProgram p)
+    {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(p, &n);
+        std::string log(n + 1, '\0');
+        if (n) hiprtcGetProgramLog(p, &log[0]);
+        return log;
+    }
+
+    bool read_file(const std::string& path, std::vector<char>& out)
+    {
+        std::ifstream f(path, std::ios::binary);
+        if (!f) return false;
+        out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+        return !out.empty();
+    }
+
+    void mkdirs(const std::string& dir)
+    {
+        std::string cur;
+        for (size_t i = 0; i < dir.size(); i++) {
+            cur += dir[i];
+            if (dir[i] == '/' || i + 1 == dir.size()) mkdir(cur.c_str(), 0755);
+        }
+    }
+
+    const std::vector<std::string>& jit_options(const std::string& arch)
+    {
+        static std::vector<std::string> opts;
+        opts = { "--offload-arch=" + arch, "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops" };
+        return opts;
+    }
+
+    int jit_compile_impl(symx_hip_ctx ctx, const char* source, const char* arch_c, const char* cache_dir, std::vector<char>& code,
+                         std::string& key, int& was_cached)
+    {
+        const std::string arch = arch_c ? arch_c : "gfx950";
+        const auto& opts = jit_options(arch);
+        Sha256 sha;
+        sha.update(source, std::strlen(source));
+        for (const auto& o : opts) sha.update(o);
+        key = sha.hex();
+        was_cached = 0;
+        std::string path;
+        if (cache_dir && cache_dir[0]) {
+            path = std::string(cache_dir) + "/" + key + ".hsaco";
+            if (read_file(path, code)) {
+                was_cached = 1;
+                return SYMX_HIP_OK;
+            }
+        }
+        hiprtcProgram prog;
+        if (hiprtcCreateProgram(&prog, source, "symx_generated.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+            return fail(ctx, SYMX_HIP_ERR_JIT, "hiprtcCreateProgram failed");
+        std::vector<const char*> copts;
+        for (const auto& o : opts) copts.push_back(o.c_str());
+        hiprtcResult r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+        if (r != HIPRTC_SUCCESS) {
+            std::string log = hiprtc_log(prog);
+            hiprtcDestroyProgram(&prog);
+            return fail(ctx, SYMX_HIP_ERR_JIT, std::string("hiprtc compilation failed: ") + hiprtcGetErrorString(r) + "\n" + log);
+        }
+        size_t n = 0;
+        hiprtcGetCodeSize(prog, &n);
+        code.resize(n);
+        hiprtcGetCode(prog, code.data());
+        hiprtcDestroyProgram(&prog);
+        if (!path.empty()) {
+            mkdirs(cache_dir);
+            const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+            std::ofstream f(tmp, std::ios::binary);
+            f.write(code.data(), (std::streamsize)code.size());
+            f.close();
+            std::rename(tmp.c_str(), path.c_str());
+        }
+        return SYMX_HIP_OK;
+    }
+
+    double* vec_ptr(symx_hip_problem p, int32_t v)
+    {
+        if (v < 0 || v > 4) return nullptr;
+        return p->d_vec[v];
+    }
+
+    int check_problem(symx_hip_problem p)
+    {
+        if (!p) return fail(nullptr, SYMX_HIP_ERR_ARG, "null problem");
+        return SYMX_HIP_OK;
+    }
+
+    int sync_scalar(symx_hip_ctx ctx, const double* d, double* out)
+    {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        *out = ctx->h_pinned[0];
+        return SYMX_HIP_OK;
+    }
+}
+
+extern "C" {
+
+int symx_hip_device_count(int* n)
+{
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    if (n) *n = c;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_ctx_create(int device, symx_hip_ctx* out)
+{
+    if (!out) return fail(nullptr, SYMX_HIP_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c == 0) return fail(nullptr, SYMX_HIP_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= c) return fail(nullptr, SYMX_HIP_ERR_ARG, "device index out of range");
+    auto* ctx = new symx_hip_ctx_s();
+    ctx->device = device;
+    HIP_TRY(ctx, hipSetDevice(device));
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    int rc = dalloc(ctx, &ctx->d_ticket, 4);
+    if (!rc) rc = dalloc(ctx, &ctx->d_scalars, 64);
+    if (!rc) rc = dalloc(ctx, &ctx->d_pcg, 1);
+    if (!rc) rc = ensure_partials(ctx, 4096);
+    if (rc) {
+        symx_hip_ctx_destroy(ctx);
+        return rc;
+    }
+    HIP_TRY(ctx, hipMemset(ctx->d_ticket, 0, 4 * sizeof(unsigned int)));
+    HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pcg, sizeof(PcgState), hipHostMallocDefault));
+    *out = ctx;
+    return SYMX_HIP_OK;
+}
+
+void symx_hip_ctx_destroy(symx_hip_ctx ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    dfree(ctx->d_partials);
+    dfree(ctx->d_ticket);
+    dfree(ctx->d_scalars);
+    dfree(ctx->d_pcg);
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    if (ctx->h_pcg) (void)hipHostFree(ctx->h_pcg);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* symx_hip_last_error(symx_hip_ctx ctx)
+{
+    if (ctx) return ctx->err.c_str();
+    return g_thread_error.c_str();
+}
+
+const char* symx_hip_eval_prelude(void) { return kPrelude; }
+
+int symx_hip_jit_compile(const char* source, const char* arch, const char* cache_dir, char* key_out, int* was_cached, size_t* code_size)
+{
+    if (!source) return fail(nullptr, SYMX_HIP_ERR_ARG, "null source");
+    std::vector<char> code;
+    std::string key;
+    int cached = 0;
+    int rc = jit_compile_impl(nullptr, source, arch, cache_dir, code, key, cached);
+    if (rc) return rc;
+    if (key_out) std::memcpy(key_out, key.c_str(), key.size() + 1);
+    if (was_cached) *was_cached = cached;
+    if (code_size) *code_size = code.size();
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_module_load(symx_hip_ctx ctx, const char* source, const char* cache_dir, symx_hip_module* out)
+{
+    if (!ctx || !source || !out) return fail(ctx, SYMX_HIP_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::vector<char> code;
+    std::string key;
+    int cached = 0;
+    int rc = jit_compile_impl(ctx, source, "gfx950", cache_dir, code, key, cached);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    auto* m = new symx_hip_module_s();
+    m->ctx = ctx;
+    hipError_t e = hipModuleLoadData(&m->mod, code.data());
+    if (e != hipSuccess) {
+        delete m;
+        return fail(ctx, SYMX_HIP_ERR_JIT, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
+    }
+    *out = m;
+    return SYMX_HIP_OK;
+}
+
+void symx_hip_module_unload(symx_hip_module mod)
+{
+    if (!mod) return;
+    if (mod->mod) (void)hipModuleUnload(mod->mod);
+    delete mod;
+}
+
+static int module_function(symx_hip_module mod, const char* name, hipFunction_t* f)
+{
+    auto it = mod->fns.find(name);
+    if (it != mod->fns.end()) {
+        *f = it->second;
+        return SYMX_HIP_OK;
+    }
+    hipError_t e = hipModuleGetFunction(f, mod->mod, name);
+    if (e != hipSuccess) return fail(mod->ctx, SYMX_HIP_ERR_JIT, std::string("kernel not found in module: ") + name);
+    mod->fns[name] = *f;
+    return SYMX_HIP_OK;
+}
+
+static int launch_eval(symx_hip_ctx ctx, symx_hip_module mod, const std::string& name, const symx_hip_eval_params& params)
+{
+    if (params.n_elem <= 0) return SYMX_HIP_OK;
+    hipFunction_t f;
+    int rc = module_function(mod, name.c_str(), &f);
+    if (rc) return rc;
+    symx_hip_eval_params p = params;
+    size_t sz = sizeof(p);
+    void* config[] = { HIP_LAUNCH_PARAM_BUFFER_POINTER, &p, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END };
+    const unsigned grid = (unsigned)((params.n_elem + SYMX_HIP_BLOCK - 1) / SYMX_HIP_BLOCK);
+    HIP_TRY(ctx, hipModuleLaunchKernel(f, grid, 1, 1, SYMX_HIP_BLOCK, 1, 1, 0, ctx->stream, nullptr, config));
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_module_launch_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const symx_hip_eval_params* params)
+{
+    if (!ctx || !mod || !name || !params) return fail(ctx, SYMX_HIP_ERR_ARG, "null argument");
+    int rc = launch_eval(ctx, mod, name, *params);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SYMX_HIP_OK;
+}
+
+// ---- problem ------------------------------------------------------------------
+
+int symx_hip_problem_create(symx_hip_ctx ctx, int32_t n_dofs, symx_hip_problem* out)
+{
+    if (!ctx || !out) return fail(ctx, SYMX_HIP_ERR_ARG, "null argument");
+    if (n_dofs <= 0 || n_dofs % 3 != 0) return fail(ctx, SYMX_HIP_ERR_ARG, "n_dofs must be a positive multiple of 3");
+    auto* p = new symx_hip_problem_s();
+    p->ctx = ctx;
+    p->n_dofs = n_dofs;
+    p->nb = n_dofs / 3;
+    for (int v = 0; v < 5; v++) {
+        int rc = dalloc(ctx, &p->d_vec[v], (size_t)n_dofs);
+        if (rc) {
+            symx_hip_problem_destroy(p);
+            return rc;
+        }
+        HIP_TRY(ctx, hipMemsetAsync(p->d_vec[v], 0, sizeof(double) * n_dofs, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    *out = p;
+    return SYMX_HIP_OK;
+}
+
+void symx_hip_problem_destroy(symx_hip_problem p)
+{
+    if (!p) return;
+    (void)hipStreamSynchronize(p->ctx->stream);
+    for (auto& v : p->d_vec) dfree(v);
+    for (auto& pot : p->pots) {
+        dfree(pot.d_conn);
+        dfree(pot.d_active);
+        for (auto& m : pot.maps) dfree(m.d_data);
+    }
+    dfree(p->d_G);
+    dfree(p->d_H);
+    dfree(p->d_Epart);
+    dfree(p->d_goff);
+    dfree(p->d_glist);
+    dfree(p->sell.d_slice_ptr);
+    dfree(p->sell.d_col);
+    dfree(p->d_coff);
+    dfree(p->d_clist);
+    dfree(p->d_val64);
+    dfree(p->d_val32);
+    dfree(p->d_dinv64);
+    dfree(p->d_dinv32);
+    dfree(p->d_x); dfree(p->d_r); dfree(p->d_z); dfree(p->d_p0); dfree(p->d_p1); dfree(p->d_q);
+    dfree(p->f_x); dfree(p->f_r); dfree(p->f_z); dfree(p->f_p0); dfree(p->f_p1); dfree(p->f_q); dfree(p->f_b);
+    delete p;
+}
+
+int symx_hip_problem_add_potential(symx_hip_problem p, const symx_hip_potential_desc* d, int32_t* pot_id)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!d || !d->module || !d->kernel_P || !d->kernel_PG || !d->kernel_PGH) return fail(ctx, SYMX_HIP_ERR_ARG, "incomplete potential descriptor");
+    if (p->finalized) return fail(ctx, SYMX_HIP_ERR_ARG, "problem already finalized");
+    if (d->n_elem < 0 || d->conn_stride <= 0 || (d->n_elem > 0 && !d->conn)) return fail(ctx, SYMX_HIP_ERR_ARG, "bad connectivity");
+    if (d->n_maps < 0 || d->n_maps > SYMX_HIP_MAX_MAPS) return fail(ctx, SYMX_HIP_ERR_ARG, "too many data maps");
+    if (d->n_dof_blocks <= 0) return fail(ctx, SYMX_HIP_ERR_ARG, "potential without DoFs");
+    DevPotential pot;
+    pot.module = d->module;
+    pot.kP = d->kernel_P;
+    pot.kPG = d->kernel_PG;
+    pot.kPGH = d->kernel_PGH;
+    pot.kC = d->kernel_C ? d->kernel_C : "";
+    pot.n_elem = d->n_elem;
+    pot.stride = d->conn_stride;
+    pot.nb = d->n_dof_blocks;
+    pot.n_ub = pot.nb * (pot.nb + 1) / 2;
+    pot.dof_col.assign(d->dof_conn_col, d->dof_conn_col + pot.nb);
+    pot.dof_off.assign(d->dof_set_offset, d->dof_set_offset + pot.nb);
+    const int64_t ne = d->n_elem;
+    pot.h_conn.assign(d->conn, d->conn + ne * pot.stride);
+    for (int64_t i = 0; i < ne * pot.stride; i++) {
+        if (pot.h_conn[i] < 0) return fail(ctx, SYMX_HIP_ERR_ARG, "negative connectivity entry");
+    }
+    for (int a = 0; a < pot.nb; a++) {
+        if (pot.dof_col[a] < 0 || pot.dof_col[a] >= pot.stride) return fail(ctx, SYMX_HIP_ERR_ARG, "DoF connectivity column out of range");
+        if (pot.dof_off[a] < 0 || pot.dof_off[a] % 3 != 0) return fail(ctx, SYMX_HIP_ERR_ARG, "DoF set offset must be a multiple of 3");
+        for (int64_t e = 0; e < ne; e++) {
+            const int64_t dof = pot.dof_off[a] + 3 * (int64_t)pot.h_conn[e * pot.stride + pot.dof_col[a]];
+            if (dof + 3 > p->n_dofs) return fail(ctx, SYMX_HIP_ERR_ARG, "DoF index out of range");
+        }
+    }
+    // Device connectivity, transposed to [stride][n_elem] so that lane e reads conn[c*n+e].
+    {
+        std::vector<int32_t> t((size_t)ne * pot.stride);
+        for (int64_t e = 0; e < ne; e++)
+            for (int c = 0; c < pot.stride; c++) t[(size_t)c * ne + e] = pot.h_conn[(size_t)e * pot.stride + c];
+        int rc = dalloc(ctx, &pot.d_conn, t.size());
+        if (rc) return rc;
+        HIP_TRY(ctx, hipMemcpy(pot.d_conn, t.data(), t.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    for (int m = 0; m < d->n_maps; m++) {
+        DevMap dm;
+        dm.is_dof = d->maps[m].is_dof;
+        dm.dof_offset = d->maps[m].dof_offset;
+        dm.n_values = d->maps[m].n_values;
+        if (!dm.is_dof) {
+            if (dm.n_values <= 0 || !d->maps[m].host) return fail(ctx, SYMX_HIP_ERR_ARG, "data map without data");
+            int rc = dalloc(ctx, &dm.d_data, (size_t)dm.n_values);
+            if (rc) return rc;
+            HIP_TRY(ctx, hipMemcpy(dm.d_data, d->maps[m].host, dm.n_values * sizeof(double), hipMemcpyHostToDevice));
+        }
+        else if (dm.dof_offset < 0 || dm.dof_offset >= p->n_dofs) {
+            return fail(ctx, SYMX_HIP_ERR_ARG, "DoF map offset out of range");
+        }
+        pot.maps.push_back(dm);
+    }
+    if (!pot.kC.empty()) {
+        int rc = dalloc(ctx, &pot.d_active, (size_t)std::max<int64_t>(ne, 1));
+        if (rc) return rc;
+    }
+    p->pots.push_back(std::move(pot));
+    if (pot_id) *pot_id = (int32_t)p->pots.size() - 1;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_set_map_data(symx_hip_problem p, int32_t pot_id, int32_t map_id, const double* host, int64_t n)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (pot_id < 0 || pot_id >= (int32_t)p->pots.size()) return fail(ctx, SYMX_HIP_ERR_ARG, "bad potential id");
+    auto& pot = p->pots[pot_id];
+    if (map_id < 0 || map_id >= (int32_t)pot.maps.size()) return fail(ctx, SYMX_HIP_ERR_ARG, "bad map id");
+    auto& m = pot.maps[map_id];
+    if (m.is_dof) return fail(ctx, SYMX_HIP_ERR_ARG, "map is bound to the DoF vector");
+    if (n != m.n_values) return fail(ctx, SYMX_HIP_ERR_ARG, "data map size changed");
+    HIP_TRY(ctx, hipMemcpyAsync(m.d_data, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SYMX_HIP_OK;
+}
+
+// Topology: gradient scatter lists, block pattern (sliced, diagonal first) and
+// slot-sorted Hessian contribution lists. All orders are stable in
+// (potential, element, local block row, local block col), which fixes the
+// summation order of every output entry.
+int symx_hip_problem_finalize(symx_hip_problem p)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (p->finalized) return SYMX_HIP_OK;
+    if (p->pots.empty()) return fail(ctx, SYMX_HIP_ERR_ARG, "problem without potentials");
+    const int32_t nb = p->nb;
+
+    int64_t g = 0, h = 0, parts = 0;
+    uint64_t n_contrib = 0;
+    for (auto& pot : p->pots) {
+        pot.g_base = g;
+        pot.h_base = h;
+        pot.part_base = parts;
+        pot.n_groups = (pot.n_elem + SYMX_HIP_BLOCK - 1) / SYMX_HIP_BLOCK;
+        g += (int64_t)pot.n_elem * pot.nb;
+        h += (int64_t)pot.n_elem * pot.n_ub;
+        parts += pot.n_groups;
+        n_contrib += (uint64_t)pot.n_elem * pot.nb * pot.nb;
+    }
+    if (h * 2 + 2 >= (int64_t)UINT32_MAX || g >= (int64_t)UINT32_MAX || n_contrib >= (uint64_t)UINT32_MAX)
+        return fail(ctx, SYMX_HIP_ERR_ARG, "problem too large for 32-bit assembly maps");
+    p->total_g = g;
+    p->total_h = h;
+    p->total_parts = parts;
+
+    auto row_of = [&](const DevPotential& pot, int64_t e, int a) -> int32_t {
+        return pot.dof_off[a] / 3 + pot.h_conn[(size_t)e * pot.stride + pot.dof_col[a]];
+    };
+
+    // Gradient lists.
+    std::vector<uint32_t> goff((size_t)nb + 1, 0);
+    for (auto& pot : p->pots)
+        for (int64_t e = 0; e < pot.n_elem; e++)
+            for (int a = 0; a < pot.nb; a++) goff[row_of(pot, e, a) + 1]++;
+    for (int32_t r = 0; r < nb; r++) goff[r + 1] += goff[r];
+    std::vector<uint32_t> glist(goff[nb]);
+    {
+        std::vector<uint32_t> cur(goff.begin(), goff.end() - 1);
+        for (auto& pot : p->pots)
+            for (int64_t e = 0; e < pot.n_elem; e++)
+                for (int a = 0; a < pot.nb; a++) glist[cur[row_of(pot, e, a)]++] = (uint32_t)(pot.g_base + e * pot.nb + a);
+    }
+
+    // Block pattern: candidate columns per row, sorted and unique, diagonal first.
+    std::vector<uint64_t> coff((size_t)nb + 1, 0);
+    for (int32_t r = 0; r < nb; r++) coff[r + 1] = 1;  // every row keeps its diagonal block
+    for (auto& pot : p->pots)
+        for (int64_t e = 0; e < pot.n_elem; e++)
+            for (int a = 0; a < pot.nb; a++) coff[row_of(pot, e, a) + 1] += pot.nb;
+    for (int32_t r = 0; r < nb; r++) coff[r + 1] += coff[r];
+    std::vector<int32_t> ccol(coff[nb]);
+    {
+        std::vector<uint64_t> cur(coff.begin(), coff.end() - 1);
+        for (int32_t r = 0; r < nb; r++) ccol[cur[r]++] = r;
+        for (auto& pot : p->pots)
+            for (int64_t e = 0; e < pot.n_elem; e++)
+                for (int a = 0; a < pot.nb; a++) {
+                    const int32_t r = row_of(pot, e, a);
+                    for (int b = 0; b < pot.nb; b++) ccol[cur[r]++] = row_of(pot, e, b);
+                }
+    }
+    std::vector<int32_t> row_len(nb);
+    std::vector<uint64_t> row_begin(nb);
+    for (int32_t r = 0; r < nb; r++) {
+        int32_t* b = ccol.data() + coff[r];
+        int32_t* e = ccol.data() + coff[r + 1];
+        std::sort(b, e);
+        int32_t* u = std::unique(b, e);
+        const int32_t n = (int32_t)(u - b);
+        // diagonal first (always present), remaining columns ascending
+        int32_t* d = std::lower_bound(b, u, r);
+        std::rotate(b, d, d + 1);
+        row_len[r] = n;
+        row_begin[r] = coff[r];
+    }
+    SellPattern& S = p->sell;
+    S.n_rows = nb;
+    S.n_slices = (nb + 63) / 64;
+    S.h_slice_ptr.assign((size_t)S.n_slices + 1, 0);
+    S.h_row_len = row_len;
+    S.nnzb = 0;
+    for (int32_t s = 0; s < S.n_slices; s++) {
+        int32_t w = 0;
+        for (int32_t r = 64 * s; r < std::min(nb, 64 * s + 64); r++) w = std::max(w, row_len[r]);
+        S.h_slice_ptr[s + 1] = S.h_slice_ptr[s] + 64 * (int64_t)w;
+    }
+    for (int32_t r = 0; r < nb; r++) S.nnzb += row_len[r];
+    S.n_slots = S.h_slice_ptr[S.n_slices];
+    if (S.n_slots >= (int64_t)UINT32_MAX) return fail(ctx, SYMX_HIP_ERR_ARG, "block matrix too large");
+    S.h_col.assign(S.n_slots, 0);
+    for (int32_t r = 0; r < nb; r++) {
+        const int64_t base = S.h_slice_ptr[r >> 6] + (r & 63);
+        const int32_t s = r >> 6;
+        const int32_t w = (int32_t)((S.h_slice_ptr[s + 1] - S.h_slice_ptr[s]) >> 6);
+        for (int32_t j = 0; j < w; j++) S.h_col[base + 64 * (int64_t)j] = (j < row_len[r]) ? ccol[row_begin[r] + j] : r;
+    }
+    auto find_slot = [&](int32_t r, int32_t c) -> int64_t {
+        const int32_t* b = ccol.data() + row_begin[r];
+        int32_t j;
+        if (b[0] == c) j = 0;
+        else j = (int32_t)(std::lower_bound(b + 1, b + row_len[r], c) - b);
+        return S.h_slice_ptr[r >> 6] + 64 * (int64_t)j + (r & 63);
+    };
+
+    // Contributions, counting-sorted by slot (stable).
+    std::vector<uint32_t> slot_of(n_contrib);
+    std::vector<uint32_t> soff((size_t)S.n_slots + 1, 0);
+    {
+        uint64_t k = 0;
+        for (auto& pot : p->pots) {
+            std::vector<int32_t> rows(pot.nb);
+            for (int64_t e = 0; e < pot.n_elem; e++) {
+                for (int a = 0; a < pot.nb; a++) rows[a] = row_of(pot, e, a);
+                for (int a = 0; a < pot.nb; a++)
+                    for (int b = 0; b < pot.nb; b++) {
+                        const uint32_t sl = (uint32_t)find_slot(rows[a], rows[b]);
+                        slot_of[k++] = sl;
+                        soff[sl + 1]++;
+                    }
+            }
+        }
+    }
+    for (int64_t t = 0; t < S.n_slots; t++) soff[t + 1] += soff[t];
+    std::vector<uint32_t> clist(n_contrib);
+    {
+        std::vector<uint32_t> cur(soff.begin(), soff.end() - 1);
+        uint64_t k = 0;
+        for (auto& pot : p->pots) {
+            for (int64_t e = 0; e < pot.n_elem; e++) {
+                for (int a = 0; a < pot.nb; a++)
+                    for (int b = 0; b < pot.nb; b++) {
+                        const int lo = std::min(a, b), hi = std::max(a, b);
+                        const int ub = lo * pot.nb - (lo * (lo - 1)) / 2 + (hi - lo);
+                        const uint64_t blk = (uint64_t)pot.h_base + (uint64_t)e * pot.n_ub + ub;
+                        clist[cur[slot_of[k++]]++] = (uint32_t)(blk * 2 + (a > b ? 1u : 0u));
+                    }
+            }
+        }
+    }
+
+    // Upload.
+    int rc = 0;
+    rc = rc ? rc : dalloc(ctx, &p->d_G, (size_t)std::max<int64_t>(g * 3, 1));
+    rc = rc ? rc : dalloc(ctx, &p->d_H, (size_t)std::max<int64_t>(h * 9, 1));
+    rc = rc ? rc : dalloc(ctx, &p->d_Epart, (size_t)std::max<int64_t>(parts, 1));
+    rc = rc ? rc : dalloc(ctx, &p->d_goff, goff.size());
+    rc = rc ? rc : dalloc(ctx, &p->d_glist, std::max<size_t>(glist.size(), 1));
+    rc = rc ? rc : dalloc(ctx, &S.d_slice_ptr, S.h_slice_ptr.size());
+    rc = rc ? rc : dalloc(ctx, &S.d_col, (size_t)S.n_slots);
+    rc = rc ? rc : dalloc(ctx, &p->d_coff, soff.size());
+    rc = rc ? rc : dalloc(ctx, &p->d_clist, std::max<size_t>(clist.size(), 1));
+    const size_t nd = (size_t)p->n_dofs;
+    rc = rc ? rc : dalloc(ctx, &p->d_x, nd);
+    rc = rc ? rc : dalloc(ctx, &p->d_r, nd);
+    rc = rc ? rc : dalloc(ctx, &p->d_z, nd);
+    rc = rc ? rc : dalloc(ctx, &p->d_p0, nd);
+    rc = rc ? rc : dalloc(ctx, &p->d_p1, nd);
+    rc = rc ? rc : dalloc(ctx, &p->d_q, nd);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpy(p->d_goff, goff.data(), goff.size() * 4, hipMemcpyHostToDevice));
+    if (!glist.empty()) HIP_TRY(ctx, hipMemcpy(p->d_glist, glist.data(), glist.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(S.d_slice_ptr, S.h_slice_ptr.data(), S.h_slice_ptr.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(S.d_col, S.h_col.data(), S.h_col.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(p->d_coff, soff.data(), soff.size() * 4, hipMemcpyHostToDevice));
+    if (!clist.empty()) HIP_TRY(ctx, hipMemcpy(p->d_clist, clist.data(), clist.size() * 4, hipMemcpyHostToDevice));
+    rc = ensure_partials(ctx, (size_t)std::max<int64_t>(2 * (int64_t)reduction_groups(p->n_dofs) + 2 * ((nb + 255) / 256) + 16, 4096));
+    if (rc) return rc;
+    p->finalized = true;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_set_dofs(symx_hip_problem p, const double* host)
+{
+    return symx_hip_problem_upload(p, SYMX_HIP_VEC_DOFS, host);
+}
+int symx_hip_problem_get_dofs(symx_hip_problem p, double* host)
+{
+    return symx_hip_problem_download(p, SYMX_HIP_VEC_DOFS, host);
+}
+int symx_hip_problem_download(symx_hip_problem p, int32_t vec, double* host)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    double* d = vec_ptr(p, vec);
+    if (!d || !host) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id or null pointer");
+    HIP_TRY(p->ctx, hipMemcpyAsync(host, d, sizeof(double) * p->n_dofs, hipMemcpyDeviceToHost, p->ctx->stream));
+    HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));
+    return SYMX_HIP_OK;
+}
+int symx_hip_problem_upload(symx_hip_problem p, int32_t vec, const double* host)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    double* d = vec_ptr(p, vec);
+    if (!d || !host) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id or null pointer");
+    HIP_TRY(p->ctx, hipMemcpyAsync(d, host, sizeof(double) * p->n_dofs, hipMemcpyHostToDevice, p->ctx->stream));
+    HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));
+    return SYMX_HIP_OK;
+}
+int symx_hip_problem_device_ptr(symx_hip_problem p, int32_t vec, double** dptr)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    double* d = vec_ptr(p, vec);
+    if (!d || !dptr) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id or null pointer");
+    *dptr = d;
+    return SYMX_HIP_OK;
+}
+
+static void fill_params(symx_hip_problem p, DevPotential& pot, symx_hip_eval_params& prm)
+{
+    std::memset(&prm, 0, sizeof(prm));
+    prm.conn = pot.d_conn;
+    prm.active = pot.kC.empty() ? nullptr : pot.d_active;
+    for (size_t m = 0; m < pot.maps.size(); m++)
+        prm.maps[m] = pot.maps[m].is_dof ? p->d_vec[SYMX_HIP_VEC_DOFS] + pot.maps[m].dof_offset : pot.maps[m].d_data;
+    prm.E_partial = p->d_Epart + pot.part_base;
+    prm.G = p->d_G + pot.g_base * 3;
+    prm.H = p->d_H + pot.h_base * 9;
+    prm.C_out = pot.d_active;
+    prm.n_elem = pot.n_elem;
+}
+
+int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->finalized) return fail(ctx, SYMX_HIP_ERR_ARG, "problem not finalized");
+    if (mode < SYMX_HIP_EVAL_P || mode > SYMX_HIP_EVAL_PGH) return fail(ctx, SYMX_HIP_ERR_ARG, "bad evaluation mode");
+    for (auto& pot : p->pots) {
+        if (pot.n_elem == 0) continue;
+        symx_hip_eval_params prm;
+        fill_params(p, pot, prm);
+        if (!pot.kC.empty()) {
+            int rc = launch_eval(ctx, pot.module, pot.kC, prm);
+            if (rc) return rc;
+        }
+        const std::string& k = mode == SYMX_HIP_EVAL_P ? pot.kP : (mode == SYMX_HIP_EVAL_PG ? pot.kPG : pot.kPGH);
+        int rc = launch_eval(ctx, pot.module, k, prm);
+        if (rc) return rc;
+    }
+    launch_sum(ctx->stream, p->d_Epart, p->total_parts, ctx->d_scalars);
+    if (mode >= SYMX_HIP_EVAL_PG) launch_grad_assemble(ctx->stream, p->d_goff, p->d_glist, p->d_G, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb);
+    if (mode == SYMX_HIP_EVAL_PGH) p->h_valid = true;
+    HIP_TRY(ctx, hipGetLastError());
+    double e = 0.0;
+    int rc = sync_scalar(ctx, ctx->d_scalars, &e);
+    if (rc) return rc;
+    if (E) *E = e;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_n_element_hessians(symx_hip_problem p, int64_t* n)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    int64_t c = 0;
+    for (auto& pot : p->pots) c += pot.n_elem;
+    if (n) *n = c;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_element_outputs(symx_hip_problem p, int32_t pot_id, double* G, double* H)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->finalized || pot_id < 0 || pot_id >= (int32_t)p->pots.size()) return fail(ctx, SYMX_HIP_ERR_ARG, "bad potential id");
+    auto& pot = p->pots[pot_id];
+    if (G) HIP_TRY(ctx, hipMemcpy(G, p->d_G + pot.g_base * 3, sizeof(double) * pot.n_elem * pot.nb * 3, hipMemcpyDeviceToHost));
+    if (H) HIP_TRY(ctx, hipMemcpy(H, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToHost));
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int64_t* n_changed)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->h_valid) return fail(ctx, SYMX_HIP_ERR_ARG, "no element Hessians (evaluate with EVAL_PGH first)");
+    unsigned long long* d_count = (unsigned long long*)(ctx->d_scalars + 8);
+    HIP_TRY(ctx, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), ctx->stream));
+    for (auto& pot : p->pots) {
+        const double skip_tol = 0.0;
+        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, skip_tol, d_count))
+            return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports up to 4 DoF blocks per element");
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, d_count, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long c;
+    std::memcpy(&c, ctx->h_pinned, 8);
+    if (n_changed) *n_changed = (int64_t)c;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->h_valid) return fail(ctx, SYMX_HIP_ERR_ARG, "no element Hessians (evaluate with EVAL_PGH first)");
+    const SellPattern& S = p->sell;
+    const size_t nval = (size_t)S.n_slots * 9;
+    if (precision == SYMX_HIP_FP64) {
+        if (!p->d_val64) {
+            int rc = dalloc(ctx, &p->d_val64, nval);
+            if (!rc) rc = dalloc(ctx, &p->d_dinv64, (size_t)S.n_rows * 9);
+            if (rc) return rc;
+        }
+        launch_bsr_assemble(ctx->stream, S, p->d_coff, p->d_clist, p->d_H, p->d_val64, nullptr);
+        launch_dinv(ctx->stream, S, p->d_val64, nullptr, p->d_dinv64, nullptr);
+    }
+    else if (precision == SYMX_HIP_MIXED) {
+        if (!p->d_val32) {
+            int rc = dalloc(ctx, &p->d_val32, nval);
+            if (!rc) rc = dalloc(ctx, &p->d_dinv32, (size_t)S.n_rows * 9);
+            const size_t nd = (size_t)p->n_dofs;
+            if (!rc) rc = dalloc(ctx, &p->f_x, nd);
+            if (!rc) rc = dalloc(ctx, &p->f_r, nd);
+            if (!rc) rc = dalloc(ctx, &p->f_z, nd);
+            if (!rc) rc = dalloc(ctx, &p->f_p0, nd);
+            if (!rc) rc = dalloc(ctx, &p->f_p1, nd);
+            if (!rc) rc = dalloc(ctx, &p->f_q, nd);
+            if (!rc) rc = dalloc(ctx, &p->f_b, nd);
+            if (rc) return rc;
+        }
+        launch_bsr_assemble(ctx->stream, S, p->d_coff, p->d_clist, p->d_H, nullptr, p->d_val32);
+        launch_dinv(ctx->stream, S, nullptr, p->d_val32, nullptr, p->d_dinv32);
+    }
+    else {
+        return fail(ctx, SYMX_HIP_ERR_ARG, "bad precision");
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    p->assembled_precision = precision;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_export_bsr(symx_hip_problem p, int64_t* nnzb, int64_t* row_ptr, int32_t* cols, double* vals)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->finalized) return fail(ctx, SYMX_HIP_ERR_ARG, "problem not finalized");
+    const SellPattern& S = p->sell;
+    if (nnzb) *nnzb = S.nnzb;
+    if (!row_ptr && !cols && !vals) return SYMX_HIP_OK;
+    std::vector<double> v;
+    if (vals) {
+        if (p->assembled_precision < 0) return fail(ctx, SYMX_HIP_ERR_ARG, "matrix not assembled");
+        v.resize((size_t)S.n_slots * 9);
+        if (p->assembled_precision == SYMX_HIP_FP64) {
+            HIP_TRY(ctx, hipMemcpy(v.data(), p->d_val64, v.size() * 8, hipMemcpyDeviceToHost));
+        }
+        else {
+            std::vector<float> f(v.size());
+            HIP_TRY(ctx, hipMemcpy(f.data(), p->d_val32, f.size() * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < f.size(); i++) v[i] = f[i];
+        }
+    }
+    // Output rows with ascending columns (standard block-CSR).
+    int64_t k = 0;
+    for (int32_t r = 0; r < S.n_rows; r++) {
+        if (row_ptr) row_ptr[r] = k;
+        const int64_t base = S.h_slice_ptr[r >> 6] + (r & 63);
+        std::vector<std::pair<int32_t, int32_t>> order;
+        for (int32_t j = 0; j < S.h_row_len[r]; j++) order.push_back({ S.h_col[base + 64 * (int64_t)j], j });
+        std::sort(order.begin(), order.end());
+        for (auto& cj : order) {
+            if (cols) cols[k] = cj.first;
+            if (vals) {
+                const int64_t t = base + 64 * (int64_t)cj.second;
+                for (int q = 0; q < 9; q++) vals[k * 9 + q] = v[((t >> 6) * 9 + q) * 64 + (t & 63)];
+            }
+            k++;
+        }
+    }
+    if (row_ptr) row_ptr[S.n_rows] = k;
+    return SYMX_HIP_OK;
+}
+
+static int run_pcg64(symx_hip_problem p, const double* b, double* x, double abs_tol, double rel_tol, int32_t max_iter, int32_t stop_on_indef,
+                     PcgState& out)
+{
+    symx_hip_ctx ctx = p->ctx;
+    const SellPattern& S = p->sell;
+    PcgState st{};
+    st.abs_tol = abs_tol;
+    st.rel_tol = rel_tol;
+    st.max_iter = max_iter;
+    st.stop_on_indef = stop_on_indef;
+    *ctx->h_pcg = st;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pcg, ctx->h_pcg, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+    launch_pcg_init64(ctx->stream, S, p->d_dinv64, b, x, p->d_r, p->d_z, p->d_p0, ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
+    double* pb[2] = { p->d_p0, p->d_p1 };
+    int64_t launched = 0;
+    int batch = 8;
+    while (true) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pcg, ctx->d_pcg, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->h_pcg->done) break;
+        for (int i = 0; i < batch; i++, launched++) {
+            launch_pcg_iter64(ctx->stream, S, p->d_val64, p->d_dinv64, x, p->d_r, p->d_z, pb[launched & 1], pb[(launched + 1) & 1], p->d_q,
+                              ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
+        }
+        HIP_TRY(ctx, hipGetLastError());
+        batch = std::min(batch * 2, 64);
+    }
+    out = *ctx->h_pcg;
+    return SYMX_HIP_OK;
+}
+
+static int run_pcg32(symx_hip_problem p, double rel_tol, int32_t max_iter, int32_t stop_on_indef, PcgState& out)
+{
+    symx_hip_ctx ctx = p->ctx;
+    const SellPattern& S = p->sell;
+    PcgState st{};
+    st.abs_tol = 0.0;
+    st.rel_tol = rel_tol;
+    st.max_iter = max_iter;
+    st.stop_on_indef = stop_on_indef;
+    *ctx->h_pcg = st;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pcg, ctx->h_pcg, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+    launch_pcg_init32(ctx->stream, S, p->d_dinv32, p->f_b, p->f_x, p->f_r, p->f_z, p->f_p0, ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
+    float* pb[2] = { p->f_p0, p->f_p1 };
+    int64_t launched = 0;
+    int batch = 8;
+    while (true) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pcg, ctx->d_pcg, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->h_pcg->done) break;
+        for (int i = 0; i < batch; i++, launched++) {
+            launch_pcg_iter32(ctx->stream, S, p->d_val32, p->d_dinv32, p->f_x, p->f_r, p->f_z, pb[launched & 1], pb[(launched + 1) & 1], p->f_q,
+                              ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
+        }
+        HIP_TRY(ctx, hipGetLastError());
+        batch = std::min(batch * 2, 64);
+    }
+    out = *ctx->h_pcg;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_solve(symx_hip_problem p, int32_t precision, double abs_tol, double rel_tol, int32_t max_iter, int32_t stop_on_indef,
+                           symx_hip_pcg_info* info)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (p->assembled_precision != precision) return fail(ctx, SYMX_HIP_ERR_ARG, "matrix not assembled in the requested precision");
+    symx_hip_pcg_info res{};
+    double* b = p->d_vec[SYMX_HIP_VEC_RHS];
+    double* x = p->d_vec[SYMX_HIP_VEC_DU];
+    if (precision == SYMX_HIP_FP64) {
+        PcgState st;
+        int rc = run_pcg64(p, b, x, abs_tol, rel_tol, max_iter, stop_on_indef, st);
+        if (rc) return rc;
+        res.n_iterations = st.it;
+        res.converged = st.converged;
+        res.found_indefiniteness = st.indef;
+        res.error = st.err;
+        res.n_refinements = 0;
+    }
+    else {
+        // FP64 iterative refinement around FP32 PCG.
+        const SellPattern& S = p->sell;
+        double* rr_dev = ctx->d_scalars + 16;
+        HIP_TRY(ctx, hipMemsetAsync(x, 0, sizeof(double) * p->n_dofs, ctx->stream));
+        // r = b - A*0 = b exactly, r32 = (float) b, rr = ||b||^2
+        launch_residual_mixed(ctx->stream, S, p->d_val32, b, x, p->d_r, p->f_b, rr_dev, ctx->d_partials, ctx->d_ticket);
+        double bb = 0.0;
+        int rc = sync_scalar(ctx, rr_dev, &bb);
+        if (rc) return rc;
+        res.converged = 0;
+        if (bb < abs_tol * abs_tol) {
+            res.converged = 1;
+            res.error = 0.0;
+        }
+        else {
+            const double err0 = std::sqrt(bb / bb);
+            double err = err0;
+            int total = 0;
+            if (err < abs_tol) res.converged = 1;
+            const double target = std::max(abs_tol, rel_tol * err0);
+            for (int outer = 0; !res.converged && outer < 16 && total < max_iter; outer++) {
+                const double inner_rel = std::min(0.5, std::max(0.5 * target / err, 2e-6));
+                PcgState st;
+                rc = run_pcg32(p, inner_rel, max_iter - total, stop_on_indef, st);
+                if (rc) return rc;
+                total += st.it;
+                res.n_refinements++;
+                if (st.indef) res.found_indefiniteness = 1;
+                if (st.indef && stop_on_indef) {
+                    // Keep x from before this inner solve, like the FP64 path keeps its last iterate.
+                    break;
+                }
+                launch_x_update_mixed(ctx->stream, p->n_dofs, x, p->f_x);
+                launch_residual_mixed(ctx->stream, S, p->d_val32, b, x, p->d_r, p->f_b, rr_dev, ctx->d_partials, ctx->d_ticket);
+                double rr = 0.0;
+                rc = sync_scalar(ctx, rr_dev, &rr);
+                if (rc) return rc;
+                err = std::sqrt(rr / bb);
+                if (err < abs_tol || err / err0 < rel_tol) res.converged = 1;
+            }
+            res.n_iterations = total;
+            res.error = err;
+        }
+    }
+    if (info) *info = res;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_max_abs(symx_hip_problem p, int32_t vec, double* out)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    double* d = vec_ptr(p, vec);
+    if (!d) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
+    launch_max_abs(p->ctx->stream, p->n_dofs, d, p->ctx->d_scalars, p->ctx->d_partials, p->ctx->d_ticket);
+    return sync_scalar(p->ctx, p->ctx->d_scalars, out);
+}
+int symx_hip_problem_dot(symx_hip_problem p, int32_t a, int32_t b, double* out)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    double* da = vec_ptr(p, a);
+    double* db = vec_ptr(p, b);
+    if (!da || !db) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
+    launch_dot(p->ctx->stream, p->n_dofs, da, db, p->ctx->d_scalars, p->ctx->d_partials, p->ctx->d_ticket);
+    return sync_scalar(p->ctx, p->ctx->d_scalars, out);
+}
+int symx_hip_problem_axpy(symx_hip_problem p, int32_t dst, int32_t a, double s, int32_t b)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    double* dd = vec_ptr(p, dst);
+    double* da = vec_ptr(p, a);
+    double* db = vec_ptr(p, b);
+    if (!dd || !da || !db) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
+    launch_axpy(p->ctx->stream, p->n_dofs, dd, da, s, db);
+    HIP_TRY(p->ctx, hipGetLastError());
+    return SYMX_HIP_OK;
+}
+int symx_hip_problem_scale(symx_hip_problem p, int32_t dst, int32_t a, double s)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    double* dd = vec_ptr(p, dst);
+    double* da = vec_ptr(p, a);
+    if (!dd || !da) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
+    launch_scale(p->ctx->stream, p->n_dofs, dd, da, s);
+    HIP_TRY(p->ctx, hipGetLastError());
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_batch_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const double* host_in, int64_t n, int32_t n_in,
+                        double* host_out, int32_t n_out)
+{
+    if (!ctx || !mod || !name || n < 0 || n_in < 0 || n_out <= 0) return fail(ctx, SYMX_HIP_ERR_ARG, "bad argument");
+    if (n == 0) return SYMX_HIP_OK;
+    if (n > INT32_MAX) return fail(ctx, SYMX_HIP_ERR_ARG, "batch too large");
+    hipFunction_t f;
+    int rc = module_function(mod, name, &f);
+    if (rc) return rc;
+    double *d_in = nullptr, *d_out = nullptr;
+    rc = dalloc(ctx, &d_in, (size_t)std::max<int64_t>(n * n_in, 1));
+    if (!rc) rc = dalloc(ctx, &d_out, (size_t)(n * n_out));
+    if (rc) {
+        dfree(d_in);
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (n_in > 0) e = hipMemcpyAsync(d_in, host_in, sizeof(double) * n * n_in, hipMemcpyHostToDevice, ctx->stream);
+    int nn = (int)n;
+    void* args[] = { &d_in, &d_out, &nn };
+    if (e == hipSuccess)
+        e = hipModuleLaunchKernel(f, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, ctx->stream, args, nullptr);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, d_out, sizeof(double) * n * n_out, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(d_in);
+    dfree(d_out);
+    if (e != hipSuccess) return fail(ctx, SYMX_HIP_ERR_HIP, std::string("batch evaluation: ") + hipGetErrorString(e));
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_project_dense(symx_hip_ctx ctx, double* d_mats, int64_t n_mats, int32_t n, double eps, int32_t mirror, uint8_t* d_changed)
+{
+    if (!ctx || (!d_mats && n_mats > 0)) return fail(ctx, SYMX_HIP_ERR_ARG, "null argument");
+    if (launch_project_dense(ctx->stream, d_mats, n_mats, n, eps, mirror, d_changed))
+        return fail(ctx, SYMX_HIP_ERR_ARG, "dense projection supports n in {3, 6, 9, 12}");
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SYMX_HIP_OK;
+}
+
+} // extern "C"

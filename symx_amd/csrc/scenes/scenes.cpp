@@ -1,0 +1,526 @@
+// Example problems behind include/symx_scenes.h, written against the C++ front
+// end exactly as a SymX user would write them (cf. examples/NeoHookean.cpp,
+// examples/rubber_extrusion.cpp:68-107, examples/dynamic_elasticity_with_contact.cpp:67-102).
+#include "../../../include/symx_scenes.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../api/solver.h"
+
+using namespace symx;
+
+namespace
+{
+    thread_local std::string g_err;
+
+    int guard(const std::function<void()>& f)
+    {
+        try {
+            f();
+            return 0;
+        }
+        catch (const std::exception& e) {
+            g_err = e.what();
+            return 1;
+        }
+    }
+
+    // SplitMix64 -> uniform double in [0, 1) with 53 random bits.
+    struct SplitMix64
+    {
+        uint64_t s;
+        explicit SplitMix64(uint64_t seed) : s(seed) {}
+        uint64_t next()
+        {
+            uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            return z ^ (z >> 31);
+        }
+        double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    };
+
+    Scalar neohookean_energy_density(const Matrix& F, const Scalar& l, const Scalar& m)
+    {
+        // 0.5 mu (I_C - 3) - mu ln J + 0.5 lambda ln^2 J (examples/elasticity_potentials.h:7-13)
+        const Matrix C = F.transpose() * F;
+        const Scalar Ic = C.trace();
+        const Scalar logdetF = log(F.det());
+        return 0.5 * m * (Ic - 3.0) - m * logdetF + 0.5 * l * logdetF.powN(2);
+    }
+
+    Scalar stable_neohookean_energy_density(const Matrix& F, const Scalar& youngs, const Scalar& poisson)
+    {
+        // Smith et al. stable Neo-Hookean (examples/elasticity_potentials.h:15-30)
+        const Scalar mu = youngs / (2.0 * (1.0 + poisson));
+        const Scalar lambda = (youngs * poisson) / ((1.0 + poisson) * (1.0 - 2.0 * poisson));
+        const Scalar mu_ = 4.0 / 3.0 * mu;
+        const Scalar lambda_ = lambda + 5.0 / 6.0 * mu;
+        const Scalar alpha = 1.0 + mu_ / lambda_ - mu_ / (4.0 * lambda_);
+        const Scalar detF = F.det();
+        const Scalar Ic = F.frobenius_norm_sq();
+        return 0.5 * mu_ * (Ic - 3.0) + 0.5 * lambda_ * (detF - alpha).powN(2) - 0.5 * mu_ * log(Ic + 1.0);
+    }
+
+    // Elastic Tet4 potential through fem_integrator, as the reference examples write it.
+    Scalar tet4_elastic(MappedWorkspace<double>& mws, const std::vector<Vector>& x, const std::vector<Vector>& X, const Scalar& a,
+                        const Scalar& b, int material)
+    {
+        return fem_integrator(mws, FEM_Element::Tet4, [&](Scalar& w, Vector& xi) {
+            const Matrix Jx = fem_jacobian(FEM_Element::Tet4, x, xi);
+            const Matrix JX = fem_jacobian(FEM_Element::Tet4, X, xi);
+            const Matrix F = Jx * JX.inv();
+            const Scalar psi = material == 0 ? neohookean_energy_density(F, a, b) : stable_neohookean_energy_density(F, a, b);
+            return psi * JX.det() * w;
+        });
+    }
+}
+
+struct symx_scene_s
+{
+    symx_tet4_block_params prm{};
+    symx_newton_params np{};
+    std::vector<std::array<double, 3>> x, x0, v, X;
+    std::vector<double> mass;
+    std::vector<int32_t> tets;      // [n][4]
+    std::vector<int32_t> tets_enum; // [n][5]: element index, v0..v3
+    std::vector<int32_t> all_vertices;
+    LabelledConnectivity<2> bc{ { "constraint_idx", "vertex_idx" } };
+    std::vector<std::array<double, 3>> bc_target;
+    std::vector<double> log_mu;
+    double lambda = 0, mu = 0, lam_over_mu = 0, dt = 0, k = 0;
+    double E_ = 0, nu_ = 0;
+    std::array<double, 3> gravity{};
+    std::array<double, 3> mg{};
+    spContext ctx;
+    spGlobalPotential G;
+    spNewtonsMethod newton;
+};
+
+static void apply_newton_params(NewtonSettings& s, const symx_newton_params& p)
+{
+    s.precision = p.precision == SYMX_HIP_MIXED ? Precision::Mixed : Precision::FP64;
+    s.projection_mode = p.projection_mode == 0 ? ProjectionToPD::Newton
+                        : p.projection_mode == 2 ? ProjectionToPD::ProjectOnDemand
+                                                 : ProjectionToPD::ProjectedNewton;
+    s.project_to_pd_use_mirroring = p.use_mirroring != 0;
+    s.max_iterations = p.max_iterations;
+    s.residual_tolerance_abs = p.residual_tolerance_abs;
+    s.residual_tolerance_rel = p.residual_tolerance_rel;
+    s.cg_rel_tolerance = p.cg_rel_tolerance;
+    s.cg_abs_tolerance = p.cg_abs_tolerance;
+    s.cg_max_iterations = p.cg_max_iterations;
+    s.projection_eps = p.projection_eps;
+    s.step_tolerance = p.step_tolerance;
+}
+
+extern "C" {
+
+void symx_newton_params_default(symx_newton_params* p)
+{
+    p->precision = SYMX_HIP_FP64;
+    p->projection_mode = 1;
+    p->use_mirroring = 0;
+    p->max_iterations = 100;
+    p->residual_tolerance_abs = 0.0;
+    p->residual_tolerance_rel = 1e-6;
+    p->cg_rel_tolerance = 1e-3;
+    p->cg_abs_tolerance = 1e-12;
+    p->cg_max_iterations = 10000;
+    p->projection_eps = 1e-10;
+    p->step_tolerance = 0.0;
+}
+
+void symx_tet4_block_params_default(symx_tet4_block_params* p)
+{
+    p->cells[0] = p->cells[1] = p->cells[2] = 20;
+    p->size[0] = p->size[1] = p->size[2] = 1.0;
+    p->youngs_modulus = 1e5;
+    p->poissons_ratio = 0.4;
+    p->density = 1000.0;
+    p->gravity[0] = 0.0;
+    p->gravity[1] = 0.0;
+    p->gravity[2] = -9.81;
+    p->dt = 0.01;
+    p->noise = 0.01;
+    p->seed = 1;
+    p->bc_stiffness = 1e8;
+    p->fixed_axis = 2;
+    p->material = 0;
+    p->log_mu_field = 0;
+}
+
+const char* symx_scene_last_error(void) { return g_err.c_str(); }
+
+int symx_scene_tet4_block_create(const symx_tet4_block_params* p, const symx_newton_params* np, symx_scene* out)
+{
+    *out = nullptr;
+    auto* s = new symx_scene_s();
+    int rc = guard([&]() {
+        s->prm = *p;
+        s->np = *np;
+        const FEMMesh mesh = generate_cuboid_Tet4_mesh({ p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] });
+        const int32_t nv = (int32_t)mesh.vertices.size();
+        s->X = mesh.vertices;
+        s->tets = mesh.connectivity;
+        s->x = s->X;
+        const double h_min = std::min({ p->size[0] / p->cells[0], p->size[1] / p->cells[1], p->size[2] / p->cells[2] });
+        (void)h_min;
+        SplitMix64 rng(p->seed);
+        std::vector<uint8_t> fixed(nv, 0);
+        for (int32_t i = 0; i < nv; i++) {
+            if (p->fixed_axis >= 0 && std::abs(s->X[i][p->fixed_axis]) < 1e-12) fixed[i] = 1;
+            for (int d = 0; d < 3; d++) {
+                const double u = rng.uniform();
+                if (!fixed[i]) s->x[i][d] += p->noise * (2.0 * u - 1.0);
+            }
+        }
+        s->x0 = s->x;
+        s->v.assign(nv, { 0.0, 0.0, 0.0 });
+        // lumped masses (dynamic_elasticity_with_contact.cpp:67-79)
+        s->mass.assign(nv, 0.0);
+        const int32_t ne = mesh.n_elements;
+        for (int32_t e = 0; e < ne; e++) {
+            const int32_t* t = &s->tets[4 * (size_t)e];
+            const auto& A = s->X[t[0]];
+            double d1[3], d2[3], d3[3];
+            for (int k = 0; k < 3; k++) {
+                d1[k] = s->X[t[1]][k] - A[k];
+                d2[k] = s->X[t[2]][k] - A[k];
+                d3[k] = s->X[t[3]][k] - A[k];
+            }
+            const double det = d1[0] * (d2[1] * d3[2] - d2[2] * d3[1]) - d1[1] * (d2[0] * d3[2] - d2[2] * d3[0]) + d1[2] * (d2[0] * d3[1] - d2[1] * d3[0]);
+            const double lumped = std::abs(det) / 6.0 * p->density / 4.0;
+            for (int k = 0; k < 4; k++) s->mass[t[k]] += lumped;
+        }
+        s->all_vertices.resize(nv);
+        for (int32_t i = 0; i < nv; i++) s->all_vertices[i] = i;
+        for (int32_t i = 0; i < nv; i++) {
+            if (fixed[i]) {
+                s->bc.numbered_push_back({ i });
+                s->bc_target.push_back(s->X[i]);
+            }
+        }
+        s->tets_enum.resize((size_t)ne * 5);
+        for (int32_t e = 0; e < ne; e++) {
+            s->tets_enum[5 * (size_t)e] = e;
+            for (int k = 0; k < 4; k++) s->tets_enum[5 * (size_t)e + 1 + k] = s->tets[4 * (size_t)e + k];
+        }
+        s->E_ = p->youngs_modulus;
+        s->nu_ = p->poissons_ratio;
+        s->mu = p->youngs_modulus / (2.0 * (1.0 + p->poissons_ratio));
+        s->lambda = p->youngs_modulus * p->poissons_ratio / ((1.0 + p->poissons_ratio) * (1.0 - 2.0 * p->poissons_ratio));
+        s->lam_over_mu = s->lambda / s->mu;
+        s->log_mu.assign(ne, std::log(s->mu));
+        s->dt = p->dt;
+        s->k = p->bc_stiffness;
+        s->gravity = { p->gravity[0], p->gravity[1], p->gravity[2] };
+
+        s->G = GlobalPotential::create();
+        symx_scene_s* S = s;
+        const int material = p->material;
+        if (p->log_mu_field) {
+            S->G->add_potential("neohookean_tet4_logmu", S->tets_enum, 5, [S, material](MappedWorkspace<double>& mws, Element& elem) {
+                std::vector<Vector> xs = mws.make_vectors(S->x, elem.slice(1, 5));
+                std::vector<Vector> Xs = mws.make_vectors(S->X, elem.slice(1, 5));
+                const Scalar logmu = mws.make_scalar(S->log_mu, elem[0]);
+                const Scalar ratio = mws.make_scalar(S->lam_over_mu);
+                const Scalar m = exp(logmu);
+                return tet4_elastic(mws, xs, Xs, ratio * m, m, 0);
+                (void)material;
+            });
+        }
+        else {
+            S->G->add_potential(material == 0 ? "neohookean_tet4" : "stable_neohookean_tet4", S->tets, 4,
+                                [S, material](MappedWorkspace<double>& mws, Element& elem) {
+                                    std::vector<Vector> xs = mws.make_vectors(S->x, elem);
+                                    std::vector<Vector> Xs = mws.make_vectors(S->X, elem);
+                                    const Scalar a = material == 0 ? mws.make_scalar(S->lambda) : mws.make_scalar(S->E_);
+                                    const Scalar b = material == 0 ? mws.make_scalar(S->mu) : mws.make_scalar(S->nu_);
+                                    return tet4_elastic(mws, xs, Xs, a, b, material);
+                                });
+        }
+        if (p->dt > 0.0) {
+            // backward Euler inertia with gravity (dynamic_elasticity_with_contact.cpp:85-102, no damping)
+            S->G->add_potential("be_inertia", S->all_vertices, 1, [S](MappedWorkspace<double>& mws, Element& elem) {
+                const Vector x1 = mws.make_vector(S->x, elem[0]);
+                const Vector x0 = mws.make_vector(S->x0, elem[0]);
+                const Vector v0 = mws.make_vector(S->v, elem[0]);
+                const Scalar m = mws.make_scalar(S->mass, elem[0]);
+                const Vector g = mws.make_vector(S->gravity);
+                const Scalar dt = mws.make_scalar(S->dt);
+                const Vector xhat = x0 + dt * v0 + dt * dt * g;
+                const Vector dev = x1 - xhat;
+                return 0.5 * m * dev.squared_norm() / dt.powN(2);
+            });
+        }
+        else if (p->gravity[0] != 0.0 || p->gravity[1] != 0.0 || p->gravity[2] != 0.0) {
+            S->G->add_potential("gravity", S->all_vertices, 1, [S](MappedWorkspace<double>& mws, Element& elem) {
+                const Vector x1 = mws.make_vector(S->x, elem[0]);
+                const Scalar m = mws.make_scalar(S->mass, elem[0]);
+                const Vector g = mws.make_vector(S->gravity);
+                return -1.0 * m * g.dot(x1);
+            });
+        }
+        if (!S->bc_target.empty()) {
+            // penalty Dirichlet condition (rubber_extrusion.cpp:95-107)
+            S->G->add_potential("boundary_conditions", S->bc, [S](MappedWorkspace<double>& mws, Element& conn) {
+                const Vector x = mws.make_vector(S->x, conn["vertex_idx"]);
+                const Vector target = mws.make_vector(S->bc_target, conn["constraint_idx"]);
+                const Scalar k = mws.make_scalar(S->k);
+                return 0.5 * k * (x - target).squared_norm();
+            });
+        }
+        S->G->add_dof(S->x);
+        S->ctx = Context::create();
+        S->newton = NewtonsMethod::create(S->G, S->ctx);
+        apply_newton_params(S->newton->settings, *np);
+    });
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+void symx_scene_destroy(symx_scene s)
+{
+    if (!s) return;
+    s->newton.reset();
+    s->G.reset();
+    s->ctx.reset();
+    delete s;
+}
+
+int symx_scene_n_vertices(symx_scene s, int32_t* n)
+{
+    *n = (int32_t)s->x.size();
+    return 0;
+}
+int symx_scene_n_elements(symx_scene s, int32_t* n)
+{
+    *n = (int32_t)(s->tets.size() / 4);
+    return 0;
+}
+int symx_scene_connectivity(symx_scene s, int32_t* conn)
+{
+    std::memcpy(conn, s->tets.data(), s->tets.size() * sizeof(int32_t));
+    return 0;
+}
+
+static std::vector<std::array<double, 3>>* vec3_array(symx_scene s, const std::string& name)
+{
+    if (name == "x") return &s->x;
+    if (name == "x0") return &s->x0;
+    if (name == "v") return &s->v;
+    if (name == "X") return &s->X;
+    return nullptr;
+}
+
+int symx_scene_get_array(symx_scene s, const char* name, double* out)
+{
+    if (std::string(name) == "mass") {
+        std::memcpy(out, s->mass.data(), s->mass.size() * sizeof(double));
+        return 0;
+    }
+    auto* a = vec3_array(s, name);
+    if (!a) {
+        g_err = std::string("unknown array ") + name;
+        return 1;
+    }
+    std::memcpy(out, a->data(), a->size() * 3 * sizeof(double));
+    return 0;
+}
+int symx_scene_set_array(symx_scene s, const char* name, const double* in)
+{
+    auto* a = vec3_array(s, name);
+    if (!a || std::string(name) == "X") {
+        g_err = std::string("array cannot be set: ") + name;
+        return 1;
+    }
+    std::memcpy(a->data(), in, a->size() * 3 * sizeof(double));
+    return 0;
+}
+int symx_scene_set_log_mu(symx_scene s, const double* log_mu)
+{
+    std::memcpy(s->log_mu.data(), log_mu, s->log_mu.size() * sizeof(double));
+    return 0;
+}
+int symx_scene_set_newton_params(symx_scene s, const symx_newton_params* np)
+{
+    s->np = *np;
+    apply_newton_params(s->newton->settings, *np);
+    return 0;
+}
+
+int symx_scene_solve(symx_scene s, symx_solve_stats* st)
+{
+    return guard([&]() {
+        s->ctx->logger->clear();
+        const auto t0 = std::chrono::steady_clock::now();
+        const SolverReturn r = s->newton->solve();
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (st) {
+            const auto& ss = s->newton->get_last_solve_stats();
+            const auto& L = *s->ctx->logger;
+            st->result = (int32_t)r;
+            st->newton_iterations = ss.newton_iterations;
+            st->cg_iterations = ss.cg_iterations;
+            st->ls_bt_iterations = ss.ls_bt_iterations;
+            st->n_refinements = ss.n_refinements;
+            st->final_energy = ss.final_energy;
+            st->final_residual = ss.final_residual;
+            st->first_residual = ss.residuals.empty() ? 0.0 : ss.residuals.front();
+            st->seconds = secs;
+            st->seconds_eval = L.get_timer_total("evaluate_P__dP_du__local_d2P_du2");
+            st->seconds_project = L.get_timer_total("project_to_PD");
+            st->seconds_assembly = L.get_timer_total("assembly");
+            st->seconds_linear = L.get_timer_total("linear_system_solve");
+            st->seconds_energy = L.get_timer_total("evaluate_P");
+        }
+    });
+}
+
+int symx_scene_step(symx_scene s, symx_solve_stats* st)
+{
+    int rc = symx_scene_solve(s, st);
+    if (rc) return rc;
+    if (s->dt > 0.0) {
+        for (size_t i = 0; i < s->x.size(); i++) {
+            for (int d = 0; d < 3; d++) {
+                s->v[i][d] = (s->x[i][d] - s->x0[i][d]) / s->dt;
+                s->x0[i][d] = s->x[i][d];
+            }
+        }
+    }
+    return 0;
+}
+
+symx_hip_problem symx_scene_problem(symx_scene s) { return s->newton->device().problem(); }
+
+int symx_scene_source(symx_scene s, int32_t i, char* buf, int64_t cap, int64_t* len)
+{
+    const auto& inf = s->newton->device().info();
+    if (i < 0 || i >= (int32_t)inf.size()) return 1;
+    const std::string& src = inf[i].source;
+    if (len) *len = (int64_t)src.size();
+    if (buf && cap > 0) {
+        const int64_t n = std::min<int64_t>(cap - 1, (int64_t)src.size());
+        std::memcpy(buf, src.data(), (size_t)n);
+        buf[n] = '\0';
+    }
+    return 0;
+}
+int symx_scene_op_counts(symx_scene s, int32_t i, int64_t* ops3)
+{
+    const auto& inf = s->newton->device().info();
+    if (i < 0 || i >= (int32_t)inf.size()) return 1;
+    ops3[0] = inf[i].ops_P;
+    ops3[1] = inf[i].ops_PG;
+    ops3[2] = inf[i].ops_PGH;
+    return 0;
+}
+double symx_scene_compile_seconds(symx_scene s) { return s->newton->device().compile_seconds; }
+
+// ---- host-only helpers ------------------------------------------------------------
+int symx_host_tet4_element(int32_t material, const double* x12, const double* X12, double a, double b, double* E, double* g12, double* H144)
+{
+    return guard([&]() {
+        std::vector<std::array<double, 3>> x(4), X(4);
+        for (int i = 0; i < 4; i++)
+            for (int d = 0; d < 3; d++) {
+                x[i][d] = x12[3 * i + d];
+                X[i][d] = X12[3 * i + d];
+            }
+        std::vector<int32_t> conn = { 0, 1, 2, 3 };
+        double av = a, bv = b;
+        auto [mws, elem] = MappedWorkspace<double>::create(conn, 4);
+        std::vector<Vector> xs = mws->make_vectors(x, elem);
+        std::vector<Vector> Xs = mws->make_vectors(X, elem);
+        const Scalar sa = mws->make_scalar(av);
+        const Scalar sb = mws->make_scalar(bv);
+        const Scalar P = tet4_elastic(*mws, xs, Xs, sa, sb, material);
+        std::vector<Scalar> dofs = collect_scalars(xs);
+        DiffCache dc;
+        const Vector g = gradient(P, dofs, dc);
+        const Matrix H = gradient(g, dofs, true, dc);
+        Graph* gr = mws->workspace().graph();
+        // bind symbol values: maps in creation order, then the single quadrature point
+        for (const DataMap& m : mws->get_maps()) {
+            const double* d = m.data();
+            const int32_t idx = m.connectivity_index >= 0 ? conn[m.connectivity_index] : 0;
+            for (int k = 0; k < m.stride; k++) gr->set_symbol_value(m.first_symbol_idx + k, d[(size_t)idx * m.stride + k]);
+        }
+        const SummationMap& sm = mws->get_summation();
+        for (int k = 0; k < sm.stride; k++) gr->set_symbol_value(sm.first_symbol_idx + k, sm.data[k]);
+        std::vector<int32_t> ids = { P.id };
+        for (const Scalar& s : g.values()) ids.push_back(s.id);
+        for (const Scalar& s : H.values()) ids.push_back(s.id);
+        const std::vector<double> v = gr->eval_many(ids);
+        *E = v[0];
+        for (int i = 0; i < 12; i++) g12[i] = v[1 + i];
+        for (int i = 0; i < 144; i++) H144[i] = v[13 + i];
+    });
+}
+
+int symx_host_point_triangle(const double* in, double* out)
+{
+    return guard([&]() {
+        Workspace ws;
+        Vector a = ws.make_vector(3), b = ws.make_vector(3), c = ws.make_vector(3), p = ws.make_vector(3);
+        Scalar sign = ws.make_scalar();
+        const Vector ap = p - a;
+        const Vector ca = a - c;
+        const Vector cb = b - c;
+        const Vector n = ca.cross3(cb).normalized();
+        const Scalar C = sign * ap.dot(n);
+        std::vector<Scalar> vars = { a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], p[0], p[1], p[2] };
+        const Vector g = gradient(C, vars);
+        const Matrix H = hessian(C, vars);
+        for (int i = 0; i < 12; i++) vars[i].set_value(in[i]);
+        sign.set_value(in[12]);
+        std::vector<int32_t> ids = { C.id };
+        for (const Scalar& s : g.values()) ids.push_back(s.id);
+        for (const Scalar& s : H.values()) ids.push_back(s.id);
+        const std::vector<double> v = ws.graph()->eval_many(ids);
+        for (int i = 0; i < 157; i++) out[i] = v[i];
+    });
+}
+
+int symx_host_jit_tet4(const char* cache_dir, int32_t material, int64_t* code_size, int64_t* n_ops_pgh)
+{
+    return guard([&]() {
+        std::vector<std::array<double, 3>> x(4, { 0.0, 0.0, 0.0 }), X(4, { 0.0, 0.0, 0.0 });
+        std::vector<int32_t> conn = { 0, 1, 2, 3 };
+        double av = 1.0, bv = 1.0;
+        auto [mws, elem] = MappedWorkspace<double>::create(conn, 4);
+        std::vector<Vector> xs = mws->make_vectors(x, elem);
+        std::vector<Vector> Xs = mws->make_vectors(X, elem);
+        const Scalar P = tet4_elastic(*mws, xs, Xs, mws->make_scalar(av), mws->make_scalar(bv), material);
+        std::vector<Scalar> dofs = collect_scalars(xs);
+        DiffCache dc;
+        const Vector g = gradient(P, dofs, dc);
+        const Matrix H = gradient(g, dofs, true, dc);
+        std::vector<KernelSpec> specs = {
+            KernelSpec{ "tet4_P", KernelKind::P, { P }, 12 },
+            KernelSpec{ "tet4_PG", KernelKind::PG, collect_scalars({ { P }, g.values() }), 12 },
+            KernelSpec{ "tet4_PGH", KernelKind::PGH, collect_scalars({ { P }, g.values(), H.values() }), 12 },
+        };
+        const std::string src = emit_hip_module("tet4", *mws->workspace().graph(), mws->get_maps(), mws->get_summation(), specs,
+                                                symx_hip_eval_prelude());
+        char key[80];
+        int wc = 0;
+        size_t cs = 0;
+        if (symx_hip_jit_compile(src.c_str(), "gfx950", cache_dir, key, &wc, &cs) != SYMX_HIP_OK)
+            throw std::runtime_error(symx_hip_last_error(nullptr));
+        *code_size = (int64_t)cs;
+        *n_ops_pgh = count_operations(*mws->workspace().graph(), specs[2].outputs);
+    });
+}
+
+} // extern "C"
