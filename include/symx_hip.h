@@ -210,6 +210,16 @@ int symx_hip_problem_scale(symx_hip_problem prob, int32_t dst, int32_t a, double
 int symx_hip_batch_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const double* host_in, int64_t n,
                         int32_t n_in, double* host_out, int32_t n_out);
 
+/* Kernel timing with HIP events on the context's stream (the stream every
+ * kernel of this library runs on): average duration in ms of `reps` launches
+ * of one phase. phase: 0 element kernel E+grad+Hessian of potential `pot`,
+ * 1 PD projection of `pot`'s element Hessians (in place, on a scratch copy),
+ * 2 block-matrix assembly (arg = precision), 3 one PCG iteration (spmv + update,
+ * arg = precision), 4 energy-only element kernel of `pot`. Needs a prior
+ * EVAL_PGH (phases 1-3) and assemble (phase 3). No counterpart in SymX: its
+ * Logger::ScopedTimer (Logger.h:17-29) times phases on the host. */
+int symx_hip_problem_profile(symx_hip_problem prob, int32_t phase, int32_t pot, int32_t arg, int32_t reps, double* avg_ms);
+
 /* ---- standalone kernels on caller-provided device memory (tests) ---------- */
 /* In-place PD projection of n_mats dense row-major n x n FP64 matrices
  * (project_to_PD.cpp:13-82). changed (nullable, device) gets 1 per rebuilt matrix. */

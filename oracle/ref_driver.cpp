@@ -1,0 +1,202 @@
+// Driver of the REAL reference (SymX built from /root/reference by oracle/Makefile
+// into oracle/_ref/). TEST INFRASTRUCTURE ONLY: it produces reference outputs for
+// parity checks and the CPU baseline; nothing in symx_amd/ links or runs it.
+//
+//   ref_driver element <lambda> <mu> <x0..x11> <X0..X11>
+//       Prints E, gradient[12], Hessian[144] of the Neo-Hookean Tet4 element built
+//       like examples/NeoHookean.cpp:12-117 (Compiled<double>).
+//   ref_driver solve <in.bin> <out.bin>
+//       Builds the Tet4 block problem (Neo-Hookean via fem_integrator, backward
+//       Euler inertia, penalty boundary) with GlobalPotential and solves one step
+//       with NewtonsMethod (examples/dynamic_elasticity_with_contact.cpp:85-125,
+//       examples/rubber_extrusion.cpp:68-107). Input/output layouts are in
+//       tests/ref_io.py.
+#include <symx>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <vector>
+
+using namespace symx;
+
+static Scalar nh_density(const Matrix& F, const Scalar& l, const Scalar& m)
+{
+    Matrix C = F.transpose() * F;
+    Scalar Ic = C.trace();
+    Scalar logdetF = log(F.det());
+    return 0.5 * m * (Ic - 3) - m * logdetF + 0.5 * l * logdetF.powN(2);
+}
+
+static int run_element(int argc, char** argv)
+{
+    if (argc < 2 + 2 + 24) return 2;
+    const double lambda = std::atof(argv[2]), mu = std::atof(argv[3]);
+    std::vector<double> xin(12), Xin(12);
+    for (int i = 0; i < 12; i++) xin[i] = std::atof(argv[4 + i]);
+    for (int i = 0; i < 12; i++) Xin[i] = std::atof(argv[16 + i]);
+    auto jac_tet4 = [](Workspace& ws, const std::vector<Vector>& xh) {
+        Matrix J = ws.get_identity_matrix(3);
+        J.set_col(0, xh[1] - xh[0]);
+        J.set_col(1, xh[2] - xh[0]);
+        J.set_col(2, xh[3] - xh[0]);
+        return J;
+    };
+    Workspace ws;
+    std::vector<Vector> x = ws.make_vectors(3, 4);
+    std::vector<Vector> X = ws.make_vectors(3, 4);
+    Scalar l = ws.make_scalar();
+    Scalar m = ws.make_scalar();
+    Matrix Jx = jac_tet4(ws, x);
+    Matrix JX = jac_tet4(ws, X);
+    Matrix F = Jx * JX.inv();
+    Scalar E = JX.det() / 6.0 * nh_density(F, l, m);
+    DiffCache dc;
+    std::vector<Scalar> dofs = collect_scalars(x);
+    Vector g = gradient(E, dofs, dc);
+    Matrix h = gradient(g, dofs, true, dc);
+    std::vector<Scalar> outs = collect_scalars({ { E }, g.values(), h.values() });
+    Compiled<double> c(outs, "ref_nh_element", symx::get_codegen_dir());
+    for (int v = 0; v < 4; v++) {
+        for (int d = 0; d < 3; d++) {
+            c.set(x[v][d], xin[3 * v + d]);
+            c.set(X[v][d], Xin[3 * v + d]);
+        }
+    }
+    c.set(l, lambda);
+    c.set(m, mu);
+    View<double> r = c.run();
+    for (int i = 0; i < 157; i++) std::printf("%.17g\n", r[i]);
+    return 0;
+}
+
+template <typename T>
+static void rd(std::ifstream& f, T* p, size_t n) { f.read(reinterpret_cast<char*>(p), sizeof(T) * n); }
+template <typename T>
+static void wr(std::ofstream& f, const T* p, size_t n) { f.write(reinterpret_cast<const char*>(p), sizeof(T) * n); }
+
+static int run_solve(int argc, char** argv)
+{
+    if (argc < 4) return 2;
+    std::ifstream f(argv[2], std::ios::binary);
+    int32_t hdr[8];
+    rd(f, hdr, 8);
+    const int nv = hdr[0], ne = hdr[1], nbc = hdr[2], has_inertia = hdr[3], threads = hdr[4], projection_mode = hdr[5], mirror = hdr[6],
+              max_iter = hdr[7];
+    double prm[12];
+    rd(f, prm, 12);
+    const double lambda = prm[0], mu = prm[1], dt = prm[2], k = prm[3];
+    const Eigen::Vector3d gravity(prm[4], prm[5], prm[6]);
+    struct Data
+    {
+        std::vector<Eigen::Vector3d> x, X, x0, v;
+        std::vector<double> mass;
+        std::vector<int32_t> tets, all;
+        LabelledConnectivity<2> bc{ { "constraint_idx", "vertex_idx" } };
+        std::vector<Eigen::Vector3d> bc_target;
+        double lambda, mu, dt, k;
+        Eigen::Vector3d gravity;
+    } d;
+    d.lambda = lambda;
+    d.mu = mu;
+    d.dt = dt;
+    d.k = k;
+    d.gravity = gravity;
+    d.tets.resize((size_t)ne * 4);
+    rd(f, d.tets.data(), d.tets.size());
+    auto rdv = [&](std::vector<Eigen::Vector3d>& a, int n) {
+        std::vector<double> t((size_t)n * 3);
+        rd(f, t.data(), t.size());
+        a.resize(n);
+        for (int i = 0; i < n; i++) a[i] = Eigen::Vector3d(t[3 * i], t[3 * i + 1], t[3 * i + 2]);
+    };
+    rdv(d.X, nv);
+    rdv(d.x, nv);
+    rdv(d.x0, nv);
+    rdv(d.v, nv);
+    d.mass.resize(nv);
+    rd(f, d.mass.data(), nv);
+    std::vector<int32_t> bcv(nbc);
+    rd(f, bcv.data(), nbc);
+    rdv(d.bc_target, nbc);
+    for (int i = 0; i < nbc; i++) d.bc.numbered_push_back({ bcv[i] });
+    d.all.resize(nv);
+    for (int i = 0; i < nv; i++) d.all[i] = i;
+
+    spGlobalPotential G = GlobalPotential::create();
+    G->add_potential("neohookean_tet4", d.tets, 4, [&](MappedWorkspace<double>& mws, Element& elem) {
+        std::vector<Vector> x = mws.make_vectors(d.x, elem);
+        std::vector<Vector> X = mws.make_vectors(d.X, elem);
+        Scalar l = mws.make_scalar(d.lambda);
+        Scalar m = mws.make_scalar(d.mu);
+        return fem_integrator(mws, FEM_Element::Tet4, [&](Scalar& w, Vector xi) {
+            Matrix Jx = fem_jacobian(FEM_Element::Tet4, x, xi);
+            Matrix JX = fem_jacobian(FEM_Element::Tet4, X, xi);
+            Matrix F = Jx * JX.inv();
+            return nh_density(F, l, m) * JX.det() * w;
+        });
+    });
+    if (has_inertia) {
+        G->add_potential("be_inertia", d.all, 1, [&](MappedWorkspace<double>& mws, Element& elem) {
+            Vector x1 = mws.make_vector(d.x, elem[0]);
+            Vector x0 = mws.make_vector(d.x0, elem[0]);
+            Vector v0 = mws.make_vector(d.v, elem[0]);
+            Scalar m = mws.make_scalar(d.mass, elem[0]);
+            Vector g = mws.make_vector(d.gravity);
+            Scalar dts = mws.make_scalar(d.dt);
+            Vector xhat = x0 + dts * v0 + dts * dts * g;
+            Vector dev = x1 - xhat;
+            return 0.5 * m * dev.squared_norm() / dts.powN(2);
+        });
+    }
+    if (nbc > 0) {
+        G->add_potential("boundary_conditions", d.bc, [&](MappedWorkspace<double>& mws, Element& conn) {
+            Vector x = mws.make_vector(d.x, conn["vertex_idx"]);
+            Vector target = mws.make_vector(d.bc_target, conn["constraint_idx"]);
+            Scalar ks = mws.make_scalar(d.k);
+            return 0.5 * ks * (x - target).squared_norm();
+        });
+    }
+    G->add_dof(d.x);
+    spContext context = Context::create();
+    context->compilation_directory = symx::get_codegen_dir();
+    context->n_threads = threads;
+    NewtonsMethod newton(G, context);
+    newton.settings.projection_mode = projection_mode == 0 ? ProjectionToPD::Newton : ProjectionToPD::ProjectedNewton;
+    newton.settings.project_to_pd_use_mirroring = mirror != 0;
+    newton.settings.max_iterations = max_iter;
+    newton.settings.residual_tolerance_abs = prm[7];
+    newton.settings.residual_tolerance_rel = prm[8];
+    newton.settings.cg_rel_tolerance = prm[9];
+    newton.settings.cg_abs_tolerance = prm[10];
+    newton.settings.projection_eps = prm[11];
+    const auto t0 = std::chrono::steady_clock::now();
+    SolverReturn ret = newton.solve();
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const auto& st = newton.get_last_solve_stats();
+    // The final energy is evaluated by the caller (a second compiled global in the same
+    // process would re-dlopen the same shared-object names).
+    const double E = std::numeric_limits<double>::quiet_NaN();
+    std::ofstream o(argv[3], std::ios::binary);
+    int32_t ohdr[4] = { (int32_t)ret, st.newton_iterations, st.cg_iterations, st.ls_bt_iterations };
+    wr(o, ohdr, 4);
+    double od[2] = { E, secs };
+    wr(o, od, 2);
+    std::vector<double> xo((size_t)nv * 3);
+    for (int i = 0; i < nv; i++)
+        for (int c = 0; c < 3; c++) xo[3 * (size_t)i + c] = d.x[i][c];
+    wr(o, xo.data(), xo.size());
+    return 0;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) return 2;
+    const std::string mode = argv[1];
+    if (mode == "element") return run_element(argc, argv);
+    if (mode == "solve") return run_solve(argc, argv);
+    return 2;
+}

@@ -1,0 +1,310 @@
+"""ctypes bindings of the two C ABIs of this package.
+
+* ``include/symx_hip.h``    - device runtime (JIT, element evaluation, projection,
+  assembly, PCG, Newton vector ops), library ``lib/libsymx_hip.so``.
+* ``include/symx_scenes.h`` - example problems built on the C++ front end
+  (GlobalPotential / NewtonsMethod), library ``lib/libsymx_amd.so``.
+
+Loading fails loudly when the libraries are missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, Structure, c_char_p, c_double, c_int, c_int32, c_int64, c_size_t, c_uint8, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+
+FP64 = 0
+MIXED = 1
+EVAL_P, EVAL_PG, EVAL_PGH = 0, 1, 2
+VEC_DOFS, VEC_GRAD, VEC_DU, VEC_DOFS_LS, VEC_RHS = 0, 1, 2, 3, 4
+
+SOLVER_RETURN = [
+    "Successful", "Running", "InvalidInitialState", "TooManyIterations", "TooManyArmijoIterations",
+    "LinearSystemSolveFailure", "TooManyInvalidIntermediateIterations", "StepDoesNotDescend", "InvalidConvergedState",
+]
+
+
+class PcgInfo(Structure):
+    _fields_ = [("n_iterations", c_int32), ("converged", c_int32), ("found_indefiniteness", c_int32),
+                ("n_refinements", c_int32), ("error", c_double)]
+
+
+class Tet4BlockParams(Structure):
+    _fields_ = [("cells", c_int32 * 3), ("size", c_double * 3), ("youngs_modulus", c_double), ("poissons_ratio", c_double),
+                ("density", c_double), ("gravity", c_double * 3), ("dt", c_double), ("noise", c_double), ("seed", c_uint64),
+                ("bc_stiffness", c_double), ("fixed_axis", c_int32), ("material", c_int32), ("log_mu_field", c_int32)]
+
+
+class NewtonParams(Structure):
+    _fields_ = [("precision", c_int32), ("projection_mode", c_int32), ("use_mirroring", c_int32), ("max_iterations", c_int32),
+                ("residual_tolerance_abs", c_double), ("residual_tolerance_rel", c_double), ("cg_rel_tolerance", c_double),
+                ("cg_abs_tolerance", c_double), ("cg_max_iterations", c_int32), ("projection_eps", c_double),
+                ("step_tolerance", c_double)]
+
+
+class SolveStats(Structure):
+    _fields_ = [("result", c_int32), ("newton_iterations", c_int32), ("cg_iterations", c_int32), ("ls_bt_iterations", c_int32),
+                ("n_refinements", c_int32), ("final_energy", c_double), ("final_residual", c_double), ("first_residual", c_double),
+                ("seconds", c_double), ("seconds_eval", c_double), ("seconds_project", c_double), ("seconds_assembly", c_double),
+                ("seconds_linear", c_double), ("seconds_energy", c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["result_name"] = SOLVER_RETURN[self.result] if 0 <= self.result < len(SOLVER_RETURN) else str(self.result)
+        return d
+
+
+_hip = None
+_amd = None
+_DP = POINTER(c_double)
+
+
+def _load(name):
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"symx_amd: {path} is missing; build it with __graft_entry__.build() (no CPU fallback exists)")
+    return ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+def hip():
+    """The device runtime library (include/symx_hip.h)."""
+    global _hip
+    if _hip is None:
+        lib = _load("libsymx_hip.so")
+        lib.symx_hip_last_error.restype = c_char_p
+        lib.symx_hip_eval_prelude.restype = c_char_p
+        lib.symx_hip_ctx_destroy.restype = None
+        lib.symx_hip_problem_destroy.restype = None
+        lib.symx_hip_module_unload.restype = None
+        _hip = lib
+    return _hip
+
+
+def amd():
+    """The C++ front end + scenes library (include/symx_scenes.h)."""
+    global _amd
+    if _amd is None:
+        hip()
+        lib = _load("libsymx_amd.so")
+        lib.symx_scene_last_error.restype = c_char_p
+        lib.symx_scene_problem.restype = c_void_p
+        lib.symx_scene_compile_seconds.restype = c_double
+        lib.symx_scene_destroy.restype = None
+        lib.symx_newton_params_default.restype = None
+        lib.symx_tet4_block_params_default.restype = None
+        _amd = lib
+    return _amd
+
+
+def device_count() -> int:
+    n = c_int(0)
+    hip().symx_hip_device_count(ctypes.byref(n))
+    return n.value
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(c_void_p)
+
+
+def check_hip(rc, ctx=None):
+    if rc != 0:
+        raise RuntimeError(f"symx_hip error {rc}: {hip().symx_hip_last_error(ctx).decode()}")
+
+
+def check_scene(rc):
+    if rc != 0:
+        raise RuntimeError(f"symx scene error: {amd().symx_scene_last_error().decode()}")
+
+
+def default_newton_params(**kw) -> NewtonParams:
+    p = NewtonParams()
+    amd().symx_newton_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def default_block_params(**kw) -> Tet4BlockParams:
+    p = Tet4BlockParams()
+    amd().symx_tet4_block_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if k in ("cells", "size", "gravity"):
+            getattr(p, k)[:] = v
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Context:
+    """A HIP context of the device runtime."""
+
+    def __init__(self, device: int = 0):
+        self.ptr = c_void_p()
+        check_hip(hip().symx_hip_ctx_create(c_int(device), ctypes.byref(self.ptr)))
+
+    def close(self):
+        if self.ptr:
+            hip().symx_hip_ctx_destroy(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+class Tet4Block:
+    """Tet4 elastic block scene (include/symx_scenes.h) driven through the C++ front end."""
+
+    def __init__(self, params: Tet4BlockParams | None = None, newton: NewtonParams | None = None):
+        self.params = params or default_block_params()
+        self.newton = newton or default_newton_params()
+        self.ptr = c_void_p()
+        check_scene(amd().symx_scene_tet4_block_create(ctypes.byref(self.params), ctypes.byref(self.newton), ctypes.byref(self.ptr)))
+        n = c_int32()
+        amd().symx_scene_n_vertices(self.ptr, ctypes.byref(n))
+        self.n_vertices = n.value
+        amd().symx_scene_n_elements(self.ptr, ctypes.byref(n))
+        self.n_elements = n.value
+
+    def close(self):
+        if self.ptr:
+            amd().symx_scene_destroy(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def problem(self):
+        return c_void_p(amd().symx_scene_problem(self.ptr))
+
+    def connectivity(self) -> np.ndarray:
+        c = np.zeros((self.n_elements, 4), np.int32)
+        amd().symx_scene_connectivity(self.ptr, _ptr(c))
+        return c
+
+    def get(self, name: str) -> np.ndarray:
+        shape = (self.n_vertices,) if name == "mass" else (self.n_vertices, 3)
+        a = np.zeros(shape)
+        check_scene(amd().symx_scene_get_array(self.ptr, name.encode(), _ptr(a)))
+        return a
+
+    def set(self, name: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check_scene(amd().symx_scene_set_array(self.ptr, name.encode(), _ptr(a)))
+
+    def set_log_mu(self, log_mu: np.ndarray):
+        a = np.ascontiguousarray(log_mu, dtype=np.float64)
+        check_scene(amd().symx_scene_set_log_mu(self.ptr, _ptr(a)))
+
+    def set_newton(self, newton: NewtonParams):
+        self.newton = newton
+        check_scene(amd().symx_scene_set_newton_params(self.ptr, ctypes.byref(newton)))
+
+    def solve(self) -> dict:
+        st = SolveStats()
+        check_scene(amd().symx_scene_solve(self.ptr, ctypes.byref(st)))
+        return st.as_dict()
+
+    def step(self) -> dict:
+        st = SolveStats()
+        check_scene(amd().symx_scene_step(self.ptr, ctypes.byref(st)))
+        return st.as_dict()
+
+    def source(self, i: int = 0) -> str:
+        n = c_int64()
+        amd().symx_scene_source(self.ptr, c_int32(i), None, c_int64(0), ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        amd().symx_scene_source(self.ptr, c_int32(i), buf, c_int64(n.value + 1), ctypes.byref(n))
+        return buf.value.decode()
+
+    def op_counts(self, i: int = 0):
+        a = (c_int64 * 3)()
+        amd().symx_scene_op_counts(self.ptr, c_int32(i), a)
+        return list(a)
+
+    # ---- direct access to the device problem (include/symx_hip.h) -------------------
+    def _p(self):
+        return self.problem
+
+    def eval(self, mode: int = EVAL_PGH) -> float:
+        E = c_double()
+        check_hip(hip().symx_hip_problem_eval(self._p(), c_int32(mode), ctypes.byref(E)))
+        return E.value
+
+    def set_dofs(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        check_hip(hip().symx_hip_problem_set_dofs(self._p(), _ptr(x)))
+
+    def download(self, vec: int) -> np.ndarray:
+        a = np.zeros(3 * self.n_vertices)
+        check_hip(hip().symx_hip_problem_download(self._p(), c_int32(vec), _ptr(a)))
+        return a
+
+    def upload(self, vec: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        check_hip(hip().symx_hip_problem_upload(self._p(), c_int32(vec), _ptr(a)))
+
+    def element_outputs(self, pot: int, n_elem: int, nb: int):
+        G = np.zeros((n_elem, 3 * nb))
+        H = np.zeros((n_elem, nb * (nb + 1) // 2 * 9))
+        check_hip(hip().symx_hip_problem_element_outputs(self._p(), c_int32(pot), _ptr(G), _ptr(H)))
+        return G, H
+
+    def project(self, eps: float = 1e-10, mirror: bool = False) -> int:
+        n = c_int64()
+        check_hip(hip().symx_hip_problem_project(self._p(), c_double(eps), c_int32(int(mirror)), ctypes.byref(n)))
+        return n.value
+
+    def assemble(self, precision: int = FP64):
+        check_hip(hip().symx_hip_problem_assemble(self._p(), c_int32(precision)))
+
+    def export_bsr(self):
+        nnzb = c_int64()
+        check_hip(hip().symx_hip_problem_export_bsr(self._p(), ctypes.byref(nnzb), None, None, None))
+        rp = np.zeros(self.n_vertices + 1, np.int64)
+        cols = np.zeros(nnzb.value, np.int32)
+        vals = np.zeros((nnzb.value, 3, 3))
+        check_hip(hip().symx_hip_problem_export_bsr(self._p(), ctypes.byref(nnzb), _ptr(rp), _ptr(cols), _ptr(vals)))
+        return rp, cols, vals
+
+    def linear_solve(self, precision: int, abs_tol: float, rel_tol: float, max_iter: int = 10000, stop_on_indef: bool = True) -> dict:
+        info = PcgInfo()
+        check_hip(hip().symx_hip_problem_solve(self._p(), c_int32(precision), c_double(abs_tol), c_double(rel_tol),
+                                               c_int32(max_iter), c_int32(int(stop_on_indef)), ctypes.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+
+def host_tet4_element(x12, X12, a, b, material=0):
+    """E, gradient, Hessian of the Tet4 elastic potential from the host expression evaluator (no GPU)."""
+    x12 = np.ascontiguousarray(x12, np.float64).reshape(12)
+    X12 = np.ascontiguousarray(X12, np.float64).reshape(12)
+    E = c_double()
+    g = np.zeros(12)
+    H = np.zeros(144)
+    check_scene(amd().symx_host_tet4_element(c_int32(material), _ptr(x12), _ptr(X12), c_double(a), c_double(b), ctypes.byref(E),
+                                             _ptr(g), _ptr(H)))
+    return E.value, g, H.reshape(12, 12)
+
+
+def host_point_triangle(in13):
+    a = np.ascontiguousarray(in13, np.float64).reshape(13)
+    out = np.zeros(157)
+    check_scene(amd().symx_host_point_triangle(_ptr(a), _ptr(out)))
+    return out
+
+
+def host_jit_tet4(cache_dir: str, material: int = 0):
+    cs = c_int64()
+    ops = c_int64()
+    check_scene(amd().symx_host_jit_tet4(cache_dir.encode(), c_int32(material), ctypes.byref(cs), ctypes.byref(ops)))
+    return cs.value, ops.value
+
+
+def project_dense(ctx: Context, mats_dev_ptr: int, n_mats: int, n: int, eps: float, mirror: bool, changed_dev_ptr: int | None = None):
+    check_hip(hip().symx_hip_project_dense(ctx.ptr, c_void_p(mats_dev_ptr), c_int64(n_mats), c_int32(n), c_double(eps),
+                                           c_int32(int(mirror)), c_void_p(changed_dev_ptr) if changed_dev_ptr else None), ctx.ptr)

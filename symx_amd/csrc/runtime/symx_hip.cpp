@@ -1022,6 +1022,108 @@ int symx_hip_problem_scale(symx_hip_problem p, int32_t dst, int32_t a, double s)
     return SYMX_HIP_OK;
 }
 
+int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, int32_t arg, int32_t reps, double* avg_ms)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->finalized || reps <= 0) return fail(ctx, SYMX_HIP_ERR_ARG, "problem not finalized or reps <= 0");
+    if (pot_id < 0 || pot_id >= (int32_t)p->pots.size()) return fail(ctx, SYMX_HIP_ERR_ARG, "bad potential id");
+    DevPotential& pot = p->pots[pot_id];
+    const SellPattern& S = p->sell;
+    double* scratch = nullptr;
+    if (phase == 1) {
+        if (!p->h_valid) return fail(ctx, SYMX_HIP_ERR_ARG, "no element Hessians");
+        int rc = dalloc(ctx, &scratch, (size_t)std::max<int64_t>((int64_t)pot.n_elem * pot.n_ub * 9, 1));
+        if (rc) return rc;
+    }
+    if ((phase == 2 || phase == 3) && p->assembled_precision != arg) {
+        int rc = symx_hip_problem_assemble(p, arg);
+        if (rc) return rc;
+    }
+    auto launch_once = [&](int i) -> int {
+        symx_hip_eval_params prm;
+        fill_params(p, pot, prm);
+        switch (phase) {
+        case 0: return launch_eval(ctx, pot.module, pot.kPGH, prm);
+        case 4: return launch_eval(ctx, pot.module, pot.kP, prm);
+        case 1: {
+            // project a fresh copy of the element Hessians each repetition
+            hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice, ctx->stream);
+            return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, 0.0, nullptr) ? SYMX_HIP_ERR_ARG : 0;
+        }
+        case 2:
+            launch_bsr_assemble(ctx->stream, S, p->d_coff, p->d_clist, p->d_H, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
+                                arg == SYMX_HIP_FP64 ? nullptr : p->d_val32);
+            return 0;
+        case 3:
+            if (arg == SYMX_HIP_FP64)
+                launch_pcg_iter64(ctx->stream, S, p->d_val64, p->d_dinv64, p->d_x, p->d_r, p->d_z, (i & 1) ? p->d_p1 : p->d_p0,
+                                  (i & 1) ? p->d_p0 : p->d_p1, p->d_q, ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
+            else
+                launch_pcg_iter32(ctx->stream, S, p->d_val32, p->d_dinv32, p->f_x, p->f_r, p->f_z, (i & 1) ? p->f_p1 : p->f_p0,
+                                  (i & 1) ? p->f_p0 : p->f_p1, p->f_q, ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
+            return 0;
+        default: return fail(ctx, SYMX_HIP_ERR_ARG, "bad profile phase");
+        }
+    };
+    if (phase == 3) {
+        // a PCG run that never converges (zero tolerances) so every launch does work
+        PcgState st{};
+        st.abs_tol = 0.0;
+        st.rel_tol = 0.0;
+        st.max_iter = 1 << 30;
+        st.stop_on_indef = 0;
+        *ctx->h_pcg = st;
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pcg, ctx->h_pcg, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+        if (arg == SYMX_HIP_FP64)
+            launch_pcg_init64(ctx->stream, S, p->d_dinv64, p->d_vec[SYMX_HIP_VEC_GRAD], p->d_x, p->d_r, p->d_z, p->d_p0, ctx->d_pcg,
+                              ctx->d_partials, ctx->d_ticket);
+        else {
+            if (!p->f_b) return fail(ctx, SYMX_HIP_ERR_ARG, "mixed buffers missing");
+            launch_residual_mixed(ctx->stream, S, p->d_val32, p->d_vec[SYMX_HIP_VEC_GRAD], p->d_vec[SYMX_HIP_VEC_DU], p->d_r, p->f_b,
+                                  ctx->d_scalars + 16, ctx->d_partials, ctx->d_ticket);
+            launch_pcg_init32(ctx->stream, S, p->d_dinv32, p->f_b, p->f_x, p->f_r, p->f_z, p->f_p0, ctx->d_pcg, ctx->d_partials,
+                              ctx->d_ticket);
+        }
+    }
+    int rc = launch_once(0);  // warm-up
+    if (rc) {
+        dfree(scratch);
+        return rc;
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    HIP_TRY(ctx, hipEventCreate(&e1));
+    float total_ms = 0.0f;
+    if (phase == 1) {
+        // time projection launches only (copies excluded): one event pair per repetition
+        for (int i = 0; i < reps && !rc; i++) {
+            hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice, ctx->stream);
+            hipEventRecord(e0, ctx->stream);
+            launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, 0.0, nullptr);
+            hipEventRecord(e1, ctx->stream);
+            hipEventSynchronize(e1);
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, e0, e1);
+            total_ms += ms;
+        }
+    }
+    else {
+        hipEventRecord(e0, ctx->stream);
+        for (int i = 1; i <= reps && !rc; i++) rc = launch_once(i);
+        hipEventRecord(e1, ctx->stream);
+        hipEventSynchronize(e1);
+        hipEventElapsedTime(&total_ms, e0, e1);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    dfree(scratch);
+    HIP_TRY(ctx, hipGetLastError());
+    if (rc) return rc;
+    if (avg_ms) *avg_ms = (double)total_ms / reps;
+    return SYMX_HIP_OK;
+}
+
 int symx_hip_batch_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const double* host_in, int64_t n, int32_t n_in,
                         double* host_out, int32_t n_out)
 {

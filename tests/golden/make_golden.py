@@ -1,0 +1,84 @@
+"""Writes the golden fixtures of tests/golden/ by running the REAL reference built from
+its sources by oracle/Makefile (`make -C oracle ref`, needs /root/reference). Run in the
+build container:  python tests/golden/make_golden.py
+
+  nh_tet4_reference.json     E, gradient, Hessian of the Neo-Hookean Tet4 element from the
+                             reference's Compiled<double> (examples/NeoHookean.cpp:12-117)
+  point_triangle_sympy.json  the reference's sympy-generated point-triangle derivatives
+                             (tests/sympy_derivatives.cpp, used by tests/test_diff.cpp:74-137)
+  newton_ref_*.npz           final positions and iteration counts of the reference
+                             NewtonsMethod on Tet4 block problems (n_threads = 1)
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref")
+ENV = {k: v for k, v in os.environ.items() if k != "CXX"}
+
+
+def fmt(vals):
+    return ["%.17g" % v for v in vals]
+
+
+def nh_cases():
+    rng = np.random.default_rng(20261203)
+    cases = []
+    E_, nu = 1e6, 0.3
+    lam, mu = E_ * nu / ((1 + nu) * (1 - 2 * nu)), E_ / (2 * (1 + nu))
+    inputs = [(np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1], float), np.array([0, 0, 0, .9, 0, 0, 0, .8, 0, 0, 0, .7], float), lam, mu)]
+    for _ in range(7):
+        X = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1], float) * rng.uniform(0.05, 1.0) + rng.uniform(-0.02, 0.02, 12)
+        x = X + rng.uniform(-0.1, 0.1, 12) * np.abs(X).max()
+        inputs.append((x, X, 142857.14285714287, 35714.28571428572))
+    for x, X, l, m in inputs:
+        out = subprocess.run([os.path.join(REF, "ref_driver"), "element"] + fmt([l, m, *x, *X]), capture_output=True, text=True,
+                             check=True, env=ENV).stdout.split()
+        cases.append({"x": x.tolist(), "X": X.tolist(), "lambda": l, "mu": m, "out": [float(v) for v in out]})
+    json.dump({"source": "reference Compiled<double> via oracle/ref_driver element", "cases": cases},
+              open(os.path.join(HERE, "nh_tet4_reference.json"), "w"))
+
+
+def point_triangle_cases():
+    rng = np.random.default_rng(74)
+    cases = []
+    for _ in range(8):
+        inp = np.concatenate([rng.uniform(-1, 1, 12), [rng.choice([-1.0, 1.0])]])
+        out = subprocess.run([os.path.join(REF, "sympy_golden")] + fmt(inp), capture_output=True, text=True, check=True).stdout.split()
+        cases.append({"in": inp.tolist(), "out": [float(v) for v in out]})
+    json.dump({"source": "reference tests/sympy_derivatives.cpp", "cases": cases}, open(os.path.join(HERE, "point_triangle_sympy.json"), "w"))
+
+
+def newton_case(name, cells, noise, seed, rel_tol, cg_rel, threads=1):
+    from ref_io import read_solve_output, write_solve_input
+    from scene_np import tet4_block
+    from symx_amd import NewtonParams
+
+    S = tet4_block(cells=cells, noise=noise, seed=seed)
+    newton = NewtonParams(0, 1, 0, 100, 0.0, rel_tol, cg_rel, 1e-12, 10000, 1e-10, 0.0)
+    with tempfile.TemporaryDirectory() as d:
+        inp, outp = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        write_solve_input(inp, S, newton, threads)
+        subprocess.run([os.path.join(REF, "ref_driver"), "solve", inp, outp], check=True, env=ENV)
+        res = read_solve_output(outp, S.X.shape[0])
+    np.savez_compressed(os.path.join(HERE, f"newton_ref_{name}.npz"), x=res["x"], cells=np.array(cells), noise=noise, seed=seed,
+                        rel_tol=rel_tol, cg_rel=cg_rel, result=res["result"], newton_iterations=res["newton_iterations"],
+                        cg_iterations=res["cg_iterations"], ls_bt_iterations=res["ls_bt_iterations"], energy=res["energy"],
+                        seconds=res["seconds"], threads=threads)
+    print(name, {k: v for k, v in res.items() if k != "x"})
+
+
+if __name__ == "__main__":
+    nh_cases()
+    point_triangle_cases()
+    newton_case("block6", (6, 6, 6), 0.2 / 6, 11, 1e-6, 1e-3)
+    newton_case("block6_tight", (6, 6, 6), 0.2 / 6, 11, 1e-8, 1e-10)
+    newton_case("cfg1", (20, 20, 20), 0.01, 1, 1e-6, 1e-3)

@@ -1,0 +1,167 @@
+"""ctypes wrapper of the C oracle (oracle/liboracle.so) - test infrastructure only.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker
+or the timed CPU baseline, never by the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, Structure, c_double, c_int, c_int32, c_int64, c_void_p
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+
+class OracleScene(Structure):
+    _fields_ = [("nv", c_int32), ("ne", c_int32), ("nbc", c_int32), ("has_inertia", c_int32),
+                ("tets", c_void_p), ("X", c_void_p), ("x", c_void_p), ("x0", c_void_p), ("v0", c_void_p), ("mass", c_void_p),
+                ("bc_vertex", c_void_p), ("bc_target", c_void_p), ("mu_e", c_void_p),
+                ("lambda_over_mu", c_double), ("lam", c_double), ("mu", c_double), ("dt", c_double), ("k", c_double),
+                ("gravity", c_double * 3)]
+
+
+class OracleNewtonParams(Structure):
+    _fields_ = [("precision", c_int32), ("projection_mode", c_int32), ("use_mirroring", c_int32), ("max_iterations", c_int32),
+                ("residual_tolerance_abs", c_double), ("residual_tolerance_rel", c_double), ("cg_rel_tolerance", c_double),
+                ("cg_abs_tolerance", c_double), ("cg_max_iterations", c_int32), ("projection_eps", c_double),
+                ("step_tolerance", c_double)]
+
+
+class OracleStats(Structure):
+    _fields_ = [("result", c_int32), ("newton_iterations", c_int32), ("cg_iterations", c_int32), ("ls_bt_iterations", c_int32),
+                ("final_energy", c_double), ("final_residual", c_double), ("first_residual", c_double),
+                ("seconds_eval", c_double), ("seconds_linear", c_double)]
+
+
+class PcgInfo(Structure):
+    _fields_ = [("n_iterations", c_int32), ("converged", c_int32), ("found_indefiniteness", c_int32), ("pad", c_int32),
+                ("error", c_double)]
+
+
+def _p(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+class SceneData:
+    """Arrays of a Tet4 block problem, kept alive for the oracle."""
+
+    def __init__(self, tets, X, x, x0, v0, mass, bc_vertex, bc_target, lam, mu, dt, k, gravity, mu_e=None, lambda_over_mu=0.0):
+        self.tets = np.ascontiguousarray(tets, np.int32)
+        self.X = np.ascontiguousarray(X, np.float64)
+        self.x = np.ascontiguousarray(x, np.float64).copy()
+        self.x0 = np.ascontiguousarray(x0, np.float64)
+        self.v0 = np.ascontiguousarray(v0, np.float64)
+        self.mass = np.ascontiguousarray(mass, np.float64)
+        self.bc_vertex = np.ascontiguousarray(bc_vertex, np.int32)
+        self.bc_target = np.ascontiguousarray(bc_target, np.float64)
+        self.mu_e = None if mu_e is None else np.ascontiguousarray(mu_e, np.float64)
+        self.lam, self.mu, self.dt, self.k = lam, mu, dt, k
+        self.lambda_over_mu = lambda_over_mu
+        self.gravity = np.asarray(gravity, np.float64)
+
+    def struct(self) -> OracleScene:
+        s = OracleScene()
+        s.nv = self.X.shape[0]
+        s.ne = self.tets.shape[0]
+        s.nbc = self.bc_vertex.shape[0]
+        s.has_inertia = 1 if self.dt > 0 else 0
+        s.tets, s.X, s.x, s.x0, s.v0, s.mass = _p(self.tets), _p(self.X), _p(self.x), _p(self.x0), _p(self.v0), _p(self.mass)
+        s.bc_vertex, s.bc_target = _p(self.bc_vertex), _p(self.bc_target)
+        s.mu_e = _p(self.mu_e) if self.mu_e is not None else None
+        s.lambda_over_mu = self.lambda_over_mu
+        s.lam, s.mu, s.dt, s.k = self.lam, self.mu, self.dt, self.k
+        s.gravity[:] = self.gravity
+        return s
+
+
+def scene_arrays(scene) -> SceneData:
+    """Arrays of a symx_amd.Tet4Block scene (same inputs as the GPU path)."""
+    p = scene.params
+    X = scene.get("X")
+    fixed = np.zeros(X.shape[0], bool)
+    if p.fixed_axis >= 0:
+        fixed = np.abs(X[:, p.fixed_axis]) < 1e-12
+    bc_vertex = np.nonzero(fixed)[0]
+    E, nu = p.youngs_modulus, p.poissons_ratio
+    mu = E / (2 * (1 + nu))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    return SceneData(scene.connectivity(), X, scene.get("x"), scene.get("x0"), scene.get("v"), scene.get("mass"), bc_vertex,
+                     X[bc_vertex], lam, mu, p.dt, p.bc_stiffness, list(p.gravity))
+
+
+class Oracle:
+    def __init__(self):
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run __graft_entry__.build()")
+        self.lib = ctypes.CDLL(LIB)
+        self.lib.oracle_scene_energy.restype = c_double
+        self.lib.oracle_eval_project_assemble.restype = c_double
+
+    def nh_tet4(self, x12, X12, lam, mu):
+        x12 = np.ascontiguousarray(x12, np.float64).reshape(12)
+        X12 = np.ascontiguousarray(X12, np.float64).reshape(12)
+        E = c_double()
+        g = np.zeros(12)
+        H = np.zeros(144)
+        self.lib.oracle_nh_tet4(_p(x12), _p(X12), c_double(lam), c_double(mu), ctypes.byref(E), _p(g), _p(H))
+        return E.value, g, H.reshape(12, 12)
+
+    def project(self, A, eps=1e-10, mirror=False):
+        A = np.array(A, np.float64, copy=True)
+        n = A.shape[-1]
+        flat = np.ascontiguousarray(A.reshape(-1, n, n))
+        changed = np.zeros(flat.shape[0], np.uint8)
+        self.lib.oracle_project_many(_p(flat), c_int64(flat.shape[0]), c_int(n), c_double(eps), c_int(int(mirror)), _p(changed))
+        return flat.reshape(A.shape), changed.astype(bool)
+
+    def pcg(self, rp, cols, vals, b, abs_tol, rel_tol, max_iter=10000, stop_on_indef=True):
+        rp = np.ascontiguousarray(rp, np.int64)
+        cols = np.ascontiguousarray(cols, np.int32)
+        vals = np.ascontiguousarray(vals, np.float64).reshape(-1, 9)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros_like(b)
+        info = PcgInfo()
+        self.lib.oracle_pcg_bsr(c_int(rp.shape[0] - 1), _p(rp), _p(cols), _p(vals), _p(b), _p(x), c_double(abs_tol), c_double(rel_tol),
+                                c_int(max_iter), c_int(int(stop_on_indef)), ctypes.byref(info))
+        return x, {k: getattr(info, k) for k, _ in info._fields_}
+
+    def energy(self, S: SceneData, x=None, with_grad=True):
+        st = S.struct()
+        xx = np.ascontiguousarray(S.x if x is None else x, np.float64)
+        g = np.zeros(xx.size) if with_grad else None
+        E = self.lib.oracle_scene_energy(ctypes.byref(st), _p(xx), _p(g) if with_grad else None)
+        return (E, g) if with_grad else E
+
+    def element_hessians(self, S: SceneData, x=None):
+        st = S.struct()
+        xx = np.ascontiguousarray(S.x if x is None else x, np.float64)
+        He = np.zeros((S.tets.shape[0], 12, 12))
+        self.lib.oracle_scene_element_hessians(ctypes.byref(st), _p(xx), _p(He))
+        return He
+
+    def eval_project_assemble(self, S: SceneData, project=True, eps=1e-10):
+        st = S.struct()
+        return self.lib.oracle_eval_project_assemble(ctypes.byref(st), c_int(int(project)), c_double(eps), None)
+
+    def newton(self, S: SceneData, newton_params, max_record=256):
+        st = S.struct()
+        onp = OracleNewtonParams()
+        for k, _ in onp._fields_:
+            setattr(onp, k, getattr(newton_params, k))
+        stats = OracleStats()
+        Es = np.zeros(max_record)
+        Rs = np.zeros(max_record)
+        self.lib.oracle_newton_solve(ctypes.byref(st), ctypes.byref(onp), ctypes.byref(stats), _p(Es), _p(Rs), c_int(max_record))
+        d = {k: getattr(stats, k) for k, _ in stats._fields_}
+        n = min(d["newton_iterations"] + 1, max_record)
+        d["energies"] = Es[:n].copy()
+        d["residuals"] = Rs[:n].copy()
+        d["x"] = S.x.reshape(-1, 3).copy()
+        return d
+
+    def num_threads(self) -> int:
+        return self.lib.oracle_num_threads()

@@ -1,0 +1,56 @@
+"""numpy restatement of the Tet4 block inputs (mesh, noise, masses, boundary) used by
+the scenes of include/symx_scenes.h, so that CPU tests can build the same problem
+without a GPU. Mesh split: mesh_generation.cpp:64-84; masses:
+dynamic_elasticity_with_contact.cpp:67-79."""
+import numpy as np
+
+from oracle_py import SceneData
+
+
+def splitmix64_uniform(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n)
+    s = seed & 0xFFFFFFFFFFFFFFFF
+    M = 0xFFFFFFFFFFFFFFFF
+    for i in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z = z ^ (z >> 31)
+        out[i] = (z >> 11) * (1.0 / 9007199254740992.0)
+    return out
+
+
+def tet4_mesh(cells, size=(1.0, 1.0, 1.0)):
+    nx, ny, nz = cells
+    d = [size[0] / nx, size[1] / ny, size[2] / nz]
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    X = np.stack([I.ravel() * d[0], J.ravel() * d[1], K.ravel() * d[2]], 1)
+
+    def idx(i, j, k):
+        return i * (ny + 1) * (nz + 1) + j * (nz + 1) + k
+
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    v = [idx(i, j, k), idx(i + 1, j, k), idx(i + 1, j + 1, k), idx(i, j + 1, k),
+         idx(i, j, k + 1), idx(i + 1, j, k + 1), idx(i + 1, j + 1, k + 1), idx(i, j + 1, k + 1)]
+    pat = [(0, 1, 6, 5), (0, 1, 2, 6), (0, 2, 3, 6), (0, 5, 6, 4), (0, 6, 3, 7), (0, 6, 7, 4)]
+    tets = np.stack([np.stack([v[a] for a in t], 1) for t in pat], 1).reshape(-1, 4)
+    return X, tets.astype(np.int32)
+
+
+def tet4_block(cells=(4, 4, 4), size=(1.0, 1.0, 1.0), E=1e5, nu=0.4, density=1000.0, gravity=(0, 0, -9.81), dt=0.01,
+               noise=0.01, seed=1, k=1e8, fixed_axis=2) -> SceneData:
+    X, tets = tet4_mesh(cells, size)
+    nv = X.shape[0]
+    fixed = np.abs(X[:, fixed_axis]) < 1e-12 if fixed_axis >= 0 else np.zeros(nv, bool)
+    u = splitmix64_uniform(seed, 3 * nv).reshape(nv, 3)
+    x = X + np.where(fixed[:, None], 0.0, noise * (2.0 * u - 1.0))
+    A = X[tets[:, 0]]
+    det = np.linalg.det(np.stack([X[tets[:, 1]] - A, X[tets[:, 2]] - A, X[tets[:, 3]] - A], 2))
+    mass = np.zeros(nv)
+    np.add.at(mass, tets.ravel(), np.repeat(np.abs(det) / 6.0 * density / 4.0, 4))
+    mu = E / (2 * (1 + nu))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    bc = np.nonzero(fixed)[0]
+    return SceneData(tets, X, x, x.copy(), np.zeros_like(x), mass, bc, X[bc], lam, mu, dt, k, gravity)
