@@ -244,7 +244,7 @@ namespace symx
             const int n_ub = nb * (nb + 1) / 2;
 
             os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.name << "(const symx_hip_eval_params p)\n{\n";
-            os << "    const int e = blockIdx.x * 256 + threadIdx.x;\n";
+            os << "    const int e = symx_block() * 256 + threadIdx.x;\n";
             if (ks.kind != KernelKind::C) os << "    double e_out = 0.0;\n";
             os << "    if (e < p.n_elem) {\n";
             if (ks.kind != KernelKind::C) os << "        const bool act = (p.active == nullptr) || (p.active[e] != 0);\n";

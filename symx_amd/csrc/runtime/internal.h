@@ -118,6 +118,7 @@ struct symx_hip_problem_s
     double* d_G = nullptr;
     double* d_H = nullptr;
     double* d_Epart = nullptr;
+    uint8_t* d_flags = nullptr;   // per-element scratch of the projection
     bool h_valid = false;
 
     // gradient scatter (CSR by DoF block of gradient-block ids)
@@ -145,31 +146,31 @@ namespace symx_hip
     // Fixed-order sum of n doubles into *d_out (single workgroup).
     void launch_sum(hipStream_t s, const double* d_in, int64_t n, double* d_out);
     void launch_grad_assemble(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* G, double* grad, int32_t nb);
-    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, double skip_tol,
+    // flags: n_mats bytes of scratch (4-block elements that are not translation invariant)
+    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
                           unsigned long long* d_count);
     int launch_project_dense(hipStream_t s, double* mats, int64_t n_mats, int32_t n, double eps, int32_t mirror, uint8_t* changed);
     void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* coff, const uint32_t* clist, const double* H,
                              double* v64, float* v32);
     void launch_dinv(hipStream_t s, const SellPattern& P, const double* v64, const float* v32, double* dinv64, float* dinv32);
-    // PCG on the sell matrix. T = double: FP64; T = float: FP32 (FP32 matrix).
+    // PCG on the sell matrix (iteration k is 1-based). partials: >= 5 * reduction_groups(n_rows) doubles.
     void launch_pcg_init64(hipStream_t s, const SellPattern& P, const double* dinv, const double* b, double* x, double* r, double* z,
-                           double* p0, symx_hip::PcgState* st, double* partials, unsigned int* ticket);
-    void launch_pcg_iter64(hipStream_t s, const SellPattern& P, const double* val, const double* dinv, double* x, double* r, double* z,
-                           const double* p_old, double* p_new, double* q, symx_hip::PcgState* st, double* partials, unsigned int* ticket);
+                           double* p0, double* partials);
+    void launch_pcg_iter64(hipStream_t s, const SellPattern& P, int32_t k, const double* val, const double* dinv, double* x, double* r,
+                           double* z, const double* p_old, double* p_new, double* q, symx_hip::PcgState* st, double* partials);
     void launch_pcg_init32(hipStream_t s, const SellPattern& P, const float* dinv, const float* b, float* x, float* r, float* z,
-                           float* p0, symx_hip::PcgState* st, double* partials, unsigned int* ticket);
-    void launch_pcg_iter32(hipStream_t s, const SellPattern& P, const float* val, const float* dinv, float* x, float* r, float* z,
-                           const float* p_old, float* p_new, float* q, symx_hip::PcgState* st, double* partials, unsigned int* ticket);
-    // Mixed-precision refinement helpers.
-    // r = b - A x (FP64 vectors, FP32 matrix, FP64 arithmetic); partial ||r||^2 -> out (device scalar, fixed order).
+                           float* p0, double* partials);
+    void launch_pcg_iter32(hipStream_t s, const SellPattern& P, int32_t k, const float* val, const float* dinv, float* x, float* r,
+                           float* z, const float* p_old, float* p_new, float* q, symx_hip::PcgState* st, double* partials);
+    // Mixed-precision refinement: r = b - A x (FP64 vectors, FP32 matrix, FP64 arithmetic), r32 = (float) r, *rr_out = ||r||^2.
     void launch_residual_mixed(hipStream_t s, const SellPattern& P, const float* val, const double* b, const double* x, double* r,
-                               float* r32, double* rr_out, double* partials, unsigned int* ticket);
+                               float* r32, double* rr_out, double* partials);
     void launch_x_update_mixed(hipStream_t s, int64_t n, double* x, const float* d);
-    // Vector ops (FP64).
+    // Vector ops (FP64); reductions write a device scalar.
     void launch_axpy(hipStream_t s, int64_t n, double* dst, const double* a, double sc, const double* b);
     void launch_scale(hipStream_t s, int64_t n, double* dst, const double* a, double sc);
-    void launch_dot(hipStream_t s, int64_t n, const double* a, const double* b, double* out, double* partials, unsigned int* ticket);
-    void launch_max_abs(hipStream_t s, int64_t n, const double* a, double* out, double* partials, unsigned int* ticket);
+    void launch_dot(hipStream_t s, int64_t n, const double* a, const double* b, double* out, double* partials);
+    void launch_max_abs(hipStream_t s, int64_t n, const double* a, double* out, double* partials);
     void launch_fill_u8(hipStream_t s, int64_t n, uint8_t* p, uint8_t v);
     int reduction_groups(int64_t n);
 }

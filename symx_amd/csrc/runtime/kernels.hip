@@ -2,31 +2,58 @@
 // deterministic assembly into the sliced block matrix, block-Jacobi PCG and the
 // Newton vector operations. Generated element kernels live in hiprtc modules.
 //
-// Every reduction is two-level with a fixed order: each workgroup writes one
-// partial, and the last workgroup to finish (found with an integer ticket)
-// sums the partials in index order. No floating-point atomics are used, so
-// results are bitwise repeatable for a given problem size.
+// Reductions never use atomics or device-wide fences. A reduction kernel runs a
+// fixed grid (at most MAX_GROUPS workgroups, each owning a contiguous chunk of
+// rows) and writes one partial per workgroup; the scalar is formed by the next
+// kernel on the stream, which sums the partials in index order (every workgroup
+// of a PCG kernel does this redundantly, so all of them see the same alpha/beta
+// without synchronising). Results are therefore bitwise repeatable for a given
+// problem size.
+//
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// xcd_remap() gives every XCD a contiguous range of logical workgroups, so rows
+// that share neighbours (gathers through the mesh connectivity) hit the same L2.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdint>
 
 #include "internal.h"
+#include "projection.h"
 
 namespace symx_hip
 {
-    constexpr int NT = 256;           // threads per workgroup
-    constexpr int MAX_RED_GROUPS = 1024;
+    constexpr int NT = 256;  // threads per workgroup
+    constexpr int XCDS = 8;
+    constexpr int MAX_GROUPS = 1024;
 
     int reduction_groups(int64_t n)
     {
         int64_t g = (n + NT - 1) / NT;
         if (g < 1) g = 1;
-        if (g > MAX_RED_GROUPS) g = MAX_RED_GROUPS;
+        if (g > MAX_GROUPS) g = MAX_GROUPS;
         return (int)g;
     }
 
     static inline unsigned grid_for(int64_t n) { return (unsigned)((n + NT - 1) / NT > 0 ? (n + NT - 1) / NT : 1); }
+
+    // Logical workgroup index: XCD x (hardware workgroups b with b % 8 == x) gets the
+    // contiguous logical range [x*per + min(x, extra), ...).
+    __device__ __forceinline__ int64_t xcd_remap()
+    {
+        const int64_t b = blockIdx.x, nb = gridDim.x;
+        const int64_t x = b % XCDS, i = b / XCDS;
+        const int64_t per = nb / XCDS, extra = nb % XCDS;
+        return x * per + (x < extra ? x : extra) + i;
+    }
+
+    // Contiguous row chunk [r0, r1) of logical group w out of G (chunks are multiples of 64 rows).
+    __device__ __forceinline__ void row_chunk(int64_t n, int64_t w, int64_t G, int64_t& r0, int64_t& r1)
+    {
+        const int64_t per = (((n + G - 1) / G) + 63) / 64 * 64;
+        r0 = w * per < n ? w * per : n;
+        r1 = r0 + per < n ? r0 + per : n;
+    }
 
     __device__ inline double wave_sum(double v)
     {
@@ -43,89 +70,55 @@ namespace symx_hip
         return v;
     }
 
-    // Sum over the workgroup; the result is valid in thread 0.
+    // Workgroup sum in a fixed order, returned to every thread.
     __device__ inline double block_sum(double v, double* lds)
     {
         v = wave_sum(v);
         const int w = threadIdx.x >> 6;
+        __syncthreads();
         if ((threadIdx.x & 63) == 0) lds[w] = v;
         __syncthreads();
         double r = 0.0;
-        if (threadIdx.x == 0) {
-            for (int i = 0; i < NT / 64; i++) r += lds[i];
-        }
-        __syncthreads();
+        for (int i = 0; i < NT / 64; i++) r += lds[i];
         return r;
     }
     __device__ inline double block_max(double v, double* lds)
     {
         v = wave_max(v);
         const int w = threadIdx.x >> 6;
+        __syncthreads();
         if ((threadIdx.x & 63) == 0) lds[w] = v;
         __syncthreads();
-        double r = 0.0;
-        if (threadIdx.x == 0) {
-            r = lds[0];
-            for (int i = 1; i < NT / 64; i++) r = nanmax(r, lds[i]);
-        }
-        __syncthreads();
+        double r = lds[0];
+        for (int i = 1; i < NT / 64; i++) r = nanmax(r, lds[i]);
         return r;
     }
 
-    __device__ inline void store_coherent(double* p, double v)
-    {
-        __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __device__ inline double load_coherent(const double* p)
-    {
-        return __longlong_as_double((long long)__hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED,
-                                                                 __HIP_MEMORY_SCOPE_AGENT));
-    }
-
-    // Publish this workgroup's partials (already stored by thread 0) and find out
-    // whether it is the last workgroup of the grid. Resets the ticket when it is.
-    __device__ inline bool last_group(unsigned int* ticket)
-    {
-        __shared__ int is_last;
-        __threadfence();
-        if (threadIdx.x == 0) {
-            const unsigned t = atomicAdd(ticket, 1u);
-            is_last = (t == gridDim.x - 1) ? 1 : 0;
-            if (is_last) *ticket = 0u;
-        }
-        __syncthreads();
-        if (is_last) __threadfence();
-        return is_last != 0;
-    }
-
-    // Fixed-order sum of `n` partials by one workgroup (thread 0 gets the result).
-    __device__ inline double sum_partials(const double* partials, int n, int stride, int offset, double* lds)
+    // Fixed-order sum of G partials p[i*stride + off], returned to every thread.
+    __device__ inline double sum_partials(const double* __restrict__ p, int64_t G, int stride, int off, double* lds)
     {
         double acc = 0.0;
-        for (int i = threadIdx.x; i < n; i += NT) acc += load_coherent(partials + (int64_t)i * stride + offset);
+        for (int64_t i = threadIdx.x; i < G; i += NT) acc += p[i * stride + off];
         return block_sum(acc, lds);
     }
 
-    // ---- plain sum ---------------------------------------------------------------
-    __global__ __launch_bounds__(NT) void k_sum(const double* __restrict__ in, int64_t n, double* out)
+    // ---- plain sums ----------------------------------------------------------------
+    __global__ __launch_bounds__(NT) void k_sum(const double* __restrict__ in, int64_t n, int stride, int off, double* out)
     {
         __shared__ double lds[NT / 64];
-        double acc = 0.0;
-        for (int64_t i = threadIdx.x; i < n; i += NT) acc += in[i];
-        const double r = block_sum(acc, lds);
+        const double r = sum_partials(in, n, stride, off, lds);
         if (threadIdx.x == 0) *out = r;
     }
     void launch_sum(hipStream_t s, const double* d_in, int64_t n, double* d_out)
     {
-        k_sum<<<1, NT, 0, s>>>(d_in, n, d_out);
+        k_sum<<<1, NT, 0, s>>>(d_in, n, 1, 0, d_out);
     }
 
     // ---- gradient assembly: grad[3b+i] = sum of element gradient blocks -----------
     __global__ __launch_bounds__(NT) void k_grad_assemble(const uint32_t* __restrict__ off, const uint32_t* __restrict__ list,
                                                           const double* __restrict__ G, double* __restrict__ grad, int32_t nb)
     {
-        const int b = blockIdx.x * NT + threadIdx.x;
+        const int64_t b = xcd_remap() * NT + threadIdx.x;
         if (b >= nb) return;
         double g0 = 0.0, g1 = 0.0, g2 = 0.0;
         const uint32_t k1 = off[b + 1];
@@ -135,9 +128,9 @@ namespace symx_hip
             g1 += src[1];
             g2 += src[2];
         }
-        grad[3 * (int64_t)b + 0] = g0;
-        grad[3 * (int64_t)b + 1] = g1;
-        grad[3 * (int64_t)b + 2] = g2;
+        grad[3 * b + 0] = g0;
+        grad[3 * b + 1] = g1;
+        grad[3 * b + 2] = g2;
     }
     void launch_grad_assemble(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* G, double* grad, int32_t nb)
     {
@@ -145,214 +138,172 @@ namespace symx_hip
         k_grad_assemble<<<grid_for(nb), NT, 0, s>>>(off, list, G, grad, nb);
     }
 
-    // ---- PD projection -------------------------------------------------------------
-    // Cyclic Jacobi eigen-decomposition of a symmetric N x N matrix held in registers
-    // (upper triangle only), followed by A <- V f(Lambda) V^T with f(l) = eps (clamp)
-    // or -l (mirror) for l < eps, like project_to_PD.cpp:13-32. Returns true if the
-    // matrix was rebuilt.
-    template <int N>
-    struct Sym
+    // ---- PD projection (device code in projection.h) ---------------------------------
+    // Full-size projection of an element Hessian stored as upper 3x3 blocks.
+    template <int NB>
+    __device__ __forceinline__ bool project_element_full(double* __restrict__ h, double eps, bool mirror)
     {
-        static constexpr int size = N * (N + 1) / 2;
-        __device__ static constexpr int idx(int i, int j)
-        {
-            return (i <= j) ? (i * N - (i * (i - 1)) / 2 + (j - i)) : (j * N - (j * (j - 1)) / 2 + (i - j));
-        }
-    };
-
-    template <int N>
-    __device__ inline bool jacobi_project(double (&a)[Sym<N>::size], double eps, bool mirror)
-    {
-        double v[N][N];
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-#pragma unroll
-            for (int j = 0; j < N; j++) v[i][j] = (i == j) ? 1.0 : 0.0;
-        }
-        double fro = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-#pragma unroll
-            for (int j = i; j < N; j++) fro += (i == j ? 1.0 : 2.0) * a[Sym<N>::idx(i, j)] * a[Sym<N>::idx(i, j)];
-        }
-        const double tol = 1e-30 * fro;
-        for (int sweep = 0; sweep < 24; sweep++) {
-            double off = 0.0;
-#pragma unroll
-            for (int p = 0; p < N; p++) {
-#pragma unroll
-                for (int q = p + 1; q < N; q++) off += a[Sym<N>::idx(p, q)] * a[Sym<N>::idx(p, q)];
-            }
-            if (off <= tol) break;
-#pragma unroll
-            for (int p = 0; p < N; p++) {
-#pragma unroll
-                for (int q = p + 1; q < N; q++) {
-                    const double apq = a[Sym<N>::idx(p, q)];
-                    if (fabs(apq) > 1e-300) {
-                        const double app = a[Sym<N>::idx(p, p)];
-                        const double aqq = a[Sym<N>::idx(q, q)];
-                        const double theta = (aqq - app) / (2.0 * apq);
-                        double t;
-                        if (fabs(theta) > 1e150) {
-                            t = 0.5 / theta;
-                        }
-                        else {
-                            t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
-                            if (theta < 0.0) t = -t;
-                        }
-                        const double c = 1.0 / sqrt(t * t + 1.0);
-                        const double s = t * c;
-                        const double tau = s / (1.0 + c);
-                        a[Sym<N>::idx(p, p)] = app - t * apq;
-                        a[Sym<N>::idx(q, q)] = aqq + t * apq;
-                        a[Sym<N>::idx(p, q)] = 0.0;
-#pragma unroll
-                        for (int r = 0; r < N; r++) {
-                            if (r != p && r != q) {
-                                const double g = a[Sym<N>::idx(r, p)];
-                                const double h = a[Sym<N>::idx(r, q)];
-                                a[Sym<N>::idx(r, p)] = g - s * (h + g * tau);
-                                a[Sym<N>::idx(r, q)] = h + s * (g - h * tau);
-                            }
-                        }
-#pragma unroll
-                        for (int r = 0; r < N; r++) {
-                            const double g = v[r][p];
-                            const double h = v[r][q];
-                            v[r][p] = g - s * (h + g * tau);
-                            v[r][q] = h + s * (g - h * tau);
-                        }
-                    }
-                }
-            }
-        }
-        double lam[N];
-        bool changed = false;
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            const double l = a[Sym<N>::idx(i, i)];
-            if (l < eps) {
-                changed = true;
-                lam[i] = mirror ? -l : eps;
-            }
-            else {
-                lam[i] = l;
-            }
-        }
+        constexpr int N = 3 * NB;
+        double a[Sym<N>::size];
+        static_for<0, NB>([&](auto Bi) {
+            static_for<decltype(Bi)::value, NB>([&](auto Bj) {
+                constexpr int bi = decltype(Bi)::value, bj = decltype(Bj)::value;
+                constexpr int ub = bi * NB - (bi * (bi - 1)) / 2 + (bj - bi);
+                static_for<0, 3>([&](auto Ic) {
+                    static_for<0, 3>([&](auto Jc) {
+                        constexpr int i = decltype(Ic)::value, j = decltype(Jc)::value;
+                        if constexpr (bi != bj || i <= j) a[Sym<N>::idx(3 * bi + i, 3 * bj + j)] = h[ub * 9 + 3 * i + j];
+                    });
+                });
+            });
+        });
+        const bool changed = project_sym<N>(a, eps, mirror);
         if (changed) {
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-#pragma unroll
-                for (int j = i; j < N; j++) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int k = 0; k < N; k++) acc += v[i][k] * lam[k] * v[j][k];
-                    a[Sym<N>::idx(i, j)] = acc;
-                }
-            }
+            static_for<0, NB>([&](auto Bi) {
+                static_for<decltype(Bi)::value, NB>([&](auto Bj) {
+                    constexpr int bi = decltype(Bi)::value, bj = decltype(Bj)::value;
+                    constexpr int ub = bi * NB - (bi * (bi - 1)) / 2 + (bj - bi);
+                    static_for<0, 3>([&](auto Ic) {
+                        static_for<0, 3>([&](auto Jc) {
+                            constexpr int i = decltype(Ic)::value, j = decltype(Jc)::value;
+                            h[ub * 9 + 3 * i + j] = a[Sym<N>::idx(3 * bi + i, 3 * bj + j)];
+                        });
+                    });
+                });
+            });
         }
         return changed;
     }
 
-    // Cholesky of A + delta I (upper triangle, copy). True if every pivot is positive,
-    // i.e. the smallest eigenvalue of A is above about -delta.
-    template <int N>
-    __device__ inline bool shifted_cholesky_ok(const double (&a)[Sym<N>::size], double delta)
+    // 4-node element Hessian with the rigid translations as null vectors: project the
+    // 9x9 restriction M = U^T H U (U = W (x) I3) and map the null space with f(0).
+    // Returns false (and leaves h untouched) if the block rows do not sum to zero.
+    __device__ __forceinline__ bool project_element_reduced4(double* __restrict__ h, double eps, bool mirror, bool& changed)
     {
-        double l[Sym<N>::size];
-#pragma unroll
-        for (int i = 0; i < Sym<N>::size; i++) l[i] = a[i];
-#pragma unroll
-        for (int i = 0; i < N; i++) l[Sym<N>::idx(i, i)] += delta;
-#pragma unroll
-        for (int j = 0; j < N; j++) {
-            double d = l[Sym<N>::idx(j, j)];
-#pragma unroll
-            for (int k = 0; k < j; k++) d -= l[Sym<N>::idx(k, j)] * l[Sym<N>::idx(k, j)];
-            if (!(d > 0.0)) return false;
-            const double rd = 1.0 / sqrt(d);
-            l[Sym<N>::idx(j, j)] = d * rd;
-#pragma unroll
-            for (int i = j + 1; i < N; i++) {
-                double x = l[Sym<N>::idx(j, i)];
-#pragma unroll
-                for (int k = 0; k < j; k++) x -= l[Sym<N>::idx(k, j)] * l[Sym<N>::idx(k, i)];
-                l[Sym<N>::idx(j, i)] = x * rd;
-            }
-        }
+        double scale = 0.0, rowsum = 0.0;
+        static_for<0, 90>([&](auto Kc) { scale = fmax(scale, fabs(h[decltype(Kc)::value])); });
+        static_for<0, 4>([&](auto Ac) {
+            static_for<0, 3>([&](auto Ic) {
+                static_for<0, 3>([&](auto Jc) {
+                    constexpr int A = decltype(Ac)::value, I = decltype(Ic)::value, J = decltype(Jc)::value;
+                    const double s = h_entry<A, 0, I, J>(h) + h_entry<A, 1, I, J>(h) + h_entry<A, 2, I, J>(h) + h_entry<A, 3, I, J>(h);
+                    rowsum = fmax(rowsum, fabs(s));
+                });
+            });
+        });
+        if (!(rowsum <= 1e-10 * scale)) return false;
+        double m[Sym<9>::size];
+        static_for<0, 3>([&](auto Al) {
+            static_for<decltype(Al)::value, 3>([&](auto Be) {
+                static_for<0, 3>([&](auto Ic) {
+                    static_for<0, 3>([&](auto Jc) {
+                        constexpr int al = decltype(Al)::value, be = decltype(Be)::value;
+                        constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
+                        if constexpr (al != be || I <= J) {
+                            double acc = 0.0;
+                            static_for<0, 4>([&](auto Ac) {
+                                static_for<0, 4>([&](auto Bc) {
+                                    constexpr int A = decltype(Ac)::value, B = decltype(Bc)::value;
+                                    constexpr double w = helmert(A, al) * helmert(B, be);
+                                    if constexpr (w != 0.0) acc += w * h_entry<A, B, I, J>(h);
+                                });
+                            });
+                            m[Sym<9>::idx(3 * al + I, 3 * be + J)] = acc;
+                        }
+                    });
+                });
+            });
+        });
+        double v[9][9];
+        jacobi_eig<9>(m, v);
+        double lam[9];
+        bool any = false;
+        static_for<0, 9>([&](auto Ic) {
+            constexpr int I = decltype(Ic)::value;
+            lam[I] = pd_map(m[Sym<9>::idx(I, I)], eps, mirror, any);
+        });
+        // Mp = V f(Lambda) V^T (always rebuilt: the null eigenvalues are below eps)
+        static_for<0, 9>([&](auto Ic) {
+            static_for<decltype(Ic)::value, 9>([&](auto Jc) {
+                constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
+                double acc = 0.0;
+                static_for<0, 9>([&](auto Kc) {
+                    constexpr int K = decltype(Kc)::value;
+                    acc += v[I][K] * lam[K] * v[J][K];
+                });
+                m[Sym<9>::idx(I, J)] = acc;
+            });
+        });
+        const double null_term = mirror ? 0.0 : 0.25 * eps;  // f(0) on span{translations}: eps T T^T, T T^T = (1/4) 1 1^T (x) I3
+        static_for<0, 4>([&](auto Ac) {
+            static_for<decltype(Ac)::value, 4>([&](auto Bc) {
+                static_for<0, 3>([&](auto Ic) {
+                    static_for<0, 3>([&](auto Jc) {
+                        constexpr int A = decltype(Ac)::value, B = decltype(Bc)::value;
+                        constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
+                        double acc = (I == J) ? null_term : 0.0;
+                        static_for<0, 3>([&](auto Al) {
+                            static_for<0, 3>([&](auto Be) {
+                                constexpr int al = decltype(Al)::value, be = decltype(Be)::value;
+                                constexpr double w = helmert(A, al) * helmert(B, be);
+                                if constexpr (w != 0.0) acc += w * m[Sym<9>::idx(3 * al + I, 3 * be + J)];
+                            });
+                        });
+                        h[ub4(A, B) * 9 + 3 * I + J] = acc;
+                    });
+                });
+            });
+        });
+        changed = true;
         return true;
     }
 
-    // Element Hessians stored as upper 3x3 blocks (a <= b), 9 row-major values each.
-    template <int NB>
-    __global__ __launch_bounds__(64) void k_project_ub(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
-                                                      double skip_tol, unsigned long long* count)
+    __device__ inline void count_changed(bool changed, unsigned long long* count)
     {
-        constexpr int N = 3 * NB;
-        constexpr int NUB = NB * (NB + 1) / 2;
-        const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
-        bool changed = false;
-        if (e < n_mats) {
-            double* h = H + e * (NUB * 9);
-            double a[Sym<N>::size];
-            int ub = 0;
-#pragma unroll
-            for (int bi = 0; bi < NB; bi++) {
-#pragma unroll
-                for (int bj = bi; bj < NB; bj++) {
-#pragma unroll
-                    for (int i = 0; i < 3; i++) {
-#pragma unroll
-                        for (int j = 0; j < 3; j++) {
-                            if (bi != bj || i <= j) a[Sym<N>::idx(3 * bi + i, 3 * bj + j)] = h[ub * 9 + 3 * i + j];
-                        }
-                    }
-                    ub++;
-                }
-            }
-            bool skip = false;
-            if (skip_tol > 0.0) {
-                double dmax = 0.0;
-#pragma unroll
-                for (int i = 0; i < N; i++) dmax = fmax(dmax, fabs(a[Sym<N>::idx(i, i)]));
-                skip = shifted_cholesky_ok<N>(a, skip_tol * dmax + eps);
-            }
-            if (!skip) {
-                changed = jacobi_project<N>(a, eps, mirror != 0);
-                if (changed) {
-                    ub = 0;
-#pragma unroll
-                    for (int bi = 0; bi < NB; bi++) {
-#pragma unroll
-                        for (int bj = bi; bj < NB; bj++) {
-#pragma unroll
-                            for (int i = 0; i < 3; i++) {
-#pragma unroll
-                                for (int j = 0; j < 3; j++) h[ub * 9 + 3 * i + j] = a[Sym<N>::idx(3 * bi + i, 3 * bj + j)];
-                            }
-                            ub++;
-                        }
-                    }
-                }
-            }
-        }
         if (count != nullptr) {
             const unsigned long long m = __ballot(changed);
             if ((threadIdx.x & 63) == 0 && m != 0ull) atomicAdd(count, (unsigned long long)__popcll(m));
         }
     }
 
-    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, double skip_tol,
+    // 4-block elements: reduced 9x9 path; elements that are not translation invariant
+    // are flagged for k_project_ub_full.
+    __global__ __launch_bounds__(64) void k_project_ub4_reduced(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
+                                                               unsigned long long* count, uint8_t* __restrict__ needs_full)
+    {
+        const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
+        bool changed = false;
+        if (e < n_mats) {
+            const bool ok = project_element_reduced4(H + e * 90, eps, mirror != 0, changed);
+            needs_full[e] = ok ? 0 : 1;
+        }
+        count_changed(changed, count);
+    }
+
+    template <int NB>
+    __global__ __launch_bounds__(64) void k_project_ub_full(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
+                                                           unsigned long long* count, const uint8_t* __restrict__ only)
+    {
+        constexpr int NUB = NB * (NB + 1) / 2;
+        const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
+        bool changed = false;
+        if (e < n_mats && (only == nullptr || only[e])) changed = project_element_full<NB>(H + e * (NUB * 9), eps, mirror != 0);
+        count_changed(changed, count);
+    }
+
+    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
                           unsigned long long* d_count)
     {
         if (n_mats <= 0) return 0;
         const unsigned g = (unsigned)((n_mats + 63) / 64);
         switch (nb) {
-        case 1: k_project_ub<1><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, skip_tol, d_count); return 0;
-        case 2: k_project_ub<2><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, skip_tol, d_count); return 0;
-        case 3: k_project_ub<3><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, skip_tol, d_count); return 0;
-        case 4: k_project_ub<4><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, skip_tol, d_count); return 0;
+        case 1: k_project_ub_full<1><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, nullptr); return 0;
+        case 2: k_project_ub_full<2><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, nullptr); return 0;
+        case 3: k_project_ub_full<3><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, nullptr); return 0;
+        case 4:
+            k_project_ub4_reduced<<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags);
+            k_project_ub_full<4><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags);
+            return 0;
         default: return -1;
         }
     }
@@ -365,18 +316,20 @@ namespace symx_hip
         if (e >= n_mats) return;
         double* m = M + e * (N * N);
         double a[Sym<N>::size];
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-#pragma unroll
-            for (int j = i; j < N; j++) a[Sym<N>::idx(i, j)] = m[i * N + j];
-        }
-        const bool changed = jacobi_project<N>(a, eps, mirror != 0);
+        static_for<0, N>([&](auto Ic) {
+            static_for<decltype(Ic)::value, N>([&](auto Jc) {
+                constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
+                a[Sym<N>::idx(I, J)] = m[I * N + J];
+            });
+        });
+        const bool changed = project_sym<N>(a, eps, mirror != 0);
         if (changed) {
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-#pragma unroll
-                for (int j = 0; j < N; j++) m[i * N + j] = a[Sym<N>::idx(i, j)];
-            }
+            static_for<0, N>([&](auto Ic) {
+                static_for<0, N>([&](auto Jc) {
+                    constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
+                    m[I * N + J] = a[Sym<N>::idx(I, J)];
+                });
+            });
         }
         if (changed_out) changed_out[e] = changed ? 1 : 0;
     }
@@ -400,7 +353,7 @@ namespace symx_hip
                                                          const uint32_t* __restrict__ clist, const double* __restrict__ H,
                                                          TS* __restrict__ val)
     {
-        const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+        const int64_t t = xcd_remap() * NT + threadIdx.x;
         if (t >= n_slots) return;
         double acc[9];
 #pragma unroll
@@ -440,7 +393,7 @@ namespace symx_hip
     __global__ __launch_bounds__(NT) void k_dinv(int32_t n_rows, const int64_t* __restrict__ slice_ptr, const TS* __restrict__ val,
                                                  TS* __restrict__ dinv)
     {
-        const int row = blockIdx.x * NT + threadIdx.x;
+        const int64_t row = xcd_remap() * NT + threadIdx.x;
         if (row >= n_rows) return;
         const int64_t base = slice_ptr[row >> 6];
         const TS* m = val + (base >> 6) * 9 * 64 + (row & 63);  // j = 0 is the diagonal block
@@ -477,219 +430,252 @@ namespace symx_hip
     }
 
     // ---- PCG -------------------------------------------------------------------------
-    // Iteration = two kernels:
-    //   k_pcg_spmv:   p_new = z + beta p_old (own row, and on the fly for neighbours),
-    //                 q = A p_new, partial p.q; last group: alpha = rz / pAp.
-    //   k_pcg_update: x += alpha p, r -= alpha q, z = D^-1 r, partial r.r and r.z;
-    //                 last group: convergence test of solve_pcg.h:200-210, beta.
-    // Both return at once when the solve is done, so the host can enqueue batches.
+    // Iteration k (1-based) = two kernels:
+    //   S_k  prologue: rr, rz of iteration k-1 from the partials of U_{k-1} (or init);
+    //                  convergence test of solve_pcg.h:200-210 for iteration k-1 (for
+    //                  k = 1 the early exits of :116-154); beta = rz / rz_prev.
+    //        body:     p_new = z + beta p_old (own row, and on the fly for neighbours),
+    //                  q = A p_new, partial p.q.
+    //   U_k  prologue: pAp from S_k's partials, alpha = rz / pAp, indefiniteness stop.
+    //        body:     x += alpha p, r -= alpha q, z = D^-1 r, partial r.r and r.z.
+    // Workgroup 0 publishes the solver state; a finished solve turns every later
+    // kernel into a no-op, so the host enqueues batches of iterations.
+    // Partials: pa[parity][2*G] (r.r, r.z) written by init / U_k with parity k & 1,
+    // pb[G] (p.q) written by S_k.
+
+    __device__ __forceinline__ int64_t pcg_groups(int64_t n_rows) { return n_rows <= 0 ? 1 : ((n_rows + NT - 1) / NT < MAX_GROUPS ? (n_rows + NT - 1) / NT : MAX_GROUPS); }
+
+    template <typename T>
+    __device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int64_t n_rows, int64_t row, const T (&rv)[3], T (&zv)[3])
+    {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            zv[i] = dinv[(3 * i + 0) * n_rows + row] * rv[0] + dinv[(3 * i + 1) * n_rows + row] * rv[1] + dinv[(3 * i + 2) * n_rows + row] * rv[2];
+    }
 
     template <typename T>
     __global__ __launch_bounds__(NT) void k_pcg_init(int32_t n_rows, const T* __restrict__ dinv, const T* __restrict__ b,
                                                      T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, T* __restrict__ p0,
-                                                     PcgState* st, double* partials, unsigned int* ticket)
+                                                     double* __restrict__ pa0)
     {
         __shared__ double lds[NT / 64];
-        const int row = blockIdx.x * NT + threadIdx.x;
+        const int64_t w = xcd_remap();
+        int64_t r0, r1;
+        row_chunk(n_rows, w, gridDim.x, r0, r1);
         double bb = 0.0, rz = 0.0;
-        if (row < n_rows) {
-            T rv[3];
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
+            T rv[3], zv[3];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                const int64_t k = 3 * (int64_t)row + i;
+                const int64_t k = 3 * row + i;
                 rv[i] = b[k];
                 x[k] = (T)0;
                 r[k] = rv[i];
                 p0[k] = (T)0;
             }
+            apply_dinv(dinv, n_rows, row, rv, zv);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                T zi = dinv[(int64_t)(3 * i + 0) * n_rows + row] * rv[0] + dinv[(int64_t)(3 * i + 1) * n_rows + row] * rv[1] +
-                       dinv[(int64_t)(3 * i + 2) * n_rows + row] * rv[2];
-                z[3 * (int64_t)row + i] = zi;
+                z[3 * row + i] = zv[i];
                 bb += (double)rv[i] * (double)rv[i];
-                rz += (double)rv[i] * (double)zi;
+                rz += (double)rv[i] * (double)zv[i];
             }
         }
         const double sbb = block_sum(bb, lds);
         const double srz = block_sum(rz, lds);
         if (threadIdx.x == 0) {
-            store_coherent(partials + 2 * blockIdx.x + 0, sbb);
-            store_coherent(partials + 2 * blockIdx.x + 1, srz);
-        }
-        if (!last_group(ticket)) return;
-        const double tbb = sum_partials(partials, gridDim.x, 2, 0, lds);
-        const double trz = sum_partials(partials, gridDim.x, 2, 1, lds);
-        if (threadIdx.x == 0) {
-            st->bb = tbb;
-            st->rz = trz;
-            st->beta = 0.0;
-            st->it = 0;
-            st->indef = 0;
-            st->done = 0;
-            st->converged = 0;
-            if (tbb < st->abs_tol * st->abs_tol) {
-                st->done = 1;
-                st->converged = 1;
-                st->err = 0.0;
-                st->err0 = 0.0;
-            }
-            else {
-                // x = 0 so r = b and ||r|| / ||b|| is exactly sqrt(bb / bb).
-                const double err = sqrt(tbb / tbb);
-                st->rr = tbb;
-                st->err = err;
-                st->err0 = err;
-                if (err < st->abs_tol) {
-                    st->done = 1;
-                    st->converged = 1;
-                }
-                else if (st->max_iter <= 0) {
-                    st->done = 1;
-                }
-            }
+            pa0[2 * w + 0] = sbb;
+            pa0[2 * w + 1] = srz;
         }
     }
 
     template <typename TS, typename TV>
-    __global__ __launch_bounds__(NT) void k_pcg_spmv(int32_t n_rows, const int64_t* __restrict__ slice_ptr,
+    __global__ __launch_bounds__(NT) void k_pcg_spmv(int32_t n_rows, int32_t k, const int64_t* __restrict__ slice_ptr,
                                                      const int32_t* __restrict__ col, const TS* __restrict__ val,
                                                      const TV* __restrict__ z, const TV* __restrict__ p_old, TV* __restrict__ p_new,
-                                                     TV* __restrict__ q, PcgState* st, double* partials, unsigned int* ticket)
+                                                     TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_prev,
+                                                     const double* __restrict__ pa_prev2, double* __restrict__ pb)
     {
         __shared__ double lds[NT / 64];
         if (st->done) return;
-        const TV beta = (TV)st->beta;
-        const int row = blockIdx.x * NT + threadIdx.x;
+        const int64_t G = gridDim.x;
+        const double rr = sum_partials(pa_prev, G, 2, 0, lds);
+        const double rz = sum_partials(pa_prev, G, 2, 1, lds);
+        double beta = 0.0;
+        int done = 0, converged = 0, it = k - 1;
+        double err = 0.0;
+        if (k == 1) {
+            const double bb = rr;  // x = 0, so r = b
+            if (bb < st->abs_tol * st->abs_tol) {
+                done = 1;
+                converged = 1;
+            }
+            else {
+                err = sqrt(rr / bb);
+                if (err < st->abs_tol) {
+                    done = 1;
+                    converged = 1;
+                }
+                else if (st->max_iter <= 0) {
+                    done = 1;
+                }
+            }
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                st->bb = bb;
+                st->err0 = err;
+                st->rz = rz;
+            }
+        }
+        else {
+            err = sqrt(rr / st->bb);
+            if (err < st->abs_tol || err / st->err0 < st->rel_tol) {
+                done = 1;
+                converged = 1;
+            }
+            else if (k - 1 >= st->max_iter) {
+                done = 1;
+            }
+            else {
+                beta = rz / sum_partials(pa_prev2, G, 2, 1, lds);
+            }
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->it = it;
+            st->err = err;
+            st->rr = rr;
+            st->beta = beta;
+            if (done) {
+                st->done = 1;
+                st->converged = converged;
+            }
+        }
+        if (done) return;
+        const TV tb = (TV)beta;
+        const int64_t w = xcd_remap();
+        int64_t r0, r1;
+        row_chunk(n_rows, w, G, r0, r1);
         double dot = 0.0;
-        if (row < n_rows) {
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
             const int64_t b0 = slice_ptr[row >> 6];
-            const int w = (int)((slice_ptr[(row >> 6) + 1] - b0) >> 6);
-            const int lane = row & 63;
+            const int wd = (int)((slice_ptr[(row >> 6) + 1] - b0) >> 6);
+            const int lane = (int)(row & 63);
             const int32_t* cp = col + b0 + lane;
             const TS* vp = val + b0 * 9 + lane;
             TV a0 = (TV)0, a1 = (TV)0, a2 = (TV)0;
-            for (int j = 0; j < w; j++) {
+            for (int j = 0; j < wd; j++) {
                 const int64_t c = 3 * (int64_t)cp[j * 64];
-                const TV x0 = z[c + 0] + beta * p_old[c + 0];
-                const TV x1 = z[c + 1] + beta * p_old[c + 1];
-                const TV x2 = z[c + 2] + beta * p_old[c + 2];
+                const TV x0 = z[c + 0] + tb * p_old[c + 0];
+                const TV x1 = z[c + 1] + tb * p_old[c + 1];
+                const TV x2 = z[c + 2] + tb * p_old[c + 2];
                 const TS* v = vp + (int64_t)j * 576;
                 a0 += (TV)v[0] * x0 + (TV)v[64] * x1 + (TV)v[128] * x2;
                 a1 += (TV)v[192] * x0 + (TV)v[256] * x1 + (TV)v[320] * x2;
                 a2 += (TV)v[384] * x0 + (TV)v[448] * x1 + (TV)v[512] * x2;
             }
-            const int64_t k = 3 * (int64_t)row;
-            const TV p0 = z[k + 0] + beta * p_old[k + 0];
-            const TV p1 = z[k + 1] + beta * p_old[k + 1];
-            const TV p2 = z[k + 2] + beta * p_old[k + 2];
-            p_new[k + 0] = p0;
-            p_new[k + 1] = p1;
-            p_new[k + 2] = p2;
-            q[k + 0] = a0;
-            q[k + 1] = a1;
-            q[k + 2] = a2;
-            dot = (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
+            const int64_t kk = 3 * row;
+            const TV p0 = z[kk + 0] + tb * p_old[kk + 0];
+            const TV p1 = z[kk + 1] + tb * p_old[kk + 1];
+            const TV p2 = z[kk + 2] + tb * p_old[kk + 2];
+            p_new[kk + 0] = p0;
+            p_new[kk + 1] = p1;
+            p_new[kk + 2] = p2;
+            q[kk + 0] = a0;
+            q[kk + 1] = a1;
+            q[kk + 2] = a2;
+            dot += (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
         }
         const double s = block_sum(dot, lds);
-        if (threadIdx.x == 0) store_coherent(partials + blockIdx.x, s);
-        if (!last_group(ticket)) return;
-        const double pAp = sum_partials(partials, gridDim.x, 1, 0, lds);
-        if (threadIdx.x == 0) {
-            st->pAp = pAp;
-            st->alpha = st->rz / pAp;
-            if (pAp <= 0.0) {
+        if (threadIdx.x == 0) pb[w] = s;
+    }
+
+    template <typename TV>
+    __global__ __launch_bounds__(NT) void k_pcg_update(int32_t n_rows, int32_t k, const TV* __restrict__ dinv, TV* __restrict__ x,
+                                                       TV* __restrict__ r, TV* __restrict__ z, const TV* __restrict__ p,
+                                                       const TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_cur,
+                                                       double* __restrict__ pa_next, const double* __restrict__ pb)
+    {
+        __shared__ double lds[NT / 64];
+        if (st->done) return;
+        const int64_t G = gridDim.x;
+        const double pAp = sum_partials(pb, G, 1, 0, lds);
+        const double rz = sum_partials(pa_cur, G, 2, 1, lds);
+        const double alpha = rz / pAp;
+        if (pAp <= 0.0) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
                 st->indef = 1;
                 if (st->stop_on_indef) {
                     st->done = 1;
                     st->converged = 0;
-                    st->it = st->it + 1;
+                    st->it = k;
                 }
             }
+            if (st->stop_on_indef) return;
         }
-    }
-
-    template <typename TV>
-    __global__ __launch_bounds__(NT) void k_pcg_update(int32_t n_rows, const TV* __restrict__ dinv, TV* __restrict__ x,
-                                                       TV* __restrict__ r, TV* __restrict__ z, const TV* __restrict__ p,
-                                                       const TV* __restrict__ q, PcgState* st, double* partials, unsigned int* ticket)
-    {
-        __shared__ double lds[NT / 64];
-        if (st->done) return;
-        const TV alpha = (TV)st->alpha;
-        const int row = blockIdx.x * NT + threadIdx.x;
-        double rr = 0.0, rz = 0.0;
-        if (row < n_rows) {
-            const int64_t k = 3 * (int64_t)row;
-            TV rv[3];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->pAp = pAp;
+            st->alpha = alpha;
+        }
+        const TV ta = (TV)alpha;
+        const int64_t w = xcd_remap();
+        int64_t r0, r1;
+        row_chunk(n_rows, w, G, r0, r1);
+        double rr = 0.0, rzn = 0.0;
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
+            const int64_t kk = 3 * row;
+            TV rv[3], zv[3];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                x[k + i] = x[k + i] + alpha * p[k + i];
-                rv[i] = r[k + i] - alpha * q[k + i];
-                r[k + i] = rv[i];
+                x[kk + i] = x[kk + i] + ta * p[kk + i];
+                rv[i] = r[kk + i] - ta * q[kk + i];
+                r[kk + i] = rv[i];
             }
+            apply_dinv(dinv, n_rows, row, rv, zv);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                const TV zi = dinv[(int64_t)(3 * i + 0) * n_rows + row] * rv[0] + dinv[(int64_t)(3 * i + 1) * n_rows + row] * rv[1] +
-                              dinv[(int64_t)(3 * i + 2) * n_rows + row] * rv[2];
-                z[k + i] = zi;
+                z[kk + i] = zv[i];
                 rr += (double)rv[i] * (double)rv[i];
-                rz += (double)rv[i] * (double)zi;
+                rzn += (double)rv[i] * (double)zv[i];
             }
         }
         const double srr = block_sum(rr, lds);
-        const double srz = block_sum(rz, lds);
+        const double srz = block_sum(rzn, lds);
         if (threadIdx.x == 0) {
-            store_coherent(partials + 2 * blockIdx.x + 0, srr);
-            store_coherent(partials + 2 * blockIdx.x + 1, srz);
-        }
-        if (!last_group(ticket)) return;
-        const double trr = sum_partials(partials, gridDim.x, 2, 0, lds);
-        const double trz = sum_partials(partials, gridDim.x, 2, 1, lds);
-        if (threadIdx.x == 0) {
-            const int it = st->it + 1;
-            const double err = sqrt(trr / st->bb);
-            st->rr = trr;
-            st->err = err;
-            st->it = it;
-            if (err < st->abs_tol || err / st->err0 < st->rel_tol) {
-                st->done = 1;
-                st->converged = 1;
-            }
-            else if (it >= st->max_iter) {
-                st->done = 1;
-                st->converged = 0;
-            }
-            else {
-                st->beta = trz / st->rz;
-                st->rz = trz;
-            }
+            pa_next[2 * w + 0] = srr;
+            pa_next[2 * w + 1] = srz;
         }
     }
 
+    int pcg_grid(int32_t n_rows) { return reduction_groups(n_rows); }
+
     void launch_pcg_init64(hipStream_t s, const SellPattern& P, const double* dinv, const double* b, double* x, double* r, double* z,
-                           double* p0, PcgState* st, double* partials, unsigned int* ticket)
+                           double* p0, double* partials)
     {
-        k_pcg_init<double><<<grid_for(P.n_rows), NT, 0, s>>>(P.n_rows, dinv, b, x, r, z, p0, st, partials, ticket);
+        k_pcg_init<double><<<pcg_grid(P.n_rows), NT, 0, s>>>(P.n_rows, dinv, b, x, r, z, p0, partials);
     }
-    void launch_pcg_iter64(hipStream_t s, const SellPattern& P, const double* val, const double* dinv, double* x, double* r, double* z,
-                           const double* p_old, double* p_new, double* q, PcgState* st, double* partials, unsigned int* ticket)
+    void launch_pcg_iter64(hipStream_t s, const SellPattern& P, int32_t k, const double* val, const double* dinv, double* x, double* r,
+                           double* z, const double* p_old, double* p_new, double* q, PcgState* st, double* partials)
     {
-        const unsigned g = grid_for(P.n_rows);
-        k_pcg_spmv<double, double><<<g, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_col, val, z, p_old, p_new, q, st, partials, ticket);
-        k_pcg_update<double><<<g, NT, 0, s>>>(P.n_rows, dinv, x, r, z, p_new, q, st, partials, ticket);
+        const int G = pcg_grid(P.n_rows);
+        double* pa[2] = { partials, partials + 2 * G };
+        double* pb = partials + 4 * G;
+        k_pcg_spmv<double, double><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_col, val, z, p_old, p_new, q, st, pa[(k - 1) & 1],
+                                                    pa[k & 1], pb);
+        k_pcg_update<double><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
     void launch_pcg_init32(hipStream_t s, const SellPattern& P, const float* dinv, const float* b, float* x, float* r, float* z,
-                           float* p0, PcgState* st, double* partials, unsigned int* ticket)
+                           float* p0, double* partials)
     {
-        k_pcg_init<float><<<grid_for(P.n_rows), NT, 0, s>>>(P.n_rows, dinv, b, x, r, z, p0, st, partials, ticket);
+        k_pcg_init<float><<<pcg_grid(P.n_rows), NT, 0, s>>>(P.n_rows, dinv, b, x, r, z, p0, partials);
     }
-    void launch_pcg_iter32(hipStream_t s, const SellPattern& P, const float* val, const float* dinv, float* x, float* r, float* z,
-                           const float* p_old, float* p_new, float* q, PcgState* st, double* partials, unsigned int* ticket)
+    void launch_pcg_iter32(hipStream_t s, const SellPattern& P, int32_t k, const float* val, const float* dinv, float* x, float* r,
+                           float* z, const float* p_old, float* p_new, float* q, PcgState* st, double* partials)
     {
-        const unsigned g = grid_for(P.n_rows);
-        k_pcg_spmv<float, float><<<g, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_col, val, z, p_old, p_new, q, st, partials, ticket);
-        k_pcg_update<float><<<g, NT, 0, s>>>(P.n_rows, dinv, x, r, z, p_new, q, st, partials, ticket);
+        const int G = pcg_grid(P.n_rows);
+        double* pa[2] = { partials, partials + 2 * G };
+        double* pb = partials + 4 * G;
+        k_pcg_spmv<float, float><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_col, val, z, p_old, p_new, q, st, pa[(k - 1) & 1],
+                                                  pa[k & 1], pb);
+        k_pcg_update<float><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
 
     // ---- mixed-precision refinement -----------------------------------------------
@@ -698,20 +684,21 @@ namespace symx_hip
     __global__ __launch_bounds__(NT) void k_residual_mixed(int32_t n_rows, const int64_t* __restrict__ slice_ptr,
                                                            const int32_t* __restrict__ col, const float* __restrict__ val,
                                                            const double* __restrict__ b, const double* __restrict__ x,
-                                                           double* __restrict__ r, float* __restrict__ r32, double* rr_out,
-                                                           double* partials, unsigned int* ticket)
+                                                           double* __restrict__ r, float* __restrict__ r32, double* __restrict__ partials)
     {
         __shared__ double lds[NT / 64];
-        const int row = blockIdx.x * NT + threadIdx.x;
+        const int64_t w = xcd_remap();
+        int64_t r0, r1;
+        row_chunk(n_rows, w, gridDim.x, r0, r1);
         double rr = 0.0;
-        if (row < n_rows) {
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
             const int64_t b0 = slice_ptr[row >> 6];
-            const int w = (int)((slice_ptr[(row >> 6) + 1] - b0) >> 6);
-            const int lane = row & 63;
+            const int wd = (int)((slice_ptr[(row >> 6) + 1] - b0) >> 6);
+            const int lane = (int)(row & 63);
             const int32_t* cp = col + b0 + lane;
             const float* vp = val + b0 * 9 + lane;
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-            for (int j = 0; j < w; j++) {
+            for (int j = 0; j < wd; j++) {
                 const int64_t c = 3 * (int64_t)cp[j * 64];
                 const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
                 const float* v = vp + (int64_t)j * 576;
@@ -719,32 +706,30 @@ namespace symx_hip
                 a1 += (double)v[192] * x0 + (double)v[256] * x1 + (double)v[320] * x2;
                 a2 += (double)v[384] * x0 + (double)v[448] * x1 + (double)v[512] * x2;
             }
-            const int64_t k = 3 * (int64_t)row;
-            const double r0 = b[k] - a0, r1 = b[k + 1] - a1, r2 = b[k + 2] - a2;
-            r[k] = r0;
-            r[k + 1] = r1;
-            r[k + 2] = r2;
-            r32[k] = (float)r0;
-            r32[k + 1] = (float)r1;
-            r32[k + 2] = (float)r2;
-            rr = r0 * r0 + r1 * r1 + r2 * r2;
+            const int64_t k = 3 * row;
+            const double e0 = b[k] - a0, e1 = b[k + 1] - a1, e2 = b[k + 2] - a2;
+            r[k] = e0;
+            r[k + 1] = e1;
+            r[k + 2] = e2;
+            r32[k] = (float)e0;
+            r32[k + 1] = (float)e1;
+            r32[k + 2] = (float)e2;
+            rr += e0 * e0 + e1 * e1 + e2 * e2;
         }
         const double s = block_sum(rr, lds);
-        if (threadIdx.x == 0) store_coherent(partials + blockIdx.x, s);
-        if (!last_group(ticket)) return;
-        const double t = sum_partials(partials, gridDim.x, 1, 0, lds);
-        if (threadIdx.x == 0) *rr_out = t;
+        if (threadIdx.x == 0) partials[w] = s;
     }
     void launch_residual_mixed(hipStream_t s, const SellPattern& P, const float* val, const double* b, const double* x, double* r,
-                               float* r32, double* rr_out, double* partials, unsigned int* ticket)
+                               float* r32, double* rr_out, double* partials)
     {
-        k_residual_mixed<<<grid_for(P.n_rows), NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_col, val, b, x, r, r32, rr_out, partials,
-                                                          ticket);
+        const int G = pcg_grid(P.n_rows);
+        k_residual_mixed<<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_col, val, b, x, r, r32, partials);
+        k_sum<<<1, NT, 0, s>>>(partials, G, 1, 0, rr_out);
     }
 
     __global__ __launch_bounds__(NT) void k_x_update_mixed(int64_t n, double* __restrict__ x, const float* __restrict__ d)
     {
-        const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+        const int64_t i = xcd_remap() * NT + threadIdx.x;
         if (i < n) x[i] += (double)d[i];
     }
     void launch_x_update_mixed(hipStream_t s, int64_t n, double* x, const float* d)
@@ -773,44 +758,49 @@ namespace symx_hip
         if (n > 0) k_scale<<<grid_for(n), NT, 0, s>>>(n, dst, a, sc);
     }
 
-    __global__ __launch_bounds__(NT) void k_dot(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* out,
-                                                double* partials, unsigned int* ticket)
+    __global__ __launch_bounds__(NT) void k_dot_partials(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                                         double* __restrict__ partials)
     {
         __shared__ double lds[NT / 64];
+        const int64_t w = xcd_remap();
+        int64_t r0, r1;
+        row_chunk(n, w, gridDim.x, r0, r1);
         double acc = 0.0;
-        for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) acc += a[i] * b[i];
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) acc += a[i] * b[i];
         const double s = block_sum(acc, lds);
-        if (threadIdx.x == 0) store_coherent(partials + blockIdx.x, s);
-        if (!last_group(ticket)) return;
-        const double t = sum_partials(partials, gridDim.x, 1, 0, lds);
-        if (threadIdx.x == 0) *out = t;
+        if (threadIdx.x == 0) partials[w] = s;
     }
-    void launch_dot(hipStream_t s, int64_t n, const double* a, const double* b, double* out, double* partials, unsigned int* ticket)
+    void launch_dot(hipStream_t s, int64_t n, const double* a, const double* b, double* out, double* partials)
     {
-        k_dot<<<reduction_groups(n), NT, 0, s>>>(n, a, b, out, partials, ticket);
+        const int G = reduction_groups(n);
+        k_dot_partials<<<G, NT, 0, s>>>(n, a, b, partials);
+        k_sum<<<1, NT, 0, s>>>(partials, G, 1, 0, out);
     }
 
-    __global__ __launch_bounds__(NT) void k_max_abs(int64_t n, const double* __restrict__ a, double* out, double* partials,
-                                                    unsigned int* ticket)
+    __global__ __launch_bounds__(NT) void k_max_abs_partials(int64_t n, const double* __restrict__ a, double* __restrict__ partials)
+    {
+        __shared__ double lds[NT / 64];
+        const int64_t w = xcd_remap();
+        int64_t r0, r1;
+        row_chunk(n, w, gridDim.x, r0, r1);
+        double m = 0.0;
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) m = nanmax(m, fabs(a[i]));
+        const double s = block_max(m, lds);
+        if (threadIdx.x == 0) partials[w] = s;
+    }
+    __global__ __launch_bounds__(NT) void k_max_final(const double* __restrict__ partials, int64_t G, double* out)
     {
         __shared__ double lds[NT / 64];
         double m = 0.0;
-        for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-            m = nanmax(m, fabs(a[i]));
-        }
-        const double s = block_max(m, lds);
-        if (threadIdx.x == 0) store_coherent(partials + blockIdx.x, s);
-        if (!last_group(ticket)) return;
-        double acc = 0.0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += NT) {
-            acc = nanmax(acc, load_coherent(partials + i));
-        }
-        const double t = block_max(acc, lds);
-        if (threadIdx.x == 0) *out = t;
+        for (int64_t i = threadIdx.x; i < G; i += NT) m = nanmax(m, partials[i]);
+        const double r = block_max(m, lds);
+        if (threadIdx.x == 0) *out = r;
     }
-    void launch_max_abs(hipStream_t s, int64_t n, const double* a, double* out, double* partials, unsigned int* ticket)
+    void launch_max_abs(hipStream_t s, int64_t n, const double* a, double* out, double* partials)
     {
-        k_max_abs<<<reduction_groups(n), NT, 0, s>>>(n, a, out, partials, ticket);
+        const int G = reduction_groups(n);
+        k_max_abs_partials<<<G, NT, 0, s>>>(n, a, partials);
+        k_max_final<<<1, NT, 0, s>>>(partials, G, out);
     }
 
     __global__ __launch_bounds__(NT) void k_fill_u8(int64_t n, uint8_t* p, uint8_t v)

@@ -92,6 +92,14 @@ __device__ __forceinline__ double symx_powi(double x, int n)
     while (m) { if (m & 1u) r *= b; b *= b; m >>= 1; }
     return neg ? 1.0 / r : r;
 }
+// Logical workgroup index: the hardware dispatches workgroups round-robin over the
+// 8 XCDs; each XCD gets a contiguous range of elements (shared vertices stay in its L2).
+__device__ __forceinline__ int symx_block()
+{
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int x = b % 8, i = b / 8, per = nb / 8, extra = nb % 8;
+    return x * per + (x < extra ? x : extra) + i;
+}
 // Fixed-order workgroup sum of the per-element energies (256 threads).
 __device__ __forceinline__ void symx_store_energy(double v, double* E_partial)
 {
@@ -99,7 +107,7 @@ __device__ __forceinline__ void symx_store_energy(double v, double* E_partial)
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) E_partial[blockIdx.x] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    if (threadIdx.x == 0) E_partial[symx_block()] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
 }
 // ---- end of prelude ----
 )PRELUDE";
@@ -228,7 +236,7 @@ int symx_hip_ctx_create(int device, symx_hip_ctx* out)
     int rc = dalloc(ctx, &ctx->d_ticket, 4);
     if (!rc) rc = dalloc(ctx, &ctx->d_scalars, 64);
     if (!rc) rc = dalloc(ctx, &ctx->d_pcg, 1);
-    if (!rc) rc = ensure_partials(ctx, 4096);
+    if (!rc) rc = ensure_partials(ctx, 16384);
     if (rc) {
         symx_hip_ctx_destroy(ctx);
         return rc;
@@ -377,6 +385,7 @@ void symx_hip_problem_destroy(symx_hip_problem p)
     dfree(p->d_G);
     dfree(p->d_H);
     dfree(p->d_Epart);
+    dfree(p->d_flags);
     dfree(p->d_goff);
     dfree(p->d_glist);
     dfree(p->sell.d_slice_ptr);
@@ -625,6 +634,11 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     rc = rc ? rc : dalloc(ctx, &p->d_G, (size_t)std::max<int64_t>(g * 3, 1));
     rc = rc ? rc : dalloc(ctx, &p->d_H, (size_t)std::max<int64_t>(h * 9, 1));
     rc = rc ? rc : dalloc(ctx, &p->d_Epart, (size_t)std::max<int64_t>(parts, 1));
+    {
+        int64_t max_elem = 1;
+        for (auto& pot : p->pots) max_elem = std::max<int64_t>(max_elem, pot.n_elem);
+        rc = rc ? rc : dalloc(ctx, &p->d_flags, (size_t)max_elem);
+    }
     rc = rc ? rc : dalloc(ctx, &p->d_goff, goff.size());
     rc = rc ? rc : dalloc(ctx, &p->d_glist, std::max<size_t>(glist.size(), 1));
     rc = rc ? rc : dalloc(ctx, &S.d_slice_ptr, S.h_slice_ptr.size());
@@ -757,8 +771,7 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
     unsigned long long* d_count = (unsigned long long*)(ctx->d_scalars + 8);
     HIP_TRY(ctx, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), ctx->stream));
     for (auto& pot : p->pots) {
-        const double skip_tol = 0.0;
-        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, skip_tol, d_count))
+        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, d_count))
             return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports up to 4 DoF blocks per element");
     }
     HIP_TRY(ctx, hipGetLastError());
@@ -854,8 +867,8 @@ int symx_hip_problem_export_bsr(symx_hip_problem p, int64_t* nnzb, int64_t* row_
     return SYMX_HIP_OK;
 }
 
-static int run_pcg64(symx_hip_problem p, const double* b, double* x, double abs_tol, double rel_tol, int32_t max_iter, int32_t stop_on_indef,
-                     PcgState& out)
+static int run_pcg(symx_hip_problem p, bool fp32, const double* b64, double* x64, double abs_tol, double rel_tol, int32_t max_iter,
+                   int32_t stop_on_indef, PcgState& out)
 {
     symx_hip_ctx ctx = p->ctx;
     const SellPattern& S = p->sell;
@@ -866,49 +879,23 @@ static int run_pcg64(symx_hip_problem p, const double* b, double* x, double abs_
     st.stop_on_indef = stop_on_indef;
     *ctx->h_pcg = st;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pcg, ctx->h_pcg, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
-    launch_pcg_init64(ctx->stream, S, p->d_dinv64, b, x, p->d_r, p->d_z, p->d_p0, ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
-    double* pb[2] = { p->d_p0, p->d_p1 };
-    int64_t launched = 0;
+    if (fp32) launch_pcg_init32(ctx->stream, S, p->d_dinv32, p->f_b, p->f_x, p->f_r, p->f_z, p->f_p0, ctx->d_partials);
+    else launch_pcg_init64(ctx->stream, S, p->d_dinv64, b64, x64, p->d_r, p->d_z, p->d_p0, ctx->d_partials);
+    int32_t k = 1;
     int batch = 8;
     while (true) {
+        for (int i = 0; i < batch; i++, k++) {
+            if (fp32)
+                launch_pcg_iter32(ctx->stream, S, k, p->d_val32, p->d_dinv32, p->f_x, p->f_r, p->f_z, (k & 1) ? p->f_p0 : p->f_p1,
+                                  (k & 1) ? p->f_p1 : p->f_p0, p->f_q, ctx->d_pcg, ctx->d_partials);
+            else
+                launch_pcg_iter64(ctx->stream, S, k, p->d_val64, p->d_dinv64, x64, p->d_r, p->d_z, (k & 1) ? p->d_p0 : p->d_p1,
+                                  (k & 1) ? p->d_p1 : p->d_p0, p->d_q, ctx->d_pcg, ctx->d_partials);
+        }
+        HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pcg, ctx->d_pcg, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->h_pcg->done) break;
-        for (int i = 0; i < batch; i++, launched++) {
-            launch_pcg_iter64(ctx->stream, S, p->d_val64, p->d_dinv64, x, p->d_r, p->d_z, pb[launched & 1], pb[(launched + 1) & 1], p->d_q,
-                              ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
-        }
-        HIP_TRY(ctx, hipGetLastError());
-        batch = std::min(batch * 2, 64);
-    }
-    out = *ctx->h_pcg;
-    return SYMX_HIP_OK;
-}
-
-static int run_pcg32(symx_hip_problem p, double rel_tol, int32_t max_iter, int32_t stop_on_indef, PcgState& out)
-{
-    symx_hip_ctx ctx = p->ctx;
-    const SellPattern& S = p->sell;
-    PcgState st{};
-    st.abs_tol = 0.0;
-    st.rel_tol = rel_tol;
-    st.max_iter = max_iter;
-    st.stop_on_indef = stop_on_indef;
-    *ctx->h_pcg = st;
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pcg, ctx->h_pcg, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
-    launch_pcg_init32(ctx->stream, S, p->d_dinv32, p->f_b, p->f_x, p->f_r, p->f_z, p->f_p0, ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
-    float* pb[2] = { p->f_p0, p->f_p1 };
-    int64_t launched = 0;
-    int batch = 8;
-    while (true) {
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pcg, ctx->d_pcg, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        if (ctx->h_pcg->done) break;
-        for (int i = 0; i < batch; i++, launched++) {
-            launch_pcg_iter32(ctx->stream, S, p->d_val32, p->d_dinv32, p->f_x, p->f_r, p->f_z, pb[launched & 1], pb[(launched + 1) & 1], p->f_q,
-                              ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
-        }
-        HIP_TRY(ctx, hipGetLastError());
         batch = std::min(batch * 2, 64);
     }
     out = *ctx->h_pcg;
@@ -926,7 +913,7 @@ int symx_hip_problem_solve(symx_hip_problem p, int32_t precision, double abs_tol
     double* x = p->d_vec[SYMX_HIP_VEC_DU];
     if (precision == SYMX_HIP_FP64) {
         PcgState st;
-        int rc = run_pcg64(p, b, x, abs_tol, rel_tol, max_iter, stop_on_indef, st);
+        int rc = run_pcg(p, false, b, x, abs_tol, rel_tol, max_iter, stop_on_indef, st);
         if (rc) return rc;
         res.n_iterations = st.it;
         res.converged = st.converged;
@@ -940,7 +927,7 @@ int symx_hip_problem_solve(symx_hip_problem p, int32_t precision, double abs_tol
         double* rr_dev = ctx->d_scalars + 16;
         HIP_TRY(ctx, hipMemsetAsync(x, 0, sizeof(double) * p->n_dofs, ctx->stream));
         // r = b - A*0 = b exactly, r32 = (float) b, rr = ||b||^2
-        launch_residual_mixed(ctx->stream, S, p->d_val32, b, x, p->d_r, p->f_b, rr_dev, ctx->d_partials, ctx->d_ticket);
+        launch_residual_mixed(ctx->stream, S, p->d_val32, b, x, p->d_r, p->f_b, rr_dev, ctx->d_partials);
         double bb = 0.0;
         int rc = sync_scalar(ctx, rr_dev, &bb);
         if (rc) return rc;
@@ -958,7 +945,7 @@ int symx_hip_problem_solve(symx_hip_problem p, int32_t precision, double abs_tol
             for (int outer = 0; !res.converged && outer < 16 && total < max_iter; outer++) {
                 const double inner_rel = std::min(0.5, std::max(0.5 * target / err, 2e-6));
                 PcgState st;
-                rc = run_pcg32(p, inner_rel, max_iter - total, stop_on_indef, st);
+                rc = run_pcg(p, true, nullptr, nullptr, 0.0, inner_rel, max_iter - total, stop_on_indef, st);
                 if (rc) return rc;
                 total += st.it;
                 res.n_refinements++;
@@ -968,7 +955,7 @@ int symx_hip_problem_solve(symx_hip_problem p, int32_t precision, double abs_tol
                     break;
                 }
                 launch_x_update_mixed(ctx->stream, p->n_dofs, x, p->f_x);
-                launch_residual_mixed(ctx->stream, S, p->d_val32, b, x, p->d_r, p->f_b, rr_dev, ctx->d_partials, ctx->d_ticket);
+                launch_residual_mixed(ctx->stream, S, p->d_val32, b, x, p->d_r, p->f_b, rr_dev, ctx->d_partials);
                 double rr = 0.0;
                 rc = sync_scalar(ctx, rr_dev, &rr);
                 if (rc) return rc;
@@ -988,7 +975,7 @@ int symx_hip_problem_max_abs(symx_hip_problem p, int32_t vec, double* out)
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
     double* d = vec_ptr(p, vec);
     if (!d) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
-    launch_max_abs(p->ctx->stream, p->n_dofs, d, p->ctx->d_scalars, p->ctx->d_partials, p->ctx->d_ticket);
+    launch_max_abs(p->ctx->stream, p->n_dofs, d, p->ctx->d_scalars, p->ctx->d_partials);
     return sync_scalar(p->ctx, p->ctx->d_scalars, out);
 }
 int symx_hip_problem_dot(symx_hip_problem p, int32_t a, int32_t b, double* out)
@@ -997,7 +984,7 @@ int symx_hip_problem_dot(symx_hip_problem p, int32_t a, int32_t b, double* out)
     double* da = vec_ptr(p, a);
     double* db = vec_ptr(p, b);
     if (!da || !db) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
-    launch_dot(p->ctx->stream, p->n_dofs, da, db, p->ctx->d_scalars, p->ctx->d_partials, p->ctx->d_ticket);
+    launch_dot(p->ctx->stream, p->n_dofs, da, db, p->ctx->d_scalars, p->ctx->d_partials);
     return sync_scalar(p->ctx, p->ctx->d_scalars, out);
 }
 int symx_hip_problem_axpy(symx_hip_problem p, int32_t dst, int32_t a, double s, int32_t b)
@@ -1049,7 +1036,7 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
         case 1: {
             // project a fresh copy of the element Hessians each repetition
             hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice, ctx->stream);
-            return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, 0.0, nullptr) ? SYMX_HIP_ERR_ARG : 0;
+            return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, nullptr) ? SYMX_HIP_ERR_ARG : 0;
         }
         case 2:
             launch_bsr_assemble(ctx->stream, S, p->d_coff, p->d_clist, p->d_H, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
@@ -1057,11 +1044,11 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
             return 0;
         case 3:
             if (arg == SYMX_HIP_FP64)
-                launch_pcg_iter64(ctx->stream, S, p->d_val64, p->d_dinv64, p->d_x, p->d_r, p->d_z, (i & 1) ? p->d_p1 : p->d_p0,
-                                  (i & 1) ? p->d_p0 : p->d_p1, p->d_q, ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
+                launch_pcg_iter64(ctx->stream, S, i + 1, p->d_val64, p->d_dinv64, p->d_x, p->d_r, p->d_z, ((i + 1) & 1) ? p->d_p0 : p->d_p1,
+                                  ((i + 1) & 1) ? p->d_p1 : p->d_p0, p->d_q, ctx->d_pcg, ctx->d_partials);
             else
-                launch_pcg_iter32(ctx->stream, S, p->d_val32, p->d_dinv32, p->f_x, p->f_r, p->f_z, (i & 1) ? p->f_p1 : p->f_p0,
-                                  (i & 1) ? p->f_p0 : p->f_p1, p->f_q, ctx->d_pcg, ctx->d_partials, ctx->d_ticket);
+                launch_pcg_iter32(ctx->stream, S, i + 1, p->d_val32, p->d_dinv32, p->f_x, p->f_r, p->f_z, ((i + 1) & 1) ? p->f_p0 : p->f_p1,
+                                  ((i + 1) & 1) ? p->f_p1 : p->f_p0, p->f_q, ctx->d_pcg, ctx->d_partials);
             return 0;
         default: return fail(ctx, SYMX_HIP_ERR_ARG, "bad profile phase");
         }
@@ -1076,14 +1063,12 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
         *ctx->h_pcg = st;
         HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pcg, ctx->h_pcg, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
         if (arg == SYMX_HIP_FP64)
-            launch_pcg_init64(ctx->stream, S, p->d_dinv64, p->d_vec[SYMX_HIP_VEC_GRAD], p->d_x, p->d_r, p->d_z, p->d_p0, ctx->d_pcg,
-                              ctx->d_partials, ctx->d_ticket);
+            launch_pcg_init64(ctx->stream, S, p->d_dinv64, p->d_vec[SYMX_HIP_VEC_GRAD], p->d_x, p->d_r, p->d_z, p->d_p0, ctx->d_partials);
         else {
             if (!p->f_b) return fail(ctx, SYMX_HIP_ERR_ARG, "mixed buffers missing");
             launch_residual_mixed(ctx->stream, S, p->d_val32, p->d_vec[SYMX_HIP_VEC_GRAD], p->d_vec[SYMX_HIP_VEC_DU], p->d_r, p->f_b,
-                                  ctx->d_scalars + 16, ctx->d_partials, ctx->d_ticket);
-            launch_pcg_init32(ctx->stream, S, p->d_dinv32, p->f_b, p->f_x, p->f_r, p->f_z, p->f_p0, ctx->d_pcg, ctx->d_partials,
-                              ctx->d_ticket);
+                                  ctx->d_scalars + 16, ctx->d_partials + 8192);
+            launch_pcg_init32(ctx->stream, S, p->d_dinv32, p->f_b, p->f_x, p->f_r, p->f_z, p->f_p0, ctx->d_partials);
         }
     }
     int rc = launch_once(0);  // warm-up
@@ -1100,7 +1085,7 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
         for (int i = 0; i < reps && !rc; i++) {
             hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice, ctx->stream);
             hipEventRecord(e0, ctx->stream);
-            launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, 0.0, nullptr);
+            launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, nullptr);
             hipEventRecord(e1, ctx->stream);
             hipEventSynchronize(e1);
             float ms = 0.0f;
