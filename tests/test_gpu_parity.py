@@ -188,7 +188,7 @@ def test_mixed_precision_solve_meets_the_same_stopping_rule(gpu, oracle):
         assert err < rel * 1.05 + 1e-7, (rel, err)
 
 
-@pytest.mark.parametrize("name", ["block6", "block6_tight"])
+@pytest.mark.parametrize("name", ["block6", "block6_tight", "cfg1"])
 def test_newton_matches_reference_fixture(gpu, oracle, name):
     f = np.load(os.path.join(GOLDEN, f"newton_ref_{name}.npz"))
     cells = tuple(int(c) for c in f["cells"])
@@ -199,10 +199,11 @@ def test_newton_matches_reference_fixture(gpu, oracle, name):
     st = s.solve()
     assert st["result_name"] == "Successful"
     assert st["newton_iterations"] == int(f["newton_iterations"])
+    assert abs(st["cg_iterations"] - int(f["cg_iterations"])) <= st["newton_iterations"]
     x_ref = f["x"]
     x = s.get("x")
     disp = np.abs(x_ref - S.X).max()
-    tol = 1e-6 if name == "block6" else 1e-8
+    tol = 1e-8 if name == "block6_tight" else 1e-6
     assert np.abs(x - x_ref).max() <= tol * disp
     E_ref = oracle.energy(S, x_ref, with_grad=False)
     E = oracle.energy(S, x, with_grad=False)
