@@ -124,6 +124,114 @@ def cpu_baseline(log, cells=40):
     }
 
 
+def bench_adjoint(args, log):
+    """Adjoint gradients/s on the quasistatic Neo-Hookean cantilever: 64x16x16 vertex grid
+    (63x15x15 cells, 85,050 tets, 49,152 DoF) of length 0.1 m, x=0 face held, gravity,
+    per-tet log(mu) ~ N(log 3.5e4, 0.3^2) (seed 1), nu = 0.4, L = 0.5 |x_eq - x_target|^2
+    with x_target the equilibrium of a second field (seed 2). One step = one gradient
+    dL/dlog_mu at the equilibrium: E/grad/Hessian evaluation, unprojected assembly, PCG
+    for H lambda = dL/dx to 1e-10 relative, mixed-partial kernel, scatter."""
+    import ctypes
+
+    import symx_amd
+
+    cells = [63, 15, 15]
+    L = 0.1
+    params = symx_amd.default_block_params(cells=cells, size=[L, L * 15 / 63, L * 15 / 63], youngs_modulus=1e5, poissons_ratio=0.4,
+                                           density=1000.0, gravity=[0.0, 0.0, -9.81], dt=0.0, noise=0.0, fixed_axis=0, log_mu_field=1)
+    prec = symx_amd.FP64 if args.adjoint_precision == "fp64" else symx_amd.MIXED
+    newton = symx_amd.default_newton_params(precision=prec, projection_mode=1, residual_tolerance_rel=1e-4, max_iterations=300,
+                                            cg_max_iterations=100000)
+    scene = symx_amd.Tet4Block(params, newton)
+    ne, nv = scene.n_elements, scene.n_vertices
+    X = scene.get("X")
+
+    def field(seed):
+        return np.log(3.5e4) + 0.3 * np.random.default_rng(seed).standard_normal(ne)
+
+    def equilibrium(log_mu):
+        scene.set_log_mu(log_mu)
+        scene.set("x", X)
+        t = time.perf_counter()
+        st = scene.solve()
+        log(f"forward solve: {st['result_name']} newton={st['newton_iterations']} cg={st['cg_iterations']} {time.perf_counter() - t:.3f}s")
+        return scene.get("x").reshape(-1), st
+
+    x_t, _ = equilibrium(field(2))
+    x, fwd = equilibrium(field(1))
+    rhs = x - x_t
+    for i in range(args.warmup):
+        g, st = scene.adjoint(rhs, rel_tol=1e-10)
+        log(f"warmup adjoint {i}: cg={st['cg_iterations']} converged={st['converged']} {st['seconds']:.4f}s")
+    cg = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g, st = scene.adjoint(rhs, rel_tol=1e-10)
+        cg += st["cg_iterations"]
+    t1 = time.perf_counter()
+    log(f"adjoint: {args.steps} gradients in {t1 - t0:.4f}s, cg/gradient={cg / args.steps:.0f}, max|dL/dp|={np.abs(g).max():.3e}")
+
+    # kernel times (HIP events on the runtime stream) at the equilibrium
+    p = scene.problem
+    prof = {}
+    scene.set_dofs(x)
+    scene.eval(symx_amd.EVAL_PGH)
+    for name, phase, arg in [("element", 0, 0), ("assembly", 2, prec), ("pcg_iteration", 3, prec)]:
+        ms = ctypes.c_double()
+        symx_amd.check_hip(symx_amd.hip().symx_hip_problem_profile(p, ctypes.c_int32(phase), ctypes.c_int32(0), ctypes.c_int32(arg),
+                                                                   ctypes.c_int32(5), ctypes.byref(ms)))
+        prof[name] = ms.value
+    nnzb_c = ctypes.c_int64()
+    symx_amd.check_hip(symx_amd.hip().symx_hip_problem_export_bsr(p, ctypes.byref(nnzb_c), None, None, None))
+    ab = algorithmic_bytes(ne, nv, nnzb_c.value, prec)
+    per_grad_ms = {"element": prof["element"], "assembly": prof["assembly"], "pcg_iteration": prof["pcg_iteration"] * cg / args.steps}
+    dominant = max(per_grad_ms, key=per_grad_ms.get)
+    unit_bytes = ab[dominant] * (ne if dominant == "element" else 1)
+    achieved = unit_bytes / (prof[dominant] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "kernel": dominant, "avg_ms": round(prof[dominant], 4), "algorithmic_bytes_per_launch": int(unit_bytes)}
+    cpu = None
+    if not args.no_cpu:
+        from oracle_py import Oracle, SceneData
+
+        orc = Oracle()
+        tets = scene.connectivity()
+        bc = np.nonzero(np.abs(X[:, 0]) < 1e-12)[0]
+        mu, lam = 1e5 / (2 * 1.4), 1e5 * 0.4 / (1.4 * 0.2)
+        S = SceneData(tets, X, x.reshape(-1, 3), X, np.zeros_like(X), scene.get("mass"), bc, X[bc], lam, mu, 0.0, 1e8, [0.0, 0.0, -9.81],
+                      mu_e=np.exp(field(1)), lambda_over_mu=lam / mu)
+        tc = time.perf_counter()
+        g_cpu, _, info = orc.adjoint_logmu(S, rhs, rel_tol=1e-10)
+        tcpu = time.perf_counter() - tc
+        log(f"cpu adjoint ({orc.num_threads()} threads): {tcpu:.2f}s cg={info['n_iterations']} "
+            f"max|g_gpu-g_cpu|/max|g|={np.abs(g - g_cpu).max() / np.abs(g_cpu).max():.2e}")
+        cpu = {"value": round(1.0 / tcpu, 4), "unit": "adjoint gradients/s", "cores": orc.num_threads(), "kind": "port",
+               "sample": f"oracle/symx_oracle.c oracle_adjoint_logmu, one full gradient of the same problem ({tcpu:.2f}s)"}
+    line = {
+        "metric": "adjoint gradients/s (dL/dlog_mu, 85,050 parameters)",
+        "value": round(args.steps / (t1 - t0), 4),
+        "unit": "adjoint gradients/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * (t1 - t0) / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == symx_amd.FP64 else "f32 Hessian/BSR/PCG + f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": "quasistatic Tet4 Neo-Hookean cantilever 64x16x16 vertices (85050 tets, 49152 DoF), per-tet log(mu), "
+                               "adjoint of L=0.5|x-x_target|^2, PCG to 1e-10 rel.", "n_elem": ne, "n_dof": 3 * nv, "nnzb": nnzb_c.value},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forward_solve": {"newton_iterations": fwd["newton_iterations"], "cg_iterations": fwd["cg_iterations"], "seconds": round(fwd["seconds"], 4)},
+        "cg_iterations_per_gradient": cg / args.steps,
+        "kernel_ms": {k: round(v, 4) for k, v in prof.items()},
+    }
+    print(json.dumps(line), flush=True)
+    scene.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +241,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cells", type=int, default=40)
+    ap.add_argument("--workload", choices=["block", "cantilever_adjoint"], default="block",
+                    help="block: the headline Newton benchmark; cantilever_adjoint: adjoint gradients/s")
+    ap.add_argument("--adjoint-precision", choices=["fp64", "mixed"], default="fp64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is designed, not built)", file=sys.stderr)
@@ -145,6 +256,9 @@ def main():
 
     if symx_amd.device_count() < 1:
         raise RuntimeError("bench.py needs a GPU")
+    if args.workload == "cantilever_adjoint":
+        bench_adjoint(args, log)
+        return
     cells = args.cells
     t0 = time.perf_counter()
     scene = symx_amd.Tet4Block(block_params(cells), symx_amd.default_newton_params(precision=symx_amd.MIXED))

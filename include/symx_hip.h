@@ -210,6 +210,30 @@ int symx_hip_problem_scale(symx_hip_problem prob, int32_t dst, int32_t a, double
 int symx_hip_batch_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const double* host_in, int64_t n,
                         int32_t n_in, double* host_out, int32_t n_out);
 
+/* ---- adjoint of a quasistatic equilibrium (no counterpart in SymX; composed
+ * from its pieces: unprojected Hessian, PCG, mixed partial kernels) ----------
+ * Solve H lambda = rhs, where H is assembled here, without PD projection, from the
+ * element Hessians of the last EVAL_PGH (the equilibrium). rhs (host, n_dofs) is
+ * dL/dx; lambda stays on the device in VEC_DU and is copied to host_lambda when
+ * it is not NULL. Stopping rule of symx_hip_problem_solve with abs_tol = 0. */
+int symx_hip_problem_adjoint_solve(symx_hip_problem prob, int32_t precision, double rel_tol, int32_t max_iter,
+                                   const double* host_rhs, double* host_lambda, symx_hip_pcg_info* info);
+/* Second argument of generated adjoint kernels, which have the signature
+ * (const symx_hip_eval_params p, const symx_hip_adjoint_params a). lambda[m] points
+ * into the adjoint vector (VEC_DU) at the DoF offset of data map m (NULL for
+ * non-DoF maps); the kernel writes out[e * n_out + j] = d(lambda . dE_e/dx)/dp_j. */
+typedef struct symx_hip_adjoint_params {
+    const double* lambda[SYMX_HIP_MAX_MAPS];
+    double* out;
+    int32_t n_out;
+    int32_t pad_;
+} symx_hip_adjoint_params;
+
+/* Launch the adjoint kernel `name` (in the potential's own module) over the
+ * elements of potential pot_id with lambda = VEC_DU; out [n_elem][n_out] is copied
+ * to host_out. */
+int symx_hip_problem_launch_adjoint(symx_hip_problem prob, int32_t pot_id, const char* name, int32_t n_out, double* host_out);
+
 /* Kernel timing with HIP events on the context's stream (the stream every
  * kernel of this library runs on): average duration in ms of `reps` launches
  * of one phase. phase: 0 element kernel E+grad+Hessian of potential `pot`,

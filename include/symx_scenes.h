@@ -82,11 +82,27 @@ int symx_scene_connectivity(symx_scene s, int32_t* conn);
 /* positions (DoFs), previous positions x0, velocities v0, rest positions, masses */
 int symx_scene_get_array(symx_scene s, const char* name, double* out);
 int symx_scene_set_array(symx_scene s, const char* name, const double* in);
-/* per-element log(mu) parameters (scenes created with log_mu_field = 1) */
+/* per-element log(mu) parameters (scenes created with log_mu_field = 1); also
+ * refreshes the device copy of the scene's data, so direct symx_hip_* evaluations
+ * see the new values */
 int symx_scene_set_log_mu(symx_scene s, const double* log_mu);
 int symx_scene_set_newton_params(symx_scene s, const symx_newton_params* np);
 /* Newton solve of the current time step (does not advance time). */
 int symx_scene_solve(symx_scene s, symx_solve_stats* st);
+/* Adjoint of the quasistatic equilibrium in x (scenes with log_mu_field = 1):
+ * solves H lambda = dL_dx on the unprojected Hessian at the current positions and
+ * writes dL_dlog_mu[e] = -lambda^T d^2E/dx dlog_mu_e (n_elements values). Solved in
+ * FP64 whatever the scene's precision; rel_tol is the PCG tolerance relative to |dL_dx|. */
+typedef struct symx_adjoint_stats {
+    int32_t cg_iterations;
+    int32_t n_refinements;
+    int32_t converged;
+    int32_t found_indefiniteness;
+    double cg_error;
+    double energy;
+    double seconds;
+} symx_adjoint_stats;
+int symx_scene_adjoint(symx_scene s, const double* dL_dx, double* dL_dlog_mu, double rel_tol, int32_t max_iter, symx_adjoint_stats* st);
 /* Solve then advance: v = (x - x0)/dt, x0 = x. */
 int symx_scene_step(symx_scene s, symx_solve_stats* st);
 /* Device problem of the scene, for direct symx_hip_* calls. */
@@ -110,6 +126,14 @@ int symx_host_point_triangle(const double* in13, double* out157);
 /* Generate the HIP module of the Tet4 block scene's elastic potential and
  * compile it with hiprtc for gfx950 (no GPU needed); returns code size. */
 int symx_host_jit_tet4(const char* cache_dir, int32_t material, int64_t* code_size, int64_t* n_ops_pgh);
+/* Generate every potential module of a Tet4 block scene (adjoint kernels included
+ * when log_mu_field = 1) and compile them with hiprtc for gfx950. */
+int symx_host_scene_jit(const symx_tet4_block_params* p, const char* cache_dir, int32_t* n_potentials, int64_t* code_bytes,
+                        int64_t* ops_adjoint);
+/* Evaluate the generated adjoint outputs d(lambda . dE_e/dx)/dlog_mu_e of element e of
+ * a log_mu_field scene on the host (symbolic graph, kernel's gather rules); lambda
+ * has one value per DoF. */
+int symx_host_scene_adjoint_element(const symx_tet4_block_params* p, const double* log_mu, int32_t e, const double* lambda, double* out);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,11 @@
 //       with NewtonsMethod (examples/dynamic_elasticity_with_contact.cpp:85-125,
 //       examples/rubber_extrusion.cpp:68-107). Input/output layouts are in
 //       tests/ref_io.py.
+//   ref_driver adjoint <in.bin> <out.bin>
+//       Adjoint dL/dlog_mu of the quasistatic cantilever at given positions, composed
+//       from the reference's Hessian assembly, DirectLLT solver and Compiled<double>.
 #include <symx>
+#include <Eigen/SparseCholesky>
 
 #include <chrono>
 #include <cstdio>
@@ -192,11 +196,167 @@ static int run_solve(int argc, char** argv)
     return 0;
 }
 
+// Adjoint of a quasistatic equilibrium composed from the reference's public pieces
+// (SymX itself has no adjoint): SecondOrderCompiledGlobal element Hessians assembled
+// without projection (NewtonsMethod.cpp:102,269), Eigen SimplicialLLT as in its
+// DirectLLT branch (NewtonsMethod.cpp:395-418), and the mixed partial
+// d(lambda . dE_e/dx)/dlog_mu_e from a Compiled<double> of the same element energy.
+static int run_adjoint(int argc, char** argv)
+{
+    if (argc < 4) return 2;
+    std::ifstream f(argv[2], std::ios::binary);
+    int32_t hdr[4];
+    rd(f, hdr, 4);
+    const int nv = hdr[0], ne = hdr[1], nbc = hdr[2], threads = hdr[3];
+    double prm[5];
+    rd(f, prm, 5);
+    struct Data
+    {
+        std::vector<Eigen::Vector3d> x, X, bc_target;
+        std::vector<double> mass, log_mu, dLdx;
+        std::vector<int32_t> tets_enum, all;
+        LabelledConnectivity<2> bc{ { "constraint_idx", "vertex_idx" } };
+        double ratio, k;
+        Eigen::Vector3d gravity;
+    } d;
+    d.ratio = prm[0];
+    d.k = prm[1];
+    d.gravity = Eigen::Vector3d(prm[2], prm[3], prm[4]);
+    std::vector<int32_t> tets((size_t)ne * 4);
+    rd(f, tets.data(), tets.size());
+    auto rdv = [&](std::vector<Eigen::Vector3d>& a, int n) {
+        std::vector<double> t((size_t)n * 3);
+        rd(f, t.data(), t.size());
+        a.resize(n);
+        for (int i = 0; i < n; i++) a[i] = Eigen::Vector3d(t[3 * i], t[3 * i + 1], t[3 * i + 2]);
+    };
+    rdv(d.X, nv);
+    rdv(d.x, nv);
+    d.mass.resize(nv);
+    rd(f, d.mass.data(), nv);
+    std::vector<int32_t> bcv(nbc);
+    rd(f, bcv.data(), nbc);
+    rdv(d.bc_target, nbc);
+    d.log_mu.resize(ne);
+    rd(f, d.log_mu.data(), ne);
+    d.dLdx.resize((size_t)nv * 3);
+    rd(f, d.dLdx.data(), d.dLdx.size());
+    for (int i = 0; i < nbc; i++) d.bc.numbered_push_back({ bcv[i] });
+    d.all.resize(nv);
+    for (int i = 0; i < nv; i++) d.all[i] = i;
+    d.tets_enum.resize((size_t)ne * 5);
+    for (int e = 0; e < ne; e++) {
+        d.tets_enum[5 * (size_t)e] = e;
+        for (int a = 0; a < 4; a++) d.tets_enum[5 * (size_t)e + 1 + a] = tets[4 * (size_t)e + a];
+    }
+
+    // Element energy of tet e as a function of its 4 positions and log(mu).
+    auto element_energy = [&](std::vector<Vector>& x, std::vector<Vector>& X, const Scalar& logmu, auto&& integrate) {
+        Scalar m = exp(logmu);
+        Scalar l = d.ratio * m;
+        return integrate([&](Scalar& w, Vector xi) {
+            Matrix Jx = fem_jacobian(FEM_Element::Tet4, x, xi);
+            Matrix JX = fem_jacobian(FEM_Element::Tet4, X, xi);
+            Matrix F = Jx * JX.inv();
+            return nh_density(F, l, m) * JX.det() * w;
+        });
+    };
+
+    spGlobalPotential G = GlobalPotential::create();
+    G->add_potential("neohookean_tet4_logmu", d.tets_enum, 5, [&](MappedWorkspace<double>& mws, Element& elem) {
+        std::vector<Vector> x = mws.make_vectors(d.x, elem.slice(1, 5));
+        std::vector<Vector> X = mws.make_vectors(d.X, elem.slice(1, 5));
+        Scalar logmu = mws.make_scalar(d.log_mu, elem[0]);
+        return element_energy(x, X, logmu, [&](auto&& summand) { return fem_integrator(mws, FEM_Element::Tet4, summand); });
+    });
+    G->add_potential("gravity", d.all, 1, [&](MappedWorkspace<double>& mws, Element& elem) {
+        Vector x = mws.make_vector(d.x, elem[0]);
+        Scalar m = mws.make_scalar(d.mass, elem[0]);
+        Vector g = mws.make_vector(d.gravity);
+        return -m * g.dot(x);
+    });
+    if (nbc > 0) {
+        G->add_potential("boundary_conditions", d.bc, [&](MappedWorkspace<double>& mws, Element& conn) {
+            Vector x = mws.make_vector(d.x, conn["vertex_idx"]);
+            Vector target = mws.make_vector(d.bc_target, conn["constraint_idx"]);
+            Scalar ks = mws.make_scalar(d.k);
+            return 0.5 * ks * (x - target).squared_norm();
+        });
+    }
+    G->add_dof(d.x);
+    spContext context = Context::create();
+    context->compilation_directory = symx::get_codegen_dir();
+    context->n_threads = threads;
+    SecondOrderCompiledGlobal compiled(G, context);
+    double E = 0.0;
+    Eigen::VectorXd grad;
+    spElementHessians eh = compiled.evaluate_P__dP_du__local_d2P_du2(E, grad);
+    const int ndofs = 3 * nv;
+    ElementHessians::spBSM H = eh->assemble_global(threads, ndofs);
+    std::vector<Eigen::Triplet<double>> triplets;
+    H->to_triplets(triplets);
+    Eigen::SparseMatrix<double> A(ndofs, ndofs);
+    A.setFromTriplets(triplets.begin(), triplets.end());
+    Eigen::SimplicialLLT<Eigen::SparseMatrix<double>> llt;
+    llt.compute(A);
+    int32_t ok = llt.info() == Eigen::Success ? 1 : 0;
+    Eigen::VectorXd rhs = Eigen::Map<const Eigen::VectorXd>(d.dLdx.data(), ndofs);
+    Eigen::VectorXd lam = Eigen::VectorXd::Zero(ndofs);
+    if (ok) {
+        lam = llt.solve(rhs);
+        ok = llt.info() == Eigen::Success ? 1 : 0;
+    }
+
+    // Mixed partial of one element: d(lambda_e . dE_e/dx_e)/dlog_mu_e.
+    Workspace ws;
+    std::vector<Vector> xs = ws.make_vectors(3, 4);
+    std::vector<Vector> Xs = ws.make_vectors(3, 4);
+    Scalar logmu = ws.make_scalar();
+    std::vector<Vector> ls = ws.make_vectors(3, 4);
+    // fem_jacobian differentiates with respect to xi, so the (single) Tet4 quadrature
+    // point enters as symbols {w, xi} whose values are set from the rule
+    // ({weight, xi0, xi1, xi2}, fem_integrators.cpp:52-58).
+    Scalar qw = ws.make_scalar();
+    Vector qxi = ws.make_vector(3);
+    const std::vector<std::array<double, 4>> rule = get_integration_rule(FEM_Element::Tet4);
+    if (rule.size() != 1) return 3;
+    Scalar Ee = element_energy(xs, Xs, logmu, [&](auto&& summand) { return summand(qw, qxi); });
+    Vector ge = gradient(Ee, collect_scalars(xs));
+    std::vector<Scalar> lflat = collect_scalars(ls);
+    Scalar lg = ws.get_zero();
+    for (int i = 0; i < 12; i++) lg += lflat[i] * ge[i];
+    Scalar dp = diff(lg, logmu);
+    Compiled<double> cmix({ dp }, "ref_adjoint_mixed_partial", symx::get_codegen_dir());
+    std::vector<double> dLdp(ne);
+    for (int e = 0; e < ne; e++) {
+        for (int a = 0; a < 4; a++) {
+            const int v = tets[4 * (size_t)e + a];
+            for (int c = 0; c < 3; c++) {
+                cmix.set(xs[a][c], d.x[v][c]);
+                cmix.set(Xs[a][c], d.X[v][c]);
+                cmix.set(ls[a][c], lam[3 * v + c]);
+            }
+        }
+        cmix.set(logmu, d.log_mu[e]);
+        cmix.set(qw, rule[0][0]);
+        for (int i = 0; i < 3; i++) cmix.set(qxi[i], rule[0][1 + i]);
+        View<double> r = cmix.run();
+        dLdp[e] = -r[0];
+    }
+    std::ofstream o(argv[3], std::ios::binary);
+    wr(o, &ok, 1);
+    wr(o, &E, 1);
+    wr(o, dLdp.data(), dLdp.size());
+    wr(o, lam.data(), (size_t)ndofs);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) return 2;
     const std::string mode = argv[1];
     if (mode == "element") return run_element(argc, argv);
     if (mode == "solve") return run_solve(argc, argv);
+    if (mode == "adjoint") return run_adjoint(argc, argv);
     return 2;
 }

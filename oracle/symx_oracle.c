@@ -19,12 +19,16 @@
  *   oracle_pcg_bsr        symx/extern/BlockedSparseMatrix/.../solve_pcg.h:83-232
  *   oracle_newton_solve   symx/src/solver/NewtonsMethod.cpp:28-252 (ProjectedNewton / Newton),
  *                         :388-457 (BDPCG + forcing), :459-641 (line search)
+ *   oracle_adjoint_logmu  no SymX counterpart: unprojected assembly + PCG above, and the
+ *                         closed-form mixed partial of an energy linear in mu_e
  *
  * Parity pins (tests/): oracle_nh_tet4 against central differences of an
  * independent numpy energy and against the reference's own Compiled<double>
  * (oracle/_ref); oracle_project against numpy.linalg.eigh; PCG against
  * scipy.sparse.linalg.spsolve and textbook PCG; Newton against the reference
- * NewtonsMethod run through oracle/_ref on the same inputs.
+ * NewtonsMethod run through oracle/_ref on the same inputs; the adjoint against
+ * central differences of the loss through re-solved equilibria, a dense numpy solve,
+ * and the reference's own pieces composed in oracle/_ref (ref_driver adjoint).
  */
 #include <math.h>
 #include <stdint.h>
@@ -730,6 +734,54 @@ ORACLE_API double oracle_eval_project_assemble(oracle_scene* S, int project, dou
     free(He);
     free(vals);
     return E;
+}
+
+/* Adjoint sensitivity of a quasistatic equilibrium x with respect to per-element
+ * parameters p_e = log(mu_e), where lambda_e = lambda_over_mu * mu_e (SymX has no
+ * adjoint; this composes its pieces: the unprojected Hessian assembled as in
+ * ElementHessians.cpp:224-256 and the PCG of solve_pcg.h:83-232).
+ *   H lam = dL_dx                 (H unprojected, PCG to rel_tol, no abs_tol)
+ *   dL/dp_e = -lam_e^T d^2E_e/dx dp_e
+ * Both Lame parameters scale with mu_e, so E_e is linear in mu_e and
+ * d^2E_e/dx dp_e = mu_e d(g_e)/d(mu_e) = g_e, the element gradient itself. */
+ORACLE_API int oracle_adjoint_logmu(const oracle_scene* S, const double* dL_dx, double rel_tol, int max_iter, double* lam_out, double* dL_dp,
+                                    oracle_pcg_info* info)
+{
+    if (!S->mu_e) return 1;
+    pattern P;
+    build_pattern(S, &P);
+    const int ne = S->ne;
+    double* He = (double*)malloc(sizeof(double) * 144 * (int64_t)(ne > 0 ? ne : 1));
+    double* ge = (double*)malloc(sizeof(double) * 12 * (int64_t)(ne > 0 ? ne : 1));
+    double* vals = (double*)malloc(sizeof(double) * 9 * P.nnzb);
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int e = 0; e < ne; e++) {
+        double xe[12], Xe[12], Ee = 0.0;
+        for (int a = 0; a < 4; a++)
+            for (int d = 0; d < 3; d++) {
+                const int64_t v = S->tets[4 * (int64_t)e + a];
+                xe[3 * a + d] = S->x[3 * v + d];
+                Xe[3 * a + d] = S->X[3 * v + d];
+            }
+        bad |= oracle_nh_tet4(xe, Xe, S->lambda_over_mu * S->mu_e[e], S->mu_e[e], &Ee, ge + 12 * (int64_t)e, He + 144 * (int64_t)e);
+    }
+    int rc = bad ? 2 : 0;
+    if (!rc) {
+        assemble(S, &P, He, vals);
+        oracle_pcg_bsr(S->nv, P.rp, P.cols, vals, dL_dx, lam_out, 0.0, rel_tol, max_iter, 1, info);
+        for (int e = 0; e < ne; e++) {
+            double acc = 0.0;
+            for (int a = 0; a < 4; a++)
+                for (int d = 0; d < 3; d++) acc += lam_out[3 * (int64_t)S->tets[4 * (int64_t)e + a] + d] * ge[12 * (int64_t)e + 3 * a + d];
+            dL_dp[e] = -acc;
+        }
+    }
+    free_pattern(&P);
+    free(He);
+    free(ge);
+    free(vals);
+    return rc;
 }
 
 ORACLE_API int oracle_num_threads(void)

@@ -59,6 +59,11 @@ class SolveStats(Structure):
         return d
 
 
+class AdjointStats(Structure):
+    _fields_ = [("cg_iterations", c_int32), ("n_refinements", c_int32), ("converged", c_int32), ("found_indefiniteness", c_int32),
+                ("cg_error", c_double), ("energy", c_double), ("seconds", c_double)]
+
+
 _hip = None
 _amd = None
 _DP = POINTER(c_double)
@@ -215,6 +220,18 @@ class Tet4Block:
         check_scene(amd().symx_scene_step(self.ptr, ctypes.byref(st)))
         return st.as_dict()
 
+    def adjoint(self, dL_dx: np.ndarray, rel_tol: float = 1e-10, max_iter: int = 100000):
+        """dL/dlog_mu (one value per tet) at the current positions, an equilibrium of the
+        quasistatic energy: H lambda = dL_dx on the unprojected Hessian, then
+        dL/dlog_mu_e = -lambda^T d^2E/dx dlog_mu_e. Returns (gradient, stats)."""
+        rhs = np.ascontiguousarray(dL_dx, dtype=np.float64).reshape(-1)
+        if rhs.size != 3 * self.n_vertices:
+            raise ValueError("dL_dx must have one entry per DoF")
+        out = np.zeros(self.n_elements)
+        st = AdjointStats()
+        check_scene(amd().symx_scene_adjoint(self.ptr, _ptr(rhs), _ptr(out), c_double(rel_tol), c_int32(max_iter), ctypes.byref(st)))
+        return out, {k: getattr(st, k) for k, _ in st._fields_}
+
     def source(self, i: int = 0) -> str:
         n = c_int64()
         amd().symx_scene_source(self.ptr, c_int32(i), None, c_int64(0), ctypes.byref(n))
@@ -248,6 +265,19 @@ class Tet4Block:
     def upload(self, vec: int, a: np.ndarray):
         a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
         check_hip(hip().symx_hip_problem_upload(self._p(), c_int32(vec), _ptr(a)))
+
+    def get_dofs(self) -> np.ndarray:
+        a = np.zeros(3 * self.n_vertices)
+        check_hip(hip().symx_hip_problem_get_dofs(self._p(), _ptr(a)))
+        return a
+
+    def axpy(self, dst: int, a: int, s: float, b: int):
+        """Device vectors: dst = a + s * b."""
+        check_hip(hip().symx_hip_problem_axpy(self._p(), c_int32(dst), c_int32(a), c_double(s), c_int32(b)))
+
+    def scale(self, dst: int, a: int, s: float):
+        """Device vectors: dst = s * a."""
+        check_hip(hip().symx_hip_problem_scale(self._p(), c_int32(dst), c_int32(a), c_double(s)))
 
     def element_outputs(self, pot: int, n_elem: int, nb: int):
         G = np.zeros((n_elem, 3 * nb))
@@ -303,6 +333,24 @@ def host_jit_tet4(cache_dir: str, material: int = 0):
     ops = c_int64()
     check_scene(amd().symx_host_jit_tet4(cache_dir.encode(), c_int32(material), ctypes.byref(cs), ctypes.byref(ops)))
     return cs.value, ops.value
+
+
+def host_scene_jit(params: Tet4BlockParams, cache_dir: str):
+    """Generate and hiprtc-compile every potential module of a scene (no GPU)."""
+    n = c_int32()
+    cs = c_int64()
+    ops = c_int64()
+    check_scene(amd().symx_host_scene_jit(ctypes.byref(params), cache_dir.encode(), ctypes.byref(n), ctypes.byref(cs), ctypes.byref(ops)))
+    return n.value, cs.value, ops.value
+
+
+def host_scene_adjoint_element(params: Tet4BlockParams, log_mu: np.ndarray, e: int, lam: np.ndarray) -> np.ndarray:
+    """Generated adjoint outputs of element e evaluated on the host expression graph."""
+    log_mu = np.ascontiguousarray(log_mu, np.float64)
+    lam = np.ascontiguousarray(lam, np.float64).reshape(-1)
+    out = np.zeros(1)
+    check_scene(amd().symx_host_scene_adjoint_element(ctypes.byref(params), _ptr(log_mu), c_int32(e), _ptr(lam), _ptr(out)))
+    return out
 
 
 def project_dense(ctx: Context, mats_dev_ptr: int, n_mats: int, n: int, eps: float, mirror: bool, changed_dev_ptr: int | None = None):

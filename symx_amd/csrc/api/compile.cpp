@@ -187,7 +187,7 @@ namespace symx
         }
 
         void emit_gather(std::ostringstream& os, const Graph& g, const std::vector<DataMap>& maps, const SymbolUse& use,
-                         const SummationMap& sum)
+                         const SummationMap& sum, const std::vector<std::array<int32_t, 3>>& adjoint_symbols)
         {
             std::vector<uint8_t> covered(g.n_symbols(), 0);
             std::vector<uint8_t> col_loaded(64, 0);
@@ -197,6 +197,11 @@ namespace symx
                 for (int k = 0; k < dm.stride; k++) {
                     covered[dm.first_symbol_idx + k] = 1;
                     any = any || use.used[dm.first_symbol_idx + k];
+                }
+                for (const auto& a : adjoint_symbols) {
+                    if (a[1] != (int32_t)m) continue;
+                    covered[a[0]] = 1;
+                    any = any || use.used[a[0]];
                 }
                 if (!any) continue;
                 std::string base;
@@ -212,6 +217,11 @@ namespace symx
                     const int s = dm.first_symbol_idx + k;
                     if (!use.used[s]) continue;
                     os << "        const double s" << s << " = p.maps[" << m << "][" << base << k << "];\n";
+                }
+                for (const auto& a : adjoint_symbols) {
+                    if (a[1] != (int32_t)m || !use.used[a[0]]) continue;
+                    if (a[2] < 0 || a[2] >= dm.stride) throw std::runtime_error("symx: adjoint symbol component out of range");
+                    os << "        const double s" << a[0] << " = a.lambda[" << m << "][" << base << a[2] << "];\n";
                 }
             }
             if (sum.n_iterations > 0)
@@ -243,12 +253,15 @@ namespace symx
             const int nb = n / 3;
             const int n_ub = nb * (nb + 1) / 2;
 
-            os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.name << "(const symx_hip_eval_params p)\n{\n";
+            const bool adj = ks.kind == KernelKind::PGP;
+            os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.name << "(const symx_hip_eval_params p"
+               << (adj ? ", const symx_hip_adjoint_params a" : "") << ")\n{\n";
             os << "    const int e = symx_block() * 256 + threadIdx.x;\n";
-            if (ks.kind != KernelKind::C) os << "    double e_out = 0.0;\n";
+            if (ks.kind != KernelKind::C && !adj) os << "    double e_out = 0.0;\n";
             os << "    if (e < p.n_elem) {\n";
             if (ks.kind != KernelKind::C) os << "        const bool act = (p.active == nullptr) || (p.active[e] != 0);\n";
-            emit_gather(os, g, maps, use, sum);
+            emit_gather(os, g, maps, use, sum, ks.adjoint_symbols);
+            const std::string ea = "(long long)e * " + std::to_string(ks.outputs.size());
             const std::string eg = "(long long)e * " + std::to_string(n);
             const std::string eh = "(long long)e * " + std::to_string(n_ub * 9);
             // zero outputs (inactive elements, and accumulation start for multi-point sums)
@@ -257,6 +270,8 @@ namespace symx
                     for (int i = 0; i < n; i++) os << indent << "p.G[" << eg << " + " << i << "] = 0.0;\n";
                 if (ks.kind == KernelKind::PGH)
                     for (int i = 0; i < n_ub * 9; i++) os << indent << "p.H[" << eh << " + " << i << "] = 0.0;\n";
+                if (adj)
+                    for (size_t j = 0; j < ks.outputs.size(); j++) os << indent << "a.out[" << ea << " + " << j << "] = 0.0;\n";
             };
             if (ks.kind == KernelKind::C) {
                 if (use.uses_summation) throw std::runtime_error("symx: activity conditions may not depend on summation symbols");
@@ -287,6 +302,15 @@ namespace symx
             }
             const std::string body_in = (use.uses_summation && multi_q) ? "                " : in;
             emit_body(os, g, mark, body_in.c_str());
+            if (adj) {
+                for (size_t j = 0; j < ks.outputs.size(); j++)
+                    os << body_in << "a.out[" << ea << " + " << j << "] " << acc << " " << ref(g, ks.outputs[j].id) << ";\n";
+                if (use.uses_summation && multi_q) os << in << "}\n";
+                os << "        }\n        else {\n";
+                emit_zero("            ");
+                os << "        }\n    }\n}\n";
+                continue;
+            }
             os << body_in << "e_out += " << ref(g, ks.outputs[0].id) << ";\n";
             if (ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH) {
                 for (int i = 0; i < n; i++) os << body_in << "p.G[" << eg << " + " << i << "] " << acc << " " << ref(g, ks.outputs[1 + i].id) << ";\n";

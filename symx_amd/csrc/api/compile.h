@@ -233,8 +233,11 @@ namespace symx
     {
         std::string name;
         KernelKind kind = KernelKind::P;
-        std::vector<Scalar> outputs;   // P: [E]; PG: [E, g]; PGH: [E, g, H row-major n x n]; C: [cond]
+        std::vector<Scalar> outputs;   // P: [E]; PG: [E, g]; PGH: [E, g, H row-major n x n]; C: [cond]; PGP: [d(lambda.g)/dp_j]
         int32_t n_dofs = 0;            // DoFs of the element (multiple of 3 for PG/PGH)
+        // PGP only: adjoint symbols {symbol id, data map m, component k}. Symbol `id`
+        // reads the adjoint vector where DoF map m reads its component k.
+        std::vector<std::array<int32_t, 3>> adjoint_symbols;
     };
 
     // Emit one hiprtc translation unit holding all kernels of a potential. The

@@ -133,6 +133,23 @@ namespace symx
     {
         dof_maps.push_back(DofSet{ id, data, flat_size, name });
     }
+    void GlobalPotential::add_parameter(std::function<std::uintptr_t()> id, std::function<double*()> data, std::function<int32_t()> flat_size,
+                                        const std::string& name)
+    {
+        param_maps.push_back(DofSet{ id, data, flat_size, name });
+    }
+    const std::vector<int32_t>& GlobalPotential::get_parameter_offsets()
+    {
+        param_offsets.assign(param_maps.size() + 1, 0);
+        for (size_t i = 0; i < param_maps.size(); i++) param_offsets[i + 1] = param_offsets[i] + param_maps[i].flat_size();
+        return param_offsets;
+    }
+    int32_t GlobalPotential::get_total_n_parameters() const
+    {
+        int32_t n = 0;
+        for (auto& d : param_maps) n += d.flat_size();
+        return n;
+    }
     const std::vector<int32_t>& GlobalPotential::get_dofs_offsets()
     {
         dof_offsets.assign(dof_maps.size() + 1, 0);
@@ -174,6 +191,92 @@ namespace symx
         }
     }
 
+    // ---- per-potential code generation (SecondOrderCompiledPotential.cpp:8-70) ----------
+    PotentialModule generate_potential_module(GlobalPotential& G, const Potential& P)
+    {
+        PotentialModule pm;
+        MappedWorkspace<double>& mws = *P.mws;
+        Graph* g = mws.workspace().graph();
+        const auto& maps = mws.get_maps();
+        // DoF identification: maps whose container is a DoF set, in set order then
+        // map order (SecondOrderCompiledPotential.cpp:10-33).
+        std::vector<Scalar> dofs;
+        pm.map_dof_set.assign(maps.size(), -1);
+        std::vector<std::array<int32_t, 2>> dof_maps;  // {map, component}
+        for (int32_t s = 0; s < G.get_n_dof_sets(); s++) {
+            const std::uintptr_t id = G.get_dof_sets()[s].id();
+            for (int32_t m : mws.maps_of_container(id)) {
+                const DataMap& dm = maps[m];
+                if (dm.stride != 3)
+                    throw std::runtime_error("symx: expected DoF stride of 3, got " + std::to_string(dm.stride) + " in potential " + P.name);
+                if (dm.connectivity_index < 0) throw std::runtime_error("symx: DoF data must be indexed by the connectivity (" + P.name + ")");
+                for (int k = 0; k < 3; k++) {
+                    dofs.push_back(Scalar(g, g->symbol_node(dm.first_symbol_idx + k)));
+                    dof_maps.push_back({ m, k });
+                }
+                pm.dof_col.push_back(dm.connectivity_index);
+                pm.dof_block_set.push_back(s);
+                pm.map_dof_set[m] = s;
+            }
+        }
+        if (dofs.empty()) throw std::runtime_error("symx: potential with zero DoFs: " + P.name);
+        const int32_t n = (int32_t)dofs.size();
+
+        DiffCache dc;
+        const Vector grad = gradient(P.expression, dofs, dc);
+        const Matrix hess = gradient(grad, dofs, true, dc);
+        const std::string base = identifier(P.name);
+        std::vector<KernelSpec>& specs = pm.specs;
+        specs.resize(3);
+        specs[0] = KernelSpec{ base + "_P", KernelKind::P, { P.expression }, n };
+        specs[1] = KernelSpec{ base + "_P__dP_du", KernelKind::PG, collect_scalars({ { P.expression }, grad.values() }), n };
+        specs[2] = KernelSpec{ base + "_P__dP_du__d2P_du2", KernelKind::PGH, collect_scalars({ { P.expression }, grad.values(), hess.values() }), n };
+        if (P.has_condition) specs.push_back(KernelSpec{ base + "_C", KernelKind::C, { P.condition }, n });
+
+        DeviceGlobal::PotentialInfo& info = pm.info;
+        info.name = P.name;
+        info.n_dofs = n;
+
+        // Adjoint kernel: the mixed partials lambda^T d^2E/dx dp, generated as
+        // d(lambda . g)/dp with one adjoint symbol per DoF symbol, read where that DoF is read.
+        std::vector<Scalar> params;
+        for (int32_t s = 0; s < (int32_t)G.get_parameter_sets().size(); s++) {
+            const std::uintptr_t id = G.get_parameter_sets()[s].id();
+            for (int32_t m : mws.maps_of_container(id)) {
+                const DataMap& dm = maps[m];
+                for (int k = 0; k < dm.stride; k++) {
+                    params.push_back(Scalar(g, g->symbol_node(dm.first_symbol_idx + k)));
+                    info.param_outs.push_back(DeviceGlobal::ParamOut{ s, dm.connectivity_index, dm.stride, k });
+                }
+            }
+        }
+        if (!params.empty()) {
+            KernelSpec ks;
+            ks.name = base + "_adjoint";
+            ks.kind = KernelKind::PGP;
+            ks.n_dofs = n;
+            // The adjoint symbols are created once per potential, so that regenerating the
+            // module gives the same source (and code-object cache key).
+            while ((int32_t)P.adjoint_symbol_nodes.size() < n) P.adjoint_symbol_nodes.push_back(g->symbol());
+            Scalar lg = Scalar(g, g->constant(0.0));
+            for (int32_t i = 0; i < n; i++) {
+                const int32_t lid = P.adjoint_symbol_nodes[i];
+                ks.adjoint_symbols.push_back({ g->node(lid).a, dof_maps[i][0], dof_maps[i][1] });
+                lg = lg + Scalar(g, lid) * grad[i];
+            }
+            ks.outputs = gradient(lg, params, dc).values();
+            info.adjoint_kernel = ks.name;
+            info.ops_adjoint = count_operations(*g, ks.outputs);
+            specs.push_back(ks);
+        }
+        pm.source = emit_hip_module(base, *g, maps, mws.get_summation(), specs, symx_hip_eval_prelude());
+        info.ops_P = count_operations(*g, specs[0].outputs);
+        info.ops_PG = count_operations(*g, specs[1].outputs);
+        info.ops_PGH = count_operations(*g, specs[2].outputs);
+        info.source = pm.source;
+        return pm;
+    }
+
     // ---- DeviceGlobal (SecondOrderCompiledGlobal + SecondOrderCompiledPotential) ---------
     DeviceGlobal::DeviceGlobal(spGlobalPotential G_, spContext ctx_) : G(G_), ctx(ctx_)
     {
@@ -189,47 +292,16 @@ namespace symx
         for (const auto& pp : G->get_potentials()) {
             const Potential& P = *pp;
             MappedWorkspace<double>& mws = *P.mws;
-            Graph* g = mws.workspace().graph();
             const auto& maps = mws.get_maps();
-            // DoF identification: maps whose container is a DoF set, in set order then
-            // map order (SecondOrderCompiledPotential.cpp:10-33).
-            std::vector<Scalar> dofs;
-            std::vector<int32_t> dof_col, dof_off;
-            std::vector<int32_t> map_dof_set(maps.size(), -1);
-            for (int32_t s = 0; s < G->get_n_dof_sets(); s++) {
-                const std::uintptr_t id = G->get_dof_sets()[s].id();
-                for (int32_t m : mws.maps_of_container(id)) {
-                    const DataMap& dm = maps[m];
-                    if (dm.stride != 3)
-                        throw std::runtime_error("symx: expected DoF stride of 3, got " + std::to_string(dm.stride) + " in potential " + P.name);
-                    if (dm.connectivity_index < 0) throw std::runtime_error("symx: DoF data must be indexed by the connectivity (" + P.name + ")");
-                    for (int k = 0; k < 3; k++) dofs.push_back(Scalar(g, g->symbol_node(dm.first_symbol_idx + k)));
-                    dof_col.push_back(dm.connectivity_index);
-                    dof_off.push_back(offsets[s]);
-                    map_dof_set[m] = s;
-                }
-            }
-            if (dofs.empty()) throw std::runtime_error("symx: potential with zero DoFs: " + P.name);
-            const int32_t n = (int32_t)dofs.size();
-
-            DiffCache dc;
-            const Vector grad = gradient(P.expression, dofs, dc);
-            const Matrix hess = gradient(grad, dofs, true, dc);
-            const std::string base = identifier(P.name);
-            std::vector<KernelSpec> specs(3);
-            specs[0] = KernelSpec{ base + "_P", KernelKind::P, { P.expression }, n };
-            specs[1] = KernelSpec{ base + "_P__dP_du", KernelKind::PG, collect_scalars({ { P.expression }, grad.values() }), n };
-            specs[2] = KernelSpec{ base + "_P__dP_du__d2P_du2", KernelKind::PGH, collect_scalars({ { P.expression }, grad.values(), hess.values() }), n };
-            if (P.has_condition) specs.push_back(KernelSpec{ base + "_C", KernelKind::C, { P.condition }, n });
-            const std::string src = emit_hip_module(base, *g, maps, mws.get_summation(), specs, symx_hip_eval_prelude());
-
-            PotentialInfo info;
-            info.name = P.name;
-            info.n_dofs = n;
-            info.ops_P = count_operations(*g, specs[0].outputs);
-            info.ops_PG = count_operations(*g, specs[1].outputs);
-            info.ops_PGH = count_operations(*g, specs[2].outputs);
-            info.source = src;
+            PotentialModule pm = generate_potential_module(*G, P);
+            PotentialInfo& info = pm.info;
+            const std::vector<KernelSpec>& specs = pm.specs;
+            const std::string& src = pm.source;
+            const std::vector<int32_t>& map_dof_set = pm.map_dof_set;
+            const std::vector<int32_t>& dof_col = pm.dof_col;
+            std::vector<int32_t> dof_off;
+            for (int32_t s : pm.dof_block_set) dof_off.push_back(offsets[s]);
+            const int32_t n = info.n_dofs;
 
             symx_hip_module mod = nullptr;
             char key[80];
@@ -278,6 +350,8 @@ namespace symx
             infos.push_back(info);
             pot_index++;
         }
+        param_offsets = G->get_parameter_offsets();
+        nparams = param_offsets.back();
         check(symx_hip_problem_finalize(prob), hip, "problem_finalize");
         compile_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -291,6 +365,46 @@ namespace symx
     void DeviceGlobal::refresh_data()
     {
         for (auto& hm : host_maps) check(symx_hip_problem_set_map_data(prob, hm.pot, hm.map, hm.data(), hm.n), ctx->hip(), "refreshing data");
+    }
+
+    DeviceGlobal::AdjointStats DeviceGlobal::adjoint(const double* dL_dx, double* dL_dp, Precision precision, double rel_tol, int32_t max_iter)
+    {
+        symx_hip_ctx hip = ctx->hip();
+        const auto t0 = std::chrono::steady_clock::now();
+        AdjointStats st;
+        check(symx_hip_problem_eval(prob, SYMX_HIP_EVAL_PGH, &st.energy), hip, "adjoint: Hessian evaluation");
+        symx_hip_pcg_info info{};
+        const int32_t prec = precision == Precision::Mixed ? SYMX_HIP_MIXED : SYMX_HIP_FP64;
+        check(symx_hip_problem_adjoint_solve(prob, prec, rel_tol, max_iter, dL_dx, nullptr, &info), hip, "adjoint: linear solve");
+        st.cg_iterations = info.n_iterations;
+        st.n_refinements = info.n_refinements;
+        st.converged = info.converged != 0;
+        st.found_indefiniteness = info.found_indefiniteness != 0;
+        st.cg_error = info.error;
+        std::fill(dL_dp, dL_dp + nparams, 0.0);
+        const auto& pots = G->get_potentials();
+        std::vector<double> out;
+        for (int32_t pi = 0; pi < (int32_t)infos.size(); pi++) {
+            const PotentialInfo& inf = infos[pi];
+            if (inf.adjoint_kernel.empty() || inf.n_elem == 0) continue;
+            const int32_t n_out = (int32_t)inf.param_outs.size();
+            out.assign((size_t)inf.n_elem * n_out, 0.0);
+            check(symx_hip_problem_launch_adjoint(prob, pi, inf.adjoint_kernel.c_str(), n_out, out.data()), hip, "adjoint kernel");
+            const ConnectivityMap& conn = pots[pi]->mws->conn;
+            const int32_t* c = conn.data();
+            for (int32_t e = 0; e < inf.n_elem; e++) {
+                for (int32_t j = 0; j < n_out; j++) {
+                    const ParamOut& po = inf.param_outs[j];
+                    const int64_t item = po.conn_col < 0 ? 0 : c[(int64_t)e * conn.stride + po.conn_col];
+                    const int64_t idx = param_offsets[po.set] + item * po.stride + po.k;
+                    if (idx < param_offsets[po.set] || idx >= param_offsets[po.set + 1])
+                        throw std::runtime_error("symx: adjoint parameter index out of range in potential " + inf.name);
+                    dL_dp[idx] -= out[(size_t)e * n_out + j];
+                }
+            }
+        }
+        st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return st;
     }
 
     bool DeviceGlobal::matches_topology() const
@@ -327,6 +441,24 @@ namespace symx
         double r = 0.0;
         check(symx_hip_problem_max_abs(dev->problem(), SYMX_HIP_VEC_GRAD, &r), context->hip(), "max_abs");
         return r;
+    }
+
+    DeviceGlobal::AdjointStats NewtonsMethod::adjoint(const std::vector<double>& dL_dx, std::vector<double>& dL_dp, double rel_tol, int32_t max_iter)
+    {
+        const int ndofs = G->get_total_n_dofs();
+        if ((int)dL_dx.size() != ndofs) throw std::runtime_error("symx: adjoint right-hand side must have one entry per DoF");
+        if (!dev->matches_topology()) dev = std::make_unique<DeviceGlobal>(G, context);
+        if (dev->n_parameters() == 0) throw std::runtime_error("symx: adjoint needs parameters (GlobalPotential::add_parameter)");
+        symx_hip_ctx hip = context->hip();
+        dev->refresh_data();
+        host_buf.resize(ndofs);
+        G->get_dofs(host_buf.data());
+        check(symx_hip_problem_set_dofs(dev->problem(), host_buf.data()), hip, "set_dofs");
+        dL_dp.assign(dev->n_parameters(), 0.0);
+        // Always FP64: with FP32 matrix storage the refined solution is that of the rounded
+        // matrix, whose error (about 1e-4 relative on penalty-stiff systems) is far above
+        // what a gradient check needs, and the FP32 solve takes more iterations.
+        return dev->adjoint(dL_dx.data(), dL_dp.data(), Precision::FP64, rel_tol, max_iter);
     }
 
     SolverReturn NewtonsMethod::solve()

@@ -170,6 +170,8 @@ namespace symx
         Scalar expression;
         Scalar condition;
         bool has_condition = false;
+        // Graph nodes of the adjoint symbols (one per DoF symbol), created on first use.
+        mutable std::vector<int32_t> adjoint_symbol_nodes;
     };
 
     class GlobalPotential
@@ -221,6 +223,21 @@ namespace symx
                     [&arr]() { return (int32_t)(arr.size() * stride); }, name);
         }
 
+        // Parameters of an adjoint sensitivity analysis (no counterpart in SymX): data
+        // containers, identified like DoF sets by their address, whose symbols become the
+        // p of d^2E/dx dp. Potentials bind them with the ordinary make_scalar/make_vector.
+        void add_parameter(std::function<std::uintptr_t()> id, std::function<double*()> data, std::function<int32_t()> flat_size,
+                           const std::string& name = "");
+        template <typename DYNAMIC_VECTOR>
+        void add_parameter(DYNAMIC_VECTOR& arr, const std::string& name = "")
+        {
+            constexpr int32_t stride = (int32_t)(sizeof(typename DYNAMIC_VECTOR::value_type) / sizeof(double));
+            add_parameter([&arr]() { return reinterpret_cast<std::uintptr_t>(&arr); }, [&arr]() { return reinterpret_cast<double*>(arr.data()); },
+                          [&arr]() { return (int32_t)(arr.size() * stride); }, name);
+        }
+        const std::vector<int32_t>& get_parameter_offsets();
+        int32_t get_total_n_parameters() const;
+
         const std::vector<int32_t>& get_dofs_offsets();
         int32_t get_n_dof_sets() const { return (int32_t)dof_maps.size(); }
         int32_t get_n_dofs(int32_t set_i) const { return dof_maps[set_i].flat_size(); }
@@ -239,11 +256,14 @@ namespace symx
             std::string name;
         };
         const std::vector<DofSet>& get_dof_sets() const { return dof_maps; }
+        const std::vector<DofSet>& get_parameter_sets() const { return param_maps; }
 
     private:
         std::vector<std::unique_ptr<Potential>> potentials;
         std::vector<DofSet> dof_maps;
         std::vector<int32_t> dof_offsets;
+        std::vector<DofSet> param_maps;
+        std::vector<int32_t> param_offsets;
     };
     using spGlobalPotential = std::shared_ptr<GlobalPotential>;
 
@@ -260,14 +280,40 @@ namespace symx
         // Whether the potentials' element counts / DoF counts still match the device problem.
         bool matches_topology() const;
         int32_t n_dofs() const { return ndofs; }
+        // Where output j of a potential's adjoint kernel lands in dL/dp.
+        struct ParamOut
+        {
+            int32_t set = 0;       // parameter set
+            int32_t conn_col = -1; // connectivity column of the data map (-1: one value for all elements)
+            int32_t stride = 1;    // values per item of the data map
+            int32_t k = 0;         // component within the item
+        };
         struct PotentialInfo
         {
             std::string name;
             int32_t n_elem = 0;
             int32_t n_dofs = 0;
-            int64_t ops_P = 0, ops_PG = 0, ops_PGH = 0;
+            int64_t ops_P = 0, ops_PG = 0, ops_PGH = 0, ops_adjoint = 0;
             std::string source;
+            std::string adjoint_kernel;          // empty: the potential depends on no parameter
+            std::vector<ParamOut> param_outs;
         };
+        struct AdjointStats
+        {
+            int cg_iterations = 0;
+            int n_refinements = 0;
+            bool converged = false;
+            bool found_indefiniteness = false;
+            double cg_error = 0.0;
+            double energy = 0.0;
+            double seconds = 0.0;
+        };
+        // Adjoint sensitivity at the DoFs currently on the device (an equilibrium):
+        // H lambda = dL_dx with H the unprojected Hessian, then
+        // dL_dp = -lambda^T d^2E/dx dp over all registered parameter sets (in their
+        // order, flattened). The explicit dL/dp of the loss is not included.
+        AdjointStats adjoint(const double* dL_dx, double* dL_dp, Precision precision, double rel_tol, int32_t max_iter);
+        int32_t n_parameters() const { return nparams; }
         const std::vector<PotentialInfo>& info() const { return infos; }
         double compile_seconds = 0.0;
         int n_cached = 0;
@@ -279,9 +325,24 @@ namespace symx
         std::vector<symx_hip_module> modules;
         std::vector<PotentialInfo> infos;
         int32_t ndofs = 0;
+        int32_t nparams = 0;
+        std::vector<int32_t> param_offsets;
         struct HostMap { int32_t pot, map; std::function<const double*()> data; int64_t n; };
         std::vector<HostMap> host_maps;
     };
+
+    // Generated kernels of one potential (P, PG, PGH, optional C and adjoint) and the
+    // DoF layout they assume. Needs no device.
+    struct PotentialModule
+    {
+        std::vector<KernelSpec> specs;
+        std::string source;
+        DeviceGlobal::PotentialInfo info;
+        std::vector<int32_t> dof_col;        // connectivity column per DoF block
+        std::vector<int32_t> dof_block_set;  // DoF set per DoF block
+        std::vector<int32_t> map_dof_set;    // DoF set per data map (-1: not a DoF)
+    };
+    PotentialModule generate_potential_module(GlobalPotential& G, const Potential& P);
 
     class NewtonsMethod
     {
@@ -312,6 +373,11 @@ namespace symx
             return std::make_shared<NewtonsMethod>(G, context, callbacks);
         }
         SolverReturn solve();
+        // Adjoint gradient dL/dp at the current DoFs of the user arrays, normally the
+        // equilibrium that solve() returned (see DeviceGlobal::adjoint). Solved in FP64
+        // whatever settings.precision is.
+        DeviceGlobal::AdjointStats adjoint(const std::vector<double>& dL_dx, std::vector<double>& dL_dp, double rel_tol = 1e-10,
+                                           int32_t max_iter = 100000);
         const SolveStats& get_last_solve_stats() const { return stats; }
         void print_summary(double total_time = -1.0) const;
         DeviceGlobal& device() { return *dev; }

@@ -19,6 +19,10 @@
 
 using namespace symx_hip;
 
+// The device prelude restates these structs; both sides must agree byte for byte.
+static_assert(sizeof(symx_hip_eval_params) == 440, "symx_hip_eval_params layout");
+static_assert(sizeof(symx_hip_adjoint_params) == 400, "symx_hip_adjoint_params layout");
+
 namespace
 {
     thread_local std::string g_thread_error;
@@ -82,6 +86,13 @@ struct symx_hip_eval_params {
     symx_i32 pad_;
 };
 static_assert(sizeof(symx_hip_eval_params) == 440, "symx_hip_eval_params layout");
+struct symx_hip_adjoint_params {
+    const double* lambda[SYMX_HIP_MAX_MAPS];
+    double* out;
+    symx_i32 n_out;
+    symx_i32 pad_;
+};
+static_assert(sizeof(symx_hip_adjoint_params) == 400, "symx_hip_adjoint_params layout");
 __device__ __forceinline__ double symx_ln(double x) { return (x <= 0.0) ? -__builtin_inf() : log(x); }
 __device__ __forceinline__ double symx_log10(double x) { return (x <= 0.0) ? -__builtin_inf() : log10(x); }
 __device__ __forceinline__ double symx_powi(double x, int n)
@@ -942,7 +953,11 @@ int symx_hip_problem_solve(symx_hip_problem p, int32_t precision, double abs_tol
             int total = 0;
             if (err < abs_tol) res.converged = 1;
             const double target = std::max(abs_tol, rel_tol * err0);
-            for (int outer = 0; !res.converged && outer < 16 && total < max_iter; outer++) {
+            // Each refinement shrinks the FP64 residual by what the FP32 inner solve really
+            // achieved (about 0.2 on stiff penalty systems), so allow many refinements and
+            // stop when two in a row gain less than 10%.
+            int stalled = 0;
+            for (int outer = 0; !res.converged && outer < 64 && total < max_iter && stalled < 2; outer++) {
                 const double inner_rel = std::min(0.5, std::max(0.5 * target / err, 2e-6));
                 PcgState st;
                 rc = run_pcg(p, true, nullptr, nullptr, 0.0, inner_rel, max_iter - total, stop_on_indef, st);
@@ -959,7 +974,9 @@ int symx_hip_problem_solve(symx_hip_problem p, int32_t precision, double abs_tol
                 double rr = 0.0;
                 rc = sync_scalar(ctx, rr_dev, &rr);
                 if (rc) return rc;
-                err = std::sqrt(rr / bb);
+                const double err_new = std::sqrt(rr / bb);
+                stalled = err_new > 0.9 * err ? stalled + 1 : 0;
+                err = err_new;
                 if (err < abs_tol || err / err0 < rel_tol) res.converged = 1;
             }
             res.n_iterations = total;
@@ -1006,6 +1023,60 @@ int symx_hip_problem_scale(symx_hip_problem p, int32_t dst, int32_t a, double s)
     if (!dd || !da) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
     launch_scale(p->ctx->stream, p->n_dofs, dd, da, s);
     HIP_TRY(p->ctx, hipGetLastError());
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_adjoint_solve(symx_hip_problem p, int32_t precision, double rel_tol, int32_t max_iter, const double* host_rhs,
+                                   double* host_lambda, symx_hip_pcg_info* info)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!host_rhs) return fail(ctx, SYMX_HIP_ERR_ARG, "null rhs");
+    // assemble the unprojected Hessian of the last evaluation
+    int rc = symx_hip_problem_assemble(p, precision);
+    if (rc) return rc;
+    rc = symx_hip_problem_upload(p, SYMX_HIP_VEC_RHS, host_rhs);
+    if (rc) return rc;
+    rc = symx_hip_problem_solve(p, precision, 0.0, rel_tol, max_iter, 1, info);
+    if (rc) return rc;
+    if (host_lambda) return symx_hip_problem_download(p, SYMX_HIP_VEC_DU, host_lambda);
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_launch_adjoint(symx_hip_problem p, int32_t pot_id, const char* name, int32_t n_out, double* host_out)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->finalized || pot_id < 0 || pot_id >= (int32_t)p->pots.size() || !name || n_out <= 0 || !host_out)
+        return fail(ctx, SYMX_HIP_ERR_ARG, "bad adjoint launch arguments");
+    DevPotential& pot = p->pots[pot_id];
+    if (pot.n_elem == 0) return SYMX_HIP_OK;
+    hipFunction_t f;
+    int rc = module_function(pot.module, name, &f);
+    if (rc) return rc;
+    double* d_out = nullptr;
+    rc = dalloc(ctx, &d_out, (size_t)pot.n_elem * n_out);
+    if (rc) return rc;
+    struct
+    {
+        symx_hip_eval_params prm;
+        symx_hip_adjoint_params adj;
+    } args;
+    static_assert(sizeof(args) == sizeof(symx_hip_eval_params) + sizeof(symx_hip_adjoint_params), "kernel argument layout");
+    fill_params(p, pot, args.prm);
+    std::memset(&args.adj, 0, sizeof(args.adj));
+    for (size_t m = 0; m < pot.maps.size(); m++)
+        if (pot.maps[m].is_dof) args.adj.lambda[m] = p->d_vec[SYMX_HIP_VEC_DU] + pot.maps[m].dof_offset;
+    args.adj.out = d_out;
+    args.adj.n_out = n_out;
+    size_t sz = sizeof(args);
+    void* config[] = { HIP_LAUNCH_PARAM_BUFFER_POINTER, &args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END };
+    const unsigned grid = (unsigned)((pot.n_elem + SYMX_HIP_BLOCK - 1) / SYMX_HIP_BLOCK);
+    hipError_t e = hipModuleLaunchKernel(f, grid, 1, 1, SYMX_HIP_BLOCK, 1, 1, 0, ctx->stream, nullptr, config);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, d_out, sizeof(double) * pot.n_elem * n_out, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(d_out);
+    if (e != hipSuccess) return fail(ctx, SYMX_HIP_ERR_HIP, std::string("adjoint kernel: ") + hipGetErrorString(e));
     return SYMX_HIP_OK;
 }
 

@@ -118,6 +118,127 @@ static void apply_newton_params(NewtonSettings& s, const symx_newton_params& p)
     s.step_tolerance = p.step_tolerance;
 }
 
+// Mesh, data and potentials of the Tet4 block scene (no device work).
+static void build_scene(symx_scene_s* s, const symx_tet4_block_params* p, const symx_newton_params* np)
+{
+    s->prm = *p;
+    s->np = *np;
+    const FEMMesh mesh = generate_cuboid_Tet4_mesh({ p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] });
+    const int32_t nv = (int32_t)mesh.vertices.size();
+    s->X = mesh.vertices;
+    s->tets = mesh.connectivity;
+    s->x = s->X;
+    const double h_min = std::min({ p->size[0] / p->cells[0], p->size[1] / p->cells[1], p->size[2] / p->cells[2] });
+    (void)h_min;
+    SplitMix64 rng(p->seed);
+    std::vector<uint8_t> fixed(nv, 0);
+    for (int32_t i = 0; i < nv; i++) {
+        if (p->fixed_axis >= 0 && std::abs(s->X[i][p->fixed_axis]) < 1e-12) fixed[i] = 1;
+        for (int d = 0; d < 3; d++) {
+            const double u = rng.uniform();
+            if (!fixed[i]) s->x[i][d] += p->noise * (2.0 * u - 1.0);
+        }
+    }
+    s->x0 = s->x;
+    s->v.assign(nv, { 0.0, 0.0, 0.0 });
+    // lumped masses (dynamic_elasticity_with_contact.cpp:67-79)
+    s->mass.assign(nv, 0.0);
+    const int32_t ne = mesh.n_elements;
+    for (int32_t e = 0; e < ne; e++) {
+        const int32_t* t = &s->tets[4 * (size_t)e];
+        const auto& A = s->X[t[0]];
+        double d1[3], d2[3], d3[3];
+        for (int k = 0; k < 3; k++) {
+            d1[k] = s->X[t[1]][k] - A[k];
+            d2[k] = s->X[t[2]][k] - A[k];
+            d3[k] = s->X[t[3]][k] - A[k];
+        }
+        const double det = d1[0] * (d2[1] * d3[2] - d2[2] * d3[1]) - d1[1] * (d2[0] * d3[2] - d2[2] * d3[0]) + d1[2] * (d2[0] * d3[1] - d2[1] * d3[0]);
+        const double lumped = std::abs(det) / 6.0 * p->density / 4.0;
+        for (int k = 0; k < 4; k++) s->mass[t[k]] += lumped;
+    }
+    s->all_vertices.resize(nv);
+    for (int32_t i = 0; i < nv; i++) s->all_vertices[i] = i;
+    for (int32_t i = 0; i < nv; i++) {
+        if (fixed[i]) {
+            s->bc.numbered_push_back({ i });
+            s->bc_target.push_back(s->X[i]);
+        }
+    }
+    s->tets_enum.resize((size_t)ne * 5);
+    for (int32_t e = 0; e < ne; e++) {
+        s->tets_enum[5 * (size_t)e] = e;
+        for (int k = 0; k < 4; k++) s->tets_enum[5 * (size_t)e + 1 + k] = s->tets[4 * (size_t)e + k];
+    }
+    s->E_ = p->youngs_modulus;
+    s->nu_ = p->poissons_ratio;
+    s->mu = p->youngs_modulus / (2.0 * (1.0 + p->poissons_ratio));
+    s->lambda = p->youngs_modulus * p->poissons_ratio / ((1.0 + p->poissons_ratio) * (1.0 - 2.0 * p->poissons_ratio));
+    s->lam_over_mu = s->lambda / s->mu;
+    s->log_mu.assign(ne, std::log(s->mu));
+    s->dt = p->dt;
+    s->k = p->bc_stiffness;
+    s->gravity = { p->gravity[0], p->gravity[1], p->gravity[2] };
+
+    s->G = GlobalPotential::create();
+    symx_scene_s* S = s;
+    const int material = p->material;
+    if (p->log_mu_field) {
+        S->G->add_potential("neohookean_tet4_logmu", S->tets_enum, 5, [S, material](MappedWorkspace<double>& mws, Element& elem) {
+            std::vector<Vector> xs = mws.make_vectors(S->x, elem.slice(1, 5));
+            std::vector<Vector> Xs = mws.make_vectors(S->X, elem.slice(1, 5));
+            const Scalar logmu = mws.make_scalar(S->log_mu, elem[0]);
+            const Scalar ratio = mws.make_scalar(S->lam_over_mu);
+            const Scalar m = exp(logmu);
+            return tet4_elastic(mws, xs, Xs, ratio * m, m, 0);
+            (void)material;
+        });
+    }
+    else {
+        S->G->add_potential(material == 0 ? "neohookean_tet4" : "stable_neohookean_tet4", S->tets, 4,
+                            [S, material](MappedWorkspace<double>& mws, Element& elem) {
+                                std::vector<Vector> xs = mws.make_vectors(S->x, elem);
+                                std::vector<Vector> Xs = mws.make_vectors(S->X, elem);
+                                const Scalar a = material == 0 ? mws.make_scalar(S->lambda) : mws.make_scalar(S->E_);
+                                const Scalar b = material == 0 ? mws.make_scalar(S->mu) : mws.make_scalar(S->nu_);
+                                return tet4_elastic(mws, xs, Xs, a, b, material);
+                            });
+    }
+    if (p->dt > 0.0) {
+        // backward Euler inertia with gravity (dynamic_elasticity_with_contact.cpp:85-102, no damping)
+        S->G->add_potential("be_inertia", S->all_vertices, 1, [S](MappedWorkspace<double>& mws, Element& elem) {
+            const Vector x1 = mws.make_vector(S->x, elem[0]);
+            const Vector x0 = mws.make_vector(S->x0, elem[0]);
+            const Vector v0 = mws.make_vector(S->v, elem[0]);
+            const Scalar m = mws.make_scalar(S->mass, elem[0]);
+            const Vector g = mws.make_vector(S->gravity);
+            const Scalar dt = mws.make_scalar(S->dt);
+            const Vector xhat = x0 + dt * v0 + dt * dt * g;
+            const Vector dev = x1 - xhat;
+            return 0.5 * m * dev.squared_norm() / dt.powN(2);
+        });
+    }
+    else if (p->gravity[0] != 0.0 || p->gravity[1] != 0.0 || p->gravity[2] != 0.0) {
+        S->G->add_potential("gravity", S->all_vertices, 1, [S](MappedWorkspace<double>& mws, Element& elem) {
+            const Vector x1 = mws.make_vector(S->x, elem[0]);
+            const Scalar m = mws.make_scalar(S->mass, elem[0]);
+            const Vector g = mws.make_vector(S->gravity);
+            return -1.0 * m * g.dot(x1);
+        });
+    }
+    if (!S->bc_target.empty()) {
+        // penalty Dirichlet condition (rubber_extrusion.cpp:95-107)
+        S->G->add_potential("boundary_conditions", S->bc, [S](MappedWorkspace<double>& mws, Element& conn) {
+            const Vector x = mws.make_vector(S->x, conn["vertex_idx"]);
+            const Vector target = mws.make_vector(S->bc_target, conn["constraint_idx"]);
+            const Scalar k = mws.make_scalar(S->k);
+            return 0.5 * k * (x - target).squared_norm();
+        });
+    }
+    S->G->add_dof(S->x);
+    if (p->log_mu_field) S->G->add_parameter(S->log_mu, "log_mu");
+}
+
 extern "C" {
 
 void symx_newton_params_default(symx_newton_params* p)
@@ -161,121 +282,8 @@ int symx_scene_tet4_block_create(const symx_tet4_block_params* p, const symx_new
     *out = nullptr;
     auto* s = new symx_scene_s();
     int rc = guard([&]() {
-        s->prm = *p;
-        s->np = *np;
-        const FEMMesh mesh = generate_cuboid_Tet4_mesh({ p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] });
-        const int32_t nv = (int32_t)mesh.vertices.size();
-        s->X = mesh.vertices;
-        s->tets = mesh.connectivity;
-        s->x = s->X;
-        const double h_min = std::min({ p->size[0] / p->cells[0], p->size[1] / p->cells[1], p->size[2] / p->cells[2] });
-        (void)h_min;
-        SplitMix64 rng(p->seed);
-        std::vector<uint8_t> fixed(nv, 0);
-        for (int32_t i = 0; i < nv; i++) {
-            if (p->fixed_axis >= 0 && std::abs(s->X[i][p->fixed_axis]) < 1e-12) fixed[i] = 1;
-            for (int d = 0; d < 3; d++) {
-                const double u = rng.uniform();
-                if (!fixed[i]) s->x[i][d] += p->noise * (2.0 * u - 1.0);
-            }
-        }
-        s->x0 = s->x;
-        s->v.assign(nv, { 0.0, 0.0, 0.0 });
-        // lumped masses (dynamic_elasticity_with_contact.cpp:67-79)
-        s->mass.assign(nv, 0.0);
-        const int32_t ne = mesh.n_elements;
-        for (int32_t e = 0; e < ne; e++) {
-            const int32_t* t = &s->tets[4 * (size_t)e];
-            const auto& A = s->X[t[0]];
-            double d1[3], d2[3], d3[3];
-            for (int k = 0; k < 3; k++) {
-                d1[k] = s->X[t[1]][k] - A[k];
-                d2[k] = s->X[t[2]][k] - A[k];
-                d3[k] = s->X[t[3]][k] - A[k];
-            }
-            const double det = d1[0] * (d2[1] * d3[2] - d2[2] * d3[1]) - d1[1] * (d2[0] * d3[2] - d2[2] * d3[0]) + d1[2] * (d2[0] * d3[1] - d2[1] * d3[0]);
-            const double lumped = std::abs(det) / 6.0 * p->density / 4.0;
-            for (int k = 0; k < 4; k++) s->mass[t[k]] += lumped;
-        }
-        s->all_vertices.resize(nv);
-        for (int32_t i = 0; i < nv; i++) s->all_vertices[i] = i;
-        for (int32_t i = 0; i < nv; i++) {
-            if (fixed[i]) {
-                s->bc.numbered_push_back({ i });
-                s->bc_target.push_back(s->X[i]);
-            }
-        }
-        s->tets_enum.resize((size_t)ne * 5);
-        for (int32_t e = 0; e < ne; e++) {
-            s->tets_enum[5 * (size_t)e] = e;
-            for (int k = 0; k < 4; k++) s->tets_enum[5 * (size_t)e + 1 + k] = s->tets[4 * (size_t)e + k];
-        }
-        s->E_ = p->youngs_modulus;
-        s->nu_ = p->poissons_ratio;
-        s->mu = p->youngs_modulus / (2.0 * (1.0 + p->poissons_ratio));
-        s->lambda = p->youngs_modulus * p->poissons_ratio / ((1.0 + p->poissons_ratio) * (1.0 - 2.0 * p->poissons_ratio));
-        s->lam_over_mu = s->lambda / s->mu;
-        s->log_mu.assign(ne, std::log(s->mu));
-        s->dt = p->dt;
-        s->k = p->bc_stiffness;
-        s->gravity = { p->gravity[0], p->gravity[1], p->gravity[2] };
-
-        s->G = GlobalPotential::create();
+        build_scene(s, p, np);
         symx_scene_s* S = s;
-        const int material = p->material;
-        if (p->log_mu_field) {
-            S->G->add_potential("neohookean_tet4_logmu", S->tets_enum, 5, [S, material](MappedWorkspace<double>& mws, Element& elem) {
-                std::vector<Vector> xs = mws.make_vectors(S->x, elem.slice(1, 5));
-                std::vector<Vector> Xs = mws.make_vectors(S->X, elem.slice(1, 5));
-                const Scalar logmu = mws.make_scalar(S->log_mu, elem[0]);
-                const Scalar ratio = mws.make_scalar(S->lam_over_mu);
-                const Scalar m = exp(logmu);
-                return tet4_elastic(mws, xs, Xs, ratio * m, m, 0);
-                (void)material;
-            });
-        }
-        else {
-            S->G->add_potential(material == 0 ? "neohookean_tet4" : "stable_neohookean_tet4", S->tets, 4,
-                                [S, material](MappedWorkspace<double>& mws, Element& elem) {
-                                    std::vector<Vector> xs = mws.make_vectors(S->x, elem);
-                                    std::vector<Vector> Xs = mws.make_vectors(S->X, elem);
-                                    const Scalar a = material == 0 ? mws.make_scalar(S->lambda) : mws.make_scalar(S->E_);
-                                    const Scalar b = material == 0 ? mws.make_scalar(S->mu) : mws.make_scalar(S->nu_);
-                                    return tet4_elastic(mws, xs, Xs, a, b, material);
-                                });
-        }
-        if (p->dt > 0.0) {
-            // backward Euler inertia with gravity (dynamic_elasticity_with_contact.cpp:85-102, no damping)
-            S->G->add_potential("be_inertia", S->all_vertices, 1, [S](MappedWorkspace<double>& mws, Element& elem) {
-                const Vector x1 = mws.make_vector(S->x, elem[0]);
-                const Vector x0 = mws.make_vector(S->x0, elem[0]);
-                const Vector v0 = mws.make_vector(S->v, elem[0]);
-                const Scalar m = mws.make_scalar(S->mass, elem[0]);
-                const Vector g = mws.make_vector(S->gravity);
-                const Scalar dt = mws.make_scalar(S->dt);
-                const Vector xhat = x0 + dt * v0 + dt * dt * g;
-                const Vector dev = x1 - xhat;
-                return 0.5 * m * dev.squared_norm() / dt.powN(2);
-            });
-        }
-        else if (p->gravity[0] != 0.0 || p->gravity[1] != 0.0 || p->gravity[2] != 0.0) {
-            S->G->add_potential("gravity", S->all_vertices, 1, [S](MappedWorkspace<double>& mws, Element& elem) {
-                const Vector x1 = mws.make_vector(S->x, elem[0]);
-                const Scalar m = mws.make_scalar(S->mass, elem[0]);
-                const Vector g = mws.make_vector(S->gravity);
-                return -1.0 * m * g.dot(x1);
-            });
-        }
-        if (!S->bc_target.empty()) {
-            // penalty Dirichlet condition (rubber_extrusion.cpp:95-107)
-            S->G->add_potential("boundary_conditions", S->bc, [S](MappedWorkspace<double>& mws, Element& conn) {
-                const Vector x = mws.make_vector(S->x, conn["vertex_idx"]);
-                const Vector target = mws.make_vector(S->bc_target, conn["constraint_idx"]);
-                const Scalar k = mws.make_scalar(S->k);
-                return 0.5 * k * (x - target).squared_norm();
-            });
-        }
-        S->G->add_dof(S->x);
         S->ctx = Context::create();
         S->newton = NewtonsMethod::create(S->G, S->ctx);
         apply_newton_params(S->newton->settings, *np);
@@ -348,8 +356,10 @@ int symx_scene_set_array(symx_scene s, const char* name, const double* in)
 }
 int symx_scene_set_log_mu(symx_scene s, const double* log_mu)
 {
-    std::memcpy(s->log_mu.data(), log_mu, s->log_mu.size() * sizeof(double));
-    return 0;
+    return guard([&]() {
+        std::memcpy(s->log_mu.data(), log_mu, s->log_mu.size() * sizeof(double));
+        s->newton->device().refresh_data();
+    });
 }
 int symx_scene_set_newton_params(symx_scene s, const symx_newton_params* np)
 {
@@ -382,6 +392,27 @@ int symx_scene_solve(symx_scene s, symx_solve_stats* st)
             st->seconds_assembly = L.get_timer_total("assembly");
             st->seconds_linear = L.get_timer_total("linear_system_solve");
             st->seconds_energy = L.get_timer_total("evaluate_P");
+        }
+    });
+}
+
+int symx_scene_adjoint(symx_scene s, const double* dL_dx, double* dL_dlog_mu, double rel_tol, int32_t max_iter, symx_adjoint_stats* st)
+{
+    return guard([&]() {
+        if (s->log_mu.empty() || s->newton->device().n_parameters() != (int32_t)s->log_mu.size())
+            throw std::runtime_error("scene has no per-element log(mu) parameters");
+        const std::vector<double> rhs(dL_dx, dL_dx + s->x.size() * 3);
+        std::vector<double> grad;
+        const auto a = s->newton->adjoint(rhs, grad, rel_tol, max_iter);
+        std::memcpy(dL_dlog_mu, grad.data(), grad.size() * sizeof(double));
+        if (st) {
+            st->cg_iterations = a.cg_iterations;
+            st->n_refinements = a.n_refinements;
+            st->converged = a.converged;
+            st->found_indefiniteness = a.found_indefiniteness;
+            st->cg_error = a.cg_error;
+            st->energy = a.energy;
+            st->seconds = a.seconds;
         }
     });
 }
@@ -520,6 +551,70 @@ int symx_host_jit_tet4(const char* cache_dir, int32_t material, int64_t* code_si
             throw std::runtime_error(symx_hip_last_error(nullptr));
         *code_size = (int64_t)cs;
         *n_ops_pgh = count_operations(*mws->workspace().graph(), specs[2].outputs);
+    });
+}
+
+int symx_host_scene_jit(const symx_tet4_block_params* p, const char* cache_dir, int32_t* n_potentials, int64_t* code_bytes, int64_t* ops_adjoint)
+{
+    return guard([&]() {
+        symx_newton_params np;
+        symx_newton_params_default(&np);
+        symx_scene_s s;
+        build_scene(&s, p, &np);
+        *n_potentials = 0;
+        *code_bytes = 0;
+        *ops_adjoint = 0;
+        for (const auto& pp : s.G->get_potentials()) {
+            const PotentialModule pm = generate_potential_module(*s.G, *pp);
+            char key[80];
+            int wc = 0;
+            size_t cs = 0;
+            if (symx_hip_jit_compile(pm.source.c_str(), "gfx950", cache_dir, key, &wc, &cs) != SYMX_HIP_OK)
+                throw std::runtime_error(std::string("JIT of ") + pp->name + ": " + symx_hip_last_error(nullptr));
+            *n_potentials += 1;
+            *code_bytes += (int64_t)cs;
+            *ops_adjoint += pm.info.ops_adjoint;
+        }
+    });
+}
+
+int symx_host_scene_adjoint_element(const symx_tet4_block_params* p, const double* log_mu, int32_t e, const double* lambda, double* out)
+{
+    return guard([&]() {
+        if (!p->log_mu_field) throw std::runtime_error("scene has no log(mu) parameters");
+        symx_newton_params np;
+        symx_newton_params_default(&np);
+        symx_scene_s s;
+        build_scene(&s, p, &np);
+        std::memcpy(s.log_mu.data(), log_mu, s.log_mu.size() * sizeof(double));
+        const Potential& P = *s.G->get_potentials()[0];
+        const PotentialModule pm = generate_potential_module(*s.G, P);
+        const KernelSpec& ks = pm.specs.back();
+        if (ks.kind != KernelKind::PGP) throw std::runtime_error("elastic potential has no adjoint kernel");
+        MappedWorkspace<double>& mws = *P.mws;
+        Graph* g = mws.workspace().graph();
+        const int32_t* conn = mws.conn.data() + (size_t)e * mws.conn.stride;
+        // Gather like the generated kernel: data maps through their connectivity column,
+        // adjoint symbols where their DoF map reads (the DoF set starts at offset 0 here).
+        for (const DataMap& m : mws.get_maps()) {
+            const int64_t item = m.connectivity_index >= 0 ? conn[m.connectivity_index] : 0;
+            for (int k = 0; k < m.stride; k++) g->set_symbol_value(m.first_symbol_idx + k, m.data()[item * m.stride + k]);
+        }
+        for (const auto& a : ks.adjoint_symbols) {
+            const DataMap& m = mws.get_maps()[a[1]];
+            g->set_symbol_value(a[0], lambda[(int64_t)conn[m.connectivity_index] * m.stride + a[2]]);
+        }
+        std::vector<int32_t> ids;
+        for (const Scalar& v : ks.outputs) ids.push_back(v.id);
+        // summation (quadrature) symbols: accumulate over the iterations like the kernel
+        const SummationMap& sm = mws.get_summation();
+        const int32_t nq = std::max<int32_t>(sm.n_iterations, 1);
+        for (size_t i = 0; i < ids.size(); i++) out[i] = 0.0;
+        for (int32_t q = 0; q < nq; q++) {
+            for (int k = 0; k < sm.stride; k++) g->set_symbol_value(sm.first_symbol_idx + k, sm.data[(size_t)q * sm.stride + k]);
+            const std::vector<double> v = g->eval_many(ids);
+            for (size_t i = 0; i < v.size(); i++) out[i] += v[i];
+        }
     });
 }
 

@@ -8,6 +8,9 @@ build container:  python tests/golden/make_golden.py
                              (tests/sympy_derivatives.cpp, used by tests/test_diff.cpp:74-137)
   newton_ref_*.npz           final positions and iteration counts of the reference
                              NewtonsMethod on Tet4 block problems (n_threads = 1)
+  adjoint_ref_*.npz          dL/dlog_mu of the quasistatic cantilever from the reference's
+                             own pieces (ref_driver adjoint) at the oracle's equilibrium
+                             (`python tests/golden/make_golden.py adjoint` writes only these)
 """
 import json
 import os
@@ -76,7 +79,36 @@ def newton_case(name, cells, noise, seed, rel_tol, cg_rel, threads=1):
     print(name, {k: v for k, v in res.items() if k != "x"})
 
 
+def adjoint_case(name, cells):
+    """Reference-composed adjoint of the cantilever at the oracle's equilibrium."""
+    from adjoint_harness import log_mu_field, oracle_equilibrium, scene_np
+    from oracle_py import Oracle
+    from ref_io import read_adjoint_output, write_adjoint_input
+
+    O = Oracle()
+    S = scene_np(cells)
+    ne = S.tets.shape[0]
+    log_mu_t = log_mu_field(ne, 2)
+    x_t = oracle_equilibrium(O, S, log_mu_t)
+    log_mu = log_mu_field(ne, 1)
+    x = oracle_equilibrium(O, S, log_mu)
+    S.x[...] = x.reshape(S.x.shape)
+    dL_dx = x - x_t
+    with tempfile.TemporaryDirectory() as d:
+        inp, outp = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        write_adjoint_input(inp, S, log_mu, dL_dx)
+        subprocess.run([os.path.join(REF, "ref_driver"), "adjoint", inp, outp], check=True, env=ENV)
+        res = read_adjoint_output(outp, S.X.shape[0], ne)
+    assert res["ok"] == 1
+    np.savez_compressed(os.path.join(HERE, f"adjoint_ref_{name}.npz"), cells=np.array(cells), log_mu=log_mu, x=x, x_target=x_t,
+                        dL_dlog_mu=res["dL_dlog_mu"], lam=res["lambda"], energy=res["energy"])
+    print(name, "adjoint", "max|dL/dp|", np.abs(res["dL_dlog_mu"]).max())
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "adjoint":
+        adjoint_case("cantilever16", (15, 3, 3))
+        sys.exit(0)
     nh_cases()
     point_triangle_cases()
     newton_case("block6", (6, 6, 6), 0.2 / 6, 11, 1e-6, 1e-3)

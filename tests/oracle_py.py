@@ -78,8 +78,9 @@ class SceneData:
         return s
 
 
-def scene_arrays(scene) -> SceneData:
-    """Arrays of a symx_amd.Tet4Block scene (same inputs as the GPU path)."""
+def scene_arrays(scene, log_mu=None) -> SceneData:
+    """Arrays of a symx_amd.Tet4Block scene (same inputs as the GPU path). log_mu:
+    per-element log(mu) of a log_mu_field scene (lambda_e = (lambda/mu) * mu_e)."""
     p = scene.params
     X = scene.get("X")
     fixed = np.zeros(X.shape[0], bool)
@@ -89,8 +90,9 @@ def scene_arrays(scene) -> SceneData:
     E, nu = p.youngs_modulus, p.poissons_ratio
     mu = E / (2 * (1 + nu))
     lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu_e = None if log_mu is None else np.exp(np.asarray(log_mu, np.float64))
     return SceneData(scene.connectivity(), X, scene.get("x"), scene.get("x0"), scene.get("v"), scene.get("mass"), bc_vertex,
-                     X[bc_vertex], lam, mu, p.dt, p.bc_stiffness, list(p.gravity))
+                     X[bc_vertex], lam, mu, p.dt, p.bc_stiffness, list(p.gravity), mu_e=mu_e, lambda_over_mu=lam / mu)
 
 
 class Oracle:
@@ -162,6 +164,18 @@ class Oracle:
         d["residuals"] = Rs[:n].copy()
         d["x"] = S.x.reshape(-1, 3).copy()
         return d
+
+    def adjoint_logmu(self, S: SceneData, dL_dx, rel_tol=1e-12, max_iter=100000):
+        """(dL/dlog_mu, lambda, pcg info) at the positions S.x (S.mu_e must be set)."""
+        st = S.struct()
+        b = np.ascontiguousarray(dL_dx, np.float64).reshape(-1)
+        lam = np.zeros_like(b)
+        g = np.zeros(S.tets.shape[0])
+        info = PcgInfo()
+        rc = self.lib.oracle_adjoint_logmu(ctypes.byref(st), _p(b), c_double(rel_tol), c_int(max_iter), _p(lam), _p(g), ctypes.byref(info))
+        if rc:
+            raise RuntimeError(f"oracle_adjoint_logmu failed ({rc})")
+        return g, lam, {k: getattr(info, k) for k, _ in info._fields_}
 
     def num_threads(self) -> int:
         return self.lib.oracle_num_threads()

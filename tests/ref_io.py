@@ -1,4 +1,4 @@
-"""Binary input/output layout of oracle/ref_driver's `solve` mode (test infrastructure)."""
+"""Binary input/output layouts of oracle/ref_driver's `solve` and `adjoint` modes (test infrastructure)."""
 import numpy as np
 
 
@@ -25,3 +25,29 @@ def read_solve_output(path, nv):
     x = np.frombuffer(raw[32:32 + 24 * nv], np.float64).reshape(nv, 3).copy()
     return {"result": int(hdr[0]), "newton_iterations": int(hdr[1]), "cg_iterations": int(hdr[2]), "ls_bt_iterations": int(hdr[3]),
             "energy": float(E), "seconds": float(secs), "x": x}
+
+
+def write_adjoint_input(path, S, log_mu, dL_dx, threads=1):
+    """S: SceneData of the quasistatic cantilever with S.x the equilibrium."""
+    hdr = np.array([S.X.shape[0], S.tets.shape[0], S.bc_vertex.shape[0], threads], np.int32)
+    prm = np.array([S.lambda_over_mu, S.k, *S.gravity], np.float64)
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(prm.tobytes())
+        f.write(S.tets.astype(np.int32).tobytes())
+        for a in (S.X, S.x):
+            f.write(np.ascontiguousarray(a, np.float64).tobytes())
+        f.write(S.mass.astype(np.float64).tobytes())
+        f.write(S.bc_vertex.astype(np.int32).tobytes())
+        f.write(np.ascontiguousarray(S.bc_target, np.float64).tobytes())
+        f.write(np.ascontiguousarray(log_mu, np.float64).tobytes())
+        f.write(np.ascontiguousarray(dL_dx, np.float64).reshape(-1).tobytes())
+
+
+def read_adjoint_output(path, nv, ne):
+    raw = open(path, "rb").read()
+    ok = int(np.frombuffer(raw[:4], np.int32)[0])
+    E = float(np.frombuffer(raw[4:12], np.float64)[0])
+    g = np.frombuffer(raw[12:12 + 8 * ne], np.float64).copy()
+    lam = np.frombuffer(raw[12 + 8 * ne:12 + 8 * ne + 24 * nv], np.float64).copy()
+    return {"ok": ok, "energy": E, "dL_dlog_mu": g, "lambda": lam}
