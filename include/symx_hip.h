@@ -168,6 +168,14 @@ int symx_hip_problem_element_outputs(symx_hip_problem prob, int32_t pot_id, doub
  * eigenvalues < eps become eps (clamp) or -lambda (mirror). *n_changed
  * receives the number of matrices that were rebuilt (may be NULL). */
 int symx_hip_problem_project(symx_hip_problem prob, double eps, int32_t mirror, int64_t* n_changed);
+/* Progressive projection step (NewtonsMethod.cpp:306-330 with
+ * ElementHessians::project_to_PD_for_update__selectively, ElementHessians.cpp:68-165):
+ * a DoF block is active when max |VEC_GRAD| over its 3 entries >= threshold (0: all
+ * blocks); every element touching an active block that has not been projected since
+ * the last EVAL_PGH is projected and remembered. Outputs (each may be NULL): matrices
+ * rebuilt, elements newly projected, active blocks. Assemble afterwards. */
+int symx_hip_problem_project_selective(symx_hip_problem prob, double eps, int32_t mirror, double threshold, int64_t* n_changed,
+                                       int64_t* n_newly_projected, int64_t* n_active_blocks);
 /* Assemble element Hessians into the device block matrix in FP64 or FP32
  * storage (ElementHessians::assemble_global, ElementHessians.cpp:224-256) and
  * prepare the block-Jacobi preconditioner (BlockedSparseMatrix.h:1147-1214).
@@ -233,6 +241,16 @@ typedef struct symx_hip_adjoint_params {
  * elements of potential pot_id with lambda = VEC_DU; out [n_elem][n_out] is copied
  * to host_out. */
 int symx_hip_problem_launch_adjoint(symx_hip_problem prob, int32_t pot_id, const char* name, int32_t n_out, double* host_out);
+
+/* Stateless per-element evaluation (CompiledInLoop, CompiledInLoop_run.h:66-472):
+ * uploads the connectivity conn [n_elem][conn_stride] (host, row-major) and the data
+ * maps (host pointers, maps[m].n_values values each; is_dof is ignored), launches the
+ * kernel `name` of `mod` (signature (const symx_hip_eval_params p, const
+ * symx_hip_adjoint_params a), writing a.out[e * n_out + j]) and copies the outputs to
+ * host_out [n_elem][n_out]. active (host, may be NULL) holds one flag per element;
+ * inactive elements get zeros. Device buffers are freed before returning. */
+int symx_hip_loop_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const int32_t* conn, int32_t n_elem, int32_t conn_stride,
+                       const symx_hip_map_desc* maps, int32_t n_maps, const uint8_t* active, int32_t n_out, double* host_out);
 
 /* Kernel timing with HIP events on the context's stream (the stream every
  * kernel of this library runs on): average duration in ms of `reps` launches

@@ -39,7 +39,7 @@ typedef struct symx_tet4_block_params {
 /* Newton settings exposed to callers (subset of NewtonSettings, solver_utils.h:173-258). */
 typedef struct symx_newton_params {
     int32_t precision;               /* SYMX_HIP_FP64 or SYMX_HIP_MIXED */
-    int32_t projection_mode;         /* 0 Newton, 1 ProjectedNewton, 2 ProjectOnDemand */
+    int32_t projection_mode;         /* 0 Newton, 1 ProjectedNewton, 2 ProjectOnDemand, 3 Progressive */
     int32_t use_mirroring;
     int32_t max_iterations;
     double residual_tolerance_abs;

@@ -616,6 +616,10 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
     double* x_ls = (double*)malloc(sizeof(double) * n);
     double* He = (double*)malloc(sizeof(double) * 144 * (int64_t)(S->ne > 0 ? S->ne : 1));
     double* vals = (double*)malloc(sizeof(double) * 9 * P.nnzb);
+    unsigned char* projected = (unsigned char*)malloc((size_t)(S->ne > 0 ? S->ne : 1));
+    unsigned char* active = (unsigned char*)malloc((size_t)(S->nv > 0 ? S->nv : 1));
+    int pdn_countdown = 0;
+    double ppn_threshold = -1.0;
     memset(st, 0, sizeof(*st));
     double res0 = 1e308;
     int result = 1; /* Running */
@@ -652,29 +656,77 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
             result = 0;
             break;
         }
-        if (np->projection_mode == 1) {
+        /* Projection + solve, retried with more projection until the step descends
+         * (NewtonsMethod.cpp:133-185, _project_and_assemble :254-352, _increase/_decrease
+         * :354-386). Modes: 0 Newton, 1 ProjectedNewton, 2 ProjectOnDemand, 3 Progressive
+         * (default countdown 4, tightening 0.5, release 2). */
+        memset(projected, 0, (size_t)(S->ne > 0 ? S->ne : 1));
+        double dug = 0.0;
+        for (;;) {
+            int all_projected = 0;
+            const int mode = np->projection_mode;
+            if (mode == 1 || (mode == 2 && pdn_countdown > 0)) {
 #pragma omp parallel for schedule(dynamic, 64)
-            for (int e = 0; e < S->ne; e++) oracle_project(He + 144 * (int64_t)e, 12, np->projection_eps, np->use_mirroring);
+                for (int e = 0; e < S->ne; e++) {
+                    if (!projected[e]) oracle_project(He + 144 * (int64_t)e, 12, np->projection_eps, np->use_mirroring);
+                    projected[e] = 1;
+                }
+                all_projected = 1;
+            }
+            else if (mode == 3 && ppn_threshold >= 0.0) {
+                if (ppn_threshold > 0.0 && ppn_threshold < 1e-12) ppn_threshold = 0.0;
+                int n_active = 0;
+                for (int v = 0; v < S->nv; v++) {
+                    const double m = fmax(fabs(grad[3 * (int64_t)v]), fmax(fabs(grad[3 * (int64_t)v + 1]), fabs(grad[3 * (int64_t)v + 2])));
+                    active[v] = m >= ppn_threshold;
+                    n_active += active[v];
+                }
+#pragma omp parallel for schedule(dynamic, 64)
+                for (int e = 0; e < S->ne; e++) {
+                    if (projected[e]) continue;
+                    const int32_t* t = S->tets + 4 * (int64_t)e;
+                    if (active[t[0]] || active[t[1]] || active[t[2]] || active[t[3]]) {
+                        oracle_project(He + 144 * (int64_t)e, 12, np->projection_eps, np->use_mirroring);
+                        projected[e] = 1;
+                    }
+                }
+                all_projected = n_active == S->nv;
+            }
+            assemble(S, &P, He, vals);
+            st->seconds_eval += now_seconds() - t0;
+            t0 = now_seconds();
+            for (int64_t i = 0; i < n; i++) rhs[i] = -grad[i];
+            const double forcing = fmin(1e-2, r * fmin(0.5, sqrt(r)));
+            const double abs_tol = fmax(forcing, np->cg_abs_tolerance);
+            oracle_pcg_info info;
+            oracle_pcg_bsr(P.nb, P.rp, P.cols, vals, rhs, du, abs_tol, np->cg_rel_tolerance, np->cg_max_iterations, 1, &info);
+            st->cg_iterations += info.n_iterations;
+            st->seconds_linear += now_seconds() - t0;
+            t0 = now_seconds();
+            const int can_project_more = mode != 0 && !all_projected;
+            if (!info.converged && !can_project_more) {
+                result = 5; /* LinearSystemSolveFailure */
+                break;
+            }
+            int descends = 0;
+            if (info.converged) {
+                dug = vdot(n, du, grad);
+                descends = dug < 0.0;
+                if (!descends && !can_project_more) {
+                    result = 7; /* StepDoesNotDescend */
+                    break;
+                }
+            }
+            if (info.converged && descends) break;
+            if (mode == 2) pdn_countdown = 4;
+            if (mode == 3) {
+                if (ppn_threshold < 0.0) ppn_threshold = r;
+                ppn_threshold *= 0.5;
+            }
         }
-        assemble(S, &P, He, vals);
-        st->seconds_eval += now_seconds() - t0;
-        t0 = now_seconds();
-        for (int64_t i = 0; i < n; i++) rhs[i] = -grad[i];
-        const double forcing = fmin(1e-2, r * fmin(0.5, sqrt(r)));
-        const double abs_tol = fmax(forcing, np->cg_abs_tolerance);
-        oracle_pcg_info info;
-        oracle_pcg_bsr(P.nb, P.rp, P.cols, vals, rhs, du, abs_tol, np->cg_rel_tolerance, np->cg_max_iterations, 1, &info);
-        st->cg_iterations += info.n_iterations;
-        st->seconds_linear += now_seconds() - t0;
-        if (!info.converged) {
-            result = 5; /* LinearSystemSolveFailure */
-            break;
-        }
-        const double dug = vdot(n, du, grad);
-        if (!(dug < 0.0)) {
-            result = 7; /* StepDoesNotDescend */
-            break;
-        }
+        if (result != 1) break;
+        if (np->projection_mode == 2) pdn_countdown--;
+        if (np->projection_mode == 3) ppn_threshold *= 2.0;
         double du_max = 0.0;
         for (int64_t i = 0; i < n; i++) du_max = fmax(du_max, fabs(du[i]));
         if (du_max < np->step_tolerance) {
@@ -702,6 +754,8 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
     }
     st->result = result;
     st->newton_iterations = it;
+    free(projected);
+    free(active);
     free_pattern(&P);
     free(grad);
     free(rhs);

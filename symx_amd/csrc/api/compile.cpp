@@ -253,7 +253,7 @@ namespace symx
             const int nb = n / 3;
             const int n_ub = nb * (nb + 1) / 2;
 
-            const bool adj = ks.kind == KernelKind::PGP;
+            const bool adj = ks.kind == KernelKind::PGP || ks.kind == KernelKind::LOOP;
             os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.name << "(const symx_hip_eval_params p"
                << (adj ? ", const symx_hip_adjoint_params a" : "") << ")\n{\n";
             os << "    const int e = symx_block() * 256 + threadIdx.x;\n";

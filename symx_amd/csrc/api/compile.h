@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <functional>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -189,6 +190,16 @@ namespace symx
             return r;
         }
 
+        // Per element matrices (row-major, shape[0] x shape[1] doubles per item)
+        template <typename STATIC_MATRIX>
+        Matrix make_matrix(const std::vector<STATIC_MATRIX>& arr, const std::array<int32_t, 2>& shape, const Index& idx)
+        {
+            constexpr int32_t stride = (int32_t)(sizeof(STATIC_MATRIX) / sizeof(double));
+            if (stride != shape[0] * shape[1]) throw std::runtime_error("symx: make_matrix shape does not match the item size");
+            Vector v = make_vector(arr, idx);
+            return Matrix(v.values(), shape);
+        }
+
         // Summation over a table of iteration vectors (quadrature).
         Vector make_summation_vector(const double* data, int32_t stride, int32_t n_iterations);
         template <typename STATIC_VECTOR>
@@ -228,7 +239,10 @@ namespace symx
 
     // ---- HIP emitter --------------------------------------------------------------
     // Description of one generated element kernel.
-    enum class KernelKind { P, PG, PGH, C, PGP };
+    // P: energy; PG: + gradient; PGH: + Hessian; C: activity condition; PGP: adjoint mixed
+    // partials; LOOP: arbitrary per-element outputs (CompiledInLoop). PGP and LOOP take a
+    // second symx_hip_adjoint_params argument and write its `out` array.
+    enum class KernelKind { P, PG, PGH, C, PGP, LOOP };
     struct KernelSpec
     {
         std::string name;

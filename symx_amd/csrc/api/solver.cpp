@@ -55,7 +55,7 @@ namespace symx
     }
     void OutputSink::print(const std::string& s, Verbosity v)
     {
-        if ((int)v <= (int)verbosity && verbosity != Verbosity::None) std::cout << s << std::flush;
+        if (enabled && (int)v <= (int)verbosity && verbosity != Verbosity::None) std::cout << s << std::flush;
     }
     void OutputSink::print_with_new_line(const std::string& s, Verbosity v) { print("\n" + s, v); }
 
@@ -468,8 +468,6 @@ namespace symx
         if (ndofs % 3 != 0) throw std::runtime_error("symx error NewtonsMethod::solve(): ndofs must be divisible by 3.");
         if (settings.linear_solver != LinearSolver::BDPCG)
             throw std::runtime_error("symx: the HIP backend solves with BDPCG only (DirectLLT is not available on the device)");
-        if (settings.projection_mode == ProjectionToPD::Progressive)
-            throw std::runtime_error("symx: ProjectionToPD::Progressive is not implemented by the HIP backend yet");
         if (!dev->matches_topology()) dev = std::make_unique<DeviceGlobal>(G, context);
         symx_hip_ctx hip = context->hip();
         symx_hip_problem prob = dev->problem();
@@ -540,27 +538,45 @@ namespace symx
             }
 
             const int initial_cg = stats.cg_iterations;
-            bool assembled = false;
+            int64_t n_projected_now = 0;  // element Hessians projected in this Newton iteration
             bool ok = false;
             while (!ok) {
-                // Project element Hessians and assemble (NewtonsMethod.cpp:254-352).
+                // Project element Hessians and assemble (NewtonsMethod.cpp:254-352). Every
+                // pass re-assembles from the element Hessians, projected ones included, which
+                // gives the matrix that the reference's incremental update_global builds.
                 bool all_projected = false;
                 {
                     auto t = logger->time("project_to_PD");
-                    bool project = false;
-                    if (settings.projection_mode == ProjectionToPD::ProjectedNewton) project = true;
-                    else if (settings.projection_mode == ProjectionToPD::ProjectOnDemand) project = pdn_countdown > 0;
-                    if (project) {
-                        int64_t changed = 0;
-                        check(symx_hip_problem_project(prob, settings.projection_eps, settings.project_to_pd_use_mirroring ? 1 : 0, &changed), hip,
-                              "projection");
-                        all_projected = true;
+                    const int32_t mirror = settings.project_to_pd_use_mirroring ? 1 : 0;
+                    int64_t changed = 0;
+                    switch (settings.projection_mode) {
+                    case ProjectionToPD::Newton: break;
+                    case ProjectionToPD::ProjectedNewton:
+                    case ProjectionToPD::ProjectOnDemand:
+                        if (settings.projection_mode == ProjectionToPD::ProjectedNewton || pdn_countdown > 0) {
+                            check(symx_hip_problem_project(prob, settings.projection_eps, mirror, &changed), hip, "projection");
+                            all_projected = true;
+                            n_projected_now = n_hessians;
+                        }
+                        break;
+                    case ProjectionToPD::Progressive:
+                        // The reference tests ppn_threshold > 0 here, so once the threshold has
+                        // dropped to 0 it stops projecting and retries forever; 0 projects
+                        // every block here (its "default to PN" intent).
+                        if (ppn_threshold >= 0.0) {
+                            if (ppn_threshold > 0.0 && ppn_threshold < 1e-12) ppn_threshold = 0.0;
+                            int64_t fresh = 0, active = 0;
+                            check(symx_hip_problem_project_selective(prob, settings.projection_eps, mirror, ppn_threshold, &changed, &fresh, &active),
+                                  hip, "selective projection");
+                            n_projected_now += fresh;
+                            all_projected = active == ndofs / 3;
+                        }
+                        break;
                     }
                 }
                 {
                     auto t = logger->time("assembly");
                     check(symx_hip_problem_assemble(prob, precision_mixed ? SYMX_HIP_MIXED : SYMX_HIP_FP64), hip, "assembly");
-                    assembled = true;
                 }
                 ok = solve_linear_system(residual_norm);
                 const bool can_project_more = settings.projection_mode != ProjectionToPD::Newton && !all_projected;
@@ -581,23 +597,23 @@ namespace symx
                 }
                 if (ok && descends) break;
                 ok = false;
-                // _increase_projection
+                // _increase_projection (NewtonsMethod.cpp:354-368)
                 if (settings.projection_mode == ProjectionToPD::ProjectOnDemand) pdn_countdown = settings.project_on_demand_countdown;
+                if (settings.projection_mode == ProjectionToPD::Progressive) {
+                    if (ppn_threshold < 0.0) check(symx_hip_problem_max_abs(prob, SYMX_HIP_VEC_GRAD, &ppn_threshold), hip, "max_abs");
+                    ppn_threshold *= settings.ppn_tightening_factor;
+                }
             }
-            (void)assembled;
             if (result != SolverReturn::Running) {
                 out->print_with_new_line("Newton failure: Could not solve the linear system or find a descend direction.", Verbosity::Summary);
                 break;
             }
             stats.cg_per_iteration.push_back(stats.cg_iterations - initial_cg);
-            // _decrease_projection
+            // _decrease_projection (NewtonsMethod.cpp:370-386)
             if (settings.projection_mode == ProjectionToPD::ProjectOnDemand) pdn_countdown--;
-            const bool projected_now = settings.projection_mode == ProjectionToPD::ProjectedNewton ||
-                                       (settings.projection_mode == ProjectionToPD::ProjectOnDemand && pdn_countdown >= 0 && pdn_countdown + 1 > 0 &&
-                                        false);
-            (void)projected_now;
+            if (settings.projection_mode == ProjectionToPD::Progressive) ppn_threshold *= settings.ppn_release_factor;
             stats.n_hessians += (uint64_t)n_hessians;
-            if (settings.projection_mode == ProjectionToPD::ProjectedNewton) stats.n_projected_hessians += (uint64_t)n_hessians;
+            stats.n_projected_hessians += (uint64_t)n_projected_now;
             logger->add_and_append("cg_iterations", last_cg_iterations);
 
             double du_max = 0.0;
@@ -803,6 +819,167 @@ namespace symx
     void Compiled<double>::run_batch(const double* inputs, int64_t n, double* outputs)
     {
         check(symx_hip_batch_eval(ctx, mod, name.c_str(), inputs, n, n_in, outputs, n_out), ctx, "run_batch");
+    }
+
+    // ---- host BSR ----------------------------------------------------------------------
+    void BlockedSparseMatrix::to_triplets(std::vector<Triplet>& t) const
+    {
+        t.clear();
+        for (int32_t r = 0; r < n_block_rows; r++)
+            for (int64_t k = rows[r]; k < rows[r + 1]; k++)
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) t.push_back(Triplet{ 3 * r + i, 3 * cols[k] + j, vals[9 * k + 3 * i + j] });
+    }
+    std::vector<double> BlockedSparseMatrix::to_dense() const
+    {
+        const int64_t n = 3 * (int64_t)n_block_rows;
+        std::vector<double> d((size_t)(n * n), 0.0);
+        std::vector<Triplet> t;
+        to_triplets(t);
+        for (const Triplet& x : t) d[(size_t)x.r * n + x.c] += x.v;
+        return d;
+    }
+    void BlockedSparseMatrix::spmxv(const double* x, double* y) const
+    {
+        for (int32_t r = 0; r < n_block_rows; r++) {
+            double acc[3] = { 0.0, 0.0, 0.0 };
+            for (int64_t k = rows[r]; k < rows[r + 1]; k++)
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) acc[i] += vals[9 * k + 3 * i + j] * x[3 * (int64_t)cols[k] + j];
+            for (int i = 0; i < 3; i++) y[3 * (int64_t)r + i] = acc[i];
+        }
+    }
+
+    // ---- ElementHessians / SecondOrderCompiledGlobal ----------------------------------
+    int64_t ElementHessians::project_to_PD_inplace__all(double eps, bool mirror)
+    {
+        int64_t n = 0;
+        check(symx_hip_problem_project(prob, eps, mirror ? 1 : 0, &n), ctx, "project_to_PD");
+        return n;
+    }
+    spBSM ElementHessians::assemble_global(int n_threads, int ndofs)
+    {
+        (void)n_threads;
+        check(symx_hip_problem_assemble(prob, SYMX_HIP_FP64), ctx, "assembly");
+        auto m = std::make_shared<BlockedSparseMatrix>();
+        m->n_block_rows = ndofs / 3;
+        int64_t nnzb = 0;
+        check(symx_hip_problem_export_bsr(prob, &nnzb, nullptr, nullptr, nullptr), ctx, "export_bsr");
+        m->rows.resize(m->n_block_rows + 1);
+        m->cols.resize(nnzb);
+        m->vals.resize(9 * nnzb);
+        check(symx_hip_problem_export_bsr(prob, &nnzb, m->rows.data(), m->cols.data(), m->vals.data()), ctx, "export_bsr");
+        return m;
+    }
+
+    SecondOrderCompiledGlobal::SecondOrderCompiledGlobal(spGlobalPotential G_, spContext context_) : G(G_), context(context_)
+    {
+        dev = std::make_unique<DeviceGlobal>(G, context);
+    }
+    void SecondOrderCompiledGlobal::upload()
+    {
+        if (!dev->matches_topology()) dev = std::make_unique<DeviceGlobal>(G, context);
+        dev->refresh_data();
+        std::vector<double> u(dev->n_dofs());
+        G->get_dofs(u.data());
+        check(symx_hip_problem_set_dofs(dev->problem(), u.data()), context->hip(), "set_dofs");
+    }
+    void SecondOrderCompiledGlobal::evaluate_P(double& E)
+    {
+        upload();
+        check(symx_hip_problem_eval(dev->problem(), SYMX_HIP_EVAL_P, &E), context->hip(), "evaluate_P");
+    }
+    void SecondOrderCompiledGlobal::evaluate_P__dP_du(double& E, std::vector<double>& grad)
+    {
+        upload();
+        check(symx_hip_problem_eval(dev->problem(), SYMX_HIP_EVAL_PG, &E), context->hip(), "evaluate_P__dP_du");
+        grad.resize(dev->n_dofs());
+        check(symx_hip_problem_download(dev->problem(), SYMX_HIP_VEC_GRAD, grad.data()), context->hip(), "gradient");
+    }
+    spElementHessians SecondOrderCompiledGlobal::evaluate_P__dP_du__local_d2P_du2(double& E, std::vector<double>& grad)
+    {
+        upload();
+        check(symx_hip_problem_eval(dev->problem(), SYMX_HIP_EVAL_PGH, &E), context->hip(), "evaluate_P__dP_du__local_d2P_du2");
+        grad.resize(dev->n_dofs());
+        check(symx_hip_problem_download(dev->problem(), SYMX_HIP_VEC_GRAD, grad.data()), context->hip(), "gradient");
+        return std::make_shared<ElementHessians>(dev->problem(), context->hip());
+    }
+    double SecondOrderCompiledGlobal::test_derivatives_with_FD(double h)
+    {
+        const int n = G->get_total_n_dofs();
+        std::vector<double> u(n), grad, gp, gm;
+        G->get_dofs(u.data());
+        double E = 0.0, Ep = 0.0, Em = 0.0;
+        const spBSM H = evaluate_P__dP_du__local_d2P_du2(E, grad)->assemble_global(1, n);
+        const std::vector<double> Hd = H->to_dense();
+        double max_g = 0.0, max_h = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double ui = u[i];
+            u[i] = ui + h;
+            G->set_dofs(u.data());
+            evaluate_P__dP_du(Ep, gp);
+            u[i] = ui - h;
+            G->set_dofs(u.data());
+            evaluate_P__dP_du(Em, gm);
+            u[i] = ui;
+            max_g = std::max(max_g, std::abs((Ep - Em) / (2.0 * h) - grad[i]));
+            for (int j = 0; j < n; j++) max_h = std::max(max_h, std::abs((gp[j] - gm[j]) / (2.0 * h) - Hd[(size_t)i * n + j]));
+        }
+        G->set_dofs(u.data());
+        return std::max(max_g, max_h);
+    }
+
+    // ---- CompiledInLoop<double> ---------------------------------------------------------
+    CompiledInLoop<double>::CompiledInLoop(const spMWS<double>& mws_, const std::vector<Scalar>& expr, const std::string& name,
+                                           const std::string& folder, const std::string& cache_id)
+        : mws(mws_), kernel(identifier(name) + "_loop")
+    {
+        (void)cache_id;
+        if (expr.empty()) throw std::runtime_error("symx: CompiledInLoop without expressions");
+        if ((int32_t)mws->get_maps().size() > SYMX_HIP_MAX_MAPS) throw std::runtime_error("symx: too many data maps in one loop (max 48)");
+        n_out = (int32_t)expr.size();
+        KernelSpec ks;
+        ks.name = kernel;
+        ks.kind = KernelKind::LOOP;
+        ks.outputs = expr;
+        src = emit_hip_module(identifier(name), *mws->workspace().graph(), mws->get_maps(), mws->get_summation(), { ks }, symx_hip_eval_prelude());
+        ctx = default_ctx();
+        const std::string dir = folder.empty() ? get_codegen_dir() : folder;
+        char key[80];
+        int wc = 0;
+        size_t cs = 0;
+        check(symx_hip_jit_compile(src.c_str(), "gfx950", dir.c_str(), key, &wc, &cs), nullptr, "JIT");
+        cached = wc != 0;
+        check(symx_hip_module_load(ctx, src.c_str(), dir.c_str(), &mod), ctx, "module load");
+    }
+    CompiledInLoop<double>::~CompiledInLoop()
+    {
+        if (mod) symx_hip_module_unload(mod);
+    }
+    void CompiledInLoop<double>::run(int32_t n_threads, const Callback& cb) { run(n_threads, cb, {}); }
+    void CompiledInLoop<double>::run(int32_t n_threads, const Callback& cb, const std::vector<uint8_t>& active)
+    {
+        (void)n_threads;
+        const int32_t ne = mws->conn.n_elements();
+        const int32_t stride = mws->conn.stride;
+        if (!active.empty() && (int32_t)active.size() != ne) throw std::runtime_error("symx: is_element_active must have one flag per element");
+        const auto& maps = mws->get_maps();
+        std::vector<symx_hip_map_desc> md(maps.size());
+        for (size_t m = 0; m < maps.size(); m++) {
+            md[m].is_dof = 0;
+            md[m].dof_offset = 0;
+            md[m].host = maps[m].data();
+            md[m].n_values = (int64_t)maps[m].n_elements() * maps[m].stride;
+        }
+        out.assign((size_t)ne * n_out, 0.0);
+        const int32_t* conn = mws->conn.data();
+        check(symx_hip_loop_eval(ctx, mod, kernel.c_str(), conn, ne, stride, md.data(), (int32_t)md.size(), active.empty() ? nullptr : active.data(),
+                                 n_out, out.data()),
+              ctx, "CompiledInLoop::run");
+        for (int32_t e = 0; e < ne; e++) {
+            if (!active.empty() && !active[e]) continue;
+            cb(View<double>(out.data() + (size_t)e * n_out, n_out), e, 0, View<int32_t>(const_cast<int32_t*>(conn) + (size_t)e * stride, stride));
+        }
     }
 
     // ---- FEM helpers (fem_integrators.cpp) -------------------------------------------

@@ -54,8 +54,10 @@ namespace symx
         void print_new_line(Verbosity v = Verbosity::Medium) { print("\n", v); }
         void set_console_verbosity(Verbosity v) { verbosity = v; }
         Verbosity get_console_verbosity() const { return verbosity; }
+        void set_enabled(bool on) { enabled = on; }
     private:
         Verbosity verbosity = Verbosity::None;
+        bool enabled = true;
     };
     using spOutputSink = std::shared_ptr<OutputSink>;
 
@@ -343,6 +345,119 @@ namespace symx
         std::vector<int32_t> map_dof_set;    // DoF set per data map (-1: not a DoF)
     };
     PotentialModule generate_potential_module(GlobalPotential& G, const Potential& P);
+
+    // ---- views (View.h) ----------------------------------------------------------------
+    template <typename T>
+    class View
+    {
+    public:
+        View(T* data, int32_t size) : d(data), n(size) {}
+        int32_t size() const { return n; }
+        T* data() { return d; }
+        const T* data() const { return d; }
+        T& operator[](int32_t i) { return d[i]; }
+        const T& operator[](int32_t i) const { return d[i]; }
+        View slice(int32_t begin, int32_t end) const { return View(d + begin, end - begin); }
+        View slice(int32_t begin) const { return View(d + begin, n - begin); }
+
+    private:
+        T* d;
+        int32_t n;
+    };
+
+    // ---- host copy of the assembled Hessian (BlockedSparseMatrix<3,3,double> subset) ------
+    struct Triplet
+    {
+        int32_t r = 0, c = 0;
+        double v = 0.0;
+        int32_t row() const { return r; }
+        int32_t col() const { return c; }
+        double value() const { return v; }
+    };
+    class BlockedSparseMatrix
+    {
+    public:
+        int32_t n_block_rows = 0;
+        std::vector<int64_t> rows;  // block-row offsets (n_block_rows + 1)
+        std::vector<int32_t> cols;  // block column of each 3x3 block
+        std::vector<double> vals;   // 9 row-major values per block
+        void to_triplets(std::vector<Triplet>& triplets) const;
+        std::vector<double> to_dense() const;  // row-major (3n x 3n)
+        void spmxv(const double* x, double* y) const;
+    };
+    using spBSM = std::shared_ptr<BlockedSparseMatrix>;
+
+    // Element Hessians of the last evaluation, resident on the device (ElementHessians.h).
+    class ElementHessians
+    {
+    public:
+        ElementHessians(symx_hip_problem p, symx_hip_ctx c) : prob(p), ctx(c) {}
+        // project_to_PD.cpp semantics on every element Hessian; returns the number changed
+        int64_t project_to_PD_inplace__all(double projection_eps, bool use_mirroring);
+        // FP64 assembly on the device, copied to the host (n_threads is unused)
+        spBSM assemble_global(int n_threads, int ndofs);
+
+    private:
+        symx_hip_problem prob;
+        symx_hip_ctx ctx;
+    };
+    using spElementHessians = std::shared_ptr<ElementHessians>;
+
+    // SecondOrderCompiledGlobal.h: evaluation of a GlobalPotential at the DoFs and data
+    // currently in the user's arrays (uploaded before each evaluation).
+    class SecondOrderCompiledGlobal
+    {
+    public:
+        SecondOrderCompiledGlobal(spGlobalPotential G, spContext context);
+        void evaluate_P(double& out_P);
+        void evaluate_P__dP_du(double& out_P, std::vector<double>& out_dP_du);
+        spElementHessians evaluate_P__dP_du__local_d2P_du2(double& out_P, std::vector<double>& out_dP_du);
+        // Central differences of E -> gradient and gradient -> Hessian against the
+        // compiled derivatives; returns the larger max abs difference
+        // (SecondOrderCompiledGlobal.cpp:144-191). Dense: small problems only.
+        double test_derivatives_with_FD(double h);
+        DeviceGlobal& device() { return *dev; }
+
+    private:
+        void upload();
+        spGlobalPotential G;
+        spContext context;
+        std::unique_ptr<DeviceGlobal> dev;
+    };
+
+    // CompiledInLoop<double> (CompiledInLoop.h:100-135): per-element outputs of a
+    // mapped workspace evaluated on the device, then handed to a host callback per
+    // element in element order (thread_id 0; n_threads is unused).
+    template <typename T>
+    class CompiledInLoop;
+    template <>
+    class CompiledInLoop<double>
+    {
+    public:
+        using Callback = std::function<void(View<double> solution, int32_t element_idx, int32_t thread_id, View<int32_t> element_conn)>;
+        CompiledInLoop(const spMWS<double>& mws, const std::vector<Scalar>& expr, const std::string& name, const std::string& folder = "",
+                       const std::string& cache_id = "");
+        ~CompiledInLoop();
+        static std::shared_ptr<CompiledInLoop<double>> create(const spMWS<double>& mws, const std::vector<Scalar>& expr, const std::string& name,
+                                                              const std::string& folder = "", const std::string& cache_id = "")
+        {
+            return std::make_shared<CompiledInLoop<double>>(mws, expr, name, folder, cache_id);
+        }
+        void run(int32_t n_threads, const Callback& callback);
+        void run(int32_t n_threads, const Callback& callback, const std::vector<uint8_t>& is_element_active);
+        bool was_cached() const { return cached; }
+        int32_t get_n_outputs() const { return n_out; }
+        const std::string& source() const { return src; }
+
+    private:
+        spMWS<double> mws;
+        std::string kernel, src;
+        int32_t n_out = 0;
+        bool cached = false;
+        symx_hip_ctx ctx = nullptr;
+        symx_hip_module mod = nullptr;
+        std::vector<double> out;
+    };
 
     class NewtonsMethod
     {

@@ -100,6 +100,7 @@ namespace symx_hip
         int64_t g_base = 0;                 // first gradient block (3 doubles)
         int64_t h_base = 0;                 // first Hessian block (9 doubles)
         int64_t part_base = 0;              // first energy partial
+        int64_t e_base = 0;                 // first element in problem-wide element numbering
         int32_t n_groups = 0;               // workgroups of the element kernel
     };
 }
@@ -120,6 +121,11 @@ struct symx_hip_problem_s
     double* d_Epart = nullptr;
     uint8_t* d_flags = nullptr;   // per-element scratch of the projection
     bool h_valid = false;
+    int64_t total_elem = 0;
+    // progressive projection: elements projected since the last EVAL_PGH, and the
+    // selection of the current call (problem-wide element numbering; allocated on first use)
+    uint8_t* d_projected = nullptr;
+    uint8_t* d_select = nullptr;
 
     // gradient scatter (CSR by DoF block of gradient-block ids)
     uint32_t* d_goff = nullptr;
@@ -147,7 +153,24 @@ namespace symx_hip
     void launch_sum(hipStream_t s, const double* d_in, int64_t n, double* d_out);
     void launch_grad_assemble(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* G, double* grad, int32_t nb);
     // flags: n_mats bytes of scratch (4-block elements that are not translation invariant)
+    // Elements of the active DoF blocks (max |grad| of the block >= threshold) that are not
+    // projected yet get select[e] = 1; the number of active blocks is added to n_active.
+    struct PotRanges
+    {
+        int32_t n = 0;
+        int32_t nb[16];
+        int64_t g_base[16];
+        int64_t g_end[16];
+        int64_t e_base[16];
+    };
+    void launch_ppn_select(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* grad, int32_t nblocks, double threshold,
+                           const PotRanges& pr, const uint8_t* projected, uint8_t* select, unsigned long long* n_active);
+    // projected |= select, select = 0; adds the number of newly projected elements to n_new.
+    void launch_ppn_commit(hipStream_t s, uint8_t* projected, uint8_t* select, int64_t n, unsigned long long* n_new);
+    void launch_fill_u8(hipStream_t s, uint8_t* p, int64_t n);
+    // select (may be NULL): only elements with select[e] != 0 are projected.
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+                          const uint8_t* select,
                           unsigned long long* d_count);
     int launch_project_dense(hipStream_t s, double* mats, int64_t n_mats, int32_t n, double eps, int32_t mirror, uint8_t* changed);
     void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* coff, const uint32_t* clist, const double* H,

@@ -269,13 +269,19 @@ namespace symx_hip
     // 4-block elements: reduced 9x9 path; elements that are not translation invariant
     // are flagged for k_project_ub_full.
     __global__ __launch_bounds__(64) void k_project_ub4_reduced(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
-                                                               unsigned long long* count, uint8_t* __restrict__ needs_full)
+                                                               unsigned long long* count, uint8_t* __restrict__ needs_full,
+                                                               const uint8_t* __restrict__ select)
     {
         const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
         bool changed = false;
         if (e < n_mats) {
-            const bool ok = project_element_reduced4(H + e * 90, eps, mirror != 0, changed);
-            needs_full[e] = ok ? 0 : 1;
+            if (select == nullptr || select[e]) {
+                const bool ok = project_element_reduced4(H + e * 90, eps, mirror != 0, changed);
+                needs_full[e] = ok ? 0 : 1;
+            }
+            else {
+                needs_full[e] = 0;
+            }
         }
         count_changed(changed, count);
     }
@@ -292,20 +298,84 @@ namespace symx_hip
     }
 
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
-                          unsigned long long* d_count)
+                          const uint8_t* select, unsigned long long* d_count)
     {
         if (n_mats <= 0) return 0;
         const unsigned g = (unsigned)((n_mats + 63) / 64);
         switch (nb) {
-        case 1: k_project_ub_full<1><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, nullptr); return 0;
-        case 2: k_project_ub_full<2><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, nullptr); return 0;
-        case 3: k_project_ub_full<3><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, nullptr); return 0;
+        case 1: k_project_ub_full<1><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 2: k_project_ub_full<2><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 3: k_project_ub_full<3><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
         case 4:
-            k_project_ub4_reduced<<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags);
+            k_project_ub4_reduced<<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags, select);
             k_project_ub_full<4><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags);
             return 0;
         default: return -1;
         }
+    }
+
+    // ---- progressive projection (NewtonsMethod.cpp:306-330, ElementHessians.cpp:79-165) ----
+    __global__ __launch_bounds__(NT) void k_ppn_select(const uint32_t* __restrict__ off, const uint32_t* __restrict__ list,
+                                                      const double* __restrict__ grad, int32_t nblocks, double threshold, PotRanges pr,
+                                                      const uint8_t* __restrict__ projected, uint8_t* __restrict__ select,
+                                                      unsigned long long* n_active)
+    {
+        const int64_t b = (int64_t)blockIdx.x * NT + threadIdx.x;
+        bool active = false;
+        if (b < nblocks) {
+            const double m = fmax(fabs(grad[3 * b]), fmax(fabs(grad[3 * b + 1]), fabs(grad[3 * b + 2])));
+            active = m >= threshold;
+            if (active) {
+                for (uint32_t k = off[b]; k < off[b + 1]; k++) {
+                    const int64_t gi = list[k];
+                    for (int q = 0; q < pr.n; q++) {
+                        if (gi >= pr.g_base[q] && gi < pr.g_end[q]) {
+                            const int64_t e = pr.e_base[q] + (gi - pr.g_base[q]) / pr.nb[q];
+                            if (!projected[e]) select[e] = 1;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+        const unsigned long long m = __ballot(active);
+        if ((threadIdx.x & 63) == 0 && m != 0ull) atomicAdd(n_active, (unsigned long long)__popcll(m));
+    }
+    void launch_ppn_select(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* grad, int32_t nblocks, double threshold,
+                           const PotRanges& pr, const uint8_t* projected, uint8_t* select, unsigned long long* n_active)
+    {
+        if (nblocks <= 0) return;
+        k_ppn_select<<<(unsigned)((nblocks + NT - 1) / NT), NT, 0, s>>>(off, list, grad, nblocks, threshold, pr, projected, select, n_active);
+    }
+
+    __global__ __launch_bounds__(NT) void k_ppn_commit(uint8_t* __restrict__ projected, uint8_t* __restrict__ select, int64_t n,
+                                                      unsigned long long* n_new)
+    {
+        const int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x;
+        bool fresh = false;
+        if (e < n) {
+            fresh = select[e] != 0;
+            if (fresh) projected[e] = 1;
+            select[e] = 0;
+        }
+        const unsigned long long m = __ballot(fresh);
+        if ((threadIdx.x & 63) == 0 && m != 0ull) atomicAdd(n_new, (unsigned long long)__popcll(m));
+    }
+    void launch_ppn_commit(hipStream_t s, uint8_t* projected, uint8_t* select, int64_t n, unsigned long long* n_new)
+    {
+        if (n <= 0) return;
+        k_ppn_commit<<<(unsigned)((n + NT - 1) / NT), NT, 0, s>>>(projected, select, n, n_new);
+    }
+
+    __global__ __launch_bounds__(NT) void k_fill_u8(uint8_t* __restrict__ p, int64_t n)
+    {
+        const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (i < n) p[i] = 0;
+    }
+    void launch_fill_u8(hipStream_t s, uint8_t* p, int64_t n)
+    {
+        if (n <= 0) return;
+        k_fill_u8<<<(unsigned)((n + NT - 1) / NT), NT, 0, s>>>(p, n);
     }
 
     template <int N>

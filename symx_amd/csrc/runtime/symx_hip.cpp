@@ -397,6 +397,8 @@ void symx_hip_problem_destroy(symx_hip_problem p)
     dfree(p->d_H);
     dfree(p->d_Epart);
     dfree(p->d_flags);
+    dfree(p->d_projected);
+    dfree(p->d_select);
     dfree(p->d_goff);
     dfree(p->d_glist);
     dfree(p->sell.d_slice_ptr);
@@ -507,12 +509,14 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     if (p->pots.empty()) return fail(ctx, SYMX_HIP_ERR_ARG, "problem without potentials");
     const int32_t nb = p->nb;
 
-    int64_t g = 0, h = 0, parts = 0;
+    int64_t g = 0, h = 0, parts = 0, elems = 0;
     uint64_t n_contrib = 0;
     for (auto& pot : p->pots) {
         pot.g_base = g;
         pot.h_base = h;
         pot.part_base = parts;
+        pot.e_base = elems;
+        elems += pot.n_elem;
         pot.n_groups = (pot.n_elem + SYMX_HIP_BLOCK - 1) / SYMX_HIP_BLOCK;
         g += (int64_t)pot.n_elem * pot.nb;
         h += (int64_t)pot.n_elem * pot.n_ub;
@@ -524,6 +528,7 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     p->total_g = g;
     p->total_h = h;
     p->total_parts = parts;
+    p->total_elem = elems;
 
     auto row_of = [&](const DevPotential& pot, int64_t e, int a) -> int32_t {
         return pot.dof_off[a] / 3 + pot.h_conn[(size_t)e * pot.stride + pot.dof_col[a]];
@@ -743,6 +748,7 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
         int rc = launch_eval(ctx, pot.module, k, prm);
         if (rc) return rc;
     }
+    if (mode == SYMX_HIP_EVAL_PGH && p->d_projected) launch_fill_u8(ctx->stream, p->d_projected, p->total_elem);
     launch_sum(ctx->stream, p->d_Epart, p->total_parts, ctx->d_scalars);
     if (mode >= SYMX_HIP_EVAL_PG) launch_grad_assemble(ctx->stream, p->d_goff, p->d_glist, p->d_G, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb);
     if (mode == SYMX_HIP_EVAL_PGH) p->h_valid = true;
@@ -782,7 +788,7 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
     unsigned long long* d_count = (unsigned long long*)(ctx->d_scalars + 8);
     HIP_TRY(ctx, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), ctx->stream));
     for (auto& pot : p->pots) {
-        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, d_count))
+        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, nullptr, d_count))
             return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports up to 4 DoF blocks per element");
     }
     HIP_TRY(ctx, hipGetLastError());
@@ -791,6 +797,49 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
     unsigned long long c;
     std::memcpy(&c, ctx->h_pinned, 8);
     if (n_changed) *n_changed = (int64_t)c;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_project_selective(symx_hip_problem p, double eps, int32_t mirror, double threshold, int64_t* n_changed,
+                                       int64_t* n_newly_projected, int64_t* n_active_blocks)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->h_valid) return fail(ctx, SYMX_HIP_ERR_ARG, "no element Hessians (evaluate with EVAL_PGH first)");
+    if (!(threshold >= 0.0)) return fail(ctx, SYMX_HIP_ERR_ARG, "threshold must be >= 0");
+    if (p->pots.size() > 16) return fail(ctx, SYMX_HIP_ERR_ARG, "selective projection supports up to 16 potentials");
+    if (!p->d_projected) {
+        int rc = dalloc(ctx, &p->d_projected, (size_t)std::max<int64_t>(p->total_elem, 1));
+        if (!rc) rc = dalloc(ctx, &p->d_select, (size_t)std::max<int64_t>(p->total_elem, 1));
+        if (rc) return rc;
+        // the Hessians of the current evaluation are all unprojected so far
+        launch_fill_u8(ctx->stream, p->d_projected, p->total_elem);
+        launch_fill_u8(ctx->stream, p->d_select, p->total_elem);
+    }
+    PotRanges pr;
+    pr.n = (int32_t)p->pots.size();
+    for (int32_t q = 0; q < pr.n; q++) {
+        const DevPotential& pot = p->pots[q];
+        if (pot.nb > 4) return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports up to 4 DoF blocks per element");
+        pr.nb[q] = pot.nb;
+        pr.g_base[q] = pot.g_base;
+        pr.g_end[q] = pot.g_base + (int64_t)pot.n_elem * pot.nb;
+        pr.e_base[q] = pot.e_base;
+    }
+    unsigned long long* cnt = (unsigned long long*)(ctx->d_scalars + 8);  // [changed, newly projected, active blocks]
+    HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    launch_ppn_select(ctx->stream, p->d_goff, p->d_glist, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb, threshold, pr, p->d_projected, p->d_select, cnt + 2);
+    for (auto& pot : p->pots)
+        launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_select + pot.e_base, cnt);
+    launch_ppn_commit(ctx->stream, p->d_projected, p->d_select, p->total_elem, cnt + 1);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long c[3];
+    std::memcpy(c, ctx->h_pinned, sizeof(c));
+    if (n_changed) *n_changed = (int64_t)c[0];
+    if (n_newly_projected) *n_newly_projected = (int64_t)c[1];
+    if (n_active_blocks) *n_active_blocks = (int64_t)c[2];
     return SYMX_HIP_OK;
 }
 
@@ -1080,6 +1129,71 @@ int symx_hip_problem_launch_adjoint(symx_hip_problem p, int32_t pot_id, const ch
     return SYMX_HIP_OK;
 }
 
+int symx_hip_loop_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const int32_t* conn, int32_t n_elem, int32_t conn_stride,
+                       const symx_hip_map_desc* maps, int32_t n_maps, const uint8_t* active, int32_t n_out, double* host_out)
+{
+    if (!ctx || !mod || !name || n_elem < 0 || conn_stride <= 0 || n_maps < 0 || n_maps > SYMX_HIP_MAX_MAPS || n_out <= 0 || !host_out ||
+        (n_elem > 0 && !conn))
+        return fail(ctx, SYMX_HIP_ERR_ARG, "bad loop arguments");
+    if (n_elem == 0) return SYMX_HIP_OK;
+    hipFunction_t f;
+    int rc = module_function(mod, name, &f);
+    if (rc) return rc;
+    // transposed connectivity [col][elem], as the element kernels read it
+    std::vector<int32_t> t((size_t)n_elem * conn_stride);
+    for (int64_t e = 0; e < n_elem; e++)
+        for (int32_t c = 0; c < conn_stride; c++) t[(size_t)c * n_elem + e] = conn[e * conn_stride + c];
+    int32_t* d_conn = nullptr;
+    uint8_t* d_active = nullptr;
+    double* d_out = nullptr;
+    std::vector<double*> d_maps(n_maps, nullptr);
+    auto cleanup = [&]() {
+        dfree(d_conn);
+        dfree(d_active);
+        dfree(d_out);
+        for (auto& m : d_maps) dfree(m);
+    };
+    rc = dalloc(ctx, &d_conn, t.size());
+    if (!rc && active) rc = dalloc(ctx, &d_active, (size_t)n_elem);
+    if (!rc) rc = dalloc(ctx, &d_out, (size_t)n_elem * n_out);
+    for (int32_t m = 0; !rc && m < n_maps; m++) {
+        if (maps[m].n_values < 0 || (maps[m].n_values > 0 && !maps[m].host)) rc = fail(ctx, SYMX_HIP_ERR_ARG, "bad data map");
+        else rc = dalloc(ctx, &d_maps[m], (size_t)std::max<int64_t>(maps[m].n_values, 1));
+    }
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpyAsync(d_conn, t.data(), sizeof(int32_t) * t.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (!rc && e == hipSuccess && active) e = hipMemcpyAsync(d_active, active, (size_t)n_elem, hipMemcpyHostToDevice, ctx->stream);
+    for (int32_t m = 0; !rc && e == hipSuccess && m < n_maps; m++)
+        if (maps[m].n_values > 0)
+            e = hipMemcpyAsync(d_maps[m], maps[m].host, sizeof(double) * maps[m].n_values, hipMemcpyHostToDevice, ctx->stream);
+    if (!rc && e == hipSuccess) {
+        struct
+        {
+            symx_hip_eval_params prm;
+            symx_hip_adjoint_params adj;
+        } args;
+        std::memset(&args, 0, sizeof(args));
+        args.prm.conn = d_conn;
+        args.prm.active = d_active;
+        for (int32_t m = 0; m < n_maps; m++) args.prm.maps[m] = d_maps[m];
+        args.prm.n_elem = n_elem;
+        args.adj.out = d_out;
+        args.adj.n_out = n_out;
+        size_t sz = sizeof(args);
+        void* config[] = { HIP_LAUNCH_PARAM_BUFFER_POINTER, &args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END };
+        const unsigned grid = (unsigned)((n_elem + SYMX_HIP_BLOCK - 1) / SYMX_HIP_BLOCK);
+        e = hipModuleLaunchKernel(f, grid, 1, 1, SYMX_HIP_BLOCK, 1, 1, 0, ctx->stream, nullptr, config);
+        if (e == hipSuccess) e = hipMemcpyAsync(host_out, d_out, sizeof(double) * (size_t)n_elem * n_out, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
+    if (!rc && e != hipSuccess) {
+        (void)hipStreamSynchronize(ctx->stream);
+        rc = fail(ctx, SYMX_HIP_ERR_HIP, std::string("loop evaluation: ") + hipGetErrorString(e));
+    }
+    cleanup();
+    return rc;
+}
+
 int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, int32_t arg, int32_t reps, double* avg_ms)
 {
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
@@ -1106,8 +1220,10 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
         case 4: return launch_eval(ctx, pot.module, pot.kP, prm);
         case 1: {
             // project a fresh copy of the element Hessians each repetition
-            hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice, ctx->stream);
-            return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, nullptr) ? SYMX_HIP_ERR_ARG : 0;
+            if (hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice,
+                               ctx->stream) != hipSuccess)
+                return SYMX_HIP_ERR_HIP;
+            return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, nullptr, nullptr) ? SYMX_HIP_ERR_ARG : 0;
         }
         case 2:
             launch_bsr_assemble(ctx->stream, S, p->d_coff, p->d_clist, p->d_H, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
@@ -1150,29 +1266,32 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
     hipEvent_t e0, e1;
     HIP_TRY(ctx, hipEventCreate(&e0));
     HIP_TRY(ctx, hipEventCreate(&e1));
+    auto chk = [&](hipError_t e) {
+        if (e != hipSuccess && !rc) rc = fail(ctx, SYMX_HIP_ERR_HIP, std::string("profile: ") + hipGetErrorString(e));
+    };
     float total_ms = 0.0f;
     if (phase == 1) {
         // time projection launches only (copies excluded): one event pair per repetition
         for (int i = 0; i < reps && !rc; i++) {
-            hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice, ctx->stream);
-            hipEventRecord(e0, ctx->stream);
-            launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, nullptr);
-            hipEventRecord(e1, ctx->stream);
-            hipEventSynchronize(e1);
+            chk(hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice, ctx->stream));
+            chk(hipEventRecord(e0, ctx->stream));
+            launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, nullptr, nullptr);
+            chk(hipEventRecord(e1, ctx->stream));
+            chk(hipEventSynchronize(e1));
             float ms = 0.0f;
-            hipEventElapsedTime(&ms, e0, e1);
+            chk(hipEventElapsedTime(&ms, e0, e1));
             total_ms += ms;
         }
     }
     else {
-        hipEventRecord(e0, ctx->stream);
+        chk(hipEventRecord(e0, ctx->stream));
         for (int i = 1; i <= reps && !rc; i++) rc = launch_once(i);
-        hipEventRecord(e1, ctx->stream);
-        hipEventSynchronize(e1);
-        hipEventElapsedTime(&total_ms, e0, e1);
+        chk(hipEventRecord(e1, ctx->stream));
+        chk(hipEventSynchronize(e1));
+        chk(hipEventElapsedTime(&total_ms, e0, e1));
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    chk(hipEventDestroy(e0));
+    chk(hipEventDestroy(e1));
     dfree(scratch);
     HIP_TRY(ctx, hipGetLastError());
     if (rc) return rc;

@@ -104,8 +104,9 @@ struct symx_scene_s
 static void apply_newton_params(NewtonSettings& s, const symx_newton_params& p)
 {
     s.precision = p.precision == SYMX_HIP_MIXED ? Precision::Mixed : Precision::FP64;
-    s.projection_mode = p.projection_mode == 0 ? ProjectionToPD::Newton
+    s.projection_mode = p.projection_mode == 0   ? ProjectionToPD::Newton
                         : p.projection_mode == 2 ? ProjectionToPD::ProjectOnDemand
+                        : p.projection_mode == 3 ? ProjectionToPD::Progressive
                                                  : ProjectionToPD::ProjectedNewton;
     s.project_to_pd_use_mirroring = p.use_mirroring != 0;
     s.max_iterations = p.max_iterations;

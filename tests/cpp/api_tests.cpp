@@ -1,0 +1,263 @@
+// C++ API tests of the front end on the GPU (built by symx_amd/Makefile as
+// symx_amd/lib/api_tests, run by tests/test_cpp_api.py). They exercise the user-facing
+// SymX surface the way the reference's own integration tests do:
+//   stable_neo_hookean   tests/test_global_potential.cpp:30-182 (FD check of the compiled
+//                        derivatives, DoF manipulation, assembled E/grad/Hessian against
+//                        a per-element CompiledInLoop sum)
+//   conditional          tests/test_global_potential.cpp:184-271 (two quadratic
+//                        potentials, one conditional, against the closed form)
+//   polynomial_newton    tests/test_newton_method.cpp:10-49
+//   compiled_ops         tests/test_compilation.cpp:16-94 (compiled vs host evaluation)
+// Each case prints "PASS <name>" or "FAIL <name>: <reason>"; the exit status is the
+// number of failed cases.
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "solver.h"
+
+using namespace symx;
+
+namespace
+{
+    struct Failure
+    {
+        std::string what;
+    };
+    void require(bool ok, const std::string& what)
+    {
+        if (!ok) throw Failure{ what };
+    }
+    double norm(const std::vector<double>& a)
+    {
+        double s = 0.0;
+        for (double v : a) s += v * v;
+        return std::sqrt(s);
+    }
+    // Eigen's isApprox: |a - b| <= prec * min(|a|, |b|)
+    bool is_approx(const std::vector<double>& a, const std::vector<double>& b, double prec)
+    {
+        if (a.size() != b.size()) return false;
+        std::vector<double> d(a.size());
+        for (size_t i = 0; i < a.size(); i++) d[i] = a[i] - b[i];
+        return norm(d) <= prec * std::min(norm(a), norm(b));
+    }
+    bool approx(double a, double b, double tol) { return std::abs(a - b) <= tol * std::max(1.0, std::max(std::abs(a), std::abs(b))); }
+
+    spContext quiet_context()
+    {
+        spContext c = Context::create();
+        c->n_threads = 1;
+        c->output->set_enabled(false);
+        return c;
+    }
+
+    void stable_neo_hookean()
+    {
+        const int n_elements = 100;
+        std::vector<std::array<double, 3>> x1 = { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 1, 1, 1 } };
+        const std::vector<std::array<double, 3>> X = { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } };
+        const std::array<double, 2> lame = { 1e4, 0.3 };
+        const std::vector<std::array<int32_t, 5>> tets(n_elements, { 0, 0, 1, 2, 3 });
+        // rest Jacobian (columns X_i - X_0) and volume
+        std::vector<std::array<double, 9>> J(1);
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) J[0][3 * r + c] = X[c + 1][r] - X[0][r];
+        const double det = J[0][0] * (J[0][4] * J[0][8] - J[0][5] * J[0][7]) - J[0][1] * (J[0][3] * J[0][8] - J[0][5] * J[0][6]) +
+                           J[0][2] * (J[0][3] * J[0][7] - J[0][4] * J[0][6]);
+        const std::vector<double> volume = { std::abs(det) / 6.0 };
+
+        auto energy = [&](MappedWorkspace<double>& mws, Element& tet) {
+            Vector lm = mws.make_vector(lame);
+            Scalar vol = mws.make_scalar(volume, tet[0]);
+            Matrix Jm = mws.make_matrix(J, { 3, 3 }, tet[0]);
+            std::vector<Vector> x = mws.make_vectors(x1, tet.slice(1, 5));
+            Scalar E = lm[0];
+            Scalar nu = lm[1];
+            Scalar mu = E / (2.0 * (1.0 + nu));
+            Scalar lambda = (E * nu) / ((1.0 + nu) * (1.0 - 2.0 * nu));
+            Scalar mu_ = 4.0 / 3.0 * mu;
+            Scalar lambda_ = lambda + 5.0 / 6.0 * mu;
+            Scalar alpha = 1.0 + mu_ / lambda_ - mu_ / (4.0 * lambda_);
+            Matrix j({ x[1][0] - x[0][0], x[2][0] - x[0][0], x[3][0] - x[0][0],
+                       x[1][1] - x[0][1], x[2][1] - x[0][1], x[3][1] - x[0][1],
+                       x[1][2] - x[0][2], x[2][2] - x[0][2], x[3][2] - x[0][2] }, { 3, 3 });
+            Matrix F = j * Jm.inv();
+            Scalar detF = F.det();
+            Scalar Ic = F.frobenius_norm_sq();
+            Scalar psi = 0.5 * mu_ * (Ic - 3.0) + 0.5 * lambda_ * (detF - alpha).powN(2) - 0.5 * mu_ * log(Ic + 1.0);
+            return vol * psi;
+        };
+
+        spGlobalPotential G = GlobalPotential::create();
+        G->add_potential("stable_neo_hookean_tet4", tets, [&](MappedWorkspace<double>& mws, Element& tet) { return energy(mws, tet); });
+        G->add_dof(x1);
+        spContext context = quiet_context();
+        SecondOrderCompiledGlobal compiled(G, context);
+        const double fd = compiled.test_derivatives_with_FD(1e-7);
+        require(fd < 0.002, "FD check " + std::to_string(fd));
+
+        // DoF manipulation
+        std::vector<double> u(12, 1.0), du(12, 1.0);
+        G->get_dofs(u.data());
+        for (int i = 0; i < 12; i++) u[i] += du[i];
+        require(x1[0][0] == 0.0, "x before set_dofs");
+        G->set_dofs(u.data());
+        require(x1[0][0] == 1.0, "x after set_dofs");
+        for (double& v : du) v = -v;
+        G->apply_dof_increment(du.data());
+        require(x1[0][0] == 0.0, "x after apply_dof_increment");
+
+        double P = 0.0;
+        std::vector<double> grad;
+        spElementHessians eh = compiled.evaluate_P__dP_du__local_d2P_du2(P, grad);
+        const std::vector<double> H = eh->assemble_global(1, 12)->to_dense();
+
+        // the same expression through a per-element loop
+        auto [mws, conn] = MappedWorkspace<double>::create(tets);
+        Element tet = conn;
+        Scalar e = energy(*mws, tet);
+        std::vector<Scalar> dofs;
+        for (const DataMap& m : mws->get_maps())
+            if (m.id() == reinterpret_cast<std::uintptr_t>(&x1))
+                for (int k = 0; k < 3; k++) dofs.push_back(Scalar(mws->workspace().graph(), mws->workspace().graph()->symbol_node(m.first_symbol_idx + k)));
+        std::vector<Scalar> vals = value_gradient_hessian(e, dofs);
+        CompiledInLoop<double> loop(mws, vals, "snh_loop");
+        double sum_E = 0.0;
+        std::vector<double> sum_g(12, 0.0), sum_H(144, 0.0);
+        loop.run(1, [&](View<double> s, int32_t, int32_t, View<int32_t>) {
+            sum_E += s[0];
+            for (int i = 0; i < 12; i++) sum_g[i] += s[1 + i];
+            for (int i = 0; i < 144; i++) sum_H[i] += s[13 + i];
+        });
+        require(approx(P, sum_E, 1e-8), "energy vs loop sum");
+        require(is_approx(grad, sum_g, 1e-8), "gradient vs loop sum");
+        require(is_approx(H, sum_H, 1e-8), "Hessian vs loop sum");
+    }
+
+    void conditional()
+    {
+        const int n = 100;
+        const double k_ = 1.0;
+        std::mt19937_64 rng(7);
+        std::uniform_real_distribution<double> U(-1.0, 1.0);
+        std::vector<std::array<double, 3>> x(n);
+        std::vector<std::array<int32_t, 1>> conn(n);
+        for (int i = 0; i < n; i++) {
+            x[i] = { U(rng), U(rng), U(rng) };
+            conn[i] = { i };
+        }
+        spGlobalPotential G = GlobalPotential::create();
+        G->add_potential("attract_to_zero", conn, [&](MappedWorkspace<double>& mws, Element& el) {
+            Vector xv = mws.make_vector(x, el[0]);
+            Scalar k = mws.make_scalar(k_);
+            return 0.5 * k * xv.squared_norm();
+        });
+        G->add_potential("attract_to_zero_if_z_plus", conn, [&](MappedWorkspace<double>& mws, Element& el) -> std::pair<Scalar, Scalar> {
+            Vector xv = mws.make_vector(x, el[0]);
+            Scalar k = mws.make_scalar(k_);
+            return { 0.5 * k * xv.squared_norm(), xv[2] > 0.0 };
+        });
+        G->add_dof(x);
+        spContext context = quiet_context();
+        SecondOrderCompiledGlobal compiled(G, context);
+        const double fd = compiled.test_derivatives_with_FD(1e-5);
+        require(fd < 1e-5, "FD check " + std::to_string(fd));
+        double P_s = 0.0;
+        std::vector<double> g_s;
+        const std::vector<double> H_s = compiled.evaluate_P__dP_du__local_d2P_du2(P_s, g_s)->assemble_global(1, 3 * n)->to_dense();
+        double P = 0.0;
+        std::vector<double> g(3 * n, 0.0), H((size_t)9 * n * n, 0.0);
+        for (int i = 0; i < n; i++) {
+            const int reps = x[i][2] > 0.0 ? 2 : 1;
+            for (int r = 0; r < reps; r++) {
+                P += 0.5 * k_ * (x[i][0] * x[i][0] + x[i][1] * x[i][1] + x[i][2] * x[i][2]);
+                for (int d = 0; d < 3; d++) {
+                    g[3 * i + d] += k_ * x[i][d];
+                    H[(size_t)(3 * i + d) * 3 * n + 3 * i + d] += k_;
+                }
+            }
+        }
+        require(approx(P_s, P, 1e-8), "energy");
+        require(is_approx(g_s, g, 1e-8), "gradient");
+        require(is_approx(H_s, H, 1e-8), "Hessian");
+    }
+
+    void polynomial_newton()
+    {
+        std::vector<std::array<int32_t, 1>> conn = { { 0 } };
+        std::vector<std::array<double, 3>> target = { { 1.0, 2.0, 3.0 } };
+        std::vector<std::array<double, 3>> x = { { -1.0, -1.0, -1.0 } };
+        spGlobalPotential G = GlobalPotential::create();
+        G->add_potential("Polynomial", conn, [&](MappedWorkspace<double>& mws, Element& el) {
+            Vector xv = mws.make_vector(x, el[0]);
+            Vector t = mws.make_vector(target, el[0]);
+            return (xv - t).squared_norm().powN(2);
+        });
+        G->add_dof(x);
+        spContext context = quiet_context();
+        NewtonsMethod newton(G, context);
+        const SolverReturn ret = newton.solve();
+        require(ret == SolverReturn::Successful, std::string("solver returned ") + to_string(ret));
+        for (int d = 0; d < 3; d++) require(std::abs(x[0][d] - target[0][d]) <= 1e-1, "x far from the target");
+    }
+
+    void compiled_ops()
+    {
+        Workspace ws;
+        Scalar a = ws.make_scalar();
+        Scalar b = ws.make_scalar();
+        std::vector<Scalar> exprs = { a + b, a - b, a * b, a / b, sqrt(a), exp(b), log(a), sin(a), cos(b), tan(a), asin(a / 4.0), acos(b / 4.0),
+                                      atan(a), a.powN(3), a.powF(2.5), branch(a - b, a, b), branch(b - a, a, b), log(a - 2.0) };
+        Compiled<double> c(exprs, "api_ops");
+        const double av = 1.3, bv = 0.7;
+        c.set(a, av);
+        c.set(b, bv);
+        const std::vector<double> expect = { av + bv, av - bv, av * bv, av / bv, std::sqrt(av), std::exp(bv), std::log(av), std::sin(av),
+                                             std::cos(bv), std::tan(av), std::asin(av / 4.0), std::acos(bv / 4.0), std::atan(av),
+                                             av * av * av, std::pow(av, 2.5), av, bv, -INFINITY };
+        std::vector<double> got;
+        try {
+            got = c.run();
+        }
+        catch (const std::exception&) {
+            // run() rejects NaN; -inf is allowed
+            throw;
+        }
+        for (size_t i = 0; i < exprs.size(); i++) {
+            const bool ok = std::isinf(expect[i]) ? (std::isinf(got[i]) && got[i] < 0) : approx(got[i], expect[i], 1e-12);
+            require(ok, "output " + std::to_string(i) + ": " + std::to_string(got[i]) + " vs " + std::to_string(expect[i]));
+        }
+    }
+}
+
+int main()
+{
+    const std::vector<std::pair<std::string, std::function<void()>>> cases = {
+        { "stable_neo_hookean", stable_neo_hookean },
+        { "conditional", conditional },
+        { "polynomial_newton", polynomial_newton },
+        { "compiled_ops", compiled_ops },
+    };
+    int failed = 0;
+    for (const auto& [name, f] : cases) {
+        try {
+            f();
+            std::printf("PASS %s\n", name.c_str());
+        }
+        catch (const Failure& e) {
+            std::printf("FAIL %s: %s\n", name.c_str(), e.what.c_str());
+            failed++;
+        }
+        catch (const std::exception& e) {
+            std::printf("FAIL %s: exception %s\n", name.c_str(), e.what());
+            failed++;
+        }
+        std::fflush(stdout);
+    }
+    return failed;
+}

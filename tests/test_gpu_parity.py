@@ -221,6 +221,24 @@ def test_newton_fp64_matches_oracle_trajectory(gpu, oracle):
     assert np.abs(s.get("x") - res["x"]).max() <= 1e-7 * np.abs(res["x"] - S.X).max()
 
 
+@pytest.mark.parametrize("mode", [0, 2, 3])
+@pytest.mark.parametrize("dt", [0.01, 0.1])
+def test_projection_modes_match_oracle(gpu, oracle, mode, dt):
+    """Newton (no projection; fails on the indefinite first system like the oracle),
+    ProjectOnDemand and Progressive projection (NewtonsMethod.cpp:254-386) against the
+    oracle's restatement of the same retry logic."""
+    s = make_scene((6, 6, 6), seed=11, dt=dt)
+    s.set_newton(symx_amd.default_newton_params(projection_mode=mode))
+    S = scene_arrays(s)
+    st = s.solve()
+    res = oracle.newton(S, s.newton)
+    assert st["result"] == res["result"], (st["result_name"], res["result"])
+    assert st["newton_iterations"] == res["newton_iterations"]
+    assert abs(st["cg_iterations"] - res["cg_iterations"]) <= max(2, st["newton_iterations"])
+    if res["result"] == 0:
+        assert np.abs(s.get("x") - res["x"]).max() <= 1e-7 * np.abs(res["x"] - S.X).max()
+
+
 def test_mixed_newton_reaches_fp64_solution(gpu, oracle):
     tight = dict(residual_tolerance_rel=1e-8, cg_rel_tolerance=1e-6)
     s64 = make_scene((6, 6, 6))
