@@ -232,6 +232,92 @@ def bench_adjoint(args, log):
     scene.close()
 
 
+def bench_contact(args, log):
+    """Ball drop with contact: Tet4 Neo-Hookean ball of radius 0.5 (jittered-grid
+    tetrahedralisation, 50 cells across, seed 7: 393,000 tets) dropped at 2 m/s from 1 cm
+    above the plane z = 0; IPC log-barrier per surface vertex (dhat 1e-3, kappa 1e6),
+    plane step filter (eta 0.9), backward Euler dt = 1/200, Newton with Progressive
+    projection (the projection mode of SymX's contact example) to 1e-5 relative residual
+    (1e-6 is below the round-off floor of the energy-based line search here) with
+    mixed-precision PCG (FP64 barrier energies, gradients and line search). One step = one
+    time step; the workload is `steps` time steps (50 for the reference configuration)."""
+    import symx_amd
+
+    p = symx_amd.default_sphere_params()
+    newton = symx_amd.default_newton_params(precision=symx_amd.MIXED, projection_mode=3, residual_tolerance_rel=1e-5)
+    t0 = time.perf_counter()
+    s = symx_amd.Sphere(p, newton)
+    log(f"setup: {s.n_elements} tets, {s.n_vertices} vertices, {len(s.contact_vertices())} contact vertices in {time.perf_counter() - t0:.1f}s")
+    for i in range(args.warmup):
+        st = s.solve()  # a solve without advancing time
+        log(f"warmup solve {i}: {st['result_name']} newton={st['newton_iterations']} {st['seconds']:.3f}s")
+    newton_it = cg = 0
+    results = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = s.step()
+        newton_it += st["newton_iterations"]
+        cg += st["cg_iterations"]
+        results.append(st["result_name"])
+        if i % 10 == 0 or st["result"] != 0:
+            log(f"step {i}: {st['result_name']} newton={st['newton_iterations']} cg={st['cg_iterations']} bt={st['ls_bt_iterations']} "
+                f"zmin={s.get('x')[:, 2].min():.3e} {st['seconds']:.3f}s")
+    t1 = time.perf_counter()
+    x = s.get("x")
+    log(f"{args.steps} steps in {t1 - t0:.3f}s: newton={newton_it} cg={cg} zmin={x[:, 2].min():.3e} vz={s.get('v')[:, 2].mean():.3f}")
+    import ctypes
+
+    prof = profile_kernels(s, symx_amd.MIXED)
+    nnzb_c = ctypes.c_int64()
+    symx_amd.check_hip(symx_amd.hip().symx_hip_problem_export_bsr(s.problem, ctypes.byref(nnzb_c), None, None, None))
+    ab = algorithmic_bytes(s.n_elements, s.n_vertices, nnzb_c.value, symx_amd.MIXED)
+    per_iter = {"element": prof["element"], "projection": prof["projection"], "assembly": prof["assembly"],
+                "pcg_iteration": prof["pcg_iteration"] * cg / max(newton_it, 1)}
+    dominant = max(per_iter, key=per_iter.get)
+    unit_bytes = ab[dominant] * (s.n_elements if dominant in ("element", "projection") else 1)
+    achieved = unit_bytes / (prof[dominant] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "kernel": dominant, "avg_ms": round(prof[dominant], 4), "algorithmic_bytes_per_launch": int(unit_bytes)}
+    cpu = None
+    if not args.no_cpu:
+        from oracle_py import Oracle, sphere_arrays
+
+        orc = Oracle()
+        s2 = symx_amd.Sphere(p, symx_amd.default_newton_params(precision=symx_amd.FP64))
+        S = sphere_arrays(s2)
+        s2.close()
+        tc = time.perf_counter()
+        r = orc.newton(S, symx_amd.default_newton_params(precision=symx_amd.FP64, projection_mode=3, residual_tolerance_rel=1e-5))
+        tcpu = time.perf_counter() - tc
+        log(f"cpu ({orc.num_threads()} threads): first time step {r['result']} newton={r['newton_iterations']} {tcpu:.2f}s")
+        cpu = {"value": round(1.0 / tcpu, 4), "unit": "time steps/s", "cores": orc.num_threads(), "kind": "port",
+               "sample": f"oracle/symx_oracle.c, the first time step of the same ball drop ({r['newton_iterations']} Newton iterations, {tcpu:.1f}s)"}
+    line = {
+        "metric": "time steps/s, ball drop with IPC contact",
+        "value": round(args.steps / (t1 - t0), 4),
+        "unit": "time steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * (t1 - t0) / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 Hessian/BSR/PCG + f64 refinement, f64 energies",
+        "data": "synthetic",
+        "config": {"workload": f"Tet4 Neo-Hookean ball r=0.5 ({s.n_elements} tets) dropped at 2 m/s on z=0, IPC barrier dhat=1e-3 kappa=1e6, "
+                               "step filter 0.9, dt=1/200, Progressive projection, 1e-5 rel. residual", "n_elem": s.n_elements, "n_dof": 3 * s.n_vertices, "nnzb": nnzb_c.value},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "newton_iterations": newton_it,
+        "cg_iterations": cg,
+        "results": sorted(set(results)),
+        "kernel_ms": {k: round(v, 4) for k, v in prof.items()},
+    }
+    print(json.dumps(line), flush=True)
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,8 +327,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cells", type=int, default=40)
-    ap.add_argument("--workload", choices=["block", "cantilever_adjoint"], default="block",
-                    help="block: the headline Newton benchmark; cantilever_adjoint: adjoint gradients/s")
+    ap.add_argument("--workload", choices=["block", "cantilever_adjoint", "sphere_contact"], default="block",
+                    help="block: the headline Newton benchmark; cantilever_adjoint: adjoint gradients/s; sphere_contact: ball drop")
     ap.add_argument("--adjoint-precision", choices=["fp64", "mixed"], default="fp64")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -258,6 +344,9 @@ def main():
         raise RuntimeError("bench.py needs a GPU")
     if args.workload == "cantilever_adjoint":
         bench_adjoint(args, log)
+        return
+    if args.workload == "sphere_contact":
+        bench_contact(args, log)
         return
     cells = args.cells
     t0 = time.perf_counter()

@@ -242,6 +242,14 @@ typedef struct symx_hip_adjoint_params {
  * to host_out. */
 int symx_hip_problem_launch_adjoint(symx_hip_problem prob, int32_t pot_id, const char* name, int32_t n_out, double* host_out);
 
+/* Step filter against a plane (the max_allowed_step callback of SymX's line search,
+ * solver_utils.h:74 and NewtonsMethod.cpp:494-506, for contact with x[axis] = plane):
+ * set_plane_filter stores the DoF blocks (vertices) to watch; plane_max_step returns
+ * t = min(1, min over watched blocks with du[axis] < 0 of eta * (x[axis] - plane) / -du[axis])
+ * from VEC_DOFS and VEC_DU, so a step of t * du keeps every watched vertex on its side. */
+int symx_hip_problem_set_plane_filter(symx_hip_problem prob, const int32_t* blocks, int32_t n_blocks, int32_t axis, double plane);
+int symx_hip_problem_plane_max_step(symx_hip_problem prob, double eta, double* t_max);
+
 /* Stateless per-element evaluation (CompiledInLoop, CompiledInLoop_run.h:66-472):
  * uploads the connectivity conn [n_elem][conn_stride] (host, row-major) and the data
  * maps (host pointers, maps[m].n_values values each; is_dof is ignored), launches the

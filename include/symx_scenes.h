@@ -36,6 +36,27 @@ typedef struct symx_tet4_block_params {
     int32_t log_mu_field;    /* 1: per-tet mu = exp(log_mu[e]) parameters (see symx_scene_set_log_mu) */
 } symx_tet4_block_params;
 
+/* Ball dropped on the ground plane z = 0 (contact scene): Tet4 jittered-grid
+ * tetrahedralisation, backward-Euler inertia, IPC log-barrier per surface vertex and
+ * the plane step filter in the line search. */
+typedef struct symx_sphere_params {
+    double radius;
+    int32_t cells;             /* grid cells across the bounding cube (50: about 390k tets) */
+    double jitter;             /* vertex jitter, fraction of the cell width (< 0.25) */
+    uint64_t seed;
+    double center[3];
+    double velocity[3];        /* initial velocity of every vertex */
+    double youngs_modulus;
+    double poissons_ratio;
+    double density;
+    double gravity[3];
+    double dt;
+    int32_t material;          /* 0: Neo-Hookean, 1: stable Neo-Hookean */
+    double contact_stiffness;  /* barrier kappa (0: no contact) */
+    double dhat;               /* barrier activation distance */
+    double step_filter;        /* eta of the plane step filter (0: off) */
+} symx_sphere_params;
+
 /* Newton settings exposed to callers (subset of NewtonSettings, solver_utils.h:173-258). */
 typedef struct symx_newton_params {
     int32_t precision;               /* SYMX_HIP_FP64 or SYMX_HIP_MIXED */
@@ -89,6 +110,11 @@ int symx_scene_set_log_mu(symx_scene s, const double* log_mu);
 int symx_scene_set_newton_params(symx_scene s, const symx_newton_params* np);
 /* Newton solve of the current time step (does not advance time). */
 int symx_scene_solve(symx_scene s, symx_solve_stats* st);
+void symx_sphere_params_default(symx_sphere_params* p);
+int symx_scene_sphere_create(const symx_sphere_params* p, const symx_newton_params* np, symx_scene* out);
+/* surface (contact) vertices of a sphere scene: *n receives the count; out may be NULL */
+int symx_scene_contact_vertices(symx_scene s, int32_t* n, int32_t* out);
+
 /* Adjoint of the quasistatic equilibrium in x (scenes with log_mu_field = 1):
  * solves H lambda = dL_dx on the unprojected Hessian at the current positions and
  * writes dL_dlog_mu[e] = -lambda^T d^2E/dx dlog_mu_e (n_elements values). Solved in
@@ -130,6 +156,9 @@ int symx_host_jit_tet4(const char* cache_dir, int32_t material, int64_t* code_si
  * when log_mu_field = 1) and compile them with hiprtc for gfx950. */
 int symx_host_scene_jit(const symx_tet4_block_params* p, const char* cache_dir, int32_t* n_potentials, int64_t* code_bytes,
                         int64_t* ops_adjoint);
+/* Sphere mesh of symx_scene_sphere_create on the host: counts, and (when not NULL)
+ * rest positions [nv][3], tets [ne][4], contact vertices [nc]. */
+int symx_host_sphere_mesh(const symx_sphere_params* p, int32_t* nv, int32_t* ne, int32_t* nc, double* X, int32_t* tets, int32_t* contact);
 /* Evaluate the generated adjoint outputs d(lambda . dE_e/dx)/dlog_mu_e of element e of
  * a log_mu_field scene on the host (symbolic graph, kernel's gather rules); lambda
  * has one value per DoF. */

@@ -381,7 +381,22 @@ typedef struct oracle_scene {
     double lambda_over_mu;   /* used with mu_e */
     double lambda, mu, dt, k;
     double gravity[3];
+    /* ground contact: IPC log-barrier per contact vertex against z = 0, active below dhat,
+     * and the plane step filter with factor step_eta (0: off) */
+    int32_t ncontact;
+    int32_t pad_;
+    const int32_t* contact_vertex;
+    double kappa, dhat, step_eta;
 } oracle_scene;
+
+/* b(d) = -kappa (d - dhat)^2 ln(d / dhat) and its first two derivatives, 0 < d < dhat */
+static void barrier(double kappa, double dhat, double d, double* b, double* db, double* ddb)
+{
+    const double l = log(d / dhat), e = d - dhat;
+    *b = -kappa * e * e * l;
+    *db = -kappa * (2.0 * e * l + e * e / d);
+    *ddb = -kappa * (2.0 * l + 4.0 * e / d - e * e / (d * d));
+}
 
 typedef struct oracle_newton_params {
     int32_t precision;
@@ -538,6 +553,20 @@ static double scene_eval(const oracle_scene* S, const double* x, double* grad, d
                 if (grad) grad[3 * (int64_t)v + d] += -S->mass[v] * S->gravity[d];
             }
     }
+    double E_contact = 0.0;
+    for (int c = 0; c < S->ncontact; c++) {
+        const int64_t v = S->contact_vertex[c];
+        const double d = x[3 * v + 2];
+        if (!(d < S->dhat)) continue;
+        if (d <= 0.0) {
+            E_contact = INFINITY;
+            continue;
+        }
+        double b, db, ddb;
+        barrier(S->kappa, S->dhat, d, &b, &db, &ddb);
+        E_contact += b;
+        if (grad) grad[3 * v + 2] += db;
+    }
     double E_bc = 0.0;
     for (int i = 0; i < S->nbc; i++) {
         const int64_t v = S->bc_vertex[i];
@@ -555,7 +584,29 @@ static double scene_eval(const oracle_scene* S, const double* x, double* grad, d
                 for (int d = 0; d < 3; d++) grad[3 * (int64_t)S->tets[4 * (int64_t)e + a] + d] += ge[12 * (int64_t)e + 3 * a + d];
         free(ge);
     }
-    return E_el + E_in + E_bc;
+    return E_el + E_in + E_bc + E_contact;
+}
+
+/* Barrier Hessians diag(0, 0, b'') of the active contact vertices; projected ones (PD
+ * projection of the 3x3 element Hessian) get eps (clamp) or 0 (mirror) on the two zero
+ * eigenvalues. */
+static void add_barrier_hessians(const oracle_scene* S, const pattern* P, const double* x, const unsigned char* bproj, double eps, int mirror,
+                                 double* vals)
+{
+    for (int c = 0; c < S->ncontact; c++) {
+        const int64_t v = S->contact_vertex[c];
+        const double d = x[3 * v + 2];
+        if (!(d < S->dhat) || d <= 0.0) continue;
+        double b, db, ddb;
+        barrier(S->kappa, S->dhat, d, &b, &db, &ddb);
+        double* dst = vals + 9 * P->v_slot[v];
+        double h[3] = { 0.0, 0.0, ddb };
+        if (bproj && bproj[c]) {
+            for (int k = 0; k < 3; k++)
+                if (h[k] < eps) h[k] = mirror ? -h[k] : eps;
+        }
+        for (int k = 0; k < 3; k++) dst[4 * k] += h[k];
+    }
 }
 
 ORACLE_API double oracle_scene_energy(const oracle_scene* S, const double* x, double* grad) { return scene_eval(S, x, grad, NULL); }
@@ -617,6 +668,7 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
     double* He = (double*)malloc(sizeof(double) * 144 * (int64_t)(S->ne > 0 ? S->ne : 1));
     double* vals = (double*)malloc(sizeof(double) * 9 * P.nnzb);
     unsigned char* projected = (unsigned char*)malloc((size_t)(S->ne > 0 ? S->ne : 1));
+    unsigned char* bproj = (unsigned char*)malloc((size_t)(S->ncontact > 0 ? S->ncontact : 1));
     unsigned char* active = (unsigned char*)malloc((size_t)(S->nv > 0 ? S->nv : 1));
     int pdn_countdown = 0;
     double ppn_threshold = -1.0;
@@ -661,6 +713,7 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
          * :354-386). Modes: 0 Newton, 1 ProjectedNewton, 2 ProjectOnDemand, 3 Progressive
          * (default countdown 4, tightening 0.5, release 2). */
         memset(projected, 0, (size_t)(S->ne > 0 ? S->ne : 1));
+        memset(bproj, 0, (size_t)(S->ncontact > 0 ? S->ncontact : 1));
         double dug = 0.0;
         for (;;) {
             int all_projected = 0;
@@ -671,6 +724,7 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
                     if (!projected[e]) oracle_project(He + 144 * (int64_t)e, 12, np->projection_eps, np->use_mirroring);
                     projected[e] = 1;
                 }
+                memset(bproj, 1, (size_t)(S->ncontact > 0 ? S->ncontact : 1));
                 all_projected = 1;
             }
             else if (mode == 3 && ppn_threshold >= 0.0) {
@@ -690,9 +744,12 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
                         projected[e] = 1;
                     }
                 }
+                for (int c = 0; c < S->ncontact; c++)
+                    if (active[S->contact_vertex[c]]) bproj[c] = 1;
                 all_projected = n_active == S->nv;
             }
             assemble(S, &P, He, vals);
+            add_barrier_hessians(S, &P, S->x, bproj, np->projection_eps, np->use_mirroring, vals);
             st->seconds_eval += now_seconds() - t0;
             t0 = now_seconds();
             for (int64_t i = 0; i < n; i++) rhs[i] = -grad[i];
@@ -733,6 +790,18 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
             result = 0;
             break;
         }
+        /* plane step filter (max_allowed_step, NewtonsMethod.cpp:494-506) */
+        if (S->step_eta > 0.0) {
+            double t = 1.0;
+            for (int c = 0; c < S->ncontact; c++) {
+                const int64_t k = 3 * (int64_t)S->contact_vertex[c] + 2;
+                if (du[k] < 0.0) t = fmin(t, fmax(0.0, S->step_eta * S->x[k] / -du[k]));
+            }
+            if (t < 1.0) {
+                for (int64_t i = 0; i < n; i++) du[i] *= t;
+                dug *= t;
+            }
+        }
         memcpy(x_ls, S->x, sizeof(double) * n);
         double step = 1.0;
         for (int64_t i = 0; i < n; i++) S->x[i] = x_ls[i] + step * du[i];
@@ -755,6 +824,7 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
     st->result = result;
     st->newton_iterations = it;
     free(projected);
+    free(bproj);
     free(active);
     free_pattern(&P);
     free(grad);

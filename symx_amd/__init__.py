@@ -40,6 +40,13 @@ class Tet4BlockParams(Structure):
                 ("bc_stiffness", c_double), ("fixed_axis", c_int32), ("material", c_int32), ("log_mu_field", c_int32)]
 
 
+class SphereParams(Structure):
+    _fields_ = [("radius", c_double), ("cells", c_int32), ("jitter", c_double), ("seed", c_uint64), ("center", c_double * 3),
+                ("velocity", c_double * 3), ("youngs_modulus", c_double), ("poissons_ratio", c_double), ("density", c_double),
+                ("gravity", c_double * 3), ("dt", c_double), ("material", c_int32), ("contact_stiffness", c_double), ("dhat", c_double),
+                ("step_filter", c_double)]
+
+
 class NewtonParams(Structure):
     _fields_ = [("precision", c_int32), ("projection_mode", c_int32), ("use_mirroring", c_int32), ("max_iterations", c_int32),
                 ("residual_tolerance_abs", c_double), ("residual_tolerance_rel", c_double), ("cg_rel_tolerance", c_double),
@@ -102,6 +109,7 @@ def amd():
         lib.symx_scene_destroy.restype = None
         lib.symx_newton_params_default.restype = None
         lib.symx_tet4_block_params_default.restype = None
+        lib.symx_sphere_params_default.restype = None
         _amd = lib
     return _amd
 
@@ -134,6 +142,28 @@ def default_newton_params(**kw) -> NewtonParams:
     return p
 
 
+def default_sphere_params(**kw) -> SphereParams:
+    p = SphereParams()
+    amd().symx_sphere_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if k in ("center", "velocity", "gravity"):
+            getattr(p, k)[:] = v
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def host_sphere_mesh(params: SphereParams):
+    """(X [nv,3], tets [ne,4], contact vertices) of the sphere scene's mesh (no GPU)."""
+    nv, ne, nc = c_int32(), c_int32(), c_int32()
+    check_scene(amd().symx_host_sphere_mesh(ctypes.byref(params), ctypes.byref(nv), ctypes.byref(ne), ctypes.byref(nc), None, None, None))
+    X = np.zeros((nv.value, 3))
+    T = np.zeros((ne.value, 4), np.int32)
+    C = np.zeros(nc.value, np.int32)
+    check_scene(amd().symx_host_sphere_mesh(ctypes.byref(params), ctypes.byref(nv), ctypes.byref(ne), ctypes.byref(nc), _ptr(X), _ptr(T), _ptr(C)))
+    return X, T, C
+
+
 def default_block_params(**kw) -> Tet4BlockParams:
     p = Tet4BlockParams()
     amd().symx_tet4_block_params_default(ctypes.byref(p))
@@ -164,11 +194,16 @@ class Context:
 class Tet4Block:
     """Tet4 elastic block scene (include/symx_scenes.h) driven through the C++ front end."""
 
-    def __init__(self, params: Tet4BlockParams | None = None, newton: NewtonParams | None = None):
-        self.params = params or default_block_params()
+    def __init__(self, params: Tet4BlockParams | None = None, newton: NewtonParams | None = None, _sphere: SphereParams | None = None):
         self.newton = newton or default_newton_params()
         self.ptr = c_void_p()
-        check_scene(amd().symx_scene_tet4_block_create(ctypes.byref(self.params), ctypes.byref(self.newton), ctypes.byref(self.ptr)))
+        self.sphere = _sphere
+        if _sphere is not None:
+            self.params = None
+            check_scene(amd().symx_scene_sphere_create(ctypes.byref(_sphere), ctypes.byref(self.newton), ctypes.byref(self.ptr)))
+        else:
+            self.params = params or default_block_params()
+            check_scene(amd().symx_scene_tet4_block_create(ctypes.byref(self.params), ctypes.byref(self.newton), ctypes.byref(self.ptr)))
         n = c_int32()
         amd().symx_scene_n_vertices(self.ptr, ctypes.byref(n))
         self.n_vertices = n.value
@@ -201,6 +236,13 @@ class Tet4Block:
     def set(self, name: str, a: np.ndarray):
         a = np.ascontiguousarray(a, dtype=np.float64)
         check_scene(amd().symx_scene_set_array(self.ptr, name.encode(), _ptr(a)))
+
+    def contact_vertices(self) -> np.ndarray:
+        n = c_int32()
+        check_scene(amd().symx_scene_contact_vertices(self.ptr, ctypes.byref(n), None))
+        out = np.zeros(n.value, np.int32)
+        check_scene(amd().symx_scene_contact_vertices(self.ptr, ctypes.byref(n), _ptr(out)))
+        return out
 
     def set_log_mu(self, log_mu: np.ndarray):
         a = np.ascontiguousarray(log_mu, dtype=np.float64)
@@ -307,6 +349,13 @@ class Tet4Block:
         check_hip(hip().symx_hip_problem_solve(self._p(), c_int32(precision), c_double(abs_tol), c_double(rel_tol),
                                                c_int32(max_iter), c_int32(int(stop_on_indef)), ctypes.byref(info)))
         return {k: getattr(info, k) for k, _ in info._fields_}
+
+
+class Sphere(Tet4Block):
+    """Ball dropped on the ground plane (contact scene of include/symx_scenes.h)."""
+
+    def __init__(self, params: SphereParams | None = None, newton: NewtonParams | None = None):
+        super().__init__(None, newton, _sphere=params or default_sphere_params())
 
 
 def host_tet4_element(x12, X12, a, b, material=0):

@@ -687,6 +687,7 @@ namespace symx
             sync_dofs_to_host();
             for (auto& f : callbacks->max_allowed_step) max_step = std::min(max_step, f());
         }
+        for (auto& f : callbacks->max_allowed_step_device) max_step = std::min(max_step, f(prob));
         if (max_step < 1.0) {
             retraction *= max_step;
             check(symx_hip_problem_scale(prob, SYMX_HIP_VEC_DU, SYMX_HIP_VEC_DU, max_step), hip, "max step");

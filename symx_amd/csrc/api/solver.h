@@ -148,6 +148,10 @@ namespace symx
         void add_is_converged(std::function<bool()> f) { is_converged.push_back(f); }
         void add_is_converged_state_valid(std::function<bool()> f) { is_converged_state_valid.push_back(f); }
         void add_max_allowed_step(std::function<double()> f) { max_allowed_step.push_back(f); }
+        // Device-side variant (no counterpart in SymX): called with the device problem,
+        // whose VEC_DOFS holds the DoFs before the step and VEC_DU the (capped) step, e.g.
+        // symx_hip_problem_plane_max_step. No host synchronisation of the DoFs.
+        void add_max_allowed_step_device(std::function<double(symx_hip_problem)> f) { max_allowed_step_device.push_back(f); }
 
         std::vector<std::function<void()>> before_energy_evaluation;
         std::vector<std::function<bool()>> is_initial_state_valid;
@@ -157,6 +161,7 @@ namespace symx
         std::vector<std::function<bool()>> is_converged;
         std::vector<std::function<bool()>> is_converged_state_valid;
         std::vector<std::function<double()>> max_allowed_step;
+        std::vector<std::function<double(symx_hip_problem)>> max_allowed_step_device;
         std::function<double(std::vector<double>&)> residual;
         bool has_custom_residual = false;
         spContext context;

@@ -126,6 +126,11 @@ struct symx_hip_problem_s
     // selection of the current call (problem-wide element numbering; allocated on first use)
     uint8_t* d_projected = nullptr;
     uint8_t* d_select = nullptr;
+    // plane step filter
+    int32_t* d_filter_blocks = nullptr;
+    int32_t n_filter_blocks = 0;
+    int32_t filter_axis = 2;
+    double filter_plane = 0.0;
 
     // gradient scatter (CSR by DoF block of gradient-block ids)
     uint32_t* d_goff = nullptr;
@@ -162,12 +167,16 @@ namespace symx_hip
         int64_t g_base[16];
         int64_t g_end[16];
         int64_t e_base[16];
+        const uint8_t* active[16];  // activity flags of conditional potentials (NULL: all active)
     };
     void launch_ppn_select(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* grad, int32_t nblocks, double threshold,
                            const PotRanges& pr, const uint8_t* projected, uint8_t* select, unsigned long long* n_active);
     // projected |= select, select = 0; adds the number of newly projected elements to n_new.
     void launch_ppn_commit(hipStream_t s, uint8_t* projected, uint8_t* select, int64_t n, unsigned long long* n_new);
     void launch_fill_u8(hipStream_t s, uint8_t* p, int64_t n);
+    // out (one uint64, preset to the bits of 1.0) = min over blocks of the plane step bound
+    void launch_plane_max_step(hipStream_t s, const int32_t* blocks, int32_t n, const double* x, const double* du, int32_t axis, double plane,
+                               double eta, unsigned long long* out);
     // select (may be NULL): only elements with select[e] != 0 are projected.
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
                           const uint8_t* select,

@@ -330,8 +330,9 @@ namespace symx_hip
                     const int64_t gi = list[k];
                     for (int q = 0; q < pr.n; q++) {
                         if (gi >= pr.g_base[q] && gi < pr.g_end[q]) {
-                            const int64_t e = pr.e_base[q] + (gi - pr.g_base[q]) / pr.nb[q];
-                            if (!projected[e]) select[e] = 1;
+                            const int64_t le = (gi - pr.g_base[q]) / pr.nb[q];
+                            const int64_t e = pr.e_base[q] + le;
+                            if (!projected[e] && (pr.active[q] == nullptr || pr.active[q][le])) select[e] = 1;
                             break;
                         }
                     }
@@ -365,6 +366,28 @@ namespace symx_hip
     {
         if (n <= 0) return;
         k_ppn_commit<<<(unsigned)((n + NT - 1) / NT), NT, 0, s>>>(projected, select, n, n_new);
+    }
+
+    // Non-negative doubles order like their bit patterns, so the minimum is an integer atomicMin.
+    __global__ __launch_bounds__(NT) void k_plane_max_step(const int32_t* __restrict__ blocks, int32_t n, const double* __restrict__ x,
+                                                          const double* __restrict__ du, int32_t axis, double plane, double eta,
+                                                          unsigned long long* out)
+    {
+        const int32_t i = blockIdx.x * NT + threadIdx.x;
+        double t = 1.0;
+        if (i < n) {
+            const int64_t k = 3 * (int64_t)blocks[i] + axis;
+            const double d = du[k];
+            if (d < 0.0) t = fmin(1.0, fmax(0.0, eta * (x[k] - plane) / -d));
+        }
+        for (int off = 32; off > 0; off >>= 1) t = fmin(t, __shfl_xor(t, off, 64));
+        if ((threadIdx.x & 63) == 0 && t < 1.0) atomicMin(out, (unsigned long long)__double_as_longlong(t));
+    }
+    void launch_plane_max_step(hipStream_t s, const int32_t* blocks, int32_t n, const double* x, const double* du, int32_t axis, double plane,
+                               double eta, unsigned long long* out)
+    {
+        if (n <= 0) return;
+        k_plane_max_step<<<(unsigned)((n + NT - 1) / NT), NT, 0, s>>>(blocks, n, x, du, axis, plane, eta, out);
     }
 
     __global__ __launch_bounds__(NT) void k_fill_u8(uint8_t* __restrict__ p, int64_t n)

@@ -399,6 +399,7 @@ void symx_hip_problem_destroy(symx_hip_problem p)
     dfree(p->d_flags);
     dfree(p->d_projected);
     dfree(p->d_select);
+    dfree(p->d_filter_blocks);
     dfree(p->d_goff);
     dfree(p->d_glist);
     dfree(p->sell.d_slice_ptr);
@@ -788,7 +789,9 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
     unsigned long long* d_count = (unsigned long long*)(ctx->d_scalars + 8);
     HIP_TRY(ctx, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), ctx->stream));
     for (auto& pot : p->pots) {
-        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, nullptr, d_count))
+        // elements of a conditional potential that are inactive have no Hessian to project
+        const uint8_t* only = pot.kC.empty() ? nullptr : pot.d_active;
+        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, only, d_count))
             return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports up to 4 DoF blocks per element");
     }
     HIP_TRY(ctx, hipGetLastError());
@@ -825,6 +828,7 @@ int symx_hip_problem_project_selective(symx_hip_problem p, double eps, int32_t m
         pr.g_base[q] = pot.g_base;
         pr.g_end[q] = pot.g_base + (int64_t)pot.n_elem * pot.nb;
         pr.e_base[q] = pot.e_base;
+        pr.active[q] = pot.kC.empty() ? nullptr : pot.d_active;
     }
     unsigned long long* cnt = (unsigned long long*)(ctx->d_scalars + 8);  // [changed, newly projected, active blocks]
     HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), ctx->stream));
@@ -1127,6 +1131,42 @@ int symx_hip_problem_launch_adjoint(symx_hip_problem p, int32_t pot_id, const ch
     dfree(d_out);
     if (e != hipSuccess) return fail(ctx, SYMX_HIP_ERR_HIP, std::string("adjoint kernel: ") + hipGetErrorString(e));
     return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_set_plane_filter(symx_hip_problem p, const int32_t* blocks, int32_t n, int32_t axis, double plane)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (n < 0 || (n > 0 && !blocks) || axis < 0 || axis > 2) return fail(ctx, SYMX_HIP_ERR_ARG, "bad plane filter");
+    for (int32_t i = 0; i < n; i++)
+        if (blocks[i] < 0 || blocks[i] >= p->nb) return fail(ctx, SYMX_HIP_ERR_ARG, "plane filter block out of range");
+    dfree(p->d_filter_blocks);
+    p->n_filter_blocks = 0;
+    if (n > 0) {
+        int rc = dalloc(ctx, &p->d_filter_blocks, (size_t)n);
+        if (rc) return rc;
+        HIP_TRY(ctx, hipMemcpy(p->d_filter_blocks, blocks, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    }
+    p->n_filter_blocks = n;
+    p->filter_axis = axis;
+    p->filter_plane = plane;
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_plane_max_step(symx_hip_problem p, double eta, double* t_max)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!t_max || !(eta > 0.0)) return fail(ctx, SYMX_HIP_ERR_ARG, "bad plane step arguments");
+    *t_max = 1.0;
+    if (p->n_filter_blocks == 0) return SYMX_HIP_OK;
+    unsigned long long* d = (unsigned long long*)(ctx->d_scalars + 12);
+    const double one = 1.0;
+    HIP_TRY(ctx, hipMemcpyAsync(d, &one, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    launch_plane_max_step(ctx->stream, p->d_filter_blocks, p->n_filter_blocks, p->d_vec[SYMX_HIP_VEC_DOFS], p->d_vec[SYMX_HIP_VEC_DU],
+                          p->filter_axis, p->filter_plane, eta, d);
+    HIP_TRY(ctx, hipGetLastError());
+    return sync_scalar(ctx, (const double*)d, t_max);
 }
 
 int symx_hip_loop_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const int32_t* conn, int32_t n_elem, int32_t conn_stride,

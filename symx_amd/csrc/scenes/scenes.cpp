@@ -3,6 +3,8 @@
 // examples/rubber_extrusion.cpp:68-107, examples/dynamic_elasticity_with_contact.cpp:67-102).
 #include "../../../include/symx_scenes.h"
 
+#include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -92,6 +94,10 @@ struct symx_scene_s
     LabelledConnectivity<2> bc{ { "constraint_idx", "vertex_idx" } };
     std::vector<std::array<double, 3>> bc_target;
     std::vector<double> log_mu;
+    std::vector<int32_t> contact;   // surface vertices watched by the ground barrier
+    double kappa = 0, dhat = 0;
+    symx_sphere_params sph{};
+    bool is_sphere = false;
     double lambda = 0, mu = 0, lam_over_mu = 0, dt = 0, k = 0;
     double E_ = 0, nu_ = 0;
     std::array<double, 3> gravity{};
@@ -119,32 +125,25 @@ static void apply_newton_params(NewtonSettings& s, const symx_newton_params& p)
     s.step_tolerance = p.step_tolerance;
 }
 
-// Mesh, data and potentials of the Tet4 block scene (no device work).
-static void build_scene(symx_scene_s* s, const symx_tet4_block_params* p, const symx_newton_params* np)
+// Material, loads and contact of a Tet4 scene (shared by the block and sphere scenes).
+struct SceneSpec
 {
-    s->prm = *p;
-    s->np = *np;
-    const FEMMesh mesh = generate_cuboid_Tet4_mesh({ p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] });
-    const int32_t nv = (int32_t)mesh.vertices.size();
-    s->X = mesh.vertices;
-    s->tets = mesh.connectivity;
-    s->x = s->X;
-    const double h_min = std::min({ p->size[0] / p->cells[0], p->size[1] / p->cells[1], p->size[2] / p->cells[2] });
-    (void)h_min;
-    SplitMix64 rng(p->seed);
-    std::vector<uint8_t> fixed(nv, 0);
-    for (int32_t i = 0; i < nv; i++) {
-        if (p->fixed_axis >= 0 && std::abs(s->X[i][p->fixed_axis]) < 1e-12) fixed[i] = 1;
-        for (int d = 0; d < 3; d++) {
-            const double u = rng.uniform();
-            if (!fixed[i]) s->x[i][d] += p->noise * (2.0 * u - 1.0);
-        }
-    }
-    s->x0 = s->x;
-    s->v.assign(nv, { 0.0, 0.0, 0.0 });
+    double youngs_modulus = 1e5, poissons_ratio = 0.4, density = 1000.0;
+    std::array<double, 3> gravity{ 0.0, 0.0, -9.81 };
+    double dt = 0.01, bc_stiffness = 1e8;
+    int32_t material = 0, log_mu_field = 0;
+    bool contact = false;
+    double kappa = 0.0, dhat = 0.0;
+};
+
+// Masses, per-element data and potentials of a Tet4 scene whose mesh (X, tets), state
+// (x, x0, v) and boundary vertices (bc, bc_target) are set (no device work).
+static void finish_scene(symx_scene_s* s, const SceneSpec& p)
+{
+    const int32_t nv = (int32_t)s->X.size();
+    const int32_t ne = (int32_t)(s->tets.size() / 4);
     // lumped masses (dynamic_elasticity_with_contact.cpp:67-79)
     s->mass.assign(nv, 0.0);
-    const int32_t ne = mesh.n_elements;
     for (int32_t e = 0; e < ne; e++) {
         const int32_t* t = &s->tets[4 * (size_t)e];
         const auto& A = s->X[t[0]];
@@ -155,44 +154,39 @@ static void build_scene(symx_scene_s* s, const symx_tet4_block_params* p, const 
             d3[k] = s->X[t[3]][k] - A[k];
         }
         const double det = d1[0] * (d2[1] * d3[2] - d2[2] * d3[1]) - d1[1] * (d2[0] * d3[2] - d2[2] * d3[0]) + d1[2] * (d2[0] * d3[1] - d2[1] * d3[0]);
-        const double lumped = std::abs(det) / 6.0 * p->density / 4.0;
+        const double lumped = std::abs(det) / 6.0 * p.density / 4.0;
         for (int k = 0; k < 4; k++) s->mass[t[k]] += lumped;
     }
     s->all_vertices.resize(nv);
     for (int32_t i = 0; i < nv; i++) s->all_vertices[i] = i;
-    for (int32_t i = 0; i < nv; i++) {
-        if (fixed[i]) {
-            s->bc.numbered_push_back({ i });
-            s->bc_target.push_back(s->X[i]);
-        }
-    }
     s->tets_enum.resize((size_t)ne * 5);
     for (int32_t e = 0; e < ne; e++) {
         s->tets_enum[5 * (size_t)e] = e;
         for (int k = 0; k < 4; k++) s->tets_enum[5 * (size_t)e + 1 + k] = s->tets[4 * (size_t)e + k];
     }
-    s->E_ = p->youngs_modulus;
-    s->nu_ = p->poissons_ratio;
-    s->mu = p->youngs_modulus / (2.0 * (1.0 + p->poissons_ratio));
-    s->lambda = p->youngs_modulus * p->poissons_ratio / ((1.0 + p->poissons_ratio) * (1.0 - 2.0 * p->poissons_ratio));
+    s->E_ = p.youngs_modulus;
+    s->nu_ = p.poissons_ratio;
+    s->mu = p.youngs_modulus / (2.0 * (1.0 + p.poissons_ratio));
+    s->lambda = p.youngs_modulus * p.poissons_ratio / ((1.0 + p.poissons_ratio) * (1.0 - 2.0 * p.poissons_ratio));
     s->lam_over_mu = s->lambda / s->mu;
     s->log_mu.assign(ne, std::log(s->mu));
-    s->dt = p->dt;
-    s->k = p->bc_stiffness;
-    s->gravity = { p->gravity[0], p->gravity[1], p->gravity[2] };
+    s->dt = p.dt;
+    s->k = p.bc_stiffness;
+    s->gravity = p.gravity;
+    s->kappa = p.kappa;
+    s->dhat = p.dhat;
 
     s->G = GlobalPotential::create();
     symx_scene_s* S = s;
-    const int material = p->material;
-    if (p->log_mu_field) {
-        S->G->add_potential("neohookean_tet4_logmu", S->tets_enum, 5, [S, material](MappedWorkspace<double>& mws, Element& elem) {
+    const int material = p.material;
+    if (p.log_mu_field) {
+        S->G->add_potential("neohookean_tet4_logmu", S->tets_enum, 5, [S](MappedWorkspace<double>& mws, Element& elem) {
             std::vector<Vector> xs = mws.make_vectors(S->x, elem.slice(1, 5));
             std::vector<Vector> Xs = mws.make_vectors(S->X, elem.slice(1, 5));
             const Scalar logmu = mws.make_scalar(S->log_mu, elem[0]);
             const Scalar ratio = mws.make_scalar(S->lam_over_mu);
             const Scalar m = exp(logmu);
             return tet4_elastic(mws, xs, Xs, ratio * m, m, 0);
-            (void)material;
         });
     }
     else {
@@ -205,7 +199,7 @@ static void build_scene(symx_scene_s* s, const symx_tet4_block_params* p, const 
                                 return tet4_elastic(mws, xs, Xs, a, b, material);
                             });
     }
-    if (p->dt > 0.0) {
+    if (p.dt > 0.0) {
         // backward Euler inertia with gravity (dynamic_elasticity_with_contact.cpp:85-102, no damping)
         S->G->add_potential("be_inertia", S->all_vertices, 1, [S](MappedWorkspace<double>& mws, Element& elem) {
             const Vector x1 = mws.make_vector(S->x, elem[0]);
@@ -219,7 +213,7 @@ static void build_scene(symx_scene_s* s, const symx_tet4_block_params* p, const 
             return 0.5 * m * dev.squared_norm() / dt.powN(2);
         });
     }
-    else if (p->gravity[0] != 0.0 || p->gravity[1] != 0.0 || p->gravity[2] != 0.0) {
+    else if (p.gravity[0] != 0.0 || p.gravity[1] != 0.0 || p.gravity[2] != 0.0) {
         S->G->add_potential("gravity", S->all_vertices, 1, [S](MappedWorkspace<double>& mws, Element& elem) {
             const Vector x1 = mws.make_vector(S->x, elem[0]);
             const Scalar m = mws.make_scalar(S->mass, elem[0]);
@@ -236,8 +230,147 @@ static void build_scene(symx_scene_s* s, const symx_tet4_block_params* p, const 
             return 0.5 * k * (x - target).squared_norm();
         });
     }
+    if (p.contact && !S->contact.empty()) {
+        // IPC log-barrier against the ground plane z = 0 for each surface vertex, active
+        // below dhat: b(d) = -kappa (d - dhat)^2 ln(d / dhat), +inf for d <= 0. Written as a
+        // conditional potential (GlobalPotential.h:20,38), like the ground contact of
+        // dynamic_elasticity_with_contact.cpp:127-137.
+        S->G->add_potential("ipc_ground_barrier", S->contact, 1, [S](MappedWorkspace<double>& mws, Element& elem) -> std::pair<Scalar, Scalar> {
+            const Vector x1 = mws.make_vector(S->x, elem[0]);
+            const Scalar kappa = mws.make_scalar(S->kappa);
+            const Scalar dhat = mws.make_scalar(S->dhat);
+            const Scalar d = x1[2];
+            const Scalar b = -1.0 * kappa * (d - dhat).powN(2) * log(d / dhat);
+            return { b, dhat > d };
+        });
+    }
     S->G->add_dof(S->x);
-    if (p->log_mu_field) S->G->add_parameter(S->log_mu, "log_mu");
+    if (p.log_mu_field) S->G->add_parameter(S->log_mu, "log_mu");
+}
+
+// Mesh, data and potentials of the Tet4 block scene (no device work).
+static void build_scene(symx_scene_s* s, const symx_tet4_block_params* p, const symx_newton_params* np)
+{
+    s->prm = *p;
+    s->np = *np;
+    const FEMMesh mesh = generate_cuboid_Tet4_mesh({ p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] });
+    const int32_t nv = (int32_t)mesh.vertices.size();
+    s->X = mesh.vertices;
+    s->tets = mesh.connectivity;
+    s->x = s->X;
+    SplitMix64 rng(p->seed);
+    std::vector<uint8_t> fixed(nv, 0);
+    for (int32_t i = 0; i < nv; i++) {
+        if (p->fixed_axis >= 0 && std::abs(s->X[i][p->fixed_axis]) < 1e-12) fixed[i] = 1;
+        for (int d = 0; d < 3; d++) {
+            const double u = rng.uniform();
+            if (!fixed[i]) s->x[i][d] += p->noise * (2.0 * u - 1.0);
+        }
+    }
+    s->x0 = s->x;
+    s->v.assign(nv, { 0.0, 0.0, 0.0 });
+    for (int32_t i = 0; i < nv; i++) {
+        if (fixed[i]) {
+            s->bc.numbered_push_back({ i });
+            s->bc_target.push_back(s->X[i]);
+        }
+    }
+    SceneSpec spec;
+    spec.youngs_modulus = p->youngs_modulus;
+    spec.poissons_ratio = p->poissons_ratio;
+    spec.density = p->density;
+    spec.gravity = { p->gravity[0], p->gravity[1], p->gravity[2] };
+    spec.dt = p->dt;
+    spec.bc_stiffness = p->bc_stiffness;
+    spec.material = p->material;
+    spec.log_mu_field = p->log_mu_field;
+    finish_scene(s, spec);
+}
+
+// Jittered-grid tetrahedralisation of a ball: the 6-tet split of a cells^3 grid over the
+// bounding cube (mesh_generation.cpp:64-84 order), every grid vertex jittered by
+// U(-jitter, jitter) cell widths (SplitMix64 stream, 3 draws per grid vertex in index
+// order), tets kept when the centroid of their unjittered vertices lies within the
+// radius, vertices renumbered in increasing grid order. Contact vertices: vertices of
+// faces that belong to one kept tet, ascending.
+static void sphere_mesh(const symx_sphere_params* p, std::vector<std::array<double, 3>>& X, std::vector<int32_t>& tets,
+                        std::vector<int32_t>& contact)
+{
+    if (p->cells < 1 || !(p->radius > 0.0) || !(p->jitter >= 0.0 && p->jitter < 0.25)) throw std::runtime_error("bad sphere parameters");
+    const double r = p->radius, h = 2.0 * r / p->cells;
+    const FEMMesh grid = generate_cuboid_Tet4_mesh({ 2.0 * r, 2.0 * r, 2.0 * r }, { p->cells, p->cells, p->cells },
+                                                   { p->center[0] - r, p->center[1] - r, p->center[2] - r });
+    const int32_t ng = (int32_t)grid.vertices.size();
+    std::vector<std::array<double, 3>> J = grid.vertices;
+    SplitMix64 rng(p->seed);
+    for (int32_t i = 0; i < ng; i++)
+        for (int d = 0; d < 3; d++) J[i][d] += p->jitter * h * (2.0 * rng.uniform() - 1.0);
+    std::vector<int32_t> keep;
+    for (int32_t e = 0; e < grid.n_elements; e++) {
+        double c[3] = { 0.0, 0.0, 0.0 };
+        for (int a = 0; a < 4; a++)
+            for (int d = 0; d < 3; d++) c[d] += 0.25 * grid.vertices[grid.connectivity[4 * (size_t)e + a]][d];
+        const double dx = c[0] - p->center[0], dy = c[1] - p->center[1], dz = c[2] - p->center[2];
+        if (dx * dx + dy * dy + dz * dz <= r * r) keep.push_back(e);
+    }
+    std::vector<int32_t> id(ng, -1);
+    for (int32_t e : keep)
+        for (int a = 0; a < 4; a++) id[grid.connectivity[4 * (size_t)e + a]] = 0;
+    X.clear();
+    for (int32_t i = 0; i < ng; i++) {
+        if (id[i] < 0) continue;
+        id[i] = (int32_t)X.size();
+        X.push_back(J[i]);
+    }
+    tets.resize(keep.size() * 4);
+    for (size_t k = 0; k < keep.size(); k++)
+        for (int a = 0; a < 4; a++) tets[4 * k + a] = id[grid.connectivity[4 * (size_t)keep[k] + a]];
+    // boundary faces
+    std::vector<std::array<int32_t, 3>> faces;
+    faces.reserve(keep.size() * 4);
+    for (size_t k = 0; k < keep.size(); k++) {
+        const int32_t* t = &tets[4 * k];
+        const int32_t f[4][3] = { { t[0], t[1], t[2] }, { t[0], t[1], t[3] }, { t[0], t[2], t[3] }, { t[1], t[2], t[3] } };
+        for (auto& fc : f) {
+            std::array<int32_t, 3> q = { fc[0], fc[1], fc[2] };
+            std::sort(q.begin(), q.end());
+            faces.push_back(q);
+        }
+    }
+    std::sort(faces.begin(), faces.end());
+    std::vector<uint8_t> on(X.size(), 0);
+    for (size_t i = 0; i < faces.size();) {
+        size_t j = i + 1;
+        while (j < faces.size() && faces[j] == faces[i]) j++;
+        if (j - i == 1)
+            for (int32_t v : faces[i]) on[v] = 1;
+        i = j;
+    }
+    contact.clear();
+    for (int32_t v = 0; v < (int32_t)X.size(); v++)
+        if (on[v]) contact.push_back(v);
+}
+
+static void build_sphere(symx_scene_s* s, const symx_sphere_params* p, const symx_newton_params* np)
+{
+    s->sph = *p;
+    s->np = *np;
+    s->is_sphere = true;
+    sphere_mesh(p, s->X, s->tets, s->contact);
+    s->x = s->X;
+    s->x0 = s->X;
+    s->v.assign(s->X.size(), { p->velocity[0], p->velocity[1], p->velocity[2] });
+    SceneSpec spec;
+    spec.youngs_modulus = p->youngs_modulus;
+    spec.poissons_ratio = p->poissons_ratio;
+    spec.density = p->density;
+    spec.gravity = { p->gravity[0], p->gravity[1], p->gravity[2] };
+    spec.dt = p->dt;
+    spec.material = p->material;
+    spec.contact = p->contact_stiffness > 0.0;
+    spec.kappa = p->contact_stiffness;
+    spec.dhat = p->dhat;
+    finish_scene(s, spec);
 }
 
 extern "C" {
@@ -276,6 +409,31 @@ void symx_tet4_block_params_default(symx_tet4_block_params* p)
     p->log_mu_field = 0;
 }
 
+void symx_sphere_params_default(symx_sphere_params* p)
+{
+    p->radius = 0.5;
+    p->cells = 50;
+    p->jitter = 0.1;
+    p->seed = 7;
+    p->center[0] = 0.0;
+    p->center[1] = 0.0;
+    p->center[2] = 0.5 + 0.01;
+    p->velocity[0] = 0.0;
+    p->velocity[1] = 0.0;
+    p->velocity[2] = -2.0;
+    p->youngs_modulus = 1e5;
+    p->poissons_ratio = 0.4;
+    p->density = 1000.0;
+    p->gravity[0] = 0.0;
+    p->gravity[1] = 0.0;
+    p->gravity[2] = -9.81;
+    p->dt = 1.0 / 200.0;
+    p->material = 0;
+    p->contact_stiffness = 1e6;
+    p->dhat = 1e-3;
+    p->step_filter = 0.9;
+}
+
 const char* symx_scene_last_error(void) { return g_err.c_str(); }
 
 int symx_scene_tet4_block_create(const symx_tet4_block_params* p, const symx_newton_params* np, symx_scene* out)
@@ -294,6 +452,42 @@ int symx_scene_tet4_block_create(const symx_tet4_block_params* p, const symx_new
         return rc;
     }
     *out = s;
+    return 0;
+}
+
+int symx_scene_sphere_create(const symx_sphere_params* p, const symx_newton_params* np, symx_scene* out)
+{
+    *out = nullptr;
+    auto* s = new symx_scene_s();
+    int rc = guard([&]() {
+        build_sphere(s, p, np);
+        s->ctx = Context::create();
+        s->newton = NewtonsMethod::create(s->G, s->ctx);
+        apply_newton_params(s->newton->settings, *np);
+        if (p->contact_stiffness > 0.0 && p->step_filter > 0.0) {
+            symx_hip_problem prob = s->newton->device().problem();
+            if (symx_hip_problem_set_plane_filter(prob, s->contact.data(), (int32_t)s->contact.size(), 2, 0.0))
+                throw std::runtime_error(symx_hip_last_error(nullptr));
+            const double eta = p->step_filter;
+            s->newton->callbacks->add_max_allowed_step_device([eta](symx_hip_problem pr) {
+                double t = 1.0;
+                if (symx_hip_problem_plane_max_step(pr, eta, &t)) throw std::runtime_error(symx_hip_last_error(nullptr));
+                return t;
+            });
+        }
+    });
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+int symx_scene_contact_vertices(symx_scene s, int32_t* n, int32_t* out)
+{
+    *n = (int32_t)s->contact.size();
+    if (out) std::memcpy(out, s->contact.data(), s->contact.size() * sizeof(int32_t));
     return 0;
 }
 
@@ -552,6 +746,21 @@ int symx_host_jit_tet4(const char* cache_dir, int32_t material, int64_t* code_si
             throw std::runtime_error(symx_hip_last_error(nullptr));
         *code_size = (int64_t)cs;
         *n_ops_pgh = count_operations(*mws->workspace().graph(), specs[2].outputs);
+    });
+}
+
+int symx_host_sphere_mesh(const symx_sphere_params* p, int32_t* nv, int32_t* ne, int32_t* nc, double* X, int32_t* tets, int32_t* contact)
+{
+    return guard([&]() {
+        std::vector<std::array<double, 3>> Xv;
+        std::vector<int32_t> t, c;
+        sphere_mesh(p, Xv, t, c);
+        *nv = (int32_t)Xv.size();
+        *ne = (int32_t)(t.size() / 4);
+        *nc = (int32_t)c.size();
+        if (X) std::memcpy(X, Xv.data(), Xv.size() * 3 * sizeof(double));
+        if (tets) std::memcpy(tets, t.data(), t.size() * sizeof(int32_t));
+        if (contact) std::memcpy(contact, c.data(), c.size() * sizeof(int32_t));
     });
 }
 

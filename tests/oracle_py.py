@@ -21,7 +21,8 @@ class OracleScene(Structure):
                 ("tets", c_void_p), ("X", c_void_p), ("x", c_void_p), ("x0", c_void_p), ("v0", c_void_p), ("mass", c_void_p),
                 ("bc_vertex", c_void_p), ("bc_target", c_void_p), ("mu_e", c_void_p),
                 ("lambda_over_mu", c_double), ("lam", c_double), ("mu", c_double), ("dt", c_double), ("k", c_double),
-                ("gravity", c_double * 3)]
+                ("gravity", c_double * 3), ("ncontact", c_int32), ("pad_", c_int32), ("contact_vertex", c_void_p),
+                ("kappa", c_double), ("dhat", c_double), ("step_eta", c_double)]
 
 
 class OracleNewtonParams(Structure):
@@ -49,7 +50,8 @@ def _p(a):
 class SceneData:
     """Arrays of a Tet4 block problem, kept alive for the oracle."""
 
-    def __init__(self, tets, X, x, x0, v0, mass, bc_vertex, bc_target, lam, mu, dt, k, gravity, mu_e=None, lambda_over_mu=0.0):
+    def __init__(self, tets, X, x, x0, v0, mass, bc_vertex, bc_target, lam, mu, dt, k, gravity, mu_e=None, lambda_over_mu=0.0,
+                 contact=None, kappa=0.0, dhat=0.0, step_eta=0.0):
         self.tets = np.ascontiguousarray(tets, np.int32)
         self.X = np.ascontiguousarray(X, np.float64)
         self.x = np.ascontiguousarray(x, np.float64).copy()
@@ -62,6 +64,8 @@ class SceneData:
         self.lam, self.mu, self.dt, self.k = lam, mu, dt, k
         self.lambda_over_mu = lambda_over_mu
         self.gravity = np.asarray(gravity, np.float64)
+        self.contact = np.zeros(0, np.int32) if contact is None else np.ascontiguousarray(contact, np.int32)
+        self.kappa, self.dhat, self.step_eta = kappa, dhat, step_eta
 
     def struct(self) -> OracleScene:
         s = OracleScene()
@@ -75,6 +79,9 @@ class SceneData:
         s.lambda_over_mu = self.lambda_over_mu
         s.lam, s.mu, s.dt, s.k = self.lam, self.mu, self.dt, self.k
         s.gravity[:] = self.gravity
+        s.ncontact = self.contact.shape[0]
+        s.contact_vertex = _p(self.contact) if self.contact.shape[0] else None
+        s.kappa, s.dhat, s.step_eta = self.kappa, self.dhat, self.step_eta
         return s
 
 
@@ -93,6 +100,18 @@ def scene_arrays(scene, log_mu=None) -> SceneData:
     mu_e = None if log_mu is None else np.exp(np.asarray(log_mu, np.float64))
     return SceneData(scene.connectivity(), X, scene.get("x"), scene.get("x0"), scene.get("v"), scene.get("mass"), bc_vertex,
                      X[bc_vertex], lam, mu, p.dt, p.bc_stiffness, list(p.gravity), mu_e=mu_e, lambda_over_mu=lam / mu)
+
+
+def sphere_arrays(scene) -> SceneData:
+    """Arrays of a symx_amd sphere contact scene (no boundary vertices)."""
+    p = scene.sphere
+    X = scene.get("X")
+    E, nu = p.youngs_modulus, p.poissons_ratio
+    mu = E / (2 * (1 + nu))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    return SceneData(scene.connectivity(), X, scene.get("x"), scene.get("x0"), scene.get("v"), scene.get("mass"), np.zeros(0, np.int32),
+                     np.zeros((0, 3)), lam, mu, p.dt, 0.0, list(p.gravity), contact=scene.contact_vertices(), kappa=p.contact_stiffness,
+                     dhat=p.dhat, step_eta=p.step_filter)
 
 
 class Oracle:

@@ -54,3 +54,26 @@ def tet4_block(cells=(4, 4, 4), size=(1.0, 1.0, 1.0), E=1e5, nu=0.4, density=100
     lam = E * nu / ((1 + nu) * (1 - 2 * nu))
     bc = np.nonzero(fixed)[0]
     return SceneData(tets, X, x, x.copy(), np.zeros_like(x), mass, bc, X[bc], lam, mu, dt, k, gravity)
+
+
+def sphere_mesh(radius=0.5, cells=50, jitter=0.1, seed=7, center=(0.0, 0.0, 0.51)):
+    """Jittered-grid ball (symx_scenes.h sphere scene): (X, tets, contact vertices)."""
+    r = radius
+    h = 2.0 * r / cells
+    G, T = tet4_mesh((cells,) * 3, (2 * r,) * 3)
+    G = G + (np.asarray(center) - r)
+    u = splitmix64_uniform(seed, 3 * G.shape[0]).reshape(-1, 3)
+    J = G + jitter * h * (2.0 * u - 1.0)
+    c = G[T].mean(axis=1)
+    keep = ((c - np.asarray(center)) ** 2).sum(axis=1) <= r * r
+    T = T[keep]
+    used = np.zeros(G.shape[0], bool)
+    used[T.ravel()] = True
+    ids = -np.ones(G.shape[0], np.int64)
+    ids[used] = np.arange(used.sum())
+    X = J[used]
+    T = ids[T].astype(np.int32)
+    faces = np.sort(np.concatenate([T[:, [0, 1, 2]], T[:, [0, 1, 3]], T[:, [0, 2, 3]], T[:, [1, 2, 3]]]), axis=1)
+    uniq, counts = np.unique(faces, axis=0, return_counts=True)
+    contact = np.unique(uniq[counts == 1].ravel()).astype(np.int32)
+    return X, T, contact
