@@ -47,7 +47,7 @@ enum {
     SYMX_HIP_VEC_RHS = 4      /* linear-system right-hand side */
 };
 
-#define SYMX_HIP_MAX_MAPS 48
+#define SYMX_HIP_MAX_MAPS 128
 #define SYMX_HIP_BLOCK 256 /* threads per workgroup of every generated element kernel */
 
 /* Argument block of every generated element kernel (one struct by value).
@@ -132,6 +132,10 @@ typedef struct symx_hip_potential_desc {
     const int32_t* dof_set_offset; /* [n_dof_blocks] DoF offset of block i's set */
     int32_t n_maps;
     const symx_hip_map_desc* maps;
+    /* Elements per 256-thread workgroup of the P/PG/PGH kernels and per energy partial:
+     * 0 or 256 for one element per thread (generated element kernels), 4 for one element
+     * per wavefront (structured FEM kernels). The C kernel always runs one per thread. */
+    int32_t elems_per_group;
 } symx_hip_potential_desc;
 
 int symx_hip_problem_add_potential(symx_hip_problem prob, const symx_hip_potential_desc* desc, int32_t* pot_id);

@@ -57,6 +57,25 @@ typedef struct symx_sphere_params {
     double step_filter;        /* eta of the plane step filter (0: off) */
 } symx_sphere_params;
 
+/* Hexahedral block (Hex8 or Hex27 cuboid over [0, size]) with a per-element Young's
+ * modulus drawn log-uniformly in [youngs_min, youngs_max]; loads and boundary as the
+ * Tet4 block; element energy written with fem_integrator / fem_jacobian. */
+typedef struct symx_hex_block_params {
+    int32_t cells[3];
+    double size[3];
+    int32_t nodes_per_element;  /* 8 or 27 */
+    double youngs_min;
+    double youngs_max;
+    double poissons_ratio;
+    double density;
+    double gravity[3];
+    double dt;                  /* <= 0: quasistatic */
+    double noise;               /* uniform perturbation of free nodes, in node spacings */
+    uint64_t seed;
+    double bc_stiffness;
+    int32_t fixed_axis;
+} symx_hex_block_params;
+
 /* Newton settings exposed to callers (subset of NewtonSettings, solver_utils.h:173-258). */
 typedef struct symx_newton_params {
     int32_t precision;               /* SYMX_HIP_FP64 or SYMX_HIP_MIXED */
@@ -98,7 +117,7 @@ const char* symx_scene_last_error(void);
 
 int symx_scene_n_vertices(symx_scene s, int32_t* n);
 int symx_scene_n_elements(symx_scene s, int32_t* n);
-/* connectivity [n_elem][4] */
+/* connectivity [n_elem][nodes per element] */
 int symx_scene_connectivity(symx_scene s, int32_t* conn);
 /* positions (DoFs), previous positions x0, velocities v0, rest positions, masses */
 int symx_scene_get_array(symx_scene s, const char* name, double* out);
@@ -110,6 +129,11 @@ int symx_scene_set_log_mu(symx_scene s, const double* log_mu);
 int symx_scene_set_newton_params(symx_scene s, const symx_newton_params* np);
 /* Newton solve of the current time step (does not advance time). */
 int symx_scene_solve(symx_scene s, symx_solve_stats* st);
+void symx_hex_block_params_default(symx_hex_block_params* p);
+int symx_scene_hex_block_create(const symx_hex_block_params* p, const symx_newton_params* np, symx_scene* out);
+int symx_scene_nodes_per_element(symx_scene s, int32_t* n);
+/* per-element Young's moduli of a hexahedral scene (n_elements values) */
+int symx_scene_young_moduli(symx_scene s, double* out);
 void symx_sphere_params_default(symx_sphere_params* p);
 int symx_scene_sphere_create(const symx_sphere_params* p, const symx_newton_params* np, symx_scene* out);
 /* surface (contact) vertices of a sphere scene: *n receives the count; out may be NULL */
@@ -159,6 +183,14 @@ int symx_host_scene_jit(const symx_tet4_block_params* p, const char* cache_dir, 
 /* Sphere mesh of symx_scene_sphere_create on the host: counts, and (when not NULL)
  * rest positions [nv][3], tets [ne][4], contact vertices [nc]. */
 int symx_host_sphere_mesh(const symx_sphere_params* p, int32_t* nv, int32_t* ne, int32_t* nc, double* X, int32_t* tets, int32_t* contact);
+/* Generate and hiprtc-compile every potential module of a hexahedral block scene;
+ * *n_structured counts the potentials compiled as structured FEM kernels. */
+int symx_host_hex_jit(const symx_hex_block_params* p, const char* cache_dir, int32_t* n_potentials, int64_t* code_bytes, int32_t* n_structured);
+/* E, gradient [n] and Hessian [n][n] (DoF order) of element e of a hexahedral block's
+ * elastic potential evaluated on the host from the generated expressions: structured
+ * (deferred Jacobian, the structured kernel's contraction) or generic (expanded
+ * expression). *was_structured reports the path taken. */
+int symx_host_hex_element(const symx_hex_block_params* p, int32_t structured, int32_t e, double* E, double* g, double* H, int32_t* was_structured);
 /* Evaluate the generated adjoint outputs d(lambda . dE_e/dx)/dlog_mu_e of element e of
  * a log_mu_field scene on the host (symbolic graph, kernel's gather rules); lambda
  * has one value per DoF. */

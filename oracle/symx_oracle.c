@@ -154,6 +154,157 @@ ORACLE_API int oracle_nh_tet4(const double* x, const double* X, double lambda, d
     return 0;
 }
 
+/* ---- Neo-Hookean Hex8 / Hex27 elements (closed form, isoparametric) ------- */
+/* Node order, shape functions and quadrature of fem_integrators.h:84-150 and
+ * fem_integrators.cpp:69-116: per axis the node sits at -1, 0 or +1 (0, 1, 2 below);
+ * Hex8: 0.5 (1 -+ t); Hex27: t(t-1), 1 - t^2, t(t+1) scaled by 1/8, 1/4, 1/2, 1 for
+ * nodes with 0, 1, 2, 3 mid-axis positions. Rules: 2x2x2 Gauss (weight 1) and the 27-point
+ * rule with the reference's 10-digit constants, in the reference's order. */
+static const int8_t hex_nodes[27][3] = {
+    { 0, 0, 0 }, { 2, 0, 0 }, { 2, 2, 0 }, { 0, 2, 0 }, { 0, 0, 2 }, { 2, 0, 2 }, { 2, 2, 2 }, { 0, 2, 2 },
+    { 1, 0, 0 }, { 2, 1, 0 }, { 1, 2, 0 }, { 0, 1, 0 }, { 1, 0, 2 }, { 2, 1, 2 }, { 1, 2, 2 }, { 0, 1, 2 },
+    { 0, 0, 1 }, { 2, 0, 1 }, { 2, 2, 1 }, { 0, 2, 1 },
+    { 0, 1, 1 }, { 2, 1, 1 }, { 1, 2, 1 }, { 1, 0, 1 }, { 1, 1, 0 }, { 1, 1, 2 },
+    { 1, 1, 1 } };
+
+static int hex_rule(int nn, double (*q)[4])
+{
+    if (nn == 8) {
+        const double p = 1.0 / sqrt(3.0);
+        const double pts[8][3] = { { -p, -p, -p }, { p, -p, -p }, { p, p, -p }, { -p, p, -p }, { -p, -p, p }, { p, -p, p }, { p, p, p }, { -p, p, p } };
+        for (int i = 0; i < 8; i++) {
+            q[i][0] = 1.0;
+            for (int d = 0; d < 3; d++) q[i][1 + d] = pts[i][d];
+        }
+        return 8;
+    }
+    const double p = 0.7745966692, wc = 0.1714677641, we = 0.2743484225, wf = 0.4389574760, wm = 0.7023319616;
+    const double t[27][4] = { { wc, -p, -p, -p }, { wc, p, -p, -p }, { wc, -p, p, -p }, { wc, p, p, -p }, { wc, -p, -p, p }, { wc, p, -p, p },
+                              { wc, -p, p, p }, { wc, p, p, p }, { we, 0.0, -p, -p }, { we, -p, 0.0, -p }, { we, p, 0.0, -p }, { we, 0.0, p, -p },
+                              { we, -p, -p, 0.0 }, { we, p, -p, 0.0 }, { we, -p, p, 0.0 }, { we, p, p, 0.0 }, { we, 0.0, -p, p }, { we, -p, 0.0, p },
+                              { we, p, 0.0, p }, { we, 0.0, p, p }, { wf, 0.0, 0.0, -p }, { wf, 0.0, -p, 0.0 }, { wf, -p, 0.0, 0.0 },
+                              { wf, p, 0.0, 0.0 }, { wf, 0.0, p, 0.0 }, { wf, 0.0, 0.0, p }, { wm, 0.0, 0.0, 0.0 } };
+    memcpy(q, t, sizeof(t));
+    return 27;
+}
+
+/* dN_a / dxi at xi (dN [nn][3]) */
+static void hex_dshape(int nn, const double* xi, double* dN)
+{
+    for (int a = 0; a < nn; a++) {
+        double f[3], df[3];
+        int n_mid = 0;
+        for (int d = 0; d < 3; d++) {
+            const int pos = hex_nodes[a][d];
+            const double t = xi[d];
+            if (nn == 8) {
+                const double s = pos ? 1.0 : -1.0;
+                f[d] = 0.5 * (1.0 + s * t);
+                df[d] = 0.5 * s;
+            }
+            else if (pos == 0) {
+                f[d] = t * (t - 1.0);
+                df[d] = 2.0 * t - 1.0;
+            }
+            else if (pos == 2) {
+                f[d] = t * (t + 1.0);
+                df[d] = 2.0 * t + 1.0;
+            }
+            else {
+                f[d] = 1.0 - t * t;
+                df[d] = -2.0 * t;
+                n_mid++;
+            }
+        }
+        const double scale = nn == 8 ? 1.0 : (n_mid == 0 ? 0.125 : (n_mid == 1 ? 0.25 : (n_mid == 2 ? 0.5 : 1.0)));
+        dN[3 * a + 0] = scale * df[0] * f[1] * f[2];
+        dN[3 * a + 1] = scale * f[0] * df[1] * f[2];
+        dN[3 * a + 2] = scale * f[0] * f[1] * df[2];
+    }
+}
+
+/* E = sum_q w_q det(JX_q) psi(F_q), F = Jx JX^-1, psi the Neo-Hookean density above.
+ * dF_ij/dx_(a,i) = G_a[j] = sum_k dN_a[k] JX^-1[k][j]. x, X: [nn][3]; g: [3nn]; H: [3nn][3nn]. */
+ORACLE_API int oracle_nh_hex(int nn, const double* x, const double* X, double lambda, double mu, double* E, double* g, double* H)
+{
+    if (nn != 8 && nn != 27) return 2;
+    const int n = 3 * nn;
+    double q[27][4];
+    const int nq = hex_rule(nn, q);
+    double dN[27 * 3], G[27 * 3];
+    if (E) *E = 0.0;
+    if (g) memset(g, 0, sizeof(double) * n);
+    if (H) memset(H, 0, sizeof(double) * n * n);
+    for (int iq = 0; iq < nq; iq++) {
+        hex_dshape(nn, &q[iq][1], dN);
+        double Jx[9] = { 0 }, JX[9] = { 0 }, JXinv[9], F[9], Finv[9], FinvT[9];
+        for (int a = 0; a < nn; a++)
+            for (int i = 0; i < 3; i++)
+                for (int k = 0; k < 3; k++) {
+                    Jx[3 * i + k] += x[3 * a + i] * dN[3 * a + k];
+                    JX[3 * i + k] += X[3 * a + i] * dN[3 * a + k];
+                }
+        inv3(JX, JXinv);
+        const double w = q[iq][0] * det3(JX);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double s = 0.0;
+                for (int k = 0; k < 3; k++) s += Jx[3 * i + k] * JXinv[3 * k + j];
+                F[3 * i + j] = s;
+            }
+        for (int a = 0; a < nn; a++)
+            for (int j = 0; j < 3; j++) {
+                double s = 0.0;
+                for (int k = 0; k < 3; k++) s += dN[3 * a + k] * JXinv[3 * k + j];
+                G[3 * a + j] = s;
+            }
+        const double J = det3(F);
+        if (!(J > 0.0)) {
+            if (E) *E = INFINITY;
+            return 1;
+        }
+        inv3(F, Finv);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) FinvT[3 * i + j] = Finv[3 * j + i];
+        const double lnJ = log(J);
+        double IC = 0.0;
+        for (int i = 0; i < 9; i++) IC += F[i] * F[i];
+        if (E) *E += w * (0.5 * mu * (IC - 3.0) - mu * lnJ + 0.5 * lambda * lnJ * lnJ);
+        if (g) {
+            double P[9];
+            for (int i = 0; i < 9; i++) P[i] = mu * F[i] + (lambda * lnJ - mu) * FinvT[i];
+            for (int a = 0; a < nn; a++)
+                for (int i = 0; i < 3; i++) g[3 * a + i] += w * (P[3 * i + 0] * G[3 * a + 0] + P[3 * i + 1] * G[3 * a + 1] + P[3 * i + 2] * G[3 * a + 2]);
+        }
+        if (H) {
+            /* K[(i,j)][(k,l)] = dP_ij / dF_kl = mu d_ik d_jl + (mu - lambda lnJ) Finv_jk Finv_li + lambda Finv_ji Finv_lk */
+            double K[9][9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    for (int k = 0; k < 3; k++)
+                        for (int l = 0; l < 3; l++)
+                            K[3 * i + j][3 * k + l] = (i == k && j == l ? mu : 0.0) + (mu - lambda * lnJ) * Finv[3 * j + k] * Finv[3 * l + i] +
+                                                      lambda * Finv[3 * j + i] * Finv[3 * l + k];
+            for (int a = 0; a < nn; a++)
+                for (int b = 0; b < nn; b++)
+                    for (int i = 0; i < 3; i++)
+                        for (int k = 0; k < 3; k++) {
+                            double s = 0.0;
+                            for (int j = 0; j < 3; j++)
+                                for (int l = 0; l < 3; l++) s += G[3 * a + j] * K[3 * i + j][3 * k + l] * G[3 * b + l];
+                            H[(3 * a + i) * n + 3 * b + k] += w * s;
+                        }
+        }
+    }
+    return 0;
+}
+
+/* element energy of a scene element (Tet4 or hexahedron) */
+static int scene_element(int npe, const double* xe, const double* Xe, double lam, double mu, double* E, double* g, double* H)
+{
+    return npe == 4 ? oracle_nh_tet4(xe, Xe, lam, mu, E, g, H) : oracle_nh_hex(npe, xe, Xe, lam, mu, E, g, H);
+}
+
 /* ---- PD projection (project_to_PD.cpp:13-32) ---------------------------- */
 /* Eigen-decomposition of a symmetric n x n matrix by cyclic Jacobi; eigenvalues
  * below eps become eps (clamp) or -lambda (mirror); the matrix is rebuilt as
@@ -387,7 +538,12 @@ typedef struct oracle_scene {
     int32_t pad_;
     const int32_t* contact_vertex;
     double kappa, dhat, step_eta;
+    /* element type of `tets`: 4 (Tet4, also when 0), 8 (Hex8) or 27 (Hex27) */
+    int32_t nodes_per_element;
+    int32_t pad2_;
 } oracle_scene;
+
+#define NPE(S) ((S)->nodes_per_element > 0 ? (S)->nodes_per_element : 4)
 
 /* b(d) = -kappa (d - dhat)^2 ln(d / dhat) and its first two derivatives, 0 < d < dhat */
 static void barrier(double kappa, double dhat, double d, double* b, double* db, double* ddb)
@@ -429,7 +585,7 @@ typedef struct pattern {
     int64_t* rp;
     int32_t* cols;
     int64_t nnzb;
-    int64_t* elem_slot; /* [ne][16] block slot of (a, b) */
+    int64_t* elem_slot; /* [ne][npe * npe] block slot of (a, b) */
     int64_t* bc_slot;   /* [nbc] */
     int64_t* v_slot;    /* [nv] diagonal slot */
 } pattern;
@@ -453,8 +609,9 @@ static void build_pattern(const oracle_scene* S, pattern* P)
     P->nb = nv;
     int64_t* cnt = (int64_t*)calloc(nv + 1, sizeof(int64_t));
     for (int v = 0; v < nv; v++) cnt[v + 1] += 1;
+    const int npe = NPE(S);
     for (int e = 0; e < S->ne; e++)
-        for (int a = 0; a < 4; a++) cnt[S->tets[4 * (int64_t)e + a] + 1] += 4;
+        for (int a = 0; a < npe; a++) cnt[S->tets[npe * (int64_t)e + a] + 1] += npe;
     for (int v = 0; v < nv; v++) cnt[v + 1] += cnt[v];
     int32_t* cand = (int32_t*)malloc(sizeof(int32_t) * cnt[nv]);
     int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * nv);
@@ -463,9 +620,9 @@ static void build_pattern(const oracle_scene* S, pattern* P)
         cand[cur[v]++] = v;
     }
     for (int e = 0; e < S->ne; e++)
-        for (int a = 0; a < 4; a++) {
-            const int r = S->tets[4 * (int64_t)e + a];
-            for (int b = 0; b < 4; b++) cand[cur[r]++] = S->tets[4 * (int64_t)e + b];
+        for (int a = 0; a < npe; a++) {
+            const int r = S->tets[npe * (int64_t)e + a];
+            for (int b = 0; b < npe; b++) cand[cur[r]++] = S->tets[npe * (int64_t)e + b];
         }
     P->rp = (int64_t*)malloc(sizeof(int64_t) * (nv + 1));
     P->rp[0] = 0;
@@ -486,11 +643,11 @@ static void build_pattern(const oracle_scene* S, pattern* P)
         P->rp[v + 1] = P->rp[v] + cur[v];
     }
     P->nnzb = total;
-    P->elem_slot = (int64_t*)malloc(sizeof(int64_t) * 16 * (int64_t)(S->ne > 0 ? S->ne : 1));
+    P->elem_slot = (int64_t*)malloc(sizeof(int64_t) * npe * npe * (int64_t)(S->ne > 0 ? S->ne : 1));
     for (int e = 0; e < S->ne; e++)
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++)
-                P->elem_slot[16 * (int64_t)e + 4 * a + b] = find_slot(P, S->tets[4 * (int64_t)e + a], S->tets[4 * (int64_t)e + b]);
+        for (int a = 0; a < npe; a++)
+            for (int b = 0; b < npe; b++)
+                P->elem_slot[(int64_t)npe * npe * e + npe * a + b] = find_slot(P, S->tets[npe * (int64_t)e + a], S->tets[npe * (int64_t)e + b]);
     P->v_slot = (int64_t*)malloc(sizeof(int64_t) * nv);
     for (int v = 0; v < nv; v++) P->v_slot[v] = find_slot(P, v, v);
     P->bc_slot = (int64_t*)malloc(sizeof(int64_t) * (S->nbc > 0 ? S->nbc : 1));
@@ -510,24 +667,24 @@ static void free_pattern(pattern* P)
 
 /* Energy (and optionally gradient, element Hessians). E accumulated per element
  * in index order within each potential. Returns +inf for inverted elements. */
-static double scene_eval(const oracle_scene* S, const double* x, double* grad, double* He /* [ne][144] or NULL */)
+static double scene_eval(const oracle_scene* S, const double* x, double* grad, double* He /* [ne][(3 npe)^2] or NULL */)
 {
-    const int ne = S->ne, nv = S->nv;
+    const int ne = S->ne, nv = S->nv, npe = NPE(S), nd = 3 * NPE(S);
     double E_el = 0.0;
-    double* ge = grad ? (double*)malloc(sizeof(double) * 12 * (int64_t)(ne > 0 ? ne : 1)) : NULL;
+    double* ge = grad ? (double*)malloc(sizeof(double) * nd * (int64_t)(ne > 0 ? ne : 1)) : NULL;
     int bad = 0;
 #pragma omp parallel for schedule(static) reduction(+ : E_el) reduction(| : bad)
     for (int e = 0; e < ne; e++) {
-        double xe[12], Xe[12], Ee = 0.0;
-        for (int a = 0; a < 4; a++)
+        double xe[81], Xe[81], Ee = 0.0;
+        for (int a = 0; a < npe; a++)
             for (int d = 0; d < 3; d++) {
-                const int64_t v = S->tets[4 * (int64_t)e + a];
+                const int64_t v = S->tets[npe * (int64_t)e + a];
                 xe[3 * a + d] = x[3 * v + d];
                 Xe[3 * a + d] = S->X[3 * v + d];
             }
         const double mu = S->mu_e ? S->mu_e[e] : S->mu;
         const double lam = S->mu_e ? S->lambda_over_mu * S->mu_e[e] : S->lambda;
-        bad |= oracle_nh_tet4(xe, Xe, lam, mu, &Ee, ge ? ge + 12 * (int64_t)e : NULL, He ? He + 144 * (int64_t)e : NULL);
+        bad |= scene_element(npe, xe, Xe, lam, mu, &Ee, ge ? ge + nd * (int64_t)e : NULL, He ? He + (int64_t)nd * nd * e : NULL);
         E_el += Ee;
     }
     if (bad) E_el = INFINITY;
@@ -580,8 +737,8 @@ static double scene_eval(const oracle_scene* S, const double* x, double* grad, d
     }
     if (grad) {
         for (int e = 0; e < ne; e++)
-            for (int a = 0; a < 4; a++)
-                for (int d = 0; d < 3; d++) grad[3 * (int64_t)S->tets[4 * (int64_t)e + a] + d] += ge[12 * (int64_t)e + 3 * a + d];
+            for (int a = 0; a < npe; a++)
+                for (int d = 0; d < 3; d++) grad[3 * (int64_t)S->tets[npe * (int64_t)e + a] + d] += ge[nd * (int64_t)e + 3 * a + d];
         free(ge);
     }
     return E_el + E_in + E_bc + E_contact;
@@ -611,7 +768,7 @@ static void add_barrier_hessians(const oracle_scene* S, const pattern* P, const 
 
 ORACLE_API double oracle_scene_energy(const oracle_scene* S, const double* x, double* grad) { return scene_eval(S, x, grad, NULL); }
 
-/* Element Hessians of the elastic potential at x: He [ne][144]. */
+/* Element Hessians of the elastic potential at x: He [ne][(3 npe)^2]. */
 ORACLE_API int oracle_scene_element_hessians(const oracle_scene* S, const double* x, double* He)
 {
     double* g = (double*)malloc(sizeof(double) * 3 * (int64_t)S->nv);
@@ -623,13 +780,14 @@ ORACLE_API int oracle_scene_element_hessians(const oracle_scene* S, const double
 static void assemble(const oracle_scene* S, const pattern* P, const double* He, double* vals)
 {
     memset(vals, 0, sizeof(double) * 9 * P->nnzb);
+    const int npe = NPE(S), nd = 3 * NPE(S);
     for (int e = 0; e < S->ne; e++) {
-        const double* h = He + 144 * (int64_t)e;
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                double* dst = vals + 9 * P->elem_slot[16 * (int64_t)e + 4 * a + b];
+        const double* h = He + (int64_t)nd * nd * e;
+        for (int a = 0; a < npe; a++)
+            for (int b = 0; b < npe; b++) {
+                double* dst = vals + 9 * P->elem_slot[(int64_t)npe * npe * e + npe * a + b];
                 for (int i = 0; i < 3; i++)
-                    for (int j = 0; j < 3; j++) dst[3 * i + j] += h[(3 * a + i) * 12 + 3 * b + j];
+                    for (int j = 0; j < 3; j++) dst[3 * i + j] += h[(3 * a + i) * nd + 3 * b + j];
             }
     }
     if (S->has_inertia) {
@@ -665,7 +823,8 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
     double* rhs = (double*)malloc(sizeof(double) * n);
     double* du = (double*)malloc(sizeof(double) * n);
     double* x_ls = (double*)malloc(sizeof(double) * n);
-    double* He = (double*)malloc(sizeof(double) * 144 * (int64_t)(S->ne > 0 ? S->ne : 1));
+    const int npe = NPE(S), nd = 3 * NPE(S);
+    double* He = (double*)malloc(sizeof(double) * nd * nd * (int64_t)(S->ne > 0 ? S->ne : 1));
     double* vals = (double*)malloc(sizeof(double) * 9 * P.nnzb);
     unsigned char* projected = (unsigned char*)malloc((size_t)(S->ne > 0 ? S->ne : 1));
     unsigned char* bproj = (unsigned char*)malloc((size_t)(S->ncontact > 0 ? S->ncontact : 1));
@@ -721,7 +880,7 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
             if (mode == 1 || (mode == 2 && pdn_countdown > 0)) {
 #pragma omp parallel for schedule(dynamic, 64)
                 for (int e = 0; e < S->ne; e++) {
-                    if (!projected[e]) oracle_project(He + 144 * (int64_t)e, 12, np->projection_eps, np->use_mirroring);
+                    if (!projected[e]) oracle_project(He + (int64_t)nd * nd * e, nd, np->projection_eps, np->use_mirroring);
                     projected[e] = 1;
                 }
                 memset(bproj, 1, (size_t)(S->ncontact > 0 ? S->ncontact : 1));
@@ -738,9 +897,11 @@ ORACLE_API int oracle_newton_solve(oracle_scene* S, const oracle_newton_params* 
 #pragma omp parallel for schedule(dynamic, 64)
                 for (int e = 0; e < S->ne; e++) {
                     if (projected[e]) continue;
-                    const int32_t* t = S->tets + 4 * (int64_t)e;
-                    if (active[t[0]] || active[t[1]] || active[t[2]] || active[t[3]]) {
-                        oracle_project(He + 144 * (int64_t)e, 12, np->projection_eps, np->use_mirroring);
+                    const int32_t* t = S->tets + npe * (int64_t)e;
+                    int touch = 0;
+                    for (int a = 0; a < npe; a++) touch |= active[t[a]];
+                    if (touch) {
+                        oracle_project(He + (int64_t)nd * nd * e, nd, np->projection_eps, np->use_mirroring);
                         projected[e] = 1;
                     }
                 }
@@ -844,12 +1005,13 @@ ORACLE_API double oracle_eval_project_assemble(oracle_scene* S, int project, dou
     build_pattern(S, &P);
     const int64_t n = 3 * (int64_t)S->nv;
     double* grad = (double*)malloc(sizeof(double) * n);
-    double* He = (double*)malloc(sizeof(double) * 144 * (int64_t)(S->ne > 0 ? S->ne : 1));
+    const int nd = 3 * NPE(S);
+    double* He = (double*)malloc(sizeof(double) * nd * nd * (int64_t)(S->ne > 0 ? S->ne : 1));
     double* vals = (double*)malloc(sizeof(double) * 9 * P.nnzb);
     const double E = scene_eval(S, S->x, grad, He);
     if (project) {
 #pragma omp parallel for schedule(dynamic, 64)
-        for (int e = 0; e < S->ne; e++) oracle_project(He + 144 * (int64_t)e, 12, eps, 0);
+        for (int e = 0; e < S->ne; e++) oracle_project(He + (int64_t)nd * nd * e, nd, eps, 0);
     }
     assemble(S, &P, He, vals);
     if (out_vals_or_null) memcpy(out_vals_or_null, vals, sizeof(double) * 9 * P.nnzb);
@@ -871,7 +1033,7 @@ ORACLE_API double oracle_eval_project_assemble(oracle_scene* S, int project, dou
 ORACLE_API int oracle_adjoint_logmu(const oracle_scene* S, const double* dL_dx, double rel_tol, int max_iter, double* lam_out, double* dL_dp,
                                     oracle_pcg_info* info)
 {
-    if (!S->mu_e) return 1;
+    if (!S->mu_e || NPE(S) != 4) return 1;
     pattern P;
     build_pattern(S, &P);
     const int ne = S->ne;

@@ -47,6 +47,12 @@ class SphereParams(Structure):
                 ("step_filter", c_double)]
 
 
+class HexBlockParams(Structure):
+    _fields_ = [("cells", c_int32 * 3), ("size", c_double * 3), ("nodes_per_element", c_int32), ("youngs_min", c_double),
+                ("youngs_max", c_double), ("poissons_ratio", c_double), ("density", c_double), ("gravity", c_double * 3), ("dt", c_double),
+                ("noise", c_double), ("seed", c_uint64), ("bc_stiffness", c_double), ("fixed_axis", c_int32)]
+
+
 class NewtonParams(Structure):
     _fields_ = [("precision", c_int32), ("projection_mode", c_int32), ("use_mirroring", c_int32), ("max_iterations", c_int32),
                 ("residual_tolerance_abs", c_double), ("residual_tolerance_rel", c_double), ("cg_rel_tolerance", c_double),
@@ -110,6 +116,7 @@ def amd():
         lib.symx_newton_params_default.restype = None
         lib.symx_tet4_block_params_default.restype = None
         lib.symx_sphere_params_default.restype = None
+        lib.symx_hex_block_params_default.restype = None
         _amd = lib
     return _amd
 
@@ -153,6 +160,36 @@ def default_sphere_params(**kw) -> SphereParams:
     return p
 
 
+def default_hex_params(**kw) -> HexBlockParams:
+    p = HexBlockParams()
+    amd().symx_hex_block_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if k in ("cells", "size", "gravity"):
+            getattr(p, k)[:] = v
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def host_hex_element(params: HexBlockParams, e: int = 0, structured: bool = True):
+    """(E, gradient, Hessian, structured?) of element e of a hexahedral block, evaluated on the
+    host from the generated expressions (no GPU)."""
+    n = 3 * params.nodes_per_element
+    E = c_double()
+    g = np.zeros(n)
+    H = np.zeros(n * n)
+    ws = c_int32()
+    check_scene(amd().symx_host_hex_element(ctypes.byref(params), c_int32(int(structured)), c_int32(e), ctypes.byref(E), _ptr(g), _ptr(H),
+                                            ctypes.byref(ws)))
+    return E.value, g, H.reshape(n, n), bool(ws.value)
+
+
+def host_hex_jit(params: HexBlockParams, cache_dir: str):
+    n, cs, ns = c_int32(), c_int64(), c_int32()
+    check_scene(amd().symx_host_hex_jit(ctypes.byref(params), cache_dir.encode(), ctypes.byref(n), ctypes.byref(cs), ctypes.byref(ns)))
+    return n.value, cs.value, ns.value
+
+
 def host_sphere_mesh(params: SphereParams):
     """(X [nv,3], tets [ne,4], contact vertices) of the sphere scene's mesh (no GPU)."""
     nv, ne, nc = c_int32(), c_int32(), c_int32()
@@ -194,13 +231,18 @@ class Context:
 class Tet4Block:
     """Tet4 elastic block scene (include/symx_scenes.h) driven through the C++ front end."""
 
-    def __init__(self, params: Tet4BlockParams | None = None, newton: NewtonParams | None = None, _sphere: SphereParams | None = None):
+    def __init__(self, params: Tet4BlockParams | None = None, newton: NewtonParams | None = None, _sphere: SphereParams | None = None,
+                 _hex: HexBlockParams | None = None):
         self.newton = newton or default_newton_params()
         self.ptr = c_void_p()
         self.sphere = _sphere
+        self.hex = _hex
         if _sphere is not None:
             self.params = None
             check_scene(amd().symx_scene_sphere_create(ctypes.byref(_sphere), ctypes.byref(self.newton), ctypes.byref(self.ptr)))
+        elif _hex is not None:
+            self.params = None
+            check_scene(amd().symx_scene_hex_block_create(ctypes.byref(_hex), ctypes.byref(self.newton), ctypes.byref(self.ptr)))
         else:
             self.params = params or default_block_params()
             check_scene(amd().symx_scene_tet4_block_create(ctypes.byref(self.params), ctypes.byref(self.newton), ctypes.byref(self.ptr)))
@@ -209,6 +251,8 @@ class Tet4Block:
         self.n_vertices = n.value
         amd().symx_scene_n_elements(self.ptr, ctypes.byref(n))
         self.n_elements = n.value
+        amd().symx_scene_nodes_per_element(self.ptr, ctypes.byref(n))
+        self.nodes_per_element = n.value
 
     def close(self):
         if self.ptr:
@@ -223,7 +267,7 @@ class Tet4Block:
         return c_void_p(amd().symx_scene_problem(self.ptr))
 
     def connectivity(self) -> np.ndarray:
-        c = np.zeros((self.n_elements, 4), np.int32)
+        c = np.zeros((self.n_elements, self.nodes_per_element), np.int32)
         amd().symx_scene_connectivity(self.ptr, _ptr(c))
         return c
 
@@ -349,6 +393,18 @@ class Tet4Block:
         check_hip(hip().symx_hip_problem_solve(self._p(), c_int32(precision), c_double(abs_tol), c_double(rel_tol),
                                                c_int32(max_iter), c_int32(int(stop_on_indef)), ctypes.byref(info)))
         return {k: getattr(info, k) for k, _ in info._fields_}
+
+
+class HexBlock(Tet4Block):
+    """Hex8 / Hex27 block scene with per-element Young's moduli (include/symx_scenes.h)."""
+
+    def __init__(self, params: HexBlockParams | None = None, newton: NewtonParams | None = None):
+        super().__init__(None, newton, _hex=params or default_hex_params())
+
+    def young_moduli(self) -> np.ndarray:
+        out = np.zeros(self.n_elements)
+        check_scene(amd().symx_scene_young_moduli(self.ptr, _ptr(out)))
+        return out
 
 
 class Sphere(Tet4Block):

@@ -1,5 +1,7 @@
 #include "compile.h"
 
+#include "../../../include/symx_hip.h"
+
 #include <cinttypes>
 #include <cstdio>
 #include <cstring>
@@ -187,9 +189,11 @@ namespace symx
         }
 
         void emit_gather(std::ostringstream& os, const Graph& g, const std::vector<DataMap>& maps, const SymbolUse& use,
-                         const SummationMap& sum, const std::vector<std::array<int32_t, 3>>& adjoint_symbols)
+                         const SummationMap& sum, const std::vector<std::array<int32_t, 3>>& adjoint_symbols,
+                         const std::vector<int32_t>& preset_symbols = {})
         {
             std::vector<uint8_t> covered(g.n_symbols(), 0);
+            for (int32_t s : preset_symbols) covered[s] = 1;
             std::vector<uint8_t> col_loaded(64, 0);
             for (size_t m = 0; m < maps.size(); m++) {
                 const DataMap& dm = maps[m];
@@ -235,7 +239,7 @@ namespace symx
     std::string emit_hip_module(const std::string& module_name, Graph& g, const std::vector<DataMap>& maps, const SummationMap& sum,
                                 const std::vector<KernelSpec>& kernels, const std::string& prelude)
     {
-        if (maps.size() > 48) throw std::runtime_error("symx: too many data maps in one potential (max 48)");
+        if (maps.size() > SYMX_HIP_MAX_MAPS) throw std::runtime_error("symx: too many data maps in one potential (max " + std::to_string(SYMX_HIP_MAX_MAPS) + ")");
         std::ostringstream os;
         os << prelude << "\n// ---- generated module: " << module_name << " ----\n";
         const bool multi_q = sum.n_iterations > 1;
@@ -334,6 +338,138 @@ namespace symx
             os << "        }\n        else {\n";
             emit_zero("            ");
             os << "        }\n    }\n    symx_store_energy(e_out, p.E_partial);\n}\n";
+        }
+        return os.str();
+    }
+
+    std::string emit_hip_fem_module(const std::string& name, Graph& g, const std::vector<DataMap>& maps, const SummationMap& sum,
+                                    const FemStructure& fem, const std::string& kP, const std::string& kPG, const std::string& kPGH,
+                                    const std::string& prelude)
+    {
+        if (maps.size() > SYMX_HIP_MAX_MAPS) throw std::runtime_error("symx: too many data maps in one potential");
+        const int NN = fem.n_nodes, NQ = fem.n_q, R = (int)fem.records.size();
+        const int NB = (int)fem.block_of_node.size(), NUB = (int)fem.ub_nodes.size();
+        if (NB != NN || NUB != NN * (NN + 1) / 2 || (int)fem.dN.size() != NQ * NN * 3 || NQ > 64 || R < 1)
+            throw std::runtime_error("symx: inconsistent FEM structure");
+        std::ostringstream os;
+        os << prelude << "\n// ---- generated structured FEM module: " << name << " ----\n";
+        os << "__constant__ double " << name << "_dN[" << fem.dN.size() << "] = {";
+        for (size_t i = 0; i < fem.dN.size(); i++) os << (i ? ", " : "") << literal(fem.dN[i]);
+        os << "};\n";
+        if (sum.n_iterations > 0) {
+            if (sum.n_iterations != NQ) throw std::runtime_error("symx: FEM summation table does not match the quadrature");
+            os << "__constant__ double " << name << "_sum[" << sum.data.size() << "] = {";
+            for (size_t i = 0; i < sum.data.size(); i++) os << (i ? ", " : "") << literal(sum.data[i]);
+            os << "};\n";
+        }
+        os << "__constant__ int " << name << "_col[" << R * NN << "] = {";
+        for (int r = 0; r < R; r++)
+            for (int a = 0; a < NN; a++) os << ((r || a) ? ", " : "") << fem.records[r].cols[a];
+        os << "};\n__constant__ int " << name << "_blk[" << NN << "] = {";
+        for (int a = 0; a < NN; a++) os << (a ? ", " : "") << fem.block_of_node[a];
+        os << "};\n__constant__ int " << name << "_ub[" << 2 * NUB << "] = {";
+        for (int u = 0; u < NUB; u++) os << (u ? ", " : "") << fem.ub_nodes[u][0] << ", " << fem.ub_nodes[u][1];
+        os << "};\n";
+        os << "__device__ __forceinline__ constexpr int " << name << "_sym(int r, int c) { return r <= c ? r * 9 - (r * (r - 1)) / 2 + (c - r) "
+           << ": c * 9 - (c * (c - 1)) / 2 + (r - c); }\n";
+
+        std::vector<int32_t> preset;
+        for (const auto& rec : fem.records)
+            for (int k = 0; k < 9; k++) preset.push_back(rec.J[k]);
+
+        for (int kind = 0; kind < 3; kind++) {
+            const std::string& kname = kind == 0 ? kP : (kind == 1 ? kPG : kPGH);
+            const int NM = kind == 0 ? 1 : (kind == 1 ? 10 : 55);
+            std::vector<Scalar> outs = fem.f;
+            if (kind >= 1) outs.insert(outs.end(), fem.P.begin(), fem.P.end());
+            if (kind == 2) outs.insert(outs.end(), fem.A.begin(), fem.A.end());
+            std::vector<int32_t> roots;
+            for (const Scalar& o : outs) roots.push_back(o.id);
+            const std::vector<uint8_t> mark = reachable(g, roots);
+            const SymbolUse use = symbol_use(g, mark, sum);
+
+            os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << kname << "(const symx_hip_eval_params p)\n{\n";
+            os << "    __shared__ double s_pos[4][" << R * NN * 3 << "];\n";
+            os << "    __shared__ double s_mat[4][" << NQ * NM << "];\n";
+            os << "    __shared__ double s_e[4];\n";
+            os << "    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;\n";
+            os << "    const int e = symx_block() * 4 + wv;\n";
+            os << "    const bool valid = e < p.n_elem;\n";
+            os << "    const bool act = valid && (p.active == nullptr || p.active[e] != 0);\n";
+            // node positions of every record into LDS
+            os << "    if (act) {\n        for (int t = lane; t < " << R * NN << "; t += 64) {\n";
+            os << "            const int r = t / " << NN << ";\n            const double* base = ";
+            for (int r = 0; r < R - 1; r++) os << "r == " << r << " ? p.maps[" << fem.records[r].map_base << "] : ";
+            os << "p.maps[" << fem.records[R - 1].map_base << "];\n";
+            os << "            const long long i = (long long)p.conn[(long long)" << name << "_col[t] * p.n_elem + e];\n";
+            os << "            s_pos[wv][t * 3 + 0] = base[i * 3 + 0];\n            s_pos[wv][t * 3 + 1] = base[i * 3 + 1];\n"
+               << "            s_pos[wv][t * 3 + 2] = base[i * 3 + 2];\n        }\n    }\n    __syncthreads();\n";
+            // material law per quadrature point (lane q)
+            os << "    if (act && lane < " << NQ << ") {\n        const int q = lane;\n";
+            emit_gather(os, g, maps, use, sum, {}, preset);
+            if (use.uses_summation)
+                for (int k = 0; k < sum.stride; k++) {
+                    const int sidx = sum.first_symbol_idx + k;
+                    if (use.used[sidx]) os << "        const double s" << sidx << " = " << name << "_sum[q * " << sum.stride << " + " << k << "];\n";
+                }
+            for (int r = 0; r < R; r++) {
+                bool any = false;
+                for (int k = 0; k < 9; k++) any = any || use.used[fem.records[r].J[k]];
+                if (!any) continue;
+                os << "        double J" << r << "[9] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };\n";
+                os << "        for (int a = 0; a < " << NN << "; a++) {\n";
+                os << "            const double* xa = &s_pos[wv][(" << r * NN << " + a) * 3];\n";
+                os << "            const double* d = &" << name << "_dN[(q * " << NN << " + a) * 3];\n";
+                os << "            #pragma unroll\n            for (int i = 0; i < 3; i++) {\n"
+                   << "                J" << r << "[i * 3 + 0] += xa[i] * d[0];\n                J" << r << "[i * 3 + 1] += xa[i] * d[1];\n"
+                   << "                J" << r << "[i * 3 + 2] += xa[i] * d[2];\n            }\n        }\n";
+                for (int k = 0; k < 9; k++)
+                    if (use.used[fem.records[r].J[k]]) os << "        const double s" << fem.records[r].J[k] << " = J" << r << "[" << k << "];\n";
+            }
+            emit_body(os, g, mark, "        ");
+            for (int m = 0; m < NM; m++) os << "        s_mat[wv][q * " << NM << " + " << m << "] = " << ref(g, outs[m].id) << ";\n";
+            os << "    }\n    __syncthreads();\n";
+            // contraction
+            os << "    double e_out = 0.0;\n    if (act) {\n";
+            os << "        if (lane == 0)\n            for (int q = 0; q < " << NQ << "; q++) e_out += s_mat[wv][q * " << NM << "];\n";
+            if (kind >= 1) {
+                os << "        for (int t = lane; t < " << 3 * NN << "; t += 64) {\n";
+                os << "            const int a = t / 3, i = t - 3 * a;\n            double g = 0.0;\n";
+                os << "            for (int q = 0; q < " << NQ << "; q++) {\n";
+                os << "                const double* d = &" << name << "_dN[(q * " << NN << " + a) * 3];\n";
+                os << "                const double* P = &s_mat[wv][q * " << NM << " + 1 + i * 3];\n";
+                os << "                g += P[0] * d[0] + P[1] * d[1] + P[2] * d[2];\n            }\n";
+                os << "            p.G[(long long)e * " << 3 * NN << " + " << name << "_blk[a] * 3 + i] = g;\n        }\n";
+            }
+            if (kind == 2) {
+                os << "        for (int ub = lane; ub < " << NUB << "; ub += 64) {\n";
+                os << "            const int a = " << name << "_ub[2 * ub], b = " << name << "_ub[2 * ub + 1];\n";
+                os << "            double h[9] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };\n";
+                os << "            for (int q = 0; q < " << NQ << "; q++) {\n";
+                os << "                const double* da = &" << name << "_dN[(q * " << NN << " + a) * 3];\n";
+                os << "                const double* db = &" << name << "_dN[(q * " << NN << " + b) * 3];\n";
+                os << "                const double* A = &s_mat[wv][q * " << NM << " + 10];\n";
+                os << "                const double da0 = da[0], da1 = da[1], da2 = da[2], db0 = db[0], db1 = db[1], db2 = db[2];\n";
+                os << "                #pragma unroll\n                for (int i = 0; i < 3; i++) {\n";
+                os << "                    #pragma unroll\n                    for (int j = 0; j < 3; j++) {\n";
+                os << "                        double acc = 0.0;\n";
+                os << "                        #pragma unroll\n                        for (int k = 0; k < 3; k++) {\n";
+                os << "                            const double dk = k == 0 ? da0 : (k == 1 ? da1 : da2);\n";
+                os << "                            acc += dk * (A[" << name << "_sym(i * 3 + k, j * 3 + 0)] * db0 + A[" << name
+                   << "_sym(i * 3 + k, j * 3 + 1)] * db1 + A[" << name << "_sym(i * 3 + k, j * 3 + 2)] * db2);\n";
+                os << "                        }\n                        h[i * 3 + j] += acc;\n                    }\n                }\n            }\n";
+                os << "            double* dst = p.H + (long long)e * " << NUB * 9 << " + ub * 9;\n";
+                os << "            #pragma unroll\n            for (int k = 0; k < 9; k++) dst[k] = h[k];\n        }\n";
+            }
+            os << "    }\n";
+            if (kind >= 1) {
+                os << "    else if (valid) {\n";
+                os << "        for (int t = lane; t < " << 3 * NN << "; t += 64) p.G[(long long)e * " << 3 * NN << " + t] = 0.0;\n";
+                if (kind == 2) os << "        for (int t = lane; t < " << NUB * 9 << "; t += 64) p.H[(long long)e * " << NUB * 9 << " + t] = 0.0;\n";
+                os << "    }\n";
+            }
+            os << "    if (lane == 0) s_e[wv] = e_out;\n    __syncthreads();\n";
+            os << "    if (threadIdx.x == 0) p.E_partial[symx_block()] = ((s_e[0] + s_e[1]) + s_e[2]) + s_e[3];\n}\n";
         }
         return os.str();
     }

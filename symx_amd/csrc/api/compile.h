@@ -228,6 +228,19 @@ namespace symx
         // SecondOrderCompiledPotential.cpp:10-33 via MappedWorkspace.h:800-825).
         std::vector<int32_t> maps_of_container(std::uintptr_t id) const;
 
+        // Isoparametric Jacobians that fem_jacobian deferred inside fem_integrator: the
+        // expression uses the 9 symbols J (row-major, J_ik = d x_i / d xi_k), and J equals
+        // `expanded` = sum over nodes of x_a (x) dN_a/dxi. Code generation either keeps J
+        // (structured FEM kernels) or substitutes `expanded` back.
+        struct FemJacobian
+        {
+            int32_t n_nodes = 0;
+            std::vector<int32_t> node_symbols;  // [n_nodes][3] symbol ids of the node positions
+            std::array<int32_t, 9> J{};         // graph nodes of the deferred symbols
+            std::array<int32_t, 9> expanded{};  // graph nodes of the expansion
+        };
+        std::vector<FemJacobian> fem_jacobians;
+
     private:
         Workspace ws;
         std::vector<DataMap> maps;
@@ -260,6 +273,32 @@ namespace symx
     // symx_hip_eval_params.
     std::string emit_hip_module(const std::string& module_name, Graph& g, const std::vector<DataMap>& maps, const SummationMap& sum,
                                 const std::vector<KernelSpec>& kernels, const std::string& prelude);
+
+    // Structured FEM element kernels (one wavefront per element, four per workgroup):
+    // the expression depends on the DoFs only through the deferred Jacobian of record 0,
+    //   E_e = sum_q f(J_q, ...),  J_q = sum_a x_a (x) dN_a(xi_q),
+    // so g_a = sum_q P_q dN_a and H_ab = sum_q dN_a^T A_q dN_b with P = df/dJ and
+    // A = d2f/dJ2 generated symbolically (9 and 45 outputs) and the contraction written
+    // by hand. Further records (e.g. rest positions) are data Jacobians computed the same way.
+    struct FemStructure
+    {
+        struct Record
+        {
+            int32_t map_base = -1;             // data map whose array holds the node positions
+            std::vector<int32_t> cols;         // connectivity column per node
+            std::array<int32_t, 9> J{};        // symbol ids of the deferred Jacobian
+        };
+        int32_t n_nodes = 0;
+        int32_t n_q = 0;
+        std::vector<double> dN;                // [n_q][n_nodes][3]
+        std::vector<Record> records;           // records[0]: DoF positions
+        std::vector<int32_t> block_of_node;    // DoF block of each node
+        std::vector<std::array<int32_t, 2>> ub_nodes;  // nodes of each upper block (A <= B, block order)
+        std::vector<Scalar> f, P, A;           // f; df/dJ (9); d2f/dJ2 upper triangle (45)
+    };
+    std::string emit_hip_fem_module(const std::string& module_name, Graph& g, const std::vector<DataMap>& maps, const SummationMap& sum,
+                                    const FemStructure& fem, const std::string& kP, const std::string& kPG, const std::string& kPGH,
+                                    const std::string& prelude);
 
     // Emit a standalone kernel evaluating `outputs` for a batch of input vectors
     // (Compiled<double>): in [n][n_inputs], out [n][n_outputs].

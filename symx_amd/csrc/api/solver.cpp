@@ -191,6 +191,129 @@ namespace symx
         }
     }
 
+    // ---- structured FEM kernels ------------------------------------------------------------
+    // The potential qualifies when it has no condition and no adjoint parameters, exactly
+    // one deferred Jacobian is built from all of its DoFs (one DoF vector per node), every
+    // other deferred Jacobian reads non-DoF data, each record's nodes come from one array
+    // through the connectivity, and the DoFs enter the expression through that Jacobian only.
+    bool build_fem_structure(GlobalPotential& G, const Potential& P, const std::vector<Scalar>& dofs, FemStructure& fem)
+    {
+        MappedWorkspace<double>& mws = *P.mws;
+        Graph* g = mws.workspace().graph();
+        const auto& maps = mws.get_maps();
+        const SummationMap& sum = mws.get_summation();
+        if (P.has_condition || sum.n_iterations <= 0 || sum.n_iterations > 64) return false;
+        for (const auto& ps : G.get_parameter_sets())
+            if (!mws.maps_of_container(ps.id()).empty()) return false;
+        std::unordered_map<int32_t, int32_t> dof_pos;  // DoF symbol -> position in dofs
+        for (int32_t i = 0; i < (int32_t)dofs.size(); i++) dof_pos[dofs[i].get_symbol_idx()] = i;
+        // symbol -> (map, component)
+        std::vector<std::array<int32_t, 2>> sym_map(g->n_symbols(), { -1, -1 });
+        for (int32_t m = 0; m < (int32_t)maps.size(); m++)
+            for (int k = 0; k < maps[m].stride; k++) sym_map[maps[m].first_symbol_idx + k] = { m, k };
+        int dof_rec = -1;
+        for (int r = 0; r < (int)mws.fem_jacobians.size(); r++) {
+            const auto& rec = mws.fem_jacobians[r];
+            int n_dof = 0;
+            for (int32_t sidx : rec.node_symbols) n_dof += dof_pos.count(sidx) ? 1 : 0;
+            if (n_dof == 0) continue;
+            if (n_dof != (int)rec.node_symbols.size() || n_dof != (int)dofs.size() || dof_rec >= 0) return false;
+            dof_rec = r;
+        }
+        if (dof_rec < 0) return false;
+        // DoFs must not appear in the expression outside the deferred Jacobians
+        {
+            std::vector<uint8_t> seen(g->size(), 0);
+            std::vector<int32_t> stack = { P.expression.id };
+            while (!stack.empty()) {
+                const int32_t id = stack.back();
+                stack.pop_back();
+                if (id < 0 || seen[id]) continue;
+                seen[id] = 1;
+                const Node& nd = g->node(id);
+                if (nd.op == Op::Symbol) {
+                    if (dof_pos.count(nd.a)) return false;
+                    continue;
+                }
+                if (nd.op == Op::Zero || nd.op == Op::One || nd.op == Op::Const) continue;
+                stack.push_back(nd.a);
+                if (nd.op == Op::Branch) {
+                    stack.push_back(nd.b);
+                    stack.push_back(nd.c);
+                }
+                else if (nd.op != Op::PowN && nd.b >= 0) {
+                    stack.push_back(nd.b);
+                }
+            }
+        }
+        const auto& drec = mws.fem_jacobians[dof_rec];
+        const int NN = drec.n_nodes;
+        std::vector<int> order = { dof_rec };
+        for (int r = 0; r < (int)mws.fem_jacobians.size(); r++)
+            if (r != dof_rec) order.push_back(r);
+        fem = FemStructure();
+        fem.n_nodes = NN;
+        fem.n_q = sum.n_iterations;
+        for (int r : order) {
+            const auto& rec = mws.fem_jacobians[r];
+            if (rec.n_nodes != NN) return false;
+            FemStructure::Record fr;
+            for (int k = 0; k < 9; k++) fr.J[k] = g->node(rec.J[k]).a;
+            std::uintptr_t container = 0;
+            for (int a = 0; a < NN; a++) {
+                const auto mk = sym_map[rec.node_symbols[3 * a]];
+                if (mk[0] < 0 || mk[1] != 0) return false;
+                const DataMap& dm = maps[mk[0]];
+                if (dm.stride != 3 || dm.connectivity_index < 0) return false;
+                for (int k = 1; k < 3; k++)
+                    if (sym_map[rec.node_symbols[3 * a + k]] != std::array<int32_t, 2>{ mk[0], k }) return false;
+                if (a == 0) {
+                    container = dm.id();
+                    fr.map_base = mk[0];
+                }
+                else if (dm.id() != container) {
+                    return false;
+                }
+                fr.cols.push_back(dm.connectivity_index);
+            }
+            fem.records.push_back(fr);
+        }
+        // DoF block of each node
+        fem.block_of_node.resize(NN);
+        std::vector<int32_t> node_of_block(NN, -1);
+        for (int a = 0; a < NN; a++) {
+            const int32_t pos = dof_pos.at(drec.node_symbols[3 * a]);
+            if (pos % 3 != 0 || dof_pos.at(drec.node_symbols[3 * a + 1]) != pos + 1 || dof_pos.at(drec.node_symbols[3 * a + 2]) != pos + 2) return false;
+            fem.block_of_node[a] = pos / 3;
+            node_of_block[pos / 3] = a;
+        }
+        for (int A = 0; A < NN; A++)
+            for (int B = A; B < NN; B++) fem.ub_nodes.push_back({ node_of_block[A], node_of_block[B] });
+        // dN_a/dxi_k at every quadrature point: the coefficients of x_a in the expansion
+        {
+            std::vector<int32_t> ids;
+            for (int a = 0; a < NN; a++)
+                for (int k = 0; k < 3; k++)
+                    ids.push_back(diff(Scalar(g, drec.expanded[k]), Scalar(g, g->symbol_node(drec.node_symbols[3 * a]))).id);
+            fem.dN.resize((size_t)fem.n_q * NN * 3);
+            for (int q = 0; q < fem.n_q; q++) {
+                for (int k = 0; k < sum.stride; k++) g->set_symbol_value(sum.first_symbol_idx + k, sum.data[(size_t)q * sum.stride + k]);
+                const std::vector<double> v = g->eval_many(ids);
+                for (size_t i = 0; i < v.size(); i++) fem.dN[(size_t)q * NN * 3 + i] = v[i];
+            }
+        }
+        // material law and its J-derivatives
+        std::vector<Scalar> Jd;
+        for (int k = 0; k < 9; k++) Jd.push_back(Scalar(g, drec.J[k]));
+        DiffCache dc;
+        fem.f = { P.expression };
+        fem.P = gradient(P.expression, Jd, dc).values();
+        const Matrix A = gradient(Vector(fem.P), Jd, true, dc);
+        for (int r = 0; r < 9; r++)
+            for (int c = r; c < 9; c++) fem.A.push_back(A(r, c));
+        return true;
+    }
+
     // ---- per-potential code generation (SecondOrderCompiledPotential.cpp:8-70) ----------
     PotentialModule generate_potential_module(GlobalPotential& G, const Potential& P)
     {
@@ -221,17 +344,47 @@ namespace symx
         }
         if (dofs.empty()) throw std::runtime_error("symx: potential with zero DoFs: " + P.name);
         const int32_t n = (int32_t)dofs.size();
+        const std::string base = identifier(P.name);
+
+        // Deferred isoparametric Jacobians: structured kernels, or substitute them back.
+        Scalar expression = P.expression;
+        Scalar condition = P.condition;
+        if (!mws.fem_jacobians.empty()) {
+            if (build_fem_structure(G, P, dofs, pm.fem)) {
+                pm.structured = true;
+                DeviceGlobal::PotentialInfo& info = pm.info;
+                info.name = P.name;
+                info.n_dofs = n;
+                pm.specs.resize(3);
+                pm.specs[0] = KernelSpec{ base + "_P", KernelKind::P, pm.fem.f, n };
+                pm.specs[1] = KernelSpec{ base + "_P__dP_du", KernelKind::PG, collect_scalars(std::vector<std::vector<Scalar>>{ pm.fem.f, pm.fem.P }), n };
+                pm.specs[2] = KernelSpec{ base + "_P__dP_du__d2P_du2", KernelKind::PGH, collect_scalars(std::vector<std::vector<Scalar>>{ pm.fem.f, pm.fem.P, pm.fem.A }), n };
+                pm.source = emit_hip_fem_module(base, *g, maps, mws.get_summation(), pm.fem, pm.specs[0].name, pm.specs[1].name, pm.specs[2].name,
+                                                symx_hip_eval_prelude());
+                info.ops_P = count_operations(*g, pm.specs[0].outputs);
+                info.ops_PG = count_operations(*g, pm.specs[1].outputs);
+                info.ops_PGH = count_operations(*g, pm.specs[2].outputs);
+                info.source = pm.source;
+                pm.elems_per_group = 4;
+                return pm;
+            }
+            std::unordered_map<int32_t, int32_t> rep;
+            for (const auto& rec : mws.fem_jacobians)
+                for (int k = 0; k < 9; k++) rep[rec.J[k]] = rec.expanded[k];
+            const std::vector<int32_t> r = g->substitute({ P.expression.id, P.has_condition ? P.condition.id : -1 }, rep);
+            expression = Scalar(g, r[0]);
+            if (P.has_condition) condition = Scalar(g, r[1]);
+        }
 
         DiffCache dc;
-        const Vector grad = gradient(P.expression, dofs, dc);
+        const Vector grad = gradient(expression, dofs, dc);
         const Matrix hess = gradient(grad, dofs, true, dc);
-        const std::string base = identifier(P.name);
         std::vector<KernelSpec>& specs = pm.specs;
         specs.resize(3);
-        specs[0] = KernelSpec{ base + "_P", KernelKind::P, { P.expression }, n };
-        specs[1] = KernelSpec{ base + "_P__dP_du", KernelKind::PG, collect_scalars({ { P.expression }, grad.values() }), n };
-        specs[2] = KernelSpec{ base + "_P__dP_du__d2P_du2", KernelKind::PGH, collect_scalars({ { P.expression }, grad.values(), hess.values() }), n };
-        if (P.has_condition) specs.push_back(KernelSpec{ base + "_C", KernelKind::C, { P.condition }, n });
+        specs[0] = KernelSpec{ base + "_P", KernelKind::P, { expression }, n };
+        specs[1] = KernelSpec{ base + "_P__dP_du", KernelKind::PG, collect_scalars({ { expression }, grad.values() }), n };
+        specs[2] = KernelSpec{ base + "_P__dP_du__d2P_du2", KernelKind::PGH, collect_scalars({ { expression }, grad.values(), hess.values() }), n };
+        if (P.has_condition) specs.push_back(KernelSpec{ base + "_C", KernelKind::C, { condition }, n });
 
         DeviceGlobal::PotentialInfo& info = pm.info;
         info.name = P.name;
@@ -345,6 +498,7 @@ namespace symx
             d.dof_set_offset = dof_off.data();
             d.n_maps = (int32_t)mdesc.size();
             d.maps = mdesc.data();
+            d.elems_per_group = pm.elems_per_group;
             int32_t pid = -1;
             check(symx_hip_problem_add_potential(prob, &d, &pid), hip, ("adding potential " + P.name).c_str());
             infos.push_back(info);
@@ -937,7 +1091,7 @@ namespace symx
     {
         (void)cache_id;
         if (expr.empty()) throw std::runtime_error("symx: CompiledInLoop without expressions");
-        if ((int32_t)mws->get_maps().size() > SYMX_HIP_MAX_MAPS) throw std::runtime_error("symx: too many data maps in one loop (max 48)");
+        if ((int32_t)mws->get_maps().size() > SYMX_HIP_MAX_MAPS) throw std::runtime_error("symx: too many data maps in one loop (max 128)");
         n_out = (int32_t)expr.size();
         KernelSpec ks;
         ks.name = kernel;
@@ -1030,34 +1184,60 @@ namespace symx
                  { wf, 0.0, p, 0.0 }, { wf, 0.0, 0.0, p }, { wm, 0.0, 0.0, 0.0 } };
     }
 
+    // Node positions of Hex8 / Hex27 elements in the reference element [-1, 1]^3 as 0 (-1),
+    // 1 (0), 2 (+1) per axis, in SymX's node order (fem_integrators.h:84-150: corners as
+    // VTK, then mid-edges of the bottom, top and middle layers, mid-faces -x +x +y -y -z +z,
+    // and the centre). The Hex27 mesh generator uses the same table.
+    static const int8_t k_hex27_nodes[27][3] = {
+        { 0, 0, 0 }, { 2, 0, 0 }, { 2, 2, 0 }, { 0, 2, 0 }, { 0, 0, 2 }, { 2, 0, 2 }, { 2, 2, 2 }, { 0, 2, 2 },
+        { 1, 0, 0 }, { 2, 1, 0 }, { 1, 2, 0 }, { 0, 1, 0 }, { 1, 0, 2 }, { 2, 1, 2 }, { 1, 2, 2 }, { 0, 1, 2 },
+        { 0, 0, 1 }, { 2, 0, 1 }, { 2, 2, 1 }, { 0, 2, 1 },
+        { 0, 1, 1 }, { 2, 1, 1 }, { 1, 2, 1 }, { 1, 0, 1 }, { 1, 1, 0 }, { 1, 1, 2 },
+        { 1, 1, 1 } };
+
     static std::vector<Scalar> shape_functions(const FEM_Element& e, const Vector& xi)
     {
         const Scalar& u = xi[0];
         const Scalar& v = xi[1];
         const Scalar& w = xi[2];
         if (e == FEM_Element::Tet4) return { 1.0 - u - v - w, u, v, w };
+        if (e == FEM_Element::Tet10) {
+            // quadratic tetrahedron: vertices, then edge midpoints 01 12 20 03 13 23
+            const Scalar n0 = 1.0 - u - v - w;
+            return { n0 * (2.0 * n0 - 1.0), u * (2.0 * u - 1.0), v * (2.0 * v - 1.0), w * (2.0 * w - 1.0),
+                     4.0 * n0 * u, 4.0 * u * v, 4.0 * v * n0, 4.0 * n0 * w, 4.0 * u * w, 4.0 * v * w };
+        }
         if (e == FEM_Element::Hex8) {
-            // corners in lexicographic order (a + 2b + 4c), a,b,c in {0,1} <-> xi in {-1,+1}
+            // trilinear: 0.5 (1 -+ xi) per axis, corners in the order of k_hex27_nodes[0..7]
+            const Scalar m[3] = { 0.5 * (1.0 - u), 0.5 * (1.0 - v), 0.5 * (1.0 - w) };
+            const Scalar pl[3] = { 0.5 * (1.0 + u), 0.5 * (1.0 + v), 0.5 * (1.0 + w) };
             std::vector<Scalar> N;
-            for (int c = 0; c < 2; c++)
-                for (int b = 0; b < 2; b++)
-                    for (int a = 0; a < 2; a++) {
-                        const double sa = a ? 1.0 : -1.0, sb = b ? 1.0 : -1.0, sc = c ? 1.0 : -1.0;
-                        N.push_back(0.125 * (1.0 + sa * u) * (1.0 + sb * v) * (1.0 + sc * w));
-                    }
+            for (int n = 0; n < 8; n++) {
+                const Scalar& a = k_hex27_nodes[n][0] ? pl[0] : m[0];
+                const Scalar& b = k_hex27_nodes[n][1] ? pl[1] : m[1];
+                const Scalar& c = k_hex27_nodes[n][2] ? pl[2] : m[2];
+                N.push_back(a * b * c);
+            }
             return N;
         }
         if (e == FEM_Element::Hex27) {
-            // quadratic Lagrange on nodes {-1, 0, 1}; node index a + 3b + 9c
-            auto L = [](const Scalar& x, int i) -> Scalar {
-                if (i == 0) return 0.5 * x * (x - 1.0);
-                if (i == 1) return 1.0 - x * x;
-                return 0.5 * x * (x + 1.0);
+            // tri-quadratic Lagrange from x(x - 1), x(x + 1), 1 - x^2 per axis, scaled by 1/8
+            // (corners), 1/4 (edges), 1/2 (faces), 1 (centre)
+            const Scalar* X[3] = { &u, &v, &w };
+            auto f = [&](int axis, int pos) -> Scalar {
+                const Scalar& x = *X[axis];
+                if (pos == 0) return x * (x - 1.0);
+                if (pos == 2) return x * (x + 1.0);
+                return 1.0 - x.powN(2);
             };
             std::vector<Scalar> N;
-            for (int c = 0; c < 3; c++)
-                for (int b = 0; b < 3; b++)
-                    for (int a = 0; a < 3; a++) N.push_back(L(u, a) * L(v, b) * L(w, c));
+            for (int n = 0; n < 27; n++) {
+                int n_mid = 0;
+                for (int d = 0; d < 3; d++) n_mid += k_hex27_nodes[n][d] == 1;
+                const double scale = n_mid == 0 ? 8.0 : (n_mid == 1 ? 4.0 : (n_mid == 2 ? 2.0 : 1.0));
+                Scalar prod = f(0, k_hex27_nodes[n][0]) * f(1, k_hex27_nodes[n][1]) * f(2, k_hex27_nodes[n][2]);
+                N.push_back(scale == 1.0 ? prod : prod / scale);
+            }
             return N;
         }
         throw std::runtime_error("symx: shape functions for " + get_name(e) + " are not implemented");
@@ -1071,12 +1251,54 @@ namespace symx
         for (size_t i = 1; i < N.size(); i++) x = x + xh[i] * N[i];
         return x;
     }
+    namespace
+    {
+        // Set while fem_integrator evaluates its summand: fem_jacobian of node positions at
+        // the integrator's xi may then be deferred (MappedWorkspace::fem_jacobians).
+        struct FemScope
+        {
+            MappedWorkspace<double>* mws;
+            FEM_Element element;
+            std::array<int32_t, 3> xi;
+        };
+        thread_local FemScope* g_fem_scope = nullptr;
+        bool g_structured_fem = true;
+    }
+    void set_structured_fem(bool enabled) { g_structured_fem = enabled; }
+    bool get_structured_fem() { return g_structured_fem; }
+
     Matrix fem_jacobian(const FEM_Element& e, const std::vector<Vector>& xh, const Vector& xi)
     {
         const Vector x = fem_interpolation(e, xh, xi);
         std::vector<Scalar> vals;
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) vals.push_back(diff(x[i], xi[j]));
+        FemScope* sc = g_fem_scope;
+        // Elements of 8 and more nodes defer J for the structured kernels.
+        if (sc && g_structured_fem && sc->element == e && get_stride(e) >= 8 && (int)xh.size() == get_stride(e) && xi.size() == 3) {
+            bool ok = true;
+            for (int k = 0; k < 3; k++) ok = ok && xi[k].id == sc->xi[k];
+            MappedWorkspace<double>::FemJacobian rec;
+            rec.n_nodes = (int32_t)xh.size();
+            for (const Vector& v : xh) {
+                if (v.size() != 3) ok = false;
+                for (int32_t k = 0; ok && k < v.size(); k++) {
+                    if (!v[k].is_symbol()) ok = false;
+                    else rec.node_symbols.push_back(v[k].get_symbol_idx());
+                }
+            }
+            if (ok) {
+                Graph* g = xi[0].g;
+                std::vector<Scalar> J;
+                for (int k = 0; k < 9; k++) {
+                    rec.expanded[k] = vals[k].id;
+                    rec.J[k] = g->symbol();
+                    J.push_back(Scalar(g, rec.J[k]));
+                }
+                sc->mws->fem_jacobians.push_back(rec);
+                return Matrix(J, { 3, 3 });
+            }
+        }
         return Matrix(vals, { 3, 3 });
     }
     Scalar fem_integrator(MappedWorkspace<double>& mws, const FEM_Element& e, std::function<Scalar(Scalar& w, Vector& xi)> summand)
@@ -1085,7 +1307,18 @@ namespace symx
         return mws.add_for_each(rule, [&](Vector& ip) {
             Scalar w = ip[0];
             Vector xi({ ip[1], ip[2], ip[3] });
-            return summand(w, xi);
+            FemScope scope{ &mws, e, { xi[0].id, xi[1].id, xi[2].id } };
+            FemScope* prev = g_fem_scope;
+            g_fem_scope = &scope;
+            try {
+                Scalar r = summand(w, xi);
+                g_fem_scope = prev;
+                return r;
+            }
+            catch (...) {
+                g_fem_scope = prev;
+                throw;
+            }
         });
     }
 
@@ -1114,6 +1347,25 @@ namespace symx
         return m;
     }
 
+    FEMMesh generate_cuboid_Hex8_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& n, const std::array<double, 3>& origin)
+    {
+        FEMMesh m;
+        m.element_type = FEM_Element::Hex8;
+        m.connectivity_stride = 8;
+        const double d[3] = { size[0] / n[0], size[1] / n[1], size[2] / n[2] };
+        for (int i = 0; i <= n[0]; i++)
+            for (int j = 0; j <= n[1]; j++)
+                for (int k = 0; k <= n[2]; k++) m.vertices.push_back({ origin[0] + i * d[0], origin[1] + j * d[1], origin[2] + k * d[2] });
+        auto idx = [&](int i, int j, int k) { return (int32_t)(i * (n[1] + 1) * (n[2] + 1) + j * (n[2] + 1) + k); };
+        for (int i = 0; i < n[0]; i++)
+            for (int j = 0; j < n[1]; j++)
+                for (int k = 0; k < n[2]; k++)
+                    for (int c = 0; c < 8; c++)
+                        m.connectivity.push_back(idx(i + k_hex27_nodes[c][0] / 2, j + k_hex27_nodes[c][1] / 2, k + k_hex27_nodes[c][2] / 2));
+        m.n_elements = (int32_t)(m.connectivity.size() / 8);
+        return m;
+    }
+
     FEMMesh generate_cuboid_Hex27_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& n, const std::array<double, 3>& origin)
     {
         FEMMesh m;
@@ -1128,9 +1380,8 @@ namespace symx
         for (int i = 0; i < n[0]; i++)
             for (int j = 0; j < n[1]; j++)
                 for (int k = 0; k < n[2]; k++)
-                    for (int c = 0; c < 3; c++)
-                        for (int b = 0; b < 3; b++)
-                            for (int a = 0; a < 3; a++) m.connectivity.push_back(idx(2 * i + a, 2 * j + b, 2 * k + c));
+                    for (int c = 0; c < 27; c++)
+                        m.connectivity.push_back(idx(2 * i + k_hex27_nodes[c][0], 2 * j + k_hex27_nodes[c][1], 2 * k + k_hex27_nodes[c][2]));
         m.n_elements = (int32_t)(m.connectivity.size() / 27);
         return m;
     }

@@ -348,7 +348,15 @@ namespace symx
         std::vector<int32_t> dof_col;        // connectivity column per DoF block
         std::vector<int32_t> dof_block_set;  // DoF set per DoF block
         std::vector<int32_t> map_dof_set;    // DoF set per data map (-1: not a DoF)
+        bool structured = false;             // structured FEM kernels (see FemStructure)
+        FemStructure fem;
+        int32_t elems_per_group = 256;
     };
+    bool build_fem_structure(GlobalPotential& G, const Potential& P, const std::vector<Scalar>& dofs, FemStructure& fem);
+    // Structured FEM kernels for elements of 8+ nodes (on by default). Off: every
+    // fem_integrator expression is expanded and compiled as a generic element kernel.
+    void set_structured_fem(bool enabled);
+    bool get_structured_fem();
     PotentialModule generate_potential_module(GlobalPotential& G, const Potential& P);
 
     // ---- views (View.h) ----------------------------------------------------------------
@@ -579,7 +587,11 @@ namespace symx
     // 6-tetrahedra pattern of mesh_generation.cpp:78-83; vertices ordered i-major.
     FEMMesh generate_cuboid_Tet4_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
                                       const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
-    // Cuboid of Hex27 elements (quadratic Lagrange, nodes on a (2n+1)^3 lattice).
+    // Cuboid of Hex8 elements (mesh_generation.cpp:161-206 node order; no RCM renumbering).
+    FEMMesh generate_cuboid_Hex8_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
+                                      const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
+    // Cuboid of Hex27 elements (quadratic Lagrange, nodes on a (2n+1)^3 lattice, element node
+    // order of mesh_generation.cpp:209-300; no RCM renumbering).
     FEMMesh generate_cuboid_Hex27_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
                                        const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
 }

@@ -146,6 +146,49 @@ namespace symx
         return find_or_add(n);
     }
 
+    std::vector<int32_t> Graph::substitute(const std::vector<int32_t>& roots, const std::unordered_map<int32_t, int32_t>& replace)
+    {
+        const int32_t n0 = size();
+        std::vector<uint8_t> mark(n0, 0);
+        std::vector<int32_t> stack(roots.begin(), roots.end());
+        while (!stack.empty()) {
+            const int32_t id = stack.back();
+            stack.pop_back();
+            if (id < 0 || mark[id]) continue;
+            mark[id] = 1;
+            if (replace.count(id)) continue;
+            const Node& n = nodes[id];
+            if (n.op == Op::Symbol || n.op == Op::Zero || n.op == Op::One || n.op == Op::Const) continue;
+            stack.push_back(n.a);
+            if (n.op == Op::Branch) {
+                stack.push_back(n.b);
+                stack.push_back(n.c);
+            }
+            else if (n.op != Op::PowN && n.b >= 0) {
+                stack.push_back(n.b);
+            }
+        }
+        std::vector<int32_t> map(n0, -1);
+        for (int32_t id = 0; id < n0; id++) {  // operands precede their users
+            if (!mark[id]) continue;
+            auto it = replace.find(id);
+            if (it != replace.end()) {
+                map[id] = it->second;
+                continue;
+            }
+            const Node n = nodes[id];  // copy: op() may grow the node array
+            switch (n.op) {
+            case Op::Zero: case Op::One: case Op::Const: case Op::Symbol: map[id] = id; break;
+            case Op::Branch: map[id] = branch(map[n.c], map[n.a], map[n.b]); break;
+            case Op::PowN: map[id] = op(Op::PowN, map[n.a], n.b); break;
+            default: map[id] = op(n.op, map[n.a], n.b >= 0 ? map[n.b] : -1); break;
+            }
+        }
+        std::vector<int32_t> out;
+        for (int32_t r : roots) out.push_back(r < 0 ? r : map[r]);
+        return out;
+    }
+
     int32_t Graph::branch(int32_t cond, int32_t pos, int32_t neg)
     {
         if (pos == neg) return pos;

@@ -68,6 +68,11 @@ namespace symx
         // visit order) - used as a content key for the code-object cache.
         std::string canonical(const std::vector<int32_t>& roots) const;
 
+        // Rebuild the sub-DAGs of `roots` with the nodes in `replace` (node id -> node id)
+        // substituted, through op()/branch() so that folding and CSE apply; returns the
+        // new roots.
+        std::vector<int32_t> substitute(const std::vector<int32_t>& roots, const std::unordered_map<int32_t, int32_t>& replace);
+
     private:
         struct KeyHash { size_t operator()(const std::array<int64_t, 4>& k) const; };
         int32_t find_or_add(const Node& n);

@@ -80,8 +80,10 @@ namespace symx_hip
     {
         int32_t is_dof = 0;
         int32_t dof_offset = 0;
-        double* d_data = nullptr;  // owned copy of non-DoF data
+        double* d_data = nullptr;  // device copy of non-DoF data
         int64_t n_values = 0;
+        bool owner = true;           // false: d_data belongs to an earlier map of the same host array
+        const double* host = nullptr;
     };
 
     struct DevPotential
@@ -102,6 +104,7 @@ namespace symx_hip
         int64_t part_base = 0;              // first energy partial
         int64_t e_base = 0;                 // first element in problem-wide element numbering
         int32_t n_groups = 0;               // workgroups of the element kernel
+        int32_t elems_per_group = SYMX_HIP_BLOCK;
     };
 }
 

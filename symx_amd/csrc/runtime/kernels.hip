@@ -297,6 +297,141 @@ namespace symx_hip
         count_changed(changed, count);
     }
 
+    // Large element Hessians (8 and more DoF blocks: Hex8, Tet10, Hex27): one workgroup
+    // per matrix, parallel cyclic Jacobi in LDS with the round-robin pairing of Brent and
+    // Luk (n/2 disjoint rotations per round, n-1 rounds per sweep). Each rotation is the
+    // one of jacobi_rotate; A <- J^T A J as a column pass then a row pass, V <- V J.
+    template <int NB>
+    __global__ __launch_bounds__(256) void k_project_ub_lds(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
+                                                           unsigned long long* count, const uint8_t* __restrict__ select)
+    {
+        constexpr int N = 3 * NB, NP = (N + 1) & ~1, LD = NP + 1, NUB = NB * (NB + 1) / 2, M = NP / 2;
+        __shared__ double A[NP * LD];
+        __shared__ double V[NP * LD];
+        __shared__ double c_[M], s_[M];
+        __shared__ int ring[NP];
+        __shared__ int pq[2 * M];
+        __shared__ double red[4];
+        __shared__ double lam[NP];
+        __shared__ int any_changed;
+        const int64_t e = blockIdx.x;
+        if (e >= n_mats) return;
+        if (select != nullptr && !select[e]) return;
+        const int tid = threadIdx.x;
+        double* h = H + e * (int64_t)(NUB * 9);
+        auto ubi = [](int a, int b) { return a * NB - (a * (a - 1)) / 2 + (b - a); };
+        for (int t = tid; t < NP * NP; t += 256) {
+            const int r = t / NP, c = t - r * NP;
+            double v = 0.0;
+            if (r < N && c < N) {
+                const int a = r / 3, i = r - 3 * a, b = c / 3, j = c - 3 * b;
+                v = a <= b ? h[ubi(a, b) * 9 + 3 * i + j] : h[ubi(b, a) * 9 + 3 * j + i];
+            }
+            A[r * LD + c] = v;
+            V[r * LD + c] = r == c ? 1.0 : 0.0;
+        }
+        for (int t = tid; t < NP; t += 256) ring[t] = t;
+        __syncthreads();
+        auto block_sum = [&](double v) {
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((tid & 63) == 0) red[tid >> 6] = v;
+            __syncthreads();
+            const double r = ((red[0] + red[1]) + red[2]) + red[3];
+            __syncthreads();
+            return r;
+        };
+        double fro = 0.0;
+        for (int t = tid; t < N * N; t += 256) {
+            const int r = t / N, c = t - r * N;
+            if (r <= c) fro += A[r * LD + c] * A[r * LD + c];
+        }
+        const double tol = 1e-32 * block_sum(fro);
+        for (int sweep = 0; sweep < 24; sweep++) {
+            double off = 0.0;
+            for (int t = tid; t < N * N; t += 256) {
+                const int r = t / N, c = t - r * N;
+                if (r < c) off += A[r * LD + c] * A[r * LD + c];
+            }
+            if (!(block_sum(off) > tol)) break;
+            for (int round = 0; round < NP - 1; round++) {
+                if (tid < M) {
+                    int p = ring[tid], q = ring[NP - 1 - tid];
+                    if (p > q) {
+                        const int t = p;
+                        p = q;
+                        q = t;
+                    }
+                    const double apq = A[p * LD + q], app = A[p * LD + p], aqq = A[q * LD + q];
+                    const double d = aqq - app;
+                    const double rr = sqrt(d * d + 4.0 * apq * apq);
+                    const double den = d + copysign(rr, d);
+                    const double tt = (den != 0.0) ? (2.0 * apq) / den : 0.0;
+                    const double c = 1.0 / sqrt(1.0 + tt * tt);
+                    c_[tid] = c;
+                    s_[tid] = tt * c;
+                    pq[2 * tid] = p;
+                    pq[2 * tid + 1] = q;
+                }
+                __syncthreads();
+                for (int t = tid; t < NP * M; t += 256) {  // columns of A and V
+                    const int r = t / M, k = t - r * M;
+                    const int p = pq[2 * k], q = pq[2 * k + 1];
+                    const double c = c_[k], s = s_[k];
+                    const double g = A[r * LD + p], hh = A[r * LD + q];
+                    A[r * LD + p] = c * g - s * hh;
+                    A[r * LD + q] = s * g + c * hh;
+                    const double gv = V[r * LD + p], hv = V[r * LD + q];
+                    V[r * LD + p] = c * gv - s * hv;
+                    V[r * LD + q] = s * gv + c * hv;
+                }
+                __syncthreads();
+                for (int t = tid; t < M * NP; t += 256) {  // rows of A
+                    const int k = t / NP, col = t - k * NP;
+                    const int p = pq[2 * k], q = pq[2 * k + 1];
+                    const double c = c_[k], s = s_[k];
+                    const double g = A[p * LD + col], hh = A[q * LD + col];
+                    A[p * LD + col] = c * g - s * hh;
+                    A[q * LD + col] = s * g + c * hh;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    const int last = ring[NP - 1];
+                    for (int i = NP - 1; i > 1; i--) ring[i] = ring[i - 1];
+                    ring[1] = last;
+                }
+                __syncthreads();
+            }
+        }
+        if (tid == 0) any_changed = 0;
+        __syncthreads();
+        for (int t = tid; t < N; t += 256) {
+            const double l = A[t * LD + t];
+            if (l < eps) {
+                lam[t] = mirror ? -l : eps;
+                any_changed = 1;
+            }
+            else {
+                lam[t] = l;
+            }
+        }
+        __syncthreads();
+        if (!any_changed) return;
+        for (int t = tid; t < NUB * 9; t += 256) {
+            const int u = t / 9, ij = t - 9 * u;
+            int a = 0, rem = u;
+            while (rem >= NB - a) {
+                rem -= NB - a;
+                a++;
+            }
+            const int b = a + rem;
+            const int r = 3 * a + ij / 3, c = 3 * b + ij % 3;
+            double acc = 0.0;
+            for (int k = 0; k < N; k++) acc += V[r * LD + k] * lam[k] * V[c * LD + k];
+            h[t] = acc;
+        }
+        if (tid == 0 && count != nullptr) atomicAdd(count, 1ull);
+    }
+
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
                           const uint8_t* select, unsigned long long* d_count)
     {
@@ -310,6 +445,9 @@ namespace symx_hip
             k_project_ub4_reduced<<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags, select);
             k_project_ub_full<4><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags);
             return 0;
+        case 8: k_project_ub_lds<8><<<(unsigned)n_mats, 256, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 10: k_project_ub_lds<10><<<(unsigned)n_mats, 256, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 27: k_project_ub_lds<27><<<(unsigned)n_mats, 256, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
         default: return -1;
         }
     }

@@ -20,8 +20,8 @@
 using namespace symx_hip;
 
 // The device prelude restates these structs; both sides must agree byte for byte.
-static_assert(sizeof(symx_hip_eval_params) == 440, "symx_hip_eval_params layout");
-static_assert(sizeof(symx_hip_adjoint_params) == 400, "symx_hip_adjoint_params layout");
+static_assert(sizeof(symx_hip_eval_params) == 1080, "symx_hip_eval_params layout");
+static_assert(sizeof(symx_hip_adjoint_params) == 1040, "symx_hip_adjoint_params layout");
 
 namespace
 {
@@ -73,7 +73,7 @@ namespace
     const char* kPrelude = R"PRELUDE(// ---- symx_hip element-kernel prelude ----
 typedef int symx_i32;
 typedef unsigned char symx_u8;
-#define SYMX_HIP_MAX_MAPS 48
+#define SYMX_HIP_MAX_MAPS 128
 struct symx_hip_eval_params {
     const symx_i32* conn;
     const symx_u8* active;
@@ -85,14 +85,14 @@ struct symx_hip_eval_params {
     symx_i32 n_elem;
     symx_i32 pad_;
 };
-static_assert(sizeof(symx_hip_eval_params) == 440, "symx_hip_eval_params layout");
+static_assert(sizeof(symx_hip_eval_params) == 1080, "symx_hip_eval_params layout");
 struct symx_hip_adjoint_params {
     const double* lambda[SYMX_HIP_MAX_MAPS];
     double* out;
     symx_i32 n_out;
     symx_i32 pad_;
 };
-static_assert(sizeof(symx_hip_adjoint_params) == 400, "symx_hip_adjoint_params layout");
+static_assert(sizeof(symx_hip_adjoint_params) == 1040, "symx_hip_adjoint_params layout");
 __device__ __forceinline__ double symx_ln(double x) { return (x <= 0.0) ? -__builtin_inf() : log(x); }
 __device__ __forceinline__ double symx_log10(double x) { return (x <= 0.0) ? -__builtin_inf() : log10(x); }
 __device__ __forceinline__ double symx_powi(double x, int n)
@@ -337,7 +337,8 @@ static int module_function(symx_hip_module mod, const char* name, hipFunction_t*
     return SYMX_HIP_OK;
 }
 
-static int launch_eval(symx_hip_ctx ctx, symx_hip_module mod, const std::string& name, const symx_hip_eval_params& params)
+static int launch_eval(symx_hip_ctx ctx, symx_hip_module mod, const std::string& name, const symx_hip_eval_params& params,
+                       int32_t elems_per_group = SYMX_HIP_BLOCK)
 {
     if (params.n_elem <= 0) return SYMX_HIP_OK;
     hipFunction_t f;
@@ -346,7 +347,7 @@ static int launch_eval(symx_hip_ctx ctx, symx_hip_module mod, const std::string&
     symx_hip_eval_params p = params;
     size_t sz = sizeof(p);
     void* config[] = { HIP_LAUNCH_PARAM_BUFFER_POINTER, &p, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END };
-    const unsigned grid = (unsigned)((params.n_elem + SYMX_HIP_BLOCK - 1) / SYMX_HIP_BLOCK);
+    const unsigned grid = (unsigned)((params.n_elem + elems_per_group - 1) / elems_per_group);
     HIP_TRY(ctx, hipModuleLaunchKernel(f, grid, 1, 1, SYMX_HIP_BLOCK, 1, 1, 0, ctx->stream, nullptr, config));
     return SYMX_HIP_OK;
 }
@@ -391,7 +392,8 @@ void symx_hip_problem_destroy(symx_hip_problem p)
     for (auto& pot : p->pots) {
         dfree(pot.d_conn);
         dfree(pot.d_active);
-        for (auto& m : pot.maps) dfree(m.d_data);
+        for (auto& m : pot.maps)
+            if (m.owner) dfree(m.d_data);
     }
     dfree(p->d_G);
     dfree(p->d_H);
@@ -432,6 +434,9 @@ int symx_hip_problem_add_potential(symx_hip_problem p, const symx_hip_potential_
     pot.kC = d->kernel_C ? d->kernel_C : "";
     pot.n_elem = d->n_elem;
     pot.stride = d->conn_stride;
+    pot.elems_per_group = d->elems_per_group == 0 ? SYMX_HIP_BLOCK : d->elems_per_group;
+    if (pot.elems_per_group != SYMX_HIP_BLOCK && pot.elems_per_group != 4) return fail(ctx, SYMX_HIP_ERR_ARG, "elems_per_group must be 256 or 4");
+    if (pot.elems_per_group != SYMX_HIP_BLOCK && d->kernel_C) return fail(ctx, SYMX_HIP_ERR_ARG, "conditions need one element per thread");
     pot.nb = d->n_dof_blocks;
     pot.n_ub = pot.nb * (pot.nb + 1) / 2;
     pot.dof_col.assign(d->dof_conn_col, d->dof_conn_col + pot.nb);
@@ -465,9 +470,25 @@ int symx_hip_problem_add_potential(symx_hip_problem p, const symx_hip_potential_
         dm.n_values = d->maps[m].n_values;
         if (!dm.is_dof) {
             if (dm.n_values <= 0 || !d->maps[m].host) return fail(ctx, SYMX_HIP_ERR_ARG, "data map without data");
-            int rc = dalloc(ctx, &dm.d_data, (size_t)dm.n_values);
-            if (rc) return rc;
-            HIP_TRY(ctx, hipMemcpy(dm.d_data, d->maps[m].host, dm.n_values * sizeof(double), hipMemcpyHostToDevice));
+            dm.host = d->maps[m].host;
+            // Maps of the same host array (one per element node, e.g. the rest positions of a
+            // 27-node element) share one device copy.
+            const DevMap* same = nullptr;
+            auto find = [&](const std::vector<DevMap>& ms) {
+                for (const DevMap& o : ms)
+                    if (!same && !o.is_dof && o.owner && o.host == dm.host && o.n_values == dm.n_values) same = &o;
+            };
+            find(pot.maps);
+            for (const auto& q : p->pots) find(q.maps);
+            if (same) {
+                dm.d_data = same->d_data;
+                dm.owner = false;
+            }
+            else {
+                int rc = dalloc(ctx, &dm.d_data, (size_t)dm.n_values);
+                if (rc) return rc;
+                HIP_TRY(ctx, hipMemcpy(dm.d_data, d->maps[m].host, dm.n_values * sizeof(double), hipMemcpyHostToDevice));
+            }
         }
         else if (dm.dof_offset < 0 || dm.dof_offset >= p->n_dofs) {
             return fail(ctx, SYMX_HIP_ERR_ARG, "DoF map offset out of range");
@@ -493,6 +514,14 @@ int symx_hip_problem_set_map_data(symx_hip_problem p, int32_t pot_id, int32_t ma
     auto& m = pot.maps[map_id];
     if (m.is_dof) return fail(ctx, SYMX_HIP_ERR_ARG, "map is bound to the DoF vector");
     if (n != m.n_values) return fail(ctx, SYMX_HIP_ERR_ARG, "data map size changed");
+    if (!m.owner) {
+        if (host == m.host) return SYMX_HIP_OK;  // the owning map uploads this array
+        // the array moved away from the one it shared: give this map its own copy
+        int rc = dalloc(ctx, &m.d_data, (size_t)n);
+        if (rc) return rc;
+        m.owner = true;
+    }
+    m.host = host;
     HIP_TRY(ctx, hipMemcpyAsync(m.d_data, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return SYMX_HIP_OK;
@@ -518,7 +547,7 @@ int symx_hip_problem_finalize(symx_hip_problem p)
         pot.part_base = parts;
         pot.e_base = elems;
         elems += pot.n_elem;
-        pot.n_groups = (pot.n_elem + SYMX_HIP_BLOCK - 1) / SYMX_HIP_BLOCK;
+        pot.n_groups = (pot.n_elem + pot.elems_per_group - 1) / pot.elems_per_group;
         g += (int64_t)pot.n_elem * pot.nb;
         h += (int64_t)pot.n_elem * pot.n_ub;
         parts += pot.n_groups;
@@ -746,7 +775,7 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
             if (rc) return rc;
         }
         const std::string& k = mode == SYMX_HIP_EVAL_P ? pot.kP : (mode == SYMX_HIP_EVAL_PG ? pot.kPG : pot.kPGH);
-        int rc = launch_eval(ctx, pot.module, k, prm);
+        int rc = launch_eval(ctx, pot.module, k, prm, pot.elems_per_group);
         if (rc) return rc;
     }
     if (mode == SYMX_HIP_EVAL_PGH && p->d_projected) launch_fill_u8(ctx->stream, p->d_projected, p->total_elem);
@@ -792,7 +821,7 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
         // elements of a conditional potential that are inactive have no Hessian to project
         const uint8_t* only = pot.kC.empty() ? nullptr : pot.d_active;
         if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, only, d_count))
-            return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports up to 4 DoF blocks per element");
+            return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports 1-4, 8, 10 and 27 DoF blocks per element");
     }
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, d_count, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -823,7 +852,8 @@ int symx_hip_problem_project_selective(symx_hip_problem p, double eps, int32_t m
     pr.n = (int32_t)p->pots.size();
     for (int32_t q = 0; q < pr.n; q++) {
         const DevPotential& pot = p->pots[q];
-        if (pot.nb > 4) return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports up to 4 DoF blocks per element");
+        if (pot.nb > 4 && pot.nb != 8 && pot.nb != 10 && pot.nb != 27)
+            return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports 1-4, 8, 10 and 27 DoF blocks per element");
         pr.nb[q] = pot.nb;
         pr.g_base[q] = pot.g_base;
         pr.g_end[q] = pot.g_base + (int64_t)pot.n_elem * pot.nb;
@@ -1256,8 +1286,8 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
         symx_hip_eval_params prm;
         fill_params(p, pot, prm);
         switch (phase) {
-        case 0: return launch_eval(ctx, pot.module, pot.kPGH, prm);
-        case 4: return launch_eval(ctx, pot.module, pot.kP, prm);
+        case 0: return launch_eval(ctx, pot.module, pot.kPGH, prm, pot.elems_per_group);
+        case 4: return launch_eval(ctx, pot.module, pot.kP, prm, pot.elems_per_group);
         case 1: {
             // project a fresh copy of the element Hessians each repetition
             if (hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice,

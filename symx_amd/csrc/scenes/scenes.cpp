@@ -98,6 +98,9 @@ struct symx_scene_s
     double kappa = 0, dhat = 0;
     symx_sphere_params sph{};
     bool is_sphere = false;
+    int32_t npe = 4;                  // nodes per element of `tets`
+    std::vector<double> E_elem;       // per-element Young's modulus (hexahedral scenes)
+    std::vector<int32_t> elems_enum;  // [n][npe + 1]: element index, nodes
     double lambda = 0, mu = 0, lam_over_mu = 0, dt = 0, k = 0;
     double E_ = 0, nu_ = 0;
     std::array<double, 3> gravity{};
@@ -135,6 +138,8 @@ struct SceneSpec
     bool contact = false;
     double kappa = 0.0, dhat = 0.0;
 };
+
+static void add_common_potentials(symx_scene_s* S, const SceneSpec& p);
 
 // Masses, per-element data and potentials of a Tet4 scene whose mesh (X, tets), state
 // (x, x0, v) and boundary vertices (bc, bc_target) are set (no device work).
@@ -199,6 +204,12 @@ static void finish_scene(symx_scene_s* s, const SceneSpec& p)
                                 return tet4_elastic(mws, xs, Xs, a, b, material);
                             });
     }
+    add_common_potentials(S, p);
+}
+
+// Inertia or gravity, penalty boundary, ground contact, DoFs and parameters.
+static void add_common_potentials(symx_scene_s* S, const SceneSpec& p)
+{
     if (p.dt > 0.0) {
         // backward Euler inertia with gravity (dynamic_elasticity_with_contact.cpp:85-102, no damping)
         S->G->add_potential("be_inertia", S->all_vertices, 1, [S](MappedWorkspace<double>& mws, Element& elem) {
@@ -373,7 +384,135 @@ static void build_sphere(symx_scene_s* s, const symx_sphere_params* p, const sym
     finish_scene(s, spec);
 }
 
+// Hexahedral block: Hex8 or Hex27 cuboid over [0, size], per-element Young's modulus
+// log-uniform in [youngs_min, youngs_max] (SplitMix64 stream seed ^ 0x9e37..., one draw
+// per element), node noise like the Tet4 block (stream `seed`, 3 draws per node, noise in
+// node spacings), masses lumped uniformly (rho V_e / nodes per element to each node).
+static void build_hex(symx_scene_s* s, const symx_hex_block_params* p, const symx_newton_params* np)
+{
+    if (p->nodes_per_element != 8 && p->nodes_per_element != 27) throw std::runtime_error("nodes_per_element must be 8 or 27");
+    s->np = *np;
+    s->npe = p->nodes_per_element;
+    const int npe = s->npe;
+    const FEMMesh mesh = npe == 27 ? generate_cuboid_Hex27_mesh({ p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] })
+                                   : generate_cuboid_Hex8_mesh({ p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] });
+    const int32_t nv = (int32_t)mesh.vertices.size(), ne = mesh.n_elements;
+    s->X = mesh.vertices;
+    s->tets = mesh.connectivity;
+    s->x = s->X;
+    const int sub = npe == 27 ? 2 : 1;
+    const double h[3] = { p->size[0] / (sub * p->cells[0]), p->size[1] / (sub * p->cells[1]), p->size[2] / (sub * p->cells[2]) };
+    SplitMix64 rng(p->seed);
+    std::vector<uint8_t> fixed(nv, 0);
+    for (int32_t i = 0; i < nv; i++) {
+        if (p->fixed_axis >= 0 && std::abs(s->X[i][p->fixed_axis]) < 1e-12) fixed[i] = 1;
+        for (int d = 0; d < 3; d++) {
+            const double u = rng.uniform();
+            if (!fixed[i]) s->x[i][d] += p->noise * h[d] * (2.0 * u - 1.0);
+        }
+    }
+    s->x0 = s->x;
+    s->v.assign(nv, { 0.0, 0.0, 0.0 });
+    for (int32_t i = 0; i < nv; i++)
+        if (fixed[i]) {
+            s->bc.numbered_push_back({ i });
+            s->bc_target.push_back(s->X[i]);
+        }
+    SplitMix64 rng_e(p->seed ^ 0x9e3779b97f4a7c15ull);
+    s->E_elem.resize(ne);
+    const double l0 = std::log(p->youngs_min), l1 = std::log(p->youngs_max);
+    for (int32_t e = 0; e < ne; e++) s->E_elem[e] = std::exp(l0 + rng_e.uniform() * (l1 - l0));
+    const double vol = (p->size[0] / p->cells[0]) * (p->size[1] / p->cells[1]) * (p->size[2] / p->cells[2]);
+    s->mass.assign(nv, 0.0);
+    for (int32_t e = 0; e < ne; e++)
+        for (int a = 0; a < npe; a++) s->mass[s->tets[(size_t)npe * e + a]] += p->density * vol / npe;
+    s->all_vertices.resize(nv);
+    for (int32_t i = 0; i < nv; i++) s->all_vertices[i] = i;
+    s->elems_enum.resize((size_t)ne * (npe + 1));
+    for (int32_t e = 0; e < ne; e++) {
+        s->elems_enum[(size_t)(npe + 1) * e] = e;
+        for (int a = 0; a < npe; a++) s->elems_enum[(size_t)(npe + 1) * e + 1 + a] = s->tets[(size_t)npe * e + a];
+    }
+    s->nu_ = p->poissons_ratio;
+    s->dt = p->dt;
+    s->k = p->bc_stiffness;
+    s->gravity = { p->gravity[0], p->gravity[1], p->gravity[2] };
+    s->G = GlobalPotential::create();
+    symx_scene_s* S = s;
+    const FEM_Element type = npe == 27 ? FEM_Element::Hex27 : FEM_Element::Hex8;
+    S->G->add_potential(npe == 27 ? "neohookean_hex27" : "neohookean_hex8", S->elems_enum, npe + 1,
+                        [S, npe, type](MappedWorkspace<double>& mws, Element& elem) {
+                            std::vector<Vector> xs = mws.make_vectors(S->x, elem.slice(1, npe + 1));
+                            std::vector<Vector> Xs = mws.make_vectors(S->X, elem.slice(1, npe + 1));
+                            const Scalar E = mws.make_scalar(S->E_elem, elem[0]);
+                            const Scalar nu = mws.make_scalar(S->nu_);
+                            const Scalar mu = E / (2.0 * (1.0 + nu));
+                            const Scalar lambda = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+                            return fem_integrator(mws, type, [&](Scalar& w, Vector& xi) {
+                                const Matrix Jx = fem_jacobian(type, xs, xi);
+                                const Matrix JX = fem_jacobian(type, Xs, xi);
+                                const Matrix F = Jx * JX.inv();
+                                return neohookean_energy_density(F, lambda, mu) * JX.det() * w;
+                            });
+                        });
+    SceneSpec spec;
+    spec.density = p->density;
+    spec.gravity = s->gravity;
+    spec.dt = p->dt;
+    spec.bc_stiffness = p->bc_stiffness;
+    add_common_potentials(s, spec);
+}
+
 extern "C" {
+
+void symx_hex_block_params_default(symx_hex_block_params* p)
+{
+    p->cells[0] = p->cells[1] = p->cells[2] = 4;
+    p->size[0] = p->size[1] = p->size[2] = 1.0;
+    p->nodes_per_element = 27;
+    p->youngs_min = 1e4;
+    p->youngs_max = 1e6;
+    p->poissons_ratio = 0.45;
+    p->density = 1000.0;
+    p->gravity[0] = 0.0;
+    p->gravity[1] = 0.0;
+    p->gravity[2] = -9.81;
+    p->dt = 0.01;
+    p->noise = 0.1;
+    p->seed = 3;
+    p->bc_stiffness = 1e8;
+    p->fixed_axis = 2;
+}
+
+int symx_scene_hex_block_create(const symx_hex_block_params* p, const symx_newton_params* np, symx_scene* out)
+{
+    *out = nullptr;
+    auto* s = new symx_scene_s();
+    int rc = guard([&]() {
+        build_hex(s, p, np);
+        s->ctx = Context::create();
+        s->newton = NewtonsMethod::create(s->G, s->ctx);
+        apply_newton_params(s->newton->settings, *np);
+    });
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+int symx_scene_nodes_per_element(symx_scene s, int32_t* n)
+{
+    *n = s->npe;
+    return 0;
+}
+
+int symx_scene_young_moduli(symx_scene s, double* out)
+{
+    std::memcpy(out, s->E_elem.data(), s->E_elem.size() * sizeof(double));
+    return 0;
+}
 
 void symx_newton_params_default(symx_newton_params* p)
 {
@@ -507,7 +646,7 @@ int symx_scene_n_vertices(symx_scene s, int32_t* n)
 }
 int symx_scene_n_elements(symx_scene s, int32_t* n)
 {
-    *n = (int32_t)(s->tets.size() / 4);
+    *n = (int32_t)(s->tets.size() / s->npe);
     return 0;
 }
 int symx_scene_connectivity(symx_scene s, int32_t* conn)
@@ -761,6 +900,116 @@ int symx_host_sphere_mesh(const symx_sphere_params* p, int32_t* nv, int32_t* ne,
         if (X) std::memcpy(X, Xv.data(), Xv.size() * 3 * sizeof(double));
         if (tets) std::memcpy(tets, t.data(), t.size() * sizeof(int32_t));
         if (contact) std::memcpy(contact, c.data(), c.size() * sizeof(int32_t));
+    });
+}
+
+int symx_host_hex_element(const symx_hex_block_params* p, int32_t structured, int32_t e, double* E, double* g, double* H, int32_t* was_structured)
+{
+    return guard([&]() {
+        const bool prev = get_structured_fem();
+        set_structured_fem(structured != 0);
+        symx_newton_params np;
+        symx_newton_params_default(&np);
+        symx_scene_s s;
+        try {
+            build_hex(&s, p, &np);
+        }
+        catch (...) {
+            set_structured_fem(prev);
+            throw;
+        }
+        set_structured_fem(prev);
+        const Potential& P = *s.G->get_potentials()[0];
+        const PotentialModule pm = generate_potential_module(*s.G, P);
+        MappedWorkspace<double>& mws = *P.mws;
+        Graph* gr = mws.workspace().graph();
+        const SummationMap& sm = mws.get_summation();
+        const int32_t* conn = mws.conn.data() + (size_t)e * mws.conn.stride;
+        for (const DataMap& m : mws.get_maps()) {
+            const int64_t item = m.connectivity_index >= 0 ? conn[m.connectivity_index] : 0;
+            for (int k = 0; k < m.stride; k++) gr->set_symbol_value(m.first_symbol_idx + k, m.data()[item * m.stride + k]);
+        }
+        const int n = pm.info.n_dofs;
+        std::fill(g, g + n, 0.0);
+        std::fill(H, H + (size_t)n * n, 0.0);
+        *E = 0.0;
+        *was_structured = pm.structured ? 1 : 0;
+        for (int32_t q = 0; q < sm.n_iterations; q++) {
+            for (int k = 0; k < sm.stride; k++) gr->set_symbol_value(sm.first_symbol_idx + k, sm.data[(size_t)q * sm.stride + k]);
+            if (pm.structured) {
+                // the structured kernel's arithmetic, on the host
+                const FemStructure& f = pm.fem;
+                const int NN = f.n_nodes;
+                for (const auto& rec : f.records) {
+                    const DataMap& base = mws.get_maps()[rec.map_base];
+                    double J[9] = { 0 };
+                    for (int a = 0; a < NN; a++) {
+                        const double* xa = base.data() + (size_t)conn[rec.cols[a]] * 3;
+                        const double* d = &f.dN[((size_t)q * NN + a) * 3];
+                        for (int i = 0; i < 3; i++)
+                            for (int k = 0; k < 3; k++) J[i * 3 + k] += xa[i] * d[k];
+                    }
+                    for (int k = 0; k < 9; k++) gr->set_symbol_value(rec.J[k], J[k]);
+                }
+                std::vector<int32_t> ids = { f.f[0].id };
+                for (const Scalar& x : f.P) ids.push_back(x.id);
+                for (const Scalar& x : f.A) ids.push_back(x.id);
+                const std::vector<double> v = gr->eval_many(ids);
+                auto A = [&](int r, int c) {
+                    const int lo = std::min(r, c), hi = std::max(r, c);
+                    return v[10 + lo * 9 - (lo * (lo - 1)) / 2 + (hi - lo)];
+                };
+                *E += v[0];
+                for (int a = 0; a < NN; a++) {
+                    const double* da = &f.dN[((size_t)q * NN + a) * 3];
+                    const int A_ = f.block_of_node[a];
+                    for (int i = 0; i < 3; i++) g[3 * A_ + i] += v[1 + i * 3] * da[0] + v[2 + i * 3] * da[1] + v[3 + i * 3] * da[2];
+                    for (int b = 0; b < NN; b++) {
+                        const double* db = &f.dN[((size_t)q * NN + b) * 3];
+                        const int B_ = f.block_of_node[b];
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++) {
+                                double acc = 0.0;
+                                for (int k = 0; k < 3; k++)
+                                    for (int l = 0; l < 3; l++) acc += da[k] * A(i * 3 + k, j * 3 + l) * db[l];
+                                H[(size_t)(3 * A_ + i) * n + 3 * B_ + j] += acc;
+                            }
+                    }
+                }
+            }
+            else {
+                std::vector<int32_t> ids;
+                for (const Scalar& x : pm.specs[2].outputs) ids.push_back(x.id);
+                const std::vector<double> v = gr->eval_many(ids);
+                *E += v[0];
+                for (int i = 0; i < n; i++) g[i] += v[1 + i];
+                for (size_t i = 0; i < (size_t)n * n; i++) H[i] += v[1 + n + i];
+            }
+        }
+    });
+}
+
+int symx_host_hex_jit(const symx_hex_block_params* p, const char* cache_dir, int32_t* n_potentials, int64_t* code_bytes, int32_t* n_structured)
+{
+    return guard([&]() {
+        symx_newton_params np;
+        symx_newton_params_default(&np);
+        symx_scene_s s;
+        build_hex(&s, p, &np);
+        *n_potentials = 0;
+        *code_bytes = 0;
+        *n_structured = 0;
+        for (const auto& pp : s.G->get_potentials()) {
+            const PotentialModule pm = generate_potential_module(*s.G, *pp);
+            char key[80];
+            int wc = 0;
+            size_t cs = 0;
+            if (symx_hip_jit_compile(pm.source.c_str(), "gfx950", cache_dir, key, &wc, &cs) != SYMX_HIP_OK)
+                throw std::runtime_error(std::string("JIT of ") + pp->name + ": " + symx_hip_last_error(nullptr));
+            *n_potentials += 1;
+            *code_bytes += (int64_t)cs;
+            *n_structured += pm.structured ? 1 : 0;
+        }
     });
 }
 

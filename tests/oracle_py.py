@@ -22,7 +22,7 @@ class OracleScene(Structure):
                 ("bc_vertex", c_void_p), ("bc_target", c_void_p), ("mu_e", c_void_p),
                 ("lambda_over_mu", c_double), ("lam", c_double), ("mu", c_double), ("dt", c_double), ("k", c_double),
                 ("gravity", c_double * 3), ("ncontact", c_int32), ("pad_", c_int32), ("contact_vertex", c_void_p),
-                ("kappa", c_double), ("dhat", c_double), ("step_eta", c_double)]
+                ("kappa", c_double), ("dhat", c_double), ("step_eta", c_double), ("nodes_per_element", c_int32), ("pad2_", c_int32)]
 
 
 class OracleNewtonParams(Structure):
@@ -82,6 +82,7 @@ class SceneData:
         s.ncontact = self.contact.shape[0]
         s.contact_vertex = _p(self.contact) if self.contact.shape[0] else None
         s.kappa, s.dhat, s.step_eta = self.kappa, self.dhat, self.step_eta
+        s.nodes_per_element = self.tets.shape[1]
         return s
 
 
@@ -157,10 +158,24 @@ class Oracle:
         E = self.lib.oracle_scene_energy(ctypes.byref(st), _p(xx), _p(g) if with_grad else None)
         return (E, g) if with_grad else E
 
+    def nh_hex(self, x, X, lam, mu):
+        """E, gradient, Hessian of a Hex8 / Hex27 Neo-Hookean element (nodes [nn, 3])."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
+        nn = x.shape[0]
+        E = c_double()
+        g = np.zeros(3 * nn)
+        H = np.zeros(9 * nn * nn)
+        rc = self.lib.oracle_nh_hex(c_int(nn), _p(x), _p(X), c_double(lam), c_double(mu), ctypes.byref(E), _p(g), _p(H))
+        if rc == 2:
+            raise ValueError("nn must be 8 or 27")
+        return E.value, g, H.reshape(3 * nn, 3 * nn)
+
     def element_hessians(self, S: SceneData, x=None):
         st = S.struct()
         xx = np.ascontiguousarray(S.x if x is None else x, np.float64)
-        He = np.zeros((S.tets.shape[0], 12, 12))
+        nd = 3 * S.tets.shape[1]
+        He = np.zeros((S.tets.shape[0], nd, nd))
         self.lib.oracle_scene_element_hessians(ctypes.byref(st), _p(xx), _p(He))
         return He
 

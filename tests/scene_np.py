@@ -77,3 +77,51 @@ def sphere_mesh(radius=0.5, cells=50, jitter=0.1, seed=7, center=(0.0, 0.0, 0.51
     uniq, counts = np.unique(faces, axis=0, return_counts=True)
     contact = np.unique(uniq[counts == 1].ravel()).astype(np.int32)
     return X, T, contact
+
+
+HEX_NODES = [(0, 0, 0), (2, 0, 0), (2, 2, 0), (0, 2, 0), (0, 0, 2), (2, 0, 2), (2, 2, 2), (0, 2, 2),
+             (1, 0, 0), (2, 1, 0), (1, 2, 0), (0, 1, 0), (1, 0, 2), (2, 1, 2), (1, 2, 2), (0, 1, 2),
+             (0, 0, 1), (2, 0, 1), (2, 2, 1), (0, 2, 1),
+             (0, 1, 1), (2, 1, 1), (1, 2, 1), (1, 0, 1), (1, 1, 0), (1, 1, 2),
+             (1, 1, 1)]
+
+
+def hex_mesh(cells, size=(1.0, 1.0, 1.0), nn=27):
+    """Hex8 / Hex27 cuboid over [0, size]: (X, elements) in SymX's element node order."""
+    sub = 2 if nn == 27 else 1
+    N = [sub * c + 1 for c in cells]
+    d = [size[i] / (sub * cells[i]) for i in range(3)]
+    I, J, K = np.meshgrid(np.arange(N[0]), np.arange(N[1]), np.arange(N[2]), indexing="ij")
+    X = np.stack([I.ravel() * d[0], J.ravel() * d[1], K.ravel() * d[2]], 1)
+    i, j, k = np.meshgrid(np.arange(cells[0]), np.arange(cells[1]), np.arange(cells[2]), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    step = 2 if nn == 27 else 1
+    cols = []
+    for (a, b, c) in HEX_NODES[:nn]:
+        da, db, dc = (a, b, c) if nn == 27 else (a // 2, b // 2, c // 2)
+        cols.append((step * i + da) * N[1] * N[2] + (step * j + db) * N[2] + (step * k + dc))
+    return X, np.stack(cols, 1).astype(np.int32)
+
+
+def hex_block(cells=(2, 2, 2), size=(1.0, 1.0, 1.0), nn=27, E_min=1e4, E_max=1e6, nu=0.45, density=1000.0, gravity=(0, 0, -9.81),
+              dt=0.01, noise=0.1, seed=3, k=1e8, fixed_axis=2):
+    """numpy restatement of symx_scene_hex_block_create's inputs (SceneData with mu_e)."""
+    X, T = hex_mesh(cells, size, nn)
+    nv = X.shape[0]
+    sub = 2 if nn == 27 else 1
+    h = np.array([size[i] / (sub * cells[i]) for i in range(3)])
+    fixed = np.abs(X[:, fixed_axis]) < 1e-12 if fixed_axis >= 0 else np.zeros(nv, bool)
+    u = splitmix64_uniform(seed, 3 * nv).reshape(nv, 3)
+    x = X + np.where(fixed[:, None], 0.0, noise * h * (2.0 * u - 1.0))
+    ne = T.shape[0]
+    ue = splitmix64_uniform(seed ^ 0x9E3779B97F4A7C15, ne)
+    E = np.exp(np.log(E_min) + ue * (np.log(E_max) - np.log(E_min)))
+    vol = np.prod([size[i] / cells[i] for i in range(3)])
+    mass = np.zeros(nv)
+    np.add.at(mass, T.ravel(), density * vol / nn)
+    mu_e = E / (2 * (1 + nu))
+    lam_over_mu = 2 * nu / (1 - 2 * nu)
+    bc = np.nonzero(fixed)[0]
+    S = SceneData(T, X, x, x.copy(), np.zeros_like(x), mass, bc, X[bc], 0.0, 0.0, dt, k, gravity, mu_e=mu_e, lambda_over_mu=lam_over_mu)
+    S.E = E
+    return S
