@@ -50,10 +50,11 @@ def algorithmic_bytes(n_elem, n_vert, nnzb, precision):
     }
 
 
-def run_precision(scene, precision, steps, warmup, log):
+def run_precision(scene, precision, steps, warmup, log, keep_newton=False):
     import symx_amd
 
-    scene.set_newton(symx_amd.default_newton_params(precision=precision))
+    if not keep_newton:
+        scene.set_newton(symx_amd.default_newton_params(precision=precision))
     for i in range(warmup):
         st = scene.step()
         log(f"warmup step {i}: {st['result_name']} newton={st['newton_iterations']} cg={st['cg_iterations']} {st['seconds']:.3f}s")
@@ -318,6 +319,100 @@ def bench_contact(args, log):
     s.close()
 
 
+def bench_hex27(args, log):
+    """High-order: Hex27 Neo-Hookean cube of 40^3 cells over [0,1]^3 (64,000 elements,
+    531,441 nodes), per-element Young's modulus log-uniform in [1e4, 1e6] (seed 3),
+    nu = 0.45, density 1000, gravity, backward Euler dt = 0.01, z = 0 face held by the
+    penalty (k = 1e8), free nodes perturbed by 0.1 node spacings; ProjectOnDemand
+    projection, Newton to 1e-6 relative residual; mixed precision (FP32 Hessians/BSR/PCG
+    with FP64 refinement) against FP64. One step = one time step."""
+    import ctypes
+
+    import symx_amd
+
+    cells = args.hex_cells
+    p = symx_amd.default_hex_params(cells=[cells] * 3, nodes_per_element=27)
+    out = {}
+    for name, prec in (("mixed", symx_amd.MIXED), ("fp64", symx_amd.FP64)):
+        if name == "fp64" and args.no_fp64:
+            continue
+        newton = symx_amd.default_newton_params(precision=prec, projection_mode=2)
+        t0 = time.perf_counter()
+        s = symx_amd.HexBlock(p, newton)
+        log(f"[{name}] setup: {s.n_elements} elements, {s.n_vertices} nodes in {time.perf_counter() - t0:.1f}s "
+            f"(JIT + topology {symx_amd.amd().symx_scene_compile_seconds(s.ptr):.1f}s)")
+        r = run_precision(s, prec, args.steps, args.warmup, log, keep_newton=True)
+        prof = profile_kernels(s, prec, reps=3)
+        log(f"[{name}] kernel ms: {json.dumps({k: round(v, 4) for k, v in prof.items()})}")
+        nnzb_c = ctypes.c_int64()
+        symx_amd.check_hip(symx_amd.hip().symx_hip_problem_export_bsr(s.problem, ctypes.byref(nnzb_c), None, None, None))
+        ops = s.op_counts(0)
+        out[name] = {"run": r, "prof": prof, "nnzb": nnzb_c.value, "ops": ops, "n_elem": s.n_elements, "n_vert": s.n_vertices}
+        s.close()
+    m = out["mixed"]
+    n_elem, nnzb = m["n_elem"], m["nnzb"]
+    # algorithmic FLOPs of one structured Hex27 PGH element (FMA = 2): Jacobians of x and X
+    # per quadrature point, the generated material law per point, gradient and Hessian
+    # contractions (378 blocks x 9 entries x 3 x (3 + 1) FMA per point)
+    flops_elem = 27 * (2 * 27 * 9 * 2 + m["ops"][2]) + 27 * 81 * 3 * 2 + 27 * 378 * 9 * 3 * 4 * 2
+    el_ms = m["prof"]["element"]
+    achieved = flops_elem * n_elem / (el_ms * 1e-3) / 1e12
+    per_iter = {"element": el_ms, "projection": m["prof"]["projection"], "assembly": m["prof"]["assembly"],
+                "pcg_iteration": m["prof"]["pcg_iteration"] * m["run"]["cg_iterations"] / max(m["run"]["newton_iterations"], 1)}
+    roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None, "kernel": "element (structured Hex27 PGH)",
+                "avg_ms": round(el_ms, 4), "algorithmic_flops_per_launch": int(flops_elem * n_elem),
+                "per_newton_iteration_ms": {k: round(v, 3) for k, v in per_iter.items()}}
+    cpu = None
+    if not args.no_cpu:
+        from oracle_py import Oracle
+        from scene_np import hex_block
+
+        orc = Oracle()
+        c = args.cpu_hex_cells
+        S = hex_block(cells=(c, c, c), nn=27, noise=p.noise, seed=p.seed)
+        t0 = time.perf_counter()
+        res = orc.newton(S, symx_amd.default_newton_params(precision=symx_amd.FP64, projection_mode=2))
+        t1 = time.perf_counter()
+        ne_c = S.tets.shape[0]
+        log(f"cpu ({orc.num_threads()} threads): {c}^3 Hex27 cells, newton={res['newton_iterations']} cg={res['cg_iterations']} {t1 - t0:.2f}s")
+        cpu = {"value": round(res["newton_iterations"] / (t1 - t0) * ne_c / n_elem, 4), "unit": "Newton iterations/s", "cores": orc.num_threads(),
+               "kind": "port", "sample": f"oracle/symx_oracle.c, one time step of a {c}^3-cell Hex27 cube ({ne_c} elements, {t1 - t0:.1f}s), "
+                                         f"scaled linearly by element count to {n_elem} elements"}
+    line = {
+        "metric": "Newton steps/s, elem Hessians/s; Hex27 mixed vs FP64 at 1e-6 residual",
+        "value": round(m["run"]["newton_iterations"] / m["run"]["seconds"], 4),
+        "unit": "Newton iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * m["run"]["seconds"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 Hessian/BSR/PCG + f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"Hex27 Neo-Hookean cube {cells}^3 cells ({n_elem} elements, {3 * m['n_vert']} DoF), E log-uniform [1e4,1e6], "
+                               "nu=0.45, implicit Euler dt=0.01, ProjectOnDemand, 1e-6 rel. residual", "n_elem": n_elem,
+                   "n_dof": 3 * m["n_vert"], "nnzb": nnzb},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "elem_hessians_per_s": round(n_elem / (el_ms * 1e-3), 1),
+        "newton_iterations": m["run"]["newton_iterations"],
+        "cg_iterations": m["run"]["cg_iterations"],
+        "results": m["run"]["results"],
+        "kernel_ms": {k: round(v, 4) for k, v in m["prof"].items()},
+        "generated_ops": {"P": m["ops"][0], "PG": m["ops"][1], "PGH_per_quadrature_point": m["ops"][2]},
+    }
+    if "fp64" in out:
+        f = out["fp64"]
+        line["fp64"] = {"newton_per_s": round(f["run"]["newton_iterations"] / f["run"]["seconds"], 4), "seconds": round(f["run"]["seconds"], 4),
+                        "newton_iterations": f["run"]["newton_iterations"], "cg_iterations": f["run"]["cg_iterations"],
+                        "kernel_ms": {k: round(v, 4) for k, v in f["prof"].items()},
+                        "speedup_mixed_over_fp64_wall": round(f["run"]["seconds"] / m["run"]["seconds"], 3)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,8 +422,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cells", type=int, default=40)
-    ap.add_argument("--workload", choices=["block", "cantilever_adjoint", "sphere_contact"], default="block",
-                    help="block: the headline Newton benchmark; cantilever_adjoint: adjoint gradients/s; sphere_contact: ball drop")
+    ap.add_argument("--workload", choices=["block", "cantilever_adjoint", "sphere_contact", "hex27"], default="block",
+                    help="block: the headline Newton benchmark; cantilever_adjoint: adjoint gradients/s; sphere_contact: ball drop; "
+                         "hex27: high-order cube")
+    ap.add_argument("--hex-cells", type=int, default=40)
+    ap.add_argument("--cpu-hex-cells", type=int, default=6)
     ap.add_argument("--adjoint-precision", choices=["fp64", "mixed"], default="fp64")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -347,6 +445,9 @@ def main():
         return
     if args.workload == "sphere_contact":
         bench_contact(args, log)
+        return
+    if args.workload == "hex27":
+        bench_hex27(args, log)
         return
     cells = args.cells
     t0 = time.perf_counter()

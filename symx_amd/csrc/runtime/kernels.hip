@@ -298,29 +298,37 @@ namespace symx_hip
     }
 
     // Large element Hessians (8 and more DoF blocks: Hex8, Tet10, Hex27): one workgroup
-    // per matrix, parallel cyclic Jacobi in LDS with the round-robin pairing of Brent and
-    // Luk (n/2 disjoint rotations per round, n-1 rounds per sweep). Each rotation is the
-    // one of jacobi_rotate; A <- J^T A J as a column pass then a row pass, V <- V J.
+    // of 512 threads per matrix, the matrix in LDS.
+    //  1. Fast path for translation-invariant elements (every block row sums to zero, as
+    //     for every elastic energy): the translations are exact null vectors, so shifting
+    //     them up by c (adding c/NB to the diagonal entries of every 3x3 block) leaves the
+    //     other eigenvalues alone. If the Cholesky factorisation of the shifted matrix minus
+    //     eps I succeeds, the only eigenvalues below eps are the three zeros, and the
+    //     projection adds f(0) on the translation space: H + eps/NB (1 1^T (x) I3) (clamp) or
+    //     H unchanged (mirror), which the register kernels of 4-node elements also do.
+    //  2. Otherwise parallel cyclic Jacobi with the round-robin pairing of Brent and Luk
+    //     (n/2 disjoint rotations per round, n-1 rounds per sweep): the rotation of
+    //     jacobi_rotate, A <- J^T A J as a column pass then a row pass with the rotated
+    //     2x2 block set exactly, V <- V J, then V f(Lambda) V^T when some eigenvalue < eps.
     template <int NB>
-    __global__ __launch_bounds__(256) void k_project_ub_lds(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
+    __global__ __launch_bounds__(512) void k_project_ub_lds(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
                                                            unsigned long long* count, const uint8_t* __restrict__ select)
     {
-        constexpr int N = 3 * NB, NP = (N + 1) & ~1, LD = NP + 1, NUB = NB * (NB + 1) / 2, M = NP / 2;
+        constexpr int NT = 512, N = 3 * NB, NP = (N + 1) & ~1, LD = NP + 1, NUB = NB * (NB + 1) / 2, M = NP / 2;
         __shared__ double A[NP * LD];
         __shared__ double V[NP * LD];
-        __shared__ double c_[M], s_[M];
-        __shared__ int ring[NP];
+        __shared__ double c_[M], s_[M], app_[M], aqq_[M];
         __shared__ int pq[2 * M];
-        __shared__ double red[4];
+        __shared__ double red[NT / 64];
         __shared__ double lam[NP];
-        __shared__ int any_changed;
+        __shared__ int flag;
         const int64_t e = blockIdx.x;
         if (e >= n_mats) return;
         if (select != nullptr && !select[e]) return;
         const int tid = threadIdx.x;
         double* h = H + e * (int64_t)(NUB * 9);
         auto ubi = [](int a, int b) { return a * NB - (a * (a - 1)) / 2 + (b - a); };
-        for (int t = tid; t < NP * NP; t += 256) {
+        for (int t = tid; t < NP * NP; t += NT) {
             const int r = t / NP, c = t - r * NP;
             double v = 0.0;
             if (r < N && c < N) {
@@ -328,38 +336,108 @@ namespace symx_hip
                 v = a <= b ? h[ubi(a, b) * 9 + 3 * i + j] : h[ubi(b, a) * 9 + 3 * j + i];
             }
             A[r * LD + c] = v;
-            V[r * LD + c] = r == c ? 1.0 : 0.0;
         }
-        for (int t = tid; t < NP; t += 256) ring[t] = t;
         __syncthreads();
+        auto block_max = [&](double v) {
+            for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+            if ((tid & 63) == 0) red[tid >> 6] = v;
+            __syncthreads();
+            double r = red[0];
+            for (int w = 1; w < NT / 64; w++) r = fmax(r, red[w]);
+            __syncthreads();
+            return r;
+        };
         auto block_sum = [&](double v) {
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
             if ((tid & 63) == 0) red[tid >> 6] = v;
             __syncthreads();
-            const double r = ((red[0] + red[1]) + red[2]) + red[3];
+            double r = 0.0;
+            for (int w = 0; w < NT / 64; w++) r += red[w];
             __syncthreads();
             return r;
         };
+        // ---- 1. translation-invariant fast path --------------------------------------
+        double amax = 0.0, rsum = 0.0;
+        for (int t = tid; t < N * N; t += NT) amax = fmax(amax, fabs(A[(t / N) * LD + t % N]));
+        for (int t = tid; t < N * 3; t += NT) {  // row r, column offset j: sum over blocks b of A[r][3b + j]
+            const int r = t / 3, j = t - 3 * r;
+            double acc = 0.0;
+            for (int b = 0; b < NB; b++) acc += A[r * LD + 3 * b + j];
+            rsum = fmax(rsum, fabs(acc));
+        }
+        const double scale = block_max(amax);
+        const double row_sum = block_max(rsum);
+        if (row_sum <= 1e-10 * scale * NB) {
+            const double shift = scale * NB;
+            for (int t = tid; t < N * N; t += NT) {
+                const int r = t / N, c = t - r * N;
+                V[r * LD + c] = A[r * LD + c] + ((r % 3) == (c % 3) ? shift / NB : 0.0) - (r == c ? eps : 0.0);
+            }
+            if (tid == 0) flag = 0;
+            __syncthreads();
+            for (int k = 0; k < N; k++) {  // right-looking Cholesky, lower triangle
+                if (tid == 0) {
+                    const double d = V[k * LD + k];
+                    if (!(d > 0.0)) flag = 1;
+                    else V[k * LD + k] = sqrt(d);
+                }
+                __syncthreads();
+                if (flag) break;
+                const double piv = V[k * LD + k];
+                for (int i = k + 1 + tid; i < N; i += NT) V[i * LD + k] /= piv;
+                __syncthreads();
+                const int m = N - k - 1;
+                for (int t = tid; t < m * m; t += NT) {
+                    const int i = k + 1 + t / m, j = k + 1 + t % m;
+                    if (j <= i) V[i * LD + j] -= V[i * LD + k] * V[j * LD + k];
+                }
+                __syncthreads();
+            }
+            if (!flag) {
+                if (!mirror) {
+                    for (int t = tid; t < NUB * 9; t += NT) {
+                        const int ij = t % 9;
+                        if (ij / 3 == ij % 3) h[t] += eps / NB;
+                    }
+                }
+                if (tid == 0 && count != nullptr) atomicAdd(count, 1ull);
+                return;
+            }
+        }
+        // ---- 2. parallel cyclic Jacobi ----------------------------------------------------
+        for (int t = tid; t < NP * NP; t += NT) {
+            const int r = t / NP, c = t - r * NP;
+            V[r * LD + c] = r == c ? 1.0 : 0.0;
+        }
         double fro = 0.0;
-        for (int t = tid; t < N * N; t += 256) {
+        for (int t = tid; t < N * N; t += NT) {
             const int r = t / N, c = t - r * N;
             if (r <= c) fro += A[r * LD + c] * A[r * LD + c];
         }
         const double tol = 1e-32 * block_sum(fro);
         for (int sweep = 0; sweep < 24; sweep++) {
             double off = 0.0;
-            for (int t = tid; t < N * N; t += 256) {
+            for (int t = tid; t < N * N; t += NT) {
                 const int r = t / N, c = t - r * N;
                 if (r < c) off += A[r * LD + c] * A[r * LD + c];
             }
             if (!(block_sum(off) > tol)) break;
             for (int round = 0; round < NP - 1; round++) {
                 if (tid < M) {
-                    int p = ring[tid], q = ring[NP - 1 - tid];
+                    // circle method: index NP-1 stays, the others rotate
+                    int p, q;
+                    if (tid == 0) {
+                        p = round;
+                        q = NP - 1;
+                    }
+                    else {
+                        p = (round + tid) % (NP - 1);
+                        q = (round - tid + NP - 1) % (NP - 1);
+                    }
                     if (p > q) {
-                        const int t = p;
+                        const int x = p;
                         p = q;
-                        q = t;
+                        q = x;
                     }
                     const double apq = A[p * LD + q], app = A[p * LD + p], aqq = A[q * LD + q];
                     const double d = aqq - app;
@@ -369,11 +447,13 @@ namespace symx_hip
                     const double c = 1.0 / sqrt(1.0 + tt * tt);
                     c_[tid] = c;
                     s_[tid] = tt * c;
+                    app_[tid] = app - tt * apq;
+                    aqq_[tid] = aqq + tt * apq;
                     pq[2 * tid] = p;
                     pq[2 * tid + 1] = q;
                 }
                 __syncthreads();
-                for (int t = tid; t < NP * M; t += 256) {  // columns of A and V
+                for (int t = tid; t < NP * M; t += NT) {  // columns of A and V
                     const int r = t / M, k = t - r * M;
                     const int p = pq[2 * k], q = pq[2 * k + 1];
                     const double c = c_[k], s = s_[k];
@@ -385,38 +465,42 @@ namespace symx_hip
                     V[r * LD + q] = s * gv + c * hv;
                 }
                 __syncthreads();
-                for (int t = tid; t < M * NP; t += 256) {  // rows of A
+                for (int t = tid; t < M * NP; t += NT) {  // rows of A; the rotated 2x2 block exactly
                     const int k = t / NP, col = t - k * NP;
                     const int p = pq[2 * k], q = pq[2 * k + 1];
-                    const double c = c_[k], s = s_[k];
-                    const double g = A[p * LD + col], hh = A[q * LD + col];
-                    A[p * LD + col] = c * g - s * hh;
-                    A[q * LD + col] = s * g + c * hh;
-                }
-                __syncthreads();
-                if (tid == 0) {
-                    const int last = ring[NP - 1];
-                    for (int i = NP - 1; i > 1; i--) ring[i] = ring[i - 1];
-                    ring[1] = last;
+                    if (col == p) {
+                        A[p * LD + p] = app_[k];
+                        A[q * LD + p] = 0.0;
+                    }
+                    else if (col == q) {
+                        A[p * LD + q] = 0.0;
+                        A[q * LD + q] = aqq_[k];
+                    }
+                    else {
+                        const double c = c_[k], s = s_[k];
+                        const double g = A[p * LD + col], hh = A[q * LD + col];
+                        A[p * LD + col] = c * g - s * hh;
+                        A[q * LD + col] = s * g + c * hh;
+                    }
                 }
                 __syncthreads();
             }
         }
-        if (tid == 0) any_changed = 0;
+        if (tid == 0) flag = 0;
         __syncthreads();
-        for (int t = tid; t < N; t += 256) {
+        for (int t = tid; t < N; t += NT) {
             const double l = A[t * LD + t];
             if (l < eps) {
                 lam[t] = mirror ? -l : eps;
-                any_changed = 1;
+                flag = 1;
             }
             else {
                 lam[t] = l;
             }
         }
         __syncthreads();
-        if (!any_changed) return;
-        for (int t = tid; t < NUB * 9; t += 256) {
+        if (!flag) return;
+        for (int t = tid; t < NUB * 9; t += NT) {
             const int u = t / 9, ij = t - 9 * u;
             int a = 0, rem = u;
             while (rem >= NB - a) {
@@ -445,9 +529,9 @@ namespace symx_hip
             k_project_ub4_reduced<<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags, select);
             k_project_ub_full<4><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags);
             return 0;
-        case 8: k_project_ub_lds<8><<<(unsigned)n_mats, 256, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
-        case 10: k_project_ub_lds<10><<<(unsigned)n_mats, 256, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
-        case 27: k_project_ub_lds<27><<<(unsigned)n_mats, 256, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 8: k_project_ub_lds<8><<<(unsigned)n_mats, 512, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 10: k_project_ub_lds<10><<<(unsigned)n_mats, 512, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 27: k_project_ub_lds<27><<<(unsigned)n_mats, 512, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
         default: return -1;
         }
     }
