@@ -275,8 +275,9 @@ int symx_hip_loop_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, 
 int symx_hip_problem_profile(symx_hip_problem prob, int32_t phase, int32_t pot, int32_t arg, int32_t reps, double* avg_ms);
 
 /* ---- standalone kernels on caller-provided device memory (tests) ---------- */
-/* In-place PD projection of n_mats dense row-major n x n FP64 matrices
- * (project_to_PD.cpp:13-82). changed (nullable, device) gets 1 per rebuilt matrix. */
+/* In-place PD projection of n_mats dense symmetric row-major n x n FP64 matrices,
+ * n in {3, 6, 9, 12, 24, 30, 81} (project_to_PD.cpp:13-82). changed (nullable, device)
+ * gets 1 per rebuilt matrix. */
 int symx_hip_project_dense(symx_hip_ctx ctx, double* d_mats, int64_t n_mats, int32_t n, double eps,
                            int32_t mirror, uint8_t* d_changed);
 

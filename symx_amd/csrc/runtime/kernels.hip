@@ -297,41 +297,150 @@ namespace symx_hip
         count_changed(changed, count);
     }
 
-    // Large element Hessians (8 and more DoF blocks: Hex8, Tet10, Hex27): one workgroup
-    // of 512 threads per matrix, the matrix in LDS.
-    //  1. Fast path for translation-invariant elements (every block row sums to zero, as
-    //     for every elastic energy): the translations are exact null vectors, so shifting
-    //     them up by c (adding c/NB to the diagonal entries of every 3x3 block) leaves the
-    //     other eigenvalues alone. If the Cholesky factorisation of the shifted matrix minus
-    //     eps I succeeds, the only eigenvalues below eps are the three zeros, and the
-    //     projection adds f(0) on the translation space: H + eps/NB (1 1^T (x) I3) (clamp) or
-    //     H unchanged (mirror), which the register kernels of 4-node elements also do.
-    //  2. Otherwise parallel cyclic Jacobi with the round-robin pairing of Brent and Luk
-    //     (n/2 disjoint rotations per round, n-1 rounds per sweep): the rotation of
-    //     jacobi_rotate, A <- J^T A J as a column pass then a row pass with the rotated
-    //     2x2 block set exactly, V <- V J, then V f(Lambda) V^T when some eigenvalue < eps.
-    template <int NB>
-    __global__ __launch_bounds__(512) void k_project_ub_lds(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
-                                                           unsigned long long* count, const uint8_t* __restrict__ select)
+    // Large element Hessians (8 and more DoF blocks: Hex8, Tet10, Hex27) and the dense
+    // standalone entry for n > 12: one workgroup per matrix, the matrix in LDS. Instead of
+    // a full eigen-decomposition (project_to_PD.cpp:13-32 rebuilds V f(Lambda) V^T) only the
+    // eigenpairs below eps are computed, and the projection is written as the rank-k update
+    //     H + sum_j (f(lambda_j) - lambda_j) u_j u_j^T,   f = eps (clamp) or -lambda (mirror),
+    // which is the same spectral function. Steps:
+    //  1. Translation-invariant element matrices (every block row sums to zero, as for every
+    //     elastic energy) get the translations shifted up by max|H| (adding max|H|/NB to the
+    //     same-component entries of every 3x3 block), which leaves every other eigenpair
+    //     alone; their exact zero eigenvalues then map to f(0) directly: + eps/NB (1 1^T (x) I3)
+    //     for clamp, nothing for mirror, as the register kernels of 4-node elements do.
+    //  2. Householder tridiagonalisation T = Q^T A Q (LAPACK dsytd2 arithmetic; v_0 = 1 and
+    //     the rest of v stored below the subdiagonal). Two workgroup barriers per column.
+    //  3. Sturm count of T - eps I: the number k of eigenvalues below eps. k = 0 ends here.
+    //  4. The k eigenvalues by multisection: every thread evaluates Sturm counts at two
+    //     points per round; the first round spreads all points over [Gershgorin low, eps],
+    //     later rounds split each eigenvalue's bracket, until every bracket is below
+    //     4 ulp(||T||).
+    //  5. Eigenvectors of T by inverse iteration (dstein): LU of T - lambda I with partial
+    //     pivoting, three solves from a fixed pseudo-random start, tiny pivots replaced by
+    //     ulp(||T||); one lane per eigenvalue, Gram-Schmidt within clusters of close
+    //     eigenvalues between the solves.
+    //  6. Back-transform u = Q y (one wave per vector, no barriers), then the rank-k update.
+    constexpr int PNT = 256;   // threads per matrix
+    constexpr int PPTS = 2;    // Sturm points per thread per round
+    constexpr int PLU = 16;    // lanes running inverse iteration at once
+
+    // Full-wave reductions through DPP (quad permutes and row rotations stay in the VALU;
+    // __shfl_xor goes through ds_bpermute at LDS latency), then the four row results read
+    // as scalars. Every lane must be active; the result is the same in every lane and the
+    // summation order is fixed.
+    template <int CTRL>
+    __device__ __forceinline__ double dpp_mov(double v)
     {
-        constexpr int NT = 512, N = 3 * NB, NP = (N + 1) & ~1, LD = NP + 1, NUB = NB * (NB + 1) / 2, M = NP / 2;
-        __shared__ double A[NP * LD];
-        __shared__ double V[NP * LD];
-        __shared__ double c_[M], s_[M], app_[M], aqq_[M];
-        __shared__ int pq[2 * M];
-        __shared__ double red[NT / 64];
-        __shared__ double lam[NP];
-        __shared__ int flag;
+        const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+        return __hiloint2double(hi, lo);
+    }
+    __device__ __forceinline__ double readlane_d(double v, int l)
+    {
+        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l), __builtin_amdgcn_readlane(__double2loint(v), l));
+    }
+    __device__ inline double wave_sum_all(double v)
+    {
+        v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+        v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+        v += dpp_mov<0x124>(v);  // row_ror:4
+        v += dpp_mov<0x128>(v);  // row_ror:8
+        return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+    }
+    __device__ inline double wave_max_all(double v)
+    {
+        v = fmax(v, dpp_mov<0xB1>(v));
+        v = fmax(v, dpp_mov<0x4E>(v));
+        v = fmax(v, dpp_mov<0x124>(v));
+        v = fmax(v, dpp_mov<0x128>(v));
+        return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
+    }
+    __device__ inline double wave_min_all(double v)
+    {
+        v = fmin(v, dpp_mov<0xB1>(v));
+        v = fmin(v, dpp_mov<0x4E>(v));
+        v = fmin(v, dpp_mov<0x124>(v));
+        v = fmin(v, dpp_mov<0x128>(v));
+        return fmin(fmin(readlane_d(v, 0), readlane_d(v, 16)), fmin(readlane_d(v, 32), readlane_d(v, 48)));
+    }
+    // 1/x for |x| >= DBL_MIN: hardware reciprocal estimate and two Newton steps
+    // (r <- r + r (1 - x r)), within an ulp or so of the rounded quotient.
+    __device__ __forceinline__ double rcp_nr(double x)
+    {
+        double r = __builtin_amdgcn_rcp(x);
+        r = fma(r, fma(-x, r, 1.0), r);
+        r = fma(r, fma(-x, r, 1.0), r);
+        return r;
+    }
+
+    // Number of eigenvalues of the tridiagonal (dg, od) below x0 and below x1 (two
+    // interleaved recurrences; a pivot smaller than pivmin counts as -pivmin, dlaneg).
+    template <int N>
+    __device__ inline void sturm2(const double* dg, const double* od, double x0, double x1, double pivmin, int& c0,
+                                  int& c1)
+    {
+        double q0 = dg[0] - x0, q1 = dg[0] - x1;
+        if (fabs(q0) < pivmin) q0 = -pivmin;
+        if (fabs(q1) < pivmin) q1 = -pivmin;
+        int n0 = q0 < 0.0, n1 = q1 < 0.0;
+        for (int i = 1; i < N; i++) {
+            const double d = dg[i], e = od[i - 1], e2 = e * e;
+            q0 = (d - x0) - e2 * rcp_nr(q0);  // |q| >= pivmin keeps e2 / q finite
+            q1 = (d - x1) - e2 * rcp_nr(q1);
+            if (fabs(q0) < pivmin) q0 = -pivmin;
+            if (fabs(q1) < pivmin) q1 = -pivmin;
+            n0 += q0 < 0.0;
+            n1 += q1 < 0.0;
+        }
+        c0 = n0;
+        c1 = n1;
+    }
+
+    __device__ inline double start_component(int j, int i)
+    {
+        uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull + (uint64_t)(j + 1) * 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 31;
+        h *= 0x94D049BB133111EBull;
+        h ^= h >> 29;
+        return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+
+    template <int N, bool DENSE>
+    __global__ __launch_bounds__(PNT) void k_project_large(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
+                                                          unsigned long long* count, const uint8_t* __restrict__ select,
+                                                          uint8_t* __restrict__ changed_out)
+    {
+        constexpr int NB = N / 3, LD = N | 1, NUB = NB * (NB + 1) / 2, NW = PNT / 64, NPT = PNT * PPTS;
+        static_assert(N <= 128, "one wave holds a vector in two registers per lane");
+        __shared__ double A[N * LD];
+        __shared__ double Y[N * N];
+        __shared__ double dg[N], od[N], tau[N], lamv[N], pv[N], wv[N], bra[N], brb[N];
+        __shared__ double lu[3 * N * PLU];
+        __shared__ int cbuf[NPT];
+        __shared__ double red[2 * NW];
+        __shared__ int busy[3];
         const int64_t e = blockIdx.x;
         if (e >= n_mats) return;
         if (select != nullptr && !select[e]) return;
-        const int tid = threadIdx.x;
-        double* h = H + e * (int64_t)(NUB * 9);
+        const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+        double* h = DENSE ? H + e * (int64_t)(N * N) : H + e * (int64_t)(NUB * 9);
         auto ubi = [](int a, int b) { return a * NB - (a * (a - 1)) / 2 + (b - a); };
-        for (int t = tid; t < NP * NP; t += NT) {
-            const int r = t / NP, c = t - r * NP;
-            double v = 0.0;
-            if (r < N && c < N) {
+        // element (r, c) of the upper-block storage, r <= c blockwise
+        auto ub_index = [&](int t, int& r, int& c) {
+            const int u = t / 9, ij = t - 9 * u;
+            int a = 0, rem = u;
+            while (rem >= NB - a) {
+                rem -= NB - a;
+                a++;
+            }
+            r = 3 * a + ij / 3;
+            c = 3 * (a + rem) + ij % 3;
+        };
+        for (int t = tid; t < N * N; t += PNT) {
+            const int r = t / N, c = t - r * N;
+            double v;
+            if (DENSE) v = 0.5 * (h[r * N + c] + h[c * N + r]);
+            else {
                 const int a = r / 3, i = r - 3 * a, b = c / 3, j = c - 3 * b;
                 v = a <= b ? h[ubi(a, b) * 9 + 3 * i + j] : h[ubi(b, a) * 9 + 3 * j + i];
             }
@@ -339,181 +448,325 @@ namespace symx_hip
         }
         __syncthreads();
         auto block_max = [&](double v) {
-            for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-            if ((tid & 63) == 0) red[tid >> 6] = v;
+            v = wave_max_all(v);
+            if (lane == 0) red[wid] = v;
             __syncthreads();
             double r = red[0];
-            for (int w = 1; w < NT / 64; w++) r = fmax(r, red[w]);
+            for (int w = 1; w < NW; w++) r = fmax(r, red[w]);
             __syncthreads();
             return r;
         };
-        auto block_sum = [&](double v) {
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if ((tid & 63) == 0) red[tid >> 6] = v;
-            __syncthreads();
-            double r = 0.0;
-            for (int w = 0; w < NT / 64; w++) r += red[w];
-            __syncthreads();
-            return r;
-        };
-        // ---- 1. translation-invariant fast path --------------------------------------
+        // ---- 1. scale, translation invariance ---------------------------------------------
         double amax = 0.0, rsum = 0.0;
-        for (int t = tid; t < N * N; t += NT) amax = fmax(amax, fabs(A[(t / N) * LD + t % N]));
-        for (int t = tid; t < N * 3; t += NT) {  // row r, column offset j: sum over blocks b of A[r][3b + j]
-            const int r = t / 3, j = t - 3 * r;
-            double acc = 0.0;
-            for (int b = 0; b < NB; b++) acc += A[r * LD + 3 * b + j];
-            rsum = fmax(rsum, fabs(acc));
+        for (int t = tid; t < N * N; t += PNT) amax = fmax(amax, fabs(A[(t / N) * LD + t % N]));
+        if (!DENSE) {
+            for (int t = tid; t < N * 3; t += PNT) {  // row r, component j: sum over blocks b of A[r][3b + j]
+                const int r = t / 3, j = t - 3 * r;
+                double acc = 0.0;
+                for (int b = 0; b < NB; b++) acc += A[r * LD + 3 * b + j];
+                rsum = fmax(rsum, fabs(acc));
+            }
         }
         const double scale = block_max(amax);
-        const double row_sum = block_max(rsum);
-        if (row_sum <= 1e-10 * scale * NB) {
-            const double shift = scale * NB;
-            for (int t = tid; t < N * N; t += NT) {
+        const bool ti = !DENSE && block_max(rsum) <= 1e-10 * scale * NB && scale > 0.0;
+        if (!(scale > 0.0)) {
+            // zero matrix: every eigenvalue (0) is below eps -> eps I (clamp), 0 (mirror)
+            if (!mirror) {
+                for (int r = tid; r < N; r += PNT) {
+                    if (DENSE) h[r * N + r] = eps;
+                    else h[ubi(r / 3, r / 3) * 9 + 4 * (r % 3)] = eps;
+                }
+            }
+            if (tid == 0) {
+                if (count != nullptr) atomicAdd(count, 1ull);
+                if (changed_out != nullptr) changed_out[e] = 1;
+            }
+            return;
+        }
+        if (ti) {
+            const double sh = scale / NB;
+            for (int t = tid; t < N * N; t += PNT) {
                 const int r = t / N, c = t - r * N;
-                V[r * LD + c] = A[r * LD + c] + ((r % 3) == (c % 3) ? shift / NB : 0.0) - (r == c ? eps : 0.0);
+                if ((r % 3) == (c % 3)) A[r * LD + c] += sh;
             }
-            if (tid == 0) flag = 0;
             __syncthreads();
-            for (int k = 0; k < N; k++) {  // right-looking Cholesky, lower triangle
-                if (tid == 0) {
-                    const double d = V[k * LD + k];
-                    if (!(d > 0.0)) flag = 1;
-                    else V[k * LD + k] = sqrt(d);
-                }
-                __syncthreads();
-                if (flag) break;
-                const double piv = V[k * LD + k];
-                for (int i = k + 1 + tid; i < N; i += NT) V[i * LD + k] /= piv;
-                __syncthreads();
-                const int m = N - k - 1;
-                for (int t = tid; t < m * m; t += NT) {
-                    const int i = k + 1 + t / m, j = k + 1 + t % m;
-                    if (j <= i) V[i * LD + j] -= V[i * LD + k] * V[j * LD + k];
-                }
-                __syncthreads();
+        }
+        // ---- 2. Householder tridiagonalisation --------------------------------------------
+        for (int k = 0; k < N - 2; k++) {
+            const int m = N - k - 1;
+            double s = 0.0;
+            for (int i = k + 2 + lane; i < N; i += 64) {
+                const double a = A[i * LD + k];
+                s += a * a;
             }
-            if (!flag) {
-                if (!mirror) {
-                    for (int t = tid; t < NUB * 9; t += NT) {
-                        const int ij = t % 9;
-                        if (ij / 3 == ij % 3) h[t] += eps / NB;
+            s = wave_sum_all(s);
+            const double x0 = A[(k + 1) * LD + k];
+            double beta = x0, t = 0.0, invd = 0.0;
+            if (s != 0.0) {
+                beta = -copysign(sqrt(x0 * x0 + s), x0);
+                t = (beta - x0) / beta;
+                invd = 1.0 / (x0 - beta);
+            }
+            if (tid < m) {  // p = t A22 v, column access (A22 symmetric)
+                const int c = k + 1 + tid;
+                double a0 = 0.0, a1 = 0.0;
+                int j = 1;
+                for (; j + 1 < m; j += 2) {
+                    a0 += A[(k + 1 + j) * LD + c] * A[(k + 1 + j) * LD + k];
+                    a1 += A[(k + 2 + j) * LD + c] * A[(k + 2 + j) * LD + k];
+                }
+                if (j < m) a0 += A[(k + 1 + j) * LD + c] * A[(k + 1 + j) * LD + k];
+                pv[tid] = t * (A[(k + 1) * LD + c] + invd * (a0 + a1));
+                wv[tid] = tid == 0 ? 1.0 : A[c * LD + k] * invd;
+            }
+            __syncthreads();
+            double kk = 0.0;
+            for (int i = lane; i < m; i += 64) kk += pv[i] * wv[i];
+            kk = 0.5 * t * wave_sum_all(kk);
+            if (t != 0.0) {
+                const int step = PNT / m;
+                if (tid < step * m) {
+                    const int j = tid % m;
+                    const double vj = wv[j], wj = pv[j] - kk * vj;
+                    for (int i = tid / m; i < m; i += step) {
+                        const double vi = wv[i], wi = pv[i] - kk * vi;
+                        A[(k + 1 + i) * LD + k + 1 + j] -= vi * wj + wi * vj;
                     }
+                }
+            }
+            if (tid >= 1 && tid < m) A[(k + 1 + tid) * LD + k] = wv[tid];
+            if (tid == 0) {
+                dg[k] = A[k * LD + k];
+                od[k] = beta;
+                tau[k] = t;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            dg[N - 2] = A[(N - 2) * LD + N - 2];
+            od[N - 2] = A[(N - 1) * LD + N - 2];
+            dg[N - 1] = A[(N - 1) * LD + N - 1];
+            od[N - 1] = 0.0;
+            tau[N - 2] = 0.0;
+            tau[N - 1] = 0.0;
+        }
+        __syncthreads();
+        // ---- 3. count below eps -----------------------------------------------------------
+        double tn = 0.0, glo = 0.0, emax = 0.0;
+        {
+            double tnl = 0.0, gl = INFINITY, el = 0.0;
+            for (int i = lane; i < N; i += 64) {
+                const double l = i > 0 ? fabs(od[i - 1]) : 0.0, r = i < N - 1 ? fabs(od[i]) : 0.0;
+                tnl = fmax(tnl, fabs(dg[i]) + l + r);
+                gl = fmin(gl, dg[i] - l - r);
+                el = fmax(el, r * r);
+            }
+            tn = wave_max_all(tnl);
+            glo = wave_min_all(gl);
+            emax = wave_max_all(el);
+        }
+        const double ulp = 2.220446049250313e-16;
+        const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
+        const double tol = 4.0 * ulp * tn;
+        glo -= 2.0 * ulp * tn + pivmin;
+        int cnt, dummy;
+        sturm2<N>(dg, od, eps, eps, pivmin, cnt, dummy);
+        if (cnt == 0) {
+            if (ti && !mirror) {
+                for (int t = tid; t < NUB * 9; t += PNT) {
+                    const int ij = t % 9;
+                    if (ij / 3 == ij % 3) h[t] += eps / NB;
                 }
                 if (tid == 0 && count != nullptr) atomicAdd(count, 1ull);
-                return;
             }
+            if (tid == 0 && changed_out != nullptr) changed_out[e] = (ti && !mirror) ? 1 : 0;
+            return;
         }
-        // ---- 2. parallel cyclic Jacobi ----------------------------------------------------
-        for (int t = tid; t < NP * NP; t += NT) {
-            const int r = t / NP, c = t - r * NP;
-            V[r * LD + c] = r == c ? 1.0 : 0.0;
+        // ---- 4. multisection for the cnt lowest eigenvalues ------------------------------
+        {
+            const double w0 = (eps - glo) / (NPT + 1);
+            const int p0 = tid * PPTS;
+            int c0, c1;
+            sturm2<N>(dg, od, glo + w0 * (p0 + 1), glo + w0 * (p0 + 2), pivmin, c0, c1);
+            cbuf[p0] = c0;
+            cbuf[p0 + 1] = c1;
+            if (tid == 0) busy[0] = 0;
+            __syncthreads();
+            for (int j = tid; j < cnt; j += PNT) {  // first point with more than j eigenvalues below it
+                int lo = 0, hi = NPT;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cbuf[mid] > j) hi = mid;
+                    else lo = mid + 1;
+                }
+                bra[j] = lo > 0 ? glo + w0 * lo : glo;
+                brb[j] = lo < NPT ? glo + w0 * (lo + 1) : eps;
+            }
+            __syncthreads();
+            const int P = min(64, NPT / cnt);
+            for (int round = 0; round < 16; round++) {
+                int pts[PPTS];
+                double xs[PPTS];
+#pragma unroll
+                for (int q = 0; q < PPTS; q++) {
+                    const int p = tid * PPTS + q, j = p / P, l = p - j * P;
+                    pts[q] = j < cnt ? p : -1;
+                    xs[q] = j < cnt ? bra[j] + (brb[j] - bra[j]) * ((double)(l + 1) / (P + 1)) : 0.0;
+                }
+                if (pts[0] >= 0) {
+                    sturm2<N>(dg, od, xs[0], xs[1], pivmin, c0, c1);
+                    cbuf[pts[0]] = c0;
+                    if (pts[1] >= 0) cbuf[pts[1]] = c1;
+                }
+                if (tid == 0) busy[(round + 1) % 3] = 0;
+                __syncthreads();
+                for (int j = tid; j < cnt; j += PNT) {
+                    const double a = bra[j], b = brb[j];
+                    if (b - a > tol) {
+                        int lo = 0, hi = P;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (cbuf[j * P + mid] > j) hi = mid;
+                            else lo = mid + 1;
+                        }
+                        const double na = lo > 0 ? a + (b - a) * ((double)lo / (P + 1)) : a;
+                        const double nb = lo < P ? a + (b - a) * ((double)(lo + 1) / (P + 1)) : b;
+                        bra[j] = na;
+                        brb[j] = nb;
+                        if (nb - na > tol) busy[round % 3] = 1;
+                    }
+                }
+                __syncthreads();
+                if (!busy[round % 3]) break;
+            }
+            for (int j = tid; j < cnt; j += PNT) lamv[j] = 0.5 * (bra[j] + brb[j]);
+            __syncthreads();
         }
-        double fro = 0.0;
-        for (int t = tid; t < N * N; t += NT) {
-            const int r = t / N, c = t - r * N;
-            if (r <= c) fro += A[r * LD + c] * A[r * LD + c];
+        // ---- 5. block inverse iteration (dstein), one lane per eigenvalue --------------------
+        // Three rounds: every vector is solved with its own shift (LU with partial pivoting
+        // fused with the forward substitution; U's reciprocal pivots and two superdiagonals in
+        // LDS), then wave 0 normalises the vectors and orthonormalises those whose eigenvalues
+        // are within 1e-9 ||T|| of each other (numerically degenerate: inverse iteration alone
+        // cannot separate them). A last pass orthonormalises chains of eigenvalues within
+        // 1e-3 ||T|| (dstein's reorthogonalisation threshold).
+        {
+            double* ur = lu;
+            double* u1 = lu + N * PLU;
+            double* u2 = lu + 2 * N * PLU;
+            const double ptol = ulp * tn, bigr = 1.0 / ptol;
+            for (int t = tid; t < cnt * N; t += PNT) {
+                const int j = t / N;
+                Y[t] = start_component(j, t - j * N);
+            }
+            __syncthreads();
+            auto orthonormalise = [&](double gtol, int passes) {  // wave 0
+                int j0 = 0;
+                for (int j = 0; j < cnt; j++) {
+                    if (j == 0 || lamv[j] - lamv[j - 1] > gtol) j0 = j;
+                    double* y = Y + j * N;
+                    double y0 = lane < N ? y[lane] : 0.0, y1 = lane + 64 < N ? y[lane + 64] : 0.0;
+                    for (int pass = 0; pass < (j > j0 ? passes : 0); pass++) {
+                        const double inv = 1.0 / sqrt(wave_sum_all(y0 * y0 + y1 * y1));
+                        y0 *= inv;
+                        y1 *= inv;
+                        for (int i = j0; i < j; i++) {
+                            const double* u = Y + i * N;
+                            const double v0 = lane < N ? u[lane] : 0.0, v1 = lane + 64 < N ? u[lane + 64] : 0.0;
+                            const double d = wave_sum_all(v0 * y0 + v1 * y1);
+                            y0 -= d * v0;
+                            y1 -= d * v1;
+                        }
+                    }
+                    const double inv = 1.0 / sqrt(wave_sum_all(y0 * y0 + y1 * y1));
+                    if (lane < N) y[lane] = y0 * inv;
+                    if (lane + 64 < N) y[lane + 64] = y1 * inv;
+                }
+            };
+            for (int it = 0; it < 3; it++) {
+                if (tid < PLU) {
+                    for (int j = tid; j < cnt; j += PLU) {
+                        double* y = Y + j * N;
+                        const double lam = lamv[j];
+                        double a = dg[0] - lam, c = od[0], r0 = y[0];
+                        for (int k = 0; k < N - 1; k++) {
+                            const double b = od[k], an = dg[k + 1] - lam, cn = k + 1 < N - 1 ? od[k + 1] : 0.0;
+                            const double r1 = y[k + 1];
+                            // partial pivoting: rows k and k + 1 interchanged when |b| > |a|
+                            const bool sw = fabs(a) < fabs(b);
+                            const double piv = sw ? b : a, oth = sw ? a : b;
+                            const double inv = fabs(piv) >= 2.2250738585072014e-308 ? rcp_nr(piv) : 0.0;
+                            const double mult = oth * inv;  // |mult| <= 1; a = b = 0 gives 0
+                            u1[k * PLU + tid] = sw ? an : c;
+                            u2[k * PLU + tid] = sw ? cn : 0.0;
+                            y[k] = sw ? r1 : r0;
+                            r0 = sw ? r0 - mult * r1 : r1 - mult * r0;
+                            const double an2 = sw ? c - mult * an : an - mult * c;
+                            c = sw ? -mult * cn : cn;
+                            a = an2;
+                            ur[k * PLU + tid] = fabs(piv) < ptol ? (piv < 0.0 ? -bigr : bigr) : inv;
+                        }
+                        ur[(N - 1) * PLU + tid] = fabs(a) < ptol ? (a < 0.0 ? -bigr : bigr) : rcp_nr(a);
+                        double y1 = r0 * ur[(N - 1) * PLU + tid], y2 = 0.0;
+                        y[N - 1] = y1;
+                        for (int k = N - 2; k >= 0; k--) {  // back substitution, U has two superdiagonals
+                            const double v = (y[k] - u1[k * PLU + tid] * y1 - u2[k * PLU + tid] * y2) * ur[k * PLU + tid];
+                            y[k] = v;
+                            y2 = y1;
+                            y1 = v;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (wid == 0) orthonormalise(1e-9 * tn, 2);
+                __syncthreads();
+            }
+            if (wid == 0) orthonormalise(1e-3 * tn, 1);
+            __syncthreads();
         }
-        const double tol = 1e-32 * block_sum(fro);
-        for (int sweep = 0; sweep < 24; sweep++) {
-            double off = 0.0;
-            for (int t = tid; t < N * N; t += NT) {
+        // ---- 6. back-transform u = H_0 H_1 ... H_{N-3} y, one wave per vector -----------------
+        for (int j = wid; j < cnt; j += NW) {
+            double* y = Y + j * N;
+            double y0 = lane < N ? y[lane] : 0.0, y1 = lane + 64 < N ? y[lane + 64] : 0.0;
+            for (int k = N - 3; k >= 0; k--) {
+                const double t = tau[k];
+                if (t == 0.0) continue;
+                // v: 0 above k + 1, 1 at k + 1, A[i][k] below
+                const double v0 = lane < k + 1 || lane >= N ? 0.0 : (lane == k + 1 ? 1.0 : A[lane * LD + k]);
+                const int i1 = lane + 64;
+                const double v1 = i1 < k + 1 || i1 >= N ? 0.0 : (i1 == k + 1 ? 1.0 : A[i1 * LD + k]);
+                const double d = t * wave_sum_all(v0 * y0 + v1 * y1);
+                y0 -= d * v0;
+                y1 -= d * v1;
+            }
+            if (lane < N) y[lane] = y0;
+            if (lane + 64 < N) y[lane + 64] = y1;
+        }
+        __syncthreads();
+        // ---- 7. rank-cnt update -------------------------------------------------------------
+        for (int j = tid; j < cnt; j += PNT) lamv[j] = mirror ? -2.0 * lamv[j] : eps - lamv[j];
+        __syncthreads();
+        if (DENSE) {
+            for (int t = tid; t < N * N; t += PNT) {
                 const int r = t / N, c = t - r * N;
-                if (r < c) off += A[r * LD + c] * A[r * LD + c];
-            }
-            if (!(block_sum(off) > tol)) break;
-            for (int round = 0; round < NP - 1; round++) {
-                if (tid < M) {
-                    // circle method: index NP-1 stays, the others rotate
-                    int p, q;
-                    if (tid == 0) {
-                        p = round;
-                        q = NP - 1;
-                    }
-                    else {
-                        p = (round + tid) % (NP - 1);
-                        q = (round - tid + NP - 1) % (NP - 1);
-                    }
-                    if (p > q) {
-                        const int x = p;
-                        p = q;
-                        q = x;
-                    }
-                    const double apq = A[p * LD + q], app = A[p * LD + p], aqq = A[q * LD + q];
-                    const double d = aqq - app;
-                    const double rr = sqrt(d * d + 4.0 * apq * apq);
-                    const double den = d + copysign(rr, d);
-                    const double tt = (den != 0.0) ? (2.0 * apq) / den : 0.0;
-                    const double c = 1.0 / sqrt(1.0 + tt * tt);
-                    c_[tid] = c;
-                    s_[tid] = tt * c;
-                    app_[tid] = app - tt * apq;
-                    aqq_[tid] = aqq + tt * apq;
-                    pq[2 * tid] = p;
-                    pq[2 * tid + 1] = q;
-                }
-                __syncthreads();
-                for (int t = tid; t < NP * M; t += NT) {  // columns of A and V
-                    const int r = t / M, k = t - r * M;
-                    const int p = pq[2 * k], q = pq[2 * k + 1];
-                    const double c = c_[k], s = s_[k];
-                    const double g = A[r * LD + p], hh = A[r * LD + q];
-                    A[r * LD + p] = c * g - s * hh;
-                    A[r * LD + q] = s * g + c * hh;
-                    const double gv = V[r * LD + p], hv = V[r * LD + q];
-                    V[r * LD + p] = c * gv - s * hv;
-                    V[r * LD + q] = s * gv + c * hv;
-                }
-                __syncthreads();
-                for (int t = tid; t < M * NP; t += NT) {  // rows of A; the rotated 2x2 block exactly
-                    const int k = t / NP, col = t - k * NP;
-                    const int p = pq[2 * k], q = pq[2 * k + 1];
-                    if (col == p) {
-                        A[p * LD + p] = app_[k];
-                        A[q * LD + p] = 0.0;
-                    }
-                    else if (col == q) {
-                        A[p * LD + q] = 0.0;
-                        A[q * LD + q] = aqq_[k];
-                    }
-                    else {
-                        const double c = c_[k], s = s_[k];
-                        const double g = A[p * LD + col], hh = A[q * LD + col];
-                        A[p * LD + col] = c * g - s * hh;
-                        A[q * LD + col] = s * g + c * hh;
-                    }
-                }
-                __syncthreads();
+                double acc = 0.0;
+                for (int j = 0; j < cnt; j++) acc += lamv[j] * Y[j * N + r] * Y[j * N + c];
+                h[t] += acc;
             }
         }
-        if (tid == 0) flag = 0;
-        __syncthreads();
-        for (int t = tid; t < N; t += NT) {
-            const double l = A[t * LD + t];
-            if (l < eps) {
-                lam[t] = mirror ? -l : eps;
-                flag = 1;
-            }
-            else {
-                lam[t] = l;
+        else {
+            const double tshift = (ti && !mirror) ? eps / NB : 0.0;
+            for (int t = tid; t < NUB * 9; t += PNT) {
+                int r, c;
+                ub_index(t, r, c);
+                double acc = (r % 3) == (c % 3) ? tshift : 0.0;
+                for (int j = 0; j < cnt; j++) acc += lamv[j] * Y[j * N + r] * Y[j * N + c];
+                h[t] += acc;
             }
         }
-        __syncthreads();
-        if (!flag) return;
-        for (int t = tid; t < NUB * 9; t += NT) {
-            const int u = t / 9, ij = t - 9 * u;
-            int a = 0, rem = u;
-            while (rem >= NB - a) {
-                rem -= NB - a;
-                a++;
-            }
-            const int b = a + rem;
-            const int r = 3 * a + ij / 3, c = 3 * b + ij % 3;
-            double acc = 0.0;
-            for (int k = 0; k < N; k++) acc += V[r * LD + k] * lam[k] * V[c * LD + k];
-            h[t] = acc;
+        if (tid == 0) {
+            if (count != nullptr) atomicAdd(count, 1ull);
+            if (changed_out != nullptr) changed_out[e] = 1;
         }
-        if (tid == 0 && count != nullptr) atomicAdd(count, 1ull);
     }
 
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
@@ -529,9 +782,9 @@ namespace symx_hip
             k_project_ub4_reduced<<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags, select);
             k_project_ub_full<4><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags);
             return 0;
-        case 8: k_project_ub_lds<8><<<(unsigned)n_mats, 512, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
-        case 10: k_project_ub_lds<10><<<(unsigned)n_mats, 512, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
-        case 27: k_project_ub_lds<27><<<(unsigned)n_mats, 512, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 8: k_project_large<24, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
+        case 10: k_project_large<30, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
+        case 27: k_project_large<81, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
         default: return -1;
         }
     }
@@ -658,6 +911,9 @@ namespace symx_hip
         case 6: k_project_dense<6><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
         case 9: k_project_dense<9><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
         case 12: k_project_dense<12><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
+        case 24: k_project_large<24, true><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
+        case 30: k_project_large<30, true><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
+        case 81: k_project_large<81, true><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
         default: return -1;
         }
     }

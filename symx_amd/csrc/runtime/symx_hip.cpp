@@ -1403,7 +1403,7 @@ int symx_hip_project_dense(symx_hip_ctx ctx, double* d_mats, int64_t n_mats, int
 {
     if (!ctx || (!d_mats && n_mats > 0)) return fail(ctx, SYMX_HIP_ERR_ARG, "null argument");
     if (launch_project_dense(ctx->stream, d_mats, n_mats, n, eps, mirror, d_changed))
-        return fail(ctx, SYMX_HIP_ERR_ARG, "dense projection supports n in {3, 6, 9, 12}");
+        return fail(ctx, SYMX_HIP_ERR_ARG, "dense projection supports n in {3, 6, 9, 12, 24, 30, 81}");
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return SYMX_HIP_OK;

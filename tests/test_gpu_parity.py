@@ -124,13 +124,24 @@ def test_projection_and_assembly_match_oracle(gpu, oracle, mirror):
     assert n_changed >= 1
 
 
-@pytest.mark.parametrize("n", [3, 6, 9, 12])
+@pytest.mark.parametrize("n", [3, 6, 9, 12, 24, 30, 81])
 def test_dense_projection_matches_oracle(gpu, oracle, n):
+    """n <= 12: register Jacobi; n >= 24: the LDS kernel (tridiagonalisation, Sturm
+    multisection, inverse iteration, rank-k update). Besides random matrices: repeated
+    eigenvalues (an orthogonal similarity of a spectrum with triples), -3 I (every
+    eigenvalue below eps and all equal) and the zero matrix."""
     torch = pytest.importorskip("torch")
     rng = np.random.default_rng(n)
     M = rng.normal(size=(200, n, n)) * 10.0 ** rng.uniform(-2, 4, size=(200, 1, 1))
     A = M + np.transpose(M, (0, 2, 1))
     A[:20] = A[:20] @ np.transpose(A[:20], (0, 2, 1)) + np.eye(n)  # some already PD
+    for i in range(20, 30):
+        Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        lam = np.repeat(rng.normal(size=n // 3) * 10.0 ** rng.uniform(-2, 4), 3)
+        A[i] = (Q * lam) @ Q.T
+        A[i] = 0.5 * (A[i] + A[i].T)
+    A[30] = -3.0 * np.eye(n)
+    A[31] = 0.0
     for mirror in (False, True):
         ref, ch = oracle.project(A, 1e-10, mirror)
         d = torch.tensor(A, dtype=torch.float64, device="cuda")
@@ -139,7 +150,7 @@ def test_dense_projection_matches_oracle(gpu, oracle, n):
         symx_amd.project_dense(ctx, d.data_ptr(), A.shape[0], n, 1e-10, mirror, changed.data_ptr())
         out = d.cpu().numpy()
         ctx.close()
-        scale = np.abs(A).max(axis=(1, 2))
+        scale = np.maximum(np.abs(A).max(axis=(1, 2)), 1e-300)
         assert (np.abs(out - ref).max(axis=(1, 2)) / scale).max() <= 1e-12
         assert np.array_equal(changed.cpu().numpy().astype(bool), ch)
         assert np.array_equal(out[:20], A[:20])  # PD matrices untouched bitwise
