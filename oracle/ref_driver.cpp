@@ -169,7 +169,9 @@ static int run_solve(int argc, char** argv)
     context->compilation_directory = symx::get_codegen_dir();
     context->n_threads = threads;
     NewtonsMethod newton(G, context);
-    newton.settings.projection_mode = projection_mode == 0 ? ProjectionToPD::Newton : ProjectionToPD::ProjectedNewton;
+    const ProjectionToPD modes[4] = { ProjectionToPD::Newton, ProjectionToPD::ProjectedNewton, ProjectionToPD::ProjectOnDemand,
+                                      ProjectionToPD::Progressive };
+    newton.settings.projection_mode = modes[projection_mode >= 0 && projection_mode < 4 ? projection_mode : 1];
     newton.settings.project_to_pd_use_mirroring = mirror != 0;
     newton.settings.max_iterations = max_iter;
     newton.settings.residual_tolerance_abs = prm[7];

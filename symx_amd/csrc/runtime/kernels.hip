@@ -175,9 +175,15 @@ namespace symx_hip
         return changed;
     }
 
-    // 4-node element Hessian with the rigid translations as null vectors: project the
-    // 9x9 restriction M = U^T H U (U = W (x) I3) and map the null space with f(0).
-    // Returns false (and leaves h untouched) if the block rows do not sum to zero.
+    // 4-node element Hessian with the rigid translations as null vectors: H = U M U^T with
+    // U = W (x) I3, so the projection acts on the 9x9 M = U^T H U and maps the null space
+    // with f(0). Only M's eigenpairs below eps are computed (Householder tridiagonalisation,
+    // Sturm bisection, inverse iteration, Gram-Schmidt, back-transform), and H is updated as
+    //     H + sum_j (f(lambda_j) - lambda_j) z_j z_j^T + f(0) T T^T,   z_j = U Q y_j,
+    // the same spectral function as the reference's V f(Lambda) V^T rebuild
+    // (project_to_PD.cpp:13-32). Returns false (and leaves h untouched) when the block rows
+    // do not sum to zero, when more than three eigenvalues of M lie below eps, or when two
+    // of them are within 1e-9 ||M|| of each other; those elements take the full Jacobi path.
     __device__ __forceinline__ bool project_element_reduced4(double* __restrict__ h, double eps, bool mirror, bool& changed)
     {
         double scale = 0.0, rowsum = 0.0;
@@ -214,25 +220,100 @@ namespace symx_hip
                 });
             });
         });
-        double v[9][9];
-        jacobi_eig<9>(m, v);
-        double lam[9];
-        bool any = false;
+        // h is read again for the update: keep its 90 entries from staying live in registers
+        asm volatile("" ::: "memory");
+        double d[9], e[9], tau[9];
+        tridiag_reg<9>(m, d, e, tau);
+        double tn = 0.0, glo = 0.0, emax = 0.0;
         static_for<0, 9>([&](auto Ic) {
             constexpr int I = decltype(Ic)::value;
-            lam[I] = pd_map(m[Sym<9>::idx(I, I)], eps, mirror, any);
+            const double l = I > 0 ? fabs(e[I > 0 ? I - 1 : 0]) : 0.0, r = fabs(e[I]);
+            tn = fmax(tn, fabs(d[I]) + l + r);
+            glo = I == 0 ? d[0] - l - r : fmin(glo, d[I] - l - r);
+            emax = fmax(emax, r * r);
         });
-        // Mp = V f(Lambda) V^T (always rebuilt: the null eigenvalues are below eps)
-        static_for<0, 9>([&](auto Ic) {
-            static_for<decltype(Ic)::value, 9>([&](auto Jc) {
-                constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
-                double acc = 0.0;
-                static_for<0, 9>([&](auto Kc) {
-                    constexpr int K = decltype(Kc)::value;
-                    acc += v[I][K] * lam[K] * v[J][K];
-                });
-                m[Sym<9>::idx(I, J)] = acc;
+        const double ulp = 2.220446049250313e-16;
+        const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
+        const double tol = 4.0 * ulp * tn;
+        glo -= 2.0 * ulp * tn + pivmin;
+        int cnt;
+        {
+            const double xe[3] = { eps, eps, eps };
+            int c3[3];
+            sturm3_reg<9>(d, e, xe, pivmin, c3);
+            cnt = c3[0];
+        }
+        if (cnt > 3) return false;
+        // the cnt lowest eigenvalues by bisection, three brackets in step
+        double lo[3] = { glo, glo, glo }, hi[3] = { eps, eps, eps };
+        for (int it = 0; it < 64; it++) {
+            bool busy = false;
+            static_for<0, 3>([&](auto Jc) { busy |= Jc() < cnt && hi[Jc()] - lo[Jc()] > tol; });
+            if (!busy) break;
+            double mid[3];
+            int c3[3];
+            static_for<0, 3>([&](auto Jc) { mid[Jc()] = 0.5 * (lo[Jc()] + hi[Jc()]); });
+            sturm3_reg<9>(d, e, mid, pivmin, c3);
+            static_for<0, 3>([&](auto Jc) {
+                constexpr int J = decltype(Jc)::value;
+                if (hi[J] - lo[J] > tol) {
+                    if (c3[J] > J) hi[J] = mid[J];
+                    else lo[J] = mid[J];
+                }
             });
+        }
+        double lam[3];
+        static_for<0, 3>([&](auto Jc) { lam[Jc()] = 0.5 * (lo[Jc()] + hi[Jc()]); });
+        if ((cnt >= 2 && lam[1] - lam[0] <= 1e-9 * tn) || (cnt >= 3 && lam[2] - lam[1] <= 1e-9 * tn)) return false;
+        // eigenvectors of T: three inverse-iteration solves each, then Gram-Schmidt
+        const double ptol = ulp * tn, bigr = 1.0 / ptol;
+        double y[3][9];
+        static_for<0, 3>([&](auto Jc) {
+            constexpr int J = decltype(Jc)::value;
+            static_for<0, 9>([&](auto Ic) { y[J][Ic()] = start_component(J, Ic()); });
+            if (J < cnt) {
+                for (int it = 0; it < 3; it++) {
+                    double ymax = 0.0;
+                    static_for<0, 9>([&](auto Ic) { ymax = fmax(ymax, fabs(y[J][Ic()])); });
+                    const double sc = 1.0 / ymax;
+                    static_for<0, 9>([&](auto Ic) { y[J][Ic()] *= sc; });
+                    tri_solve_reg<9>(d, e, lam[J], ptol, bigr, y[J]);
+                }
+            }
+            static_for<0, J>([&](auto Pc) {
+                double dp = 0.0;
+                static_for<0, 9>([&](auto Ic) { dp += y[Pc()][Ic()] * y[J][Ic()]; });
+                static_for<0, 9>([&](auto Ic) { y[J][Ic()] -= dp * y[Pc()][Ic()]; });
+            });
+            double nn = 0.0;
+            static_for<0, 9>([&](auto Ic) { nn += y[J][Ic()] * y[J][Ic()]; });
+            const double inv = J < cnt ? 1.0 / sqrt(nn) : 0.0;
+            static_for<0, 9>([&](auto Ic) { y[J][Ic()] *= inv; });
+        });
+        // back-transform u = H_0 ... H_6 y, then z = U u and the update of H
+        double z[3][12], dl[3];
+        static_for<0, 3>([&](auto Jc) {
+            constexpr int J = decltype(Jc)::value;
+            static_for<0, 7>([&](auto Kc) {
+                constexpr int K = 6 - decltype(Kc)::value;
+                double dp = y[J][K + 1];
+                static_for<K + 2, 9>([&](auto Ic) { dp += m[Sym<9>::idx(K, Ic())] * y[J][Ic()]; });
+                dp *= tau[K];
+                y[J][K + 1] -= dp;
+                static_for<K + 2, 9>([&](auto Ic) { y[J][Ic()] -= dp * m[Sym<9>::idx(K, Ic())]; });
+            });
+            static_for<0, 4>([&](auto Ac) {
+                static_for<0, 3>([&](auto Ic) {
+                    constexpr int A = decltype(Ac)::value, I = decltype(Ic)::value;
+                    double acc = 0.0;
+                    static_for<0, 3>([&](auto Al) {
+                        constexpr double w = helmert(A, decltype(Al)::value);
+                        if constexpr (w != 0.0) acc += w * y[J][3 * decltype(Al)::value + I];
+                    });
+                    z[J][3 * A + I] = acc;
+                });
+            });
+            dl[J] = J < cnt ? (mirror ? -2.0 * lam[J] : eps - lam[J]) : 0.0;
         });
         const double null_term = mirror ? 0.0 : 0.25 * eps;  // f(0) on span{translations}: eps T T^T, T T^T = (1/4) 1 1^T (x) I3
         static_for<0, 4>([&](auto Ac) {
@@ -242,14 +323,11 @@ namespace symx_hip
                         constexpr int A = decltype(Ac)::value, B = decltype(Bc)::value;
                         constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
                         double acc = (I == J) ? null_term : 0.0;
-                        static_for<0, 3>([&](auto Al) {
-                            static_for<0, 3>([&](auto Be) {
-                                constexpr int al = decltype(Al)::value, be = decltype(Be)::value;
-                                constexpr double w = helmert(A, al) * helmert(B, be);
-                                if constexpr (w != 0.0) acc += w * m[Sym<9>::idx(3 * al + I, 3 * be + J)];
-                            });
+                        static_for<0, 3>([&](auto Sc) {
+                            constexpr int S = decltype(Sc)::value;
+                            acc += dl[S] * z[S][3 * A + I] * z[S][3 * B + J];
                         });
-                        h[ub4(A, B) * 9 + 3 * I + J] = acc;
+                        h[ub4(A, B) * 9 + 3 * I + J] += acc;
                     });
                 });
             });
@@ -363,16 +441,6 @@ namespace symx_hip
         v = fmin(v, dpp_mov<0x128>(v));
         return fmin(fmin(readlane_d(v, 0), readlane_d(v, 16)), fmin(readlane_d(v, 32), readlane_d(v, 48)));
     }
-    // 1/x for |x| >= DBL_MIN: hardware reciprocal estimate and two Newton steps
-    // (r <- r + r (1 - x r)), within an ulp or so of the rounded quotient.
-    __device__ __forceinline__ double rcp_nr(double x)
-    {
-        double r = __builtin_amdgcn_rcp(x);
-        r = fma(r, fma(-x, r, 1.0), r);
-        r = fma(r, fma(-x, r, 1.0), r);
-        return r;
-    }
-
     // Number of eigenvalues of the tridiagonal (dg, od) below x0 and below x1 (two
     // interleaved recurrences; a pivot smaller than pivmin counts as -pivmin, dlaneg).
     template <int N>
@@ -394,15 +462,6 @@ namespace symx_hip
         }
         c0 = n0;
         c1 = n1;
-    }
-
-    __device__ inline double start_component(int j, int i)
-    {
-        uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull + (uint64_t)(j + 1) * 0xBF58476D1CE4E5B9ull;
-        h ^= h >> 31;
-        h *= 0x94D049BB133111EBull;
-        h ^= h >> 29;
-        return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
     }
 
     template <int N, bool DENSE>

@@ -16,6 +16,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <type_traits>
 
 namespace symx_hip
@@ -140,6 +141,142 @@ namespace symx_hip
             });
         }
         return changed;
+    }
+
+    // 1/x for |x| >= DBL_MIN: hardware reciprocal estimate and two Newton steps
+    // (r <- r + r (1 - x r)), within an ulp or so of the rounded quotient.
+    __device__ __forceinline__ double rcp_nr(double x)
+    {
+        double r = __builtin_amdgcn_rcp(x);
+        r = fma(r, fma(-x, r, 1.0), r);
+        r = fma(r, fma(-x, r, 1.0), r);
+        return r;
+    }
+
+    // Fixed pseudo-random start vector components in (-1, 1) for inverse iteration.
+    __host__ __device__ inline double start_component(int j, int i)
+    {
+        uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull + (uint64_t)(j + 1) * 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 31;
+        h *= 0x94D049BB133111EBull;
+        h ^= h >> 29;
+        return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+
+    // Householder tridiagonalisation in registers (LAPACK dsytd2 arithmetic) of the
+    // symmetric N x N matrix a (upper triangle): T = Q^T A Q with diagonal d and
+    // off-diagonal e (e[N-1] = 0), Q = H_0 ... H_{N-3}, H_k = I - tau_k v_k v_k^T,
+    // v_k(k+1) = 1 and v_k(k+2..N-1) left in a(k, k+2..N-1).
+    template <int N>
+    __device__ __forceinline__ void tridiag_reg(double (&a)[Sym<N>::size], double (&d)[N], double (&e)[N], double (&tau)[N])
+    {
+        static_for<0, N - 2>([&](auto Kc) {
+            constexpr int K = decltype(Kc)::value;
+            double s = 0.0;
+            static_for<K + 2, N>([&](auto Ic) {
+                const double x = a[Sym<N>::idx(K, decltype(Ic)::value)];
+                s += x * x;
+            });
+            const double x0 = a[Sym<N>::idx(K, K + 1)];
+            double beta = x0, t = 0.0, invd = 0.0;
+            if (s != 0.0) {
+                beta = -copysign(sqrt(x0 * x0 + s), x0);
+                t = (beta - x0) / beta;
+                invd = 1.0 / (x0 - beta);
+            }
+            double v[N], p[N];
+            v[K + 1] = 1.0;
+            static_for<K + 2, N>([&](auto Ic) {
+                constexpr int I = decltype(Ic)::value;
+                v[I] = a[Sym<N>::idx(K, I)] * invd;
+                a[Sym<N>::idx(K, I)] = v[I];
+            });
+            static_for<K + 1, N>([&](auto Ic) {
+                constexpr int I = decltype(Ic)::value;
+                double acc = 0.0;
+                static_for<K + 1, N>([&](auto Jc) { acc += a[Sym<N>::idx(I, decltype(Jc)::value)] * v[decltype(Jc)::value]; });
+                p[I] = t * acc;
+            });
+            double kk = 0.0;
+            static_for<K + 1, N>([&](auto Ic) { kk += p[decltype(Ic)::value] * v[decltype(Ic)::value]; });
+            kk *= 0.5 * t;
+            static_for<K + 1, N>([&](auto Ic) { p[decltype(Ic)::value] -= kk * v[decltype(Ic)::value]; });
+            static_for<K + 1, N>([&](auto Ic) {
+                static_for<decltype(Ic)::value, N>([&](auto Jc) {
+                    constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
+                    a[Sym<N>::idx(I, J)] -= v[I] * p[J] + p[I] * v[J];
+                });
+            });
+            d[K] = a[Sym<N>::idx(K, K)];
+            e[K] = beta;
+            tau[K] = t;
+        });
+        d[N - 2] = a[Sym<N>::idx(N - 2, N - 2)];
+        e[N - 2] = a[Sym<N>::idx(N - 2, N - 1)];
+        d[N - 1] = a[Sym<N>::idx(N - 1, N - 1)];
+        e[N - 1] = 0.0;
+        tau[N - 2] = 0.0;
+        tau[N - 1] = 0.0;
+    }
+
+    // Number of eigenvalues of T below x0, x1, x2 (three interleaved Sturm sequences of
+    // T - x I; a pivot below pivmin counts as -pivmin, dlaneg).
+    template <int N>
+    __device__ __forceinline__ void sturm3_reg(const double (&d)[N], const double (&e)[N], const double (&x)[3], double pivmin,
+                                               int (&c)[3])
+    {
+        double q[3];
+        static_for<0, 3>([&](auto Jc) {
+            constexpr int J = decltype(Jc)::value;
+            q[J] = d[0] - x[J];
+            if (fabs(q[J]) < pivmin) q[J] = -pivmin;
+            c[J] = q[J] < 0.0;
+        });
+        static_for<1, N>([&](auto Ic) {
+            constexpr int I = decltype(Ic)::value;
+            const double e2 = e[I - 1] * e[I - 1];
+            static_for<0, 3>([&](auto Jc) {
+                constexpr int J = decltype(Jc)::value;
+                q[J] = (d[I] - x[J]) - e2 * rcp_nr(q[J]);  // |q| >= pivmin keeps e2 / q finite
+                if (fabs(q[J]) < pivmin) q[J] = -pivmin;
+                c[J] += q[J] < 0.0;
+            });
+        });
+    }
+
+    // y <- (T - lam I)^{-1} y: LU with partial pivoting fused with the forward substitution
+    // (dlagtf, dlagts), pivots below ptol replaced by +-ptol (bigr = 1 / ptol).
+    template <int N>
+    __device__ __forceinline__ void tri_solve_reg(const double (&d)[N], const double (&e)[N], double lam, double ptol, double bigr,
+                                                  double (&y)[N])
+    {
+        double u1[N], u2[N], ur[N];
+        double a = d[0] - lam, c = e[0], r0 = y[0];
+        static_for<0, N - 1>([&](auto Kc) {
+            constexpr int K = decltype(Kc)::value;
+            const double b = e[K], an = d[K + 1] - lam, cn = (K + 1 < N - 1) ? e[K + 1] : 0.0;
+            const double r1 = y[K + 1];
+            const bool sw = fabs(a) < fabs(b);  // rows K and K + 1 interchanged
+            const double piv = sw ? b : a, oth = sw ? a : b;
+            const double inv = fabs(piv) >= 2.2250738585072014e-308 ? rcp_nr(piv) : 0.0;
+            const double mult = oth * inv;  // |mult| <= 1; a = b = 0 gives 0
+            u1[K] = sw ? an : c;
+            u2[K] = sw ? cn : 0.0;
+            y[K] = sw ? r1 : r0;
+            r0 = sw ? r0 - mult * r1 : r1 - mult * r0;
+            const double an2 = sw ? c - mult * an : an - mult * c;
+            c = sw ? -mult * cn : cn;
+            a = an2;
+            ur[K] = fabs(piv) < ptol ? (piv < 0.0 ? -bigr : bigr) : inv;
+        });
+        ur[N - 1] = fabs(a) < ptol ? (a < 0.0 ? -bigr : bigr) : rcp_nr(a);
+        y[N - 1] = r0 * ur[N - 1];
+        static_for<0, N - 1>([&](auto Kc) {
+            constexpr int K = N - 2 - decltype(Kc)::value;
+            double s = y[K] - u1[K] * y[K + 1];
+            if constexpr (K + 2 < N) s -= u2[K] * y[K + 2];
+            y[K] = s * ur[K];
+        });
     }
 
     // Orthonormal basis of the complement of (1,1,1,1) (Helmert contrasts).

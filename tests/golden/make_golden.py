@@ -11,6 +11,9 @@ build container:  python tests/golden/make_golden.py
   adjoint_ref_*.npz          dL/dlog_mu of the quasistatic cantilever from the reference's
                              own pieces (ref_driver adjoint) at the oracle's equilibrium
                              (`python tests/golden/make_golden.py adjoint` writes only these)
+  newton_ref_block6_{newton,pdn,ppn}.npz  the block6 case under the other projection
+                             modes: Newton, ProjectOnDemand, Progressive
+                             (`python tests/golden/make_golden.py modes`)
 """
 import json
 import os
@@ -60,13 +63,13 @@ def point_triangle_cases():
     json.dump({"source": "reference tests/sympy_derivatives.cpp", "cases": cases}, open(os.path.join(HERE, "point_triangle_sympy.json"), "w"))
 
 
-def newton_case(name, cells, noise, seed, rel_tol, cg_rel, threads=1):
+def newton_case(name, cells, noise, seed, rel_tol, cg_rel, threads=1, mode=1):
     from ref_io import read_solve_output, write_solve_input
     from scene_np import tet4_block
     from symx_amd import NewtonParams
 
     S = tet4_block(cells=cells, noise=noise, seed=seed)
-    newton = NewtonParams(0, 1, 0, 100, 0.0, rel_tol, cg_rel, 1e-12, 10000, 1e-10, 0.0)
+    newton = NewtonParams(0, mode, 0, 100, 0.0, rel_tol, cg_rel, 1e-12, 10000, 1e-10, 0.0)
     with tempfile.TemporaryDirectory() as d:
         inp, outp = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         write_solve_input(inp, S, newton, threads)
@@ -75,7 +78,7 @@ def newton_case(name, cells, noise, seed, rel_tol, cg_rel, threads=1):
     np.savez_compressed(os.path.join(HERE, f"newton_ref_{name}.npz"), x=res["x"], cells=np.array(cells), noise=noise, seed=seed,
                         rel_tol=rel_tol, cg_rel=cg_rel, result=res["result"], newton_iterations=res["newton_iterations"],
                         cg_iterations=res["cg_iterations"], ls_bt_iterations=res["ls_bt_iterations"], energy=res["energy"],
-                        seconds=res["seconds"], threads=threads)
+                        seconds=res["seconds"], threads=threads, projection_mode=mode)
     print(name, {k: v for k, v in res.items() if k != "x"})
 
 
@@ -108,6 +111,10 @@ def adjoint_case(name, cells):
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "adjoint":
         adjoint_case("cantilever16", (15, 3, 3))
+        sys.exit(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "modes":
+        for tag, mode in (("newton", 0), ("pdn", 2), ("ppn", 3)):
+            newton_case(f"block6_{tag}", (6, 6, 6), 0.2 / 6, 11, 1e-6, 1e-3, mode=mode)
         sys.exit(0)
     nh_cases()
     point_triangle_cases()

@@ -199,12 +199,14 @@ def test_mixed_precision_solve_meets_the_same_stopping_rule(gpu, oracle):
         assert err < rel * 1.05 + 1e-7, (rel, err)
 
 
-@pytest.mark.parametrize("name", ["block6", "block6_tight", "cfg1"])
+@pytest.mark.parametrize("name", ["block6", "block6_tight", "cfg1", "block6_pdn", "block6_ppn"])
 def test_newton_matches_reference_fixture(gpu, oracle, name):
     f = np.load(os.path.join(GOLDEN, f"newton_ref_{name}.npz"))
     cells = tuple(int(c) for c in f["cells"])
     prm = symx_amd.default_block_params(cells=list(cells), noise=float(f["noise"]), seed=int(f["seed"]))
-    newton = symx_amd.default_newton_params(residual_tolerance_rel=float(f["rel_tol"]), cg_rel_tolerance=float(f["cg_rel"]))
+    mode = int(f["projection_mode"]) if "projection_mode" in f.files else 1
+    newton = symx_amd.default_newton_params(residual_tolerance_rel=float(f["rel_tol"]), cg_rel_tolerance=float(f["cg_rel"]),
+                                            projection_mode=mode)
     s = symx_amd.Tet4Block(prm, newton)
     S = scene_arrays(s)
     st = s.solve()
@@ -219,6 +221,21 @@ def test_newton_matches_reference_fixture(gpu, oracle, name):
     E_ref = oracle.energy(S, x_ref, with_grad=False)
     E = oracle.energy(S, x, with_grad=False)
     assert abs(E - E_ref) <= 1e-10 * abs(E_ref)
+
+
+def test_plain_newton_failure_matches_reference_fixture(gpu):
+    """Plain Newton (no projection) on the indefinite block6 problem: the reference stops
+    at the first linear solve with LinearSystemSolveFailure; so must the device path."""
+    f = np.load(os.path.join(GOLDEN, "newton_ref_block6_newton.npz"))
+    assert int(f["projection_mode"]) == 0
+    cells = tuple(int(c) for c in f["cells"])
+    prm = symx_amd.default_block_params(cells=list(cells), noise=float(f["noise"]), seed=int(f["seed"]))
+    newton = symx_amd.default_newton_params(residual_tolerance_rel=float(f["rel_tol"]), cg_rel_tolerance=float(f["cg_rel"]),
+                                            projection_mode=0, precision=symx_amd.FP64)
+    s = symx_amd.Tet4Block(prm, newton)
+    st = s.solve()
+    assert st["result"] == int(f["result"]) and st["result_name"] == "LinearSystemSolveFailure"
+    assert st["newton_iterations"] == int(f["newton_iterations"])
 
 
 def test_newton_fp64_matches_oracle_trajectory(gpu, oracle):
