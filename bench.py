@@ -249,9 +249,14 @@ def bench_contact(args, log):
     t0 = time.perf_counter()
     s = symx_amd.Sphere(p, newton)
     log(f"setup: {s.n_elements} tets, {s.n_vertices} vertices, {len(s.contact_vertices())} contact vertices in {time.perf_counter() - t0:.1f}s")
-    for i in range(args.warmup):
-        st = s.solve()  # a solve without advancing time
-        log(f"warmup solve {i}: {st['result_name']} newton={st['newton_iterations']} {st['seconds']:.3f}s")
+    if args.warmup > 0:
+        # warm up on a second ball so that the timed trajectory starts at t = 0 (re-solving
+        # an already converged step would make its relative residual target unreachable)
+        w = symx_amd.Sphere(p, newton)
+        for i in range(args.warmup):
+            st = w.step()
+            log(f"warmup step {i}: {st['result_name']} newton={st['newton_iterations']} {st['seconds']:.3f}s")
+        w.close()
     newton_it = cg = 0
     results = []
     t0 = time.perf_counter()
@@ -428,6 +433,9 @@ def main():
     ap.add_argument("--hex-cells", type=int, default=40)
     ap.add_argument("--cpu-hex-cells", type=int, default=6)
     ap.add_argument("--adjoint-precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (the known-size kernel that scales rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE readings into HBM bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is designed, not built)", file=sys.stderr)
@@ -440,6 +448,17 @@ def main():
 
     if symx_amd.device_count() < 1:
         raise RuntimeError("bench.py needs a GPU")
+    if args.calibration_copy_mib > 0:
+        import torch
+
+        n = args.calibration_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: 3 x {args.calibration_copy_mib} MiB read + {args.calibration_copy_mib} MiB written")
+        del src, dst
     if args.workload == "cantilever_adjoint":
         bench_adjoint(args, log)
         return

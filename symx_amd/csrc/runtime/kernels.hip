@@ -182,9 +182,10 @@ namespace symx_hip
     //     H + sum_j (f(lambda_j) - lambda_j) z_j z_j^T + f(0) T T^T,   z_j = U Q y_j,
     // the same spectral function as the reference's V f(Lambda) V^T rebuild
     // (project_to_PD.cpp:13-32). Returns false (and leaves h untouched) when the block rows
-    // do not sum to zero, when more than three eigenvalues of M lie below eps, or when two
-    // of them are within 1e-9 ||M|| of each other; those elements take the full Jacobi path.
-    __device__ __forceinline__ bool project_element_reduced4(double* __restrict__ h, double eps, bool mirror, bool& changed)
+    // do not sum to zero or when more than three eigenvalues of M lie below eps; those
+    // elements take the full Jacobi path.
+    __device__ __forceinline__ bool project_element_reduced4(double* __restrict__ h, double eps, bool mirror, bool& changed,
+                                                             double* hv)
     {
         double scale = 0.0, rowsum = 0.0;
         static_for<0, 90>([&](auto Kc) { scale = fmax(scale, fabs(h[decltype(Kc)::value])); });
@@ -224,6 +225,13 @@ namespace symx_hip
         asm volatile("" ::: "memory");
         double d[9], e[9], tau[9];
         tridiag_reg<9>(m, d, e, tau);
+        // the reflectors (28 entries) wait in LDS (hv: 28 x 64 doubles, lane-interleaved)
+        // for the back-transform, which keeps the inverse iteration within 256 VGPRs
+        static_for<0, 7>([&](auto Kc) {
+            constexpr int K = decltype(Kc)::value, B = 7 * K - (K * (K - 1)) / 2;  // sum_{k<K} (7 - k)
+            static_for<K + 2, 9>([&](auto Ic) { hv[(B + Ic() - K - 2) * 64] = m[Sym<9>::idx(K, Ic())]; });
+        });
+        asm volatile("" ::: "memory");  // reload them from LDS instead of keeping the registers
         double tn = 0.0, glo = 0.0, emax = 0.0;
         static_for<0, 9>([&](auto Ic) {
             constexpr int I = decltype(Ic)::value;
@@ -264,8 +272,10 @@ namespace symx_hip
         }
         double lam[3];
         static_for<0, 3>([&](auto Jc) { lam[Jc()] = 0.5 * (lo[Jc()] + hi[Jc()]); });
-        if ((cnt >= 2 && lam[1] - lam[0] <= 1e-9 * tn) || (cnt >= 3 && lam[2] - lam[1] <= 1e-9 * tn)) return false;
-        // eigenvectors of T: three inverse-iteration solves each, then Gram-Schmidt
+        // eigenvectors of T by inverse iteration (dstein): slot by slot, three solves each,
+        // every solve followed by Gram-Schmidt against the finished lower slots; this
+        // separates numerically degenerate eigenvalues (e.g. the three rotations of an
+        // unstressed element) and only removes round-off for separated ones
         const double ptol = ulp * tn, bigr = 1.0 / ptol;
         double y[3][9];
         static_for<0, 3>([&](auto Jc) {
@@ -278,13 +288,13 @@ namespace symx_hip
                     const double sc = 1.0 / ymax;
                     static_for<0, 9>([&](auto Ic) { y[J][Ic()] *= sc; });
                     tri_solve_reg<9>(d, e, lam[J], ptol, bigr, y[J]);
+                    static_for<0, J>([&](auto Pc) {
+                        double dp = 0.0;
+                        static_for<0, 9>([&](auto Ic) { dp += y[Pc()][Ic()] * y[J][Ic()]; });
+                        static_for<0, 9>([&](auto Ic) { y[J][Ic()] -= dp * y[Pc()][Ic()]; });
+                    });
                 }
             }
-            static_for<0, J>([&](auto Pc) {
-                double dp = 0.0;
-                static_for<0, 9>([&](auto Ic) { dp += y[Pc()][Ic()] * y[J][Ic()]; });
-                static_for<0, 9>([&](auto Ic) { y[J][Ic()] -= dp * y[Pc()][Ic()]; });
-            });
             double nn = 0.0;
             static_for<0, 9>([&](auto Ic) { nn += y[J][Ic()] * y[J][Ic()]; });
             const double inv = J < cnt ? 1.0 / sqrt(nn) : 0.0;
@@ -295,12 +305,12 @@ namespace symx_hip
         static_for<0, 3>([&](auto Jc) {
             constexpr int J = decltype(Jc)::value;
             static_for<0, 7>([&](auto Kc) {
-                constexpr int K = 6 - decltype(Kc)::value;
+                constexpr int K = 6 - decltype(Kc)::value, B = 7 * K - (K * (K - 1)) / 2;
                 double dp = y[J][K + 1];
-                static_for<K + 2, 9>([&](auto Ic) { dp += m[Sym<9>::idx(K, Ic())] * y[J][Ic()]; });
+                static_for<K + 2, 9>([&](auto Ic) { dp += hv[(B + Ic() - K - 2) * 64] * y[J][Ic()]; });
                 dp *= tau[K];
                 y[J][K + 1] -= dp;
-                static_for<K + 2, 9>([&](auto Ic) { y[J][Ic()] -= dp * m[Sym<9>::idx(K, Ic())]; });
+                static_for<K + 2, 9>([&](auto Ic) { y[J][Ic()] -= dp * hv[(B + Ic() - K - 2) * 64]; });
             });
             static_for<0, 4>([&](auto Ac) {
                 static_for<0, 3>([&](auto Ic) {
@@ -350,11 +360,12 @@ namespace symx_hip
                                                                unsigned long long* count, uint8_t* __restrict__ needs_full,
                                                                const uint8_t* __restrict__ select)
     {
+        __shared__ double s_hv[28 * 64];
         const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
         bool changed = false;
         if (e < n_mats) {
             if (select == nullptr || select[e]) {
-                const bool ok = project_element_reduced4(H + e * 90, eps, mirror != 0, changed);
+                const bool ok = project_element_reduced4(H + e * 90, eps, mirror != 0, changed, s_hv + threadIdx.x);
                 needs_full[e] = ok ? 0 : 1;
             }
             else {

@@ -96,9 +96,12 @@ def assemble_np(S, He, nv):
     return A
 
 
+@pytest.mark.parametrize("noise", [0.3 / 3, 0.0])
 @pytest.mark.parametrize("mirror", [False, True])
-def test_projection_and_assembly_match_oracle(gpu, oracle, mirror):
-    s = make_scene((3, 3, 3), noise=0.3 / 3)
+def test_projection_and_assembly_match_oracle(gpu, oracle, mirror, noise):
+    """noise 0: every element at rest, with the three rotations as an exactly degenerate
+    null triple of the reduced 9x9 (inverse iteration must separate them)."""
+    s = make_scene((3, 3, 3), noise=noise)
     S = scene_arrays(s)
     s.eval(symx_amd.EVAL_PGH)
     n_changed = s.project(1e-10, mirror)
