@@ -26,6 +26,10 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the default
+# workload (101^3 block, mixed), each counter scaled by its reading for a 1 GiB device copy
+# (--calibration-copy-mib): FETCH x 2.0, WRITE x 1.0. The per-kernel table is in profiles/.
+MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 851.9e6}
 
 
 def block_params(cells):
@@ -504,8 +508,9 @@ def main():
     dominant = max(per_iter_ms, key=per_iter_ms.get)
     unit_bytes = ab[dominant] * (n_elem if dominant in ("element", "projection") else 1)
     achieved = unit_bytes / (prof_mixed[dominant] * 1e-3) / 1e9
+    traffic = MEASURED_TRAFFIC.get(("block", dominant, symx_amd.MIXED)) if cells == 100 else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "kernel": dominant, "avg_ms": round(prof_mixed[dominant], 4),
+                "traffic": None if traffic is None else int(traffic), "kernel": dominant, "avg_ms": round(prof_mixed[dominant], 4),
                 "algorithmic_bytes_per_launch": int(unit_bytes)}
     elem_hessians_per_s = n_elem / (prof_mixed["element"] * 1e-3)
 
