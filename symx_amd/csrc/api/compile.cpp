@@ -262,6 +262,13 @@ namespace symx
                << (adj ? ", const symx_hip_adjoint_params a" : "") << ")\n{\n";
             os << "    const int e = symx_block() * 256 + threadIdx.x;\n";
             if (ks.kind != KernelKind::C && !adj) os << "    double e_out = 0.0;\n";
+            // single-point gradient/Hessian kernels collect their outputs in registers and
+            // store them through symx_store_rows (coalesced); summed kernels accumulate in place
+            const bool staged = (ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH) && !multi_q;
+            if (staged) {
+                os << "    double sg[" << n << "] = {};\n";
+                if (ks.kind == KernelKind::PGH) os << "    double sh[" << n_ub * 9 << "] = {};\n";
+            }
             os << "    if (e < p.n_elem) {\n";
             if (ks.kind != KernelKind::C) os << "        const bool act = (p.active == nullptr) || (p.active[e] != 0);\n";
             emit_gather(os, g, maps, use, sum, ks.adjoint_symbols);
@@ -316,8 +323,10 @@ namespace symx
                 continue;
             }
             os << body_in << "e_out += " << ref(g, ks.outputs[0].id) << ";\n";
+            auto g_dst = [&](int i) { return staged ? "sg[" + std::to_string(i) + "]" : "p.G[" + eg + " + " + std::to_string(i) + "]"; };
+            auto h_dst = [&](int i) { return staged ? "sh[" + std::to_string(i) + "]" : "p.H[" + eh + " + " + std::to_string(i) + "]"; };
             if (ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH) {
-                for (int i = 0; i < n; i++) os << body_in << "p.G[" << eg << " + " << i << "] " << acc << " " << ref(g, ks.outputs[1 + i].id) << ";\n";
+                for (int i = 0; i < n; i++) os << body_in << g_dst(i) << " " << acc << " " << ref(g, ks.outputs[1 + i].id) << ";\n";
             }
             if (ks.kind == KernelKind::PGH) {
                 int ub = 0;
@@ -326,8 +335,8 @@ namespace symx
                         for (int i = 0; i < 3; i++) {
                             for (int j = 0; j < 3; j++) {
                                 const int row = 3 * a + i, col = 3 * b + j;
-                                os << body_in << "p.H[" << eh << " + " << (ub * 9 + 3 * i + j) << "] " << acc << " "
-                                   << ref(g, ks.outputs[1 + n + row * n + col].id) << ";\n";
+                                os << body_in << h_dst(ub * 9 + 3 * i + j) << " " << acc << " " << ref(g, ks.outputs[1 + n + row * n + col].id)
+                                   << ";\n";
                             }
                         }
                         ub++;
@@ -335,9 +344,21 @@ namespace symx
                 }
             }
             if (use.uses_summation && multi_q) os << in << "}\n";
-            os << "        }\n        else {\n";
-            emit_zero("            ");
-            os << "        }\n    }\n    symx_store_energy(e_out, p.E_partial);\n}\n";
+            if (staged) {
+                // inactive and out-of-range elements keep the zero-initialised rows
+                os << "        }\n    }\n    {\n";
+                os << "        const long long e0 = (long long)symx_block() * 256 + (threadIdx.x & ~63);\n";
+                os << "        const long long rem = (long long)p.n_elem - e0;\n";
+                os << "        const int nv = rem < 0 ? 0 : (rem > 64 ? 64 : (int)rem);\n";
+                os << "        symx_store_rows<" << n << ">(p.G, sg, e0, nv);\n";
+                if (ks.kind == KernelKind::PGH) os << "        symx_store_rows<" << n_ub * 9 << ">(p.H, sh, e0, nv);\n";
+                os << "    }\n    symx_store_energy(e_out, p.E_partial);\n}\n";
+            }
+            else {
+                os << "        }\n        else {\n";
+                emit_zero("            ");
+                os << "        }\n    }\n    symx_store_energy(e_out, p.E_partial);\n}\n";
+            }
         }
         return os.str();
     }

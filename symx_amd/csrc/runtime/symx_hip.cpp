@@ -120,6 +120,38 @@ __device__ __forceinline__ void symx_store_energy(double v, double* E_partial)
     __syncthreads();
     if (threadIdx.x == 0) E_partial[symx_block()] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
 }
+// Element-major rows dst[e * W + i] of the 256 elements of this workgroup, written with
+// coalesced stores: each wave transposes its 64 rows through LDS in chunks of 16 values, so
+// one store instruction writes 64 consecutive doubles of a chunk instead of one double in
+// each of 64 rows. e0: first element of the wave; n_valid: its rows below n_elem. Every
+// thread of the workgroup must call it (it synchronises the workgroup).
+__device__ __forceinline__ double* symx_stage_rows_buffer()
+{
+    __shared__ double stage[4 * 64 * 17];  // one buffer for all row widths of a kernel
+    return stage;
+}
+template <int W>
+__device__ __forceinline__ void symx_store_rows(double* dst, const double (&v)[W], long long e0, int n_valid)
+{
+    const int lane = threadIdx.x & 63;
+    double* st = symx_stage_rows_buffer() + (threadIdx.x >> 6) * (64 * 17);
+#pragma unroll
+    for (int c0 = 0; c0 < W; c0 += 16) {
+        const int cw = (W - c0) < 16 ? (W - c0) : 16;
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (j < cw) st[lane * 17 + j] = v[c0 + j];
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            if (t < cw) {
+                const int idx = t * 64 + lane, r = idx / cw, j = idx - r * cw;
+                if (r < n_valid) dst[(e0 + r) * W + c0 + j] = st[r * 17 + j];
+            }
+        }
+        __syncthreads();
+    }
+}
 // ---- end of prelude ----
 )PRELUDE";
 
