@@ -32,10 +32,13 @@ namespace symx_hip
         int32_t stop_on_indef;
     };
 
-    // Sliced block-CSR (SELL-64 of 3x3 blocks): rows are grouped in slices of 64
-    // block rows; slice s stores width_s block columns for each of its rows, the
-    // diagonal block first. Slot (s, j, lane) = slice_ptr[s] + 64 j + lane; value k
-    // of that slot lives at ((slice_ptr[s] / 64 + j) * 9 + k) * 64 + lane.
+    // Sliced block-CSR (SELL-64 of 3x3 blocks): block rows are placed at positions
+    // grouped in slices of 64; slice s stores width_s block columns for each of its
+    // positions, the diagonal block first. Slot (s, j, lane) = slice_ptr[s] + 64 j +
+    // lane; value k of that slot lives at ((slice_ptr[s] / 64 + j) * 9 + k) * 64 + lane.
+    // Position = row unless sorting the rows by length inside windows of kSellSigma
+    // rows removes enough padding (SELL-C-sigma); then row_at[pos] names the row.
+    constexpr int32_t kSellSigma = 1024;
     struct SellPattern
     {
         int32_t n_rows = 0;
@@ -45,8 +48,12 @@ namespace symx_hip
         std::vector<int64_t> h_slice_ptr; // [n_slices + 1], multiples of 64
         std::vector<int32_t> h_col;       // [n_slots]
         std::vector<int32_t> h_row_len;   // [n_rows]
+        std::vector<int32_t> h_row_at;    // [n_rows] row at each position; empty = identity
+        std::vector<int32_t> h_pos;       // [n_rows] position of each row; empty = identity
         int64_t* d_slice_ptr = nullptr;
         int32_t* d_col = nullptr;
+        int32_t* d_row_at = nullptr;      // null = identity
+        int32_t pos_of(int32_t r) const { return h_pos.empty() ? r : h_pos[r]; }
     };
 
     void set_thread_error(const std::string& msg);

@@ -1054,13 +1054,14 @@ namespace symx_hip
     // Closed-form inverse of the symmetric diagonal block of each row, computed in
     // storage precision like BlockedSparseMatrix.h:1198-1214. Layout: dinv[k*n_rows+row].
     template <typename TS>
-    __global__ __launch_bounds__(NT) void k_dinv(int32_t n_rows, const int64_t* __restrict__ slice_ptr, const TS* __restrict__ val,
-                                                 TS* __restrict__ dinv)
+    __global__ __launch_bounds__(NT) void k_dinv(int32_t n_rows, const int64_t* __restrict__ slice_ptr,
+                                                 const int32_t* __restrict__ row_at, const TS* __restrict__ val, TS* __restrict__ dinv)
     {
-        const int64_t row = xcd_remap() * NT + threadIdx.x;
-        if (row >= n_rows) return;
-        const int64_t base = slice_ptr[row >> 6];
-        const TS* m = val + (base >> 6) * 9 * 64 + (row & 63);  // j = 0 is the diagonal block
+        const int64_t pos = xcd_remap() * NT + threadIdx.x;
+        if (pos >= n_rows) return;
+        const int64_t row = row_at ? row_at[pos] : pos;
+        const int64_t base = slice_ptr[pos >> 6];
+        const TS* m = val + (base >> 6) * 9 * 64 + (pos & 63);  // j = 0 is the diagonal block
         const TS m00 = m[0], m01 = m[64], m02 = m[128], m11 = m[4 * 64], m12 = m[5 * 64], m22 = m[8 * 64];
         const TS c00 = m11 * m22 - m12 * m12;
         const TS c01 = m02 * m12 - m01 * m22;
@@ -1089,8 +1090,8 @@ namespace symx_hip
     void launch_dinv(hipStream_t s, const SellPattern& P, const double* v64, const float* v32, double* dinv64, float* dinv32)
     {
         if (P.n_rows <= 0) return;
-        if (v64 && dinv64) k_dinv<double><<<grid_for(P.n_rows), NT, 0, s>>>(P.n_rows, P.d_slice_ptr, v64, dinv64);
-        if (v32 && dinv32) k_dinv<float><<<grid_for(P.n_rows), NT, 0, s>>>(P.n_rows, P.d_slice_ptr, v32, dinv32);
+        if (v64 && dinv64) k_dinv<double><<<grid_for(P.n_rows), NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, v64, dinv64);
+        if (v32 && dinv32) k_dinv<float><<<grid_for(P.n_rows), NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, v32, dinv32);
     }
 
     // ---- PCG -------------------------------------------------------------------------
@@ -1155,7 +1156,8 @@ namespace symx_hip
 
     template <typename TS, typename TV>
     __global__ __launch_bounds__(NT) void k_pcg_spmv(int32_t n_rows, int32_t k, const int64_t* __restrict__ slice_ptr,
-                                                     const int32_t* __restrict__ col, const TS* __restrict__ val,
+                                                     const int32_t* __restrict__ row_at, const int32_t* __restrict__ col,
+                                                     const TS* __restrict__ val,
                                                      const TV* __restrict__ z, const TV* __restrict__ p_old, TV* __restrict__ p_new,
                                                      TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_prev,
                                                      const double* __restrict__ pa_prev2, double* __restrict__ pb)
@@ -1219,10 +1221,11 @@ namespace symx_hip
         int64_t r0, r1;
         row_chunk(n_rows, w, G, r0, r1);
         double dot = 0.0;
-        for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
-            const int64_t b0 = slice_ptr[row >> 6];
-            const int wd = (int)((slice_ptr[(row >> 6) + 1] - b0) >> 6);
-            const int lane = (int)(row & 63);
+        for (int64_t pos = r0 + threadIdx.x; pos < r1; pos += NT) {
+            const int64_t row = row_at ? row_at[pos] : pos;
+            const int64_t b0 = slice_ptr[pos >> 6];
+            const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
+            const int lane = (int)(pos & 63);
             const int32_t* cp = col + b0 + lane;
             const TS* vp = val + b0 * 9 + lane;
             TV a0 = (TV)0, a1 = (TV)0, a2 = (TV)0;
@@ -1325,7 +1328,7 @@ namespace symx_hip
         const int G = pcg_grid(P.n_rows);
         double* pa[2] = { partials, partials + 2 * G };
         double* pb = partials + 4 * G;
-        k_pcg_spmv<double, double><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_col, val, z, p_old, p_new, q, st, pa[(k - 1) & 1],
+        k_pcg_spmv<double, double><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st, pa[(k - 1) & 1],
                                                     pa[k & 1], pb);
         k_pcg_update<double><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
@@ -1340,7 +1343,7 @@ namespace symx_hip
         const int G = pcg_grid(P.n_rows);
         double* pa[2] = { partials, partials + 2 * G };
         double* pb = partials + 4 * G;
-        k_pcg_spmv<float, float><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_col, val, z, p_old, p_new, q, st, pa[(k - 1) & 1],
+        k_pcg_spmv<float, float><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st, pa[(k - 1) & 1],
                                                   pa[k & 1], pb);
         k_pcg_update<float><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
@@ -1349,7 +1352,8 @@ namespace symx_hip
     // r = b - A x with FP32 matrix values, FP64 vectors and FP64 accumulation (the
     // outer residual of iterative refinement); r32 = (float) r; *rr_out = ||r||^2.
     __global__ __launch_bounds__(NT) void k_residual_mixed(int32_t n_rows, const int64_t* __restrict__ slice_ptr,
-                                                           const int32_t* __restrict__ col, const float* __restrict__ val,
+                                                           const int32_t* __restrict__ row_at, const int32_t* __restrict__ col,
+                                                           const float* __restrict__ val,
                                                            const double* __restrict__ b, const double* __restrict__ x,
                                                            double* __restrict__ r, float* __restrict__ r32, double* __restrict__ partials)
     {
@@ -1358,10 +1362,11 @@ namespace symx_hip
         int64_t r0, r1;
         row_chunk(n_rows, w, gridDim.x, r0, r1);
         double rr = 0.0;
-        for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
-            const int64_t b0 = slice_ptr[row >> 6];
-            const int wd = (int)((slice_ptr[(row >> 6) + 1] - b0) >> 6);
-            const int lane = (int)(row & 63);
+        for (int64_t pos = r0 + threadIdx.x; pos < r1; pos += NT) {
+            const int64_t row = row_at ? row_at[pos] : pos;
+            const int64_t b0 = slice_ptr[pos >> 6];
+            const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
+            const int lane = (int)(pos & 63);
             const int32_t* cp = col + b0 + lane;
             const float* vp = val + b0 * 9 + lane;
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
@@ -1391,7 +1396,7 @@ namespace symx_hip
                                float* r32, double* rr_out, double* partials)
     {
         const int G = pcg_grid(P.n_rows);
-        k_residual_mixed<<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_col, val, b, x, r, r32, partials);
+        k_residual_mixed<<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_col, val, b, x, r, r32, partials);
         k_sum<<<1, NT, 0, s>>>(partials, G, 1, 0, rr_out);
     }
 
