@@ -55,9 +55,18 @@ namespace symx_hip
         r1 = r0 + per < n ? r0 + per : n;
     }
 
-    // Non-temporal load (streamed once; does not displace reused lines from the caches).
-    template <typename T>
-    __device__ __forceinline__ T ntload(const T* p) { return __builtin_nontemporal_load(p); }
+    // Matrix-value load: non-temporal when the matrix is streamed (larger than the
+    // last-level caches, so it is read from HBM anyway and must not displace the
+    // gathered vector entries), ordinary when it stays cache resident across
+    // iterations (small systems, where a non-temporal read would go to HBM each time).
+    template <bool STREAM, typename T>
+    __device__ __forceinline__ T mload(const T* p)
+    {
+        if constexpr (STREAM) return __builtin_nontemporal_load(p);
+        else return *p;
+    }
+    // Matrices above this many value bytes are streamed (the MI355X infinity cache is 256 MiB).
+    constexpr double kStreamBytes = 128.0 * 1024 * 1024;
 
     __device__ inline double wave_sum(double v)
     {
@@ -1154,7 +1163,7 @@ namespace symx_hip
         }
     }
 
-    template <typename TS, typename TV>
+    template <typename TS, typename TV, bool STREAM>
     __global__ __launch_bounds__(NT) void k_pcg_spmv(int32_t n_rows, int32_t k, const int64_t* __restrict__ slice_ptr,
                                                      const int32_t* __restrict__ row_at, const int32_t* __restrict__ col,
                                                      const TS* __restrict__ val,
@@ -1229,8 +1238,9 @@ namespace symx_hip
             const int32_t* cp = col + b0 + lane;
             const TS* vp = val + b0 * 9 + lane;
             TV a0 = (TV)0, a1 = (TV)0, a2 = (TV)0;
-            // matrix values stream through once per iteration: non-temporal loads keep them
-            // from evicting the gathered vector entries (12% faster per PCG iteration)
+            // large matrices stream through once per iteration: non-temporal loads keep them
+            // from evicting the gathered vector entries (12% faster per PCG iteration on a
+            // 6M-tet block; 40% slower on a cache-resident 50k-DoF system, hence STREAM)
 #pragma unroll 4
             for (int j = 0; j < wd; j++) {
                 const int64_t c = 3 * (int64_t)cp[j * 64];
@@ -1238,9 +1248,9 @@ namespace symx_hip
                 const TV x1 = z[c + 1] + tb * p_old[c + 1];
                 const TV x2 = z[c + 2] + tb * p_old[c + 2];
                 const TS* v = vp + (int64_t)j * 576;
-                a0 += (TV)ntload(v + 0) * x0 + (TV)ntload(v + 64) * x1 + (TV)ntload(v + 128) * x2;
-                a1 += (TV)ntload(v + 192) * x0 + (TV)ntload(v + 256) * x1 + (TV)ntload(v + 320) * x2;
-                a2 += (TV)ntload(v + 384) * x0 + (TV)ntload(v + 448) * x1 + (TV)ntload(v + 512) * x2;
+                a0 += (TV)mload<STREAM>(v + 0) * x0 + (TV)mload<STREAM>(v + 64) * x1 + (TV)mload<STREAM>(v + 128) * x2;
+                a1 += (TV)mload<STREAM>(v + 192) * x0 + (TV)mload<STREAM>(v + 256) * x1 + (TV)mload<STREAM>(v + 320) * x2;
+                a2 += (TV)mload<STREAM>(v + 384) * x0 + (TV)mload<STREAM>(v + 448) * x1 + (TV)mload<STREAM>(v + 512) * x2;
             }
             const int64_t kk = 3 * row;
             const TV p0 = z[kk + 0] + tb * p_old[kk + 0];
@@ -1328,8 +1338,12 @@ namespace symx_hip
         const int G = pcg_grid(P.n_rows);
         double* pa[2] = { partials, partials + 2 * G };
         double* pb = partials + 4 * G;
-        k_pcg_spmv<double, double><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st, pa[(k - 1) & 1],
-                                                    pa[k & 1], pb);
+        if ((double)P.n_slots * 9 * sizeof(double) > kStreamBytes)
+            k_pcg_spmv<double, double, true><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st,
+                                                              pa[(k - 1) & 1], pa[k & 1], pb);
+        else
+            k_pcg_spmv<double, double, false><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st,
+                                                               pa[(k - 1) & 1], pa[k & 1], pb);
         k_pcg_update<double><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
     void launch_pcg_init32(hipStream_t s, const SellPattern& P, const float* dinv, const float* b, float* x, float* r, float* z,
@@ -1343,14 +1357,19 @@ namespace symx_hip
         const int G = pcg_grid(P.n_rows);
         double* pa[2] = { partials, partials + 2 * G };
         double* pb = partials + 4 * G;
-        k_pcg_spmv<float, float><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st, pa[(k - 1) & 1],
-                                                  pa[k & 1], pb);
+        if ((double)P.n_slots * 9 * sizeof(float) > kStreamBytes)
+            k_pcg_spmv<float, float, true><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st,
+                                                            pa[(k - 1) & 1], pa[k & 1], pb);
+        else
+            k_pcg_spmv<float, float, false><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st,
+                                                             pa[(k - 1) & 1], pa[k & 1], pb);
         k_pcg_update<float><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
 
     // ---- mixed-precision refinement -----------------------------------------------
     // r = b - A x with FP32 matrix values, FP64 vectors and FP64 accumulation (the
     // outer residual of iterative refinement); r32 = (float) r; *rr_out = ||r||^2.
+    template <bool STREAM>
     __global__ __launch_bounds__(NT) void k_residual_mixed(int32_t n_rows, const int64_t* __restrict__ slice_ptr,
                                                            const int32_t* __restrict__ row_at, const int32_t* __restrict__ col,
                                                            const float* __restrict__ val,
@@ -1375,9 +1394,9 @@ namespace symx_hip
                 const int64_t c = 3 * (int64_t)cp[j * 64];
                 const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
                 const float* v = vp + (int64_t)j * 576;
-                a0 += (double)ntload(v + 0) * x0 + (double)ntload(v + 64) * x1 + (double)ntload(v + 128) * x2;
-                a1 += (double)ntload(v + 192) * x0 + (double)ntload(v + 256) * x1 + (double)ntload(v + 320) * x2;
-                a2 += (double)ntload(v + 384) * x0 + (double)ntload(v + 448) * x1 + (double)ntload(v + 512) * x2;
+                a0 += (double)mload<STREAM>(v + 0) * x0 + (double)mload<STREAM>(v + 64) * x1 + (double)mload<STREAM>(v + 128) * x2;
+                a1 += (double)mload<STREAM>(v + 192) * x0 + (double)mload<STREAM>(v + 256) * x1 + (double)mload<STREAM>(v + 320) * x2;
+                a2 += (double)mload<STREAM>(v + 384) * x0 + (double)mload<STREAM>(v + 448) * x1 + (double)mload<STREAM>(v + 512) * x2;
             }
             const int64_t k = 3 * row;
             const double e0 = b[k] - a0, e1 = b[k + 1] - a1, e2 = b[k + 2] - a2;
@@ -1396,7 +1415,10 @@ namespace symx_hip
                                float* r32, double* rr_out, double* partials)
     {
         const int G = pcg_grid(P.n_rows);
-        k_residual_mixed<<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_col, val, b, x, r, r32, partials);
+        if ((double)P.n_slots * 9 * sizeof(float) > kStreamBytes)
+            k_residual_mixed<true><<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_col, val, b, x, r, r32, partials);
+        else
+            k_residual_mixed<false><<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_col, val, b, x, r, r32, partials);
         k_sum<<<1, NT, 0, s>>>(partials, G, 1, 0, rr_out);
     }
 

@@ -767,7 +767,9 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     if (S.d_row_at) HIP_TRY(ctx, hipMemcpy(S.d_row_at, S.h_row_at.data(), S.h_row_at.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(ctx, hipMemcpy(p->d_coff, soff.data(), soff.size() * 4, hipMemcpyHostToDevice));
     if (!clist.empty()) HIP_TRY(ctx, hipMemcpy(p->d_clist, clist.data(), clist.size() * 4, hipMemcpyHostToDevice));
-    rc = ensure_partials(ctx, (size_t)std::max<int64_t>(2 * (int64_t)reduction_groups(p->n_dofs) + 2 * ((nb + 255) / 256) + 16, 4096));
+    // PCG: 5 partials per workgroup (launch_pcg_iter*); vector reductions: 2 per workgroup + slack
+    rc = ensure_partials(ctx, (size_t)std::max<int64_t>({ 2 * (int64_t)reduction_groups(p->n_dofs) + 2 * ((nb + 255) / 256) + 16,
+                                                          5 * (int64_t)reduction_groups(nb) + 16, 4096 }));
     if (rc) return rc;
     p->finalized = true;
     return SYMX_HIP_OK;
