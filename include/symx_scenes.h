@@ -34,6 +34,8 @@ typedef struct symx_tet4_block_params {
     int32_t fixed_axis;      /* 0,1,2: vertices with coordinate == 0 on this axis are fixed; -1: none */
     int32_t material;        /* 0: Neo-Hookean, 1: stable Neo-Hookean */
     int32_t log_mu_field;    /* 1: per-tet mu = exp(log_mu[e]) parameters (see symx_scene_set_log_mu) */
+    uint64_t renumber_seed;  /* != 0: vertex ids permuted at random (Fisher-Yates, SplitMix64 with this
+                                seed) after the noise is applied: an unstructured numbering */
 } symx_tet4_block_params;
 
 /* Ball dropped on the ground plane z = 0 (contact scene): Tet4 jittered-grid

@@ -37,7 +37,8 @@ class PcgInfo(Structure):
 class Tet4BlockParams(Structure):
     _fields_ = [("cells", c_int32 * 3), ("size", c_double * 3), ("youngs_modulus", c_double), ("poissons_ratio", c_double),
                 ("density", c_double), ("gravity", c_double * 3), ("dt", c_double), ("noise", c_double), ("seed", c_uint64),
-                ("bc_stiffness", c_double), ("fixed_axis", c_int32), ("material", c_int32), ("log_mu_field", c_int32)]
+                ("bc_stiffness", c_double), ("fixed_axis", c_int32), ("material", c_int32), ("log_mu_field", c_int32),
+                ("renumber_seed", c_uint64)]
 
 
 class SphereParams(Structure):

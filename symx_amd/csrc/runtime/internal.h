@@ -51,7 +51,8 @@ namespace symx_hip
         std::vector<int32_t> h_row_at;    // [n_rows] row at each position; empty = identity
         std::vector<int32_t> h_pos;       // [n_rows] position of each row; empty = identity
         int64_t* d_slice_ptr = nullptr;
-        int32_t* d_col = nullptr;
+        int32_t* d_col = nullptr;         // column ids, or
+        int16_t* d_dcol = nullptr;        // column - row per slot when every offset fits 16 bits
         int32_t* d_row_at = nullptr;      // null = identity
         int32_t pos_of(int32_t r) const { return h_pos.empty() ? r : h_pos[r]; }
     };

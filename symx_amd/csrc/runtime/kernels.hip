@@ -65,6 +65,14 @@ namespace symx_hip
         if constexpr (STREAM) return __builtin_nontemporal_load(p);
         else return *p;
     }
+    // Block column of a SELL slot: int32 ids, or int16 offsets from the slot's row (half
+    // the index bytes; used when every offset fits).
+    template <typename CT>
+    __device__ __forceinline__ int64_t block_col(CT stored, int64_t row)
+    {
+        if constexpr (sizeof(CT) == 2) return row + (int64_t)stored;
+        else return (int64_t)stored;
+    }
     // Matrices above this many value bytes are streamed (the MI355X infinity cache is 256 MiB).
     constexpr double kStreamBytes = 128.0 * 1024 * 1024;
 
@@ -1163,9 +1171,9 @@ namespace symx_hip
         }
     }
 
-    template <typename TS, typename TV, bool STREAM>
+    template <typename TS, typename TV, bool STREAM, typename CT>
     __global__ __launch_bounds__(NT) void k_pcg_spmv(int32_t n_rows, int32_t k, const int64_t* __restrict__ slice_ptr,
-                                                     const int32_t* __restrict__ row_at, const int32_t* __restrict__ col,
+                                                     const int32_t* __restrict__ row_at, const CT* __restrict__ col,
                                                      const TS* __restrict__ val,
                                                      const TV* __restrict__ z, const TV* __restrict__ p_old, TV* __restrict__ p_new,
                                                      TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_prev,
@@ -1235,7 +1243,7 @@ namespace symx_hip
             const int64_t b0 = slice_ptr[pos >> 6];
             const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
             const int lane = (int)(pos & 63);
-            const int32_t* cp = col + b0 + lane;
+            const CT* cp = col + b0 + lane;
             const TS* vp = val + b0 * 9 + lane;
             TV a0 = (TV)0, a1 = (TV)0, a2 = (TV)0;
             // large matrices stream through once per iteration: non-temporal loads keep them
@@ -1243,7 +1251,7 @@ namespace symx_hip
             // 6M-tet block; 40% slower on a cache-resident 50k-DoF system, hence STREAM)
 #pragma unroll 4
             for (int j = 0; j < wd; j++) {
-                const int64_t c = 3 * (int64_t)cp[j * 64];
+                const int64_t c = 3 * block_col(cp[j * 64], row);
                 const TV x0 = z[c + 0] + tb * p_old[c + 0];
                 const TV x1 = z[c + 1] + tb * p_old[c + 1];
                 const TV x2 = z[c + 2] + tb * p_old[c + 2];
@@ -1327,6 +1335,30 @@ namespace symx_hip
 
     int pcg_grid(int32_t n_rows) { return reduction_groups(n_rows); }
 
+    // Instantiation by matrix size (streaming loads) and column-id width.
+    template <typename TS, typename TV>
+    void launch_spmv(hipStream_t s, const SellPattern& P, int G, int32_t k, const TS* val, const TV* z, const TV* p_old, TV* p_new,
+                     TV* q, PcgState* st, const double* pa_prev, const double* pa_prev2, double* pb)
+    {
+        const bool stream = (double)P.n_slots * 9 * sizeof(TS) > kStreamBytes;
+        if (P.d_dcol) {
+            if (stream)
+                k_pcg_spmv<TS, TV, true, int16_t><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, z, p_old, p_new, q,
+                                                                   st, pa_prev, pa_prev2, pb);
+            else
+                k_pcg_spmv<TS, TV, false, int16_t><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, z, p_old, p_new,
+                                                                    q, st, pa_prev, pa_prev2, pb);
+        }
+        else {
+            if (stream)
+                k_pcg_spmv<TS, TV, true, int32_t><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q,
+                                                                   st, pa_prev, pa_prev2, pb);
+            else
+                k_pcg_spmv<TS, TV, false, int32_t><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new,
+                                                                    q, st, pa_prev, pa_prev2, pb);
+        }
+    }
+
     void launch_pcg_init64(hipStream_t s, const SellPattern& P, const double* dinv, const double* b, double* x, double* r, double* z,
                            double* p0, double* partials)
     {
@@ -1338,12 +1370,7 @@ namespace symx_hip
         const int G = pcg_grid(P.n_rows);
         double* pa[2] = { partials, partials + 2 * G };
         double* pb = partials + 4 * G;
-        if ((double)P.n_slots * 9 * sizeof(double) > kStreamBytes)
-            k_pcg_spmv<double, double, true><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st,
-                                                              pa[(k - 1) & 1], pa[k & 1], pb);
-        else
-            k_pcg_spmv<double, double, false><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st,
-                                                               pa[(k - 1) & 1], pa[k & 1], pb);
+        launch_spmv<double, double>(s, P, G, k, val, z, p_old, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
         k_pcg_update<double><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
     void launch_pcg_init32(hipStream_t s, const SellPattern& P, const float* dinv, const float* b, float* x, float* r, float* z,
@@ -1357,21 +1384,16 @@ namespace symx_hip
         const int G = pcg_grid(P.n_rows);
         double* pa[2] = { partials, partials + 2 * G };
         double* pb = partials + 4 * G;
-        if ((double)P.n_slots * 9 * sizeof(float) > kStreamBytes)
-            k_pcg_spmv<float, float, true><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st,
-                                                            pa[(k - 1) & 1], pa[k & 1], pb);
-        else
-            k_pcg_spmv<float, float, false><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q, st,
-                                                             pa[(k - 1) & 1], pa[k & 1], pb);
+        launch_spmv<float, float>(s, P, G, k, val, z, p_old, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
         k_pcg_update<float><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
 
     // ---- mixed-precision refinement -----------------------------------------------
     // r = b - A x with FP32 matrix values, FP64 vectors and FP64 accumulation (the
     // outer residual of iterative refinement); r32 = (float) r; *rr_out = ||r||^2.
-    template <bool STREAM>
+    template <bool STREAM, typename CT>
     __global__ __launch_bounds__(NT) void k_residual_mixed(int32_t n_rows, const int64_t* __restrict__ slice_ptr,
-                                                           const int32_t* __restrict__ row_at, const int32_t* __restrict__ col,
+                                                           const int32_t* __restrict__ row_at, const CT* __restrict__ col,
                                                            const float* __restrict__ val,
                                                            const double* __restrict__ b, const double* __restrict__ x,
                                                            double* __restrict__ r, float* __restrict__ r32, double* __restrict__ partials)
@@ -1386,12 +1408,12 @@ namespace symx_hip
             const int64_t b0 = slice_ptr[pos >> 6];
             const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
             const int lane = (int)(pos & 63);
-            const int32_t* cp = col + b0 + lane;
+            const CT* cp = col + b0 + lane;
             const float* vp = val + b0 * 9 + lane;
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll 4
             for (int j = 0; j < wd; j++) {
-                const int64_t c = 3 * (int64_t)cp[j * 64];
+                const int64_t c = 3 * block_col(cp[j * 64], row);
                 const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
                 const float* v = vp + (int64_t)j * 576;
                 a0 += (double)mload<STREAM>(v + 0) * x0 + (double)mload<STREAM>(v + 64) * x1 + (double)mload<STREAM>(v + 128) * x2;
@@ -1415,10 +1437,15 @@ namespace symx_hip
                                float* r32, double* rr_out, double* partials)
     {
         const int G = pcg_grid(P.n_rows);
-        if ((double)P.n_slots * 9 * sizeof(float) > kStreamBytes)
-            k_residual_mixed<true><<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_col, val, b, x, r, r32, partials);
-        else
-            k_residual_mixed<false><<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_col, val, b, x, r, r32, partials);
+        const bool stream = (double)P.n_slots * 9 * sizeof(float) > kStreamBytes;
+        if (P.d_dcol) {
+            if (stream) k_residual_mixed<true, int16_t><<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, b, x, r, r32, partials);
+            else k_residual_mixed<false, int16_t><<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, b, x, r, r32, partials);
+        }
+        else {
+            if (stream) k_residual_mixed<true, int32_t><<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_col, val, b, x, r, r32, partials);
+            else k_residual_mixed<false, int32_t><<<G, NT, 0, s>>>(P.n_rows, P.d_slice_ptr, P.d_row_at, P.d_col, val, b, x, r, r32, partials);
+        }
         k_sum<<<1, NT, 0, s>>>(partials, G, 1, 0, rr_out);
     }
 

@@ -278,6 +278,24 @@ static void build_scene(symx_scene_s* s, const symx_tet4_block_params* p, const 
             if (!fixed[i]) s->x[i][d] += p->noise * (2.0 * u - 1.0);
         }
     }
+    if (p->renumber_seed != 0) {
+        // vertex i becomes perm[i]
+        std::vector<int32_t> perm(nv);
+        for (int32_t i = 0; i < nv; i++) perm[i] = i;
+        SplitMix64 shuffle(p->renumber_seed);
+        for (int32_t i = nv - 1; i > 0; i--) std::swap(perm[i], perm[(int32_t)(shuffle.next() % (uint64_t)(i + 1))]);
+        std::vector<std::array<double, 3>> X(nv), x(nv);
+        std::vector<uint8_t> f(nv);
+        for (int32_t i = 0; i < nv; i++) {
+            X[perm[i]] = s->X[i];
+            x[perm[i]] = s->x[i];
+            f[perm[i]] = fixed[i];
+        }
+        s->X = std::move(X);
+        s->x = std::move(x);
+        fixed = std::move(f);
+        for (auto& t : s->tets) t = perm[t];
+    }
     s->x0 = s->x;
     s->v.assign(nv, { 0.0, 0.0, 0.0 });
     for (int32_t i = 0; i < nv; i++) {
@@ -546,6 +564,7 @@ void symx_tet4_block_params_default(symx_tet4_block_params* p)
     p->fixed_axis = 2;
     p->material = 0;
     p->log_mu_field = 0;
+    p->renumber_seed = 0;
 }
 
 void symx_sphere_params_default(symx_sphere_params* p)

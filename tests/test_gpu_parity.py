@@ -294,3 +294,28 @@ def test_time_stepping_updates_state(gpu):
     v = s.get("v")
     assert np.allclose(v, (s.get("x") - x0) / s.params.dt)
     assert np.array_equal(s.get("x0"), s.get("x"))
+
+
+@pytest.mark.parametrize("precision", [symx_amd.FP64, symx_amd.MIXED])
+def test_unstructured_numbering_gives_the_same_step(gpu, precision):
+    """The 32^3-cell block (35,937 vertices) with its vertices renumbered at random: column
+    offsets no longer fit 16 bits (int32 column ids) and rows of different length
+    interleave (SELL rows sorted by length inside windows). One implicit step must match
+    the structured numbering's vertex for vertex, to the Newton tolerance."""
+    out = []
+    for renumber in (0, 5):
+        prm = symx_amd.default_block_params(cells=[32, 32, 32], noise=0.2 / 32, seed=3, renumber_seed=renumber)
+        s = symx_amd.Tet4Block(prm, symx_amd.default_newton_params(precision=precision))
+        x_start = s.get("x")
+        st = s.step()
+        X, x = s.get("X"), s.get("x")
+        order = np.lexsort((X[:, 2], X[:, 1], X[:, 0]))  # grid order, independent of numbering
+        out.append((st, x[order], x_start[order]))
+        s.close()
+    (st0, x0, xs0), (st1, x1, xs1) = out
+    assert np.array_equal(xs0, xs1)  # same geometry after the renumbering
+    assert st0["result_name"] == st1["result_name"] == "Successful"
+    assert abs(st0["newton_iterations"] - st1["newton_iterations"]) <= 1
+    moved = np.abs(x0 - xs0).max()
+    assert moved > 0
+    assert np.abs(x1 - x0).max() <= 1e-5 * moved, (np.abs(x1 - x0).max(), moved)
