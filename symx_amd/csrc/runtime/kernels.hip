@@ -73,6 +73,10 @@ namespace symx_hip
         if constexpr (sizeof(CT) == 2) return row + (int64_t)stored;
         else return (int64_t)stored;
     }
+    // SpMV and PCG-update workgroup size. The grid is the fixed set of at most MAX_GROUPS reduction
+    // workgroups; 512 threads keep 8 waves per SIMD in flight (the kernel needs ~56
+    // VGPRs). A streaming read reaches 7.3 TB/s with 8 waves per SIMD and 5.5 TB/s with 4.
+    constexpr int kSpmvThreads = 512;
     // Matrices above this many value bytes are streamed (the MI355X infinity cache is 256 MiB).
     constexpr double kStreamBytes = 128.0 * 1024 * 1024;
 
@@ -91,7 +95,8 @@ namespace symx_hip
         return v;
     }
 
-    // Workgroup sum in a fixed order, returned to every thread.
+    // Workgroup sum in a fixed order, returned to every thread (NTS threads, lds[NTS / 64]).
+    template <int NTS = NT>
     __device__ inline double block_sum(double v, double* lds)
     {
         v = wave_sum(v);
@@ -100,7 +105,7 @@ namespace symx_hip
         if ((threadIdx.x & 63) == 0) lds[w] = v;
         __syncthreads();
         double r = 0.0;
-        for (int i = 0; i < NT / 64; i++) r += lds[i];
+        for (int i = 0; i < NTS / 64; i++) r += lds[i];
         return r;
     }
     __device__ inline double block_max(double v, double* lds)
@@ -115,12 +120,16 @@ namespace symx_hip
         return r;
     }
 
-    // Fixed-order sum of G partials p[i*stride + off], returned to every thread.
+    // Fixed-order sum of G partials p[i*stride + off], returned to every thread. The first
+    // NT threads accumulate and further waves add exact zeros, so a workgroup of any
+    // NTS >= NT threads forms the same value bit for bit as an NT-thread one.
+    template <int NTS = NT>
     __device__ inline double sum_partials(const double* __restrict__ p, int64_t G, int stride, int off, double* lds)
     {
         double acc = 0.0;
-        for (int64_t i = threadIdx.x; i < G; i += NT) acc += p[i * stride + off];
-        return block_sum(acc, lds);
+        if (threadIdx.x < NT)
+            for (int64_t i = threadIdx.x; i < G; i += NT) acc += p[i * stride + off];
+        return block_sum<NTS>(acc, lds);
     }
 
     // ---- plain sums ----------------------------------------------------------------
@@ -1055,9 +1064,9 @@ namespace symx_hip
                 for (int k2 = 0; k2 < 9; k2++) acc[k2] += h[k2];
             }
         }
-        TS* dst = val + ((t >> 6) * 9) * 64 + (t & 63);
+        TS* dst = val + (t >> 6) * 576 + 3 * (t & 63);
 #pragma unroll
-        for (int k = 0; k < 9; k++) dst[k * 64] = (TS)acc[k];
+        for (int k = 0; k < 9; k++) dst[192 * (k / 3) + k % 3] = (TS)acc[k];
     }
 
     void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* coff, const uint32_t* clist, const double* H,
@@ -1078,8 +1087,8 @@ namespace symx_hip
         if (pos >= n_rows) return;
         const int64_t row = row_at ? row_at[pos] : pos;
         const int64_t base = slice_ptr[pos >> 6];
-        const TS* m = val + (base >> 6) * 9 * 64 + (pos & 63);  // j = 0 is the diagonal block
-        const TS m00 = m[0], m01 = m[64], m02 = m[128], m11 = m[4 * 64], m12 = m[5 * 64], m22 = m[8 * 64];
+        const TS* m = val + (base >> 6) * 576 + 3 * (pos & 63);  // j = 0 is the diagonal block
+        const TS m00 = m[0], m01 = m[1], m02 = m[2], m11 = m[192 + 1], m12 = m[192 + 2], m22 = m[384 + 2];
         const TS c00 = m11 * m22 - m12 * m12;
         const TS c01 = m02 * m12 - m01 * m22;
         const TS c02 = m01 * m12 - m11 * m02;
@@ -1171,19 +1180,19 @@ namespace symx_hip
         }
     }
 
-    template <typename TS, typename TV, bool STREAM, typename CT>
-    __global__ __launch_bounds__(NT) void k_pcg_spmv(int32_t n_rows, int32_t k, const int64_t* __restrict__ slice_ptr,
+    template <typename TS, typename TV, bool STREAM, typename CT, int NTS>
+    __global__ __launch_bounds__(NTS) void k_pcg_spmv(int32_t n_rows, int32_t k, const int64_t* __restrict__ slice_ptr,
                                                      const int32_t* __restrict__ row_at, const CT* __restrict__ col,
                                                      const TS* __restrict__ val,
                                                      const TV* __restrict__ z, const TV* __restrict__ p_old, TV* __restrict__ p_new,
                                                      TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_prev,
                                                      const double* __restrict__ pa_prev2, double* __restrict__ pb)
     {
-        __shared__ double lds[NT / 64];
+        __shared__ double lds[NTS / 64];
         if (st->done) return;
         const int64_t G = gridDim.x;
-        const double rr = sum_partials(pa_prev, G, 2, 0, lds);
-        const double rz = sum_partials(pa_prev, G, 2, 1, lds);
+        const double rr = sum_partials<NTS>(pa_prev, G, 2, 0, lds);
+        const double rz = sum_partials<NTS>(pa_prev, G, 2, 1, lds);
         double beta = 0.0;
         int done = 0, converged = 0, it = k - 1;
         double err = 0.0;
@@ -1219,7 +1228,7 @@ namespace symx_hip
                 done = 1;
             }
             else {
-                beta = rz / sum_partials(pa_prev2, G, 2, 1, lds);
+                beta = rz / sum_partials<NTS>(pa_prev2, G, 2, 1, lds);
             }
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1238,13 +1247,13 @@ namespace symx_hip
         int64_t r0, r1;
         row_chunk(n_rows, w, G, r0, r1);
         double dot = 0.0;
-        for (int64_t pos = r0 + threadIdx.x; pos < r1; pos += NT) {
+        for (int64_t pos = r0 + threadIdx.x; pos < r1; pos += NTS) {
             const int64_t row = row_at ? row_at[pos] : pos;
             const int64_t b0 = slice_ptr[pos >> 6];
             const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
             const int lane = (int)(pos & 63);
             const CT* cp = col + b0 + lane;
-            const TS* vp = val + b0 * 9 + lane;
+            const TS* vp = val + b0 * 9 + 3 * lane;
             TV a0 = (TV)0, a1 = (TV)0, a2 = (TV)0;
             // large matrices stream through once per iteration: non-temporal loads keep them
             // from evicting the gathered vector entries (12% faster per PCG iteration on a
@@ -1256,9 +1265,9 @@ namespace symx_hip
                 const TV x1 = z[c + 1] + tb * p_old[c + 1];
                 const TV x2 = z[c + 2] + tb * p_old[c + 2];
                 const TS* v = vp + (int64_t)j * 576;
-                a0 += (TV)mload<STREAM>(v + 0) * x0 + (TV)mload<STREAM>(v + 64) * x1 + (TV)mload<STREAM>(v + 128) * x2;
-                a1 += (TV)mload<STREAM>(v + 192) * x0 + (TV)mload<STREAM>(v + 256) * x1 + (TV)mload<STREAM>(v + 320) * x2;
-                a2 += (TV)mload<STREAM>(v + 384) * x0 + (TV)mload<STREAM>(v + 448) * x1 + (TV)mload<STREAM>(v + 512) * x2;
+                a0 += (TV)mload<STREAM>(v + 0) * x0 + (TV)mload<STREAM>(v + 1) * x1 + (TV)mload<STREAM>(v + 2) * x2;
+                a1 += (TV)mload<STREAM>(v + 192) * x0 + (TV)mload<STREAM>(v + 193) * x1 + (TV)mload<STREAM>(v + 194) * x2;
+                a2 += (TV)mload<STREAM>(v + 384) * x0 + (TV)mload<STREAM>(v + 385) * x1 + (TV)mload<STREAM>(v + 386) * x2;
             }
             const int64_t kk = 3 * row;
             const TV p0 = z[kk + 0] + tb * p_old[kk + 0];
@@ -1272,21 +1281,21 @@ namespace symx_hip
             q[kk + 2] = a2;
             dot += (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
         }
-        const double s = block_sum(dot, lds);
+        const double s = block_sum<NTS>(dot, lds);
         if (threadIdx.x == 0) pb[w] = s;
     }
 
-    template <typename TV>
-    __global__ __launch_bounds__(NT) void k_pcg_update(int32_t n_rows, int32_t k, const TV* __restrict__ dinv, TV* __restrict__ x,
+    template <typename TV, int NTS>
+    __global__ __launch_bounds__(NTS) void k_pcg_update(int32_t n_rows, int32_t k, const TV* __restrict__ dinv, TV* __restrict__ x,
                                                        TV* __restrict__ r, TV* __restrict__ z, const TV* __restrict__ p,
                                                        const TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_cur,
                                                        double* __restrict__ pa_next, const double* __restrict__ pb)
     {
-        __shared__ double lds[NT / 64];
+        __shared__ double lds[NTS / 64];
         if (st->done) return;
         const int64_t G = gridDim.x;
-        const double pAp = sum_partials(pb, G, 1, 0, lds);
-        const double rz = sum_partials(pa_cur, G, 2, 1, lds);
+        const double pAp = sum_partials<NTS>(pb, G, 1, 0, lds);
+        const double rz = sum_partials<NTS>(pa_cur, G, 2, 1, lds);
         const double alpha = rz / pAp;
         if (pAp <= 0.0) {
             if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1308,7 +1317,7 @@ namespace symx_hip
         int64_t r0, r1;
         row_chunk(n_rows, w, G, r0, r1);
         double rr = 0.0, rzn = 0.0;
-        for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += NTS) {
             const int64_t kk = 3 * row;
             TV rv[3], zv[3];
 #pragma unroll
@@ -1325,8 +1334,8 @@ namespace symx_hip
                 rzn += (double)rv[i] * (double)zv[i];
             }
         }
-        const double srr = block_sum(rr, lds);
-        const double srz = block_sum(rzn, lds);
+        const double srr = block_sum<NTS>(rr, lds);
+        const double srz = block_sum<NTS>(rzn, lds);
         if (threadIdx.x == 0) {
             pa_next[2 * w + 0] = srr;
             pa_next[2 * w + 1] = srz;
@@ -1343,18 +1352,18 @@ namespace symx_hip
         const bool stream = (double)P.n_slots * 9 * sizeof(TS) > kStreamBytes;
         if (P.d_dcol) {
             if (stream)
-                k_pcg_spmv<TS, TV, true, int16_t><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, z, p_old, p_new, q,
+                k_pcg_spmv<TS, TV, true, int16_t, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, z, p_old, p_new, q,
                                                                    st, pa_prev, pa_prev2, pb);
             else
-                k_pcg_spmv<TS, TV, false, int16_t><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, z, p_old, p_new,
+                k_pcg_spmv<TS, TV, false, int16_t, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, z, p_old, p_new,
                                                                     q, st, pa_prev, pa_prev2, pb);
         }
         else {
             if (stream)
-                k_pcg_spmv<TS, TV, true, int32_t><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q,
+                k_pcg_spmv<TS, TV, true, int32_t, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q,
                                                                    st, pa_prev, pa_prev2, pb);
             else
-                k_pcg_spmv<TS, TV, false, int32_t><<<G, NT, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new,
+                k_pcg_spmv<TS, TV, false, int32_t, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new,
                                                                     q, st, pa_prev, pa_prev2, pb);
         }
     }
@@ -1371,7 +1380,7 @@ namespace symx_hip
         double* pa[2] = { partials, partials + 2 * G };
         double* pb = partials + 4 * G;
         launch_spmv<double, double>(s, P, G, k, val, z, p_old, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
-        k_pcg_update<double><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
+        k_pcg_update<double, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
     void launch_pcg_init32(hipStream_t s, const SellPattern& P, const float* dinv, const float* b, float* x, float* r, float* z,
                            float* p0, double* partials)
@@ -1385,7 +1394,7 @@ namespace symx_hip
         double* pa[2] = { partials, partials + 2 * G };
         double* pb = partials + 4 * G;
         launch_spmv<float, float>(s, P, G, k, val, z, p_old, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
-        k_pcg_update<float><<<G, NT, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
+        k_pcg_update<float, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, dinv, x, r, z, p_new, q, st, pa[(k - 1) & 1], pa[k & 1], pb);
     }
 
     // ---- mixed-precision refinement -----------------------------------------------
@@ -1409,16 +1418,16 @@ namespace symx_hip
             const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
             const int lane = (int)(pos & 63);
             const CT* cp = col + b0 + lane;
-            const float* vp = val + b0 * 9 + lane;
+            const float* vp = val + b0 * 9 + 3 * lane;
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll 4
             for (int j = 0; j < wd; j++) {
                 const int64_t c = 3 * block_col(cp[j * 64], row);
                 const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
                 const float* v = vp + (int64_t)j * 576;
-                a0 += (double)mload<STREAM>(v + 0) * x0 + (double)mload<STREAM>(v + 64) * x1 + (double)mload<STREAM>(v + 128) * x2;
-                a1 += (double)mload<STREAM>(v + 192) * x0 + (double)mload<STREAM>(v + 256) * x1 + (double)mload<STREAM>(v + 320) * x2;
-                a2 += (double)mload<STREAM>(v + 384) * x0 + (double)mload<STREAM>(v + 448) * x1 + (double)mload<STREAM>(v + 512) * x2;
+                a0 += (double)mload<STREAM>(v + 0) * x0 + (double)mload<STREAM>(v + 1) * x1 + (double)mload<STREAM>(v + 2) * x2;
+                a1 += (double)mload<STREAM>(v + 192) * x0 + (double)mload<STREAM>(v + 193) * x1 + (double)mload<STREAM>(v + 194) * x2;
+                a2 += (double)mload<STREAM>(v + 384) * x0 + (double)mload<STREAM>(v + 385) * x1 + (double)mload<STREAM>(v + 386) * x2;
             }
             const int64_t k = 3 * row;
             const double e0 = b[k] - a0, e1 = b[k + 1] - a1, e2 = b[k + 2] - a2;
