@@ -70,7 +70,7 @@ struct symx_hip_ctx_s
     // Reduction scratch: per-workgroup partials and the last-block ticket.
     double* d_partials = nullptr;
     size_t partials_cap = 0;
-    unsigned int* d_ticket = nullptr;
+    unsigned int* d_counters = nullptr;  // small device counters (length of the projection fallback list)
     double* d_scalars = nullptr;   // small device scalar area
     double* h_pinned = nullptr;    // pinned host scalars (64 doubles)
     symx_hip::PcgState* d_pcg = nullptr;
@@ -133,6 +133,7 @@ struct symx_hip_problem_s
     double* d_H = nullptr;
     double* d_Epart = nullptr;
     uint8_t* d_flags = nullptr;   // per-element scratch of the projection
+    uint32_t* d_plist = nullptr;  // per-element scratch: elements left to the fallback projection
     bool h_valid = false;
     int64_t total_elem = 0;
     // progressive projection: elements projected since the last EVAL_PGH, and the
@@ -191,9 +192,9 @@ namespace symx_hip
     void launch_plane_max_step(hipStream_t s, const int32_t* blocks, int32_t n, const double* x, const double* du, int32_t axis, double plane,
                                double eta, unsigned long long* out);
     // select (may be NULL): only elements with select[e] != 0 are projected.
+    // flags, list: per-element scratch (n_mats entries); n_list: one device counter.
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
-                          const uint8_t* select,
-                          unsigned long long* d_count);
+                          uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count);
     int launch_project_dense(hipStream_t s, double* mats, int64_t n_mats, int32_t n, double eps, int32_t mirror, uint8_t* changed);
     void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* coff, const uint32_t* clist, const double* H,
                              double* v64, float* v32);

@@ -524,10 +524,10 @@ namespace symx_hip
         c1 = n1;
     }
 
+    // One matrix h (element e) with the whole workgroup; every early return is uniform.
     template <int N, bool DENSE>
-    __global__ __launch_bounds__(PNT) void k_project_large(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
-                                                          unsigned long long* count, const uint8_t* __restrict__ select,
-                                                          uint8_t* __restrict__ changed_out)
+    __device__ void project_large_one(double* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
+                                      uint8_t* __restrict__ changed_out)
     {
         constexpr int NB = N / 3, LD = N | 1, NUB = NB * (NB + 1) / 2, NW = PNT / 64, NPT = PNT * PPTS;
         static_assert(N <= 128, "one wave holds a vector in two registers per lane");
@@ -538,11 +538,7 @@ namespace symx_hip
         __shared__ int cbuf[NPT];
         __shared__ double red[2 * NW];
         __shared__ int busy[3];
-        const int64_t e = blockIdx.x;
-        if (e >= n_mats) return;
-        if (select != nullptr && !select[e]) return;
         const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-        double* h = DENSE ? H + e * (int64_t)(N * N) : H + e * (int64_t)(NUB * 9);
         auto ubi = [](int a, int b) { return a * NB - (a * (a - 1)) / 2 + (b - a); };
         // element (r, c) of the upper-block storage, r <= c blockwise
         auto ub_index = [&](int t, int& r, int& c) {
@@ -888,8 +884,46 @@ namespace symx_hip
         }
     }
 
+    template <int N, bool DENSE>
+    __global__ __launch_bounds__(PNT) void k_project_large(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
+                                                          unsigned long long* count, const uint8_t* __restrict__ select,
+                                                          uint8_t* __restrict__ changed_out)
+    {
+        constexpr int NUB = (N / 3) * (N / 3 + 1) / 2;
+        const int64_t e = blockIdx.x;
+        if (e >= n_mats) return;
+        if (select != nullptr && !select[e]) return;
+        project_large_one<N, DENSE>(DENSE ? H + e * (int64_t)(N * N) : H + e * (int64_t)(NUB * 9), e, eps, mirror, count, changed_out);
+    }
+
+    // The elements listed in list[0 .. *n_list) (upper-block storage), a fixed grid
+    // striding over the list.
+    template <int N>
+    __global__ __launch_bounds__(PNT) void k_project_large_list(double* __restrict__ H, const uint32_t* __restrict__ list,
+                                                               const unsigned int* __restrict__ n_list, double eps, int mirror,
+                                                               unsigned long long* count)
+    {
+        constexpr int NUB = (N / 3) * (N / 3 + 1) / 2;
+        const unsigned int n = *n_list;
+        for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
+            const int64_t e = list[i];
+            project_large_one<N, false>(H + e * (int64_t)(NUB * 9), e, eps, mirror, count, nullptr);
+            __syncthreads();  // LDS is reused by the next matrix
+        }
+    }
+
+    // list[] = indices e with flags[e] != 0, in no particular order; *n_list counts them
+    // (zeroed by the caller's k_zero_counter on the same stream).
+    __global__ __launch_bounds__(NT) void k_list_flagged(const uint8_t* __restrict__ flags, int64_t n, uint32_t* __restrict__ list,
+                                                         unsigned int* __restrict__ n_list)
+    {
+        const int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (e < n && flags[e]) list[atomicAdd(n_list, 1u)] = (uint32_t)e;
+    }
+    __global__ void k_zero_counter(unsigned int* c) { *c = 0u; }
+
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
-                          const uint8_t* select, unsigned long long* d_count)
+                          uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count)
     {
         if (n_mats <= 0) return 0;
         const unsigned g = (unsigned)((n_mats + 63) / 64);
@@ -898,8 +932,13 @@ namespace symx_hip
         case 2: k_project_ub_full<2><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
         case 3: k_project_ub_full<3><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
         case 4:
+            // reduced 9x9 path in registers; the few elements it hands back (more than three
+            // eigenvalues below eps, or not translation invariant) are listed and projected by
+            // the workgroup-per-matrix kernel
             k_project_ub4_reduced<<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags, select);
-            k_project_ub_full<4><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags);
+            k_zero_counter<<<1, 1, 0, s>>>(n_list);
+            k_list_flagged<<<grid_for(n_mats), NT, 0, s>>>(flags, n_mats, list, n_list);
+            k_project_large_list<12><<<1024, PNT, 0, s>>>(H, list, n_list, eps, mirror, d_count);
             return 0;
         case 8: k_project_large<24, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
         case 10: k_project_large<30, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;

@@ -276,7 +276,7 @@ int symx_hip_ctx_create(int device, symx_hip_ctx* out)
     ctx->device = device;
     HIP_TRY(ctx, hipSetDevice(device));
     HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    int rc = dalloc(ctx, &ctx->d_ticket, 4);
+    int rc = dalloc(ctx, &ctx->d_counters, 4);
     if (!rc) rc = dalloc(ctx, &ctx->d_scalars, 64);
     if (!rc) rc = dalloc(ctx, &ctx->d_pcg, 1);
     if (!rc) rc = ensure_partials(ctx, 16384);
@@ -284,7 +284,7 @@ int symx_hip_ctx_create(int device, symx_hip_ctx* out)
         symx_hip_ctx_destroy(ctx);
         return rc;
     }
-    HIP_TRY(ctx, hipMemset(ctx->d_ticket, 0, 4 * sizeof(unsigned int)));
+    HIP_TRY(ctx, hipMemset(ctx->d_counters, 0, 4 * sizeof(unsigned int)));
     HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
     HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pcg, sizeof(PcgState), hipHostMallocDefault));
     *out = ctx;
@@ -297,7 +297,7 @@ void symx_hip_ctx_destroy(symx_hip_ctx ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     dfree(ctx->d_partials);
-    dfree(ctx->d_ticket);
+    dfree(ctx->d_counters);
     dfree(ctx->d_scalars);
     dfree(ctx->d_pcg);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
@@ -431,6 +431,7 @@ void symx_hip_problem_destroy(symx_hip_problem p)
     dfree(p->d_H);
     dfree(p->d_Epart);
     dfree(p->d_flags);
+    dfree(p->d_plist);
     dfree(p->d_projected);
     dfree(p->d_select);
     dfree(p->d_filter_blocks);
@@ -745,6 +746,7 @@ int symx_hip_problem_finalize(symx_hip_problem p)
         int64_t max_elem = 1;
         for (auto& pot : p->pots) max_elem = std::max<int64_t>(max_elem, pot.n_elem);
         rc = rc ? rc : dalloc(ctx, &p->d_flags, (size_t)max_elem);
+        rc = rc ? rc : dalloc(ctx, &p->d_plist, (size_t)max_elem);
     }
     rc = rc ? rc : dalloc(ctx, &p->d_goff, goff.size());
     rc = rc ? rc : dalloc(ctx, &p->d_glist, std::max<size_t>(glist.size(), 1));
@@ -909,7 +911,7 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
     for (auto& pot : p->pots) {
         // elements of a conditional potential that are inactive have no Hessian to project
         const uint8_t* only = pot.kC.empty() ? nullptr : pot.d_active;
-        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, only, d_count))
+        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters, only, d_count))
             return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports 1-4, 8, 10 and 27 DoF blocks per element");
     }
     HIP_TRY(ctx, hipGetLastError());
@@ -953,7 +955,7 @@ int symx_hip_problem_project_selective(symx_hip_problem p, double eps, int32_t m
     HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), ctx->stream));
     launch_ppn_select(ctx->stream, p->d_goff, p->d_glist, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb, threshold, pr, p->d_projected, p->d_select, cnt + 2);
     for (auto& pot : p->pots)
-        launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_select + pot.e_base, cnt);
+        launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters, p->d_select + pot.e_base, cnt);
     launch_ppn_commit(ctx->stream, p->d_projected, p->d_select, p->total_elem, cnt + 1);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -1383,7 +1385,7 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
             if (hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice,
                                ctx->stream) != hipSuccess)
                 return SYMX_HIP_ERR_HIP;
-            return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, nullptr, nullptr) ? SYMX_HIP_ERR_ARG : 0;
+            return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist, ctx->d_counters, nullptr, nullptr) ? SYMX_HIP_ERR_ARG : 0;
         }
         case 2:
             launch_bsr_assemble(ctx->stream, S, p->d_coff, p->d_clist, p->d_H, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
@@ -1435,7 +1437,7 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
         for (int i = 0; i < reps && !rc; i++) {
             chk(hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice, ctx->stream));
             chk(hipEventRecord(e0, ctx->stream));
-            launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, nullptr, nullptr);
+            launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist, ctx->d_counters, nullptr, nullptr);
             chk(hipEventRecord(e1, ctx->stream));
             chk(hipEventSynchronize(e1));
             float ms = 0.0f;
