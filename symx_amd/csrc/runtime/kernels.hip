@@ -1312,12 +1312,16 @@ namespace symx_hip
             const TV p0 = z[kk + 0] + tb * p_old[kk + 0];
             const TV p1 = z[kk + 1] + tb * p_old[kk + 1];
             const TV p2 = z[kk + 2] + tb * p_old[kk + 2];
-            p_new[kk + 0] = p0;
-            p_new[kk + 1] = p1;
-            p_new[kk + 2] = p2;
-            q[kk + 0] = a0;
-            q[kk + 1] = a1;
-            q[kk + 2] = a2;
+            // non-temporal stores: the vectors go out to memory as they are written instead
+            // of staying dirty in the XCD's L2 until the end-of-kernel release, which on the
+            // next kernel's critical path writes them back (PCG iteration 0.153 -> 0.148 ms
+            // on the 6M-tet block)
+            __builtin_nontemporal_store(p0, p_new + kk + 0);
+            __builtin_nontemporal_store(p1, p_new + kk + 1);
+            __builtin_nontemporal_store(p2, p_new + kk + 2);
+            __builtin_nontemporal_store(a0, q + kk + 0);
+            __builtin_nontemporal_store(a1, q + kk + 1);
+            __builtin_nontemporal_store(a2, q + kk + 2);
             dot += (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
         }
         const double s = block_sum<NTS>(dot, lds);
@@ -1361,14 +1365,14 @@ namespace symx_hip
             TV rv[3], zv[3];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                x[kk + i] = x[kk + i] + ta * p[kk + i];
+                __builtin_nontemporal_store(x[kk + i] + ta * p[kk + i], x + kk + i);  // as in k_pcg_spmv
                 rv[i] = r[kk + i] - ta * q[kk + i];
-                r[kk + i] = rv[i];
+                __builtin_nontemporal_store(rv[i], r + kk + i);
             }
             apply_dinv(dinv, n_rows, row, rv, zv);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                z[kk + i] = zv[i];
+                __builtin_nontemporal_store(zv[i], z + kk + i);
                 rr += (double)rv[i] * (double)rv[i];
                 rzn += (double)rv[i] * (double)zv[i];
             }
