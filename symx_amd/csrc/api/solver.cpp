@@ -580,9 +580,27 @@ namespace symx
 
     void NewtonsMethod::sync_dofs_to_host()
     {
+        // one DoF array (the usual case): copy straight into it, without a host staging copy
+        const auto& sets = G->get_dof_sets();
+        if (sets.size() == 1 && sets[0].flat_size() == dev->n_dofs()) {
+            check(symx_hip_problem_get_dofs(dev->problem(), sets[0].data()), context->hip(), "get_dofs");
+            return;
+        }
         host_buf.resize(dev->n_dofs());
         check(symx_hip_problem_get_dofs(dev->problem(), host_buf.data()), context->hip(), "get_dofs");
         G->set_dofs(host_buf.data());
+    }
+
+    void NewtonsMethod::upload_dofs()
+    {
+        const auto& sets = G->get_dof_sets();
+        if (sets.size() == 1 && sets[0].flat_size() == dev->n_dofs()) {
+            check(symx_hip_problem_set_dofs(dev->problem(), sets[0].data()), context->hip(), "set_dofs");
+            return;
+        }
+        host_buf.resize(dev->n_dofs());
+        G->get_dofs(host_buf.data());
+        check(symx_hip_problem_set_dofs(dev->problem(), host_buf.data()), context->hip(), "set_dofs");
     }
 
     double NewtonsMethod::compute_residual()
@@ -629,9 +647,7 @@ namespace symx
         auto& out = context->output;
 
         dev->refresh_data();
-        host_buf.resize(ndofs);
-        G->get_dofs(host_buf.data());
-        check(symx_hip_problem_set_dofs(prob, host_buf.data()), hip, "set_dofs");
+        upload_dofs();
 
         stats = SolveStats();
         double E0 = 0.0;

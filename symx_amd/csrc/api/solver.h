@@ -518,6 +518,7 @@ namespace symx
         bool solve_linear_system(double residual_norm);
         SolverReturn line_search(double E0, double du_dot_grad, double du_max);
         void sync_dofs_to_host();
+        void upload_dofs();
         double compute_residual();
         double energy();
 
