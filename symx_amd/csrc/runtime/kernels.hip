@@ -211,48 +211,47 @@ namespace symx_hip
     // Sturm bisection, inverse iteration, Gram-Schmidt, back-transform), and H is updated as
     //     H + sum_j (f(lambda_j) - lambda_j) z_j z_j^T + f(0) T T^T,   z_j = U Q y_j,
     // the same spectral function as the reference's V f(Lambda) V^T rebuild
-    // (project_to_PD.cpp:13-32). Returns false (and leaves h untouched) when the block rows
-    // do not sum to zero or when more than three eigenvalues of M lie below eps; those
-    // elements take the full Jacobi path.
-    __device__ __forceinline__ bool project_element_reduced4(double* __restrict__ h, double eps, bool mirror, bool& changed,
-                                                             double* hv)
+    // (project_to_PD.cpp:13-32). Elements whose block rows do not sum to zero, or with more
+    // than three eigenvalues of M below eps, are handed to the workgroup-per-matrix kernel.
+
+    // Block (a, b), a <= b, of upper-block index u (inverse of ub4).
+    __host__ __device__ constexpr int ub4_row(int u) { return u < 4 ? 0 : (u < 7 ? 1 : (u < 9 ? 2 : 3)); }
+    __host__ __device__ constexpr int ub4_col(int u) { return u < 4 ? u : (u < 7 ? u - 3 : (u < 9 ? u - 5 : 3)); }
+
+    // Stored value T (block u = T / 9, entry (i, j)) is H[(a,i),(b,j)] and, for a < b, also
+    // H[(b,j),(a,i)]. It is added to the block-row sums rs[9 A + 3 I + J] = sum_B
+    // H[(A,I),(B,J)], the scale max|H| and the upper triangle of M(al,I; be,J) =
+    // sum_{A,B} w(A,al) w(B,be) H[(A,I),(B,J)]; the values may arrive in any order.
+    template <int T>
+    __device__ __forceinline__ void r4_take_value(double v, double (&m)[Sym<9>::size], double (&rs)[36], double& scale)
     {
-        double scale = 0.0, rowsum = 0.0;
-        static_for<0, 90>([&](auto Kc) { scale = fmax(scale, fabs(h[decltype(Kc)::value])); });
-        static_for<0, 4>([&](auto Ac) {
-            static_for<0, 3>([&](auto Ic) {
-                static_for<0, 3>([&](auto Jc) {
-                    constexpr int A = decltype(Ac)::value, I = decltype(Ic)::value, J = decltype(Jc)::value;
-                    const double s = h_entry<A, 0, I, J>(h) + h_entry<A, 1, I, J>(h) + h_entry<A, 2, I, J>(h) + h_entry<A, 3, I, J>(h);
-                    rowsum = fmax(rowsum, fabs(s));
-                });
-            });
-        });
-        if (!(rowsum <= 1e-10 * scale)) return false;
-        double m[Sym<9>::size];
+        constexpr int u = T / 9, i = (T % 9) / 3, j = T % 3, a = ub4_row(u), b = ub4_col(u);
+        scale = fmax(scale, fabs(v));
+        rs[9 * a + 3 * i + j] += v;
+        if constexpr (a != b) rs[9 * b + 3 * j + i] += v;
         static_for<0, 3>([&](auto Al) {
             static_for<decltype(Al)::value, 3>([&](auto Be) {
-                static_for<0, 3>([&](auto Ic) {
-                    static_for<0, 3>([&](auto Jc) {
-                        constexpr int al = decltype(Al)::value, be = decltype(Be)::value;
-                        constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
-                        if constexpr (al != be || I <= J) {
-                            double acc = 0.0;
-                            static_for<0, 4>([&](auto Ac) {
-                                static_for<0, 4>([&](auto Bc) {
-                                    constexpr int A = decltype(Ac)::value, B = decltype(Bc)::value;
-                                    constexpr double w = helmert(A, al) * helmert(B, be);
-                                    if constexpr (w != 0.0) acc += w * h_entry<A, B, I, J>(h);
-                                });
-                            });
-                            m[Sym<9>::idx(3 * al + I, 3 * be + J)] = acc;
-                        }
-                    });
-                });
+                constexpr int al = decltype(Al)::value, be = decltype(Be)::value;
+                if constexpr (al != be || i <= j) {
+                    constexpr double w = helmert(a, al) * helmert(b, be);
+                    if constexpr (w != 0.0) m[Sym<9>::idx(3 * al + i, 3 * be + j)] += w * v;
+                }
+                if constexpr (a != b && (al != be || j <= i)) {
+                    constexpr double w = helmert(b, al) * helmert(a, be);
+                    if constexpr (w != 0.0) m[Sym<9>::idx(3 * al + j, 3 * be + i)] += w * v;
+                }
             });
         });
-        // h is read again for the update: keep its 90 entries from staying live in registers
-        asm volatile("" ::: "memory");
+    }
+
+    // Eigenpairs of M below eps: the eigenvectors of M in y (Q y_S; the update vectors are
+    // z_S = U y_S) with weights dl_S (zero for S >= cnt). m is overwritten. Returns false
+    // when the element must take the workgroup path (more than three eigenvalues below eps,
+    // or an eigenvalue that left its Sturm bracket). hv: 28 lane-interleaved LDS doubles
+    // (stride 64) for the reflectors.
+    __device__ __forceinline__ bool r4_eigen_update(double (&m)[Sym<9>::size], double eps, bool mirror, double* hv, double (&y)[3][9],
+                                                    double (&dl)[3])
+    {
         double d[9], e[9], tau[9];
         tridiag_reg<9>(m, d, e, tau);
         // the reflectors (28 entries) wait in LDS (hv: 28 x 64 doubles, lane-interleaved)
@@ -310,7 +309,6 @@ namespace symx_hip
         // element). The converged quotient must stay in its Sturm bracket (widened by the
         // bracket width), else the element takes the Jacobi path.
         const double ptol = ulp * tn, bigr = 1.0 / ptol;
-        double y[3][9];
         bool ok = true;
         static_for<0, 3>([&](auto Jc) {
             constexpr int J = decltype(Jc)::value;
@@ -349,8 +347,7 @@ namespace symx_hip
             static_for<0, 9>([&](auto Ic) { y[J][Ic()] *= inv; });
         });
         if (!ok) return false;
-        // back-transform u = H_0 ... H_6 y, then z = U u and the update of H
-        double z[3][12], dl[3];
+        // back-transform y <- H_0 ... H_6 y
         static_for<0, 3>([&](auto Jc) {
             constexpr int J = decltype(Jc)::value;
             static_for<0, 7>([&](auto Kc) {
@@ -361,38 +358,38 @@ namespace symx_hip
                 y[J][K + 1] -= dp;
                 static_for<K + 2, 9>([&](auto Ic) { y[J][Ic()] -= dp * hv[(B + Ic() - K - 2) * 64]; });
             });
-            static_for<0, 4>([&](auto Ac) {
-                static_for<0, 3>([&](auto Ic) {
-                    constexpr int A = decltype(Ac)::value, I = decltype(Ic)::value;
-                    double acc = 0.0;
-                    static_for<0, 3>([&](auto Al) {
-                        constexpr double w = helmert(A, decltype(Al)::value);
-                        if constexpr (w != 0.0) acc += w * y[J][3 * decltype(Al)::value + I];
-                    });
-                    z[J][3 * A + I] = acc;
-                });
-            });
             dl[J] = J < cnt ? (mirror ? -2.0 * lam[J] : eps - lam[J]) : 0.0;
         });
-        const double null_term = mirror ? 0.0 : 0.25 * eps;  // f(0) on span{translations}: eps T T^T, T T^T = (1/4) 1 1^T (x) I3
-        static_for<0, 4>([&](auto Ac) {
-            static_for<decltype(Ac)::value, 4>([&](auto Bc) {
-                static_for<0, 3>([&](auto Ic) {
-                    static_for<0, 3>([&](auto Jc) {
-                        constexpr int A = decltype(Ac)::value, B = decltype(Bc)::value;
-                        constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
-                        double acc = (I == J) ? null_term : 0.0;
-                        static_for<0, 3>([&](auto Sc) {
-                            constexpr int S = decltype(Sc)::value;
-                            acc += dl[S] * z[S][3 * A + I] * z[S][3 * B + J];
-                        });
-                        h[ub4(A, B) * 9 + 3 * I + J] += acc;
-                    });
-                });
-            });
-        });
-        changed = true;
         return true;
+    }
+
+    // One wave's 64 consecutive 90-double records, values [c0, c0 + W): global <-> LDS
+    // stage[rec * 17 + k] with consecutive lanes on consecutive addresses (whole cache
+    // lines per instruction, instead of 64 lines when each lane reads its own record).
+    // Loads cover the records below nv; stores only those with use[rec] != 0.
+    template <int W>
+    __device__ __forceinline__ void r4_stage_in(const double* __restrict__ base, int nv, double* stage, int lane)
+    {
+        double v[W];
+#pragma unroll
+        for (int s = 0; s < W; s++) {
+            const int k = lane + 64 * s, rec = k / W, q = k - rec * W;
+            v[s] = rec < nv ? base[(int64_t)rec * 90 + q] : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < W; s++) {
+            const int k = lane + 64 * s, rec = k / W, q = k - rec * W;
+            stage[rec * 17 + q] = v[s];
+        }
+    }
+    template <int W>
+    __device__ __forceinline__ void r4_stage_out(double* __restrict__ base, const int* use, const double* stage, int lane)
+    {
+#pragma unroll 4
+        for (int k = lane; k < 64 * W; k += 64) {
+            const int rec = k / W, q = k - rec * W;
+            if (use[rec]) base[(int64_t)rec * 90 + q] = stage[rec * 17 + q];
+        }
     }
 
     __device__ inline void count_changed(bool changed, unsigned long long* count)
@@ -403,25 +400,88 @@ namespace symx_hip
         }
     }
 
-    // 4-block elements: reduced 9x9 path; elements that are not translation invariant
-    // are flagged for k_project_ub_full.
+    // 4-block elements, one element per lane, 64 per workgroup (one wave): reduced 9x9 path;
+    // the elements it hands back are flagged in needs_full. The element records are read and
+    // written through LDS 16 values at a time with coalesced transfers: M, the block-row
+    // sums and the scale accumulate chunk by chunk, and the update is added chunk by chunk.
     __global__ __launch_bounds__(64) void k_project_ub4_reduced(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
                                                                unsigned long long* count, uint8_t* __restrict__ needs_full,
                                                                const uint8_t* __restrict__ select)
     {
-        __shared__ double s_hv[28 * 64];
-        const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
-        bool changed = false;
-        if (e < n_mats) {
-            if (select == nullptr || select[e]) {
-                const bool ok = project_element_reduced4(H + e * 90, eps, mirror != 0, changed, s_hv + threadIdx.x);
-                needs_full[e] = ok ? 0 : 1;
-            }
-            else {
-                needs_full[e] = 0;
+        // reflectors during the eigen-solve (28 x 64, lane-interleaved); staging of the
+        // element records (64 x 17) before and after it
+        __shared__ double s_lds[28 * 64];
+        __shared__ int s_use[64];
+        const int lane = threadIdx.x;
+        const int64_t e0 = (int64_t)blockIdx.x * 64, e = e0 + lane;
+        const bool active = e < n_mats && (select == nullptr || select[e]);
+        const int nv = n_mats - e0 < 64 ? (int)(n_mats - e0) : 64;  // records in this wave
+        double* const base = H + e0 * 90;
+        s_use[lane] = active ? 1 : 0;
+        __syncthreads();
+        bool ok = false;
+        double y[3][9], dl[3];
+        if (__ballot(active) != 0ull) {
+            double m[Sym<9>::size], rs[36], scale = 0.0;
+            static_for<0, Sym<9>::size>([&](auto Kc) { m[Kc()] = 0.0; });
+            static_for<0, 36>([&](auto Kc) { rs[Kc()] = 0.0; });
+            static_for<0, 6>([&](auto Cc) {
+                constexpr int c0 = 16 * decltype(Cc)::value, W = 90 - c0 < 16 ? 90 - c0 : 16;
+                r4_stage_in<W>(base + c0, nv, s_lds, lane);
+                __syncthreads();
+                static_for<0, W>([&](auto Kc) { r4_take_value<c0 + decltype(Kc)::value>(s_lds[lane * 17 + Kc()], m, rs, scale); });
+                __syncthreads();
+            });
+            if (active) {
+                double rowsum = 0.0;
+                static_for<0, 36>([&](auto Kc) { rowsum = fmax(rowsum, fabs(rs[Kc()])); });
+                ok = rowsum <= 1e-10 * scale && r4_eigen_update(m, eps, mirror != 0, s_lds + lane, y, dl);
             }
         }
-        count_changed(changed, count);
+        if (e < n_mats) needs_full[e] = (active && !ok) ? 1 : 0;
+        __syncthreads();  // every lane is done with the reflectors
+        s_use[lane] = ok ? 1 : 0;
+        __syncthreads();
+        if (__ballot(ok) != 0ull) {
+            // f(0) on span{translations}: eps T T^T, T T^T = (1/4) 1 1^T (x) I3
+            const double null_term = mirror ? 0.0 : 0.25 * eps;
+            // z_S = U y_S, U = W (x) I3
+            double z[3][12];
+            static_for<0, 3>([&](auto Sc) {
+                static_for<0, 4>([&](auto Ac) {
+                    static_for<0, 3>([&](auto Ic) {
+                        constexpr int S = decltype(Sc)::value, A = decltype(Ac)::value, I = decltype(Ic)::value;
+                        double acc = 0.0;
+                        static_for<0, 3>([&](auto Al) {
+                            constexpr double w = helmert(A, decltype(Al)::value);
+                            if constexpr (w != 0.0) acc += w * y[S][3 * decltype(Al)::value + I];
+                        });
+                        z[S][3 * A + I] = acc;
+                    });
+                });
+            });
+            static_for<0, 6>([&](auto Cc) {
+                constexpr int c0 = 16 * decltype(Cc)::value, W = 90 - c0 < 16 ? 90 - c0 : 16;
+                r4_stage_in<W>(base + c0, nv, s_lds, lane);
+                __syncthreads();
+                if (ok) {
+                    static_for<0, W>([&](auto Kc) {
+                        constexpr int T = c0 + decltype(Kc)::value, u = T / 9, i = (T % 9) / 3, j = T % 3;
+                        constexpr int a = ub4_row(u), b = ub4_col(u);
+                        double acc = (i == j) ? null_term : 0.0;
+                        static_for<0, 3>([&](auto Sc) {
+                            constexpr int S = decltype(Sc)::value;
+                            acc += dl[S] * z[S][3 * a + i] * z[S][3 * b + j];
+                        });
+                        s_lds[lane * 17 + Kc()] += acc;
+                    });
+                }
+                __syncthreads();
+                r4_stage_out<W>(base + c0, s_use, s_lds, lane);
+                __syncthreads();
+            });
+        }
+        count_changed(ok, count);
     }
 
     template <int NB>
