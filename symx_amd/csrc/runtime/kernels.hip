@@ -584,12 +584,13 @@ namespace symx_hip
         c1 = n1;
     }
 
-    // One matrix h (element e) with the whole workgroup; every early return is uniform.
-    template <int N, bool DENSE>
+    // One matrix h (element e) with the whole workgroup of PT threads (a multiple of 64);
+    // every early return is uniform.
+    template <int N, bool DENSE, int PT>
     __device__ void project_large_one(double* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
                                       uint8_t* __restrict__ changed_out)
     {
-        constexpr int NB = N / 3, LD = N | 1, NUB = NB * (NB + 1) / 2, NW = PNT / 64, NPT = PNT * PPTS;
+        constexpr int NB = N / 3, LD = N | 1, NUB = NB * (NB + 1) / 2, NW = PT / 64, NPT = PT * PPTS;
         static_assert(N <= 128, "one wave holds a vector in two registers per lane");
         __shared__ double A[N * LD];
         __shared__ double Y[N * N];
@@ -611,7 +612,7 @@ namespace symx_hip
             r = 3 * a + ij / 3;
             c = 3 * (a + rem) + ij % 3;
         };
-        for (int t = tid; t < N * N; t += PNT) {
+        for (int t = tid; t < N * N; t += PT) {
             const int r = t / N, c = t - r * N;
             double v;
             if (DENSE) v = 0.5 * (h[r * N + c] + h[c * N + r]);
@@ -633,9 +634,9 @@ namespace symx_hip
         };
         // ---- 1. scale, translation invariance ---------------------------------------------
         double amax = 0.0, rsum = 0.0;
-        for (int t = tid; t < N * N; t += PNT) amax = fmax(amax, fabs(A[(t / N) * LD + t % N]));
+        for (int t = tid; t < N * N; t += PT) amax = fmax(amax, fabs(A[(t / N) * LD + t % N]));
         if (!DENSE) {
-            for (int t = tid; t < N * 3; t += PNT) {  // row r, component j: sum over blocks b of A[r][3b + j]
+            for (int t = tid; t < N * 3; t += PT) {  // row r, component j: sum over blocks b of A[r][3b + j]
                 const int r = t / 3, j = t - 3 * r;
                 double acc = 0.0;
                 for (int b = 0; b < NB; b++) acc += A[r * LD + 3 * b + j];
@@ -647,7 +648,7 @@ namespace symx_hip
         if (!(scale > 0.0)) {
             // zero matrix: every eigenvalue (0) is below eps -> eps I (clamp), 0 (mirror)
             if (!mirror) {
-                for (int r = tid; r < N; r += PNT) {
+                for (int r = tid; r < N; r += PT) {
                     if (DENSE) h[r * N + r] = eps;
                     else h[ubi(r / 3, r / 3) * 9 + 4 * (r % 3)] = eps;
                 }
@@ -660,7 +661,7 @@ namespace symx_hip
         }
         if (ti) {
             const double sh = scale / NB;
-            for (int t = tid; t < N * N; t += PNT) {
+            for (int t = tid; t < N * N; t += PT) {
                 const int r = t / N, c = t - r * N;
                 if ((r % 3) == (c % 3)) A[r * LD + c] += sh;
             }
@@ -699,7 +700,7 @@ namespace symx_hip
             for (int i = lane; i < m; i += 64) kk += pv[i] * wv[i];
             kk = 0.5 * t * wave_sum_all(kk);
             if (t != 0.0) {
-                const int step = PNT / m;
+                const int step = PT / m;
                 if (tid < step * m) {
                     const int j = tid % m;
                     const double vj = wv[j], wj = pv[j] - kk * vj;
@@ -748,7 +749,7 @@ namespace symx_hip
         sturm2<N>(dg, od, eps, eps, pivmin, cnt, dummy);
         if (cnt == 0) {
             if (ti && !mirror) {
-                for (int t = tid; t < NUB * 9; t += PNT) {
+                for (int t = tid; t < NUB * 9; t += PT) {
                     const int ij = t % 9;
                     if (ij / 3 == ij % 3) h[t] += eps / NB;
                 }
@@ -767,7 +768,7 @@ namespace symx_hip
             cbuf[p0 + 1] = c1;
             if (tid == 0) busy[0] = 0;
             __syncthreads();
-            for (int j = tid; j < cnt; j += PNT) {  // first point with more than j eigenvalues below it
+            for (int j = tid; j < cnt; j += PT) {  // first point with more than j eigenvalues below it
                 int lo = 0, hi = NPT;
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
@@ -795,7 +796,7 @@ namespace symx_hip
                 }
                 if (tid == 0) busy[(round + 1) % 3] = 0;
                 __syncthreads();
-                for (int j = tid; j < cnt; j += PNT) {
+                for (int j = tid; j < cnt; j += PT) {
                     const double a = bra[j], b = brb[j];
                     if (b - a > tol) {
                         int lo = 0, hi = P;
@@ -814,7 +815,7 @@ namespace symx_hip
                 __syncthreads();
                 if (!busy[round % 3]) break;
             }
-            for (int j = tid; j < cnt; j += PNT) lamv[j] = 0.5 * (bra[j] + brb[j]);
+            for (int j = tid; j < cnt; j += PT) lamv[j] = 0.5 * (bra[j] + brb[j]);
             __syncthreads();
         }
         // ---- 5. block inverse iteration (dstein), one lane per eigenvalue --------------------
@@ -829,7 +830,7 @@ namespace symx_hip
             double* u1 = lu + N * PLU;
             double* u2 = lu + 2 * N * PLU;
             const double ptol = ulp * tn, bigr = 1.0 / ptol;
-            for (int t = tid; t < cnt * N; t += PNT) {
+            for (int t = tid; t < cnt * N; t += PT) {
                 const int j = t / N;
                 Y[t] = start_component(j, t - j * N);
             }
@@ -918,10 +919,10 @@ namespace symx_hip
         }
         __syncthreads();
         // ---- 7. rank-cnt update -------------------------------------------------------------
-        for (int j = tid; j < cnt; j += PNT) lamv[j] = mirror ? -2.0 * lamv[j] : eps - lamv[j];
+        for (int j = tid; j < cnt; j += PT) lamv[j] = mirror ? -2.0 * lamv[j] : eps - lamv[j];
         __syncthreads();
         if (DENSE) {
-            for (int t = tid; t < N * N; t += PNT) {
+            for (int t = tid; t < N * N; t += PT) {
                 const int r = t / N, c = t - r * N;
                 double acc = 0.0;
                 for (int j = 0; j < cnt; j++) acc += lamv[j] * Y[j * N + r] * Y[j * N + c];
@@ -930,7 +931,7 @@ namespace symx_hip
         }
         else {
             const double tshift = (ti && !mirror) ? eps / NB : 0.0;
-            for (int t = tid; t < NUB * 9; t += PNT) {
+            for (int t = tid; t < NUB * 9; t += PT) {
                 int r, c;
                 ub_index(t, r, c);
                 double acc = (r % 3) == (c % 3) ? tshift : 0.0;
@@ -953,21 +954,21 @@ namespace symx_hip
         const int64_t e = blockIdx.x;
         if (e >= n_mats) return;
         if (select != nullptr && !select[e]) return;
-        project_large_one<N, DENSE>(DENSE ? H + e * (int64_t)(N * N) : H + e * (int64_t)(NUB * 9), e, eps, mirror, count, changed_out);
+        project_large_one<N, DENSE, PNT>(DENSE ? H + e * (int64_t)(N * N) : H + e * (int64_t)(NUB * 9), e, eps, mirror, count, changed_out);
     }
 
-    // The elements listed in list[0 .. *n_list) (upper-block storage), a fixed grid
-    // striding over the list.
-    template <int N>
-    __global__ __launch_bounds__(PNT) void k_project_large_list(double* __restrict__ H, const uint32_t* __restrict__ list,
-                                                               const unsigned int* __restrict__ n_list, double eps, int mirror,
-                                                               unsigned long long* count)
+    // The elements listed in list[0 .. *n_list) (upper-block storage), a fixed grid of
+    // PT-thread workgroups striding over the list (one wave per 12x12 matrix).
+    template <int N, int PT>
+    __global__ __launch_bounds__(PT) void k_project_large_list(double* __restrict__ H, const uint32_t* __restrict__ list,
+                                                              const unsigned int* __restrict__ n_list, double eps, int mirror,
+                                                              unsigned long long* count)
     {
         constexpr int NUB = (N / 3) * (N / 3 + 1) / 2;
         const unsigned int n = *n_list;
         for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
             const int64_t e = list[i];
-            project_large_one<N, false>(H + e * (int64_t)(NUB * 9), e, eps, mirror, count, nullptr);
+            project_large_one<N, false, PT>(H + e * (int64_t)(NUB * 9), e, eps, mirror, count, nullptr);
             __syncthreads();  // LDS is reused by the next matrix
         }
     }
@@ -998,7 +999,7 @@ namespace symx_hip
             k_project_ub4_reduced<<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags, select);
             k_zero_counter<<<1, 1, 0, s>>>(n_list);
             k_list_flagged<<<grid_for(n_mats), NT, 0, s>>>(flags, n_mats, list, n_list);
-            k_project_large_list<12><<<1024, PNT, 0, s>>>(H, list, n_list, eps, mirror, d_count);
+            k_project_large_list<12, 64><<<4096, 64, 0, s>>>(H, list, n_list, eps, mirror, d_count);
             return 0;
         case 8: k_project_large<24, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
         case 10: k_project_large<30, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
