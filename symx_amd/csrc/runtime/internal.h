@@ -146,13 +146,16 @@ struct symx_hip_problem_s
     int32_t filter_axis = 2;
     double filter_plane = 0.0;
 
-    // gradient scatter (CSR by DoF block of gradient-block ids)
+    // gradient scatter: CSR of gradient-block ids by DoF block, the blocks in the assembly
+    // traversal order (d_grow[t] = DoF block of list t)
     uint32_t* d_goff = nullptr;
     uint32_t* d_glist = nullptr;
+    int32_t* d_grow = nullptr;
 
     // block matrix
     symx_hip::SellPattern sell;
-    uint32_t* d_coff = nullptr;   // [n_slots + 1]
+    uint32_t* d_aslot = nullptr;  // [n_slots] slots in assembly traversal order
+    uint32_t* d_coff = nullptr;   // [n_slots + 1] contribution lists in that order
     uint32_t* d_clist = nullptr;  // Hessian block id * 2 + transpose flag
     double* d_val64 = nullptr;
     float* d_val32 = nullptr;
@@ -170,7 +173,8 @@ namespace symx_hip
 {
     // Fixed-order sum of n doubles into *d_out (single workgroup).
     void launch_sum(hipStream_t s, const double* d_in, int64_t n, double* d_out);
-    void launch_grad_assemble(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* G, double* grad, int32_t nb);
+    void launch_grad_assemble(hipStream_t s, const uint32_t* off, const uint32_t* list, const int32_t* rows, const double* G, double* grad,
+                              int32_t nb);
     // flags: n_mats bytes of scratch (4-block elements that are not translation invariant)
     // Elements of the active DoF blocks (max |grad| of the block >= threshold) that are not
     // projected yet get select[e] = 1; the number of active blocks is added to n_active.
@@ -183,7 +187,8 @@ namespace symx_hip
         int64_t e_base[16];
         const uint8_t* active[16];  // activity flags of conditional potentials (NULL: all active)
     };
-    void launch_ppn_select(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* grad, int32_t nblocks, double threshold,
+    void launch_ppn_select(hipStream_t s, const uint32_t* off, const uint32_t* list, const int32_t* rows, const double* grad, int32_t nblocks,
+                           double threshold,
                            const PotRanges& pr, const uint8_t* projected, uint8_t* select, unsigned long long* n_active);
     // projected |= select, select = 0; adds the number of newly projected elements to n_new.
     void launch_ppn_commit(hipStream_t s, uint8_t* projected, uint8_t* select, int64_t n, unsigned long long* n_new);
@@ -196,8 +201,8 @@ namespace symx_hip
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
                           uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count);
     int launch_project_dense(hipStream_t s, double* mats, int64_t n_mats, int32_t n, double eps, int32_t mirror, uint8_t* changed);
-    void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* coff, const uint32_t* clist, const double* H,
-                             double* v64, float* v32);
+    void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* aslot, const uint32_t* coff, const uint32_t* clist,
+                             const double* H, double* v64, float* v32);
     void launch_dinv(hipStream_t s, const SellPattern& P, const double* v64, const float* v32, double* dinv64, float* dinv32);
     // PCG on the sell matrix (iteration k is 1-based). partials: >= 5 * reduction_groups(n_rows) doubles.
     void launch_pcg_init64(hipStream_t s, const SellPattern& P, const double* dinv, const double* b, double* x, double* r, double* z,

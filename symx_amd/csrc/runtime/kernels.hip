@@ -145,14 +145,17 @@ namespace symx_hip
     }
 
     // ---- gradient assembly: grad[3b+i] = sum of element gradient blocks -----------
+    // List t belongs to DoF block rows[t] (lists stored in the assembly traversal order).
     __global__ __launch_bounds__(NT) void k_grad_assemble(const uint32_t* __restrict__ off, const uint32_t* __restrict__ list,
-                                                          const double* __restrict__ G, double* __restrict__ grad, int32_t nb)
+                                                          const int32_t* __restrict__ rows, const double* __restrict__ G,
+                                                          double* __restrict__ grad, int32_t nb)
     {
-        const int64_t b = xcd_remap() * NT + threadIdx.x;
-        if (b >= nb) return;
+        const int64_t t = xcd_remap() * NT + threadIdx.x;
+        if (t >= nb) return;
+        const int64_t b = rows[t];
         double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-        const uint32_t k1 = off[b + 1];
-        for (uint32_t k = off[b]; k < k1; k++) {
+        const uint32_t k1 = off[t + 1];
+        for (uint32_t k = off[t]; k < k1; k++) {
             const double* src = G + (int64_t)list[k] * 3;
             g0 += src[0];
             g1 += src[1];
@@ -162,10 +165,11 @@ namespace symx_hip
         grad[3 * b + 1] = g1;
         grad[3 * b + 2] = g2;
     }
-    void launch_grad_assemble(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* G, double* grad, int32_t nb)
+    void launch_grad_assemble(hipStream_t s, const uint32_t* off, const uint32_t* list, const int32_t* rows, const double* G, double* grad,
+                              int32_t nb)
     {
         if (nb <= 0) return;
-        k_grad_assemble<<<grid_for(nb), NT, 0, s>>>(off, list, G, grad, nb);
+        k_grad_assemble<<<grid_for(nb), NT, 0, s>>>(off, list, rows, G, grad, nb);
     }
 
     // ---- PD projection (device code in projection.h) ---------------------------------
@@ -1010,17 +1014,18 @@ namespace symx_hip
 
     // ---- progressive projection (NewtonsMethod.cpp:306-330, ElementHessians.cpp:79-165) ----
     __global__ __launch_bounds__(NT) void k_ppn_select(const uint32_t* __restrict__ off, const uint32_t* __restrict__ list,
-                                                      const double* __restrict__ grad, int32_t nblocks, double threshold, PotRanges pr,
-                                                      const uint8_t* __restrict__ projected, uint8_t* __restrict__ select,
-                                                      unsigned long long* n_active)
+                                                      const int32_t* __restrict__ rows, const double* __restrict__ grad, int32_t nblocks,
+                                                      double threshold, PotRanges pr, const uint8_t* __restrict__ projected,
+                                                      uint8_t* __restrict__ select, unsigned long long* n_active)
     {
-        const int64_t b = (int64_t)blockIdx.x * NT + threadIdx.x;
+        const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;  // gradient list t, of DoF block rows[t]
         bool active = false;
-        if (b < nblocks) {
+        if (t < nblocks) {
+            const int64_t b = rows[t];
             const double m = fmax(fabs(grad[3 * b]), fmax(fabs(grad[3 * b + 1]), fabs(grad[3 * b + 2])));
             active = m >= threshold;
             if (active) {
-                for (uint32_t k = off[b]; k < off[b + 1]; k++) {
+                for (uint32_t k = off[t]; k < off[t + 1]; k++) {
                     const int64_t gi = list[k];
                     for (int q = 0; q < pr.n; q++) {
                         if (gi >= pr.g_base[q] && gi < pr.g_end[q]) {
@@ -1036,11 +1041,12 @@ namespace symx_hip
         const unsigned long long m = __ballot(active);
         if ((threadIdx.x & 63) == 0 && m != 0ull) atomicAdd(n_active, (unsigned long long)__popcll(m));
     }
-    void launch_ppn_select(hipStream_t s, const uint32_t* off, const uint32_t* list, const double* grad, int32_t nblocks, double threshold,
-                           const PotRanges& pr, const uint8_t* projected, uint8_t* select, unsigned long long* n_active)
+    void launch_ppn_select(hipStream_t s, const uint32_t* off, const uint32_t* list, const int32_t* rows, const double* grad, int32_t nblocks,
+                           double threshold, const PotRanges& pr, const uint8_t* projected, uint8_t* select, unsigned long long* n_active)
     {
         if (nblocks <= 0) return;
-        k_ppn_select<<<(unsigned)((nblocks + NT - 1) / NT), NT, 0, s>>>(off, list, grad, nblocks, threshold, pr, projected, select, n_active);
+        k_ppn_select<<<(unsigned)((nblocks + NT - 1) / NT), NT, 0, s>>>(off, list, rows, grad, nblocks, threshold, pr, projected, select,
+                                                                        n_active);
     }
 
     __global__ __launch_bounds__(NT) void k_ppn_commit(uint8_t* __restrict__ projected, uint8_t* __restrict__ select, int64_t n,
@@ -1138,13 +1144,16 @@ namespace symx_hip
     }
 
     // ---- assembly: one thread per block-matrix slot, contributions in fixed order ----
+    // Thread t assembles slot aslot[t] from contribution list t (slots in the assembly
+    // traversal order, whole SELL slices at a time).
     template <typename TS>
-    __global__ __launch_bounds__(NT) void k_bsr_assemble(int64_t n_slots, const uint32_t* __restrict__ coff,
-                                                         const uint32_t* __restrict__ clist, const double* __restrict__ H,
-                                                         TS* __restrict__ val)
+    __global__ __launch_bounds__(NT) void k_bsr_assemble(int64_t n_slots, const uint32_t* __restrict__ aslot,
+                                                         const uint32_t* __restrict__ coff, const uint32_t* __restrict__ clist,
+                                                         const double* __restrict__ H, TS* __restrict__ val)
     {
         const int64_t t = xcd_remap() * NT + threadIdx.x;
         if (t >= n_slots) return;
+        const uint32_t slot = aslot[t];
         double acc[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) acc[k] = 0.0;
@@ -1164,17 +1173,17 @@ namespace symx_hip
                 for (int k2 = 0; k2 < 9; k2++) acc[k2] += h[k2];
             }
         }
-        TS* dst = val + (t >> 6) * 576 + 3 * (t & 63);
+        TS* dst = val + (int64_t)(slot >> 6) * 576 + 3 * (slot & 63);
 #pragma unroll
         for (int k = 0; k < 9; k++) dst[192 * (k / 3) + k % 3] = (TS)acc[k];
     }
 
-    void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* coff, const uint32_t* clist, const double* H,
-                             double* v64, float* v32)
+    void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* aslot, const uint32_t* coff, const uint32_t* clist,
+                             const double* H, double* v64, float* v32)
     {
         if (P.n_slots <= 0) return;
-        if (v64) k_bsr_assemble<double><<<grid_for(P.n_slots), NT, 0, s>>>(P.n_slots, coff, clist, H, v64);
-        if (v32) k_bsr_assemble<float><<<grid_for(P.n_slots), NT, 0, s>>>(P.n_slots, coff, clist, H, v32);
+        if (v64) k_bsr_assemble<double><<<grid_for(P.n_slots), NT, 0, s>>>(P.n_slots, aslot, coff, clist, H, v64);
+        if (v32) k_bsr_assemble<float><<<grid_for(P.n_slots), NT, 0, s>>>(P.n_slots, aslot, coff, clist, H, v32);
     }
 
     // Closed-form inverse of the symmetric diagonal block of each row, computed in

@@ -437,6 +437,8 @@ void symx_hip_problem_destroy(symx_hip_problem p)
     dfree(p->d_filter_blocks);
     dfree(p->d_goff);
     dfree(p->d_glist);
+    dfree(p->d_grow);
+    dfree(p->d_aslot);
     dfree(p->sell.d_slice_ptr);
     dfree(p->sell.d_col);
     dfree(p->sell.d_dcol);
@@ -737,6 +739,64 @@ int symx_hip_problem_finalize(symx_hip_problem p)
         }
     }
 
+    // Traversal order of the two assembly kernels: SELL slices (64 block rows) in patches of
+    // kPatchSlices grown breadth-first over the block pattern, seeds in slice order. The
+    // workgroups in flight then gather the element blocks of one compact region of the mesh,
+    // and neighbouring rows that share elements are assembled at about the same time (in
+    // slice order a lexicographically numbered grid meets every element again one grid
+    // layer later, after the L2 has dropped it). Within a slice the slot order is unchanged,
+    // so the value stores stay contiguous; the per-slot and per-row summation order is
+    // unchanged.
+    std::vector<int32_t> strav;
+    strav.reserve(S.n_slices);
+    {
+        constexpr size_t kPatchSlices = 64;
+        std::vector<uint8_t> seen(S.n_slices, 0);
+        auto row_at_pos = [&](int32_t q) { return S.h_row_at.empty() ? q : S.h_row_at[q]; };
+        for (int32_t seed = 0; seed < S.n_slices; seed++) {
+            if (seen[seed]) continue;
+            const size_t head = strav.size();
+            strav.push_back(seed);
+            seen[seed] = 1;
+            for (size_t qi = head; qi < strav.size() && strav.size() - head < kPatchSlices; qi++) {
+                const int32_t sl = strav[qi];
+                for (int32_t q = 64 * sl; q < std::min(nb, 64 * sl + 64) && strav.size() - head < kPatchSlices; q++) {
+                    const int32_t r = row_at_pos(q);
+                    for (int32_t j = 0; j < row_len[r] && strav.size() - head < kPatchSlices; j++) {
+                        const int32_t sc = S.pos_of(ccol[row_begin[r] + j]) >> 6;
+                        if (!seen[sc]) {
+                            seen[sc] = 1;
+                            strav.push_back(sc);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // gradient lists and slots in traversal order
+    std::vector<int32_t> trav;
+    trav.reserve(nb);
+    std::vector<uint32_t> aslot;
+    aslot.reserve(S.n_slots);
+    for (int32_t sl : strav) {
+        for (int32_t q = 64 * sl; q < std::min(nb, 64 * sl + 64); q++) trav.push_back(S.h_row_at.empty() ? q : S.h_row_at[q]);
+        for (int64_t t = S.h_slice_ptr[sl]; t < S.h_slice_ptr[sl + 1]; t++) aslot.push_back((uint32_t)t);
+    }
+    if ((int64_t)trav.size() != nb || (int64_t)aslot.size() != S.n_slots)
+        return fail(ctx, SYMX_HIP_ERR_ARG, "internal: assembly traversal does not cover every row and slot");
+    std::vector<uint32_t> goff_t((size_t)nb + 1, 0), glist_t(glist.size());
+    for (int32_t t = 0; t < nb; t++) {
+        const int32_t r = trav[t];
+        goff_t[t + 1] = goff_t[t] + (goff[r + 1] - goff[r]);
+        std::copy(glist.begin() + goff[r], glist.begin() + goff[r + 1], glist_t.begin() + goff_t[t]);
+    }
+    std::vector<uint32_t> coff_t((size_t)S.n_slots + 1, 0), clist_t(clist.size());
+    for (int64_t t = 0; t < S.n_slots; t++) {
+        const uint32_t sl = aslot[t];
+        coff_t[t + 1] = coff_t[t] + (soff[sl + 1] - soff[sl]);
+        std::copy(clist.begin() + soff[sl], clist.begin() + soff[sl + 1], clist_t.begin() + coff_t[t]);
+    }
+
     // Upload.
     int rc = 0;
     rc = rc ? rc : dalloc(ctx, &p->d_G, (size_t)std::max<int64_t>(g * 3, 1));
@@ -748,8 +808,10 @@ int symx_hip_problem_finalize(symx_hip_problem p)
         rc = rc ? rc : dalloc(ctx, &p->d_flags, (size_t)max_elem);
         rc = rc ? rc : dalloc(ctx, &p->d_plist, (size_t)max_elem);
     }
-    rc = rc ? rc : dalloc(ctx, &p->d_goff, goff.size());
-    rc = rc ? rc : dalloc(ctx, &p->d_glist, std::max<size_t>(glist.size(), 1));
+    rc = rc ? rc : dalloc(ctx, &p->d_goff, goff_t.size());
+    rc = rc ? rc : dalloc(ctx, &p->d_glist, std::max<size_t>(glist_t.size(), 1));
+    rc = rc ? rc : dalloc(ctx, &p->d_grow, trav.size());
+    rc = rc ? rc : dalloc(ctx, &p->d_aslot, std::max<size_t>(aslot.size(), 1));
     rc = rc ? rc : dalloc(ctx, &S.d_slice_ptr, S.h_slice_ptr.size());
     // Column offsets from the row fit 16 bits for meshes numbered with some locality
     // (structured grids up to ~180^2 vertices per layer); the SpMV then reads 2 bytes
@@ -776,8 +838,8 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     if (dcol.empty()) rc = rc ? rc : dalloc(ctx, &S.d_col, (size_t)S.n_slots);
     else rc = rc ? rc : dalloc(ctx, &S.d_dcol, (size_t)S.n_slots);
     if (!S.h_row_at.empty()) rc = rc ? rc : dalloc(ctx, &S.d_row_at, S.h_row_at.size());
-    rc = rc ? rc : dalloc(ctx, &p->d_coff, soff.size());
-    rc = rc ? rc : dalloc(ctx, &p->d_clist, std::max<size_t>(clist.size(), 1));
+    rc = rc ? rc : dalloc(ctx, &p->d_coff, coff_t.size());
+    rc = rc ? rc : dalloc(ctx, &p->d_clist, std::max<size_t>(clist_t.size(), 1));
     const size_t nd = (size_t)p->n_dofs;
     rc = rc ? rc : dalloc(ctx, &p->d_x, nd);
     rc = rc ? rc : dalloc(ctx, &p->d_r, nd);
@@ -786,14 +848,16 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     rc = rc ? rc : dalloc(ctx, &p->d_p1, nd);
     rc = rc ? rc : dalloc(ctx, &p->d_q, nd);
     if (rc) return rc;
-    HIP_TRY(ctx, hipMemcpy(p->d_goff, goff.data(), goff.size() * 4, hipMemcpyHostToDevice));
-    if (!glist.empty()) HIP_TRY(ctx, hipMemcpy(p->d_glist, glist.data(), glist.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(p->d_goff, goff_t.data(), goff_t.size() * 4, hipMemcpyHostToDevice));
+    if (!glist_t.empty()) HIP_TRY(ctx, hipMemcpy(p->d_glist, glist_t.data(), glist_t.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(p->d_grow, trav.data(), trav.size() * 4, hipMemcpyHostToDevice));
+    if (!aslot.empty()) HIP_TRY(ctx, hipMemcpy(p->d_aslot, aslot.data(), aslot.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(ctx, hipMemcpy(S.d_slice_ptr, S.h_slice_ptr.data(), S.h_slice_ptr.size() * 8, hipMemcpyHostToDevice));
     if (S.d_col) HIP_TRY(ctx, hipMemcpy(S.d_col, S.h_col.data(), S.h_col.size() * 4, hipMemcpyHostToDevice));
     if (S.d_dcol) HIP_TRY(ctx, hipMemcpy(S.d_dcol, dcol.data(), dcol.size() * 2, hipMemcpyHostToDevice));
     if (S.d_row_at) HIP_TRY(ctx, hipMemcpy(S.d_row_at, S.h_row_at.data(), S.h_row_at.size() * 4, hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMemcpy(p->d_coff, soff.data(), soff.size() * 4, hipMemcpyHostToDevice));
-    if (!clist.empty()) HIP_TRY(ctx, hipMemcpy(p->d_clist, clist.data(), clist.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(p->d_coff, coff_t.data(), coff_t.size() * 4, hipMemcpyHostToDevice));
+    if (!clist_t.empty()) HIP_TRY(ctx, hipMemcpy(p->d_clist, clist_t.data(), clist_t.size() * 4, hipMemcpyHostToDevice));
     // PCG: 5 partials per workgroup (launch_pcg_iter*); vector reductions: 2 per workgroup + slack
     rc = ensure_partials(ctx, (size_t)std::max<int64_t>({ 2 * (int64_t)reduction_groups(p->n_dofs) + 2 * ((nb + 255) / 256) + 16,
                                                           5 * (int64_t)reduction_groups(nb) + 16, 4096 }));
@@ -871,7 +935,7 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
     }
     if (mode == SYMX_HIP_EVAL_PGH && p->d_projected) launch_fill_u8(ctx->stream, p->d_projected, p->total_elem);
     launch_sum(ctx->stream, p->d_Epart, p->total_parts, ctx->d_scalars);
-    if (mode >= SYMX_HIP_EVAL_PG) launch_grad_assemble(ctx->stream, p->d_goff, p->d_glist, p->d_G, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb);
+    if (mode >= SYMX_HIP_EVAL_PG) launch_grad_assemble(ctx->stream, p->d_goff, p->d_glist, p->d_grow, p->d_G, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb);
     if (mode == SYMX_HIP_EVAL_PGH) p->h_valid = true;
     HIP_TRY(ctx, hipGetLastError());
     double e = 0.0;
@@ -953,7 +1017,7 @@ int symx_hip_problem_project_selective(symx_hip_problem p, double eps, int32_t m
     }
     unsigned long long* cnt = (unsigned long long*)(ctx->d_scalars + 8);  // [changed, newly projected, active blocks]
     HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), ctx->stream));
-    launch_ppn_select(ctx->stream, p->d_goff, p->d_glist, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb, threshold, pr, p->d_projected, p->d_select, cnt + 2);
+    launch_ppn_select(ctx->stream, p->d_goff, p->d_glist, p->d_grow, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb, threshold, pr, p->d_projected, p->d_select, cnt + 2);
     for (auto& pot : p->pots)
         launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters, p->d_select + pot.e_base, cnt);
     launch_ppn_commit(ctx->stream, p->d_projected, p->d_select, p->total_elem, cnt + 1);
@@ -981,7 +1045,7 @@ int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
             if (!rc) rc = dalloc(ctx, &p->d_dinv64, (size_t)S.n_rows * 9);
             if (rc) return rc;
         }
-        launch_bsr_assemble(ctx->stream, S, p->d_coff, p->d_clist, p->d_H, p->d_val64, nullptr);
+        launch_bsr_assemble(ctx->stream, S, p->d_aslot, p->d_coff, p->d_clist, p->d_H, p->d_val64, nullptr);
         launch_dinv(ctx->stream, S, p->d_val64, nullptr, p->d_dinv64, nullptr);
     }
     else if (precision == SYMX_HIP_MIXED) {
@@ -998,7 +1062,7 @@ int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
             if (!rc) rc = dalloc(ctx, &p->f_b, nd);
             if (rc) return rc;
         }
-        launch_bsr_assemble(ctx->stream, S, p->d_coff, p->d_clist, p->d_H, nullptr, p->d_val32);
+        launch_bsr_assemble(ctx->stream, S, p->d_aslot, p->d_coff, p->d_clist, p->d_H, nullptr, p->d_val32);
         launch_dinv(ctx->stream, S, nullptr, p->d_val32, nullptr, p->d_dinv32);
     }
     else {
@@ -1388,7 +1452,7 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
             return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist, ctx->d_counters, nullptr, nullptr) ? SYMX_HIP_ERR_ARG : 0;
         }
         case 2:
-            launch_bsr_assemble(ctx->stream, S, p->d_coff, p->d_clist, p->d_H, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
+            launch_bsr_assemble(ctx->stream, S, p->d_aslot, p->d_coff, p->d_clist, p->d_H, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
                                 arg == SYMX_HIP_FP64 ? nullptr : p->d_val32);
             return 0;
         case 3:
