@@ -1173,9 +1173,9 @@ namespace symx_hip
                 for (int k2 = 0; k2 < 9; k2++) acc[k2] += h[k2];
             }
         }
-        TS* dst = val + (int64_t)(slot >> 6) * 576 + 3 * (slot & 63);
+        TS* dst = val + (int64_t)(slot >> 6) * 576 + (slot & 63);
 #pragma unroll
-        for (int k = 0; k < 9; k++) dst[192 * (k / 3) + k % 3] = (TS)acc[k];
+        for (int k = 0; k < 9; k++) dst[64 * k] = (TS)acc[k];
     }
 
     void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* aslot, const uint32_t* coff, const uint32_t* clist,
@@ -1196,8 +1196,8 @@ namespace symx_hip
         if (pos >= n_rows) return;
         const int64_t row = row_at ? row_at[pos] : pos;
         const int64_t base = slice_ptr[pos >> 6];
-        const TS* m = val + (base >> 6) * 576 + 3 * (pos & 63);  // j = 0 is the diagonal block
-        const TS m00 = m[0], m01 = m[1], m02 = m[2], m11 = m[192 + 1], m12 = m[192 + 2], m22 = m[384 + 2];
+        const TS* m = val + (base >> 6) * 576 + (pos & 63);  // j = 0 is the diagonal block
+        const TS m00 = m[0], m01 = m[64], m02 = m[128], m11 = m[256], m12 = m[320], m22 = m[512];
         const TS c00 = m11 * m22 - m12 * m12;
         const TS c01 = m02 * m12 - m01 * m22;
         const TS c02 = m01 * m12 - m11 * m02;
@@ -1362,7 +1362,7 @@ namespace symx_hip
             const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
             const int lane = (int)(pos & 63);
             const CT* cp = col + b0 + lane;
-            const TS* vp = val + b0 * 9 + 3 * lane;
+            const TS* vp = val + b0 * 9 + lane;
             TV a0 = (TV)0, a1 = (TV)0, a2 = (TV)0;
             // large matrices stream through once per iteration: non-temporal loads keep them
             // from evicting the gathered vector entries (12% faster per PCG iteration on a
@@ -1374,9 +1374,9 @@ namespace symx_hip
                 const TV x1 = z[c + 1] + tb * p_old[c + 1];
                 const TV x2 = z[c + 2] + tb * p_old[c + 2];
                 const TS* v = vp + (int64_t)j * 576;
-                a0 += (TV)mload<STREAM>(v + 0) * x0 + (TV)mload<STREAM>(v + 1) * x1 + (TV)mload<STREAM>(v + 2) * x2;
-                a1 += (TV)mload<STREAM>(v + 192) * x0 + (TV)mload<STREAM>(v + 193) * x1 + (TV)mload<STREAM>(v + 194) * x2;
-                a2 += (TV)mload<STREAM>(v + 384) * x0 + (TV)mload<STREAM>(v + 385) * x1 + (TV)mload<STREAM>(v + 386) * x2;
+                a0 += (TV)mload<STREAM>(v + 0) * x0 + (TV)mload<STREAM>(v + 64) * x1 + (TV)mload<STREAM>(v + 128) * x2;
+                a1 += (TV)mload<STREAM>(v + 192) * x0 + (TV)mload<STREAM>(v + 256) * x1 + (TV)mload<STREAM>(v + 320) * x2;
+                a2 += (TV)mload<STREAM>(v + 384) * x0 + (TV)mload<STREAM>(v + 448) * x1 + (TV)mload<STREAM>(v + 512) * x2;
             }
             const int64_t kk = 3 * row;
             const TV p0 = z[kk + 0] + tb * p_old[kk + 0];
@@ -1531,16 +1531,16 @@ namespace symx_hip
             const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
             const int lane = (int)(pos & 63);
             const CT* cp = col + b0 + lane;
-            const float* vp = val + b0 * 9 + 3 * lane;
+            const float* vp = val + b0 * 9 + lane;
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll 4
             for (int j = 0; j < wd; j++) {
                 const int64_t c = 3 * block_col(cp[j * 64], row);
                 const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
                 const float* v = vp + (int64_t)j * 576;
-                a0 += (double)mload<STREAM>(v + 0) * x0 + (double)mload<STREAM>(v + 1) * x1 + (double)mload<STREAM>(v + 2) * x2;
-                a1 += (double)mload<STREAM>(v + 192) * x0 + (double)mload<STREAM>(v + 193) * x1 + (double)mload<STREAM>(v + 194) * x2;
-                a2 += (double)mload<STREAM>(v + 384) * x0 + (double)mload<STREAM>(v + 385) * x1 + (double)mload<STREAM>(v + 386) * x2;
+                a0 += (double)mload<STREAM>(v + 0) * x0 + (double)mload<STREAM>(v + 64) * x1 + (double)mload<STREAM>(v + 128) * x2;
+                a1 += (double)mload<STREAM>(v + 192) * x0 + (double)mload<STREAM>(v + 256) * x1 + (double)mload<STREAM>(v + 320) * x2;
+                a2 += (double)mload<STREAM>(v + 384) * x0 + (double)mload<STREAM>(v + 448) * x1 + (double)mload<STREAM>(v + 512) * x2;
             }
             const int64_t k = 3 * row;
             const double e0 = b[k] - a0, e1 = b[k + 1] - a1, e2 = b[k + 2] - a2;

@@ -1108,7 +1108,7 @@ int symx_hip_problem_export_bsr(symx_hip_problem p, int64_t* nnzb, int64_t* row_
             if (cols) cols[k] = cj.first;
             if (vals) {
                 const int64_t t = base + 64 * (int64_t)cj.second;
-                for (int q = 0; q < 9; q++) vals[k * 9 + q] = v[(t >> 6) * 576 + 192 * (q / 3) + 3 * (t & 63) + q % 3];
+                for (int q = 0; q < 9; q++) vals[k * 9 + q] = v[((t >> 6) * 9 + q) * 64 + (t & 63)];
             }
             k++;
         }
