@@ -305,16 +305,34 @@ def test_unstructured_numbering_gives_the_same_step(gpu, precision):
     offsets no longer fit 16 bits (int32 column ids) and rows of different length
     interleave (SELL rows sorted by length inside windows). One implicit step must match
     the structured numbering's vertex for vertex, to the Newton tolerance."""
-    out = []
+    out, mats = [], []
     for renumber in (0, 5):
         prm = symx_amd.default_block_params(cells=[32, 32, 32], noise=0.2 / 32, seed=3, renumber_seed=renumber)
         s = symx_amd.Tet4Block(prm, symx_amd.default_newton_params(precision=precision))
+        X = s.get("X")
+        order = np.lexsort((X[:, 2], X[:, 1], X[:, 0]))  # grid order, independent of numbering
+        rank = np.empty_like(order)
+        rank[order] = np.arange(order.size)
+        if precision == symx_amd.FP64:
+            # assembled matrix in grid numbering: same contributions in the same order per
+            # slot, so equal bit for bit between the two numberings
+            s.set_dofs(s.get("x"))
+            s.eval(symx_amd.EVAL_PGH)
+            s.project()
+            s.assemble(symx_amd.FP64)
+            rp, cols, vals = s.export_bsr()
+            rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+            key = rank[rows].astype(np.int64) * order.size + rank[cols]
+            srt = np.argsort(key)
+            mats.append((key[srt], vals[srt]))
         x_start = s.get("x")
         st = s.step()
-        X, x = s.get("X"), s.get("x")
-        order = np.lexsort((X[:, 2], X[:, 1], X[:, 0]))  # grid order, independent of numbering
+        x = s.get("x")
         out.append((st, x[order], x_start[order]))
         s.close()
+    if mats:
+        assert np.array_equal(mats[0][0], mats[1][0])
+        assert np.array_equal(mats[0][1], mats[1][1])
     (st0, x0, xs0), (st1, x1, xs1) = out
     assert np.array_equal(xs0, xs1)  # same geometry after the renumbering
     assert st0["result_name"] == st1["result_name"] == "Successful"
