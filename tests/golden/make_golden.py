@@ -120,4 +120,4 @@ if __name__ == "__main__":
     point_triangle_cases()
     newton_case("block6", (6, 6, 6), 0.2 / 6, 11, 1e-6, 1e-3)
     newton_case("block6_tight", (6, 6, 6), 0.2 / 6, 11, 1e-8, 1e-10)
-    newton_case("cfg1", (20, 20, 20), 0.01, 1, 1e-6, 1e-3)
+    newton_case("block20", (20, 20, 20), 0.01, 1, 1e-6, 1e-3)

@@ -205,7 +205,7 @@ def test_mixed_precision_solve_meets_the_same_stopping_rule(gpu, oracle):
         assert err < rel * 1.05 + 1e-7, (rel, err)
 
 
-@pytest.mark.parametrize("name", ["block6", "block6_tight", "cfg1", "block6_pdn", "block6_ppn"])
+@pytest.mark.parametrize("name", ["block6", "block6_tight", "block20", "block6_pdn", "block6_ppn"])
 def test_newton_matches_reference_fixture(gpu, oracle, name):
     f = np.load(os.path.join(GOLDEN, f"newton_ref_{name}.npz"))
     cells = tuple(int(c) for c in f["cells"])
