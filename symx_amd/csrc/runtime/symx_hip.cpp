@@ -796,6 +796,9 @@ int symx_hip_problem_finalize(symx_hip_problem p)
         coff_t[t + 1] = coff_t[t] + (soff[sl + 1] - soff[sl]);
         std::copy(clist.begin() + soff[sl], clist.begin() + soff[sl + 1], clist_t.begin() + coff_t[t]);
     }
+    std::vector<uint32_t>().swap(clist);  // the slot-order lists are not needed any more
+    std::vector<uint32_t>().swap(slot_of);
+    std::vector<uint32_t>().swap(glist);
 
     // Upload.
     int rc = 0;
