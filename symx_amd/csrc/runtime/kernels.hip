@@ -77,6 +77,8 @@ namespace symx_hip
     // workgroups; 512 threads keep 8 waves per SIMD in flight (the kernel needs ~56
     // VGPRs). A streaming read reaches 7.3 TB/s with 8 waves per SIMD and 5.5 TB/s with 4.
     constexpr int kSpmvThreads = 512;
+    // Mean blocks per row from which the search direction gets a pass of its own.
+    constexpr int kDirectionPassMeanRow = 32;
     // Matrices above this many value bytes are streamed (the MI355X infinity cache is 256 MiB).
     constexpr double kStreamBytes = 128.0 * 1024 * 1024;
 
@@ -1289,20 +1291,18 @@ namespace symx_hip
         }
     }
 
-    template <typename TS, typename TV, bool STREAM, typename CT, int NTS>
-    __global__ __launch_bounds__(NTS) void k_pcg_spmv(int32_t n_rows, int32_t k, const int64_t* __restrict__ slice_ptr,
-                                                     const int32_t* __restrict__ row_at, const CT* __restrict__ col,
-                                                     const TS* __restrict__ val,
-                                                     const TV* __restrict__ z, const TV* __restrict__ p_old, TV* __restrict__ p_new,
-                                                     TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_prev,
-                                                     const double* __restrict__ pa_prev2, double* __restrict__ pb)
+    // Start of PCG iteration k: rr, rz of iteration k-1 from the partials, the convergence
+    // test of solve_pcg.h:200-210 for iteration k-1 (for k = 1 the early exits of :116-154),
+    // beta = rz / rz_prev; workgroup 0 publishes the state. Returns true when the solve is
+    // done (every workgroup computes the same values from the same partials).
+    template <int NTS>
+    __device__ __forceinline__ bool pcg_prologue(int32_t k, PcgState* st, const double* __restrict__ pa_prev,
+                                                 const double* __restrict__ pa_prev2, double* lds, double& beta)
     {
-        __shared__ double lds[NTS / 64];
-        if (st->done) return;
         const int64_t G = gridDim.x;
         const double rr = sum_partials<NTS>(pa_prev, G, 2, 0, lds);
         const double rz = sum_partials<NTS>(pa_prev, G, 2, 1, lds);
-        double beta = 0.0;
+        beta = 0.0;
         int done = 0, converged = 0, it = k - 1;
         double err = 0.0;
         if (k == 1) {
@@ -1350,7 +1350,43 @@ namespace symx_hip
                 st->converged = converged;
             }
         }
-        if (done) return;
+        return done != 0;
+    }
+
+    // p_new = z + beta p_old over all rows, after the iteration prologue (same grid as the
+    // SpMV, whose partials the prologue reads).
+    template <typename TV, int NTS>
+    __global__ __launch_bounds__(NTS) void k_pcg_direction(int32_t n_rows, int32_t k, const TV* __restrict__ z,
+                                                           const TV* __restrict__ p_old, TV* __restrict__ p_new, PcgState* st,
+                                                           const double* __restrict__ pa_prev, const double* __restrict__ pa_prev2)
+    {
+        __shared__ double lds[NTS / 64];
+        if (st->done) return;
+        double beta;
+        if (pcg_prologue<NTS>(k, st, pa_prev, pa_prev2, lds, beta)) return;
+        const TV tb = (TV)beta;
+        const int64_t n = 3 * (int64_t)n_rows;
+        for (int64_t i = (int64_t)blockIdx.x * NTS + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTS)
+            __builtin_nontemporal_store(z[i] + tb * p_old[i], p_new + i);
+    }
+
+    // PRE: the search direction p_new = z + beta p_old was formed by k_pcg_direction, so
+    // the gathers read p_new only (long rows, where the gathers of z and p_old cost ~20%).
+    template <typename TS, typename TV, bool STREAM, typename CT, int NTS, bool PRE>
+    __global__ __launch_bounds__(NTS) void k_pcg_spmv(int32_t n_rows, int32_t k, const int64_t* __restrict__ slice_ptr,
+                                                     const int32_t* __restrict__ row_at, const CT* __restrict__ col,
+                                                     const TS* __restrict__ val,
+                                                     const TV* __restrict__ z, const TV* __restrict__ p_old, TV* __restrict__ p_new,
+                                                     TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_prev,
+                                                     const double* __restrict__ pa_prev2, double* __restrict__ pb)
+    {
+        __shared__ double lds[NTS / 64];
+        if (st->done) return;
+        double beta = 0.0;
+        if constexpr (!PRE) {
+            if (pcg_prologue<NTS>(k, st, pa_prev, pa_prev2, lds, beta)) return;
+        }
+        const int64_t G = gridDim.x;
         const TV tb = (TV)beta;
         const int64_t w = xcd_remap();
         int64_t r0, r1;
@@ -1370,25 +1406,43 @@ namespace symx_hip
 #pragma unroll 4
             for (int j = 0; j < wd; j++) {
                 const int64_t c = 3 * block_col(cp[j * 64], row);
-                const TV x0 = z[c + 0] + tb * p_old[c + 0];
-                const TV x1 = z[c + 1] + tb * p_old[c + 1];
-                const TV x2 = z[c + 2] + tb * p_old[c + 2];
+                TV x0, x1, x2;
+                if constexpr (PRE) {
+                    x0 = p_new[c + 0];
+                    x1 = p_new[c + 1];
+                    x2 = p_new[c + 2];
+                }
+                else {
+                    x0 = z[c + 0] + tb * p_old[c + 0];
+                    x1 = z[c + 1] + tb * p_old[c + 1];
+                    x2 = z[c + 2] + tb * p_old[c + 2];
+                }
                 const TS* v = vp + (int64_t)j * 576;
                 a0 += (TV)mload<STREAM>(v + 0) * x0 + (TV)mload<STREAM>(v + 64) * x1 + (TV)mload<STREAM>(v + 128) * x2;
                 a1 += (TV)mload<STREAM>(v + 192) * x0 + (TV)mload<STREAM>(v + 256) * x1 + (TV)mload<STREAM>(v + 320) * x2;
                 a2 += (TV)mload<STREAM>(v + 384) * x0 + (TV)mload<STREAM>(v + 448) * x1 + (TV)mload<STREAM>(v + 512) * x2;
             }
             const int64_t kk = 3 * row;
-            const TV p0 = z[kk + 0] + tb * p_old[kk + 0];
-            const TV p1 = z[kk + 1] + tb * p_old[kk + 1];
-            const TV p2 = z[kk + 2] + tb * p_old[kk + 2];
+            TV p0, p1, p2;
+            if constexpr (PRE) {
+                p0 = p_new[kk + 0];
+                p1 = p_new[kk + 1];
+                p2 = p_new[kk + 2];
+            }
+            else {
+                p0 = z[kk + 0] + tb * p_old[kk + 0];
+                p1 = z[kk + 1] + tb * p_old[kk + 1];
+                p2 = z[kk + 2] + tb * p_old[kk + 2];
+            }
             // non-temporal stores: the vectors go out to memory as they are written instead
             // of staying dirty in the XCD's L2 until the end-of-kernel release, which on the
             // next kernel's critical path writes them back (PCG iteration 0.153 -> 0.148 ms
             // on the 6M-tet block)
-            __builtin_nontemporal_store(p0, p_new + kk + 0);
-            __builtin_nontemporal_store(p1, p_new + kk + 1);
-            __builtin_nontemporal_store(p2, p_new + kk + 2);
+            if constexpr (!PRE) {
+                __builtin_nontemporal_store(p0, p_new + kk + 0);
+                __builtin_nontemporal_store(p1, p_new + kk + 1);
+                __builtin_nontemporal_store(p2, p_new + kk + 2);
+            }
             __builtin_nontemporal_store(a0, q + kk + 0);
             __builtin_nontemporal_store(a1, q + kk + 1);
             __builtin_nontemporal_store(a2, q + kk + 2);
@@ -1457,27 +1511,33 @@ namespace symx_hip
 
     int pcg_grid(int32_t n_rows) { return reduction_groups(n_rows); }
 
-    // Instantiation by matrix size (streaming loads) and column-id width.
+    // Instantiation by matrix size (streaming loads), column-id width and direction pass.
+    template <typename TS, typename TV, bool STREAM, typename CT>
+    void launch_spmv_c(hipStream_t s, const SellPattern& P, const CT* col, bool pre, int G, int32_t k, const TS* val, const TV* z,
+                       const TV* p_old, TV* p_new, TV* q, PcgState* st, const double* pa_prev, const double* pa_prev2, double* pb)
+    {
+        if (pre)
+            k_pcg_spmv<TS, TV, STREAM, CT, kSpmvThreads, true><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, col, val, z,
+                                                                                         p_old, p_new, q, st, pa_prev, pa_prev2, pb);
+        else
+            k_pcg_spmv<TS, TV, STREAM, CT, kSpmvThreads, false><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, col, val, z,
+                                                                                          p_old, p_new, q, st, pa_prev, pa_prev2, pb);
+    }
     template <typename TS, typename TV>
     void launch_spmv(hipStream_t s, const SellPattern& P, int G, int32_t k, const TS* val, const TV* z, const TV* p_old, TV* p_new,
                      TV* q, PcgState* st, const double* pa_prev, const double* pa_prev2, double* pb)
     {
+        // long rows (27-node hexahedra): form the direction first so the SpMV gathers one vector
+        const bool pre = P.nnzb >= (int64_t)kDirectionPassMeanRow * P.n_rows;
+        if (pre) k_pcg_direction<TV, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, z, p_old, p_new, st, pa_prev, pa_prev2);
         const bool stream = (double)P.n_slots * 9 * sizeof(TS) > kStreamBytes;
         if (P.d_dcol) {
-            if (stream)
-                k_pcg_spmv<TS, TV, true, int16_t, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, z, p_old, p_new, q,
-                                                                   st, pa_prev, pa_prev2, pb);
-            else
-                k_pcg_spmv<TS, TV, false, int16_t, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_dcol, val, z, p_old, p_new,
-                                                                    q, st, pa_prev, pa_prev2, pb);
+            if (stream) launch_spmv_c<TS, TV, true>(s, P, P.d_dcol, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
+            else launch_spmv_c<TS, TV, false>(s, P, P.d_dcol, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
         }
         else {
-            if (stream)
-                k_pcg_spmv<TS, TV, true, int32_t, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new, q,
-                                                                   st, pa_prev, pa_prev2, pb);
-            else
-                k_pcg_spmv<TS, TV, false, int32_t, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, P.d_col, val, z, p_old, p_new,
-                                                                    q, st, pa_prev, pa_prev2, pb);
+            if (stream) launch_spmv_c<TS, TV, true>(s, P, P.d_col, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
+            else launch_spmv_c<TS, TV, false>(s, P, P.d_col, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
         }
     }
 
