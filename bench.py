@@ -29,7 +29,7 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the default
 # workload (101^3 block, mixed), each counter scaled by its reading for a 1 GiB device copy
 # (--calibration-copy-mib): FETCH x 2.0, WRITE x 1.0. The per-kernel table is in profiles/.
-MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 759.9e6}
+MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 770.3e6}
 
 
 def block_params(cells):
