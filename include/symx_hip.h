@@ -170,7 +170,8 @@ int symx_hip_problem_element_outputs(symx_hip_problem prob, int32_t pot_id, doub
 /* Project every element Hessian to PD (project_to_PD.cpp:13-82 via
  * ElementHessians::project_to_PD_inplace__all, ElementHessians.cpp:48-67):
  * eigenvalues < eps become eps (clamp) or -lambda (mirror). *n_changed
- * receives the number of matrices that were rebuilt (may be NULL). */
+ * receives the number of matrices that were rebuilt; with n_changed NULL the
+ * call only enqueues the work on the context's stream and returns at once. */
 int symx_hip_problem_project(symx_hip_problem prob, double eps, int32_t mirror, int64_t* n_changed);
 /* Progressive projection step (NewtonsMethod.cpp:306-330 with
  * ElementHessians::project_to_PD_for_update__selectively, ElementHessians.cpp:68-165):
@@ -183,7 +184,9 @@ int symx_hip_problem_project_selective(symx_hip_problem prob, double eps, int32_
 /* Assemble element Hessians into the device block matrix in FP64 or FP32
  * storage (ElementHessians::assemble_global, ElementHessians.cpp:224-256) and
  * prepare the block-Jacobi preconditioner (BlockedSparseMatrix.h:1147-1214).
- * Sums run in a fixed order: results are bitwise repeatable. */
+ * Sums run in a fixed order: results are bitwise repeatable. The work is
+ * enqueued on the context's stream (later calls on the problem are ordered
+ * after it). */
 int symx_hip_problem_assemble(symx_hip_problem prob, int32_t precision);
 /* Export the assembled matrix as block-CSR (row offsets [n_blocks+1], block
  * columns [nnzb], values [nnzb][9] row-major, FP64) - used by tests. Any

@@ -724,7 +724,7 @@ namespace symx
                     case ProjectionToPD::ProjectedNewton:
                     case ProjectionToPD::ProjectOnDemand:
                         if (settings.projection_mode == ProjectionToPD::ProjectedNewton || pdn_countdown > 0) {
-                            check(symx_hip_problem_project(prob, settings.projection_eps, mirror, &changed), hip, "projection");
+                            check(symx_hip_problem_project(prob, settings.projection_eps, mirror, nullptr), hip, "projection");
                             all_projected = true;
                             n_projected_now = n_hessians;
                         }

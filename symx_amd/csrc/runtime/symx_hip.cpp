@@ -973,8 +973,9 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
     symx_hip_ctx ctx = p->ctx;
     if (!p->h_valid) return fail(ctx, SYMX_HIP_ERR_ARG, "no element Hessians (evaluate with EVAL_PGH first)");
-    unsigned long long* d_count = (unsigned long long*)(ctx->d_scalars + 8);
-    HIP_TRY(ctx, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), ctx->stream));
+    // without n_changed nothing is counted and the call returns without waiting for the device
+    unsigned long long* d_count = n_changed ? (unsigned long long*)(ctx->d_scalars + 8) : nullptr;
+    if (d_count) HIP_TRY(ctx, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), ctx->stream));
     for (auto& pot : p->pots) {
         // elements of a conditional potential that are inactive have no Hessian to project
         const uint8_t* only = pot.kC.empty() ? nullptr : pot.d_active;
@@ -982,11 +983,12 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
             return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports 1-4, 8, 10 and 27 DoF blocks per element");
     }
     HIP_TRY(ctx, hipGetLastError());
+    if (!n_changed) return SYMX_HIP_OK;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, d_count, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     unsigned long long c;
     std::memcpy(&c, ctx->h_pinned, 8);
-    if (n_changed) *n_changed = (int64_t)c;
+    *n_changed = (int64_t)c;
     return SYMX_HIP_OK;
 }
 
@@ -1071,8 +1073,7 @@ int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
     else {
         return fail(ctx, SYMX_HIP_ERR_ARG, "bad precision");
     }
-    HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipGetLastError());  // stream-ordered: the solve that follows does not wait for the host
     p->assembled_precision = precision;
     return SYMX_HIP_OK;
 }
