@@ -136,6 +136,12 @@ struct symx_hip_problem_s
     uint8_t* d_flags = nullptr;   // per-element scratch of the projection
     uint32_t* d_plist = nullptr;  // per-element scratch: elements left to the fallback projection
     bool h_valid = false;
+    // max|grad| computed with the gradient by EVAL_PG / EVAL_PGH (read back with the energy),
+    // valid until the gradient vector is written otherwise; unused once a raw device
+    // pointer to the gradient has been handed out
+    bool grad_max_valid = false;
+    bool grad_ptr_exposed = false;
+    double grad_max = 0.0;
     int64_t total_elem = 0;
     // progressive projection: elements projected since the last EVAL_PGH, and the
     // selection of the current call (problem-wide element numbering; allocated on first use)
@@ -194,7 +200,7 @@ namespace symx_hip
     // projected |= select, select = 0; adds the number of newly projected elements to n_new.
     void launch_ppn_commit(hipStream_t s, uint8_t* projected, uint8_t* select, int64_t n, unsigned long long* n_new);
     void launch_fill_u8(hipStream_t s, uint8_t* p, int64_t n);
-    // out (one uint64, preset to the bits of 1.0) = min over blocks of the plane step bound
+    // out (one uint64, set to the bits of 1.0 first) = min over blocks of the plane step bound
     void launch_plane_max_step(hipStream_t s, const int32_t* blocks, int32_t n, const double* x, const double* du, int32_t axis, double plane,
                                double eta, unsigned long long* out);
     // select (may be NULL): only elements with select[e] != 0 are projected.

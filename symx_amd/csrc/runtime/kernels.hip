@@ -1085,10 +1085,13 @@ namespace symx_hip
         for (int off = 32; off > 0; off >>= 1) t = fmin(t, __shfl_xor(t, off, 64));
         if ((threadIdx.x & 63) == 0 && t < 1.0) atomicMin(out, (unsigned long long)__double_as_longlong(t));
     }
+    __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
     void launch_plane_max_step(hipStream_t s, const int32_t* blocks, int32_t n, const double* x, const double* du, int32_t axis, double plane,
                                double eta, unsigned long long* out)
     {
         if (n <= 0) return;
+        // start from the bits of 1.0 (set on the stream: no host round trip)
+        k_set_u64<<<1, 1, 0, s>>>(out, 0x3FF0000000000000ull);
         k_plane_max_step<<<(unsigned)((n + NT - 1) / NT), NT, 0, s>>>(blocks, n, x, du, axis, plane, eta, out);
     }
 

@@ -891,6 +891,7 @@ int symx_hip_problem_upload(symx_hip_problem p, int32_t vec, const double* host)
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
     double* d = vec_ptr(p, vec);
     if (!d || !host) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id or null pointer");
+    if (vec == SYMX_HIP_VEC_GRAD) p->grad_max_valid = false;
     HIP_TRY(p->ctx, hipMemcpyAsync(d, host, sizeof(double) * p->n_dofs, hipMemcpyHostToDevice, p->ctx->stream));
     HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));
     return SYMX_HIP_OK;
@@ -900,6 +901,10 @@ int symx_hip_problem_device_ptr(symx_hip_problem p, int32_t vec, double** dptr)
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
     double* d = vec_ptr(p, vec);
     if (!d || !dptr) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id or null pointer");
+    if (vec == SYMX_HIP_VEC_GRAD) {
+        p->grad_ptr_exposed = true;  // the caller may write it: no cached max|grad| from here on
+        p->grad_max_valid = false;
+    }
     *dptr = d;
     return SYMX_HIP_OK;
 }
@@ -938,13 +943,21 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
     }
     if (mode == SYMX_HIP_EVAL_PGH && p->d_projected) launch_fill_u8(ctx->stream, p->d_projected, p->total_elem);
     launch_sum(ctx->stream, p->d_Epart, p->total_parts, ctx->d_scalars);
-    if (mode >= SYMX_HIP_EVAL_PG) launch_grad_assemble(ctx->stream, p->d_goff, p->d_glist, p->d_grow, p->d_G, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb);
+    const bool with_grad = mode >= SYMX_HIP_EVAL_PG;
+    if (with_grad) {
+        launch_grad_assemble(ctx->stream, p->d_goff, p->d_glist, p->d_grow, p->d_G, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb);
+        // max|grad| (the Newton residual) comes back with the energy: one host round trip
+        launch_max_abs(ctx->stream, p->n_dofs, p->d_vec[SYMX_HIP_VEC_GRAD], ctx->d_scalars + 1, ctx->d_partials);
+    }
     if (mode == SYMX_HIP_EVAL_PGH) p->h_valid = true;
     HIP_TRY(ctx, hipGetLastError());
-    double e = 0.0;
-    int rc = sync_scalar(ctx, ctx->d_scalars, &e);
-    if (rc) return rc;
-    if (E) *E = e;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_scalars, sizeof(double) * (with_grad ? 2 : 1), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (with_grad) {
+        p->grad_max = ctx->h_pinned[1];
+        p->grad_max_valid = !p->grad_ptr_exposed;
+    }
+    if (E) *E = ctx->h_pinned[0];
     return SYMX_HIP_OK;
 }
 
@@ -1235,6 +1248,10 @@ int symx_hip_problem_max_abs(symx_hip_problem p, int32_t vec, double* out)
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
     double* d = vec_ptr(p, vec);
     if (!d) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
+    if (vec == SYMX_HIP_VEC_GRAD && p->grad_max_valid) {
+        *out = p->grad_max;  // computed by the evaluation that wrote the gradient
+        return SYMX_HIP_OK;
+    }
     launch_max_abs(p->ctx->stream, p->n_dofs, d, p->ctx->d_scalars, p->ctx->d_partials);
     return sync_scalar(p->ctx, p->ctx->d_scalars, out);
 }
@@ -1254,6 +1271,7 @@ int symx_hip_problem_axpy(symx_hip_problem p, int32_t dst, int32_t a, double s, 
     double* da = vec_ptr(p, a);
     double* db = vec_ptr(p, b);
     if (!dd || !da || !db) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
+    if (dst == SYMX_HIP_VEC_GRAD) p->grad_max_valid = false;
     launch_axpy(p->ctx->stream, p->n_dofs, dd, da, s, db);
     HIP_TRY(p->ctx, hipGetLastError());
     return SYMX_HIP_OK;
@@ -1264,6 +1282,7 @@ int symx_hip_problem_scale(symx_hip_problem p, int32_t dst, int32_t a, double s)
     double* dd = vec_ptr(p, dst);
     double* da = vec_ptr(p, a);
     if (!dd || !da) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id");
+    if (dst == SYMX_HIP_VEC_GRAD) p->grad_max_valid = false;
     launch_scale(p->ctx->stream, p->n_dofs, dd, da, s);
     HIP_TRY(p->ctx, hipGetLastError());
     return SYMX_HIP_OK;
@@ -1351,8 +1370,6 @@ int symx_hip_problem_plane_max_step(symx_hip_problem p, double eta, double* t_ma
     *t_max = 1.0;
     if (p->n_filter_blocks == 0) return SYMX_HIP_OK;
     unsigned long long* d = (unsigned long long*)(ctx->d_scalars + 12);
-    const double one = 1.0;
-    HIP_TRY(ctx, hipMemcpyAsync(d, &one, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     launch_plane_max_step(ctx->stream, p->d_filter_blocks, p->n_filter_blocks, p->d_vec[SYMX_HIP_VEC_DOFS], p->d_vec[SYMX_HIP_VEC_DU],
                           p->filter_axis, p->filter_plane, eta, d);
     HIP_TRY(ctx, hipGetLastError());
