@@ -1134,6 +1134,25 @@ int symx_hip_problem_export_bsr(symx_hip_problem p, int64_t* nnzb, int64_t* row_
     return SYMX_HIP_OK;
 }
 
+// Iterations to enqueue before the next look at the PCG state: the remaining count at the
+// mean rate so far (err/err0 per iteration, geometric) plus two, within [4, 256]. Iterations
+// past convergence exit at once on the device, so the guess only trades launches against
+// host round trips; the iterates themselves do not depend on it.
+static int next_pcg_batch(const PcgState& h, int batch, double abs_tol, double rel_tol)
+{
+    double next = 2.0 * batch;
+    if (h.it > 0 && h.err0 > 0.0 && h.err > 0.0 && std::isfinite(h.err)) {
+        const double ratio = h.err / h.err0;
+        const double target = std::max(rel_tol, abs_tol / h.err0);
+        if (ratio < 1.0 && target < ratio) {
+            const double n = h.it * std::log(target / ratio) / std::log(ratio);
+            if (std::isfinite(n)) next = n + 2.0;
+        }
+    }
+    next = std::min(next, (double)std::max(h.max_iter - h.it, 1));
+    return (int)std::max(4.0, std::min(256.0, std::ceil(next)));
+}
+
 static int run_pcg(symx_hip_problem p, bool fp32, const double* b64, double* x64, double abs_tol, double rel_tol, int32_t max_iter,
                    int32_t stop_on_indef, PcgState& out)
 {
@@ -1163,7 +1182,7 @@ static int run_pcg(symx_hip_problem p, bool fp32, const double* b64, double* x64
         HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pcg, ctx->d_pcg, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->h_pcg->done) break;
-        batch = std::min(batch * 2, 64);
+        batch = next_pcg_batch(*ctx->h_pcg, batch, abs_tol, rel_tol);
     }
     out = *ctx->h_pcg;
     return SYMX_HIP_OK;
