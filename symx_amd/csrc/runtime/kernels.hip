@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "internal.h"
 #include "projection.h"
@@ -132,6 +133,66 @@ namespace symx_hip
         if (threadIdx.x < NT)
             for (int64_t i = threadIdx.x; i < G; i += NT) acc += p[i * stride + off];
         return block_sum<NTS>(acc, lds);
+    }
+
+    // K sums of G <= MAX_GROUPS partials at once, out[c] = sum of p[c][i * stride[c]], each
+    // formed in the order of sum_partials (bitwise the same value). All loads are issued
+    // before the first add and the K workgroup reductions share one pair of barriers, so
+    // the latency is that of one sum (lds: K * NTS / 64 doubles).
+    constexpr int kPartialsPerThread = (MAX_GROUPS + NT - 1) / NT;
+    template <int NTS, int K>
+    __device__ inline void sum_partials_k(const double* const (&p)[K], const int (&stride)[K], int64_t G, double* lds, double (&out)[K])
+    {
+        double v[K][kPartialsPerThread];
+#pragma unroll
+        for (int c = 0; c < K; c++)
+#pragma unroll
+            for (int u = 0; u < kPartialsPerThread; u++) {
+                const int64_t i = threadIdx.x + (int64_t)u * NT;
+                v[c][u] = (threadIdx.x < NT && i < G) ? p[c][i * stride[c]] : 0.0;
+            }
+        double acc[K];
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            acc[c] = 0.0;
+#pragma unroll
+            for (int u = 0; u < kPartialsPerThread; u++)
+                if (threadIdx.x < NT && threadIdx.x + (int64_t)u * NT < G) acc[c] += v[c][u];
+            acc[c] = wave_sum(acc[c]);
+        }
+        const int w = threadIdx.x >> 6;
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int c = 0; c < K; c++) lds[c * (NTS / 64) + w] = acc[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            double r = 0.0;
+            for (int i = 0; i < NTS / 64; i++) r += lds[c * (NTS / 64) + i];
+            out[c] = r;
+        }
+    }
+
+    // K workgroup sums with one pair of barriers, each bitwise equal to block_sum's
+    // (lds: K * NTS / 64 doubles).
+    template <int NTS, int K>
+    __device__ inline void block_sum_k(double (&v)[K], double* lds)
+    {
+#pragma unroll
+        for (int c = 0; c < K; c++) v[c] = wave_sum(v[c]);
+        const int w = threadIdx.x >> 6;
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int c = 0; c < K; c++) lds[c * (NTS / 64) + w] = v[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            double t = 0.0;
+            for (int i = 0; i < NTS / 64; i++) t += lds[c * (NTS / 64) + i];
+            v[c] = t;
+        }
     }
 
     // ---- plain sums ----------------------------------------------------------------
@@ -1303,8 +1364,11 @@ namespace symx_hip
                                                  const double* __restrict__ pa_prev2, double* lds, double& beta)
     {
         const int64_t G = gridDim.x;
-        const double rr = sum_partials<NTS>(pa_prev, G, 2, 0, lds);
-        const double rz = sum_partials<NTS>(pa_prev, G, 2, 1, lds);
+        // rr, rz of iteration k-1 and rz of k-2 (unused for k = 1: pa_prev2 is then scratch)
+        double sums[3];
+        sum_partials_k<NTS, 3>({ pa_prev, pa_prev + 1, pa_prev2 + 1 }, { 2, 2, 2 }, G, lds, sums);
+        const double rr = sums[0];
+        const double rz = sums[1];
         beta = 0.0;
         int done = 0, converged = 0, it = k - 1;
         double err = 0.0;
@@ -1340,7 +1404,7 @@ namespace symx_hip
                 done = 1;
             }
             else {
-                beta = rz / sum_partials<NTS>(pa_prev2, G, 2, 1, lds);
+                beta = rz / sums[2];
             }
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1356,21 +1420,61 @@ namespace symx_hip
         return done != 0;
     }
 
+    // 16-byte vectors of the PCG vector passes
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    template <typename T> struct Vec16;
+    template <> struct Vec16<float> { using type = f32x4; static constexpr int width = 4; };
+    template <> struct Vec16<double> { using type = f64x2; static constexpr int width = 2; };
+
     // p_new = z + beta p_old over all rows, after the iteration prologue (same grid as the
-    // SpMV, whose partials the prologue reads).
-    template <typename TV, int NTS>
+    // SpMV, whose partials the prologue reads). VEC: 16-byte accesses (the three vectors are
+    // 16-byte aligned); the first kDirectionAhead vectors of each thread are loaded before
+    // the prologue, whose partial sums would otherwise stall every workgroup first.
+    constexpr int kDirectionAhead = 4;
+    template <typename TV, int NTS, bool VEC>
     __global__ __launch_bounds__(NTS) void k_pcg_direction(int32_t n_rows, int32_t k, const TV* __restrict__ z,
                                                            const TV* __restrict__ p_old, TV* __restrict__ p_new, PcgState* st,
                                                            const double* __restrict__ pa_prev, const double* __restrict__ pa_prev2)
     {
-        __shared__ double lds[NTS / 64];
+        __shared__ double lds[3 * NTS / 64];
         if (st->done) return;
+        using V = typename Vec16<TV>::type;
+        constexpr int W = VEC ? Vec16<TV>::width : 1;
+        using L = typename std::conditional<VEC, V, TV>::type;
+        const int64_t n = 3 * (int64_t)n_rows;
+        const int64_t nv = n / W;
+        const int64_t step = (int64_t)gridDim.x * NTS;
+        const int64_t i0 = (int64_t)blockIdx.x * NTS + threadIdx.x;
+        const L* zv = reinterpret_cast<const L*>(z);
+        const L* pv = reinterpret_cast<const L*>(p_old);
+        L* out = reinterpret_cast<L*>(p_new);
+        L za[kDirectionAhead], pa[kDirectionAhead];
+#pragma unroll
+        for (int u = 0; u < kDirectionAhead; u++) {
+            const int64_t i = i0 + u * step;
+            if (i < nv) {
+                za[u] = zv[i];
+                pa[u] = pv[i];
+            }
+        }
         double beta;
         if (pcg_prologue<NTS>(k, st, pa_prev, pa_prev2, lds, beta)) return;
         const TV tb = (TV)beta;
-        const int64_t n = 3 * (int64_t)n_rows;
-        for (int64_t i = (int64_t)blockIdx.x * NTS + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTS)
-            __builtin_nontemporal_store(z[i] + tb * p_old[i], p_new + i);
+        // fma(beta, p_old, z): the contraction of z + beta * p_old elsewhere
+#pragma unroll
+        for (int u = 0; u < kDirectionAhead; u++) {
+            const int64_t i = i0 + u * step;
+            if (i < nv) __builtin_nontemporal_store(__builtin_elementwise_fma((L)tb, pa[u], za[u]), out + i);
+        }
+        for (int64_t i = i0 + kDirectionAhead * step; i < nv; i += step)
+            __builtin_nontemporal_store(__builtin_elementwise_fma((L)tb, pv[i], zv[i]), out + i);
+        if constexpr (VEC) {
+            if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
+                const int64_t e = nv * W + threadIdx.x;
+                p_new[e] = __builtin_elementwise_fma(tb, p_old[e], z[e]);
+            }
+        }
     }
 
     // PRE: the search direction p_new = z + beta p_old was formed by k_pcg_direction, so
@@ -1383,7 +1487,7 @@ namespace symx_hip
                                                      TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_prev,
                                                      const double* __restrict__ pa_prev2, double* __restrict__ pb)
     {
-        __shared__ double lds[NTS / 64];
+        __shared__ double lds[3 * NTS / 64];
         if (st->done) return;
         double beta = 0.0;
         if constexpr (!PRE) {
@@ -1455,17 +1559,41 @@ namespace symx_hip
         if (threadIdx.x == 0) pb[w] = s;
     }
 
+    // Rows loaded by each thread of k_pcg_update before its partial sums (two rows per
+    // thread cover a 1024-row chunk of a 512-thread workgroup).
+    constexpr int kUpdateAhead = 2;
     template <typename TV, int NTS>
     __global__ __launch_bounds__(NTS) void k_pcg_update(int32_t n_rows, int32_t k, const TV* __restrict__ dinv, TV* __restrict__ x,
                                                        TV* __restrict__ r, TV* __restrict__ z, const TV* __restrict__ p,
                                                        const TV* __restrict__ q, PcgState* st, const double* __restrict__ pa_cur,
                                                        double* __restrict__ pa_next, const double* __restrict__ pb)
     {
-        __shared__ double lds[NTS / 64];
+        __shared__ double lds[2 * NTS / 64];
         if (st->done) return;
         const int64_t G = gridDim.x;
-        const double pAp = sum_partials<NTS>(pb, G, 1, 0, lds);
-        const double rz = sum_partials<NTS>(pa_cur, G, 2, 1, lds);
+        const int64_t w = xcd_remap();
+        int64_t r0, r1;
+        row_chunk(n_rows, w, G, r0, r1);
+        TV xa[kUpdateAhead][3], ra[kUpdateAhead][3], pa[kUpdateAhead][3], qa[kUpdateAhead][3], da[kUpdateAhead][9];
+#pragma unroll
+        for (int u = 0; u < kUpdateAhead; u++) {
+            const int64_t row = r0 + threadIdx.x + u * NTS;
+            if (row < r1) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    xa[u][i] = x[3 * row + i];
+                    ra[u][i] = r[3 * row + i];
+                    pa[u][i] = p[3 * row + i];
+                    qa[u][i] = q[3 * row + i];
+                }
+#pragma unroll
+                for (int i = 0; i < 9; i++) da[u][i] = dinv[i * (int64_t)n_rows + row];
+            }
+        }
+        double sums[2];
+        sum_partials_k<NTS, 2>({ pb, pa_cur + 1 }, { 1, 2 }, G, lds, sums);
+        const double pAp = sums[0];
+        const double rz = sums[1];
         const double alpha = rz / pAp;
         if (pAp <= 0.0) {
             if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1483,32 +1611,49 @@ namespace symx_hip
             st->alpha = alpha;
         }
         const TV ta = (TV)alpha;
-        const int64_t w = xcd_remap();
-        int64_t r0, r1;
-        row_chunk(n_rows, w, G, r0, r1);
         double rr = 0.0, rzn = 0.0;
-        for (int64_t row = r0 + threadIdx.x; row < r1; row += NTS) {
+        // x += alpha p, r -= alpha q, z = D^-1 r (non-temporal stores as in k_pcg_spmv)
+        auto finish = [&](int64_t row, const TV (&xv)[3], const TV (&rv0)[3], const TV (&pv)[3], const TV (&qv)[3], const TV (&d)[9]) {
             const int64_t kk = 3 * row;
             TV rv[3], zv[3];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                __builtin_nontemporal_store(x[kk + i] + ta * p[kk + i], x + kk + i);  // as in k_pcg_spmv
-                rv[i] = r[kk + i] - ta * q[kk + i];
+                __builtin_nontemporal_store(xv[i] + ta * pv[i], x + kk + i);
+                rv[i] = rv0[i] - ta * qv[i];
                 __builtin_nontemporal_store(rv[i], r + kk + i);
             }
-            apply_dinv(dinv, n_rows, row, rv, zv);
+#pragma unroll
+            for (int i = 0; i < 3; i++) zv[i] = d[3 * i + 0] * rv[0] + d[3 * i + 1] * rv[1] + d[3 * i + 2] * rv[2];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 __builtin_nontemporal_store(zv[i], z + kk + i);
                 rr += (double)rv[i] * (double)rv[i];
                 rzn += (double)rv[i] * (double)zv[i];
             }
+        };
+#pragma unroll
+        for (int u = 0; u < kUpdateAhead; u++) {
+            const int64_t row = r0 + threadIdx.x + u * NTS;
+            if (row < r1) finish(row, xa[u], ra[u], pa[u], qa[u], da[u]);
         }
-        const double srr = block_sum<NTS>(rr, lds);
-        const double srz = block_sum<NTS>(rzn, lds);
+        for (int64_t row = r0 + threadIdx.x + kUpdateAhead * NTS; row < r1; row += NTS) {
+            TV xv[3], rv[3], pv[3], qv[3], d[9];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                xv[i] = x[3 * row + i];
+                rv[i] = r[3 * row + i];
+                pv[i] = p[3 * row + i];
+                qv[i] = q[3 * row + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 9; i++) d[i] = dinv[i * (int64_t)n_rows + row];
+            finish(row, xv, rv, pv, qv, d);
+        }
+        double red[2] = { rr, rzn };
+        block_sum_k<NTS, 2>(red, lds);
         if (threadIdx.x == 0) {
-            pa_next[2 * w + 0] = srr;
-            pa_next[2 * w + 1] = srz;
+            pa_next[2 * w + 0] = red[0];
+            pa_next[2 * w + 1] = red[1];
         }
     }
 
@@ -1535,7 +1680,12 @@ namespace symx_hip
         // cache-resident systems the extra launch costs more than the gathers it saves.
         const bool stream = (double)P.n_slots * 9 * sizeof(TS) > kStreamBytes;
         const bool pre = stream || P.nnzb >= (int64_t)kDirectionPassMeanRow * P.n_rows;
-        if (pre) k_pcg_direction<TV, kSpmvThreads><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, z, p_old, p_new, st, pa_prev, pa_prev2);
+        if (pre) {
+            if ((((uintptr_t)z | (uintptr_t)p_old | (uintptr_t)p_new) & 15) == 0)
+                k_pcg_direction<TV, kSpmvThreads, true><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, z, p_old, p_new, st, pa_prev, pa_prev2);
+            else
+                k_pcg_direction<TV, kSpmvThreads, false><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, z, p_old, p_new, st, pa_prev, pa_prev2);
+        }
         if (P.d_dcol) {
             if (stream) launch_spmv_c<TS, TV, true>(s, P, P.d_dcol, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
             else launch_spmv_c<TS, TV, false>(s, P, P.d_dcol, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
