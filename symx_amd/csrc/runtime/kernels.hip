@@ -1509,8 +1509,13 @@ namespace symx_hip
             TV a0 = (TV)0, a1 = (TV)0, a2 = (TV)0;
             // large matrices stream through once per iteration: non-temporal loads keep them
             // from evicting the gathered vector entries (12% faster per PCG iteration on a
-            // 6M-tet block; 40% slower on a cache-resident 50k-DoF system, hence STREAM)
-#pragma unroll 4
+            // 6M-tet block; 40% slower on a cache-resident 50k-DoF system, hence STREAM).
+            // Unroll by 4 only for FP32 blocks with the direction pass (they lose bandwidth
+            // below it); elsewhere 2: FP64 with the pass at 80 instead of 113 VGPRs (6 waves
+            // per SIMD) made the FP64 PCG iteration 3% (6M-tet block) to 6% (50k-DoF
+            // cantilever) faster, FP32 without it 5% faster (200k-DoF ball).
+            constexpr int kUnroll = (sizeof(TS) == 4 && PRE) ? 4 : 2;
+#pragma unroll kUnroll
             for (int j = 0; j < wd; j++) {
                 const int64_t c = 3 * block_col(cp[j * 64], row);
                 TV x0, x1, x2;
