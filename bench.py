@@ -29,7 +29,7 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the default
 # workload (101^3 block, mixed), each counter scaled by its reading for a 1 GiB device copy
 # (--calibration-copy-mib): FETCH x 2.0, WRITE x 1.0. The per-kernel table is in profiles/.
-MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 770.3e6}
+MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 757.9e6}
 
 
 def block_params(cells):
@@ -48,8 +48,9 @@ def algorithmic_bytes(n_elem, n_vert, nnzb, precision):
         # read + write of the 78 unique entries (FP64 projection)
         "projection": 2 * 78 * 8,
         # per PCG iteration: matrix + column ids (16-bit offsets from the row, which every mesh
-        # here fits) + slice offsets + D^-1 + 12 vector passes
-        "pcg_iteration": nnzb * (9 * sH + 2) + (n_vert + 1) * 4 + n_vert * 9 * sH + 12 * 3 * n_vert * sH,
+        # here fits) + slice offsets + D^-1 (6 distinct values of each symmetric block) + 12
+        # vector passes
+        "pcg_iteration": nnzb * (9 * sH + 2) + (n_vert + 1) * 4 + n_vert * 6 * sH + 12 * 3 * n_vert * sH,
         # per assembly: element Hessians (10 upper blocks incl. slot ids) + matrix write
         "assembly": n_elem * (78 * 8 + 10 * 4) + nnzb * 9 * sH,
     }

@@ -1253,7 +1253,9 @@ namespace symx_hip
     }
 
     // Closed-form inverse of the symmetric diagonal block of each row, computed in
-    // storage precision like BlockedSparseMatrix.h:1198-1214. Layout: dinv[k*n_rows+row].
+    // storage precision like BlockedSparseMatrix.h:1198-1214. The inverse is symmetric by
+    // construction, so its six distinct values are stored: dinv[s*n_rows+row] holds entry
+    // 0, 1, 2, 4, 5, 8 (s = 0..5) of the row-major 3x3 inverse.
     template <typename TS>
     __global__ __launch_bounds__(NT) void k_dinv(int32_t n_rows, const int64_t* __restrict__ slice_ptr,
                                                  const int32_t* __restrict__ row_at, const TS* __restrict__ val, TS* __restrict__ dinv)
@@ -1281,11 +1283,24 @@ namespace symx_hip
         else {
             inv[0] = inv[1] = inv[2] = inv[4] = inv[5] = inv[8] = (TS)0;
         }
-        inv[3] = inv[1];
-        inv[6] = inv[2];
-        inv[7] = inv[5];
+        dinv[0 * (int64_t)n_rows + row] = inv[0];
+        dinv[1 * (int64_t)n_rows + row] = inv[1];
+        dinv[2 * (int64_t)n_rows + row] = inv[2];
+        dinv[3 * (int64_t)n_rows + row] = inv[4];
+        dinv[4 * (int64_t)n_rows + row] = inv[5];
+        dinv[5 * (int64_t)n_rows + row] = inv[8];
+    }
+
+    // The full row-major 3x3 block of row `row` from the six stored values.
+    template <typename T>
+    __device__ __forceinline__ void load_dinv(const T* __restrict__ dinv, int64_t n_rows, int64_t row, T (&d)[9])
+    {
+        T s[6];
 #pragma unroll
-        for (int k = 0; k < 9; k++) dinv[(int64_t)k * n_rows + row] = inv[k];
+        for (int i = 0; i < 6; i++) s[i] = dinv[i * n_rows + row];
+        d[0] = s[0], d[1] = s[1], d[2] = s[2];
+        d[3] = s[1], d[4] = s[3], d[5] = s[4];
+        d[6] = s[2], d[7] = s[4], d[8] = s[5];
     }
 
     void launch_dinv(hipStream_t s, const SellPattern& P, const double* v64, const float* v32, double* dinv64, float* dinv32)
@@ -1314,9 +1329,11 @@ namespace symx_hip
     template <typename T>
     __device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int64_t n_rows, int64_t row, const T (&rv)[3], T (&zv)[3])
     {
+        T d[9];
+        load_dinv(dinv, n_rows, row, d);
 #pragma unroll
         for (int i = 0; i < 3; i++)
-            zv[i] = dinv[(3 * i + 0) * n_rows + row] * rv[0] + dinv[(3 * i + 1) * n_rows + row] * rv[1] + dinv[(3 * i + 2) * n_rows + row] * rv[2];
+            zv[i] = d[3 * i + 0] * rv[0] + d[3 * i + 1] * rv[1] + d[3 * i + 2] * rv[2];
     }
 
     template <typename T>
@@ -1591,8 +1608,7 @@ namespace symx_hip
                     pa[u][i] = p[3 * row + i];
                     qa[u][i] = q[3 * row + i];
                 }
-#pragma unroll
-                for (int i = 0; i < 9; i++) da[u][i] = dinv[i * (int64_t)n_rows + row];
+                load_dinv(dinv, n_rows, row, da[u]);
             }
         }
         double sums[2];
@@ -1650,8 +1666,7 @@ namespace symx_hip
                 pv[i] = p[3 * row + i];
                 qv[i] = q[3 * row + i];
             }
-#pragma unroll
-            for (int i = 0; i < 9; i++) d[i] = dinv[i * (int64_t)n_rows + row];
+            load_dinv(dinv, n_rows, row, d);
             finish(row, xv, rv, pv, qv, d);
         }
         double red[2] = { rr, rzn };

@@ -1060,7 +1060,7 @@ int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
     if (precision == SYMX_HIP_FP64) {
         if (!p->d_val64) {
             int rc = dalloc(ctx, &p->d_val64, nval);
-            if (!rc) rc = dalloc(ctx, &p->d_dinv64, (size_t)S.n_rows * 9);
+            if (!rc) rc = dalloc(ctx, &p->d_dinv64, (size_t)S.n_rows * 6);
             if (rc) return rc;
         }
         launch_bsr_assemble(ctx->stream, S, p->d_aslot, p->d_coff, p->d_clist, p->d_H, p->d_val64, nullptr);
@@ -1069,7 +1069,7 @@ int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
     else if (precision == SYMX_HIP_MIXED) {
         if (!p->d_val32) {
             int rc = dalloc(ctx, &p->d_val32, nval);
-            if (!rc) rc = dalloc(ctx, &p->d_dinv32, (size_t)S.n_rows * 9);
+            if (!rc) rc = dalloc(ctx, &p->d_dinv32, (size_t)S.n_rows * 6);
             const size_t nd = (size_t)p->n_dofs;
             if (!rc) rc = dalloc(ctx, &p->f_x, nd);
             if (!rc) rc = dalloc(ctx, &p->f_r, nd);
