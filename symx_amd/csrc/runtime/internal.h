@@ -161,7 +161,9 @@ struct symx_hip_problem_s
 
     // block matrix
     symx_hip::SellPattern sell;
-    uint32_t* d_aslot = nullptr;  // [n_slots] slots in assembly traversal order
+    uint32_t* d_aslot = nullptr;  // [n_assemble] assembled slots in traversal order
+    uint32_t* d_mirror = nullptr; // [n_assemble] slot that receives the transposed block, ~0u for none; null: no mirroring
+    int64_t n_assemble = 0;
     uint32_t* d_coff = nullptr;   // [n_slots + 1] contribution lists in that order
     uint32_t* d_clist = nullptr;  // Hessian block id * 2 + transpose flag
     double* d_val64 = nullptr;
@@ -208,8 +210,9 @@ namespace symx_hip
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
                           uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count);
     int launch_project_dense(hipStream_t s, double* mats, int64_t n_mats, int32_t n, double eps, int32_t mirror, uint8_t* changed);
-    void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* aslot, const uint32_t* coff, const uint32_t* clist,
-                             const double* H, double* v64, float* v32);
+    // n entries of aslot/coff; mirror (may be NULL): slot that also receives the transposed block
+    void launch_bsr_assemble(hipStream_t s, int64_t n, const uint32_t* aslot, const uint32_t* mirror, const uint32_t* coff,
+                             const uint32_t* clist, const double* H, double* v64, float* v32);
     void launch_dinv(hipStream_t s, const SellPattern& P, const double* v64, const float* v32, double* dinv64, float* dinv32);
     // PCG on the sell matrix (iteration k is 1-based). partials: >= 5 * reduction_groups(n_rows) doubles.
     void launch_pcg_init64(hipStream_t s, const SellPattern& P, const double* dinv, const double* b, double* x, double* r, double* z,

@@ -1214,8 +1214,9 @@ namespace symx_hip
     // traversal order, whole SELL slices at a time).
     template <typename TS>
     __global__ __launch_bounds__(NT) void k_bsr_assemble(int64_t n_slots, const uint32_t* __restrict__ aslot,
-                                                         const uint32_t* __restrict__ coff, const uint32_t* __restrict__ clist,
-                                                         const double* __restrict__ H, TS* __restrict__ val)
+                                                         const uint32_t* __restrict__ mirror, const uint32_t* __restrict__ coff,
+                                                         const uint32_t* __restrict__ clist, const double* __restrict__ H,
+                                                         TS* __restrict__ val)
     {
         const int64_t t = xcd_remap() * NT + threadIdx.x;
         if (t >= n_slots) return;
@@ -1242,14 +1243,26 @@ namespace symx_hip
         TS* dst = val + (int64_t)(slot >> 6) * 576 + (slot & 63);
 #pragma unroll
         for (int k = 0; k < 9; k++) dst[64 * k] = (TS)acc[k];
+        // the transposed slot sums the same element blocks transposed, in the same order
+        // (checked when the pattern is built), so its sum is this one transposed bit for bit
+        if (mirror) {
+            const uint32_t m = mirror[t];
+            if (m != ~0u) {
+                TS* md = val + (int64_t)(m >> 6) * 576 + (m & 63);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) md[64 * (3 * i + j)] = (TS)acc[3 * j + i];
+            }
+        }
     }
 
-    void launch_bsr_assemble(hipStream_t s, const SellPattern& P, const uint32_t* aslot, const uint32_t* coff, const uint32_t* clist,
-                             const double* H, double* v64, float* v32)
+    void launch_bsr_assemble(hipStream_t s, int64_t n, const uint32_t* aslot, const uint32_t* mirror, const uint32_t* coff,
+                             const uint32_t* clist, const double* H, double* v64, float* v32)
     {
-        if (P.n_slots <= 0) return;
-        if (v64) k_bsr_assemble<double><<<grid_for(P.n_slots), NT, 0, s>>>(P.n_slots, aslot, coff, clist, H, v64);
-        if (v32) k_bsr_assemble<float><<<grid_for(P.n_slots), NT, 0, s>>>(P.n_slots, aslot, coff, clist, H, v32);
+        if (n <= 0) return;
+        if (v64) k_bsr_assemble<double><<<grid_for(n), NT, 0, s>>>(n, aslot, mirror, coff, clist, H, v64);
+        if (v32) k_bsr_assemble<float><<<grid_for(n), NT, 0, s>>>(n, aslot, mirror, coff, clist, H, v32);
     }
 
     // Closed-form inverse of the symmetric diagonal block of each row, computed in

@@ -439,6 +439,7 @@ void symx_hip_problem_destroy(symx_hip_problem p)
     dfree(p->d_glist);
     dfree(p->d_grow);
     dfree(p->d_aslot);
+    dfree(p->d_mirror);
     dfree(p->sell.d_slice_ptr);
     dfree(p->sell.d_col);
     dfree(p->sell.d_dcol);
@@ -790,8 +791,56 @@ int symx_hip_problem_finalize(symx_hip_problem p)
         goff_t[t + 1] = goff_t[t] + (goff[r + 1] - goff[r]);
         std::copy(glist.begin() + goff[r], glist.begin() + goff[r + 1], glist_t.begin() + goff_t[t]);
     }
-    std::vector<uint32_t> coff_t((size_t)S.n_slots + 1, 0), clist_t(clist.size());
-    for (int64_t t = 0; t < S.n_slots; t++) {
+    // Mirrored assembly: the slot of block (c, r) lists the element blocks of (r, c) in the
+    // same (potential, element) order with the transpose flag flipped, so its sum is the
+    // transposed sum of (r, c) bit for bit. The upper slot (c > r) then forms the block once
+    // and stores it to both, and each element block is read once instead of twice. Every
+    // pair of lists is checked; if one does not mirror exactly, every slot sums its own list.
+    std::vector<uint32_t> mirror;
+    {
+        bool ok = true;
+        std::vector<uint32_t> m_of(S.n_slots, ~0u);
+        std::vector<uint8_t> lower(S.n_slots, 0);
+        for (int32_t sl = 0; sl < S.n_slices && ok; sl++) {
+            for (int64_t t = S.h_slice_ptr[sl]; t < S.h_slice_ptr[sl + 1] && ok; t++) {
+                const int64_t o = t - S.h_slice_ptr[sl];
+                const int32_t q = 64 * sl + (int32_t)(o & 63);
+                const int32_t j = (int32_t)(o >> 6);
+                if (q >= nb) continue;
+                const int32_t r = S.h_row_at.empty() ? q : S.h_row_at[q];
+                if (j == 0 || j >= row_len[r]) continue;  // diagonal or padding: its own list
+                const int32_t c = S.h_col[t];
+                if (c < r) {
+                    lower[t] = 1;
+                    continue;
+                }
+                const int64_t m = find_slot(c, r);
+                const uint32_t n0 = soff[t + 1] - soff[t];
+                if (m < 0 || m >= S.n_slots || S.h_col[m] != r || soff[m + 1] - soff[m] != n0) {
+                    ok = false;
+                    break;
+                }
+                for (uint32_t k = 0; k < n0 && ok; k++)
+                    if (clist[soff[m] + k] != (clist[soff[t] + k] ^ 1u)) ok = false;
+                m_of[t] = (uint32_t)m;
+            }
+        }
+        if (ok) {
+            std::vector<uint32_t> kept;
+            kept.reserve(aslot.size());
+            for (uint32_t sl : aslot)
+                if (!lower[sl]) {
+                    kept.push_back(sl);
+                    mirror.push_back(m_of[sl]);
+                }
+            aslot.swap(kept);
+        }
+    }
+    p->n_assemble = (int64_t)aslot.size();
+    std::vector<uint32_t> coff_t(aslot.size() + 1, 0), clist_t;
+    for (size_t t = 0; t < aslot.size(); t++) coff_t[t + 1] = coff_t[t] + (soff[aslot[t] + 1] - soff[aslot[t]]);
+    clist_t.resize(coff_t.back());
+    for (size_t t = 0; t < aslot.size(); t++) {
         const uint32_t sl = aslot[t];
         coff_t[t + 1] = coff_t[t] + (soff[sl + 1] - soff[sl]);
         std::copy(clist.begin() + soff[sl], clist.begin() + soff[sl + 1], clist_t.begin() + coff_t[t]);
@@ -815,6 +864,7 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     rc = rc ? rc : dalloc(ctx, &p->d_glist, std::max<size_t>(glist_t.size(), 1));
     rc = rc ? rc : dalloc(ctx, &p->d_grow, trav.size());
     rc = rc ? rc : dalloc(ctx, &p->d_aslot, std::max<size_t>(aslot.size(), 1));
+    if (!mirror.empty()) rc = rc ? rc : dalloc(ctx, &p->d_mirror, mirror.size());
     rc = rc ? rc : dalloc(ctx, &S.d_slice_ptr, S.h_slice_ptr.size());
     // Column offsets from the row fit 16 bits for meshes numbered with some locality
     // (structured grids up to ~180^2 vertices per layer); the SpMV then reads 2 bytes
@@ -855,6 +905,7 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     if (!glist_t.empty()) HIP_TRY(ctx, hipMemcpy(p->d_glist, glist_t.data(), glist_t.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(ctx, hipMemcpy(p->d_grow, trav.data(), trav.size() * 4, hipMemcpyHostToDevice));
     if (!aslot.empty()) HIP_TRY(ctx, hipMemcpy(p->d_aslot, aslot.data(), aslot.size() * 4, hipMemcpyHostToDevice));
+    if (p->d_mirror) HIP_TRY(ctx, hipMemcpy(p->d_mirror, mirror.data(), mirror.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(ctx, hipMemcpy(S.d_slice_ptr, S.h_slice_ptr.data(), S.h_slice_ptr.size() * 8, hipMemcpyHostToDevice));
     if (S.d_col) HIP_TRY(ctx, hipMemcpy(S.d_col, S.h_col.data(), S.h_col.size() * 4, hipMemcpyHostToDevice));
     if (S.d_dcol) HIP_TRY(ctx, hipMemcpy(S.d_dcol, dcol.data(), dcol.size() * 2, hipMemcpyHostToDevice));
@@ -1063,7 +1114,7 @@ int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
             if (!rc) rc = dalloc(ctx, &p->d_dinv64, (size_t)S.n_rows * 6);
             if (rc) return rc;
         }
-        launch_bsr_assemble(ctx->stream, S, p->d_aslot, p->d_coff, p->d_clist, p->d_H, p->d_val64, nullptr);
+        launch_bsr_assemble(ctx->stream, p->n_assemble, p->d_aslot, p->d_mirror, p->d_coff, p->d_clist, p->d_H, p->d_val64, nullptr);
         launch_dinv(ctx->stream, S, p->d_val64, nullptr, p->d_dinv64, nullptr);
     }
     else if (precision == SYMX_HIP_MIXED) {
@@ -1080,7 +1131,7 @@ int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
             if (!rc) rc = dalloc(ctx, &p->f_b, nd);
             if (rc) return rc;
         }
-        launch_bsr_assemble(ctx->stream, S, p->d_aslot, p->d_coff, p->d_clist, p->d_H, nullptr, p->d_val32);
+        launch_bsr_assemble(ctx->stream, p->n_assemble, p->d_aslot, p->d_mirror, p->d_coff, p->d_clist, p->d_H, nullptr, p->d_val32);
         launch_dinv(ctx->stream, S, nullptr, p->d_val32, nullptr, p->d_dinv32);
     }
     else {
@@ -1492,7 +1543,7 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
             return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist, ctx->d_counters, nullptr, nullptr) ? SYMX_HIP_ERR_ARG : 0;
         }
         case 2:
-            launch_bsr_assemble(ctx->stream, S, p->d_aslot, p->d_coff, p->d_clist, p->d_H, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
+            launch_bsr_assemble(ctx->stream, p->n_assemble, p->d_aslot, p->d_mirror, p->d_coff, p->d_clist, p->d_H, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
                                 arg == SYMX_HIP_FP64 ? nullptr : p->d_val32);
             return 0;
         case 3:
