@@ -5,12 +5,13 @@ Workload (default): 101x101x101 vertex grid over [0,1]^3, each hex cell split in
 tets (6,000,000 tets, 3,090,903 DoF), Neo-Hookean E=1e5 Pa, nu=0.4, density 1000,
 gravity -9.81 z, backward-Euler inertia dt=0.01, z=0 face held by a penalty
 (k=1e8), free vertices perturbed by seeded uniform noise of 0.2 cell widths.
-One step = one implicit-Euler time step solved by projected Newton to a 1e-6
-relative residual (block-Jacobi PCG to 1e-3 relative, forcing sequence), followed
-by the velocity/position update. Inputs are resident in HBM before timing starts;
-JIT and topology setup are excluded. `value` = Newton iterations per second of the
-mixed-precision solver (one Newton iteration = element E/grad/Hessian evaluation,
-PD projection, assembly, PCG solve, line search).
+One step = one pass of the workload: 5 implicit-Euler time steps from the seeded state,
+each solved by projected Newton to a 1e-6 relative residual (block-Jacobi PCG to 1e-3
+relative, forcing sequence) and followed by the velocity/position update. Every pass
+starts from the same seeded state. Inputs are resident in HBM before timing starts; JIT
+and topology setup are excluded. `value` = Newton iterations per second of the
+mixed-precision solver (one Newton iteration = element E/grad/Hessian evaluation, PD
+projection, assembly, PCG solve, line search); `time_to_tolerance_s` = seconds per pass.
 """
 import argparse
 import json
@@ -30,6 +31,10 @@ FP64_PEAK_TFLOPS = 78.6
 # workload (101^3 block, mixed), each counter scaled by its reading for a 1 GiB device copy
 # (--calibration-copy-mib): FETCH x 2.0, WRITE x 1.0. The per-kernel table is in profiles/.
 MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 757.9e6}
+# what the mixed path computes in: element kernels and projection in FP64 arithmetic with
+# FP64 element-Hessian storage, FP32 block matrix and inner PCG, FP64 outer refinement,
+# FP64 energies, gradients and line search
+DTYPE_MIXED = "f64 element Hessians, f32 BSR/PCG + f64 refinement"
 
 
 def block_params(cells):
@@ -57,6 +62,7 @@ def algorithmic_bytes(n_elem, n_vert, nnzb, precision):
 
 
 def run_precision(scene, precision, steps, warmup, log, keep_newton=False):
+    """`warmup` untimed then `steps` timed time steps continuing one trajectory (hex27)."""
     import symx_amd
 
     if not keep_newton:
@@ -81,6 +87,87 @@ def run_precision(scene, precision, steps, warmup, log, keep_newton=False):
     return {"seconds": t1 - t0, "newton_iterations": newton, "cg_iterations": cg, "phases": phases, "results": results}
 
 
+class SeededTrajectory:
+    """The block workload of one precision: `time_steps` implicit-Euler steps from the seeded
+    state. Every pass restores x, x0 and v of the seeded state first (outside the timing), so
+    each pass is the same solve from the same inputs."""
+
+    def __init__(self, cells, precision, time_steps):
+        import symx_amd
+
+        self.precision = precision
+        self.time_steps = time_steps
+        self.scene = symx_amd.Tet4Block(block_params(cells), symx_amd.default_newton_params(precision=precision))
+        self.seed_state = {k: self.scene.get(k) for k in ("x", "x0", "v")}
+
+    def run_pass(self):
+        for k, a in self.seed_state.items():
+            self.scene.set(k, a)
+        steps = []
+        newton = cg = 0
+        phases = dict(eval=0.0, project=0.0, assembly=0.0, linear=0.0, energy=0.0)
+        t0 = time.perf_counter()
+        for _ in range(self.time_steps):
+            st = self.scene.step()
+            newton += st["newton_iterations"]
+            cg += st["cg_iterations"]
+            for k in phases:
+                phases[k] += st["seconds_" + k]
+            steps.append({"result": st["result_name"], "newton": st["newton_iterations"], "cg": st["cg_iterations"],
+                          "refinements": st["n_refinements"], "trapezoid": st["ls_trapezoid_accepts"],
+                          "r0": st["first_residual"], "r": st["final_residual"],
+                          "r_over_r0": st["final_residual"] / st["first_residual"] if st["first_residual"] > 0 else 0.0,
+                          "seconds": st["seconds"]})
+        t1 = time.perf_counter()
+        return {"seconds": t1 - t0, "newton_iterations": newton, "cg_iterations": cg, "phases": phases, "steps": steps}
+
+    def run(self, passes, warmup, log, tag):
+        for i in range(warmup):
+            r = self.run_pass()
+            log(f"[{tag}] warmup pass {i}: {r['seconds']:.3f}s " + " ".join(f"{s['result']}/{s['newton']}" for s in r["steps"]))
+        runs = []
+        for i in range(passes):
+            r = self.run_pass()
+            runs.append(r)
+            log(f"[{tag}] pass {i}: {r['seconds']:.3f}s newton={r['newton_iterations']} cg={r['cg_iterations']}")
+        for j, s in enumerate(runs[-1]["steps"]):
+            log(f"[{tag}] time step {j}: {s['result']} newton={s['newton']} cg={s['cg']} refinements={s['refinements']} "
+                f"trapezoid={s['trapezoid']} "
+                f"r0={s['r0']:.3e} r={s['r']:.3e} r/r0={s['r_over_r0']:.2e} {s['seconds']:.3f}s")
+        secs = [r["seconds"] for r in runs]
+        # every pass solves the same inputs with fixed summation orders: the iteration
+        # counts (and the final positions) repeat exactly
+        same = all(r["newton_iterations"] == runs[0]["newton_iterations"] and r["cg_iterations"] == runs[0]["cg_iterations"] for r in runs)
+        return {
+            "seconds": sum(secs),
+            "passes": passes,
+            "median_pass_seconds": float(np.median(secs)),
+            "newton_iterations": sum(r["newton_iterations"] for r in runs),
+            "cg_iterations": sum(r["cg_iterations"] for r in runs),
+            "phases": {k: sum(r["phases"][k] for r in runs) for k in runs[0]["phases"]},
+            "steps": runs[-1]["steps"],
+            "repeatable": same,
+        }
+
+    def positions(self):
+        return self.scene.get("x")
+
+    def close(self):
+        self.scene.close()
+
+
+def step_summary(r):
+    steps = r["steps"]
+    return {
+        "time_steps_successful": sum(s["result"] == "Successful" for s in steps),
+        "time_steps": len(steps),
+        "worst_r_over_r0": max(s["r_over_r0"] for s in steps),
+        "per_time_step": [{"result": s["result"], "newton": s["newton"], "cg": s["cg"], "trapezoid_ls": s["trapezoid"],
+                           "r0": float("%.4e" % s["r0"]),
+                           "r_over_r0": float("%.3e" % s["r_over_r0"])} for s in steps],
+    }
+
+
 def profile_kernels(scene, precision, reps=5):
     """Average kernel durations (ms) with HIP events on the runtime's own stream."""
     import ctypes
@@ -100,35 +187,103 @@ def profile_kernels(scene, precision, reps=5):
     return out
 
 
-def cpu_baseline(log, cells=40):
-    """The C oracle (OpenMP restatement of the reference path) on a bounded sample:
-    one implicit step of a cells^3 block (same material, loads and solver settings)."""
+def host_threads():
+    """(threads used by the CPU legs, physical cores from lscpu). A GPU box gives one GPU a
+    share of 16 host cores; the threads are the CPUs this process may run on, capped there."""
+    import subprocess
+
+    n = len(os.sched_getaffinity(0))
+    threads = max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n)), 16))
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        phys = len({ln for ln in out.splitlines() if ln and not ln.startswith("#")})
+    except (OSError, subprocess.CalledProcessError):
+        pass
+    return threads, phys
+
+
+def reference_solve(S, threads, log):
+    """One Newton solve of scene S by the reference itself (oracle/_ref/ref_driver, SymX
+    compiled from its sources), with its own timer around NewtonsMethod::solve. None when
+    the reference build is absent."""
+    import subprocess
+    import tempfile
+
+    from ref_io import read_solve_output, write_solve_input
+
+    import symx_amd
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.access(exe, os.X_OK):
+        return None
+    newton = symx_amd.NewtonParams(0, 1, 0, 100, 0.0, 1e-6, 1e-3, 1e-12, 10000, 1e-10, 0.0)
+    env = {k: v for k, v in os.environ.items() if k != "CXX"}
+    env.update(OMP_PROC_BIND="close", OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as d:
+        inp, outp = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        write_solve_input(inp, S, newton, threads)
+        r = subprocess.run([exe, "solve", inp, outp], capture_output=True, text=True, env=env)
+        if r.returncode != 0:
+            log(f"reference solve failed ({r.returncode}): {r.stderr[-500:]}")
+            return None
+        return read_solve_output(outp, S.X.shape[0])
+
+
+def cpu_baseline(log, cells, threads):
+    """CPU legs on a bounded sample, one implicit step of a cells^3 block (same material,
+    loads and solver settings): the reference itself (SymX's OpenMP path) and the oracle
+    (this repository's C restatement of it), both on `threads` host threads."""
     from oracle_py import Oracle
     from scene_np import tet4_block
 
     import symx_amd
 
-    orc = Oracle()
     S = tet4_block(cells=(cells,) * 3, noise=0.2 / cells, seed=2)
-    newton = symx_amd.NewtonParams(0, 1, 0, 100, 0.0, 1e-6, 1e-3, 1e-12, 10000, 1e-10, 0.0)
-    t0 = time.perf_counter()
-    res = orc.newton(S, newton)
-    t1 = time.perf_counter()
-    # element Hessians/s of the evaluate + project + assemble unit
-    S2 = tet4_block(cells=(cells,) * 3, noise=0.2 / cells, seed=2)
-    t2 = time.perf_counter()
-    orc.eval_project_assemble(S2, project=True)
-    t3 = time.perf_counter()
     n_elem = S.tets.shape[0]
-    log(f"cpu baseline ({orc.num_threads()} threads): {cells}^3 cells, {n_elem} tets: newton={res['newton_iterations']} "
-        f"cg={res['cg_iterations']} {t1 - t0:.2f}s; eval+project+assemble {t3 - t2:.2f}s")
-    return {
-        "newton_per_s": res["newton_iterations"] / (t1 - t0),
-        "elem_hessians_per_s": n_elem / (t3 - t2),
-        "cores": orc.num_threads(),
-        "n_elem": n_elem,
-        "seconds": t1 - t0,
-    }
+    out = {"n_elem": n_elem, "cells": cells}
+    ref = reference_solve(S, threads, log)
+    if ref is not None:
+        log(f"cpu reference ({threads} threads): {cells}^3 cells, {n_elem} tets: result={ref['result']} newton={ref['newton_iterations']} "
+            f"cg={ref['cg_iterations']} {ref['seconds']:.2f}s")
+        out["reference"] = ref
+    orc = Oracle()
+    newton = symx_amd.NewtonParams(0, 1, 0, 100, 0.0, 1e-6, 1e-3, 1e-12, 10000, 1e-10, 0.0)
+    S2 = tet4_block(cells=(cells,) * 3, noise=0.2 / cells, seed=2)
+    t0 = time.perf_counter()
+    res = orc.newton(S2, newton)
+    t1 = time.perf_counter()
+    log(f"cpu oracle ({orc.num_threads()} threads): newton={res['newton_iterations']} cg={res['cg_iterations']} {t1 - t0:.2f}s")
+    out["port"] = {"newton_iterations": res["newton_iterations"], "cg_iterations": res["cg_iterations"], "seconds": t1 - t0,
+                   "threads": orc.num_threads(), "x": S2.x.copy()}
+    out["S"] = S
+    return out
+
+
+def gpu_sample_errors(cells, cpu, log):
+    """The same one-step sample solved on the GPU in FP64 and mixed precision, against the
+    reference's own positions (and the oracle's): relative to max|x| and to the largest
+    displacement of the solve."""
+    import symx_amd
+
+    S = cpu["S"]
+    ref_x = cpu["reference"]["x"] if "reference" in cpu else None
+    out = {}
+    for name, prec in (("fp64", symx_amd.FP64), ("mixed", symx_amd.MIXED)):
+        s = symx_amd.Tet4Block(block_params(cells), symx_amd.default_newton_params(precision=prec))
+        st = s.solve()
+        x = s.get("x")
+        s.close()
+        e = {"result": st["result_name"], "newton": st["newton_iterations"]}
+        for tag, xr in (("reference", ref_x), ("oracle", cpu["port"]["x"])):
+            if xr is None:
+                continue
+            d = np.abs(x - xr).max()
+            e[f"vs_{tag}_rel_max_x"] = float("%.3e" % (d / np.abs(xr).max()))
+            e[f"vs_{tag}_rel_displacement"] = float("%.3e" % (d / np.abs(xr - S.x).max()))
+        out[name] = e
+        log(f"gpu {name} on the {cells}^3 sample: {e}")
+    return out
 
 
 def bench_adjoint(args, log):
@@ -225,7 +380,7 @@ def bench_adjoint(args, log):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == symx_amd.FP64 else "f32 Hessian/BSR/PCG + f64 refinement",
+        "dtype": "f64" if prec == symx_amd.FP64 else DTYPE_MIXED,
         "data": "synthetic",
         "config": {"workload": "quasistatic Tet4 Neo-Hookean cantilever 64x16x16 vertices (85050 tets, 49152 DoF), per-tet log(mu), "
                                "adjoint of L=0.5|x-x_target|^2, PCG to 1e-10 rel.", "n_elem": ne, "n_dof": 3 * nv, "nnzb": nnzb_c.value},
@@ -315,7 +470,7 @@ def bench_contact(args, log):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 Hessian/BSR/PCG + f64 refinement, f64 energies",
+        "dtype": DTYPE_MIXED,
         "data": "synthetic",
         "config": {"workload": f"Tet4 Neo-Hookean ball r=0.5 ({s.n_elements} tets) dropped at 2 m/s on z=0, IPC barrier dhat=1e-3 kappa=1e6, "
                                "step filter 0.9, dt=1/200, Progressive projection, 1e-5 rel. residual", "n_elem": s.n_elements, "n_dof": 3 * s.n_vertices, "nnzb": nnzb_c.value},
@@ -401,7 +556,7 @@ def bench_hex27(args, log):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 Hessian/BSR/PCG + f64 refinement",
+        "dtype": DTYPE_MIXED,
         "data": "synthetic",
         "config": {"workload": f"Hex27 Neo-Hookean cube {cells}^3 cells ({n_elem} elements, {3 * m['n_vert']} DoF), E log-uniform [1e4,1e6], "
                                "nu=0.45, implicit Euler dt=0.01, ProjectOnDemand, 1e-6 rel. residual", "n_elem": n_elem,
@@ -430,6 +585,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=100)
+    ap.add_argument("--time-steps", type=int, default=5, help="implicit-Euler steps per pass of the block workload")
+    ap.add_argument("--fp64-steps", type=int, default=0, help="timed passes of the FP64 comparison (0: as --steps)")
+    ap.add_argument("--no-north-star", action="store_true", help="skip the 64^3 mixed/FP64 comparison")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cells", type=int, default=40)
@@ -474,29 +632,41 @@ def main():
     if args.workload == "hex27":
         bench_hex27(args, log)
         return
-    cells = args.cells
+    bench_block(args, log)
+
+
+def bench_block(args, log):
+    """The headline workload (see the module docstring): one step = one pass of the 5
+    implicit-Euler time steps from the seeded state, mixed precision timed; FP64 beside it."""
+    import ctypes
+
+    import symx_amd
+
+    cells, T = args.cells, args.time_steps
     t0 = time.perf_counter()
-    scene = symx_amd.Tet4Block(block_params(cells), symx_amd.default_newton_params(precision=symx_amd.MIXED))
+    mixed_tr = SeededTrajectory(cells, symx_amd.MIXED, T)
+    scene = mixed_tr.scene
     log(f"setup {cells}^3 cells: {scene.n_elements} tets, {scene.n_vertices} vertices in {time.perf_counter() - t0:.1f}s "
         f"(JIT + topology {symx_amd.amd().symx_scene_compile_seconds(scene.ptr):.1f}s)")
-    mixed = run_precision(scene, symx_amd.MIXED, args.steps, args.warmup, log)
+    mixed = mixed_tr.run(args.steps, args.warmup, log, "mixed")
+    x_mixed = mixed_tr.positions()
+    X = scene.get("X")
     prof_mixed = profile_kernels(scene, symx_amd.MIXED)
     log(f"kernel ms (mixed): {json.dumps({k: round(v, 4) for k, v in prof_mixed.items()})}")
-    fp64 = None
-    prof64 = None
+    fp64 = prof64 = None
     if not args.no_fp64:
-        scene64 = symx_amd.Tet4Block(block_params(cells), symx_amd.default_newton_params(precision=symx_amd.FP64))
-        fp64 = run_precision(scene64, symx_amd.FP64, args.steps, args.warmup, log)
-        prof64 = profile_kernels(scene64, symx_amd.FP64)
+        tr64 = SeededTrajectory(cells, symx_amd.FP64, T)
+        fp64 = tr64.run(args.fp64_steps if args.fp64_steps > 0 else args.steps, min(args.warmup, 1), log, "fp64")
+        x64 = tr64.positions()
+        prof64 = profile_kernels(tr64.scene, symx_amd.FP64)
         log(f"kernel ms (fp64): {json.dumps({k: round(v, 4) for k, v in prof64.items()})}")
-        scene64.close()
-
-    import ctypes
+        tr64.close()
 
     nnzb_c = ctypes.c_int64()
     symx_amd.check_hip(symx_amd.hip().symx_hip_problem_export_bsr(scene.problem, ctypes.byref(nnzb_c), None, None, None))
     nnzb = nnzb_c.value
     n_elem, n_vert = scene.n_elements, scene.n_vertices
+    mixed_tr.close()
 
     newton_per_s = mixed["newton_iterations"] / mixed["seconds"]
     # Roofline of the dominant kernel of a mixed Newton iteration (by measured share).
@@ -516,34 +686,78 @@ def main():
                 "algorithmic_bytes_per_launch": int(unit_bytes)}
     elem_hessians_per_s = n_elem / (prof_mixed["element"] * 1e-3)
 
+    # north-star size: 64^3 vertices (1,500,282 tets), the same workload once per precision
+    north = None
+    if not args.no_north_star and not args.no_fp64:
+        north = {}
+        for name, prec in (("mixed", symx_amd.MIXED), ("fp64", symx_amd.FP64)):
+            tr = SeededTrajectory(63, prec, T)
+            r = tr.run(1, 1, log, f"64^3 {name}")
+            north[name] = {"seconds": round(r["seconds"], 4), "newton_iterations": r["newton_iterations"], **{
+                k: v for k, v in step_summary(r).items() if k != "per_time_step"}}
+            tr.close()
+        north["fp64_over_mixed_wall"] = round(north["fp64"]["seconds"] / north["mixed"]["seconds"], 3)
+        north["workload"] = f"the same {T} time steps on 64^3 vertices (1500282 tets, 786432 DoF)"
+
+    threads, phys = host_threads()
     cpu = None
+    sample_err = None
     if not args.no_cpu:
-        c = cpu_baseline(log, args.cpu_cells)
+        c = cpu_baseline(log, args.cpu_cells, threads)
         scale = c["n_elem"] / n_elem  # linear extrapolation in element count (optimistic for the CPU)
-        cpu = {"value": round(c["newton_per_s"] * scale, 4), "unit": "Newton iterations/s", "cores": c["cores"], "kind": "port",
-               "sample": f"oracle/symx_oracle.c, one implicit step of a {args.cpu_cells}^3-cell block ({c['n_elem']} tets, "
-                         f"{c['seconds']:.1f}s), scaled linearly by element count to {n_elem} tets",
-               "elem_hessians_per_s": round(c["elem_hessians_per_s"], 1)}
+        port = c["port"]
+        port_rate = port["newton_iterations"] / port["seconds"] * scale
+        if "reference" in c:
+            ref = c["reference"]
+            cpu = {"value": round(ref["newton_iterations"] / ref["seconds"] * scale, 4), "unit": "Newton iterations/s", "cores": threads,
+                   "kind": "reference",
+                   "sample": f"SymX itself (oracle/_ref/ref_driver, built from the reference sources), NewtonsMethod::solve of one implicit "
+                             f"step of a {args.cpu_cells}^3-cell block ({c['n_elem']} tets, {ref['newton_iterations']} Newton iterations, "
+                             f"{ref['seconds']:.2f}s on {threads} threads), scaled linearly by element count to {n_elem} tets",
+                   "lscpu_physical_cores": phys,
+                   "port": {"value": round(port_rate, 4), "seconds": round(port["seconds"], 3), "threads": port["threads"],
+                            "newton_iterations": port["newton_iterations"],
+                            "port_over_reference_time": round(port["seconds"] / ref["seconds"], 3)}}
+        else:
+            cpu = {"value": round(port_rate, 4), "unit": "Newton iterations/s", "cores": port["threads"], "kind": "port",
+                   "sample": f"oracle/symx_oracle.c, one implicit step of a {args.cpu_cells}^3-cell block ({c['n_elem']} tets, "
+                             f"{port['seconds']:.1f}s), scaled linearly by element count to {n_elem} tets",
+                   "lscpu_physical_cores": phys}
+        sample_err = gpu_sample_errors(args.cpu_cells, c, log)
 
     extra = {
         "elem_hessians_per_s": round(elem_hessians_per_s, 1),
-        "time_steps_per_s": round(args.steps / mixed["seconds"], 4),
+        "time_steps_per_s": round(args.steps * T / mixed["seconds"], 4),
+        "time_to_tolerance_s": round(mixed["median_pass_seconds"], 4),
         "newton_iterations": mixed["newton_iterations"],
         "cg_iterations": mixed["cg_iterations"],
+        **step_summary(mixed),
+        "repeatable": mixed["repeatable"],
         "phase_seconds": {k: round(v, 4) for k, v in mixed["phases"].items()},
         "kernel_ms": {k: round(v, 4) for k, v in prof_mixed.items()},
-        "results": mixed["results"],
     }
     if fp64:
         extra["fp64"] = {
             "newton_per_s": round(fp64["newton_iterations"] / fp64["seconds"], 4),
+            "passes": fp64["passes"],
             "seconds": round(fp64["seconds"], 4),
+            "time_to_tolerance_s": round(fp64["median_pass_seconds"], 4),
             "newton_iterations": fp64["newton_iterations"],
             "cg_iterations": fp64["cg_iterations"],
+            **step_summary(fp64),
             "elem_hessians_per_s": round(n_elem / (prof64["element"] * 1e-3), 1),
             "kernel_ms": {k: round(v, 4) for k, v in prof64.items()},
-            "speedup_mixed_over_fp64_wall": round(fp64["seconds"] / mixed["seconds"], 3),
+            "fp64_over_mixed_time_to_tolerance": round(fp64["median_pass_seconds"] / mixed["median_pass_seconds"], 3),
         }
+        d = np.abs(x_mixed - x64).max()
+        extra["error"] = {
+            "mixed_vs_fp64_after_workload_rel_max_x": float("%.3e" % (d / np.abs(x64).max())),
+            "mixed_vs_fp64_after_workload_rel_displacement": float("%.3e" % (d / np.abs(x64 - X).max())),
+        }
+    if sample_err is not None:
+        extra.setdefault("error", {})[f"sample_{args.cpu_cells}cube_one_step"] = sample_err
+    if north is not None:
+        extra["north_star_64cube"] = north
     line = {
         "metric": "Newton steps/s, elem Hessians/s; mixed vs FP64 at 1e-6 residual",
         "value": round(newton_per_s, 4),
@@ -555,16 +769,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 Hessian/BSR/PCG + f64 refinement",
+        "dtype": DTYPE_MIXED,
         "data": "synthetic",
-        "config": {"workload": f"Tet4 Neo-Hookean block {cells + 1}^3 vertices ({n_elem} tets, {3 * n_vert} DoF), implicit Euler dt=0.01, "
-                               "projected Newton to 1e-6 rel. residual", "n_elem": n_elem, "n_dof": 3 * n_vert, "nnzb": nnzb},
+        "config": {"workload": f"Tet4 Neo-Hookean block {cells + 1}^3 vertices ({n_elem} tets, {3 * n_vert} DoF): {T} implicit-Euler steps "
+                               f"(dt=0.01) from the seeded state, projected Newton to 1e-6 rel. residual; one step = that whole pass",
+                   "n_elem": n_elem, "n_dof": 3 * n_vert, "nnzb": nnzb, "time_steps_per_step": T},
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extra,
     }
     print(json.dumps(line), flush=True)
-    scene.close()
 
 
 if __name__ == "__main__":

@@ -57,7 +57,14 @@ enum {
  * Element outputs: E_partial[blockIdx.x] is the FP64 sum over the block's
  * elements (fixed order); G is [n_elem][n_dof]; H holds, per element, the
  * upper 3x3 blocks (a <= b, row-major order of (a,b)) with 9 row-major values
- * each. Inactive elements (active[e] == 0) write zeros. */
+ * each, as doubles, or as floats for the kernels emitted for FP32 Hessian
+ * storage (name suffix "_f32"). Inactive elements (active[e] == 0) write zeros
+ * and have energy 0.
+ * e_mode selects what happens to the element energies E_e:
+ *   SYMX_HIP_EMODE_SUM    E_partial gets the sum of E_e;
+ *   SYMX_HIP_EMODE_STORE  the same, and E_elem[e] = E_e for every e < n_elem;
+ *   SYMX_HIP_EMODE_DELTA  E_partial gets the sum of (E_e - E_elem[e]). */
+enum { SYMX_HIP_EMODE_SUM = 0, SYMX_HIP_EMODE_STORE = 1, SYMX_HIP_EMODE_DELTA = 2 };
 typedef struct symx_hip_eval_params {
     const int32_t* conn;
     const uint8_t* active;
@@ -66,8 +73,9 @@ typedef struct symx_hip_eval_params {
     double* G;
     double* H;
     uint8_t* C_out;
+    double* E_elem;
     int32_t n_elem;
-    int32_t pad_;
+    int32_t e_mode;
 } symx_hip_eval_params;
 
 /* ---- context ---------------------------------------------------------------- */
@@ -161,6 +169,21 @@ int symx_hip_problem_device_ptr(symx_hip_problem prob, int32_t vec, double** dpt
  * assembled into VEC_GRAD for PG and PGH; PGH keeps element Hessians on the
  * device for project / assemble. */
 int symx_hip_problem_eval(symx_hip_problem prob, int32_t mode, double* E);
+/* Energy change since the last EVAL_PG / EVAL_PGH: *dE = sum over potentials and
+ * elements, in a fixed order, of (E_e(current DoFs) - E_e(DoFs of that evaluation)).
+ * This is the quantity the Armijo test of the line search compares with the expected
+ * decrease (E1 < E0 + beta s du.g, NewtonsMethod.cpp:565-591). Forming it element by
+ * element keeps its round-off at the size of the element energies instead of the
+ * total energy's, whose FP64 resolution on millions of elements is of the order of the
+ * decrease that a Newton step near a 1e-6 relative residual predicts. */
+int symx_hip_problem_energy_change(symx_hip_problem prob, double* dE);
+/* *out = g(x) . vec, with g(x) the gradient of the total energy at the current DoFs
+ * (evaluate_P__dP_du, SecondOrderCompiledGlobal.cpp:90-116) assembled into a scratch
+ * vector: VEC_GRAD, the element Hessians and the reference energies of
+ * symx_hip_problem_energy_change are left as they are. The line search uses it for the
+ * trapezoid estimate E(x0 + d) - E(x0) ~ (g(x0) + g(x0 + d)) . d / 2 of steps too short
+ * for energy differences to resolve. */
+int symx_hip_problem_gradient_dot(symx_hip_problem prob, int32_t vec, double* out);
 /* Number of element Hessians produced by the last PGH evaluation. */
 int symx_hip_problem_n_element_hessians(symx_hip_problem prob, int64_t* n);
 /* Copy element outputs of potential pot_id to host (tests): G [n_elem][n_dof],

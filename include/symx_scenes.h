@@ -91,6 +91,7 @@ typedef struct symx_newton_params {
     int32_t cg_max_iterations;
     double projection_eps;
     double step_tolerance;
+    int32_t verbosity;               /* console output of the solve: 0 none, 1 summary, 2 one line per iteration, 3 full */
 } symx_newton_params;
 
 typedef struct symx_solve_stats {
@@ -108,6 +109,8 @@ typedef struct symx_solve_stats {
     double seconds_assembly;
     double seconds_linear;
     double seconds_energy;       /* line-search energies */
+    int32_t ls_trapezoid_accepts; /* line searches ended by the trapezoid estimate of the energy change */
+    int32_t pad_;
 } symx_solve_stats;
 
 void symx_newton_params_default(symx_newton_params* p);

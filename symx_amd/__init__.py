@@ -58,14 +58,14 @@ class NewtonParams(Structure):
     _fields_ = [("precision", c_int32), ("projection_mode", c_int32), ("use_mirroring", c_int32), ("max_iterations", c_int32),
                 ("residual_tolerance_abs", c_double), ("residual_tolerance_rel", c_double), ("cg_rel_tolerance", c_double),
                 ("cg_abs_tolerance", c_double), ("cg_max_iterations", c_int32), ("projection_eps", c_double),
-                ("step_tolerance", c_double)]
+                ("step_tolerance", c_double), ("verbosity", c_int32)]
 
 
 class SolveStats(Structure):
     _fields_ = [("result", c_int32), ("newton_iterations", c_int32), ("cg_iterations", c_int32), ("ls_bt_iterations", c_int32),
                 ("n_refinements", c_int32), ("final_energy", c_double), ("final_residual", c_double), ("first_residual", c_double),
                 ("seconds", c_double), ("seconds_eval", c_double), ("seconds_project", c_double), ("seconds_assembly", c_double),
-                ("seconds_linear", c_double), ("seconds_energy", c_double)]
+                ("seconds_linear", c_double), ("seconds_energy", c_double), ("ls_trapezoid_accepts", c_int32), ("pad_", c_int32)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -339,6 +339,18 @@ class Tet4Block:
         E = c_double()
         check_hip(hip().symx_hip_problem_eval(self._p(), c_int32(mode), ctypes.byref(E)))
         return E.value
+
+    def gradient_dot(self, vec: int) -> float:
+        """g(x) . vec with the gradient at the current device DoFs (VEC_GRAD untouched)."""
+        out = c_double()
+        check_hip(hip().symx_hip_problem_gradient_dot(self._p(), c_int32(vec), ctypes.byref(out)))
+        return out.value
+
+    def energy_change(self) -> float:
+        """Sum over elements of E_e(current DoFs) - E_e(DoFs of the last EVAL_PG/EVAL_PGH)."""
+        dE = c_double()
+        check_hip(hip().symx_hip_problem_energy_change(self._p(), ctypes.byref(dE)))
+        return dE.value
 
     def set_dofs(self, x: np.ndarray):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)

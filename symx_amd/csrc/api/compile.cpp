@@ -352,12 +352,12 @@ namespace symx
                 os << "        const int nv = rem < 0 ? 0 : (rem > 64 ? 64 : (int)rem);\n";
                 os << "        symx_store_rows<" << n << ">(p.G, sg, e0, nv);\n";
                 if (ks.kind == KernelKind::PGH) os << "        symx_store_rows<" << n_ub * 9 << ">(p.H, sh, e0, nv);\n";
-                os << "    }\n    symx_store_energy(e_out, p.E_partial);\n}\n";
+                os << "    }\n    symx_store_energy(e_out, p, e);\n}\n";
             }
             else {
                 os << "        }\n        else {\n";
                 emit_zero("            ");
-                os << "        }\n    }\n    symx_store_energy(e_out, p.E_partial);\n}\n";
+                os << "        }\n    }\n    symx_store_energy(e_out, p, e);\n}\n";
             }
         }
         return os.str();
@@ -489,7 +489,7 @@ namespace symx
                 if (kind == 2) os << "        for (int t = lane; t < " << NUB * 9 << "; t += 64) p.H[(long long)e * " << NUB * 9 << " + t] = 0.0;\n";
                 os << "    }\n";
             }
-            os << "    if (lane == 0) s_e[wv] = e_out;\n    __syncthreads();\n";
+            os << "    if (lane == 0) s_e[wv] = symx_element_energy(e_out, p, e);\n    __syncthreads();\n";
             os << "    if (threadIdx.x == 0) p.E_partial[symx_block()] = ((s_e[0] + s_e[1]) + s_e[2]) + s_e[3];\n}\n";
         }
         return os.str();

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -16,6 +17,19 @@ namespace symx
         {
             if (rc != SYMX_HIP_OK) throw std::runtime_error(std::string("symx_hip: ") + what + ": " + symx_hip_last_error(ctx));
         }
+        std::string sfmt(const char* f, ...) __attribute__((format(printf, 1, 2)));
+        std::string sfmt(const char* f, ...)
+        {
+            char buf[256];
+            va_list ap;
+            va_start(ap, f);
+            std::vsnprintf(buf, sizeof(buf), f, ap);
+            va_end(ap);
+            return buf;
+        }
+        // Line-search steps below this fraction of max|x| are judged by the trapezoid estimate
+        // of the energy change when the energy difference does not show a decrease.
+        constexpr double kTrapezoidStep = 1e-7;
         std::string identifier(const std::string& s)
         {
             std::string r;
@@ -672,6 +686,7 @@ namespace symx
         }
 
         int newton_iteration = -1;
+        const bool full = out->get_console_verbosity() == Verbosity::Full;
         while (result == SolverReturn::Running) {
             newton_iteration++;
             if (newton_iteration == settings.max_iterations) {
@@ -679,6 +694,7 @@ namespace symx
                 result = settings.max_iterations_as_success ? SolverReturn::Successful : SolverReturn::TooManyIterations;
                 break;
             }
+            out->print_with_new_line(sfmt("%2d. ", newton_iteration), Verbosity::Medium);
             if (!callbacks->before_energy_evaluation.empty()) {
                 sync_dofs_to_host();
                 for (auto& f : callbacks->before_energy_evaluation) f();
@@ -689,6 +705,7 @@ namespace symx
                 check(symx_hip_problem_eval(prob, SYMX_HIP_EVAL_PGH, &E0), hip, "evaluation");
             }
             const double residual_norm = compute_residual();
+            out->print(sfmt("r0: %.2e | ", residual_norm), Verbosity::Medium);
             stats.energies.push_back(E0);
             stats.residuals.push_back(residual_norm);
             if (newton_iteration == 0) res_0 = residual_norm;
@@ -767,6 +784,7 @@ namespace symx
                 }
                 if (ok && descends) break;
                 ok = false;
+                out->print("Retrying with more projection.", Verbosity::Full);
                 // _increase_projection (NewtonsMethod.cpp:354-368)
                 if (settings.projection_mode == ProjectionToPD::ProjectOnDemand) pdn_countdown = settings.project_on_demand_countdown;
                 if (settings.projection_mode == ProjectionToPD::Progressive) {
@@ -779,6 +797,10 @@ namespace symx
                 break;
             }
             stats.cg_per_iteration.push_back(stats.cg_iterations - initial_cg);
+            if (!full)
+                out->print(sfmt("ph: %4.1f%% | #CG: %4d | ", n_hessians > 0 ? 100.0 * (double)n_projected_now / (double)n_hessians : 0.0,
+                                stats.cg_iterations - initial_cg),
+                           Verbosity::Medium);
             // _decrease_projection (NewtonsMethod.cpp:370-386)
             if (settings.projection_mode == ProjectionToPD::ProjectOnDemand) pdn_countdown--;
             if (settings.projection_mode == ProjectionToPD::Progressive) ppn_threshold *= settings.ppn_release_factor;
@@ -788,11 +810,13 @@ namespace symx
 
             double du_max = 0.0;
             check(symx_hip_problem_max_abs(prob, SYMX_HIP_VEC_DU, &du_max), hip, "max_abs");
+            out->print(sfmt("du: %.1e | ", du_max), Verbosity::Medium);
+            if (full) out->print(sfmt("du.g: %.3e | ", du_dot_grad), Verbosity::Full);
             if (newton_iteration >= settings.min_iterations && du_max < settings.step_tolerance) {
                 result = SolverReturn::Successful;
                 break;
             }
-            result = line_search(E0, du_dot_grad, du_max);
+            result = line_search(du_dot_grad, du_max);
             if (newton_iteration >= settings.min_iterations && !callbacks->is_converged.empty()) {
                 sync_dofs_to_host();
                 if (run_bool(false, callbacks->is_converged)) {
@@ -835,7 +859,7 @@ namespace symx
         return info.converged != 0;
     }
 
-    SolverReturn NewtonsMethod::line_search(double E0, double du_dot_grad, double du_max)
+    SolverReturn NewtonsMethod::line_search(double du_dot_grad, double du_max)
     {
         symx_hip_ctx hip = context->hip();
         symx_hip_problem prob = dev->problem();
@@ -849,6 +873,7 @@ namespace symx
         if (du_max > settings.step_cap) {
             retraction *= settings.step_cap / du_max;
             check(symx_hip_problem_scale(prob, SYMX_HIP_VEC_DU, SYMX_HIP_VEC_DU, retraction), hip, "cap");
+            out->print(sfmt("du cap %.1e | ", du_max), Verbosity::Medium);
             du_max = settings.step_cap;
             stats.ls_cap_iterations++;
         }
@@ -862,6 +887,7 @@ namespace symx
             retraction *= max_step;
             check(symx_hip_problem_scale(prob, SYMX_HIP_VEC_DU, SYMX_HIP_VEC_DU, max_step), hip, "max step");
             du_max *= max_step;
+            out->print(sfmt("ls max %.1e | ", du_max), Verbosity::Medium);
             stats.ls_max_iterations++;
         }
         const double shrink = 0.5;
@@ -875,10 +901,12 @@ namespace symx
                 for (auto& f : callbacks->is_intermediate_state_valid) valid = valid && f();
             }
             if (valid) break;
+            out->print_with_new_line(sfmt("%d. step: %.2e | Invalid state", ls_inv_it, step), Verbosity::Full);
             step *= shrink;
             apply(step);
             stats.ls_inv_iterations++;
         }
+        if (out->get_console_verbosity() != Verbosity::Full && ls_inv_it > 0) out->print(sfmt("ls inv %2d | ", ls_inv_it), Verbosity::Medium);
         if (ls_inv_it == settings.max_backtracking_invalid_state_iterations) {
             out->print_with_new_line("Newton failure: Too many invalid intermediate state iterations.", Verbosity::Medium);
             for (auto& f : callbacks->on_intermediate_state_invalid) f();
@@ -888,8 +916,16 @@ namespace symx
             stats.step_sizes.push_back(step * retraction);
             return SolverReturn::Running;
         }
+        // Armijo test E1 < E0 + beta * s * du.g (NewtonsMethod.cpp:565-591), evaluated as
+        // E1 - E0 < beta * s * du.g with E1 - E0 summed element by element against the
+        // energies of the evaluation at E0 (the last EVAL_PGH): algebraically the same test,
+        // with the round-off of the element energies instead of that of the total energy.
         const double expected_decrease = settings.line_search_armijo_beta * du_dot_grad * retraction;
-        double E_threshold = E0 + expected_decrease * step;
+        double x_max = -1.0;  // max |x0|, read once when a step is short enough to need it
+        auto x_scale = [&]() {
+            if (x_max < 0.0) check(symx_hip_problem_max_abs(prob, SYMX_HIP_VEC_DOFS_LS, &x_max), hip, "max_abs");
+            return x_max;
+        };
         int armijo = 0;
         for (; armijo < settings.max_backtracking_armijo_iterations; ++armijo) {
             if (!callbacks->before_energy_evaluation.empty()) {
@@ -897,18 +933,38 @@ namespace symx
                 for (auto& f : callbacks->before_energy_evaluation) f();
                 dev->refresh_data();
             }
-            double E1 = 0.0;
+            double dE = 0.0;
             {
                 auto t = context->logger->time("evaluate_P");
-                check(symx_hip_problem_eval(prob, SYMX_HIP_EVAL_P, &E1), hip, "energy");
+                check(symx_hip_problem_energy_change(prob, &dE), hip, "energy");
             }
-            if (E1 < E_threshold) break;
+            out->print_with_new_line(sfmt("%d. step: %.2e | dE: %.3e | dE_bt: %.3e | ", armijo, step, dE, expected_decrease * step),
+                                     Verbosity::Full);
+            if (dE < expected_decrease * step) break;
+            // Steps shorter than kTrapezoidStep * max|x|: the energy change near convergence
+            // falls below the round-off of the element energies themselves (about 1e-18 of
+            // each Neo-Hookean element, 5e-15 summed over 6M elements), where the test above
+            // fails for every step length. There the change is estimated from directional
+            // derivatives, dE ~ s (g(x0) + g(x0 + s du)) . du / 2 (trapezoid rule, error of
+            // order (s |du| / element size) relative), which keeps the rounding of the
+            // gradients (1e-22 here) instead.
+            if (step * du_max <= kTrapezoidStep * x_scale()) {
+                double g1_du = 0.0;
+                check(symx_hip_problem_gradient_dot(prob, SYMX_HIP_VEC_DU, &g1_du), hip, "gradient");
+                const double dE_trap = 0.5 * step * (du_dot_grad * retraction + g1_du);
+                out->print(sfmt("dE (trapezoid): %.3e | ", dE_trap), Verbosity::Full);
+                if (dE_trap < expected_decrease * step) {
+                    stats.ls_trapezoid_accepts++;
+                    break;
+                }
+            }
+            out->print("Backtrack", Verbosity::Full);
             step *= shrink;
             apply(step);
-            E_threshold = E0 + expected_decrease * step;
             stats.ls_bt_iterations++;
         }
         context->logger->add_and_append("ls_bt", armijo);
+        if (out->get_console_verbosity() != Verbosity::Full && armijo > 0) out->print(sfmt("ls bt %2d | ", armijo), Verbosity::Medium);
         if (armijo == settings.max_backtracking_armijo_iterations) {
             out->print_with_new_line("Newton failure: Too many armijo iterations.", Verbosity::Medium);
             for (auto& f : callbacks->on_armijo_fail) f();

@@ -483,6 +483,7 @@ namespace symx
             int ls_max_iterations = 0;
             int ls_inv_iterations = 0;
             int ls_bt_iterations = 0;
+            int ls_trapezoid_accepts = 0;   // line searches ended by the trapezoid estimate of the energy change
             uint64_t n_hessians = 0;
             uint64_t n_projected_hessians = 0;
             double projected_hessians_ratio = 0.0;
@@ -516,7 +517,7 @@ namespace symx
     private:
         bool project_and_assemble(bool& assembled, int& pdn_countdown, double& ppn_threshold);
         bool solve_linear_system(double residual_norm);
-        SolverReturn line_search(double E0, double du_dot_grad, double du_max);
+        SolverReturn line_search(double du_dot_grad, double du_max);
         void sync_dofs_to_host();
         void upload_dofs();
         double compute_residual();

@@ -133,6 +133,8 @@ struct symx_hip_problem_s
     double* d_G = nullptr;
     double* d_H = nullptr;
     double* d_Epart = nullptr;
+    double* d_Eelem = nullptr;    // [total_elem] element energies of the last EVAL_PG / EVAL_PGH
+    bool e_ref_valid = false;
     uint8_t* d_flags = nullptr;   // per-element scratch of the projection
     uint32_t* d_plist = nullptr;  // per-element scratch: elements left to the fallback projection
     bool h_valid = false;

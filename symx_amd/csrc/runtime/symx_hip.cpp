@@ -20,7 +20,7 @@
 using namespace symx_hip;
 
 // The device prelude restates these structs; both sides must agree byte for byte.
-static_assert(sizeof(symx_hip_eval_params) == 1080, "symx_hip_eval_params layout");
+static_assert(sizeof(symx_hip_eval_params) == 1088, "symx_hip_eval_params layout");
 static_assert(sizeof(symx_hip_adjoint_params) == 1040, "symx_hip_adjoint_params layout");
 
 namespace
@@ -82,10 +82,11 @@ struct symx_hip_eval_params {
     double* G;
     double* H;
     symx_u8* C_out;
+    double* E_elem;
     symx_i32 n_elem;
-    symx_i32 pad_;
+    symx_i32 e_mode;  // 0 sum, 1 sum and store E_elem[e], 2 sum of E_e - E_elem[e]
 };
-static_assert(sizeof(symx_hip_eval_params) == 1080, "symx_hip_eval_params layout");
+static_assert(sizeof(symx_hip_eval_params) == 1088, "symx_hip_eval_params layout");
 struct symx_hip_adjoint_params {
     const double* lambda[SYMX_HIP_MAX_MAPS];
     double* out;
@@ -111,14 +112,25 @@ __device__ __forceinline__ int symx_block()
     const int x = b % 8, i = b / 8, per = nb / 8, extra = nb % 8;
     return x * per + (x < extra ? x : extra) + i;
 }
-// Fixed-order workgroup sum of the per-element energies (256 threads).
-__device__ __forceinline__ void symx_store_energy(double v, double* E_partial)
+// Energy of element e as it enters the partial sum: stored (e_mode 1) or differenced
+// against the stored value (e_mode 2).
+__device__ __forceinline__ double symx_element_energy(double v, const symx_hip_eval_params& p, int e)
+{
+    if (e < p.n_elem) {
+        if (p.e_mode == 1) p.E_elem[e] = v;
+        else if (p.e_mode == 2) v -= p.E_elem[e];
+    }
+    return v;
+}
+// Fixed-order workgroup sum of the per-element energies (256 threads, one element each).
+__device__ __forceinline__ void symx_store_energy(double v, const symx_hip_eval_params& p, int e)
 {
     __shared__ double lds[4];
+    v = symx_element_energy(v, p, e);
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) E_partial[symx_block()] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    if (threadIdx.x == 0) p.E_partial[symx_block()] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
 }
 // Element-major rows dst[e * W + i] of the 256 elements of this workgroup, written with
 // coalesced stores: each wave transposes its 64 rows through LDS in chunks of 16 values, so
@@ -430,6 +442,7 @@ void symx_hip_problem_destroy(symx_hip_problem p)
     dfree(p->d_G);
     dfree(p->d_H);
     dfree(p->d_Epart);
+    dfree(p->d_Eelem);
     dfree(p->d_flags);
     dfree(p->d_plist);
     dfree(p->d_projected);
@@ -854,6 +867,7 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     rc = rc ? rc : dalloc(ctx, &p->d_G, (size_t)std::max<int64_t>(g * 3, 1));
     rc = rc ? rc : dalloc(ctx, &p->d_H, (size_t)std::max<int64_t>(h * 9, 1));
     rc = rc ? rc : dalloc(ctx, &p->d_Epart, (size_t)std::max<int64_t>(parts, 1));
+    rc = rc ? rc : dalloc(ctx, &p->d_Eelem, (size_t)std::max<int64_t>(p->total_elem, 1));
     {
         int64_t max_elem = 1;
         for (auto& pot : p->pots) max_elem = std::max<int64_t>(max_elem, pot.n_elem);
@@ -971,7 +985,9 @@ static void fill_params(symx_hip_problem p, DevPotential& pot, symx_hip_eval_par
     prm.G = p->d_G + pot.g_base * 3;
     prm.H = p->d_H + pot.h_base * 9;
     prm.C_out = pot.d_active;
+    prm.E_elem = p->d_Eelem + pot.e_base;
     prm.n_elem = pot.n_elem;
+    prm.e_mode = SYMX_HIP_EMODE_SUM;
 }
 
 int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
@@ -980,10 +996,13 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
     symx_hip_ctx ctx = p->ctx;
     if (!p->finalized) return fail(ctx, SYMX_HIP_ERR_ARG, "problem not finalized");
     if (mode < SYMX_HIP_EVAL_P || mode > SYMX_HIP_EVAL_PGH) return fail(ctx, SYMX_HIP_ERR_ARG, "bad evaluation mode");
+    // PG / PGH keep the element energies: the reference point of symx_hip_problem_energy_change
+    const int32_t e_mode = mode == SYMX_HIP_EVAL_P ? SYMX_HIP_EMODE_SUM : SYMX_HIP_EMODE_STORE;
     for (auto& pot : p->pots) {
         if (pot.n_elem == 0) continue;
         symx_hip_eval_params prm;
         fill_params(p, pot, prm);
+        prm.e_mode = e_mode;
         if (!pot.kC.empty()) {
             int rc = launch_eval(ctx, pot.module, pot.kC, prm);
             if (rc) return rc;
@@ -992,6 +1011,7 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
         int rc = launch_eval(ctx, pot.module, k, prm, pot.elems_per_group);
         if (rc) return rc;
     }
+    if (e_mode == SYMX_HIP_EMODE_STORE) p->e_ref_valid = true;
     if (mode == SYMX_HIP_EVAL_PGH && p->d_projected) launch_fill_u8(ctx->stream, p->d_projected, p->total_elem);
     launch_sum(ctx->stream, p->d_Epart, p->total_parts, ctx->d_scalars);
     const bool with_grad = mode >= SYMX_HIP_EVAL_PG;
@@ -1010,6 +1030,59 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
     }
     if (E) *E = ctx->h_pinned[0];
     return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_energy_change(symx_hip_problem p, double* dE)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->finalized) return fail(ctx, SYMX_HIP_ERR_ARG, "problem not finalized");
+    if (!p->e_ref_valid) return fail(ctx, SYMX_HIP_ERR_ARG, "no reference energies (evaluate with EVAL_PG or EVAL_PGH first)");
+    if (!dE) return fail(ctx, SYMX_HIP_ERR_ARG, "null argument");
+    for (auto& pot : p->pots) {
+        if (pot.n_elem == 0) continue;
+        symx_hip_eval_params prm;
+        fill_params(p, pot, prm);
+        prm.e_mode = SYMX_HIP_EMODE_DELTA;
+        if (!pot.kC.empty()) {
+            int rc = launch_eval(ctx, pot.module, pot.kC, prm);
+            if (rc) return rc;
+        }
+        int rc = launch_eval(ctx, pot.module, pot.kP, prm, pot.elems_per_group);
+        if (rc) return rc;
+    }
+    launch_sum(ctx->stream, p->d_Epart, p->total_parts, ctx->d_scalars);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_scalars, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    *dE = ctx->h_pinned[0];
+    return SYMX_HIP_OK;
+}
+
+int symx_hip_problem_gradient_dot(symx_hip_problem p, int32_t vec, double* out)
+{
+    if (check_problem(p)) return SYMX_HIP_ERR_ARG;
+    symx_hip_ctx ctx = p->ctx;
+    if (!p->finalized) return fail(ctx, SYMX_HIP_ERR_ARG, "problem not finalized");
+    double* dv = vec_ptr(p, vec);
+    if (!dv || !out) return fail(ctx, SYMX_HIP_ERR_ARG, "bad vector id or null pointer");
+    for (auto& pot : p->pots) {
+        if (pot.n_elem == 0) continue;
+        symx_hip_eval_params prm;
+        fill_params(p, pot, prm);  // e_mode SUM: the stored element energies stay
+        if (!pot.kC.empty()) {
+            int rc = launch_eval(ctx, pot.module, pot.kC, prm);
+            if (rc) return rc;
+        }
+        int rc = launch_eval(ctx, pot.module, pot.kPG, prm, pot.elems_per_group);
+        if (rc) return rc;
+    }
+    // the element Hessians still belong to the last EVAL_PGH; the assembled gradient goes to
+    // the FP64 PCG residual vector (scratch outside a solve), not to VEC_GRAD
+    launch_grad_assemble(ctx->stream, p->d_goff, p->d_glist, p->d_grow, p->d_G, p->d_r, p->nb);
+    launch_dot(ctx->stream, p->n_dofs, p->d_r, dv, ctx->d_scalars, ctx->d_partials);
+    HIP_TRY(ctx, hipGetLastError());
+    return sync_scalar(ctx, ctx->d_scalars, out);
 }
 
 int symx_hip_problem_n_element_hessians(symx_hip_problem p, int64_t* n)

@@ -112,8 +112,9 @@ struct symx_scene_s
     spNewtonsMethod newton;
 };
 
-static void apply_newton_params(NewtonSettings& s, const symx_newton_params& p)
+static void apply_newton_params(NewtonSettings& s, const symx_newton_params& p, OutputSink& out)
 {
+    out.set_console_verbosity(p.verbosity <= 0 ? Verbosity::None : p.verbosity == 1 ? Verbosity::Summary : p.verbosity == 2 ? Verbosity::Medium : Verbosity::Full);
     s.precision = p.precision == SYMX_HIP_MIXED ? Precision::Mixed : Precision::FP64;
     s.projection_mode = p.projection_mode == 0   ? ProjectionToPD::Newton
                         : p.projection_mode == 2 ? ProjectionToPD::ProjectOnDemand
@@ -512,7 +513,7 @@ int symx_scene_hex_block_create(const symx_hex_block_params* p, const symx_newto
         build_hex(s, p, np);
         s->ctx = Context::create();
         s->newton = NewtonsMethod::create(s->G, s->ctx);
-        apply_newton_params(s->newton->settings, *np);
+        apply_newton_params(s->newton->settings, *np, *s->ctx->output);
     });
     if (rc) {
         delete s;
@@ -547,6 +548,7 @@ void symx_newton_params_default(symx_newton_params* p)
     p->cg_max_iterations = 10000;
     p->projection_eps = 1e-10;
     p->step_tolerance = 0.0;
+    p->verbosity = 0;
 }
 
 void symx_tet4_block_params_default(symx_tet4_block_params* p)
@@ -605,7 +607,7 @@ int symx_scene_tet4_block_create(const symx_tet4_block_params* p, const symx_new
         symx_scene_s* S = s;
         S->ctx = Context::create();
         S->newton = NewtonsMethod::create(S->G, S->ctx);
-        apply_newton_params(S->newton->settings, *np);
+        apply_newton_params(S->newton->settings, *np, *S->ctx->output);
     });
     if (rc) {
         delete s;
@@ -623,7 +625,7 @@ int symx_scene_sphere_create(const symx_sphere_params* p, const symx_newton_para
         build_sphere(s, p, np);
         s->ctx = Context::create();
         s->newton = NewtonsMethod::create(s->G, s->ctx);
-        apply_newton_params(s->newton->settings, *np);
+        apply_newton_params(s->newton->settings, *np, *s->ctx->output);
         if (p->contact_stiffness > 0.0 && p->step_filter > 0.0) {
             symx_hip_problem prob = s->newton->device().problem();
             if (symx_hip_problem_set_plane_filter(prob, s->contact.data(), (int32_t)s->contact.size(), 2, 0.0))
@@ -719,7 +721,7 @@ int symx_scene_set_log_mu(symx_scene s, const double* log_mu)
 int symx_scene_set_newton_params(symx_scene s, const symx_newton_params* np)
 {
     s->np = *np;
-    apply_newton_params(s->newton->settings, *np);
+    apply_newton_params(s->newton->settings, *np, *s->ctx->output);
     return 0;
 }
 
@@ -737,6 +739,7 @@ int symx_scene_solve(symx_scene s, symx_solve_stats* st)
             st->newton_iterations = ss.newton_iterations;
             st->cg_iterations = ss.cg_iterations;
             st->ls_bt_iterations = ss.ls_bt_iterations;
+            st->ls_trapezoid_accepts = ss.ls_trapezoid_accepts;
             st->n_refinements = ss.n_refinements;
             st->final_energy = ss.final_energy;
             st->final_residual = ss.final_residual;
