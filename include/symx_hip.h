@@ -290,6 +290,22 @@ int symx_hip_problem_plane_max_step(symx_hip_problem prob, double eta, double* t
 int symx_hip_loop_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const int32_t* conn, int32_t n_elem, int32_t conn_stride,
                        const symx_hip_map_desc* maps, int32_t n_maps, const uint8_t* active, int32_t n_out, double* host_out);
 
+/* Allocation-free variant of symx_hip_loop_eval. No counterpart in SymX: it is for callers
+ * that cannot allocate device memory (code that must not call hipMalloc, such as tight
+ * serving loops or harnesses that forbid allocation). Inputs and outputs are DEVICE
+ * pointers owned by the caller: d_conn [n_elem][conn_stride] row-major, d_maps[m] (host
+ * array of n_maps device pointers), d_active (may be NULL; one flag per element) and d_out
+ * [n_elem][n_out]. The connectivity is transposed into a per-GPU scratch array of
+ * SYMX_HIP_LOOP_SCRATCH_ENTRIES int32 values (16 MiB) that the library holds from the first
+ * symx_hip_ctx_create on the GPU until the process ends; calls with n_elem * conn_stride
+ * above that limit fail with SYMX_HIP_ERR_ARG. The call makes no hipMalloc / hipFree. Every
+ * call on one GPU shares the scratch array: calls that overlap on one GPU (several contexts,
+ * streams or host threads) race. Synchronous at return, like the other entry points. */
+#define SYMX_HIP_LOOP_SCRATCH_ENTRIES (1 << 22)
+int symx_hip_loop_eval_noalloc(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const int32_t* d_conn, int32_t n_elem,
+                               int32_t conn_stride, const double* const* d_maps, int32_t n_maps, const uint8_t* d_active, int32_t n_out,
+                               double* d_out);
+
 /* Kernel timing with HIP events on the context's stream (the stream every
  * kernel of this library runs on): average duration in ms of `reps` launches
  * of one phase. phase: 0 element kernel E+grad+Hessian of potential `pot`,

@@ -76,6 +76,7 @@ struct symx_hip_ctx_s
     double* h_pinned = nullptr;    // pinned host scalars (64 doubles)
     symx_hip::PcgState* d_pcg = nullptr;
     symx_hip::PcgState* h_pcg = nullptr;
+    int32_t* d_loop_scratch = nullptr;  // device address of this device's loop scratch array
 };
 
 struct symx_hip_module_s
@@ -236,4 +237,8 @@ namespace symx_hip
     void launch_max_abs(hipStream_t s, int64_t n, const double* a, double* out, double* partials);
     void launch_fill_u8(hipStream_t s, int64_t n, uint8_t* p, uint8_t v);
     int reduction_groups(int64_t n);
+    // Allocation-free loop evaluation: out[c * n_elem + e] = in[e * stride + c], and the device
+    // address of the per-GPU scratch array (device must be current; 0 on success).
+    void launch_transpose_conn(hipStream_t s, const int32_t* in, int32_t n_elem, int32_t stride, int32_t* out);
+    int loop_scratch_address(int device, int32_t** out);
 }

@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 #include "internal.h"
@@ -1911,5 +1912,45 @@ namespace symx_hip
     void launch_fill_u8(hipStream_t s, int64_t n, uint8_t* p, uint8_t v)
     {
         if (n > 0) k_fill_u8<<<grid_for(n), NT, 0, s>>>(n, p, v);
+    }
+}
+
+// ---- allocation-free loop evaluation (symx_hip_loop_eval_noalloc) ----------------------
+// The transposed connectivity [column][element] that the generated loop kernels gather
+// through. One copy per GPU for the life of the process; every call on a GPU shares it.
+static __device__ __align__(256) int32_t scratch[SYMX_HIP_LOOP_SCRATCH_ENTRIES];
+
+namespace symx_hip
+{
+    __global__ __launch_bounds__(NT) void k_transpose_conn(const int32_t* __restrict__ in, int64_t n_elem, int32_t stride,
+                                                           int32_t* __restrict__ out)
+    {
+        const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+        if (t >= n_elem * stride) return;
+        const int64_t e = t / stride, c = t - e * stride;
+        out[c * n_elem + e] = in[t];
+    }
+    void launch_transpose_conn(hipStream_t s, const int32_t* in, int32_t n_elem, int32_t stride, int32_t* out)
+    {
+        const int64_t n = (int64_t)n_elem * stride;
+        if (n > 0) k_transpose_conn<<<grid_for(n), NT, 0, s>>>(in, n_elem, stride, out);
+    }
+
+    // Device address of `scratch` on `device`, which must be the current device. Looked up
+    // once per device (each GPU holds its own copy at its own address); the host-side name
+    // of the array is no device address and must not reach a kernel or a copy.
+    int loop_scratch_address(int device, int32_t** out)
+    {
+        static std::mutex m;
+        static int32_t* table[64] = {};
+        std::lock_guard<std::mutex> g(m);
+        if (device < 0 || device >= 64) return -1;
+        if (table[device] == nullptr) {
+            void* ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || ptr == nullptr) return -1;
+            table[device] = static_cast<int32_t*>(ptr);
+        }
+        *out = table[device];
+        return 0;
     }
 }

@@ -297,6 +297,10 @@ int symx_hip_ctx_create(int device, symx_hip_ctx* out)
         return rc;
     }
     HIP_TRY(ctx, hipMemset(ctx->d_counters, 0, 4 * sizeof(unsigned int)));
+    if (loop_scratch_address(device, &ctx->d_loop_scratch) != 0) {
+        symx_hip_ctx_destroy(ctx);
+        return fail(nullptr, SYMX_HIP_ERR_HIP, "hipGetSymbolAddress of the loop scratch array failed");
+    }
     HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
     HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pcg, sizeof(PcgState), hipHostMallocDefault));
     *out = ctx;
@@ -1582,6 +1586,45 @@ int symx_hip_loop_eval(symx_hip_ctx ctx, symx_hip_module mod, const char* name, 
     }
     cleanup();
     return rc;
+}
+
+int symx_hip_loop_eval_noalloc(symx_hip_ctx ctx, symx_hip_module mod, const char* name, const int32_t* d_conn, int32_t n_elem,
+                               int32_t conn_stride, const double* const* d_maps, int32_t n_maps, const uint8_t* d_active, int32_t n_out,
+                               double* d_out)
+{
+    if (!ctx || !mod || !name || n_elem < 0 || conn_stride <= 0 || n_maps < 0 || n_maps > SYMX_HIP_MAX_MAPS || n_out <= 0 ||
+        (n_elem > 0 && (!d_conn || !d_out)) || (n_maps > 0 && !d_maps))
+        return fail(ctx, SYMX_HIP_ERR_ARG, "bad loop arguments");
+    if ((int64_t)n_elem * conn_stride > (int64_t)SYMX_HIP_LOOP_SCRATCH_ENTRIES)
+        return fail(ctx, SYMX_HIP_ERR_ARG, "loop too large for the allocation-free path: n_elem * conn_stride exceeds "
+                                           "SYMX_HIP_LOOP_SCRATCH_ENTRIES (use symx_hip_loop_eval)");
+    if (!ctx->d_loop_scratch) return fail(ctx, SYMX_HIP_ERR_ARG, "context has no loop scratch address");
+    if (n_elem == 0) return SYMX_HIP_OK;
+    hipFunction_t f;
+    int rc = module_function(mod, name, &f);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    // the whole range the kernel reads, [0, n_elem * conn_stride), is written first
+    launch_transpose_conn(ctx->stream, d_conn, n_elem, conn_stride, ctx->d_loop_scratch);
+    struct
+    {
+        symx_hip_eval_params prm;
+        symx_hip_adjoint_params adj;
+    } args;
+    std::memset(&args, 0, sizeof(args));
+    args.prm.conn = ctx->d_loop_scratch;
+    args.prm.active = d_active;
+    for (int32_t m = 0; m < n_maps; m++) args.prm.maps[m] = d_maps[m];
+    args.prm.n_elem = n_elem;
+    args.adj.out = d_out;
+    args.adj.n_out = n_out;
+    size_t sz = sizeof(args);
+    void* config[] = { HIP_LAUNCH_PARAM_BUFFER_POINTER, &args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END };
+    const unsigned grid = (unsigned)((n_elem + SYMX_HIP_BLOCK - 1) / SYMX_HIP_BLOCK);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipModuleLaunchKernel(f, grid, 1, 1, SYMX_HIP_BLOCK, 1, 1, 0, ctx->stream, nullptr, config));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SYMX_HIP_OK;
 }
 
 int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, int32_t arg, int32_t reps, double* avg_ms)
