@@ -32,9 +32,9 @@ FP64_PEAK_TFLOPS = 78.6
 # (--calibration-copy-mib): FETCH x 2.0, WRITE x 1.0. The per-kernel table is in profiles/.
 MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 757.9e6}
 # what the mixed path computes in: element kernels and projection in FP64 arithmetic with
-# FP64 element-Hessian storage, FP32 block matrix and inner PCG, FP64 outer refinement,
+# FP32 element-Hessian storage, FP32 block matrix and inner PCG, FP64 outer refinement,
 # FP64 energies, gradients and line search
-DTYPE_MIXED = "f64 element Hessians, f32 BSR/PCG + f64 refinement"
+DTYPE_MIXED = "f32 element Hessians/BSR/PCG (f64 arithmetic in element and projection kernels) + f64 refinement"
 
 
 def block_params(cells):
@@ -45,19 +45,20 @@ def block_params(cells):
 
 def algorithmic_bytes(n_elem, n_vert, nnzb, precision):
     """Compulsory HBM bytes per unit of work (each array's unique footprint once)."""
-    sH = 4 if precision == 1 else 8
+    sH = 4 if precision == 1 else 8  # element-Hessian and block-matrix storage
     nv_per_e = n_vert / n_elem
     return {
-        # conn 16 + x,X 48 per vertex amortised + E partial + grad 96 + 78 unique Hessian entries
-        "element": 16 + 48 * nv_per_e + 8 / 256 + 96 + 78 * sH,
-        # read + write of the 78 unique entries (FP64 projection)
-        "projection": 2 * 78 * 8,
+        # conn 16 + x,X 48 per vertex amortised + E partial + E_e 8 (line-search reference) + grad 96
+        # + 78 unique Hessian entries
+        "element": 16 + 48 * nv_per_e + 8 / 256 + 8 + 96 + 78 * sH,
+        # read + write of the 78 unique entries
+        "projection": 2 * 78 * sH,
         # per PCG iteration: matrix + column ids (16-bit offsets from the row, which every mesh
         # here fits) + slice offsets + D^-1 (6 distinct values of each symmetric block) + 12
         # vector passes
         "pcg_iteration": nnzb * (9 * sH + 2) + (n_vert + 1) * 4 + n_vert * 6 * sH + 12 * 3 * n_vert * sH,
-        # per assembly: element Hessians (10 upper blocks incl. slot ids) + matrix write
-        "assembly": n_elem * (78 * 8 + 10 * 4) + nnzb * 9 * sH,
+        # per assembly: element Hessians (78 unique entries + 10 slot ids) + matrix write
+        "assembly": n_elem * (78 * sH + 10 * 4) + nnzb * 9 * sH,
     }
 
 
@@ -177,8 +178,10 @@ def profile_kernels(scene, precision, reps=5):
     p = scene.problem
     out = {}
     scene.set_dofs(scene.get("x"))
-    scene.eval(symx_amd.EVAL_PGH)
-    for name, phase, arg in [("element", 0, 0), ("energy", 4, 0), ("projection", 1, 0), ("assembly", 2, precision),
+    scene.eval(symx_amd.EVAL_PGH_F32 if precision == symx_amd.MIXED else symx_amd.EVAL_PGH)
+    # element kernel with the Hessian storage of `precision`; projection and assembly then
+    # work on that storage
+    for name, phase, arg in [("element", 0, precision), ("energy", 4, 0), ("projection", 1, 0), ("assembly", 2, precision),
                              ("pcg_iteration", 3, precision)]:
         ms = ctypes.c_double()
         symx_amd.check_hip(symx_amd.hip().symx_hip_problem_profile(p, ctypes.c_int32(phase), ctypes.c_int32(0), ctypes.c_int32(arg),

@@ -38,6 +38,11 @@ enum {
 enum { SYMX_HIP_FP64 = 0, SYMX_HIP_MIXED = 1 };
 /* What a potential evaluation produces (SecondOrderCompiledGlobal.h:27-31). */
 enum { SYMX_HIP_EVAL_P = 0, SYMX_HIP_EVAL_PG = 1, SYMX_HIP_EVAL_PGH = 2 };
+/* PGH with the element Hessians stored as floats (each potential's kernel_PGH_f32): the
+ * storage of SYMX_HESS_STORAGE_FLOAT=float moved from the block matrix
+ * (ElementHessians.cpp:236-249) to the element Hessians; projection and assembly then
+ * read floats, work and sum in FP64, and store in their own precision. */
+enum { SYMX_HIP_EVAL_PGH_F32 = 3 };
 /* Named device vectors of a problem, for the Newton vector operations. */
 enum {
     SYMX_HIP_VEC_DOFS = 0,    /* current DoFs (what element kernels read) */
@@ -132,6 +137,7 @@ typedef struct symx_hip_potential_desc {
     const char* kernel_PG;    /* energy + gradient                   */
     const char* kernel_PGH;   /* energy + gradient + upper Hessian    */
     const char* kernel_C;     /* activity condition (NULL if none)    */
+    const char* kernel_PGH_f32; /* PGH storing the Hessian as floats (NULL if none: no SYMX_HIP_EVAL_PGH_F32) */
     int32_t n_elem;
     int32_t conn_stride;
     const int32_t* conn;      /* host, row-major [n_elem][conn_stride] */

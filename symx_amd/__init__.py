@@ -20,7 +20,7 @@ LIB_DIR = os.path.join(_HERE, "lib")
 
 FP64 = 0
 MIXED = 1
-EVAL_P, EVAL_PG, EVAL_PGH = 0, 1, 2
+EVAL_P, EVAL_PG, EVAL_PGH, EVAL_PGH_F32 = 0, 1, 2, 3
 VEC_DOFS, VEC_GRAD, VEC_DU, VEC_DOFS_LS, VEC_RHS = 0, 1, 2, 3, 4
 
 SOLVER_RETURN = [

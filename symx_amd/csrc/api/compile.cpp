@@ -248,7 +248,16 @@ namespace symx
             for (size_t i = 0; i < sum.data.size(); i++) os << (i ? ", " : "") << literal(sum.data[i]);
             os << "};\n";
         }
+        // every PGH kernel also gets a twin for FP32 Hessian storage ("_f32"): the same FP64
+        // arithmetic, the Hessian rounded to float once when it is stored
+        std::vector<std::pair<const KernelSpec*, bool>> todo;
         for (const KernelSpec& ks : kernels) {
+            todo.push_back({ &ks, false });
+            if (ks.kind == KernelKind::PGH) todo.push_back({ &ks, true });
+        }
+        for (const auto& item : todo) {
+            const KernelSpec& ks = *item.first;
+            const bool h32 = item.second;
             std::vector<int32_t> roots;
             for (const Scalar& s : ks.outputs) roots.push_back(s.id);
             const std::vector<uint8_t> mark = reachable(g, roots);
@@ -258,16 +267,17 @@ namespace symx
             const int n_ub = nb * (nb + 1) / 2;
 
             const bool adj = ks.kind == KernelKind::PGP || ks.kind == KernelKind::LOOP;
-            os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.name << "(const symx_hip_eval_params p"
+            os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.name << (h32 ? "_f32" : "") << "(const symx_hip_eval_params p"
                << (adj ? ", const symx_hip_adjoint_params a" : "") << ")\n{\n";
             os << "    const int e = symx_block() * 256 + threadIdx.x;\n";
             if (ks.kind != KernelKind::C && !adj) os << "    double e_out = 0.0;\n";
             // single-point gradient/Hessian kernels collect their outputs in registers and
-            // store them through symx_store_rows (coalesced); summed kernels accumulate in place
-            const bool staged = (ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH) && !multi_q;
+            // store them through symx_store_rows (coalesced); summed kernels accumulate in place,
+            // except the FP32-storage twins, which sum the points in FP64 registers and round once
+            const bool staged = (ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH) && (!multi_q || h32);
             if (staged) {
                 os << "    double sg[" << n << "] = {};\n";
-                if (ks.kind == KernelKind::PGH) os << "    double sh[" << n_ub * 9 << "] = {};\n";
+                if (ks.kind == KernelKind::PGH) os << "    " << (h32 && !multi_q ? "float" : "double") << " sh[" << n_ub * 9 << "] = {};\n";
             }
             os << "    if (e < p.n_elem) {\n";
             if (ks.kind != KernelKind::C) os << "        const bool act = (p.active == nullptr) || (p.active[e] != 0);\n";
@@ -296,7 +306,7 @@ namespace symx
             std::string acc = "=";
             if (use.uses_summation) {
                 if (multi_q) {
-                    emit_zero(in);
+                    if (!staged) emit_zero(in);
                     acc = "+=";
                     os << in << "for (int q = 0; q < " << sum.n_iterations << "; ++q) {\n";
                     for (int k = 0; k < sum.stride; k++) {
@@ -351,7 +361,8 @@ namespace symx
                 os << "        const long long rem = (long long)p.n_elem - e0;\n";
                 os << "        const int nv = rem < 0 ? 0 : (rem > 64 ? 64 : (int)rem);\n";
                 os << "        symx_store_rows<" << n << ">(p.G, sg, e0, nv);\n";
-                if (ks.kind == KernelKind::PGH) os << "        symx_store_rows<" << n_ub * 9 << ">(p.H, sh, e0, nv);\n";
+                if (ks.kind == KernelKind::PGH && h32) os << "        symx_store_rows_f32<" << n_ub * 9 << ">((float*)p.H, sh, e0, nv);\n";
+                if (ks.kind == KernelKind::PGH && !h32) os << "        symx_store_rows<" << n_ub * 9 << ">(p.H, sh, e0, nv);\n";
                 os << "    }\n    symx_store_energy(e_out, p, e);\n}\n";
             }
             else {
@@ -398,8 +409,11 @@ namespace symx
         for (const auto& rec : fem.records)
             for (int k = 0; k < 9; k++) preset.push_back(rec.J[k]);
 
-        for (int kind = 0; kind < 3; kind++) {
-            const std::string& kname = kind == 0 ? kP : (kind == 1 ? kPG : kPGH);
+        // kinds: 0 P, 1 PG, 2 PGH, 3 PGH with FP32 Hessian storage ("_f32")
+        for (int kind4 = 0; kind4 < 4; kind4++) {
+            const bool h32 = kind4 == 3;
+            const int kind = h32 ? 2 : kind4;
+            const std::string kname = kind == 0 ? kP : (kind == 1 ? kPG : (h32 ? kPGH + "_f32" : kPGH));
             const int NM = kind == 0 ? 1 : (kind == 1 ? 10 : 55);
             std::vector<Scalar> outs = fem.f;
             if (kind >= 1) outs.insert(outs.end(), fem.P.begin(), fem.P.end());
@@ -479,14 +493,22 @@ namespace symx
                 os << "                            acc += dk * (A[" << name << "_sym(i * 3 + k, j * 3 + 0)] * db0 + A[" << name
                    << "_sym(i * 3 + k, j * 3 + 1)] * db1 + A[" << name << "_sym(i * 3 + k, j * 3 + 2)] * db2);\n";
                 os << "                        }\n                        h[i * 3 + j] += acc;\n                    }\n                }\n            }\n";
-                os << "            double* dst = p.H + (long long)e * " << NUB * 9 << " + ub * 9;\n";
-                os << "            #pragma unroll\n            for (int k = 0; k < 9; k++) dst[k] = h[k];\n        }\n";
+                if (h32) {
+                    os << "            float* dst = (float*)p.H + (long long)e * " << NUB * 9 << " + ub * 9;\n";
+                    os << "            #pragma unroll\n            for (int k = 0; k < 9; k++) dst[k] = (float)h[k];\n        }\n";
+                }
+                else {
+                    os << "            double* dst = p.H + (long long)e * " << NUB * 9 << " + ub * 9;\n";
+                    os << "            #pragma unroll\n            for (int k = 0; k < 9; k++) dst[k] = h[k];\n        }\n";
+                }
             }
             os << "    }\n";
             if (kind >= 1) {
                 os << "    else if (valid) {\n";
                 os << "        for (int t = lane; t < " << 3 * NN << "; t += 64) p.G[(long long)e * " << 3 * NN << " + t] = 0.0;\n";
-                if (kind == 2) os << "        for (int t = lane; t < " << NUB * 9 << "; t += 64) p.H[(long long)e * " << NUB * 9 << " + t] = 0.0;\n";
+                if (kind == 2)
+                    os << "        for (int t = lane; t < " << NUB * 9 << "; t += 64) " << (h32 ? "((float*)p.H)" : "p.H") << "[(long long)e * " << NUB * 9
+                       << " + t] = 0.0;\n";
                 os << "    }\n";
             }
             os << "    if (lane == 0) s_e[wv] = symx_element_energy(e_out, p, e);\n    __syncthreads();\n";

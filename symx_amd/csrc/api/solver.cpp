@@ -504,6 +504,8 @@ namespace symx
             d.kernel_PG = specs[1].name.c_str();
             d.kernel_PGH = specs[2].name.c_str();
             d.kernel_C = P.has_condition ? specs[3].name.c_str() : nullptr;
+            const std::string k_pgh32 = specs[2].name + "_f32";  // emitted beside every PGH kernel
+            d.kernel_PGH_f32 = k_pgh32.c_str();
             d.n_elem = n_elem;
             d.conn_stride = mws.conn.stride;
             d.conn = mws.conn.data();
@@ -702,7 +704,8 @@ namespace symx
             }
             {
                 auto t = logger->time("evaluate_P__dP_du__local_d2P_du2");
-                check(symx_hip_problem_eval(prob, SYMX_HIP_EVAL_PGH, &E0), hip, "evaluation");
+                // mixed precision stores the element Hessians as floats (FP64 arithmetic)
+                check(symx_hip_problem_eval(prob, precision_mixed ? (int32_t)SYMX_HIP_EVAL_PGH_F32 : (int32_t)SYMX_HIP_EVAL_PGH, &E0), hip, "evaluation");
             }
             const double residual_norm = compute_residual();
             out->print(sfmt("r0: %.2e | ", residual_norm), Verbosity::Medium);

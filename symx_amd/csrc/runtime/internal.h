@@ -102,6 +102,7 @@ namespace symx_hip
     {
         symx_hip_module module = nullptr;
         std::string kP, kPG, kPGH, kC;
+        std::string kPGH32;                 // PGH with FP32 Hessian storage (empty: none)
         int32_t n_elem = 0;
         int32_t stride = 0;
         std::vector<int32_t> h_conn;        // row-major copy (topology)
@@ -139,6 +140,7 @@ struct symx_hip_problem_s
     uint8_t* d_flags = nullptr;   // per-element scratch of the projection
     uint32_t* d_plist = nullptr;  // per-element scratch: elements left to the fallback projection
     bool h_valid = false;
+    bool h_f32 = false;           // element Hessians of the last PGH evaluation are stored as floats
     // max|grad| computed with the gradient by EVAL_PG / EVAL_PGH (read back with the energy),
     // valid until the gradient vector is written otherwise; unused once a raw device
     // pointer to the gradient has been handed out
@@ -212,10 +214,13 @@ namespace symx_hip
     // flags, list: per-element scratch (n_mats entries); n_list: one device counter.
     int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
                           uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count);
+    int launch_project_ub(hipStream_t s, float* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+                          uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count);
     int launch_project_dense(hipStream_t s, double* mats, int64_t n_mats, int32_t n, double eps, int32_t mirror, uint8_t* changed);
     // n entries of aslot/coff; mirror (may be NULL): slot that also receives the transposed block
+    // H32 (may be NULL): FP32 element-Hessian storage, read instead of H; sums run in FP64
     void launch_bsr_assemble(hipStream_t s, int64_t n, const uint32_t* aslot, const uint32_t* mirror, const uint32_t* coff,
-                             const uint32_t* clist, const double* H, double* v64, float* v32);
+                             const uint32_t* clist, const double* H, const float* H32, double* v64, float* v32);
     void launch_dinv(hipStream_t s, const SellPattern& P, const double* v64, const float* v32, double* dinv64, float* dinv32);
     // PCG on the sell matrix (iteration k is 1-based). partials: >= 5 * reduction_groups(n_rows) doubles.
     void launch_pcg_init64(hipStream_t s, const SellPattern& P, const double* dinv, const double* b, double* x, double* r, double* z,

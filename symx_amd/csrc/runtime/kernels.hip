@@ -238,8 +238,8 @@ namespace symx_hip
 
     // ---- PD projection (device code in projection.h) ---------------------------------
     // Full-size projection of an element Hessian stored as upper 3x3 blocks.
-    template <int NB>
-    __device__ __forceinline__ bool project_element_full(double* __restrict__ h, double eps, bool mirror)
+    template <int NB, typename T>
+    __device__ __forceinline__ bool project_element_full(T* __restrict__ h, double eps, bool mirror)
     {
         constexpr int N = 3 * NB;
         double a[Sym<N>::size];
@@ -282,6 +282,28 @@ namespace symx_hip
     // (project_to_PD.cpp:13-32). Elements whose block rows do not sum to zero, or with more
     // than three eigenvalues of M below eps, are handed to the workgroup-per-matrix kernel.
 
+    // An element Hessian counts as translation invariant when each block-row sum is below
+    // this fraction of max|H| (per block for the large kernel): round-off of FP64 sums, or the
+    // rounding of FP32 storage (2^-24 per entry, a few entries per sum). The dropped coupling
+    // with the translations is then at the level of the storage rounding.
+    template <typename T>
+    __host__ __device__ constexpr double translation_tol()
+    {
+        return sizeof(T) == 4 ? 1e-6 : 1e-10;
+    }
+
+    // Eigenvalues below this value are projected: eps (project_to_PD.cpp:13-32). With FP32
+    // storage the stored N x N matrix carries rounding of up to 2^-24 max|H| per entry, which
+    // moves its eigenvalues by up to N 2^-24 max|H| (Frobenius bound); eigenvalues in
+    // [-N 2^-24 max|H|, eps) are then indistinguishable from that rounding, and so is their
+    // projection (a change of H below the storage's own resolution), so they are left alone.
+    template <typename T, int N>
+    __device__ __forceinline__ double projection_cut(double eps, double scale)
+    {
+        if constexpr (sizeof(T) == 4) return fmin(eps, -(double)N * 0x1p-24 * scale);
+        else return eps;
+    }
+
     // Block (a, b), a <= b, of upper-block index u (inverse of ub4).
     __host__ __device__ constexpr int ub4_row(int u) { return u < 4 ? 0 : (u < 7 ? 1 : (u < 9 ? 2 : 3)); }
     __host__ __device__ constexpr int ub4_col(int u) { return u < 4 ? u : (u < 7 ? u - 3 : (u < 9 ? u - 5 : 3)); }
@@ -317,8 +339,8 @@ namespace symx_hip
     // when the element must take the workgroup path (more than three eigenvalues below eps,
     // or an eigenvalue that left its Sturm bracket). hv: 28 lane-interleaved LDS doubles
     // (stride 64) for the reflectors.
-    __device__ __forceinline__ bool r4_eigen_update(double (&m)[Sym<9>::size], double eps, bool mirror, double* hv, double (&y)[3][9],
-                                                    double (&dl)[3])
+    __device__ __forceinline__ bool r4_eigen_update(double (&m)[Sym<9>::size], double eps, double cut, bool mirror, double* hv,
+                                                    double (&y)[3][9], double (&dl)[3])
     {
         double d[9], e[9], tau[9];
         tridiag_reg<9>(m, d, e, tau);
@@ -343,7 +365,7 @@ namespace symx_hip
         glo -= 2.0 * ulp * tn + pivmin;
         int cnt;
         {
-            const double xe[3] = { eps, eps, eps };
+            const double xe[3] = { cut, cut, cut };
             int c3[3];
             sturm3_reg<9>(d, e, xe, pivmin, c3);
             cnt = c3[0];
@@ -351,7 +373,7 @@ namespace symx_hip
         if (cnt > 3) return false;
         // brackets of the cnt lowest eigenvalues by bisection (three in step) down to about
         // 1e-7 ||T||; Rayleigh-quotient iteration below finishes them
-        double lo[3] = { glo, glo, glo }, hi[3] = { eps, eps, eps };
+        double lo[3] = { glo, glo, glo }, hi[3] = { cut, cut, cut };
         for (int it = 0; it < 24; it++) {
             bool busy = false;
             static_for<0, 3>([&](auto Jc) { busy |= Jc() < cnt && hi[Jc()] - lo[Jc()] > tol; });
@@ -431,32 +453,45 @@ namespace symx_hip
         return true;
     }
 
-    // One wave's 64 consecutive 90-double records, values [c0, c0 + W): global <-> LDS
-    // stage[rec * 17 + k] with consecutive lanes on consecutive addresses (whole cache
-    // lines per instruction, instead of 64 lines when each lane reads its own record).
-    // Loads cover the records below nv; stores only those with use[rec] != 0.
-    template <int W>
-    __device__ __forceinline__ void r4_stage_in(const double* __restrict__ base, int nv, double* stage, int lane)
+    // One wave's 64 consecutive 90-value records, values [c0, c0 + W): global <-> LDS
+    // stage[rec * SS + k] with consecutive lanes on consecutive addresses (whole runs of a
+    // record per instruction, instead of 64 cache lines when each lane reads its own record).
+    // Loads cover the records below nv (nv >= 1); stores only those with use[rec] != 0.
+    template <int W, int SS, typename T>
+    __device__ __forceinline__ void r4_stage_in(const T* __restrict__ base, int nv, double* stage, int lane)
     {
-        double v[W];
+        T v[W];
+        if constexpr (sizeof(T) == 4) {
+            // Every lane loads (records past nv read the last valid record and are zeroed
+            // afterwards): a conditional load becomes a branch per load with the wait, and
+            // the widening to FP64, inside it, which serialises the loads. (For FP64 storage
+            // the conditional form schedules better within the 256-VGPR budget.)
 #pragma unroll
-        for (int s = 0; s < W; s++) {
-            const int k = lane + 64 * s, rec = k / W, q = k - rec * W;
-            v[s] = rec < nv ? base[(int64_t)rec * 90 + q] : 0.0;
+            for (int s = 0; s < W; s++) {
+                const int k = lane + 64 * s, rec = k / W, q = k - rec * W;
+                v[s] = base[(int64_t)(rec < nv ? rec : nv - 1) * 90 + q];
+            }
+        }
+        else {
+#pragma unroll
+            for (int s = 0; s < W; s++) {
+                const int k = lane + 64 * s, rec = k / W, q = k - rec * W;
+                v[s] = rec < nv ? base[(int64_t)rec * 90 + q] : T(0);
+            }
         }
 #pragma unroll
         for (int s = 0; s < W; s++) {
             const int k = lane + 64 * s, rec = k / W, q = k - rec * W;
-            stage[rec * 17 + q] = v[s];
+            stage[rec * SS + q] = rec < nv ? (double)v[s] : 0.0;
         }
     }
-    template <int W>
-    __device__ __forceinline__ void r4_stage_out(double* __restrict__ base, const int* use, const double* stage, int lane)
+    template <int W, int SS, typename T>
+    __device__ __forceinline__ void r4_stage_out(T* __restrict__ base, const int* use, const double* stage, int lane)
     {
 #pragma unroll 4
         for (int k = lane; k < 64 * W; k += 64) {
             const int rec = k / W, q = k - rec * W;
-            if (use[rec]) base[(int64_t)rec * 90 + q] = stage[rec * 17 + q];
+            if (use[rec]) base[(int64_t)rec * 90 + q] = (T)stage[rec * SS + q];
         }
     }
 
@@ -472,19 +507,22 @@ namespace symx_hip
     // the elements it hands back are flagged in needs_full. The element records are read and
     // written through LDS 16 values at a time with coalesced transfers: M, the block-row
     // sums and the scale accumulate chunk by chunk, and the update is added chunk by chunk.
-    __global__ __launch_bounds__(64) void k_project_ub4_reduced(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
+    template <typename TH>
+    __global__ __launch_bounds__(64) void k_project_ub4_reduced(TH* __restrict__ H, int64_t n_mats, double eps, int mirror,
                                                                unsigned long long* count, uint8_t* __restrict__ needs_full,
                                                                const uint8_t* __restrict__ select)
     {
         // reflectors during the eigen-solve (28 x 64, lane-interleaved); staging of the
         // element records (64 x 17) before and after it
-        __shared__ double s_lds[28 * 64];
+        // Chunks of CW values per record: 16 doubles or 32 floats (128-byte runs either way)
+        constexpr int CW = sizeof(TH) == 4 ? 32 : 16, SS = CW + 1, NCH = (90 + CW - 1) / CW;
+        __shared__ double s_lds[(64 * SS > 28 * 64) ? 64 * SS : 28 * 64];
         __shared__ int s_use[64];
         const int lane = threadIdx.x;
         const int64_t e0 = (int64_t)blockIdx.x * 64, e = e0 + lane;
         const bool active = e < n_mats && (select == nullptr || select[e]);
         const int nv = n_mats - e0 < 64 ? (int)(n_mats - e0) : 64;  // records in this wave
-        double* const base = H + e0 * 90;
+        TH* const base = H + e0 * 90;
         s_use[lane] = active ? 1 : 0;
         __syncthreads();
         bool ok = false;
@@ -493,17 +531,17 @@ namespace symx_hip
             double m[Sym<9>::size], rs[36], scale = 0.0;
             static_for<0, Sym<9>::size>([&](auto Kc) { m[Kc()] = 0.0; });
             static_for<0, 36>([&](auto Kc) { rs[Kc()] = 0.0; });
-            static_for<0, 6>([&](auto Cc) {
-                constexpr int c0 = 16 * decltype(Cc)::value, W = 90 - c0 < 16 ? 90 - c0 : 16;
-                r4_stage_in<W>(base + c0, nv, s_lds, lane);
+            static_for<0, NCH>([&](auto Cc) {
+                constexpr int c0 = CW * decltype(Cc)::value, W = 90 - c0 < CW ? 90 - c0 : CW;
+                r4_stage_in<W, SS>(base + c0, nv, s_lds, lane);
                 __syncthreads();
-                static_for<0, W>([&](auto Kc) { r4_take_value<c0 + decltype(Kc)::value>(s_lds[lane * 17 + Kc()], m, rs, scale); });
+                static_for<0, W>([&](auto Kc) { r4_take_value<c0 + decltype(Kc)::value>(s_lds[lane * SS + Kc()], m, rs, scale); });
                 __syncthreads();
             });
             if (active) {
                 double rowsum = 0.0;
                 static_for<0, 36>([&](auto Kc) { rowsum = fmax(rowsum, fabs(rs[Kc()])); });
-                ok = rowsum <= 1e-10 * scale && r4_eigen_update(m, eps, mirror != 0, s_lds + lane, y, dl);
+                ok = rowsum <= translation_tol<TH>() * scale && r4_eigen_update(m, eps, projection_cut<TH, 12>(eps, scale), mirror != 0, s_lds + lane, y, dl);
             }
         }
         if (e < n_mats) needs_full[e] = (active && !ok) ? 1 : 0;
@@ -528,9 +566,9 @@ namespace symx_hip
                     });
                 });
             });
-            static_for<0, 6>([&](auto Cc) {
-                constexpr int c0 = 16 * decltype(Cc)::value, W = 90 - c0 < 16 ? 90 - c0 : 16;
-                r4_stage_in<W>(base + c0, nv, s_lds, lane);
+            static_for<0, NCH>([&](auto Cc) {
+                constexpr int c0 = CW * decltype(Cc)::value, W = 90 - c0 < CW ? 90 - c0 : CW;
+                r4_stage_in<W, SS>(base + c0, nv, s_lds, lane);
                 __syncthreads();
                 if (ok) {
                     static_for<0, W>([&](auto Kc) {
@@ -541,19 +579,19 @@ namespace symx_hip
                             constexpr int S = decltype(Sc)::value;
                             acc += dl[S] * z[S][3 * a + i] * z[S][3 * b + j];
                         });
-                        s_lds[lane * 17 + Kc()] += acc;
+                        s_lds[lane * SS + Kc()] += acc;
                     });
                 }
                 __syncthreads();
-                r4_stage_out<W>(base + c0, s_use, s_lds, lane);
+                r4_stage_out<W, SS>(base + c0, s_use, s_lds, lane);
                 __syncthreads();
             });
         }
         count_changed(ok, count);
     }
 
-    template <int NB>
-    __global__ __launch_bounds__(64) void k_project_ub_full(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
+    template <int NB, typename T>
+    __global__ __launch_bounds__(64) void k_project_ub_full(T* __restrict__ H, int64_t n_mats, double eps, int mirror,
                                                            unsigned long long* count, const uint8_t* __restrict__ only)
     {
         constexpr int NUB = NB * (NB + 1) / 2;
@@ -654,8 +692,8 @@ namespace symx_hip
 
     // One matrix h (element e) with the whole workgroup of PT threads (a multiple of 64);
     // every early return is uniform.
-    template <int N, bool DENSE, int PT>
-    __device__ void project_large_one(double* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
+    template <int N, bool DENSE, int PT, typename T>
+    __device__ void project_large_one(T* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
                                       uint8_t* __restrict__ changed_out)
     {
         constexpr int NB = N / 3, LD = N | 1, NUB = NB * (NB + 1) / 2, NW = PT / 64, NPT = PT * PPTS;
@@ -683,10 +721,10 @@ namespace symx_hip
         for (int t = tid; t < N * N; t += PT) {
             const int r = t / N, c = t - r * N;
             double v;
-            if (DENSE) v = 0.5 * (h[r * N + c] + h[c * N + r]);
+            if (DENSE) v = 0.5 * ((double)h[r * N + c] + (double)h[c * N + r]);
             else {
                 const int a = r / 3, i = r - 3 * a, b = c / 3, j = c - 3 * b;
-                v = a <= b ? h[ubi(a, b) * 9 + 3 * i + j] : h[ubi(b, a) * 9 + 3 * j + i];
+                v = a <= b ? (double)h[ubi(a, b) * 9 + 3 * i + j] : (double)h[ubi(b, a) * 9 + 3 * j + i];
             }
             A[r * LD + c] = v;
         }
@@ -712,7 +750,7 @@ namespace symx_hip
             }
         }
         const double scale = block_max(amax);
-        const bool ti = !DENSE && block_max(rsum) <= 1e-10 * scale * NB && scale > 0.0;
+        const bool ti = !DENSE && block_max(rsum) <= translation_tol<T>() * scale * NB && scale > 0.0;
         if (!(scale > 0.0)) {
             // zero matrix: every eigenvalue (0) is below eps -> eps I (clamp), 0 (mirror)
             if (!mirror) {
@@ -813,8 +851,9 @@ namespace symx_hip
         const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
         const double tol = 4.0 * ulp * tn;
         glo -= 2.0 * ulp * tn + pivmin;
+        const double cut = DENSE ? eps : projection_cut<T, N>(eps, scale);
         int cnt, dummy;
-        sturm2<N>(dg, od, eps, eps, pivmin, cnt, dummy);
+        sturm2<N>(dg, od, cut, cut, pivmin, cnt, dummy);
         if (cnt == 0) {
             if (ti && !mirror) {
                 for (int t = tid; t < NUB * 9; t += PT) {
@@ -828,7 +867,7 @@ namespace symx_hip
         }
         // ---- 4. multisection for the cnt lowest eigenvalues ------------------------------
         {
-            const double w0 = (eps - glo) / (NPT + 1);
+            const double w0 = (cut - glo) / (NPT + 1);
             const int p0 = tid * PPTS;
             int c0, c1;
             sturm2<N>(dg, od, glo + w0 * (p0 + 1), glo + w0 * (p0 + 2), pivmin, c0, c1);
@@ -844,7 +883,7 @@ namespace symx_hip
                     else lo = mid + 1;
                 }
                 bra[j] = lo > 0 ? glo + w0 * lo : glo;
-                brb[j] = lo < NPT ? glo + w0 * (lo + 1) : eps;
+                brb[j] = lo < NPT ? glo + w0 * (lo + 1) : cut;
             }
             __syncthreads();
             const int P = min(64, NPT / cnt);
@@ -994,7 +1033,7 @@ namespace symx_hip
                 const int r = t / N, c = t - r * N;
                 double acc = 0.0;
                 for (int j = 0; j < cnt; j++) acc += lamv[j] * Y[j * N + r] * Y[j * N + c];
-                h[t] += acc;
+                h[t] = (T)((double)h[t] + acc);
             }
         }
         else {
@@ -1004,7 +1043,7 @@ namespace symx_hip
                 ub_index(t, r, c);
                 double acc = (r % 3) == (c % 3) ? tshift : 0.0;
                 for (int j = 0; j < cnt; j++) acc += lamv[j] * Y[j * N + r] * Y[j * N + c];
-                h[t] += acc;
+                h[t] = (T)((double)h[t] + acc);
             }
         }
         if (tid == 0) {
@@ -1013,8 +1052,8 @@ namespace symx_hip
         }
     }
 
-    template <int N, bool DENSE>
-    __global__ __launch_bounds__(PNT) void k_project_large(double* __restrict__ H, int64_t n_mats, double eps, int mirror,
+    template <int N, bool DENSE, typename T>
+    __global__ __launch_bounds__(PNT) void k_project_large(T* __restrict__ H, int64_t n_mats, double eps, int mirror,
                                                           unsigned long long* count, const uint8_t* __restrict__ select,
                                                           uint8_t* __restrict__ changed_out)
     {
@@ -1027,8 +1066,8 @@ namespace symx_hip
 
     // The elements listed in list[0 .. *n_list) (upper-block storage), a fixed grid of
     // PT-thread workgroups striding over the list (one wave per 12x12 matrix).
-    template <int N, int PT>
-    __global__ __launch_bounds__(PT) void k_project_large_list(double* __restrict__ H, const uint32_t* __restrict__ list,
+    template <int N, int PT, typename T>
+    __global__ __launch_bounds__(PT) void k_project_large_list(T* __restrict__ H, const uint32_t* __restrict__ list,
                                                               const unsigned int* __restrict__ n_list, double eps, int mirror,
                                                               unsigned long long* count)
     {
@@ -1051,29 +1090,41 @@ namespace symx_hip
     }
     __global__ void k_zero_counter(unsigned int* c) { *c = 0u; }
 
-    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
-                          uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count)
+    template <typename T>
+    int launch_project_ub_t(hipStream_t s, T* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+                            uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count)
     {
         if (n_mats <= 0) return 0;
         const unsigned g = (unsigned)((n_mats + 63) / 64);
         switch (nb) {
-        case 1: k_project_ub_full<1><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
-        case 2: k_project_ub_full<2><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
-        case 3: k_project_ub_full<3><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 1: k_project_ub_full<1, T><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 2: k_project_ub_full<2, T><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 3: k_project_ub_full<3, T><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
         case 4:
             // reduced 9x9 path in registers; the few elements it hands back (more than three
             // eigenvalues below eps, or not translation invariant) are listed and projected by
             // the workgroup-per-matrix kernel
-            k_project_ub4_reduced<<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags, select);
+            k_project_ub4_reduced<T><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags, select);
             k_zero_counter<<<1, 1, 0, s>>>(n_list);
             k_list_flagged<<<grid_for(n_mats), NT, 0, s>>>(flags, n_mats, list, n_list);
-            k_project_large_list<12, 64><<<4096, 64, 0, s>>>(H, list, n_list, eps, mirror, d_count);
+            k_project_large_list<12, 64, T><<<4096, 64, 0, s>>>(H, list, n_list, eps, mirror, d_count);
             return 0;
-        case 8: k_project_large<24, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
-        case 10: k_project_large<30, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
-        case 27: k_project_large<81, false><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
+        case 8: k_project_large<24, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
+        case 10: k_project_large<30, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
+        case 27: k_project_large<81, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
         default: return -1;
         }
+    }
+
+    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+                          uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count)
+    {
+        return launch_project_ub_t(s, H, n_mats, nb, eps, mirror, flags, list, n_list, select, d_count);
+    }
+    int launch_project_ub(hipStream_t s, float* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+                          uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count)
+    {
+        return launch_project_ub_t(s, H, n_mats, nb, eps, mirror, flags, list, n_list, select, d_count);
     }
 
     // ---- progressive projection (NewtonsMethod.cpp:306-330, ElementHessians.cpp:79-165) ----
@@ -1203,9 +1254,9 @@ namespace symx_hip
         case 6: k_project_dense<6><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
         case 9: k_project_dense<9><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
         case 12: k_project_dense<12><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
-        case 24: k_project_large<24, true><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
-        case 30: k_project_large<30, true><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
-        case 81: k_project_large<81, true><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
+        case 24: k_project_large<24, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
+        case 30: k_project_large<30, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
+        case 81: k_project_large<81, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
         default: return -1;
         }
     }
@@ -1213,10 +1264,10 @@ namespace symx_hip
     // ---- assembly: one thread per block-matrix slot, contributions in fixed order ----
     // Thread t assembles slot aslot[t] from contribution list t (slots in the assembly
     // traversal order, whole SELL slices at a time).
-    template <typename TS>
+    template <typename TS, typename TH>
     __global__ __launch_bounds__(NT) void k_bsr_assemble(int64_t n_slots, const uint32_t* __restrict__ aslot,
                                                          const uint32_t* __restrict__ mirror, const uint32_t* __restrict__ coff,
-                                                         const uint32_t* __restrict__ clist, const double* __restrict__ H,
+                                                         const uint32_t* __restrict__ clist, const TH* __restrict__ H,
                                                          TS* __restrict__ val)
     {
         const int64_t t = xcd_remap() * NT + threadIdx.x;
@@ -1228,17 +1279,17 @@ namespace symx_hip
         const uint32_t k1 = coff[t + 1];
         for (uint32_t k = coff[t]; k < k1; k++) {
             const uint32_t code = clist[k];
-            const double* h = H + (int64_t)(code >> 1) * 9;
+            const TH* h = H + (int64_t)(code >> 1) * 9;
             if (code & 1u) {
 #pragma unroll
                 for (int i = 0; i < 3; i++) {
 #pragma unroll
-                    for (int j = 0; j < 3; j++) acc[3 * i + j] += h[3 * j + i];
+                    for (int j = 0; j < 3; j++) acc[3 * i + j] += (double)h[3 * j + i];
                 }
             }
             else {
 #pragma unroll
-                for (int k2 = 0; k2 < 9; k2++) acc[k2] += h[k2];
+                for (int k2 = 0; k2 < 9; k2++) acc[k2] += (double)h[k2];
             }
         }
         TS* dst = val + (int64_t)(slot >> 6) * 576 + (slot & 63);
@@ -1259,11 +1310,16 @@ namespace symx_hip
     }
 
     void launch_bsr_assemble(hipStream_t s, int64_t n, const uint32_t* aslot, const uint32_t* mirror, const uint32_t* coff,
-                             const uint32_t* clist, const double* H, double* v64, float* v32)
+                             const uint32_t* clist, const double* H, const float* H32, double* v64, float* v32)
     {
         if (n <= 0) return;
-        if (v64) k_bsr_assemble<double><<<grid_for(n), NT, 0, s>>>(n, aslot, mirror, coff, clist, H, v64);
-        if (v32) k_bsr_assemble<float><<<grid_for(n), NT, 0, s>>>(n, aslot, mirror, coff, clist, H, v32);
+        if (H32) {
+            if (v64) k_bsr_assemble<double, float><<<grid_for(n), NT, 0, s>>>(n, aslot, mirror, coff, clist, H32, v64);
+            if (v32) k_bsr_assemble<float, float><<<grid_for(n), NT, 0, s>>>(n, aslot, mirror, coff, clist, H32, v32);
+            return;
+        }
+        if (v64) k_bsr_assemble<double, double><<<grid_for(n), NT, 0, s>>>(n, aslot, mirror, coff, clist, H, v64);
+        if (v32) k_bsr_assemble<float, double><<<grid_for(n), NT, 0, s>>>(n, aslot, mirror, coff, clist, H, v32);
     }
 
     // Closed-form inverse of the symmetric diagonal block of each row, computed in

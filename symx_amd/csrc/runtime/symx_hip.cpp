@@ -62,6 +62,7 @@ namespace
     int ensure_partials(symx_hip_ctx ctx, size_t n)
     {
         if (n <= ctx->partials_cap) return SYMX_HIP_OK;
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);  // queued reductions may still use the old buffer
         dfree(ctx->d_partials);
         const size_t cap = std::max<size_t>(n, 4096);
         int rc = dalloc(ctx, &ctx->d_partials, cap);
@@ -159,6 +160,31 @@ __device__ __forceinline__ void symx_store_rows(double* dst, const double (&v)[W
             if (t < cw) {
                 const int idx = t * 64 + lane, r = idx / cw, j = idx - r * cw;
                 if (r < n_valid) dst[(e0 + r) * W + c0 + j] = st[r * 17 + j];
+            }
+        }
+        __syncthreads();
+    }
+}
+// The same for rows stored as floats (FP32 element-Hessian storage): values are rounded
+// once, on the way into LDS, and a wave moves 32 values per row per chunk through its own
+// part of the same staging buffer (64 rows x 33 floats).
+template <int W, typename V>
+__device__ __forceinline__ void symx_store_rows_f32(float* dst, const V (&v)[W], long long e0, int n_valid)
+{
+    const int lane = threadIdx.x & 63;
+    float* st = reinterpret_cast<float*>(symx_stage_rows_buffer()) + (threadIdx.x >> 6) * (64 * 34);
+#pragma unroll
+    for (int c0 = 0; c0 < W; c0 += 32) {
+        const int cw = (W - c0) < 32 ? (W - c0) : 32;
+#pragma unroll
+        for (int j = 0; j < 32; j++)
+            if (j < cw) st[lane * 33 + j] = (float)v[c0 + j];
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 32; t++) {
+            if (t < cw) {
+                const int idx = t * 64 + lane, r = idx / cw, j = idx - r * cw;
+                if (r < n_valid) dst[(e0 + r) * W + c0 + j] = st[r * 33 + j];
             }
         }
         __syncthreads();
@@ -487,6 +513,7 @@ int symx_hip_problem_add_potential(symx_hip_problem p, const symx_hip_potential_
     pot.kPG = d->kernel_PG;
     pot.kPGH = d->kernel_PGH;
     pot.kC = d->kernel_C ? d->kernel_C : "";
+    pot.kPGH32 = d->kernel_PGH_f32 ? d->kernel_PGH_f32 : "";
     pot.n_elem = d->n_elem;
     pot.stride = d->conn_stride;
     pot.elems_per_group = d->elems_per_group == 0 ? SYMX_HIP_BLOCK : d->elems_per_group;
@@ -978,6 +1005,12 @@ int symx_hip_problem_device_ptr(symx_hip_problem p, int32_t vec, double** dptr)
     return SYMX_HIP_OK;
 }
 
+// FP32 element-Hessian storage reuses the FP64 buffer (it needs half of it).
+static float* h32_of(symx_hip_problem p, const DevPotential& pot)
+{
+    return reinterpret_cast<float*>(p->d_H) + pot.h_base * 9;
+}
+
 static void fill_params(symx_hip_problem p, DevPotential& pot, symx_hip_eval_params& prm)
 {
     std::memset(&prm, 0, sizeof(prm));
@@ -999,7 +1032,11 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
     symx_hip_ctx ctx = p->ctx;
     if (!p->finalized) return fail(ctx, SYMX_HIP_ERR_ARG, "problem not finalized");
-    if (mode < SYMX_HIP_EVAL_P || mode > SYMX_HIP_EVAL_PGH) return fail(ctx, SYMX_HIP_ERR_ARG, "bad evaluation mode");
+    if (mode < SYMX_HIP_EVAL_P || mode > SYMX_HIP_EVAL_PGH_F32) return fail(ctx, SYMX_HIP_ERR_ARG, "bad evaluation mode");
+    const bool f32 = mode == SYMX_HIP_EVAL_PGH_F32;
+    if (f32)
+        for (auto& pot : p->pots)
+            if (pot.n_elem > 0 && pot.kPGH32.empty()) return fail(ctx, SYMX_HIP_ERR_ARG, "a potential has no FP32-storage Hessian kernel");
     // PG / PGH keep the element energies: the reference point of symx_hip_problem_energy_change
     const int32_t e_mode = mode == SYMX_HIP_EVAL_P ? SYMX_HIP_EMODE_SUM : SYMX_HIP_EMODE_STORE;
     for (auto& pot : p->pots) {
@@ -1007,16 +1044,18 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
         symx_hip_eval_params prm;
         fill_params(p, pot, prm);
         prm.e_mode = e_mode;
+        if (f32) prm.H = reinterpret_cast<double*>(h32_of(p, pot));  // the _f32 kernels store floats
         if (!pot.kC.empty()) {
             int rc = launch_eval(ctx, pot.module, pot.kC, prm);
             if (rc) return rc;
         }
-        const std::string& k = mode == SYMX_HIP_EVAL_P ? pot.kP : (mode == SYMX_HIP_EVAL_PG ? pot.kPG : pot.kPGH);
+        const std::string& k = mode == SYMX_HIP_EVAL_P ? pot.kP : (mode == SYMX_HIP_EVAL_PG ? pot.kPG : (f32 ? pot.kPGH32 : pot.kPGH));
         int rc = launch_eval(ctx, pot.module, k, prm, pot.elems_per_group);
         if (rc) return rc;
     }
     if (e_mode == SYMX_HIP_EMODE_STORE) p->e_ref_valid = true;
-    if (mode == SYMX_HIP_EVAL_PGH && p->d_projected) launch_fill_u8(ctx->stream, p->d_projected, p->total_elem);
+    const bool with_h = mode >= SYMX_HIP_EVAL_PGH;
+    if (with_h && p->d_projected) launch_fill_u8(ctx->stream, p->d_projected, p->total_elem);
     launch_sum(ctx->stream, p->d_Epart, p->total_parts, ctx->d_scalars);
     const bool with_grad = mode >= SYMX_HIP_EVAL_PG;
     if (with_grad) {
@@ -1024,7 +1063,10 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
         // max|grad| (the Newton residual) comes back with the energy: one host round trip
         launch_max_abs(ctx->stream, p->n_dofs, p->d_vec[SYMX_HIP_VEC_GRAD], ctx->d_scalars + 1, ctx->d_partials);
     }
-    if (mode == SYMX_HIP_EVAL_PGH) p->h_valid = true;
+    if (with_h) {
+        p->h_valid = true;
+        p->h_f32 = f32;
+    }
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_scalars, sizeof(double) * (with_grad ? 2 : 1), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -1104,8 +1146,16 @@ int symx_hip_problem_element_outputs(symx_hip_problem p, int32_t pot_id, double*
     symx_hip_ctx ctx = p->ctx;
     if (!p->finalized || pot_id < 0 || pot_id >= (int32_t)p->pots.size()) return fail(ctx, SYMX_HIP_ERR_ARG, "bad potential id");
     auto& pot = p->pots[pot_id];
+    // the context's stream is non-blocking: a plain hipMemcpy does not wait for its kernels
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (G) HIP_TRY(ctx, hipMemcpy(G, p->d_G + pot.g_base * 3, sizeof(double) * pot.n_elem * pot.nb * 3, hipMemcpyDeviceToHost));
-    if (H) HIP_TRY(ctx, hipMemcpy(H, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToHost));
+    if (H && !p->h_f32) HIP_TRY(ctx, hipMemcpy(H, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToHost));
+    if (H && p->h_f32) {
+        const size_t n = (size_t)pot.n_elem * pot.n_ub * 9;
+        std::vector<float> f(n);
+        HIP_TRY(ctx, hipMemcpy(f.data(), h32_of(p, pot), sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) H[i] = (double)f[i];
+    }
     return SYMX_HIP_OK;
 }
 
@@ -1120,7 +1170,11 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
     for (auto& pot : p->pots) {
         // elements of a conditional potential that are inactive have no Hessian to project
         const uint8_t* only = pot.kC.empty() ? nullptr : pot.d_active;
-        if (launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters, only, d_count))
+        const int bad = p->h_f32 ? launch_project_ub(ctx->stream, h32_of(p, pot), pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters,
+                                                     only, d_count)
+                                 : launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist,
+                                                     ctx->d_counters, only, d_count);
+        if (bad)
             return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports 1-4, 8, 10 and 27 DoF blocks per element");
     }
     HIP_TRY(ctx, hipGetLastError());
@@ -1164,8 +1218,13 @@ int symx_hip_problem_project_selective(symx_hip_problem p, double eps, int32_t m
     unsigned long long* cnt = (unsigned long long*)(ctx->d_scalars + 8);  // [changed, newly projected, active blocks]
     HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), ctx->stream));
     launch_ppn_select(ctx->stream, p->d_goff, p->d_glist, p->d_grow, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb, threshold, pr, p->d_projected, p->d_select, cnt + 2);
-    for (auto& pot : p->pots)
-        launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters, p->d_select + pot.e_base, cnt);
+    for (auto& pot : p->pots) {
+        if (p->h_f32)
+            launch_project_ub(ctx->stream, h32_of(p, pot), pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters, p->d_select + pot.e_base, cnt);
+        else
+            launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters,
+                              p->d_select + pot.e_base, cnt);
+    }
     launch_ppn_commit(ctx->stream, p->d_projected, p->d_select, p->total_elem, cnt + 1);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -1185,13 +1244,14 @@ int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
     if (!p->h_valid) return fail(ctx, SYMX_HIP_ERR_ARG, "no element Hessians (evaluate with EVAL_PGH first)");
     const SellPattern& S = p->sell;
     const size_t nval = (size_t)S.n_slots * 9;
+    const float* h32 = p->h_f32 ? reinterpret_cast<const float*>(p->d_H) : nullptr;
     if (precision == SYMX_HIP_FP64) {
         if (!p->d_val64) {
             int rc = dalloc(ctx, &p->d_val64, nval);
             if (!rc) rc = dalloc(ctx, &p->d_dinv64, (size_t)S.n_rows * 6);
             if (rc) return rc;
         }
-        launch_bsr_assemble(ctx->stream, p->n_assemble, p->d_aslot, p->d_mirror, p->d_coff, p->d_clist, p->d_H, p->d_val64, nullptr);
+        launch_bsr_assemble(ctx->stream, p->n_assemble, p->d_aslot, p->d_mirror, p->d_coff, p->d_clist, p->d_H, h32, p->d_val64, nullptr);
         launch_dinv(ctx->stream, S, p->d_val64, nullptr, p->d_dinv64, nullptr);
     }
     else if (precision == SYMX_HIP_MIXED) {
@@ -1208,7 +1268,7 @@ int symx_hip_problem_assemble(symx_hip_problem p, int32_t precision)
             if (!rc) rc = dalloc(ctx, &p->f_b, nd);
             if (rc) return rc;
         }
-        launch_bsr_assemble(ctx->stream, p->n_assemble, p->d_aslot, p->d_mirror, p->d_coff, p->d_clist, p->d_H, nullptr, p->d_val32);
+        launch_bsr_assemble(ctx->stream, p->n_assemble, p->d_aslot, p->d_mirror, p->d_coff, p->d_clist, p->d_H, h32, nullptr, p->d_val32);
         launch_dinv(ctx->stream, S, nullptr, p->d_val32, nullptr, p->d_dinv32);
     }
     else {
@@ -1227,6 +1287,8 @@ int symx_hip_problem_export_bsr(symx_hip_problem p, int64_t* nnzb, int64_t* row_
     const SellPattern& S = p->sell;
     if (nnzb) *nnzb = S.nnzb;
     if (!row_ptr && !cols && !vals) return SYMX_HIP_OK;
+    // assembly is enqueued without waiting; the copies below run outside the context's stream
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<double> v;
     if (vals) {
         if (p->assembled_precision < 0) return fail(ctx, SYMX_HIP_ERR_ARG, "matrix not assembled");
@@ -1496,6 +1558,7 @@ int symx_hip_problem_set_plane_filter(symx_hip_problem p, const int32_t* blocks,
     if (n < 0 || (n > 0 && !blocks) || axis < 0 || axis > 2) return fail(ctx, SYMX_HIP_ERR_ARG, "bad plane filter");
     for (int32_t i = 0; i < n; i++)
         if (blocks[i] < 0 || blocks[i] >= p->nb) return fail(ctx, SYMX_HIP_ERR_ARG, "plane filter block out of range");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // no queued step-filter kernel may still read the old list
     dfree(p->d_filter_blocks);
     p->n_filter_blocks = 0;
     if (n > 0) {
@@ -1636,9 +1699,19 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
     DevPotential& pot = p->pots[pot_id];
     const SellPattern& S = p->sell;
     double* scratch = nullptr;
+    // phase 1 projects a copy of the element Hessians in their storage precision
+    const size_t h_vals = (size_t)pot.n_elem * pot.n_ub * 9;
+    const size_t h_bytes = h_vals * (p->h_f32 ? sizeof(float) : sizeof(double));
+    const void* h_src = p->h_f32 ? (const void*)h32_of(p, pot) : (const void*)(p->d_H + pot.h_base * 9);
+    auto project_scratch = [&]() {
+        return p->h_f32 ? launch_project_ub(ctx->stream, reinterpret_cast<float*>(scratch), pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist,
+                                            ctx->d_counters, nullptr, nullptr)
+                        : launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist, ctx->d_counters, nullptr,
+                                            nullptr);
+    };
     if (phase == 1) {
         if (!p->h_valid) return fail(ctx, SYMX_HIP_ERR_ARG, "no element Hessians");
-        int rc = dalloc(ctx, &scratch, (size_t)std::max<int64_t>((int64_t)pot.n_elem * pot.n_ub * 9, 1));
+        int rc = dalloc(ctx, &scratch, std::max<size_t>(h_vals, 1));
         if (rc) return rc;
     }
     if ((phase == 2 || phase == 3) && p->assembled_precision != arg) {
@@ -1649,17 +1722,24 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
         symx_hip_eval_params prm;
         fill_params(p, pot, prm);
         switch (phase) {
-        case 0: return launch_eval(ctx, pot.module, pot.kPGH, prm, pot.elems_per_group);
+        case 0:
+            if (arg == SYMX_HIP_MIXED) {
+                if (pot.kPGH32.empty()) return fail(ctx, SYMX_HIP_ERR_ARG, "no FP32-storage Hessian kernel");
+                prm.H = reinterpret_cast<double*>(h32_of(p, pot));
+                p->h_f32 = true;  // the element Hessians now hold this launch's FP32 values
+                return launch_eval(ctx, pot.module, pot.kPGH32, prm, pot.elems_per_group);
+            }
+            p->h_f32 = false;
+            return launch_eval(ctx, pot.module, pot.kPGH, prm, pot.elems_per_group);
         case 4: return launch_eval(ctx, pot.module, pot.kP, prm, pot.elems_per_group);
         case 1: {
             // project a fresh copy of the element Hessians each repetition
-            if (hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice,
-                               ctx->stream) != hipSuccess)
-                return SYMX_HIP_ERR_HIP;
-            return launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist, ctx->d_counters, nullptr, nullptr) ? SYMX_HIP_ERR_ARG : 0;
+            if (hipMemcpyAsync(scratch, h_src, h_bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) return SYMX_HIP_ERR_HIP;
+            return project_scratch() ? SYMX_HIP_ERR_ARG : 0;
         }
         case 2:
-            launch_bsr_assemble(ctx->stream, p->n_assemble, p->d_aslot, p->d_mirror, p->d_coff, p->d_clist, p->d_H, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
+            launch_bsr_assemble(ctx->stream, p->n_assemble, p->d_aslot, p->d_mirror, p->d_coff, p->d_clist, p->d_H,
+                                p->h_f32 ? reinterpret_cast<const float*>(p->d_H) : nullptr, arg == SYMX_HIP_FP64 ? p->d_val64 : nullptr,
                                 arg == SYMX_HIP_FP64 ? nullptr : p->d_val32);
             return 0;
         case 3:
@@ -1706,9 +1786,9 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
     if (phase == 1) {
         // time projection launches only (copies excluded): one event pair per repetition
         for (int i = 0; i < reps && !rc; i++) {
-            chk(hipMemcpyAsync(scratch, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToDevice, ctx->stream));
+            chk(hipMemcpyAsync(scratch, h_src, h_bytes, hipMemcpyDeviceToDevice, ctx->stream));
             chk(hipEventRecord(e0, ctx->stream));
-            launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist, ctx->d_counters, nullptr, nullptr);
+            project_scratch();
             chk(hipEventRecord(e1, ctx->stream));
             chk(hipEventSynchronize(e1));
             float ms = 0.0f;
