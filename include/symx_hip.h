@@ -101,9 +101,11 @@ const char* symx_hip_last_error(symx_hip_ctx ctx);
 const char* symx_hip_eval_prelude(void);
 /* Compile HIP source with hiprtc for `arch` (NULL = "gfx950") into a code
  * object cached under cache_dir as <sha256>.hsaco (key = SHA-256 of source,
- * arch and options). Needs no GPU. key_out (>= 65 bytes) receives the key;
- * *was_cached tells whether the cache hit. Replaces the g++ + dlopen JIT
- * (Compilation.cpp:84-127) and its SHA-256 cache check (:128-226). */
+ * hiprtc version, arch and options; the file holds the code's size and SHA-256
+ * ahead of it, and a file that does not match them is compiled again). Needs
+ * no GPU. key_out (>= 65 bytes) receives the key; *was_cached tells whether the
+ * cache hit. Replaces the g++ + dlopen JIT (Compilation.cpp:84-127) and its
+ * SHA-256 cache check (:128-226). */
 int symx_hip_jit_compile(const char* source, const char* arch, const char* cache_dir,
                          char* key_out, int* was_cached, size_t* code_size);
 /* Compile (or fetch from cache) and load onto the context's device. */

@@ -1340,19 +1340,16 @@ namespace symx_hip
         const TS c01 = m02 * m12 - m01 * m22;
         const TS c02 = m01 * m12 - m11 * m02;
         const TS det = m00 * c00 + m01 * c01 + m02 * c02;
+        // divided through like BlockedSparseMatrix.h:1198-1214: a singular diagonal block gives
+        // inf / NaN entries, and the PCG that uses them ends unconverged, as the reference's does
         TS inv[9];
-        if (det != (TS)0) {
-            const TS id = (TS)(1.0 / (double)det);
-            inv[0] = c00 * id;
-            inv[1] = c01 * id;
-            inv[2] = c02 * id;
-            inv[4] = (m00 * m22 - m02 * m02) * id;
-            inv[5] = (m01 * m02 - m00 * m12) * id;
-            inv[8] = (m00 * m11 - m01 * m01) * id;
-        }
-        else {
-            inv[0] = inv[1] = inv[2] = inv[4] = inv[5] = inv[8] = (TS)0;
-        }
+        const TS id = (TS)(1.0 / (double)det);
+        inv[0] = c00 * id;
+        inv[1] = c01 * id;
+        inv[2] = c02 * id;
+        inv[4] = (m00 * m22 - m02 * m02) * id;
+        inv[5] = (m01 * m02 - m00 * m12) * id;
+        inv[8] = (m00 * m11 - m01 * m01) * id;
         dinv[0 * (int64_t)n_rows + row] = inv[0];
         dinv[1 * (int64_t)n_rows + row] = inv[1];
         dinv[2 * (int64_t)n_rows + row] = inv[2];
@@ -1394,7 +1391,6 @@ namespace symx_hip
     // Partials: pa[parity][2*G] (r.r, r.z) written by init / U_k with parity k & 1,
     // pb[G] (p.q) written by S_k.
 
-    __device__ __forceinline__ int64_t pcg_groups(int64_t n_rows) { return n_rows <= 0 ? 1 : ((n_rows + NT - 1) / NT < MAX_GROUPS ? (n_rows + NT - 1) / NT : MAX_GROUPS); }
 
     template <typename T>
     __device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int64_t n_rows, int64_t row, const T (&rv)[3], T (&zv)[3])

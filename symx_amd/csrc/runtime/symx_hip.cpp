@@ -219,27 +219,68 @@ __device__ __forceinline__ void symx_store_rows_f32(float* dst, const V (&v)[W],
         }
     }
 
-    const std::vector<std::string>& jit_options(const std::string& arch)
+    std::vector<std::string> jit_options(const std::string& arch)
     {
-        static std::vector<std::string> opts;
-        opts = { "--offload-arch=" + arch, "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops" };
-        return opts;
+        return { "--offload-arch=" + arch, "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops" };
+    }
+
+    // Cached code objects: "SYMXHSA1", the code size as 16 hex digits, the SHA-256 of the
+    // code (64 hex digits), then the code. A file whose header does not match its contents
+    // (truncated, foreign or corrupted) is not used; the source is compiled again.
+    constexpr char kCacheMagic[] = "SYMXHSA1";
+    constexpr size_t kCacheHeader = 8 + 16 + 64;
+
+    bool read_cached_object(const std::string& path, std::vector<char>& code)
+    {
+        std::vector<char> raw;
+        if (!read_file(path, raw) || raw.size() < kCacheHeader || std::memcmp(raw.data(), kCacheMagic, 8) != 0) return false;
+        const std::string size_hex(raw.data() + 8, 16), sha_hex(raw.data() + 24, 64);
+        char* end = nullptr;
+        const unsigned long long n = std::strtoull(size_hex.c_str(), &end, 16);
+        if (end != size_hex.c_str() + 16 || n != raw.size() - kCacheHeader || n == 0) return false;
+        Sha256 sha;
+        sha.update(raw.data() + kCacheHeader, (size_t)n);
+        if (sha.hex() != sha_hex) return false;
+        code.assign(raw.begin() + kCacheHeader, raw.end());
+        return true;
+    }
+
+    void write_cached_object(const std::string& cache_dir, const std::string& path, const std::vector<char>& code)
+    {
+        mkdirs(cache_dir);
+        char size_hex[17];
+        std::snprintf(size_hex, sizeof(size_hex), "%016llx", (unsigned long long)code.size());
+        Sha256 sha;
+        sha.update(code.data(), code.size());
+        const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+        std::ofstream f(tmp, std::ios::binary);
+        f.write(kCacheMagic, 8);
+        f.write(size_hex, 16);
+        f.write(sha.hex().data(), 64);
+        f.write(code.data(), (std::streamsize)code.size());
+        f.close();
+        if (f) std::rename(tmp.c_str(), path.c_str());
+        else std::remove(tmp.c_str());
     }
 
     int jit_compile_impl(symx_hip_ctx ctx, const char* source, const char* arch_c, const char* cache_dir, std::vector<char>& code,
                          std::string& key, int& was_cached)
     {
         const std::string arch = arch_c ? arch_c : "gfx950";
-        const auto& opts = jit_options(arch);
+        const std::vector<std::string> opts = jit_options(arch);
+        // key: the source, the compiler (hiprtc version), the target and the options
+        int major = 0, minor = 0;
+        hiprtcVersion(&major, &minor);
         Sha256 sha;
         sha.update(source, std::strlen(source));
+        sha.update("hiprtc " + std::to_string(major) + "." + std::to_string(minor) + " " + arch);
         for (const auto& o : opts) sha.update(o);
         key = sha.hex();
         was_cached = 0;
         std::string path;
         if (cache_dir && cache_dir[0]) {
             path = std::string(cache_dir) + "/" + key + ".hsaco";
-            if (read_file(path, code)) {
+            if (read_cached_object(path, code)) {
                 was_cached = 1;
                 return SYMX_HIP_OK;
             }
@@ -260,14 +301,7 @@ __device__ __forceinline__ void symx_store_rows_f32(float* dst, const V (&v)[W],
         code.resize(n);
         hiprtcGetCode(prog, code.data());
         hiprtcDestroyProgram(&prog);
-        if (!path.empty()) {
-            mkdirs(cache_dir);
-            const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
-            std::ofstream f(tmp, std::ios::binary);
-            f.write(code.data(), (std::streamsize)code.size());
-            f.close();
-            std::rename(tmp.c_str(), path.c_str());
-        }
+        if (!path.empty()) write_cached_object(cache_dir, path, code);
         return SYMX_HIP_OK;
     }
 

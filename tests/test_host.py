@@ -91,3 +91,35 @@ def test_generated_module_compiles_for_gfx950():
         assert any(f.endswith(".hsaco") for f in files)
         size2, _ = symx_amd.host_jit_tet4(d)  # cache hit
         assert size2 == size
+
+
+def test_jit_cache_validates_cached_objects():
+    """A cached code object is used only when its stored size and SHA-256 match its
+    contents; a truncated or altered file is compiled again (Compilation.cpp:128-226
+    validates its embedded SHA the same way)."""
+    import ctypes
+
+    lib = symx_amd.hip()
+    src = (lib.symx_hip_eval_prelude().decode() +
+           'extern "C" __global__ void k_cache_probe(double* x) { x[threadIdx.x] = 2.0 * x[threadIdx.x]; }\n').encode()
+    with tempfile.TemporaryDirectory() as d:
+        key = ctypes.create_string_buffer(80)
+        cached, size = ctypes.c_int(), ctypes.c_size_t()
+
+        def jit():
+            assert lib.symx_hip_jit_compile(src, b"gfx950", d.encode(), key, ctypes.byref(cached), ctypes.byref(size)) == 0
+            return cached.value, size.value
+
+        c0, n0 = jit()
+        assert c0 == 0 and n0 > 0
+        assert jit() == (1, n0)
+        path = os.path.join(d, key.value.decode() + ".hsaco")
+        raw = open(path, "rb").read()
+        assert raw[:8] == b"SYMXHSA1" and len(raw) == 8 + 16 + 64 + n0
+        open(path, "wb").write(raw[:-100])  # truncated
+        assert jit() == (0, n0)
+        raw = bytearray(open(path, "rb").read())
+        raw[-1] ^= 0xFF  # one byte altered
+        open(path, "wb").write(bytes(raw))
+        assert jit() == (0, n0)
+        assert jit() == (1, n0)
