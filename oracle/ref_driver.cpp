@@ -14,6 +14,13 @@
 //   ref_driver adjoint <in.bin> <out.bin>
 //       Adjoint dL/dlog_mu of the quasistatic cantilever at given positions, composed
 //       from the reference's Hessian assembly, DirectLLT solver and Compiled<double>.
+//   ref_driver hex_element <in.bin> <out.bin>
+//       E, gradient and Hessian of one Neo-Hookean Hex8 / Hex27 element per case, built with
+//       the reference's fem_integrator / fem_jacobian (fem_integrators.cpp:41-133) and
+//       evaluated by SecondOrderCompiledGlobal (unprojected element Hessian).
+//   ref_driver solve_hex <in.bin> <out.bin>
+//       One implicit step of a hexahedral block (Neo-Hookean with per-element mu via
+//       fem_integrator, backward Euler inertia, penalty boundary) with NewtonsMethod.
 #include <symx>
 #include <Eigen/SparseCholesky>
 
@@ -353,6 +360,193 @@ static int run_adjoint(int argc, char** argv)
     return 0;
 }
 
+// Neo-Hookean hexahedron (nn = 8 or 27 nodes) through the reference's own isoparametric
+// integrator: sum over the element's quadrature points of psi(F) det(J_X) w.
+static Scalar nh_hex_energy(MappedWorkspace<double>& mws, int nn, std::vector<Vector>& x, std::vector<Vector>& X, const Scalar& l,
+                            const Scalar& m)
+{
+    const FEM_Element el = nn == 27 ? FEM_Element::Hex27 : FEM_Element::Hex8;
+    return fem_integrator(mws, el, [&](Scalar& w, Vector& xi) {
+        Matrix Jx = fem_jacobian(el, x, xi);
+        Matrix JX = fem_jacobian(el, X, xi);
+        Matrix F = Jx * JX.inv();
+        return nh_density(F, l, m) * JX.det() * w;
+    });
+}
+
+static int run_hex_element(int argc, char** argv)
+{
+    if (argc < 4) return 2;
+    std::ifstream f(argv[2], std::ios::binary);
+    int32_t hdr[2];
+    rd(f, hdr, 2);
+    const int nn = hdr[0], ncase = hdr[1];
+    if (nn != 8 && nn != 27) return 3;
+    const int n = 3 * nn;
+    struct
+    {
+        std::vector<Eigen::Vector3d> x, X;
+        std::vector<int32_t> conn;
+        double lambda = 0.0, mu = 0.0;
+    } d;
+    d.x.resize(nn);
+    d.X.resize(nn);
+    d.conn.resize(nn);
+    for (int a = 0; a < nn; a++) d.conn[a] = a;
+    spGlobalPotential G = GlobalPotential::create();
+    G->add_potential("nh_hex_element", d.conn, nn, [&](MappedWorkspace<double>& mws, Element& elem) {
+        std::vector<Vector> x = mws.make_vectors(d.x, elem);
+        std::vector<Vector> X = mws.make_vectors(d.X, elem);
+        Scalar l = mws.make_scalar(d.lambda);
+        Scalar m = mws.make_scalar(d.mu);
+        return nh_hex_energy(mws, nn, x, X, l, m);
+    });
+    G->add_dof(d.x);
+    spContext context = Context::create();
+    context->compilation_directory = symx::get_codegen_dir();
+    context->n_threads = 1;
+    SecondOrderCompiledGlobal compiled(G, context);
+    std::ofstream o(argv[3], std::ios::binary);
+    for (int c = 0; c < ncase; c++) {
+        double prm[2];
+        rd(f, prm, 2);
+        d.lambda = prm[0];
+        d.mu = prm[1];
+        std::vector<double> t((size_t)2 * n);
+        rd(f, t.data(), t.size());
+        for (int a = 0; a < nn; a++) {
+            d.x[a] = Eigen::Vector3d(t[3 * a], t[3 * a + 1], t[3 * a + 2]);
+            d.X[a] = Eigen::Vector3d(t[n + 3 * a], t[n + 3 * a + 1], t[n + 3 * a + 2]);
+        }
+        double E = 0.0;
+        Eigen::VectorXd grad;
+        spElementHessians eh = compiled.evaluate_P__dP_du__local_d2P_du2(E, grad);
+        ElementHessians::spBSM H = eh->assemble_global(1, n);
+        std::vector<Eigen::Triplet<double>> triplets;
+        H->to_triplets(triplets);
+        std::vector<double> dense((size_t)n * n, 0.0);
+        for (const auto& tr : triplets) dense[(size_t)tr.row() * n + tr.col()] += tr.value();
+        wr(o, &E, 1);
+        wr(o, grad.data(), (size_t)n);
+        wr(o, dense.data(), dense.size());
+    }
+    return 0;
+}
+
+static int run_solve_hex(int argc, char** argv)
+{
+    if (argc < 4) return 2;
+    std::ifstream f(argv[2], std::ios::binary);
+    int32_t hdr[9];
+    rd(f, hdr, 9);
+    const int nv = hdr[0], ne = hdr[1], nbc = hdr[2], has_inertia = hdr[3], threads = hdr[4], projection_mode = hdr[5], mirror = hdr[6],
+              max_iter = hdr[7], nn = hdr[8];
+    if (nn != 8 && nn != 27) return 3;
+    double prm[11];
+    rd(f, prm, 11);
+    struct Data
+    {
+        std::vector<Eigen::Vector3d> x, X, x0, v, bc_target;
+        std::vector<double> mass, mu;
+        std::vector<int32_t> elems, all;
+        LabelledConnectivity<2> bc{ { "constraint_idx", "vertex_idx" } };
+        double ratio, dt, k;
+        Eigen::Vector3d gravity;
+    } d;
+    d.ratio = prm[0];
+    d.dt = prm[1];
+    d.k = prm[2];
+    d.gravity = Eigen::Vector3d(prm[3], prm[4], prm[5]);
+    d.elems.resize((size_t)ne * nn);
+    rd(f, d.elems.data(), d.elems.size());
+    auto rdv = [&](std::vector<Eigen::Vector3d>& a, int n) {
+        std::vector<double> t((size_t)n * 3);
+        rd(f, t.data(), t.size());
+        a.resize(n);
+        for (int i = 0; i < n; i++) a[i] = Eigen::Vector3d(t[3 * i], t[3 * i + 1], t[3 * i + 2]);
+    };
+    rdv(d.X, nv);
+    rdv(d.x, nv);
+    rdv(d.x0, nv);
+    rdv(d.v, nv);
+    d.mass.resize(nv);
+    rd(f, d.mass.data(), nv);
+    d.mu.resize(ne);
+    rd(f, d.mu.data(), ne);
+    std::vector<int32_t> bcv(nbc);
+    rd(f, bcv.data(), nbc);
+    rdv(d.bc_target, nbc);
+    for (int i = 0; i < nbc; i++) d.bc.numbered_push_back({ bcv[i] });
+    d.all.resize(nv);
+    for (int i = 0; i < nv; i++) d.all[i] = i;
+    // connectivity with the element index first, so the per-element mu is gathered like a vertex value
+    std::vector<int32_t> elems_enum((size_t)ne * (nn + 1));
+    for (int e = 0; e < ne; e++) {
+        elems_enum[(size_t)e * (nn + 1)] = e;
+        for (int a = 0; a < nn; a++) elems_enum[(size_t)e * (nn + 1) + 1 + a] = d.elems[(size_t)e * nn + a];
+    }
+
+    spGlobalPotential G = GlobalPotential::create();
+    G->add_potential("neohookean_hex", elems_enum, nn + 1, [&](MappedWorkspace<double>& mws, Element& elem) {
+        std::vector<Vector> x = mws.make_vectors(d.x, elem.slice(1, nn + 1));
+        std::vector<Vector> X = mws.make_vectors(d.X, elem.slice(1, nn + 1));
+        Scalar m = mws.make_scalar(d.mu, elem[0]);
+        Scalar l = mws.make_scalar(d.ratio) * m;
+        return nh_hex_energy(mws, nn, x, X, l, m);
+    });
+    if (has_inertia) {
+        G->add_potential("be_inertia", d.all, 1, [&](MappedWorkspace<double>& mws, Element& elem) {
+            Vector x1 = mws.make_vector(d.x, elem[0]);
+            Vector x0 = mws.make_vector(d.x0, elem[0]);
+            Vector v0 = mws.make_vector(d.v, elem[0]);
+            Scalar m = mws.make_scalar(d.mass, elem[0]);
+            Vector g = mws.make_vector(d.gravity);
+            Scalar dts = mws.make_scalar(d.dt);
+            Vector xhat = x0 + dts * v0 + dts * dts * g;
+            Vector dev = x1 - xhat;
+            return 0.5 * m * dev.squared_norm() / dts.powN(2);
+        });
+    }
+    if (nbc > 0) {
+        G->add_potential("boundary_conditions", d.bc, [&](MappedWorkspace<double>& mws, Element& conn) {
+            Vector x = mws.make_vector(d.x, conn["vertex_idx"]);
+            Vector target = mws.make_vector(d.bc_target, conn["constraint_idx"]);
+            Scalar ks = mws.make_scalar(d.k);
+            return 0.5 * ks * (x - target).squared_norm();
+        });
+    }
+    G->add_dof(d.x);
+    spContext context = Context::create();
+    context->compilation_directory = symx::get_codegen_dir();
+    context->n_threads = threads;
+    NewtonsMethod newton(G, context);
+    const ProjectionToPD modes[4] = { ProjectionToPD::Newton, ProjectionToPD::ProjectedNewton, ProjectionToPD::ProjectOnDemand,
+                                      ProjectionToPD::Progressive };
+    newton.settings.projection_mode = modes[projection_mode >= 0 && projection_mode < 4 ? projection_mode : 1];
+    newton.settings.project_to_pd_use_mirroring = mirror != 0;
+    newton.settings.max_iterations = max_iter;
+    newton.settings.residual_tolerance_abs = prm[6];
+    newton.settings.residual_tolerance_rel = prm[7];
+    newton.settings.cg_rel_tolerance = prm[8];
+    newton.settings.cg_abs_tolerance = prm[9];
+    newton.settings.projection_eps = prm[10];
+    const auto t0 = std::chrono::steady_clock::now();
+    SolverReturn ret = newton.solve();
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const auto& st = newton.get_last_solve_stats();
+    const double E = std::numeric_limits<double>::quiet_NaN();
+    std::ofstream o(argv[3], std::ios::binary);
+    int32_t ohdr[4] = { (int32_t)ret, st.newton_iterations, st.cg_iterations, st.ls_bt_iterations };
+    wr(o, ohdr, 4);
+    double od[2] = { E, secs };
+    wr(o, od, 2);
+    std::vector<double> xo((size_t)nv * 3);
+    for (int i = 0; i < nv; i++)
+        for (int c = 0; c < 3; c++) xo[3 * (size_t)i + c] = d.x[i][c];
+    wr(o, xo.data(), xo.size());
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) return 2;
@@ -360,5 +554,7 @@ int main(int argc, char** argv)
     if (mode == "element") return run_element(argc, argv);
     if (mode == "solve") return run_solve(argc, argv);
     if (mode == "adjoint") return run_adjoint(argc, argv);
+    if (mode == "hex_element") return run_hex_element(argc, argv);
+    if (mode == "solve_hex") return run_solve_hex(argc, argv);
     return 2;
 }

@@ -8,6 +8,8 @@
 //                        potentials, one conditional, against the closed form)
 //   polynomial_newton    tests/test_newton_method.cpp:10-49
 //   compiled_ops         tests/test_compilation.cpp:16-94 (compiled vs host evaluation)
+//   loop_triangle_area   tests/test_loops.cpp:58-233 (CompiledInLoop against closed-form
+//                        triangle areas: summation, activity mask, element counts)
 // Each case prints "PASS <name>" or "FAIL <name>: <reason>"; the exit status is the
 // number of failed cases.
 #include <array>
@@ -233,6 +235,82 @@ namespace
             require(ok, "output " + std::to_string(i) + ": " + std::to_string(got[i]) + " vs " + std::to_string(expect[i]));
         }
     }
+
+    // A 5 x 5 quad grid over [-0.5, 0.5]^2 (z = 0), two triangles per quad (as
+    // generate_triangle_grid, mesh_generation.cpp:329-406, builds it up to the order of the
+    // triangles, which the checks below do not depend on).
+    void triangle_grid(std::vector<std::array<double, 3>>& V, std::vector<std::array<int32_t, 3>>& T)
+    {
+        const int q = 5, n = q + 1;
+        const double h = 1.0 / q;
+        V.clear();
+        T.clear();
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) V.push_back({ -0.5 + i * h, -0.5 + j * h, 0.0 });
+        for (int i = 0; i < q; i++)
+            for (int j = 0; j < q; j++) {
+                const int a = n * i + j, b = n * (i + 1) + j, c = n * i + j + 1, d = n * (i + 1) + j + 1;
+                T.push_back({ a, b, d });
+                T.push_back({ a, d, c });
+            }
+    }
+
+    void loop_triangle_area_case(bool use_conditional, int n_summations, int num_elements)
+    {
+        std::vector<std::array<double, 3>> V;
+        std::vector<std::array<int32_t, 3>> T;
+        triangle_grid(V, T);
+        if (num_elements > 0 && num_elements < (int)T.size()) T.resize(num_elements);
+        auto [mws, conn] = MappedWorkspace<double>::create(T);
+        std::vector<Vector> x = mws->make_vectors(V, conn);
+        const Scalar area = 0.5 * (x[1] - x[0]).cross3(x[2] - x[0]).norm();
+        std::vector<Scalar> exprs;
+        if (n_summations > 1) {
+            std::vector<std::array<double, 1>> it(n_summations, std::array<double, 1>{ 1.0 });
+            exprs.push_back(mws->add_for_each(it, [&](Vector& v) { return area * v[0]; }));
+        }
+        else {
+            exprs.push_back(area);
+        }
+        CompiledInLoop<double> loop(mws, exprs, "triangle_area_" + std::to_string(n_summations), get_codegen_dir());
+        const size_t ne = T.size();
+        std::vector<double> ref(ne);
+        std::vector<uint8_t> active(ne, 1);
+        double ref_sum = 0.0;
+        for (size_t e = 0; e < ne; e++) {
+            const auto &A = V[T[e][0]], &B = V[T[e][1]], &C = V[T[e][2]];
+            const double ux = B[0] - A[0], uy = B[1] - A[1], vx = C[0] - A[0], vy = C[1] - A[1];
+            ref[e] = 0.5 * std::abs(ux * vy - uy * vx) * n_summations;
+            const double cx = (A[0] + B[0] + C[0]) / 3.0;
+            if (use_conditional) active[e] = cx > 0.0 ? 1 : 0;
+            if (active[e]) ref_sum += ref[e];
+        }
+        std::vector<int> calls(ne, 0);
+        int32_t last = -1;
+        double sum = 0.0;
+        auto cb = [&](View<double> sol, int32_t e, int32_t, View<int32_t> c) {
+            require(e >= 0 && (size_t)e < ne, "element index out of range");
+            require(e > last, "callbacks out of element order");
+            last = e;
+            require(active[e] != 0, "callback for an inactive element " + std::to_string(e));
+            for (int k = 0; k < 3; k++) require(c[k] == T[e][k], "connectivity view of element " + std::to_string(e));
+            require(approx(sol[0], ref[e], 1e-13), "area of element " + std::to_string(e) + ": " + std::to_string(sol[0]) + " vs " + std::to_string(ref[e]));
+            calls[e]++;
+            sum += sol[0];
+        };
+        if (use_conditional) loop.run(4, cb, active);
+        else loop.run(4, cb);
+        for (size_t e = 0; e < ne; e++)
+            require(calls[e] == (active[e] ? 1 : 0), "element " + std::to_string(e) + " called " + std::to_string(calls[e]) + " times");
+        require(approx(sum, ref_sum, 1e-13), "area sum " + std::to_string(sum) + " vs " + std::to_string(ref_sum));
+    }
+
+    void loop_triangle_area()
+    {
+        for (bool cond : { false, true })
+            for (int ns : { 1, 3 })
+                for (int num : { -1, 1, 4, 8, 9 }) loop_triangle_area_case(cond, ns, num);
+    }
 }
 
 int main()
@@ -242,6 +320,7 @@ int main()
         { "conditional", conditional },
         { "polynomial_newton", polynomial_newton },
         { "compiled_ops", compiled_ops },
+        { "loop_triangle_area", loop_triangle_area },
     };
     int failed = 0;
     for (const auto& [name, f] : cases) {

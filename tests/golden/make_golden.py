@@ -14,6 +14,11 @@ build container:  python tests/golden/make_golden.py
   newton_ref_block6_{newton,pdn,ppn}.npz  the block6 case under the other projection
                              modes: Newton, ProjectOnDemand, Progressive
                              (`python tests/golden/make_golden.py modes`)
+  hex_ref_elements.npz       E, gradient, Hessian of Neo-Hookean Hex8 and Hex27 elements from
+                             the reference's fem_integrator (ref_driver hex_element)
+  hex27_ref_block2.npz       the reference NewtonsMethod (ProjectOnDemand) on a 2^3-cell Hex27
+                             block with per-element Young's moduli (ref_driver solve_hex)
+                             (`python tests/golden/make_golden.py hex` writes these two)
 """
 import json
 import os
@@ -108,7 +113,64 @@ def adjoint_case(name, cells):
     print(name, "adjoint", "max|dL/dp|", np.abs(res["dL_dlog_mu"]).max())
 
 
+def hex_cases():
+    """Perturbed Hex8 / Hex27 elements: scaled, rotated-and-stretched and noisy reference
+    cubes (HEX_NODES order), random materials."""
+    from ref_io import read_hex_element_output, write_hex_element_input
+    from scene_np import HEX_NODES
+
+    rng = np.random.default_rng(20261204)
+    out = {}
+    for nn in (8, 27):
+        base = np.array(HEX_NODES[:nn], float) / 2.0  # unit cube, nodes at 0, 0.5, 1
+        if nn == 8:
+            base = np.array([[a // 2, b // 2, c // 2] for (a, b, c) in HEX_NODES[:8]], float)
+        cases = []
+        for i in range(6):
+            h = 10.0 ** rng.uniform(-2, 0)
+            X = base * h + rng.uniform(-0.05, 0.05, base.shape) * h
+            A = np.eye(3) + rng.uniform(-0.15, 0.15, (3, 3))
+            x = X @ A.T + rng.uniform(-0.03, 0.03, X.shape) * h
+            E_, nu = 10.0 ** rng.uniform(4, 6), rng.uniform(0.2, 0.45)
+            lam, mu = E_ * nu / ((1 + nu) * (1 - 2 * nu)), E_ / (2 * (1 + nu))
+            cases.append((lam, mu, x, X))
+        with tempfile.TemporaryDirectory() as d:
+            inp, outp = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+            write_hex_element_input(inp, nn, cases)
+            subprocess.run([os.path.join(REF, "ref_driver"), "hex_element", inp, outp], check=True, env=ENV)
+            E, g, H = read_hex_element_output(outp, nn, len(cases))
+        out[f"x{nn}"] = np.stack([c[2] for c in cases])
+        out[f"X{nn}"] = np.stack([c[3] for c in cases])
+        out[f"lam{nn}"] = np.array([c[0] for c in cases])
+        out[f"mu{nn}"] = np.array([c[1] for c in cases])
+        out[f"E{nn}"], out[f"g{nn}"], out[f"H{nn}"] = E, g, H
+    np.savez_compressed(os.path.join(HERE, "hex_ref_elements.npz"), **out)
+    print("hex elements", {k: v.shape for k, v in out.items()})
+
+
+def hex_newton_case(name, cells, mode=2, rel_tol=1e-6, threads=1):
+    from ref_io import read_solve_output, write_solve_hex_input
+    from scene_np import hex_block
+    from symx_amd import NewtonParams
+
+    S = hex_block(cells=cells, nn=27)
+    newton = NewtonParams(0, mode, 0, 100, 0.0, rel_tol, 1e-3, 1e-12, 10000, 1e-10, 0.0)
+    with tempfile.TemporaryDirectory() as d:
+        inp, outp = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        write_solve_hex_input(inp, S, newton, threads)
+        subprocess.run([os.path.join(REF, "ref_driver"), "solve_hex", inp, outp], check=True, env=ENV)
+        res = read_solve_output(outp, S.X.shape[0])
+    np.savez_compressed(os.path.join(HERE, f"hex27_ref_{name}.npz"), x=res["x"], cells=np.array(cells), rel_tol=rel_tol, result=res["result"],
+                        newton_iterations=res["newton_iterations"], cg_iterations=res["cg_iterations"],
+                        ls_bt_iterations=res["ls_bt_iterations"], seconds=res["seconds"], threads=threads, projection_mode=mode)
+    print(name, {k: v for k, v in res.items() if k != "x"})
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "hex":
+        hex_cases()
+        hex_newton_case("block2", (2, 2, 2))
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "adjoint":
         adjoint_case("cantilever16", (15, 3, 3))
         sys.exit(0)
