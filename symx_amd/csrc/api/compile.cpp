@@ -542,6 +542,36 @@ namespace symx
         return os.str();
     }
 
+    std::string emit_host_function(const std::string& name, Graph& g, const std::vector<Scalar>& outputs, int32_t n_inputs,
+                                   const std::string& sha256_hex)
+    {
+        std::vector<int32_t> roots;
+        for (const Scalar& s : outputs) roots.push_back(s.id);
+        const std::vector<uint8_t> mark = reachable(g, roots);
+        std::ostringstream os;
+        os << "// generated by symx_amd: " << name << "\n#include <cmath>\n#include <cstring>\nusing namespace std;\n"
+           << "static inline double symx_ln(double x) { return (x <= 0.0) ? -INFINITY : log(x); }\n"
+           << "static inline double symx_log10(double x) { return (x <= 0.0) ? -INFINITY : log10(x); }\n"
+           << "static inline double symx_powi(double x, int n) { const bool ng = n < 0; unsigned m = ng ? -n : n; double r = 1.0, b = x;"
+              " while (m) { if (m & 1u) r *= b; b *= b; m >>= 1; } return ng ? 1.0 / r : r; }\n";
+        os << "extern \"C\" void " << name << "(const double* in, double* out)\n{\n";
+        for (int32_t id = 0; id < g.size(); id++) {
+            if (mark[id] && g.node(id).op == Op::Symbol) {
+                const int s = g.node(id).a;
+                if (s >= n_inputs) throw std::runtime_error("symx: symbol outside the input range");
+                os << "    const double s" << s << " = in[" << s << "];\n";
+            }
+        }
+        emit_body(os, g, mark, "    ");
+        for (size_t k = 0; k < outputs.size(); k++) os << "    out[" << k << "] = " << ref(g, outputs[k].id) << ";\n";
+        os << "}\n";
+        os << "extern \"C\" int get_n_inputs() { return " << n_inputs << "; }\n";
+        os << "extern \"C\" int get_n_outputs() { return " << outputs.size() << "; }\n";
+        os << "extern \"C\" int get_float_type() { return 0; }\n";  // FloatType::Double
+        os << "extern \"C\" void get_sha256(char* out) { std::memcpy(out, \"" << sha256_hex << "\", 64); }\n";
+        return os.str();
+    }
+
     int64_t count_operations(Graph& g, const std::vector<Scalar>& outputs)
     {
         std::vector<int32_t> roots;

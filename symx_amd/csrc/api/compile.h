@@ -303,6 +303,11 @@ namespace symx
     // Emit a standalone kernel evaluating `outputs` for a batch of input vectors
     // (Compiled<double>): in [n][n_inputs], out [n][n_outputs].
     std::string emit_hip_batch_function(const std::string& name, Graph& g, const std::vector<Scalar>& outputs, int32_t n_inputs);
+    // Host C++ translation unit of the same expressions for single host calls (the reference's
+    // own JIT target, Compilation.cpp:243-336): extern "C" void name(const double* in, double* out),
+    // get_n_inputs(), get_n_outputs(), get_float_type() and get_sha256(char* 64 hex digits).
+    std::string emit_host_function(const std::string& name, Graph& g, const std::vector<Scalar>& outputs, int32_t n_inputs,
+                                   const std::string& sha256_hex);
 
     // Number of SSA operations that a kernel with these outputs executes (for reports).
     int64_t count_operations(Graph& g, const std::vector<Scalar>& outputs);

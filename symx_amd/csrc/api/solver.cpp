@@ -1,13 +1,20 @@
 #include "solver.h"
 
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
+
+#include "../runtime/sha256.h"
 
 namespace symx
 {
@@ -995,60 +1002,230 @@ namespace symx
         }
     }
 
-    // ---- Compiled<double> -----------------------------------------------------------------
+    // ---- Compiled<double> (Compiled.cpp, Compilation.cpp) ----------------------------------
     namespace
     {
+        bool g_check_mode = false;
+
         symx_hip_ctx default_ctx()
         {
             static symx_hip_ctx c = nullptr;
             if (!c) check(symx_hip_ctx_create(0, &c), nullptr, "creating the HIP context");
             return c;
         }
+        std::string sha256_hex(const std::string& s)
+        {
+            symx_hip::Sha256 h;
+            h.update(s);
+            return h.hex();
+        }
+        // The host compiler of single-call code: SYMX_AMD_HOST_CXX, else CXX, else g++ (the
+        // reference's JIT runs the system compiler the same way, Compilation.cpp:84-127).
+        std::string host_compiler()
+        {
+            if (const char* c = std::getenv("SYMX_AMD_HOST_CXX")) return c;
+            if (const char* c = std::getenv("CXX")) return c;
+            return "g++";
+        }
+        const char* kHostFlags = "-O3 -march=native -std=c++17 -fPIC -shared";
+        bool write_text(const std::string& path, const std::string& text)
+        {
+            const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+            std::ofstream f(tmp);
+            f << text;
+            f.close();
+            if (!f) return false;
+            return std::rename(tmp.c_str(), path.c_str()) == 0;
+        }
+        bool read_text(const std::string& path, std::string& text)
+        {
+            std::ifstream f(path);
+            if (!f) return false;
+            std::ostringstream ss;
+            ss << f.rdbuf();
+            text = ss.str();
+            return true;
+        }
     }
-    Compiled<double>::Compiled(const std::vector<Scalar>& exprs, const std::string& name_, const std::string& folder, const std::string& cache_id)
-        : name(identifier(name_))
+
+    void check_mode_ON(bool check) { g_check_mode = check; }
+    bool get_check_mode_ON() { return g_check_mode; }
+
+    void CompilationInfo::print() const
     {
-        (void)cache_id;
-        if (exprs.empty()) throw std::runtime_error("symx: Compiled<double> without expressions");
-        Graph* g = exprs[0].g;
-        n_in = g->n_symbols();
-        n_out = (int32_t)exprs.size();
-        in.assign(n_in, 0.0);
-        out.assign(n_out, 0.0);
-        const std::string src = emit_hip_batch_function(name, *g, exprs, n_in);
-        ctx = default_ctx();
-        char key[80];
-        int wc = 0;
-        size_t cs = 0;
-        const std::string dir = folder.empty() ? get_codegen_dir() : folder;
-        check(symx_hip_jit_compile(src.c_str(), "gfx950", dir.c_str(), key, &wc, &cs), nullptr, "JIT");
-        cached = wc != 0;
-        check(symx_hip_module_load(ctx, src.c_str(), dir.c_str(), &mod), ctx, "module load");
+        std::printf("n_inputs: %d\nn_outputs: %d\nwas_cached: %d\nruntime_codegen: %.6f s\nruntime_compilation: %.6f s\nn_bytes_symbols: %d\n", n_inputs,
+                    n_outputs, was_cached ? 1 : 0, runtime_codegen, runtime_compilation, n_bytes_symbols);
     }
-    Compiled<double>::~Compiled()
+
+    Compiled<double>::Compiled(const std::vector<Scalar>& expr, const std::string& name_, const std::string& folder, const std::string& cache_id)
+    {
+        try_load_otherwise_compile(expr, name_, folder, cache_id);
+    }
+    Compiled<double>::~Compiled() { release(); }
+
+    void Compiled<double>::release()
     {
         if (mod) symx_hip_module_unload(mod);
+        mod = nullptr;
+        if (lib) dlclose(lib);
+        lib = nullptr;
+        host_f = nullptr;
     }
-    void Compiled<double>::set(const Scalar& s, double v)
+
+    void Compiled<double>::resize_buffers()
+    {
+        thread_input.assign(n_threads, std::vector<double>(n_in, 0.0));
+        thread_output.assign(n_threads, std::vector<double>(n_out, 0.0));
+    }
+
+    void Compiled<double>::set_n_threads(int n)
+    {
+        if (n < 1) throw std::runtime_error("symx: Compiled::set_n_threads needs at least one thread");
+        n_threads = n;
+        thread_input.resize(n, std::vector<double>(n_in, 0.0));
+        thread_output.resize(n, std::vector<double>(n_out, 0.0));
+    }
+
+    // Files of one compiled version: <folder>/<name>_<sha:16>.{cpp,so,hip}; the shared object
+    // exports get_sha256, which must equal the full SHA-256 of (cache_id + float type).
+    bool Compiled<double>::load_files(const std::string& name_, const std::string& folder, const std::string& sha)
+    {
+        const std::string b = folder + "/" + name_ + "_" + sha.substr(0, 16);
+        std::ifstream probe(b + ".so");
+        if (!probe.good()) return false;
+        void* h = dlopen((b + ".so").c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!h) return false;
+        auto get_sha = reinterpret_cast<void (*)(char*)>(dlsym(h, "get_sha256"));
+        auto get_ni = reinterpret_cast<int (*)()>(dlsym(h, "get_n_inputs"));
+        auto get_no = reinterpret_cast<int (*)()>(dlsym(h, "get_n_outputs"));
+        auto f = reinterpret_cast<void (*)(const double*, double*)>(dlsym(h, identifier(name_).c_str()));
+        std::string got(64, '\0');
+        if (get_sha) get_sha(&got[0]);
+        std::string dsrc;
+        if (!get_sha || !get_ni || !get_no || !f || got != sha || !read_text(b + ".hip", dsrc)) {
+            dlclose(h);
+            return false;
+        }
+        release();
+        lib = h;
+        host_f = f;
+        n_in = get_ni();
+        n_out = get_no();
+        name = name_;
+        base = b;
+        device_source = dsrc;
+        return true;
+    }
+
+    bool Compiled<double>::load_if_cached(const std::string& name_, const std::string& folder, const std::string& cache_id)
+    {
+        if (cache_id.empty()) throw std::runtime_error("symx: Compiled::load_if_cached needs a cache_id");
+        const std::string dir = folder.empty() ? get_codegen_dir() : folder;
+        if (!load_files(name_, dir, sha256_hex(cache_id + "0"))) return false;
+        info = CompilationInfo{};
+        info.n_inputs = n_in;
+        info.n_outputs = n_out;
+        info.was_cached = true;
+        info.n_bytes_symbols = 8 * n_in;
+        resize_buffers();
+        return true;
+    }
+
+    void Compiled<double>::try_load_otherwise_compile(const std::vector<Scalar>& expr, const std::string& name_, const std::string& folder,
+                                                      const std::string& cache_id)
+    {
+        if (expr.empty()) throw std::runtime_error("symx: Compiled<double> without expressions");
+        // without a cache_id the content of the generated code identifies the version
+        // (the reference hashes the expressions, Compilation.cpp:227-232)
+        std::string id = cache_id;
+        if (id.empty()) {
+            Graph* g = expr[0].g;
+            id = sha256_hex(emit_host_function(identifier(name_), *g, expr, g->n_symbols(), std::string(64, '0')));
+        }
+        if (!load_if_cached(name_, folder, id)) compile(expr, name_, folder, id);
+    }
+
+    void Compiled<double>::compile(const std::vector<Scalar>& expr, const std::string& name_, const std::string& folder, const std::string& cache_id)
+    {
+        if (expr.empty()) throw std::runtime_error("symx: Compiled<double> without expressions");
+        const auto t0 = std::chrono::steady_clock::now();
+        Graph* g = expr[0].g;
+        const std::string fname = identifier(name_);
+        const std::string dir = folder.empty() ? get_codegen_dir() : folder;
+        const std::string id = cache_id.empty() ? sha256_hex(emit_host_function(fname, *g, expr, g->n_symbols(), std::string(64, '0'))) : cache_id;
+        const std::string sha = sha256_hex(id + "0");
+        const std::string host_src = emit_host_function(fname, *g, expr, g->n_symbols(), sha);
+        const std::string dev_src = emit_hip_batch_function(fname, *g, expr, g->n_symbols());
+        const auto t1 = std::chrono::steady_clock::now();
+        mkdir(dir.c_str(), 0755);
+        const std::string b = dir + "/" + name_ + "_" + sha.substr(0, 16);
+        if (!write_text(b + ".cpp", host_src) || !write_text(b + ".hip", dev_src))
+            throw std::runtime_error("symx: cannot write generated code to " + dir);
+        const std::string so_tmp = b + ".so.tmp" + std::to_string((long long)getpid());
+        const std::string cmd = host_compiler() + " " + kHostFlags + " -o \"" + so_tmp + "\" \"" + b + ".cpp\" 2>&1";
+        std::string log;
+        int rc = -1;
+        if (FILE* pipe = popen(cmd.c_str(), "r")) {
+            char buf[512];
+            while (std::fgets(buf, sizeof(buf), pipe)) log += buf;
+            rc = pclose(pipe);
+        }
+        if (rc != 0 || std::rename(so_tmp.c_str(), (b + ".so").c_str()) != 0) {
+            std::remove(so_tmp.c_str());
+            throw std::runtime_error("symx: host compilation of '" + name_ + "' failed (" + cmd + "):\n" + log);
+        }
+        const auto t2 = std::chrono::steady_clock::now();
+        if (!load_files(name_, dir, sha)) throw std::runtime_error("symx: cannot load the compiled code of '" + name_ + "' from " + b + ".so");
+        info = CompilationInfo{};
+        info.n_inputs = n_in;
+        info.n_outputs = n_out;
+        info.was_cached = false;
+        info.runtime_codegen = std::chrono::duration<double>(t1 - t0).count();
+        info.runtime_compilation = std::chrono::duration<double>(t2 - t1).count();
+        info.n_bytes_symbols = 8 * n_in;
+        resize_buffers();
+    }
+
+    void Compiled<double>::set(const Scalar& s, const double& v, int thread_id)
     {
         if (!s.is_symbol()) throw std::runtime_error("symx: Compiled::set on a non-symbol");
-        in[s.get_symbol_idx()] = v;
+        if (thread_id < 0 || thread_id >= n_threads) throw std::runtime_error("symx: Compiled::set thread id out of range");
+        const int32_t i = s.get_symbol_idx();
+        if (i >= n_in) throw std::runtime_error("symx: Compiled::set symbol outside the compiled workspace");
+        thread_input[thread_id][i] = v;
     }
-    void Compiled<double>::set(const Vector& v, const double* values)
+
+    void Compiled<double>::set(const Matrix& matrix, const double* v, int thread_id)
     {
-        for (int32_t i = 0; i < v.size(); i++) set(v[i], values[i]);
+        for (int32_t i = 0; i < matrix.rows(); i++)
+            for (int32_t j = 0; j < matrix.cols(); j++) set(matrix(i, j), v[i * matrix.cols() + j], thread_id);
     }
-    const std::vector<double>& Compiled<double>::run()
+
+    View<double> Compiled<double>::run(int thread_id)
     {
-        check(symx_hip_batch_eval(ctx, mod, name.c_str(), in.data(), 1, n_in, out.data(), n_out), ctx, "run");
-        for (double x : out) {
-            if (std::isnan(x)) throw std::runtime_error("symx: Compiled::run() produced NaN");
-        }
-        return out;
+        if (!is_valid()) throw std::runtime_error("Cannot run invalid compilation for '" + name + "'");
+        if (thread_id < 0 || thread_id >= n_threads) throw std::runtime_error("symx: Compiled::run thread id out of range");
+        std::vector<double>& in = thread_input[thread_id];
+        std::vector<double>& out = thread_output[thread_id];
+        if (g_check_mode)
+            for (size_t i = 0; i < in.size(); i++)
+                if (std::isnan(in[i])) throw std::runtime_error("NaN detected in input #" + std::to_string(i) + " of (compiled function '" + name + "')");
+        host_f(in.data(), out.data());
+        if (g_check_mode)
+            for (size_t i = 0; i < out.size(); i++)
+                if (std::isnan(out[i])) throw std::runtime_error("NaN detected in output #" + std::to_string(i) + " of (compiled function '" + name + "')");
+        return View<double>(out.data(), (int32_t)out.size());
     }
+
     void Compiled<double>::run_batch(const double* inputs, int64_t n, double* outputs)
     {
-        check(symx_hip_batch_eval(ctx, mod, name.c_str(), inputs, n, n_in, outputs, n_out), ctx, "run_batch");
+        if (!is_valid()) throw std::runtime_error("Cannot run invalid compilation for '" + name + "'");
+        if (!mod) {
+            ctx = default_ctx();
+            const std::string dir = base.substr(0, base.find_last_of('/'));
+            check(symx_hip_module_load(ctx, device_source.c_str(), dir.c_str(), &mod), ctx, "module load");
+        }
+        check(symx_hip_batch_eval(ctx, mod, identifier(name).c_str(), inputs, n, n_in, outputs, n_out), ctx, "run_batch");
     }
 
     // ---- host BSR ----------------------------------------------------------------------

@@ -11,6 +11,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/symx_hip.h"
@@ -533,7 +534,28 @@ namespace symx
     using spNewtonsMethod = std::shared_ptr<NewtonsMethod>;
 
     // ---- single-expression runtime (Compiled.h) ----------------------------------------
-    // Evaluates a list of expressions on the GPU for one or many input vectors.
+    // NaN checks of Compiled<double> inputs and outputs (static_methods.h:15-16; off by default).
+    void check_mode_ON(bool check = true);
+    bool get_check_mode_ON();
+
+    // Compilation::Info (Compilation.h:47-57).
+    struct CompilationInfo
+    {
+        int n_inputs = -1;
+        int n_outputs = -1;
+        bool was_cached = false;
+        double runtime_codegen = 0.0;      // seconds to emit the code
+        double runtime_compilation = 0.0;  // seconds in the host compiler
+        int n_bytes_symbols = 0;
+        void print() const;
+    };
+
+    // Compiled<double> (Compiled.h:44-91): a list of expressions of one workspace evaluated
+    // for the values set by the caller. Single calls (run) execute host code compiled from
+    // the expressions, like the reference's own JIT (a C++ translation unit built by the
+    // system compiler, cached in `folder` and validated by its embedded SHA-256); batches
+    // (run_batch) run on the GPU, whose module is built on the first batch. Per-thread input
+    // and output buffers as in the reference (set_n_threads, the thread_id overloads).
     template <typename T>
     class Compiled;
 
@@ -541,31 +563,81 @@ namespace symx
     class Compiled<double>
     {
     public:
-        Compiled(const std::vector<Scalar>& expressions, const std::string& name, const std::string& folder = "", const std::string& cache_id = "");
+        Compiled() = default;
+        Compiled(const Compiled&) = delete;
+        Compiled(const std::vector<Scalar>& expr, const std::string& name, const std::string& folder, const std::string& cache_id = "");
         ~Compiled();
-        void set(const Scalar& symbol, double value);
-        void set(const Vector& symbol, const double* values);
-        template <typename V>
-        void set(const Vector& symbol, const V& values) { set(symbol, values.data()); }
-        // Evaluate at the values set (one launch); returns the outputs.
-        const std::vector<double>& run();
-        // Evaluate n input vectors ([n][n_inputs] -> [n][n_outputs]).
+
+        void compile(const std::vector<Scalar>& expr, const std::string& name, const std::string& folder, const std::string& cache_id = "");
+        bool load_if_cached(const std::string& name, const std::string& folder, const std::string& cache_id);
+        void try_load_otherwise_compile(const std::vector<Scalar>& expr, const std::string& name, const std::string& folder,
+                                        const std::string& cache_id = "");
+
+        bool is_valid() const { return host_f != nullptr; }
+        bool was_cached() const { return info.was_cached; }
+        CompilationInfo get_compilation_info() const { return info; }
+
+        void set_n_threads(int n_threads);
+        int get_n_threads() const { return n_threads; }
+
+        void set(const Scalar& scalar, const double& v) { set(scalar, v, 0); }
+        void set(const Scalar& scalar, const double& v, int thread_id);
+        template <typename IndexableVector>
+        void set(const Vector& vector, const IndexableVector& v)
+        {
+            set(vector, v, 0);
+        }
+        template <typename IndexableVector>
+        void set(const Vector& vector, const IndexableVector& v, int thread_id)
+        {
+            for (int32_t i = 0; i < vector.size(); i++) set(vector[i], (double)v[i], thread_id);
+        }
+        void set(const Matrix& matrix, const double* v) { set(matrix, v, 0); }
+        void set(const Matrix& matrix, const double* v, int thread_id);
+        template <typename RowMajorParenthesisIndexableMatrix>
+        typename std::enable_if<!std::is_pointer<RowMajorParenthesisIndexableMatrix>::value, void>::type set(const Matrix& matrix,
+                                                                                                             const RowMajorParenthesisIndexableMatrix& m)
+        {
+            set(matrix, m, 0);
+        }
+        template <typename RowMajorParenthesisIndexableMatrix>
+        typename std::enable_if<!std::is_pointer<RowMajorParenthesisIndexableMatrix>::value, void>::type
+        set(const Matrix& matrix, const RowMajorParenthesisIndexableMatrix& m, int thread_id)
+        {
+            for (int32_t i = 0; i < matrix.rows(); i++)
+                for (int32_t j = 0; j < matrix.cols(); j++) set(matrix(i, j), (double)m(i, j), thread_id);
+        }
+
+        View<double> run() { return run(0); }
+        View<double> run(int thread_id);
+        // Evaluate n input vectors on the GPU ([n][n_inputs] -> [n][n_outputs]).
         void run_batch(const double* inputs, int64_t n, double* outputs);
-        int32_t get_n_inputs() const { return n_in; }
-        int32_t get_n_outputs() const { return n_out; }
-        bool was_cached() const { return cached; }
+
+        int get_n_inputs() const { return n_in; }
+        int get_n_outputs() const { return n_out; }
+        std::string get_name() const { return name; }
+        double* get_input_buffer(int thread_id = 0) { return thread_input.at(thread_id).data(); }
+        double* get_output_buffer(int thread_id = 0) { return thread_output.at(thread_id).data(); }
+        const double* get_input_buffer(int thread_id = 0) const { return thread_input.at(thread_id).data(); }
+        const double* get_output_buffer(int thread_id = 0) const { return thread_output.at(thread_id).data(); }
+
+        std::vector<std::vector<double>> thread_input;
+        std::vector<std::vector<double>> thread_output;
 
     private:
-        std::shared_ptr<Graph> g_keep;
+        void release();
+        void resize_buffers();
+        bool load_files(const std::string& name, const std::string& folder, const std::string& sha);
         std::string name;
+        std::string base;        // folder + "/" + name + "_" + first 16 digits of the SHA-256
+        int n_threads = 1;
         int32_t n_in = 0, n_out = 0;
-        std::vector<double> in, out;
+        CompilationInfo info;
+        void* lib = nullptr;
+        void (*host_f)(const double*, double*) = nullptr;
+        std::string device_source;  // HIP source of the batch kernel (module built on the first batch)
         symx_hip_ctx ctx = nullptr;
         symx_hip_module mod = nullptr;
-        bool cached = false;
-        double* d_in = nullptr;
-        double* d_out = nullptr;
-        int64_t cap = 0;
     };
 
     // ---- FEM helpers (fem_integrators.h) and meshes (mesh_generation.h) ----------------

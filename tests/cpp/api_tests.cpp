@@ -15,10 +15,13 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <functional>
 #include <random>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "solver.h"
 
@@ -215,25 +218,69 @@ namespace
         Scalar b = ws.make_scalar();
         std::vector<Scalar> exprs = { a + b, a - b, a * b, a / b, sqrt(a), exp(b), log(a), sin(a), cos(b), tan(a), asin(a / 4.0), acos(b / 4.0),
                                       atan(a), a.powN(3), a.powF(2.5), branch(a - b, a, b), branch(b - a, a, b), log(a - 2.0) };
-        Compiled<double> c(exprs, "api_ops");
-        const double av = 1.3, bv = 0.7;
-        c.set(a, av);
-        c.set(b, bv);
-        const std::vector<double> expect = { av + bv, av - bv, av * bv, av / bv, std::sqrt(av), std::exp(bv), std::log(av), std::sin(av),
-                                             std::cos(bv), std::tan(av), std::asin(av / 4.0), std::acos(bv / 4.0), std::atan(av),
-                                             av * av * av, std::pow(av, 2.5), av, bv, -INFINITY };
-        std::vector<double> got;
+        Compiled<double> c(exprs, "api_ops", get_codegen_dir());
+        require(c.is_valid() && c.get_n_inputs() == 2 && c.get_n_outputs() == (int)exprs.size(), "compiled function shape");
+        auto expect_at = [](double av, double bv) {
+            return std::vector<double>{ av + bv, av - bv, av * bv, av / bv, std::sqrt(av), std::exp(bv), std::log(av), std::sin(av),
+                                        std::cos(bv), std::tan(av), std::asin(av / 4.0), std::acos(bv / 4.0), std::atan(av),
+                                        av * av * av, std::pow(av, 2.5), av - bv > 0.0 ? av : bv, bv - av > 0.0 ? av : bv, std::log(av - 2.0) };
+        };
+        auto check_out = [&](const double* got, double av, double bv, double tol, const std::string& what) {
+            const std::vector<double> e = expect_at(av, bv);
+            for (size_t i = 0; i < exprs.size(); i++) {
+                const bool ok = std::isinf(e[i]) || std::isnan(e[i]) ? (std::isinf(got[i]) && got[i] < 0) : approx(got[i], e[i], tol);
+                require(ok, what + " output " + std::to_string(i) + ": " + std::to_string(got[i]) + " vs " + std::to_string(e[i]));
+            }
+        };
+        // single calls on the host, per thread buffers (Compiled.h:44-91)
+        c.set_n_threads(3);
+        const double in[3][2] = { { 1.3, 0.7 }, { 1.9, 1.1 }, { 0.6, 1.7 } };
+        for (int t = 0; t < 3; t++) {
+            c.set(a, in[t][0], t);
+            c.set(b, in[t][1], t);
+        }
+        for (int t = 0; t < 3; t++) {
+            View<double> got = c.run(t);
+            require(got.size() == (int)exprs.size(), "output view size");
+            check_out(got.data(), in[t][0], in[t][1], 1e-14, "host thread " + std::to_string(t));
+        }
+        // the same expressions as a GPU batch: equal to the host values to rounding
+        std::vector<double> bin = { 1.3, 0.7, 1.9, 1.1, 0.6, 1.7 }, bout(3 * exprs.size());
+        c.run_batch(bin.data(), 3, bout.data());
+        for (int t = 0; t < 3; t++) check_out(bout.data() + t * exprs.size(), in[t][0], in[t][1], 1e-13, "batch item " + std::to_string(t));
+        // NaN checks only in check mode (static_methods.h:15-16)
+        Scalar s = ws.make_scalar();
+        Compiled<double> cn({ sqrt(s) }, "api_nan", get_codegen_dir());
+        cn.set(s, -1.0);
+        require(std::isnan(cn.run()[0]), "sqrt(-1) without check mode");
+        check_mode_ON(true);
+        bool threw = false;
         try {
-            got = c.run();
+            cn.run();
         }
-        catch (const std::exception&) {
-            // run() rejects NaN; -inf is allowed
-            throw;
+        catch (const std::runtime_error&) {
+            threw = true;
         }
-        for (size_t i = 0; i < exprs.size(); i++) {
-            const bool ok = std::isinf(expect[i]) ? (std::isinf(got[i]) && got[i] < 0) : approx(got[i], expect[i], 1e-12);
-            require(ok, "output " + std::to_string(i) + ": " + std::to_string(got[i]) + " vs " + std::to_string(expect[i]));
-        }
+        check_mode_ON(false);
+        require(threw, "check mode rejects a NaN output");
+        // matrix inputs, row major (Compiled.cpp:92-108), and the cache: a second object with
+        // the same cache_id loads the compiled code, another cache_id does not
+        Matrix M = ws.make_matrix({ 2, 2 });
+        const std::vector<Scalar> mexpr = { M.det(), M.trace() };
+        const std::string cid = "api_matrix_v1_" + std::to_string((long long)getpid());
+        Compiled<double> cm(mexpr, "api_matrix", get_codegen_dir(), cid);
+        require(!cm.was_cached(), "first compilation is not cached");
+        const double mv[4] = { 2.0, 3.0, -1.0, 4.0 };
+        cm.set(M, mv);
+        View<double> r = cm.run();
+        require(approx(r[0], 11.0, 1e-14) && approx(r[1], 6.0, 1e-14), "matrix det / trace");
+        Compiled<double> cm2(mexpr, "api_matrix", get_codegen_dir(), cid);
+        require(cm2.was_cached() && cm2.get_compilation_info().was_cached, "same cache_id loads the cached code");
+        Compiled<double> cm3;
+        require(!cm3.load_if_cached("api_matrix", get_codegen_dir(), cid + "_other"), "another cache_id does not load");
+        require(cm3.load_if_cached("api_matrix", get_codegen_dir(), cid), "load_if_cached by name and cache_id");
+        cm3.set(M, mv);
+        require(approx(cm3.run()[0], 11.0, 1e-14), "loaded code evaluates");
     }
 
     // A 5 x 5 quad grid over [-0.5, 0.5]^2 (z = 0), two triangles per quad (as
@@ -311,10 +358,62 @@ namespace
             for (int ns : { 1, 3 })
                 for (int num : { -1, 1, 4, 8, 9 }) loop_triangle_area_case(cond, ns, num);
     }
+    Scalar nh_density(const Matrix& F, const Scalar& l, const Scalar& m)
+    {
+        Matrix C = F.transpose() * F;
+        Scalar Ic = C.trace();
+        Scalar logdetF = log(F.det());
+        return 0.5 * m * (Ic - 3) - m * logdetF + 0.5 * l * logdetF.powN(2);
+    }
+
+    // The element body of oracle/ref_driver.cpp (run_element, examples/NeoHookean.cpp:12-117),
+    // compiled against this front end instead of the reference: prints E, gradient, Hessian.
+    int run_element(int argc, char** argv)
+    {
+        if (argc < 2 + 2 + 24) return 2;
+        const double lambda = std::atof(argv[2]), mu = std::atof(argv[3]);
+        std::vector<double> xin(12), Xin(12);
+        for (int i = 0; i < 12; i++) xin[i] = std::atof(argv[4 + i]);
+        for (int i = 0; i < 12; i++) Xin[i] = std::atof(argv[16 + i]);
+        auto jac_tet4 = [](Workspace& ws, const std::vector<Vector>& xh) {
+            Matrix J = ws.get_identity_matrix(3);
+            J.set_col(0, xh[1] - xh[0]);
+            J.set_col(1, xh[2] - xh[0]);
+            J.set_col(2, xh[3] - xh[0]);
+            return J;
+        };
+        Workspace ws;
+        std::vector<Vector> x = ws.make_vectors(3, 4);
+        std::vector<Vector> X = ws.make_vectors(3, 4);
+        Scalar l = ws.make_scalar();
+        Scalar m = ws.make_scalar();
+        Matrix Jx = jac_tet4(ws, x);
+        Matrix JX = jac_tet4(ws, X);
+        Matrix F = Jx * JX.inv();
+        Scalar E = JX.det() / 6.0 * nh_density(F, l, m);
+        DiffCache dc;
+        std::vector<Scalar> dofs = collect_scalars(x);
+        Vector g = gradient(E, dofs, dc);
+        Matrix h = gradient(g, dofs, true, dc);
+        std::vector<Scalar> outs = collect_scalars({ { E }, g.values(), h.values() });
+        Compiled<double> c(outs, "ref_nh_element", symx::get_codegen_dir());
+        for (int v = 0; v < 4; v++) {
+            for (int d = 0; d < 3; d++) {
+                c.set(x[v][d], xin[3 * v + d]);
+                c.set(X[v][d], Xin[3 * v + d]);
+            }
+        }
+        c.set(l, lambda);
+        c.set(m, mu);
+        View<double> r = c.run();
+        for (int i = 0; i < 157; i++) std::printf("%.17g\n", r[i]);
+        return 0;
+    }
 }
 
-int main()
+int main(int argc, char** argv)
 {
+    if (argc > 1 && std::string(argv[1]) == "element") return run_element(argc, argv);
     const std::vector<std::pair<std::string, std::function<void()>>> cases = {
         { "stable_neo_hookean", stable_neo_hookean },
         { "conditional", conditional },

@@ -123,3 +123,26 @@ def test_jit_cache_validates_cached_objects():
         open(path, "wb").write(bytes(raw))
         assert jit() == (0, n0)
         assert jit() == (1, n0)
+
+
+def test_reference_element_code_compiles_and_matches_reference_outputs():
+    """The Compiled<double> element code of the reference's NeoHookean example (the same body
+    as oracle/ref_driver.cpp run_element: Workspace, fem Jacobians, DiffCache gradients,
+    collect_scalars, Compiled<double>(outs, name, folder), set, View<double> r = c.run())
+    compiled against this front end (tests/cpp/api_tests.cpp, mode `element`) and run on
+    the host: outputs equal the reference's own (nh_tet4_reference.json) to round-off."""
+    import json
+    import subprocess
+
+    exe = os.path.join(ROOT, "symx_amd", "lib", "api_tests")
+    cases = json.load(open(os.path.join(ROOT, "tests", "golden", "nh_tet4_reference.json")))["cases"]
+    with tempfile.TemporaryDirectory() as d:
+        env = dict(os.environ, SYMX_AMD_CODEGEN_DIR=d)
+        for c in cases:
+            args = ["%.17g" % v for v in [c["lambda"], c["mu"], *c["x"], *c["X"]]]
+            out = subprocess.run([exe, "element", *args], capture_output=True, text=True, check=True, env=env).stdout.split()
+            got, ref = np.array([float(v) for v in out]), np.array(c["out"])
+            assert got.shape == (157,)
+            assert abs(got[0] - ref[0]) <= 1e-12 * abs(ref[0])
+            assert np.abs(got[1:13] - ref[1:13]).max() <= 1e-11 * np.abs(ref[1:13]).max()
+            assert np.abs(got[13:] - ref[13:]).max() <= 1e-11 * np.abs(ref[13:]).max()
