@@ -92,6 +92,7 @@ typedef struct symx_newton_params {
     double projection_eps;
     double step_tolerance;
     int32_t verbosity;               /* console output of the solve: 0 none, 1 summary, 2 one line per iteration, 3 full */
+    int32_t linear_solver;           /* 0 BDPCG, 1 DirectLLT (solved to round-off on the device) */
 } symx_newton_params;
 
 typedef struct symx_solve_stats {

@@ -5,7 +5,7 @@
 //   ref_driver element <lambda> <mu> <x0..x11> <X0..X11>
 //       Prints E, gradient[12], Hessian[144] of the Neo-Hookean Tet4 element built
 //       like examples/NeoHookean.cpp:12-117 (Compiled<double>).
-//   ref_driver solve <in.bin> <out.bin>
+//   ref_driver solve <in.bin> <out.bin> [llt]
 //       Builds the Tet4 block problem (Neo-Hookean via fem_integrator, backward
 //       Euler inertia, penalty boundary) with GlobalPotential and solves one step
 //       with NewtonsMethod (examples/dynamic_elasticity_with_contact.cpp:85-125,
@@ -186,6 +186,7 @@ static int run_solve(int argc, char** argv)
     newton.settings.cg_rel_tolerance = prm[9];
     newton.settings.cg_abs_tolerance = prm[10];
     newton.settings.projection_eps = prm[11];
+    if (argc > 4 && std::string(argv[4]) == "llt") newton.settings.linear_solver = LinearSolver::DirectLLT;
     const auto t0 = std::chrono::steady_clock::now();
     SolverReturn ret = newton.solve();
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

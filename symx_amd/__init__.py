@@ -58,7 +58,7 @@ class NewtonParams(Structure):
     _fields_ = [("precision", c_int32), ("projection_mode", c_int32), ("use_mirroring", c_int32), ("max_iterations", c_int32),
                 ("residual_tolerance_abs", c_double), ("residual_tolerance_rel", c_double), ("cg_rel_tolerance", c_double),
                 ("cg_abs_tolerance", c_double), ("cg_max_iterations", c_int32), ("projection_eps", c_double),
-                ("step_tolerance", c_double), ("verbosity", c_int32)]
+                ("step_tolerance", c_double), ("verbosity", c_int32), ("linear_solver", c_int32)]
 
 
 class SolveStats(Structure):

@@ -37,6 +37,9 @@ namespace symx
         // Line-search steps below this fraction of max|x| are judged by the trapezoid estimate
         // of the energy change when the energy difference does not show a decrease.
         constexpr double kTrapezoidStep = 1e-7;
+        // Relative residual to which LinearSolver::DirectLLT solves (FP64 round-off level for the
+        // penalty-stiff systems of the examples, condition numbers up to about 1e7).
+        constexpr double kDirectSolveTol = 1e-12;
         std::string identifier(const std::string& s)
         {
             std::string r;
@@ -661,8 +664,6 @@ namespace symx
         const int ndofs = G->get_total_n_dofs();
         if (ndofs <= 0) throw std::runtime_error("symx error NewtonsMethod::solve(): No degrees of freedom.");
         if (ndofs % 3 != 0) throw std::runtime_error("symx error NewtonsMethod::solve(): ndofs must be divisible by 3.");
-        if (settings.linear_solver != LinearSolver::BDPCG)
-            throw std::runtime_error("symx: the HIP backend solves with BDPCG only (DirectLLT is not available on the device)");
         if (!dev->matches_topology()) dev = std::make_unique<DeviceGlobal>(G, context);
         symx_hip_ctx hip = context->hip();
         symx_hip_problem prob = dev->problem();
@@ -773,7 +774,9 @@ namespace symx
                 }
                 {
                     auto t = logger->time("assembly");
-                    check(symx_hip_problem_assemble(prob, precision_mixed ? SYMX_HIP_MIXED : SYMX_HIP_FP64), hip, "assembly");
+                    // DirectLLT factors the FP64 matrix in the reference (NewtonsMethod.cpp:395-418)
+                    const bool fp32_matrix = precision_mixed && settings.linear_solver == LinearSolver::BDPCG;
+                    check(symx_hip_problem_assemble(prob, fp32_matrix ? SYMX_HIP_MIXED : SYMX_HIP_FP64), hip, "assembly");
                 }
                 ok = solve_linear_system(residual_norm);
                 const bool can_project_more = settings.projection_mode != ProjectionToPD::Newton && !all_projected;
@@ -856,6 +859,18 @@ namespace symx
         symx_hip_ctx hip = context->hip();
         symx_hip_problem prob = dev->problem();
         check(symx_hip_problem_scale(prob, SYMX_HIP_VEC_RHS, SYMX_HIP_VEC_GRAD, -1.0), hip, "rhs");
+        if (settings.linear_solver == LinearSolver::DirectLLT) {
+            // DirectLLT (NewtonsMethod.cpp:395-418: Eigen SimplicialLLT of the projected FP64
+            // matrix, failing when the matrix is not positive definite): the same matrix solved
+            // to round-off on the device by FP64 block-Jacobi PCG, failing on p.Ap <= 0.
+            symx_hip_pcg_info info{};
+            const int32_t max_it = std::max<int32_t>(20 * dev->n_dofs(), 10000);
+            check(symx_hip_problem_solve(prob, SYMX_HIP_FP64, 0.0, kDirectSolveTol, max_it, 1, &info), hip, "direct solve");
+            stats.cg_iterations += info.n_iterations;
+            last_cg_iterations = info.n_iterations;
+            context->output->print(sfmt("#CG %5d | ", info.n_iterations), Verbosity::Full);
+            return info.converged != 0 && info.found_indefiniteness == 0;
+        }
         // Forcing sequence (NewtonsMethod.cpp:423-424).
         const double forcing = std::min(1e-2, residual_norm * std::min(0.5, std::sqrt(residual_norm)));
         const double abs_tol = std::max(forcing, settings.cg_abs_tolerance);

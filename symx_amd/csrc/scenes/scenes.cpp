@@ -129,6 +129,7 @@ static void apply_newton_params(NewtonSettings& s, const symx_newton_params& p, 
     s.cg_max_iterations = p.cg_max_iterations;
     s.projection_eps = p.projection_eps;
     s.step_tolerance = p.step_tolerance;
+    s.linear_solver = p.linear_solver == 1 ? LinearSolver::DirectLLT : LinearSolver::BDPCG;
 }
 
 // Material, loads and contact of a Tet4 scene (shared by the block and sphere scenes).
@@ -549,6 +550,7 @@ void symx_newton_params_default(symx_newton_params* p)
     p->projection_eps = 1e-10;
     p->step_tolerance = 0.0;
     p->verbosity = 0;
+    p->linear_solver = 0;
 }
 
 void symx_tet4_block_params_default(symx_tet4_block_params* p)

@@ -19,6 +19,8 @@ build container:  python tests/golden/make_golden.py
   hex27_ref_block2.npz       the reference NewtonsMethod (ProjectOnDemand) on a 2^3-cell Hex27
                              block with per-element Young's moduli (ref_driver solve_hex)
                              (`python tests/golden/make_golden.py hex` writes these two)
+  newton_ref_block6_llt.npz  the block6 case with LinearSolver::DirectLLT to 1e-8 relative
+                             (`python tests/golden/make_golden.py llt`)
 """
 import json
 import os
@@ -68,7 +70,7 @@ def point_triangle_cases():
     json.dump({"source": "reference tests/sympy_derivatives.cpp", "cases": cases}, open(os.path.join(HERE, "point_triangle_sympy.json"), "w"))
 
 
-def newton_case(name, cells, noise, seed, rel_tol, cg_rel, threads=1, mode=1):
+def newton_case(name, cells, noise, seed, rel_tol, cg_rel, threads=1, mode=1, llt=False):
     from ref_io import read_solve_output, write_solve_input
     from scene_np import tet4_block
     from symx_amd import NewtonParams
@@ -78,9 +80,9 @@ def newton_case(name, cells, noise, seed, rel_tol, cg_rel, threads=1, mode=1):
     with tempfile.TemporaryDirectory() as d:
         inp, outp = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         write_solve_input(inp, S, newton, threads)
-        subprocess.run([os.path.join(REF, "ref_driver"), "solve", inp, outp], check=True, env=ENV)
+        subprocess.run([os.path.join(REF, "ref_driver"), "solve", inp, outp] + (["llt"] if llt else []), check=True, env=ENV)
         res = read_solve_output(outp, S.X.shape[0])
-    np.savez_compressed(os.path.join(HERE, f"newton_ref_{name}.npz"), x=res["x"], cells=np.array(cells), noise=noise, seed=seed,
+    np.savez_compressed(os.path.join(HERE, f"newton_ref_{name}.npz"), x=res["x"], cells=np.array(cells), noise=noise, seed=seed, llt=llt,
                         rel_tol=rel_tol, cg_rel=cg_rel, result=res["result"], newton_iterations=res["newton_iterations"],
                         cg_iterations=res["cg_iterations"], ls_bt_iterations=res["ls_bt_iterations"], energy=res["energy"],
                         seconds=res["seconds"], threads=threads, projection_mode=mode)
@@ -167,6 +169,9 @@ def hex_newton_case(name, cells, mode=2, rel_tol=1e-6, threads=1):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "llt":
+        newton_case("block6_llt", (6, 6, 6), 0.2 / 6, 11, 1e-8, 1e-3, llt=True)
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "hex":
         hex_cases()
         hex_newton_case("block2", (2, 2, 2))

@@ -343,3 +343,26 @@ def test_unstructured_numbering_gives_the_same_step(gpu, precision, cells):
     moved = np.abs(x0 - xs0).max()
     assert moved > 0
     assert np.abs(x1 - x0).max() <= 1e-5 * moved, (np.abs(x1 - x0).max(), moved)
+
+
+@pytest.mark.parametrize("precision", [symx_amd.FP64, symx_amd.MIXED])
+def test_direct_llt_matches_reference_fixture(gpu, oracle, precision):
+    """LinearSolver::DirectLLT (NewtonsMethod.cpp:395-418; the reference factors the
+    projected FP64 matrix with Eigen's SimplicialLLT) against the reference's own DirectLLT
+    solve of the block6 case to 1e-8 (newton_ref_block6_llt.npz). The device solves the same
+    FP64 matrix to round-off (in both precision settings), so the Newton iterates are the
+    reference's."""
+    f = np.load(os.path.join(GOLDEN, "newton_ref_block6_llt.npz"))
+    assert bool(f["llt"])
+    cells = tuple(int(c) for c in f["cells"])
+    prm = symx_amd.default_block_params(cells=list(cells), noise=float(f["noise"]), seed=int(f["seed"]))
+    newton = symx_amd.default_newton_params(residual_tolerance_rel=float(f["rel_tol"]), cg_rel_tolerance=float(f["cg_rel"]),
+                                            precision=precision, linear_solver=1)
+    s = symx_amd.Tet4Block(prm, newton)
+    S = scene_arrays(s)
+    st = s.solve()
+    assert st["result_name"] == "Successful" and int(f["result"]) == 0
+    assert st["newton_iterations"] == int(f["newton_iterations"])
+    disp = np.abs(f["x"] - S.X).max()
+    assert np.abs(s.get("x") - f["x"]).max() <= 1e-9 * disp
+    s.close()
