@@ -230,6 +230,19 @@ namespace symx
             add_dof([&arr]() { return reinterpret_cast<std::uintptr_t>(&arr); }, [&arr]() { return reinterpret_cast<double*>(arr.data()); },
                     [&arr]() { return (int32_t)(arr.size() * stride); }, name);
         }
+        // Any contiguous container of doubles with data() and size(): a dynamic vector, or a
+        // row-major matrix (GlobalPotential.h:55-61 takes EigenMatrixRM<double> this way).
+        template <typename DYNAMIC_VECTOR>
+        void add_dof(DYNAMIC_VECTOR& arr, const std::string& name = "")
+        {
+            add_dof([&arr]() { return reinterpret_cast<std::uintptr_t>(&arr); }, [&arr]() { return reinterpret_cast<double*>(arr.data()); },
+                    [&arr]() { return (int32_t)arr.size(); }, name);
+        }
+        void add_dof(double& scalar, const std::string& name = "")
+        {
+            add_dof([&scalar]() { return reinterpret_cast<std::uintptr_t>(&scalar); }, [&scalar]() { return &scalar; }, []() { return (int32_t)1; },
+                    name);
+        }
 
         // Parameters of an adjoint sensitivity analysis (no counterpart in SymX): data
         // containers, identified like DoF sets by their address, whose symbols become the

@@ -8,6 +8,8 @@
 //                        potentials, one conditional, against the closed form)
 //   polynomial_newton    tests/test_newton_method.cpp:10-49
 //   compiled_ops         tests/test_compilation.cpp:16-94 (compiled vs host evaluation)
+//   add_dof_overloads    GlobalPotential.h:55-61 (array, dynamic vector, row-major matrix
+//                        and scalar DoF containers)
 //   loop_triangle_area   tests/test_loops.cpp:58-233 (CompiledInLoop against closed-form
 //                        triangle areas: summation, activity mask, element counts)
 // Each case prints "PASS <name>" or "FAIL <name>: <reason>"; the exit status is the
@@ -209,6 +211,59 @@ namespace
         const SolverReturn ret = newton.solve();
         require(ret == SolverReturn::Successful, std::string("solver returned ") + to_string(ret));
         for (int d = 0; d < 3; d++) require(std::abs(x[0][d] - target[0][d]) <= 1e-1, "x far from the target");
+    }
+
+    // A row-major matrix of doubles with data() and size(), as EigenMatrixRM<double> offers.
+    struct RowMajorMatrix
+    {
+        std::vector<double> v;
+        double* data() { return v.data(); }
+        const double* data() const { return v.data(); }
+        int32_t size() const { return (int32_t)v.size(); }
+    };
+
+    // The DoF containers of GlobalPotential::add_dof (GlobalPotential.h:55-61): an array of
+    // 3-vectors, a dynamic vector and a row-major matrix, in one problem; Newton on
+    // 0.5 sum |x_i - t_i|^2 (+ a quartic term) reaches the targets in every container, and the
+    // flattened DoF vector concatenates the sets in the order they were added.
+    void add_dof_overloads()
+    {
+        std::vector<std::array<double, 3>> a = { { 0.0, 0.0, 0.0 }, { 1.0, 1.0, 1.0 } };
+        std::vector<double> b = { 2.0, 2.0, 2.0 };
+        RowMajorMatrix c{ { 3.0, 3.0, 3.0, 4.0, 4.0, 4.0 } };
+        const std::vector<std::array<double, 3>> ta = { { 0.5, -0.25, 1.0 }, { 2.0, 0.0, -1.0 } }, tb = { { -3.0, 0.125, 7.0 } },
+                                                 tc = { { 1.0, 2.0, 3.0 }, { -1.0, -2.0, -3.0 } };
+        spGlobalPotential G = GlobalPotential::create();
+        std::vector<std::array<int32_t, 1>> ca = { { 0 }, { 1 } }, cb = { { 0 } }, cc = { { 0 }, { 1 } };
+        auto pot = [](const Vector& x, const Vector& t) { return 0.5 * (x - t).squared_norm() + (x - t).squared_norm().powN(2); };
+        G->add_potential("a", ca, [&](MappedWorkspace<double>& mws, Element& el) { return pot(mws.make_vector(a, el[0]), mws.make_vector(ta, el[0])); });
+        G->add_potential("b", cb, [&](MappedWorkspace<double>& mws, Element& el) { return pot(mws.make_vector(b, 3, el[0]), mws.make_vector(tb, el[0])); });
+        G->add_potential("c", cc, [&](MappedWorkspace<double>& mws, Element& el) { return pot(mws.make_vector(c, 3, el[0]), mws.make_vector(tc, el[0])); });
+        G->add_dof(a);
+        G->add_dof(b);
+        G->add_dof(c);
+        require(G->get_total_n_dofs() == 15, "15 DoFs in three sets");
+        const std::vector<int32_t> off = G->get_dofs_offsets();
+        require(off.size() >= 3 && off[0] == 0 && off[1] == 6 && off[2] == 9, "DoF set offsets");
+        std::vector<double> u(15);
+        G->get_dofs(u.data());
+        require(u[6] == 2.0 && u[9] == 3.0 && u[14] == 4.0, "flattened DoFs in set order");
+        spContext context = quiet_context();
+        NewtonsMethod newton(G, context);
+        newton.settings.residual_tolerance_abs = 1e-10;
+        const SolverReturn ret = newton.solve();
+        require(ret == SolverReturn::Successful, std::string("solver returned ") + to_string(ret));
+        for (int i = 0; i < 2; i++)
+            for (int d = 0; d < 3; d++) {
+                require(std::abs(a[i][d] - ta[i][d]) <= 1e-6, "array DoF set");
+                require(std::abs(c.v[3 * i + d] - tc[i][d]) <= 1e-6, "row-major matrix DoF set");
+            }
+        for (int d = 0; d < 3; d++) require(std::abs(b[d] - tb[0][d]) <= 1e-6, "dynamic vector DoF set");
+        // a scalar DoF registers one DoF
+        double s = 1.0;
+        spGlobalPotential G2 = GlobalPotential::create();
+        G2->add_dof(s);
+        require(G2->get_total_n_dofs() == 1, "scalar DoF set");
     }
 
     void compiled_ops()
@@ -420,6 +475,7 @@ int main(int argc, char** argv)
         { "polynomial_newton", polynomial_newton },
         { "compiled_ops", compiled_ops },
         { "loop_triangle_area", loop_triangle_area },
+        { "add_dof_overloads", add_dof_overloads },
     };
     int failed = 0;
     for (const auto& [name, f] : cases) {

@@ -21,7 +21,7 @@ def test_cpp_api(gpu):
     print(r.stdout)
     print(r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
-    cases = ["stable_neo_hookean", "conditional", "polynomial_newton", "compiled_ops", "loop_triangle_area"]
+    cases = ["stable_neo_hookean", "conditional", "polynomial_newton", "compiled_ops", "loop_triangle_area", "add_dof_overloads"]
     for c in cases:
         assert f"PASS {c}\n" in r.stdout, c
     assert "FAIL" not in r.stdout
