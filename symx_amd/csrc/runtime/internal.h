@@ -35,10 +35,26 @@ namespace symx_hip
     // Sliced block-CSR (SELL-64 of 3x3 blocks): block rows are placed at positions
     // grouped in slices of 64; slice s stores width_s block columns for each of its
     // positions, the diagonal block first. Slot (s, j, lane) = slice_ptr[s] + 64 j +
-    // lane; value k of its 3x3 block lives at ((slice_ptr[s] / 64 + j) * 9 + k) * 64 +
-    // lane, so the 64 lanes of a wave read 64 consecutive values per load. (A layout with
-    // each block row as one 3-value load per lane made the FP64 SpMV 25% slower and the
-    // FP32 one no faster.)
+    // lane; the 64 blocks of one (s, j) form a group of 576 values starting at
+    // (slice_ptr[s] / 64 + j) * 576 (sell_val_index): values 0..7 of a block (row-major)
+    // in runs of W = SellRun<T>::width consecutive values per lane, the runs of the 64
+    // lanes side by side, then value 8 of every lane. A load of one run is then W values
+    // per lane and 64 W consecutive values per wave (16-byte loads for FP32). (A layout
+    // with each block row as one 3-value load per lane made the FP64 SpMV 25% slower and
+    // the FP32 one no faster.)
+    template <typename T>
+    struct SellRun
+    {
+        static constexpr int width = sizeof(T) == 4 ? 4 : 1;
+    };
+    template <typename T>
+    __host__ __device__ constexpr int64_t sell_val_index(int64_t slot, int q)
+    {
+        constexpr int W = SellRun<T>::width;
+        const int64_t g = (slot >> 6) * 576;
+        const int lane = (int)(slot & 63);
+        return q < 8 ? g + (q / W) * 64 * W + lane * W + q % W : g + 512 + lane;
+    }
     // Position = row unless sorting the rows by length inside windows of kSellSigma
     // rows removes enough padding (SELL-C-sigma); then row_at[pos] names the row.
     constexpr int32_t kSellSigma = 1024;
@@ -75,7 +91,8 @@ struct symx_hip_ctx_s
     double* d_scalars = nullptr;   // small device scalar area
     double* h_pinned = nullptr;    // pinned host scalars (64 doubles)
     symx_hip::PcgState* d_pcg = nullptr;
-    symx_hip::PcgState* h_pcg = nullptr;
+    symx_hip::PcgState* h_pcg = nullptr;  // two pinned copies: the PCG batch pipeline reads one while the other is in flight
+    hipEvent_t pcg_ev[2] = { nullptr, nullptr };
     int32_t* d_loop_scratch = nullptr;  // device address of this device's loop scratch array
 };
 

@@ -67,6 +67,27 @@ namespace symx_hip
         if constexpr (STREAM) return __builtin_nontemporal_load(p);
         else return *p;
     }
+    // The nine values of the 3x3 block in `lane` of a 64-slot value group g (layout of
+    // sell_val_index): runs of SellRun<TS>::width values as one load each, then value 8.
+    template <typename TS, bool STREAM>
+    __device__ __forceinline__ void load_block(const TS* __restrict__ g, int lane, TS (&v)[9])
+    {
+        constexpr int W = SellRun<TS>::width;
+        if constexpr (W == 1) {
+#pragma unroll
+            for (int q = 0; q < 9; q++) v[q] = mload<STREAM>(g + 64 * q + lane);
+        }
+        else {
+            typedef TS run_t __attribute__((ext_vector_type(W)));
+#pragma unroll
+            for (int r = 0; r < 8 / W; r++) {
+                const run_t t = mload<STREAM>(reinterpret_cast<const run_t*>(g + r * 64 * W) + lane);
+#pragma unroll
+                for (int u = 0; u < W; u++) v[r * W + u] = t[u];
+            }
+            v[8] = mload<STREAM>(g + 512 + lane);
+        }
+    }
     // Block column of a SELL slot: int32 ids, or int16 offsets from the slot's row (half
     // the index bytes; used when every offset fits).
     template <typename CT>
@@ -363,11 +384,23 @@ namespace symx_hip
         const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
         const double tol = 4.0 * ulp * tn;
         glo -= 2.0 * ulp * tn + pivmin;
+        // T scaled by a power of two (exactly) to ||T|| < 1 for the minor recurrence of the
+        // Sturm counts (sturm3_det), whose steps are one FMA deep
+        double ds[9], e2s[9];
+        int tex = 0;
+        frexp(tn, &tex);
+        const double sc = tn > 0.0 ? ldexp(1.0, -tex) : 1.0;
+        static_for<0, 9>([&](auto Ic) {
+            constexpr int I = decltype(Ic)::value;
+            ds[I] = d[I] * sc;
+            const double es = e[I] * sc;
+            e2s[I] = es * es;
+        });
         int cnt;
         {
-            const double xe[3] = { cut, cut, cut };
+            const double xe[3] = { cut * sc, cut * sc, cut * sc };
             int c3[3];
-            sturm3_reg<9>(d, e, xe, pivmin, c3);
+            sturm3_det<9>(ds, e2s, xe, c3);
             cnt = c3[0];
         }
         if (cnt > 3) return false;
@@ -378,10 +411,13 @@ namespace symx_hip
             bool busy = false;
             static_for<0, 3>([&](auto Jc) { busy |= Jc() < cnt && hi[Jc()] - lo[Jc()] > tol; });
             if (!busy) break;
-            double mid[3];
+            double mid[3], mids[3];
             int c3[3];
-            static_for<0, 3>([&](auto Jc) { mid[Jc()] = 0.5 * (lo[Jc()] + hi[Jc()]); });
-            sturm3_reg<9>(d, e, mid, pivmin, c3);
+            static_for<0, 3>([&](auto Jc) {
+                mid[Jc()] = 0.5 * (lo[Jc()] + hi[Jc()]);
+                mids[Jc()] = mid[Jc()] * sc;
+            });
+            sturm3_det<9>(ds, e2s, mids, c3);
             static_for<0, 3>([&](auto Jc) {
                 constexpr int J = decltype(Jc)::value;
                 if (hi[J] - lo[J] > tol) {
@@ -508,7 +544,7 @@ namespace symx_hip
     // written through LDS 16 values at a time with coalesced transfers: M, the block-row
     // sums and the scale accumulate chunk by chunk, and the update is added chunk by chunk.
     template <typename TH>
-    __global__ __launch_bounds__(64) void k_project_ub4_reduced(TH* __restrict__ H, int64_t n_mats, double eps, int mirror,
+    __global__ __launch_bounds__(64, 2) void k_project_ub4_reduced(TH* __restrict__ H, int64_t n_mats, double eps, int mirror,
                                                                unsigned long long* count, uint8_t* __restrict__ needs_full,
                                                                const uint8_t* __restrict__ select)
     {
@@ -1292,19 +1328,17 @@ namespace symx_hip
                 for (int k2 = 0; k2 < 9; k2++) acc[k2] += (double)h[k2];
             }
         }
-        TS* dst = val + (int64_t)(slot >> 6) * 576 + (slot & 63);
 #pragma unroll
-        for (int k = 0; k < 9; k++) dst[64 * k] = (TS)acc[k];
+        for (int k = 0; k < 9; k++) val[sell_val_index<TS>(slot, k)] = (TS)acc[k];
         // the transposed slot sums the same element blocks transposed, in the same order
         // (checked when the pattern is built), so its sum is this one transposed bit for bit
         if (mirror) {
             const uint32_t m = mirror[t];
             if (m != ~0u) {
-                TS* md = val + (int64_t)(m >> 6) * 576 + (m & 63);
 #pragma unroll
                 for (int i = 0; i < 3; i++)
 #pragma unroll
-                    for (int j = 0; j < 3; j++) md[64 * (3 * i + j)] = (TS)acc[3 * j + i];
+                    for (int j = 0; j < 3; j++) val[sell_val_index<TS>(m, 3 * i + j)] = (TS)acc[3 * j + i];
             }
         }
     }
@@ -1334,8 +1368,9 @@ namespace symx_hip
         if (pos >= n_rows) return;
         const int64_t row = row_at ? row_at[pos] : pos;
         const int64_t base = slice_ptr[pos >> 6];
-        const TS* m = val + (base >> 6) * 576 + (pos & 63);  // j = 0 is the diagonal block
-        const TS m00 = m[0], m01 = m[64], m02 = m[128], m11 = m[256], m12 = m[320], m22 = m[512];
+        const int64_t t = base + (pos & 63);  // j = 0 is the diagonal block
+        const TS m00 = val[sell_val_index<TS>(t, 0)], m01 = val[sell_val_index<TS>(t, 1)], m02 = val[sell_val_index<TS>(t, 2)];
+        const TS m11 = val[sell_val_index<TS>(t, 4)], m12 = val[sell_val_index<TS>(t, 5)], m22 = val[sell_val_index<TS>(t, 8)];
         const TS c00 = m11 * m22 - m12 * m12;
         const TS c01 = m02 * m12 - m01 * m22;
         const TS c02 = m01 * m12 - m11 * m02;
@@ -1588,16 +1623,18 @@ namespace symx_hip
             const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
             const int lane = (int)(pos & 63);
             const CT* cp = col + b0 + lane;
-            const TS* vp = val + b0 * 9 + lane;
+            const TS* vp = val + b0 * 9;
             TV a0 = (TV)0, a1 = (TV)0, a2 = (TV)0;
             // large matrices stream through once per iteration: non-temporal loads keep them
             // from evicting the gathered vector entries (12% faster per PCG iteration on a
             // 6M-tet block; 40% slower on a cache-resident 50k-DoF system, hence STREAM).
-            // Unroll by 4 only for FP32 blocks with the direction pass (they lose bandwidth
-            // below it); elsewhere 2: FP64 with the pass at 80 instead of 113 VGPRs (6 waves
-            // per SIMD) made the FP64 PCG iteration 3% (6M-tet block) to 6% (50k-DoF
-            // cantilever) faster, FP32 without it 5% faster (200k-DoF ball).
-            constexpr int kUnroll = (sizeof(TS) == 4 && PRE) ? 4 : 2;
+            // Unrolled by 2: FP64 with the pass at 80 instead of 113 VGPRs (6 waves per SIMD)
+            // made the FP64 PCG iteration 3% (6M-tet block) to 6% (50k-DoF cantilever)
+            // faster, FP32 without it 5% faster (200k-DoF ball). FP32 blocks, loaded as two
+            // 16-byte runs and one value, take 0.141 ms per iteration on the 6M-tet block
+            // unrolled by 2 and 0.154 ms unrolled by 4 (85 VGPRs, 5 waves per SIMD); with
+            // nine single-value loads and unrolled by 4 they took 0.143 ms.
+            constexpr int kUnroll = 2;
 #pragma unroll kUnroll
             for (int j = 0; j < wd; j++) {
                 const int64_t c = 3 * block_col(cp[j * 64], row);
@@ -1612,10 +1649,11 @@ namespace symx_hip
                     x1 = z[c + 1] + tb * p_old[c + 1];
                     x2 = z[c + 2] + tb * p_old[c + 2];
                 }
-                const TS* v = vp + (int64_t)j * 576;
-                a0 += (TV)mload<STREAM>(v + 0) * x0 + (TV)mload<STREAM>(v + 64) * x1 + (TV)mload<STREAM>(v + 128) * x2;
-                a1 += (TV)mload<STREAM>(v + 192) * x0 + (TV)mload<STREAM>(v + 256) * x1 + (TV)mload<STREAM>(v + 320) * x2;
-                a2 += (TV)mload<STREAM>(v + 384) * x0 + (TV)mload<STREAM>(v + 448) * x1 + (TV)mload<STREAM>(v + 512) * x2;
+                TS v[9];
+                load_block<TS, STREAM>(vp + (int64_t)j * 576, lane, v);
+                a0 += (TV)v[0] * x0 + (TV)v[1] * x1 + (TV)v[2] * x2;
+                a1 += (TV)v[3] * x0 + (TV)v[4] * x1 + (TV)v[5] * x2;
+                a2 += (TV)v[6] * x0 + (TV)v[7] * x1 + (TV)v[8] * x2;
             }
             const int64_t kk = 3 * row;
             TV p0, p1, p2;
@@ -1840,16 +1878,17 @@ namespace symx_hip
             const int wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
             const int lane = (int)(pos & 63);
             const CT* cp = col + b0 + lane;
-            const float* vp = val + b0 * 9 + lane;
+            const float* vp = val + b0 * 9;
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll 4
             for (int j = 0; j < wd; j++) {
                 const int64_t c = 3 * block_col(cp[j * 64], row);
                 const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
-                const float* v = vp + (int64_t)j * 576;
-                a0 += (double)mload<STREAM>(v + 0) * x0 + (double)mload<STREAM>(v + 64) * x1 + (double)mload<STREAM>(v + 128) * x2;
-                a1 += (double)mload<STREAM>(v + 192) * x0 + (double)mload<STREAM>(v + 256) * x1 + (double)mload<STREAM>(v + 320) * x2;
-                a2 += (double)mload<STREAM>(v + 384) * x0 + (double)mload<STREAM>(v + 448) * x1 + (double)mload<STREAM>(v + 512) * x2;
+                float v[9];
+                load_block<float, STREAM>(vp + (int64_t)j * 576, lane, v);
+                a0 += (double)v[0] * x0 + (double)v[1] * x1 + (double)v[2] * x2;
+                a1 += (double)v[3] * x0 + (double)v[4] * x1 + (double)v[5] * x2;
+                a2 += (double)v[6] * x0 + (double)v[7] * x1 + (double)v[8] * x2;
             }
             const int64_t k = 3 * row;
             const double e0 = b[k] - a0, e1 = b[k + 1] - a1, e2 = b[k + 2] - a2;

@@ -22,7 +22,7 @@
 namespace symx_hip
 {
     template <int I, int E, typename F>
-    __device__ __forceinline__ void static_for(F&& f)
+    __host__ __device__ __forceinline__ void static_for(F&& f)
     {
         if constexpr (I < E) {
             f(std::integral_constant<int, I>());
@@ -225,6 +225,39 @@ namespace symx_hip
         e[N - 1] = 0.0;
         tau[N - 2] = 0.0;
         tau[N - 1] = 0.0;
+    }
+
+    // Number of eigenvalues of T below x0, x1, x2 from the leading principal minors of
+    // T - x I, p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2} with p_{-1} = 1: the count is the
+    // number of sign changes along p_{-1}, p_0, ..., p_{N-1} (Sturm), the same as the number of
+    // negative ratios q_i = p_i / p_{i-1} that sturm3_reg forms with a reciprocal per step;
+    // here a step is one FMA deep. An exact zero p_i is replaced by -2^-100 p_{i-1}, i.e. the
+    // ratio q_i by -2^-100 (sturm3_reg's -pivmin), which also carries the recurrence across an
+    // exact split (e_i = 0). d, e2 (= e^2) and x must be scaled so that ||T|| < 1: |p_i| then
+    // stays below (1 + sqrt 2)^N, and the zero replacements (at most N of them) stay far
+    // above the underflow threshold.
+    template <int N>
+    __host__ __device__ __forceinline__ void sturm3_det(const double (&d)[N], const double (&e2)[N], const double (&x)[3], int (&c)[3])
+    {
+        double p0[3], p1[3];
+        static_for<0, 3>([&](auto Jc) {
+            constexpr int J = decltype(Jc)::value;
+            p0[J] = 1.0;
+            p1[J] = d[0] - x[J];
+            if (p1[J] == 0.0) p1[J] = -0x1p-100;
+            c[J] = p1[J] < 0.0;
+        });
+        static_for<1, N>([&](auto Ic) {
+            constexpr int I = decltype(Ic)::value;
+            static_for<0, 3>([&](auto Jc) {
+                constexpr int J = decltype(Jc)::value;
+                double p2 = fma(d[I] - x[J], p1[J], -e2[I - 1] * p0[J]);
+                if (p2 == 0.0) p2 = -0x1p-100 * p1[J];
+                c[J] += (p2 < 0.0) != (p1[J] < 0.0);
+                p0[J] = p1[J];
+                p1[J] = p2;
+            });
+        });
     }
 
     // Number of eigenvalues of T below x0, x1, x2 (three interleaved Sturm sequences of

@@ -362,7 +362,8 @@ int symx_hip_ctx_create(int device, symx_hip_ctx* out)
         return fail(nullptr, SYMX_HIP_ERR_HIP, "hipGetSymbolAddress of the loop scratch array failed");
     }
     HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
-    HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pcg, sizeof(PcgState), hipHostMallocDefault));
+    HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_pcg, 2 * sizeof(PcgState), hipHostMallocDefault));
+    for (auto& ev : ctx->pcg_ev) HIP_TRY(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     *out = ctx;
     return SYMX_HIP_OK;
 }
@@ -378,6 +379,8 @@ void symx_hip_ctx_destroy(symx_hip_ctx ctx)
     dfree(ctx->d_pcg);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->h_pcg) (void)hipHostFree(ctx->h_pcg);
+    for (auto& ev : ctx->pcg_ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1324,10 +1327,11 @@ int symx_hip_problem_export_bsr(symx_hip_problem p, int64_t* nnzb, int64_t* row_
     // assembly is enqueued without waiting; the copies below run outside the context's stream
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<double> v;
+    const bool f64 = p->assembled_precision == SYMX_HIP_FP64;
     if (vals) {
         if (p->assembled_precision < 0) return fail(ctx, SYMX_HIP_ERR_ARG, "matrix not assembled");
         v.resize((size_t)S.n_slots * 9);
-        if (p->assembled_precision == SYMX_HIP_FP64) {
+        if (f64) {
             HIP_TRY(ctx, hipMemcpy(v.data(), p->d_val64, v.size() * 8, hipMemcpyDeviceToHost));
         }
         else {
@@ -1349,7 +1353,7 @@ int symx_hip_problem_export_bsr(symx_hip_problem p, int64_t* nnzb, int64_t* row_
             if (cols) cols[k] = cj.first;
             if (vals) {
                 const int64_t t = base + 64 * (int64_t)cj.second;
-                for (int q = 0; q < 9; q++) vals[k * 9 + q] = v[((t >> 6) * 9 + q) * 64 + (t & 63)];
+                for (int q = 0; q < 9; q++) vals[k * 9 + q] = v[f64 ? sell_val_index<double>(t, q) : sell_val_index<float>(t, q)];
             }
             k++;
         }
@@ -1358,23 +1362,32 @@ int symx_hip_problem_export_bsr(symx_hip_problem p, int64_t* nnzb, int64_t* row_
     return SYMX_HIP_OK;
 }
 
-// Iterations to enqueue before the next look at the PCG state: the remaining count at the
-// mean rate so far (err/err0 per iteration, geometric) plus two, within [4, 256]. Iterations
-// past convergence exit at once on the device, so the guess only trades launches against
-// host round trips; the iterates themselves do not depend on it.
-static int next_pcg_batch(const PcgState& h, int batch, double abs_tol, double rel_tol)
+// Iterations to enqueue before the next look at the PCG state. The remaining count is
+// estimated from the rate since the previous look (err per iteration, geometric; CG speeds
+// up as it goes, so the mean rate since the start overestimates it) and from the mean rate,
+// taking the smaller; 85% of it is enqueued, at least two and at most 256 iterations. A look
+// costs one host round trip, an iteration enqueued past convergence one no-op launch per
+// kernel (each about 30 us at the 6M-tet block), so the guess aims just short of the end.
+// Iterations past convergence exit at once on the device: the guess only trades launches
+// against round trips, and the iterates do not depend on it.
+static int next_pcg_batch(const PcgState& h, int batch, double abs_tol, double rel_tol, int32_t prev_it, double prev_err)
 {
     double next = 2.0 * batch;
     if (h.it > 0 && h.err0 > 0.0 && h.err > 0.0 && std::isfinite(h.err)) {
         const double ratio = h.err / h.err0;
         const double target = std::max(rel_tol, abs_tol / h.err0);
         if (ratio < 1.0 && target < ratio) {
-            const double n = h.it * std::log(target / ratio) / std::log(ratio);
-            if (std::isfinite(n)) next = n + 2.0;
+            double n = h.it * std::log(target / ratio) / std::log(ratio);
+            if (h.it > prev_it && prev_err > h.err && h.err > 0.0) {
+                const double per_it = std::log(h.err / prev_err) / (h.it - prev_it);
+                const double n_recent = std::log(target / ratio) / per_it;
+                if (std::isfinite(n_recent) && n_recent > 0.0) n = std::min(n, n_recent);
+            }
+            if (std::isfinite(n)) next = n < 4.0 ? std::ceil(n) : std::ceil(0.85 * n);
         }
     }
     next = std::min(next, (double)std::max(h.max_iter - h.it, 1));
-    return (int)std::max(4.0, std::min(256.0, std::ceil(next)));
+    return (int)std::max(2.0, std::min(256.0, next));
 }
 
 static int run_pcg(symx_hip_problem p, bool fp32, const double* b64, double* x64, double abs_tol, double rel_tol, int32_t max_iter,
@@ -1391,10 +1404,16 @@ static int run_pcg(symx_hip_problem p, bool fp32, const double* b64, double* x64
     HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pcg, ctx->h_pcg, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
     if (fp32) launch_pcg_init32(ctx->stream, S, p->d_dinv32, p->f_b, p->f_x, p->f_r, p->f_z, p->f_p0, ctx->d_partials);
     else launch_pcg_init64(ctx->stream, S, p->d_dinv64, b64, x64, p->d_r, p->d_z, p->d_p0, ctx->d_partials);
+    // Batches of iterations in a two-deep pipeline: while the device runs one batch, the
+    // host reads the state at the end of the previous one (a copy into its own pinned slot,
+    // waited on by event) and enqueues the next, so the device does not idle for the round
+    // trip; the next batch holds the predicted remainder beyond the batch in flight, and
+    // none is enqueued while the one in flight is expected to finish the solve. Iterations
+    // past convergence exit at once (done is sticky).
     int32_t k = 1;
-    int batch = 8;
-    while (true) {
-        for (int i = 0; i < batch; i++, k++) {
+    int pending[2] = { 0, 0 };  // iterations enqueued ahead of each slot's state copy
+    auto enqueue = [&](int n, int slot) -> int {
+        for (int i = 0; i < n; i++, k++) {
             if (fp32)
                 launch_pcg_iter32(ctx->stream, S, k, p->d_val32, p->d_dinv32, p->f_x, p->f_r, p->f_z, (k & 1) ? p->f_p0 : p->f_p1,
                                   (k & 1) ? p->f_p1 : p->f_p0, p->f_q, ctx->d_pcg, ctx->d_partials);
@@ -1403,11 +1422,44 @@ static int run_pcg(symx_hip_problem p, bool fp32, const double* b64, double* x64
                                   (k & 1) ? p->d_p1 : p->d_p0, p->d_q, ctx->d_pcg, ctx->d_partials);
         }
         HIP_TRY(ctx, hipGetLastError());
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pcg, ctx->d_pcg, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        if (ctx->h_pcg->done) break;
-        batch = next_pcg_batch(*ctx->h_pcg, batch, abs_tol, rel_tol);
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pcg + slot, ctx->d_pcg, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->pcg_ev[slot], ctx->stream));
+        pending[slot] = n;
+        return SYMX_HIP_OK;
+    };
+    int rc = enqueue(8, 0);
+    if (!rc) rc = enqueue(8, 1);
+    if (rc) return rc;
+    int cur = 0, batch = 8;
+    int32_t prev_it = 0;
+    double prev_err = 0.0;
+    while (true) {
+        HIP_TRY(ctx, hipEventSynchronize(ctx->pcg_ev[cur]));
+        const PcgState h = ctx->h_pcg[cur];
+        pending[cur] = 0;
+        if (h.done) break;
+        const int other = cur ^ 1;
+        if (pending[other] == 0) {
+            // nothing in flight: the solve is not done, so enqueue the predicted rest now
+            batch = next_pcg_batch(h, batch, abs_tol, rel_tol, prev_it, prev_err);
+            rc = enqueue(batch, cur);
+            if (rc) return rc;
+        }
+        else {
+            const int rest = next_pcg_batch(h, batch, abs_tol, rel_tol, prev_it, prev_err) - pending[other];
+            if (rest >= 2) {
+                batch = rest;
+                rc = enqueue(batch, cur);
+                if (rc) return rc;
+            }
+            cur = other;
+        }
+        prev_it = h.it;
+        prev_err = h.err;
     }
+    // the state of the finished solve (later batches leave it unchanged)
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(ctx->h_pcg, ctx->d_pcg, sizeof(PcgState), hipMemcpyDeviceToHost));
     out = *ctx->h_pcg;
     return SYMX_HIP_OK;
 }
