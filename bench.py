@@ -30,7 +30,7 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the default
 # workload (101^3 block, mixed), each counter scaled by its reading for a 1 GiB device copy
 # (--calibration-copy-mib): FETCH x 2.0, WRITE x 1.0. The per-kernel table is in profiles/.
-MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 758.8e6}
+MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 759.3e6}
 # what the mixed path computes in: element kernels and projection in FP64 arithmetic with
 # FP32 element-Hessian storage, FP32 block matrix and inner PCG, FP64 outer refinement,
 # FP64 energies, gradients and line search
