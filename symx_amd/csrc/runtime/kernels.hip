@@ -680,7 +680,7 @@ namespace symx_hip
     //  6. Back-transform u = Q y (one wave per vector, no barriers), then the rank-k update.
     constexpr int PNT = 256;   // threads per matrix
     constexpr int PPTS = 2;    // Sturm points per thread per round
-    constexpr int PLU = 16;    // lanes running inverse iteration at once
+    constexpr int PLU = 16;    // lanes running inverse iteration at once (full-size variant)
 
     // Full-wave reductions through DPP (quad permutes and row rotations stay in the VALU;
     // __shfl_xor goes through ds_bpermute at LDS latency), then the four row results read
@@ -745,17 +745,21 @@ namespace symx_hip
     }
 
     // One matrix h (element e) with the whole workgroup of PT threads (a multiple of 64);
-    // every early return is uniform.
-    template <int N, bool DENSE, int PT, typename T>
-    __device__ void project_large_one(T* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
+    // every early return is uniform. KMAX bounds the eigenvectors held in LDS and PL the
+    // lanes running inverse iteration at once: with more than KMAX eigenvalues below eps the
+    // matrix is left untouched and true is returned (the caller hands it to a variant with
+    // KMAX = N). A smaller LDS footprint lets a CU hold several matrices at once.
+    template <int N, bool DENSE, int PT, typename T, int KMAX = N, int PL = PLU>
+    __device__ bool project_large_one(T* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
                                       uint8_t* __restrict__ changed_out)
     {
         constexpr int NB = N / 3, LD = N | 1, NUB = NB * (NB + 1) / 2, NW = PT / 64, NPT = PT * PPTS;
         static_assert(N <= 128, "one wave holds a vector in two registers per lane");
+        static_assert(KMAX >= 1 && KMAX <= N && PL >= 1 && PL <= 64, "eigenvector and solve lane bounds");
         __shared__ double A[N * LD];
-        __shared__ double Y[N * N];
+        __shared__ double Y[KMAX * N];
         __shared__ double dg[N], od[N], tau[N], lamv[N], pv[N], wv[N], bra[N], brb[N];
-        __shared__ double lu[3 * N * PLU];
+        __shared__ double lu[3 * N * PL];
         __shared__ int cbuf[NPT];
         __shared__ double red[2 * NW];
         __shared__ int busy[3];
@@ -817,7 +821,7 @@ namespace symx_hip
                 if (count != nullptr) atomicAdd(count, 1ull);
                 if (changed_out != nullptr) changed_out[e] = 1;
             }
-            return;
+            return false;
         }
         if (ti) {
             const double sh = scale / NB;
@@ -917,8 +921,9 @@ namespace symx_hip
                 if (tid == 0 && count != nullptr) atomicAdd(count, 1ull);
             }
             if (tid == 0 && changed_out != nullptr) changed_out[e] = (ti && !mirror) ? 1 : 0;
-            return;
+            return false;
         }
+        if (cnt > KMAX) return true;
         // ---- 4. multisection for the cnt lowest eigenvalues ------------------------------
         {
             const double w0 = (cut - glo) / (NPT + 1);
@@ -988,8 +993,8 @@ namespace symx_hip
         // 1e-3 ||T|| (dstein's reorthogonalisation threshold).
         {
             double* ur = lu;
-            double* u1 = lu + N * PLU;
-            double* u2 = lu + 2 * N * PLU;
+            double* u1 = lu + N * PL;
+            double* u2 = lu + 2 * N * PL;
             const double ptol = ulp * tn, bigr = 1.0 / ptol;
             for (int t = tid; t < cnt * N; t += PT) {
                 const int j = t / N;
@@ -1020,8 +1025,8 @@ namespace symx_hip
                 }
             };
             for (int it = 0; it < 3; it++) {
-                if (tid < PLU) {
-                    for (int j = tid; j < cnt; j += PLU) {
+                if (tid < PL) {
+                    for (int j = tid; j < cnt; j += PL) {
                         double* y = Y + j * N;
                         const double lam = lamv[j];
                         double a = dg[0] - lam, c = od[0], r0 = y[0];
@@ -1033,20 +1038,20 @@ namespace symx_hip
                             const double piv = sw ? b : a, oth = sw ? a : b;
                             const double inv = fabs(piv) >= 2.2250738585072014e-308 ? rcp_nr(piv) : 0.0;
                             const double mult = oth * inv;  // |mult| <= 1; a = b = 0 gives 0
-                            u1[k * PLU + tid] = sw ? an : c;
-                            u2[k * PLU + tid] = sw ? cn : 0.0;
+                            u1[k * PL + tid] = sw ? an : c;
+                            u2[k * PL + tid] = sw ? cn : 0.0;
                             y[k] = sw ? r1 : r0;
                             r0 = sw ? r0 - mult * r1 : r1 - mult * r0;
                             const double an2 = sw ? c - mult * an : an - mult * c;
                             c = sw ? -mult * cn : cn;
                             a = an2;
-                            ur[k * PLU + tid] = fabs(piv) < ptol ? (piv < 0.0 ? -bigr : bigr) : inv;
+                            ur[k * PL + tid] = fabs(piv) < ptol ? (piv < 0.0 ? -bigr : bigr) : inv;
                         }
-                        ur[(N - 1) * PLU + tid] = fabs(a) < ptol ? (a < 0.0 ? -bigr : bigr) : rcp_nr(a);
-                        double y1 = r0 * ur[(N - 1) * PLU + tid], y2 = 0.0;
+                        ur[(N - 1) * PL + tid] = fabs(a) < ptol ? (a < 0.0 ? -bigr : bigr) : rcp_nr(a);
+                        double y1 = r0 * ur[(N - 1) * PL + tid], y2 = 0.0;
                         y[N - 1] = y1;
                         for (int k = N - 2; k >= 0; k--) {  // back substitution, U has two superdiagonals
-                            const double v = (y[k] - u1[k * PLU + tid] * y1 - u2[k * PLU + tid] * y2) * ur[k * PLU + tid];
+                            const double v = (y[k] - u1[k * PL + tid] * y1 - u2[k * PL + tid] * y2) * ur[k * PL + tid];
                             y[k] = v;
                             y2 = y1;
                             y1 = v;
@@ -1104,18 +1109,24 @@ namespace symx_hip
             if (count != nullptr) atomicAdd(count, 1ull);
             if (changed_out != nullptr) changed_out[e] = 1;
         }
+        return false;
     }
 
-    template <int N, bool DENSE, typename T>
+    // too_many (nullable): 1 for each matrix left to the full-size variant (KMAX < N), 0 for
+    // every other one, selected or not.
+    template <int N, bool DENSE, typename T, int KMAX = N, int PL = PLU>
     __global__ __launch_bounds__(PNT) void k_project_large(T* __restrict__ H, int64_t n_mats, double eps, int mirror,
                                                           unsigned long long* count, const uint8_t* __restrict__ select,
-                                                          uint8_t* __restrict__ changed_out)
+                                                          uint8_t* __restrict__ changed_out, uint8_t* __restrict__ too_many)
     {
         constexpr int NUB = (N / 3) * (N / 3 + 1) / 2;
         const int64_t e = blockIdx.x;
         if (e >= n_mats) return;
-        if (select != nullptr && !select[e]) return;
-        project_large_one<N, DENSE, PNT>(DENSE ? H + e * (int64_t)(N * N) : H + e * (int64_t)(NUB * 9), e, eps, mirror, count, changed_out);
+        bool left = false;
+        if (select == nullptr || select[e])
+            left = project_large_one<N, DENSE, PNT, T, KMAX, PL>(DENSE ? H + e * (int64_t)(N * N) : H + e * (int64_t)(NUB * 9), e, eps, mirror,
+                                                                 count, changed_out);
+        if (too_many != nullptr && threadIdx.x == 0) too_many[e] = left ? 1 : 0;
     }
 
     // The elements listed in list[0 .. *n_list) (upper-block storage), a fixed grid of
@@ -1129,7 +1140,7 @@ namespace symx_hip
         const unsigned int n = *n_list;
         for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
             const int64_t e = list[i];
-            project_large_one<N, false, PT>(H + e * (int64_t)(NUB * 9), e, eps, mirror, count, nullptr);
+            project_large_one<N, false, PT, T>(H + e * (int64_t)(NUB * 9), e, eps, mirror, count, nullptr);
             __syncthreads();  // LDS is reused by the next matrix
         }
     }
@@ -1163,9 +1174,18 @@ namespace symx_hip
             k_list_flagged<<<grid_for(n_mats), NT, 0, s>>>(flags, n_mats, list, n_list);
             k_project_large_list<12, 64, T><<<4096, 64, 0, s>>>(H, list, n_list, eps, mirror, d_count);
             return 0;
-        case 8: k_project_large<24, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
-        case 10: k_project_large<30, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
-        case 27: k_project_large<81, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr); return 0;
+        case 8: k_project_large<24, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr, nullptr); return 0;
+        case 10: k_project_large<30, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr, nullptr); return 0;
+        case 27:
+            // 81x81: eigenvectors for at most 8 eigenvalues below eps and four inverse-iteration
+            // lanes keep the LDS at 73 KB, two matrices per CU (the full-size variant takes
+            // 143 KB, one per CU); Hex27 elements have at most 3 such eigenvalues as a rule, the
+            // others are listed and projected by the full-size variant
+            k_project_large<81, false, T, 8, 4><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr, flags);
+            k_zero_counter<<<1, 1, 0, s>>>(n_list);
+            k_list_flagged<<<grid_for(n_mats), NT, 0, s>>>(flags, n_mats, list, n_list);
+            k_project_large_list<81, PNT, T><<<1024, PNT, 0, s>>>(H, list, n_list, eps, mirror, d_count);
+            return 0;
         default: return -1;
         }
     }
@@ -1308,9 +1328,9 @@ namespace symx_hip
         case 6: k_project_dense<6><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
         case 9: k_project_dense<9><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
         case 12: k_project_dense<12><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
-        case 24: k_project_large<24, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
-        case 30: k_project_large<30, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
-        case 81: k_project_large<81, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed); return 0;
+        case 24: k_project_large<24, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed, nullptr); return 0;
+        case 30: k_project_large<30, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed, nullptr); return 0;
+        case 81: k_project_large<81, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed, nullptr); return 0;
         default: return -1;
         }
     }
