@@ -200,6 +200,13 @@ int symx_host_hex_element(const symx_hex_block_params* p, int32_t structured, in
 /* Evaluate the generated adjoint outputs d(lambda . dE_e/dx)/dlog_mu_e of element e of
  * a log_mu_field scene on the host (symbolic graph, kernel's gather rules); lambda
  * has one value per DoF. */
+/* Generated E, gradient (n) and Hessian (n x n, row-major) of element e of potential
+ * `potential` of a Tet4 block scene, evaluated on the host from the expressions the element
+ * kernels are generated from: with chain_rule != 0 the derivatives go through the linear cut
+ * of the expression where that is chosen (*was_factorised = 1), else they are the flat
+ * symbolic derivatives. *n_dofs receives n (g and H must hold 36 and 1296 values). */
+int symx_host_scene_element(const symx_tet4_block_params* p, int32_t chain_rule, int32_t potential, int32_t e, double* E, double* g,
+                            double* H, int32_t* n_dofs, int32_t* was_factorised);
 int symx_host_scene_adjoint_element(const symx_tet4_block_params* p, const double* log_mu, int32_t e, const double* lambda, double* out);
 
 #ifdef __cplusplus

@@ -462,6 +462,20 @@ def host_scene_jit(params: Tet4BlockParams, cache_dir: str):
     return n.value, cs.value, ops.value
 
 
+def host_scene_element(params: Tet4BlockParams, potential: int, e: int, chain_rule: bool = True):
+    """(E, gradient, Hessian, factorised?) of element e of a potential of a Tet4 block scene,
+    evaluated on the host from the generated expressions (no GPU)."""
+    E = ctypes.c_double()
+    g = np.zeros(36)
+    H = np.zeros(36 * 36)
+    n = c_int32()
+    fac = c_int32()
+    check_scene(amd().symx_host_scene_element(ctypes.byref(params), c_int32(int(chain_rule)), c_int32(potential), c_int32(e),
+                                              ctypes.byref(E), _ptr(g), _ptr(H), ctypes.byref(n), ctypes.byref(fac)))
+    k = n.value
+    return E.value, g[:k].copy(), H[: k * k].reshape(k, k).copy(), bool(fac.value)
+
+
 def host_scene_adjoint_element(params: Tet4BlockParams, log_mu: np.ndarray, e: int, lam: np.ndarray) -> np.ndarray:
     """Generated adjoint outputs of element e evaluated on the host expression graph."""
     log_mu = np.ascontiguousarray(log_mu, np.float64)

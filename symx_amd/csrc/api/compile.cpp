@@ -7,6 +7,7 @@
 #include <cstring>
 #include <sstream>
 #include <stdexcept>
+#include <unordered_map>
 
 namespace symx
 {
@@ -190,7 +191,7 @@ namespace symx
 
         void emit_gather(std::ostringstream& os, const Graph& g, const std::vector<DataMap>& maps, const SymbolUse& use,
                          const SummationMap& sum, const std::vector<std::array<int32_t, 3>>& adjoint_symbols,
-                         const std::vector<int32_t>& preset_symbols = {})
+                         const std::vector<int32_t>& preset_symbols = {}, const char* elem = "e")
         {
             std::vector<uint8_t> covered(g.n_symbols(), 0);
             for (int32_t s : preset_symbols) covered[s] = 1;
@@ -212,7 +213,7 @@ namespace symx
                 if (dm.connectivity_index >= 0) {
                     const int c = dm.connectivity_index;
                     if (c < 64 && !col_loaded[c]) {
-                        os << "        const long long i" << c << " = (long long)p.conn[" << c << " * (long long)p.n_elem + e];\n";
+                        os << "        const long long i" << c << " = (long long)p.conn[" << c << " * (long long)p.n_elem + " << elem << "];\n";
                         col_loaded[c] = 1;
                     }
                     base = "i" + std::to_string(c) + " * " + std::to_string(dm.stride) + " + ";
@@ -233,6 +234,125 @@ namespace symx
             for (int32_t s = 0; s < g.n_symbols(); s++) {
                 if (use.used[s] && !covered[s]) throw std::runtime_error("symx: expression uses a symbol that is not bound to data");
             }
+        }
+
+        // Single-point gradient / gradient+Hessian kernel with chunked output staging. The
+        // outputs are written in chunks of 128 bytes per row (16 doubles, or 32 floats for
+        // FP32 Hessian storage): the statements are emitted segment by segment (energy and
+        // the first gradient chunk, the further gradient chunks, then each Hessian chunk),
+        // each segment holding the not yet emitted nodes its outputs reach, in id order; an
+        // output goes to the wave's LDS staging right after its statement, and the chunk is
+        // stored with coalesced writes at the end of its segment. Only values that later
+        // segments reuse stay in registers, instead of every output until the end of the
+        // kernel. The body runs for every lane (the element index clamped to the last
+        // element, so that every gather stays in bounds); the staged rows of inactive or
+        // out-of-range lanes are zeroed before each flush, and a workgroup without an active
+        // element only writes zero rows.
+        void emit_chunked_kernel(std::ostringstream& os, const Graph& g, const std::vector<DataMap>& maps, const SummationMap& sum,
+                                 const KernelSpec& ks, bool h32)
+        {
+            const int n = ks.n_dofs, nb = n / 3, n_ub = nb * (nb + 1) / 2, nh = n_ub * 9;
+            const bool pgh = ks.kind == KernelKind::PGH;
+            std::vector<int32_t> roots;
+            for (const Scalar& s : ks.outputs) roots.push_back(s.id);
+            const SymbolUse use = symbol_use(g, reachable(g, roots), sum);
+            // output slots: gradient entries, then the Hessian's upper 3x3 blocks in storage order
+            struct Chunk
+            {
+                bool hess;
+                int c0, cw;
+                std::vector<int32_t> node;  // node id of each position
+            };
+            std::vector<Chunk> chunks;
+            for (int c0 = 0; c0 < n; c0 += 16) {
+                Chunk c{ false, c0, std::min(16, n - c0), {} };
+                for (int i = c0; i < c0 + c.cw; i++) c.node.push_back(ks.outputs[1 + i].id);
+                chunks.push_back(c);
+            }
+            if (pgh) {
+                std::vector<int32_t> hv;
+                for (int a = 0; a < nb; a++)
+                    for (int b = a; b < nb; b++)
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++) hv.push_back(ks.outputs[1 + n + (3 * a + i) * n + 3 * b + j].id);
+                const int cw = h32 ? 32 : 16;
+                for (int c0 = 0; c0 < nh; c0 += cw) {
+                    Chunk c{ true, c0, std::min(cw, nh - c0), {} };
+                    c.node.assign(hv.begin() + c0, hv.begin() + c0 + c.cw);
+                    chunks.push_back(c);
+                }
+            }
+            os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.name << (h32 ? "_f32" : "") << "(const symx_hip_eval_params p)\n{\n";
+            os << "    const int e = symx_block() * 256 + threadIdx.x;\n";
+            os << "    const int ec = e < p.n_elem ? e : p.n_elem - 1;\n";
+            os << "    const bool act = e < p.n_elem && (p.active == nullptr || p.active[ec] != 0);\n";
+            os << "    const long long e0 = (long long)symx_block() * 256 + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63);\n";
+            os << "    const long long rem = (long long)p.n_elem - e0;\n";
+            os << "    const int nv = rem < 0 ? 0 : (rem > 64 ? 64 : (int)rem);\n";
+            const std::string hdst = h32 ? "(float*)p.H" : "p.H";
+            os << "    if (!__syncthreads_or(act)) {\n";
+            os << "        symx_zero_rows<" << n << ">(p.G, e0, nv);\n";
+            if (pgh) os << "        symx_zero_rows<" << nh << ">(" << hdst << ", e0, nv);\n";
+            os << "        symx_store_energy(0.0, p, e);\n        return;\n    }\n";
+            os << "    double* const st = symx_stage_wave();\n";
+            emit_gather(os, g, maps, use, sum, {}, {}, "ec");
+            if (use.uses_summation) {  // a single summation point: its values are constants
+                for (int k = 0; k < sum.stride; k++) {
+                    const int s = sum.first_symbol_idx + k;
+                    if (use.used[s]) os << "    const double s" << s << " = " << literal(sum.data[k]) << ";\n";
+                }
+            }
+            std::vector<uint8_t> emitted(g.size(), 0);
+            for (size_t ci = 0; ci < chunks.size(); ci++) {
+                const Chunk& c = chunks[ci];
+                std::vector<int32_t> seg_roots = c.node;
+                if (ci == 0) seg_roots.push_back(ks.outputs[0].id);
+                // positions of each node in this chunk
+                std::unordered_map<int32_t, std::vector<int>> pos;
+                for (int k = 0; k < c.cw; k++) pos[c.node[k]].push_back(k);
+                std::vector<uint8_t> staged(c.cw, 0);
+                const char* put = (c.hess && h32) ? "symx_put_f32" : "symx_put";
+                // depth-first post-order from the outputs in position order: each output's
+                // statements come right before it
+                for (int32_t root : seg_roots) {
+                    std::vector<std::pair<int32_t, int>> stack = { { root, 0 } };
+                    while (!stack.empty()) {
+                        const int32_t id = stack.back().first;
+                        const int state = stack.back().second;
+                        if (emitted[id]) {
+                            stack.pop_back();
+                            continue;
+                        }
+                        const Node& nd = g.node(id);
+                        if (!is_op(nd)) {
+                            emitted[id] = 1;
+                            stack.pop_back();
+                            continue;
+                        }
+                        const int32_t kids[3] = { nd.a, (nd.op == Op::PowN) ? -1 : nd.b, (nd.op == Op::Branch) ? nd.c : -1 };
+                        if (state < 3) {
+                            stack.back().second = state + 1;
+                            if (kids[state] >= 0 && !emitted[kids[state]]) stack.push_back({ kids[state], 0 });
+                            continue;
+                        }
+                        stack.pop_back();
+                        emitted[id] = 1;
+                        os << "    const double t" << id << " = " << expr_of(g, id) << ";\n";
+                        auto it = pos.find(id);
+                        if (it == pos.end()) continue;
+                        for (int k : it->second) {
+                            os << "    " << put << "(st, " << k << ", t" << id << ");\n";
+                            staged[k] = 1;
+                        }
+                    }
+                }
+                for (int k = 0; k < c.cw; k++)
+                    if (!staged[k]) os << "    " << put << "(st, " << k << ", " << ref(g, c.node[k]) << ");\n";
+                if (ci == 0) os << "    const double e_out = act ? " << ref(g, ks.outputs[0].id) << " : 0.0;\n";
+                if (c.hess) os << "    symx_flush_rows<" << nh << ", " << c.c0 << ", " << c.cw << ">(" << hdst << ", st, e0, nv, act);\n";
+                else os << "    symx_flush_rows<" << n << ", " << c.c0 << ", " << c.cw << ">(p.G, st, e0, nv, act);\n";
+            }
+            os << "    symx_store_energy(e_out, p, e);\n}\n";
         }
     }
 
@@ -260,6 +380,10 @@ namespace symx
             const bool h32 = item.second;
             std::vector<int32_t> roots;
             for (const Scalar& s : ks.outputs) roots.push_back(s.id);
+            if ((ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH) && !multi_q) {
+                emit_chunked_kernel(os, g, maps, sum, ks, h32);
+                continue;
+            }
             const std::vector<uint8_t> mark = reachable(g, roots);
             const SymbolUse use = symbol_use(g, mark, sum);
             const int n = ks.n_dofs;

@@ -338,6 +338,186 @@ namespace symx
         return true;
     }
 
+    // ---- chain rule through the linear cut of an expression --------------------------------
+    // Most energies reach their DoFs through a few quantities that are affine in the DoFs
+    // with DoF-free coefficients: the deformation gradient F = Ds Dm^-1 of an elastic
+    // element (9 quantities for 12 DoFs), a distance vector, a strain. With L those
+    // quantities (the "cut": affine nodes that feed a non-affine operation) and
+    // E(x) = f(L(x)),
+    //     dE/dx = C^T df/dL,   d2E/dx2 = C^T (d2f/dL2) C,   C = dL/dx (DoF-free),
+    // so the generated code evaluates the material law's m first and m(m+1)/2 second
+    // derivatives once and contracts them with C's nonzeros, instead of differentiating the
+    // flat expression twice with respect to every DoF. The derivatives are the same
+    // functions (the rounding differs). The factorised form keeps fewer values live in the
+    // generated kernel, so it is used unless it takes over 1.5 times the flat form's
+    // operations.
+    namespace
+    {
+        bool g_chain_rule = true;
+
+        // Affine-cut nodes of `root` with respect to `dofs`; false when there is none worth
+        // factorising through (the expression is affine, or the cut is not narrower than
+        // the DoFs).
+        bool find_linear_cut(const Graph& g, int32_t root, const std::vector<Scalar>& dofs, std::vector<int32_t>& cut)
+        {
+            std::vector<uint8_t> is_dof(g.n_symbols(), 0);
+            for (const Scalar& d : dofs) is_dof[d.get_symbol_idx()] = 1;
+            // degree: 0 DoF-free, 1 affine in the DoFs with DoF-free coefficients, 2 otherwise
+            std::vector<uint8_t> deg(root + 1, 0), seen(root + 1, 0);
+            std::vector<int32_t> stack = { root };
+            while (!stack.empty()) {
+                const int32_t id = stack.back();
+                stack.pop_back();
+                if (id < 0 || seen[id]) continue;
+                seen[id] = 1;
+                const Node& n = g.node(id);
+                if (n.op == Op::Zero || n.op == Op::One || n.op == Op::Const || n.op == Op::Symbol) continue;
+                stack.push_back(n.a);
+                if (n.op == Op::Branch) {
+                    stack.push_back(n.b);
+                    stack.push_back(n.c);
+                }
+                else if (n.op != Op::PowN && n.b >= 0) {
+                    stack.push_back(n.b);
+                }
+            }
+            std::vector<uint8_t> feeds_nonaffine(root + 1, 0);
+            for (int32_t id = 0; id <= root; id++) {
+                if (!seen[id]) continue;
+                const Node& n = g.node(id);
+                uint8_t d = 0;
+                switch (n.op) {
+                case Op::Zero: case Op::One: case Op::Const: d = 0; break;
+                case Op::Symbol: d = is_dof[n.a] ? 1 : 0; break;
+                case Op::Add: case Op::Sub: d = std::max(deg[n.a], deg[n.b]); break;
+                case Op::Mul: d = deg[n.a] == 0 ? deg[n.b] : (deg[n.b] == 0 ? deg[n.a] : 2); break;
+                case Op::Branch: d = deg[n.c] == 0 ? std::max(deg[n.a], deg[n.b]) : 2; break;
+                case Op::PowN: d = deg[n.a] == 0 ? 0 : 2; break;
+                case Op::PowF: d = (deg[n.a] == 0 && deg[n.b] == 0) ? 0 : 2; break;
+                default: d = deg[n.a] == 0 ? 0 : 2; break;  // one-operand functions
+                }
+                deg[id] = d;
+                if (d == 2) {
+                    feeds_nonaffine[n.a] = 1;
+                    if (n.op == Op::Branch) {
+                        feeds_nonaffine[n.b] = 1;
+                        feeds_nonaffine[n.c] = 1;
+                    }
+                    else if (n.op != Op::PowN && n.b >= 0) {
+                        feeds_nonaffine[n.b] = 1;
+                    }
+                }
+            }
+            if (deg[root] != 2) return false;
+            cut.clear();
+            for (int32_t id = 0; id <= root; id++)
+                if (seen[id] && deg[id] == 1 && feeds_nonaffine[id]) cut.push_back(id);
+            return !cut.empty() && cut.size() < dofs.size();
+        }
+    }
+
+    void set_chain_rule(bool enabled) { g_chain_rule = enabled; }
+    bool get_chain_rule() { return g_chain_rule; }
+
+    bool chain_rule_derivatives(Graph& g, const Potential& P, const Scalar& expression, const std::vector<Scalar>& dofs, DiffCache& dc,
+                                Vector& grad, Matrix& hess)
+    {
+        std::vector<int32_t> cut;
+        if (!find_linear_cut(g, expression.id, dofs, cut)) return false;
+        const int32_t m = (int32_t)cut.size(), n = (int32_t)dofs.size();
+        while ((int32_t)P.cut_symbol_nodes.size() < m) P.cut_symbol_nodes.push_back(g.symbol());
+        std::vector<Scalar> Y;
+        std::unordered_map<int32_t, int32_t> to_y, to_cut;
+        for (int32_t r = 0; r < m; r++) {
+            Y.push_back(Scalar(&g, P.cut_symbol_nodes[r]));
+            to_y[cut[r]] = P.cut_symbol_nodes[r];
+            to_cut[P.cut_symbol_nodes[r]] = cut[r];
+        }
+        const Scalar f(&g, g.substitute({ expression.id }, to_y)[0]);
+        // f must reach the DoFs through the cut only
+        {
+            std::vector<uint8_t> dof_sym(g.n_symbols(), 0);
+            for (const Scalar& d : dofs) dof_sym[d.get_symbol_idx()] = 1;
+            std::vector<uint8_t> seen(g.size(), 0);
+            std::vector<int32_t> stack = { f.id };
+            while (!stack.empty()) {
+                const int32_t id = stack.back();
+                stack.pop_back();
+                if (id < 0 || seen[id]) continue;
+                seen[id] = 1;
+                const Node& nd = g.node(id);
+                if (nd.op == Op::Symbol) {
+                    if (dof_sym[nd.a]) return false;
+                    continue;
+                }
+                if (nd.op == Op::Zero || nd.op == Op::One || nd.op == Op::Const) continue;
+                stack.push_back(nd.a);
+                if (nd.op == Op::Branch) {
+                    stack.push_back(nd.b);
+                    stack.push_back(nd.c);
+                }
+                else if (nd.op != Op::PowN && nd.b >= 0) {
+                    stack.push_back(nd.b);
+                }
+            }
+        }
+        DiffCache dcy;
+        const Vector gy = gradient(f, Y, dcy);
+        const Matrix Ay = gradient(gy, Y, true, dcy);
+        // back to the cut nodes: df/dL and d2f/dL2 as functions of the element's data
+        std::vector<int32_t> roots;
+        for (int32_t r = 0; r < m; r++) roots.push_back(gy[r].id);
+        for (int32_t r = 0; r < m; r++)
+            for (int32_t s = r; s < m; s++) roots.push_back(Ay(r, s).id);
+        const std::vector<int32_t> back = g.substitute(roots, to_cut);
+        std::vector<Scalar> gL(m);
+        std::vector<std::vector<Scalar>> AL(m, std::vector<Scalar>(m));
+        {
+            size_t k = 0;
+            for (int32_t r = 0; r < m; r++) gL[r] = Scalar(&g, back[k++]);
+            for (int32_t r = 0; r < m; r++)
+                for (int32_t s = r; s < m; s++) AL[r][s] = AL[s][r] = Scalar(&g, back[k++]);
+        }
+        // C = dL/dx (DoF-free) and its nonzeros per DoF
+        std::vector<std::vector<std::pair<int32_t, Scalar>>> col(n);
+        for (int32_t r = 0; r < m; r++) {
+            for (int32_t d = 0; d < n; d++) {
+                const Scalar c = diff(Scalar(&g, cut[r]), dofs[d], dc);
+                if (!c.is_zero()) col[d].push_back({ r, c });
+            }
+        }
+        std::vector<Scalar> gv(n);
+        for (int32_t d = 0; d < n; d++) {
+            Scalar acc(&g, 0);
+            for (const auto& rc : col[d]) acc = acc + rc.second * gL[rc.first];
+            gv[d] = acc;
+        }
+        // Hessian entries in the order of the upper 3x3-block storage (block rows of a
+        // 3-DoF-per-node layout), so the generated code finishes them chunk by chunk
+        hess = Matrix::zero({ n, n }, Scalar(&g, 0));
+        std::vector<uint8_t> done((size_t)n * n, 0);
+        auto entry = [&](int32_t d, int32_t e) {
+            if (done[(size_t)d * n + e]) return;
+            Scalar acc(&g, 0);
+            for (const auto& rc : col[d])
+                for (const auto& sc : col[e]) acc = acc + (rc.second * sc.second) * AL[rc.first][sc.first];
+            hess(d, e) = acc;
+            hess(e, d) = acc;
+            done[(size_t)d * n + e] = done[(size_t)e * n + d] = 1;
+        };
+        if (n % 3 == 0) {
+            const int32_t nb = n / 3;
+            for (int32_t a = 0; a < nb; a++)
+                for (int32_t b = a; b < nb; b++)
+                    for (int32_t i = 0; i < 3; i++)
+                        for (int32_t j = 0; j < 3; j++) entry(3 * a + i, 3 * b + j);
+        }
+        for (int32_t d = 0; d < n; d++)
+            for (int32_t e = d; e < n; e++) entry(d, e);
+        grad = Vector(gv);
+        return true;
+    }
+
     // ---- per-potential code generation (SecondOrderCompiledPotential.cpp:8-70) ----------
     PotentialModule generate_potential_module(GlobalPotential& G, const Potential& P)
     {
@@ -401,8 +581,23 @@ namespace symx
         }
 
         DiffCache dc;
-        const Vector grad = gradient(expression, dofs, dc);
-        const Matrix hess = gradient(grad, dofs, true, dc);
+        Vector grad = gradient(expression, dofs, dc);
+        Matrix hess = gradient(grad, dofs, true, dc);
+        {
+            Vector grad_c;
+            Matrix hess_c;
+            if (g_chain_rule && chain_rule_derivatives(*g, P, expression, dofs, dc, grad_c, hess_c)) {
+                const int64_t flat = count_operations(*g, collect_scalars({ { expression }, grad.values(), hess.values() }));
+                const int64_t fact = count_operations(*g, collect_scalars({ { expression }, grad_c.values(), hess_c.values() }));
+                // the factorised form keeps fewer values live (the cut's derivatives and the
+                // contraction coefficients instead of per-DoF chains): worth some more operations
+                if (2 * fact <= 3 * flat) {
+                    grad = grad_c;
+                    hess = hess_c;
+                    pm.factorised = true;
+                }
+            }
+        }
         std::vector<KernelSpec>& specs = pm.specs;
         specs.resize(3);
         specs[0] = KernelSpec{ base + "_P", KernelKind::P, { expression }, n };

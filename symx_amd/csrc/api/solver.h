@@ -180,6 +180,9 @@ namespace symx
         bool has_condition = false;
         // Graph nodes of the adjoint symbols (one per DoF symbol), created on first use.
         mutable std::vector<int32_t> adjoint_symbol_nodes;
+        // Graph nodes of the symbols standing for the linear cut of the expression (the
+        // chain-rule factorisation of its derivatives), created on first use.
+        mutable std::vector<int32_t> cut_symbol_nodes;
     };
 
     class GlobalPotential
@@ -363,6 +366,7 @@ namespace symx
         std::vector<int32_t> dof_block_set;  // DoF set per DoF block
         std::vector<int32_t> map_dof_set;    // DoF set per data map (-1: not a DoF)
         bool structured = false;             // structured FEM kernels (see FemStructure)
+        bool factorised = false;             // derivatives through the linear cut (chain_rule_derivatives)
         FemStructure fem;
         int32_t elems_per_group = 256;
     };
@@ -371,6 +375,12 @@ namespace symx
     // fem_integrator expression is expanded and compiled as a generic element kernel.
     void set_structured_fem(bool enabled);
     bool get_structured_fem();
+    // Derivatives of generic potentials through the linear cut of their expression (chain
+    // rule, used when it takes fewer operations than the flat derivatives); on by default.
+    void set_chain_rule(bool enabled);
+    bool get_chain_rule();
+    bool chain_rule_derivatives(Graph& g, const Potential& P, const Scalar& expression, const std::vector<Scalar>& dofs, DiffCache& dc,
+                                Vector& grad, Matrix& hess);
     PotentialModule generate_potential_module(GlobalPotential& G, const Potential& P);
 
     // ---- views (View.h) ----------------------------------------------------------------

@@ -190,6 +190,64 @@ __device__ __forceinline__ void symx_store_rows_f32(float* dst, const V (&v)[W],
         __syncthreads();
     }
 }
+// Chunked output staging of single-point kernels: each output of the chunk [C0, C0 + CW)
+// of a W-wide element-major row is put into the wave's part of the staging buffer as soon
+// as it is computed (symx_put / symx_put_f32, lane-major with stride CW + 1), and
+// symx_flush_rows writes the chunk with coalesced stores, so no output waits in a register
+// for the end of the kernel. Every thread of the workgroup must call the flush.
+__device__ __forceinline__ double* symx_stage_wave()
+{
+    return symx_stage_rows_buffer() + (threadIdx.x >> 6) * (64 * 17);
+}
+__device__ __forceinline__ void symx_put(double* st, int j, double v)
+{
+    st[(threadIdx.x & 63) * 17 + j] = v;
+}
+__device__ __forceinline__ void symx_put_f32(double* st, int j, double v)
+{
+    reinterpret_cast<float*>(st)[(threadIdx.x & 63) * 33 + j] = (float)v;
+}
+// Lane l stores value t * 64 + l of the chunk, i.e. row r = q0 + t*64/CW (+1 when the
+// column wraps) and column j of it; rows at or past n_valid are not stored when GUARD.
+template <int W, int CW, int SS, bool GUARD, typename T>
+__device__ __forceinline__ void symx_flush_store(T* base, const T* st, int n_valid)
+{
+    const int lane = threadIdx.x & 63;
+    const int q0 = lane / CW, j0 = lane - q0 * CW;
+#pragma unroll
+    for (int t = 0; t < CW; t++) {
+        const int a = (t * 64) / CW, b = (t * 64) % CW;
+        const int jj = j0 + b;
+        const bool wrap = jj >= CW;
+        const int r = q0 + a + (wrap ? 1 : 0), j = wrap ? jj - CW : jj;
+        if (!GUARD || r < n_valid) base[r * W + j] = st[r * SS + j];
+    }
+}
+// e0 is the wave's first element (wave-uniform); rows of inactive elements are zeroed in
+// the staging first.
+template <int W, int C0, int CW, typename T>
+__device__ __forceinline__ void symx_flush_rows(T* dst, double* st_d, long long e0, int n_valid, bool act)
+{
+    constexpr int SS = sizeof(T) == 4 ? 33 : 17;
+    T* st = reinterpret_cast<T*>(st_d);
+    if (!act) {
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int j = 0; j < CW; j++) st[lane * SS + j] = (T)0;
+    }
+    __syncthreads();
+    T* const base = dst + (e0 * W + C0);
+    if (n_valid == 64) symx_flush_store<W, CW, SS, false>(base, st, n_valid);
+    else symx_flush_store<W, CW, SS, true>(base, st, n_valid);
+    __syncthreads();
+}
+// Zero rows [e0, e0 + n_valid) of width W (a workgroup with no active element).
+template <int W, typename T>
+__device__ __forceinline__ void symx_zero_rows(T* dst, long long e0, int n_valid)
+{
+    const int lane = threadIdx.x & 63;
+    for (int i = lane; i < n_valid * W; i += 64) dst[e0 * W + i] = (T)0;
+}
 // ---- end of prelude ----
 )PRELUDE";
 
@@ -301,7 +359,13 @@ __device__ __forceinline__ void symx_store_rows_f32(float* dst, const V (&v)[W],
         code.resize(n);
         hiprtcGetCode(prog, code.data());
         hiprtcDestroyProgram(&prog);
-        if (!path.empty()) write_cached_object(cache_dir, path, code);
+        if (!path.empty()) {
+            write_cached_object(cache_dir, path, code);
+            // the generated source beside its object, as SymX keeps <name>.cpp in its codegen
+            // folder (Compilation.cpp:329); it is not read back
+            std::ofstream src(std::string(cache_dir) + "/" + key + ".hip");
+            src << source;
+        }
         return SYMX_HIP_OK;
     }
 
@@ -1074,13 +1138,10 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
     if (f32)
         for (auto& pot : p->pots)
             if (pot.n_elem > 0 && pot.kPGH32.empty()) return fail(ctx, SYMX_HIP_ERR_ARG, "a potential has no FP32-storage Hessian kernel");
-    // PG / PGH keep the element energies: the reference point of symx_hip_problem_energy_change
-    const int32_t e_mode = mode == SYMX_HIP_EVAL_P ? SYMX_HIP_EMODE_SUM : SYMX_HIP_EMODE_STORE;
     for (auto& pot : p->pots) {
         if (pot.n_elem == 0) continue;
         symx_hip_eval_params prm;
         fill_params(p, pot, prm);
-        prm.e_mode = e_mode;
         if (f32) prm.H = reinterpret_cast<double*>(h32_of(p, pot));  // the _f32 kernels store floats
         if (!pot.kC.empty()) {
             int rc = launch_eval(ctx, pot.module, pot.kC, prm);
@@ -1089,8 +1150,18 @@ int symx_hip_problem_eval(symx_hip_problem p, int32_t mode, double* E)
         const std::string& k = mode == SYMX_HIP_EVAL_P ? pot.kP : (mode == SYMX_HIP_EVAL_PG ? pot.kPG : (f32 ? pot.kPGH32 : pot.kPGH));
         int rc = launch_eval(ctx, pot.module, k, prm, pot.elems_per_group);
         if (rc) return rc;
+        if (mode != SYMX_HIP_EVAL_P) {
+            // PG / PGH keep the element energies, the reference point of
+            // symx_hip_problem_energy_change, and take them from the energy kernel that
+            // evaluates the trial points there: the same code gives bitwise the same E_e at
+            // the same DoFs, and the differences carry no rounding of the derivative
+            // kernels' different instruction mix (its partial sums replace theirs)
+            prm.e_mode = SYMX_HIP_EMODE_STORE;
+            rc = launch_eval(ctx, pot.module, pot.kP, prm, pot.elems_per_group);
+            if (rc) return rc;
+        }
     }
-    if (e_mode == SYMX_HIP_EMODE_STORE) p->e_ref_valid = true;
+    if (mode != SYMX_HIP_EVAL_P) p->e_ref_valid = true;
     const bool with_h = mode >= SYMX_HIP_EVAL_PGH;
     if (with_h && p->d_projected) launch_fill_u8(ctx->stream, p->d_projected, p->total_elem);
     launch_sum(ctx->stream, p->d_Epart, p->total_parts, ctx->d_scalars);

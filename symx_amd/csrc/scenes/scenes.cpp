@@ -1073,6 +1073,52 @@ int symx_host_scene_jit(const symx_tet4_block_params* p, const char* cache_dir, 
     });
 }
 
+int symx_host_scene_element(const symx_tet4_block_params* p, int32_t chain_rule, int32_t potential, int32_t e, double* E, double* g,
+                            double* H, int32_t* n_dofs, int32_t* was_factorised)
+{
+    return guard([&]() {
+        symx_newton_params np;
+        symx_newton_params_default(&np);
+        symx_scene_s s;
+        build_scene(&s, p, &np);
+        const auto& pots = s.G->get_potentials();
+        if (potential < 0 || potential >= (int32_t)pots.size()) throw std::runtime_error("potential index out of range");
+        const Potential& P = *pots[potential];
+        const bool prev = get_chain_rule();
+        set_chain_rule(chain_rule != 0);
+        PotentialModule pm;
+        try {
+            pm = generate_potential_module(*s.G, P);
+        }
+        catch (...) {
+            set_chain_rule(prev);
+            throw;
+        }
+        set_chain_rule(prev);
+        const int n = pm.info.n_dofs;
+        if (n > 36) throw std::runtime_error("element has more than 36 DoFs");
+        MappedWorkspace<double>& mws = *P.mws;
+        if (e < 0 || e >= mws.conn.n_elements()) throw std::runtime_error("element index out of range");
+        Graph* gr = mws.workspace().graph();
+        const SummationMap& sm = mws.get_summation();
+        if (sm.n_iterations > 1) throw std::runtime_error("summed (multi-point) potential");
+        const int32_t* conn = mws.conn.data() + (size_t)e * mws.conn.stride;
+        for (const DataMap& m : mws.get_maps()) {
+            const int64_t item = m.connectivity_index >= 0 ? conn[m.connectivity_index] : 0;
+            for (int k = 0; k < m.stride; k++) gr->set_symbol_value(m.first_symbol_idx + k, m.data()[item * m.stride + k]);
+        }
+        for (int k = 0; k < sm.stride && sm.n_iterations == 1; k++) gr->set_symbol_value(sm.first_symbol_idx + k, sm.data[k]);
+        std::vector<int32_t> ids;
+        for (const Scalar& x : pm.specs[2].outputs) ids.push_back(x.id);
+        const std::vector<double> v = gr->eval_many(ids);
+        *E = v[0];
+        std::copy(v.begin() + 1, v.begin() + 1 + n, g);
+        std::copy(v.begin() + 1 + n, v.begin() + 1 + n + (size_t)n * n, H);
+        *n_dofs = n;
+        *was_factorised = pm.factorised ? 1 : 0;
+    });
+}
+
 int symx_host_scene_adjoint_element(const symx_tet4_block_params* p, const double* log_mu, int32_t e, const double* lambda, double* out)
 {
     return guard([&]() {

@@ -82,6 +82,36 @@ def test_stable_nh_is_translation_invariant_and_symmetric():
     assert np.abs(g - gfd).max() <= 1e-6 * np.abs(g).max()
 
 
+@pytest.mark.parametrize("material", [0, 1])
+def test_chain_rule_derivatives_match_flat_derivatives_and_oracle(oracle, material):
+    """The element kernels of a Neo-Hookean block differentiate through the linear cut of the
+    energy (F = Ds Dm^-1: 9 quantities for 12 DoFs): E, gradient and Hessian equal the flat
+    symbolic derivatives to round-off, and the oracle's closed form (NH). The inertia and
+    penalty potentials have no cut narrower than their DoFs and keep the flat form."""
+    p = symx_amd.default_block_params(cells=[2, 2, 2], noise=0.15, seed=3, material=material)
+    S = None
+    if material == 0:
+        from scene_np import tet4_block
+        S = tet4_block(cells=(2, 2, 2), noise=0.15, seed=3)  # the same scene inputs (pinned by a GPU test)
+    for e in (0, 5, 17, 47):
+        E1, g1, H1, fac = symx_amd.host_scene_element(p, 0, e, chain_rule=True)
+        E0, g0, H0, fac0 = symx_amd.host_scene_element(p, 0, e, chain_rule=False)
+        assert fac and not fac0
+        assert E1 == E0
+        assert np.abs(g1 - g0).max() <= 1e-13 * np.abs(g0).max()
+        assert np.abs(H1 - H0).max() <= 1e-13 * np.abs(H0).max()
+        assert np.abs(H1 - H1.T).max() == 0.0
+        if S is not None:
+            t = S.tets[e]
+            Eo, go, Ho = oracle.nh_tet4(S.x[t].ravel(), S.X[t].ravel(), S.lam, S.mu)
+            assert abs(E1 - Eo) <= 1e-12 * abs(Eo)
+            assert np.abs(g1 - go).max() <= 1e-11 * np.abs(go).max()
+            assert np.abs(H1 - Ho).max() <= 1e-11 * np.abs(Ho).max()
+    for pot in (1, 2):
+        _, _, _, fac = symx_amd.host_scene_element(p, pot, 0, chain_rule=True)
+        assert not fac
+
+
 def test_generated_module_compiles_for_gfx950():
     with tempfile.TemporaryDirectory() as d:
         size, ops = symx_amd.host_jit_tet4(d)
