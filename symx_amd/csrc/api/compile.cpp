@@ -158,6 +158,53 @@ namespace symx
 
         bool is_op(const Node& n) { return !(n.op == Op::Zero || n.op == Op::One || n.op == Op::Const || n.op == Op::Symbol); }
 
+        // FP32 form of a node whose operands are FP32 nodes of the same region (f<id>) or
+        // FP64 values rounded at the use.
+        std::string ref_f32(const Graph& g, int32_t id, const std::vector<uint8_t>& in_f32)
+        {
+            const Node& n = g.node(id);
+            switch (n.op) {
+            case Op::Zero: return "0.0f";
+            case Op::One: return "1.0f";
+            case Op::Const: {
+                char buf[64];
+                std::snprintf(buf, sizeof(buf), "%af", (double)(float)n.value);
+                return std::string("(") + buf + ")";
+            }
+            case Op::Symbol: return "(float)s" + std::to_string(n.a);
+            default: return (in_f32[id] ? "f" : "(float)t") + std::to_string(id);
+            }
+        }
+        std::string expr_of_f32(const Graph& g, int32_t id, const std::vector<uint8_t>& in_f32)
+        {
+            const Node& n = g.node(id);
+            const std::string a = n.a >= 0 ? ref_f32(g, n.a, in_f32) : "";
+            switch (n.op) {
+            case Op::Add: return a + " + " + ref_f32(g, n.b, in_f32);
+            case Op::Sub: return a + " - " + ref_f32(g, n.b, in_f32);
+            case Op::Mul: return a + " * " + ref_f32(g, n.b, in_f32);
+            case Op::Recip: return "1.0f / " + a;
+            case Op::PowN:
+                if (n.b == 2) return a + " * " + a;
+                if (n.b == 3) return a + " * " + a + " * " + a;
+                if (n.b == -1) return "1.0f / " + a;
+                return "symx_powif(" + a + ", " + std::to_string(n.b) + ")";
+            case Op::PowF: return "powf(" + a + ", " + ref_f32(g, n.b, in_f32) + ")";
+            case Op::Sqrt: return "sqrtf(" + a + ")";
+            case Op::Ln: return "symx_lnf(" + a + ")";
+            case Op::Log10: return "symx_log10f(" + a + ")";
+            case Op::Exp: return "expf(" + a + ")";
+            case Op::Sin: return "sinf(" + a + ")";
+            case Op::Cos: return "cosf(" + a + ")";
+            case Op::Tan: return "tanf(" + a + ")";
+            case Op::ASin: return "asinf(" + a + ")";
+            case Op::ACos: return "acosf(" + a + ")";
+            case Op::ATan: return "atanf(" + a + ")";
+            case Op::Branch: return "(" + ref_f32(g, n.c, in_f32) + " > 0.0f) ? " + a + " : " + ref_f32(g, n.b, in_f32);
+            default: throw std::runtime_error("symx: emitter met a non-operation node");
+            }
+        }
+
         // Statements for all reachable operation nodes, in topological (id) order.
         void emit_body(std::ostringstream& os, const Graph& g, const std::vector<uint8_t>& mark, const char* indent)
         {
@@ -282,6 +329,37 @@ namespace symx
                     chunks.push_back(c);
                 }
             }
+            // FP32 region of the FP32-storage twin: Hessian-only nodes between the outputs and
+            // the contraction inputs (which stay FP64 and are rounded at their uses)
+            std::vector<uint8_t> in_f32(g.size(), 0);
+            if (pgh && h32 && !ks.f32_from.empty()) {
+                std::vector<int32_t> eg_roots;
+                for (int i = 0; i <= n; i++) eg_roots.push_back(ks.outputs[i].id);
+                const std::vector<uint8_t> eg = reachable(g, eg_roots);
+                std::vector<uint8_t> stop(g.size(), 0);
+                for (int32_t id : ks.f32_from) stop[id] = 1;
+                std::vector<int32_t> stack;
+                for (const Chunk& c : chunks)
+                    if (c.hess) stack.insert(stack.end(), c.node.begin(), c.node.end());
+                std::vector<uint8_t> seen(g.size(), 0);
+                while (!stack.empty()) {
+                    const int32_t id = stack.back();
+                    stack.pop_back();
+                    if (id < 0 || seen[id]) continue;
+                    seen[id] = 1;
+                    const Node& nd = g.node(id);
+                    if (stop[id] || eg[id] || !is_op(nd)) continue;
+                    in_f32[id] = 1;
+                    stack.push_back(nd.a);
+                    if (nd.op == Op::Branch) {
+                        stack.push_back(nd.b);
+                        stack.push_back(nd.c);
+                    }
+                    else if (nd.op != Op::PowN && nd.b >= 0) {
+                        stack.push_back(nd.b);
+                    }
+                }
+            }
             os << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.name << (h32 ? "_f32" : "") << "(const symx_hip_eval_params p)\n{\n";
             os << "    const int e = symx_block() * 256 + threadIdx.x;\n";
             os << "    const int ec = e < p.n_elem ? e : p.n_elem - 1;\n";
@@ -337,17 +415,21 @@ namespace symx
                         }
                         stack.pop_back();
                         emitted[id] = 1;
-                        os << "    const double t" << id << " = " << expr_of(g, id) << ";\n";
+                        if (in_f32[id]) os << "    const float f" << id << " = " << expr_of_f32(g, id, in_f32) << ";\n";
+                        else os << "    const double t" << id << " = " << expr_of(g, id) << ";\n";
                         auto it = pos.find(id);
                         if (it == pos.end()) continue;
                         for (int k : it->second) {
-                            os << "    " << put << "(st, " << k << ", t" << id << ");\n";
+                            os << "    " << put << "(st, " << k << ", " << (in_f32[id] ? "f" : "t") << id << ");\n";
                             staged[k] = 1;
                         }
                     }
                 }
-                for (int k = 0; k < c.cw; k++)
-                    if (!staged[k]) os << "    " << put << "(st, " << k << ", " << ref(g, c.node[k]) << ");\n";
+                for (int k = 0; k < c.cw; k++) {
+                    if (staged[k]) continue;
+                    const int32_t id = c.node[k];
+                    os << "    " << put << "(st, " << k << ", " << (in_f32[id] ? "f" + std::to_string(id) : ref(g, id)) << ");\n";
+                }
                 if (ci == 0) os << "    const double e_out = act ? " << ref(g, ks.outputs[0].id) << " : 0.0;\n";
                 if (c.hess) os << "    symx_flush_rows<" << nh << ", " << c.c0 << ", " << c.cw << ">(" << hdst << ", st, e0, nv, act);\n";
                 else os << "    symx_flush_rows<" << n << ", " << c.c0 << ", " << c.cw << ">(p.G, st, e0, nv, act);\n";

@@ -265,6 +265,11 @@ namespace symx
         // PGP only: adjoint symbols {symbol id, data map m, component k}. Symbol `id`
         // reads the adjoint vector where DoF map m reads its component k.
         std::vector<std::array<int32_t, 3>> adjoint_symbols;
+        // PGH with derivatives through a linear cut: the FP32-storage twin forms the Hessian
+        // contraction in FP32, i.e. the Hessian-only nodes between the outputs and these nodes
+        // (the cut's second derivatives and the coefficients dL/dx, computed in FP64 and
+        // rounded once); empty: the whole kernel computes in FP64.
+        std::vector<int32_t> f32_from;
     };
 
     // Emit one hiprtc translation unit holding all kernels of a potential. The

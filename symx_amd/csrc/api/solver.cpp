@@ -420,7 +420,7 @@ namespace symx
     bool get_chain_rule() { return g_chain_rule; }
 
     bool chain_rule_derivatives(Graph& g, const Potential& P, const Scalar& expression, const std::vector<Scalar>& dofs, DiffCache& dc,
-                                Vector& grad, Matrix& hess)
+                                Vector& grad, Matrix& hess, std::vector<int32_t>* contraction_inputs)
     {
         std::vector<int32_t> cut;
         if (!find_linear_cut(g, expression.id, dofs, cut)) return false;
@@ -515,6 +515,13 @@ namespace symx
         for (int32_t d = 0; d < n; d++)
             for (int32_t e = d; e < n; e++) entry(d, e);
         grad = Vector(gv);
+        if (contraction_inputs) {
+            contraction_inputs->clear();
+            for (int32_t r = 0; r < m; r++)
+                for (int32_t s = r; s < m; s++) contraction_inputs->push_back(AL[r][s].id);
+            for (int32_t d = 0; d < n; d++)
+                for (const auto& rc : col[d]) contraction_inputs->push_back(rc.second.id);
+        }
         return true;
     }
 
@@ -583,10 +590,11 @@ namespace symx
         DiffCache dc;
         Vector grad = gradient(expression, dofs, dc);
         Matrix hess = gradient(grad, dofs, true, dc);
+        std::vector<int32_t> contraction_inputs;
         {
             Vector grad_c;
             Matrix hess_c;
-            if (g_chain_rule && chain_rule_derivatives(*g, P, expression, dofs, dc, grad_c, hess_c)) {
+            if (g_chain_rule && chain_rule_derivatives(*g, P, expression, dofs, dc, grad_c, hess_c, &contraction_inputs)) {
                 const int64_t flat = count_operations(*g, collect_scalars({ { expression }, grad.values(), hess.values() }));
                 const int64_t fact = count_operations(*g, collect_scalars({ { expression }, grad_c.values(), hess_c.values() }));
                 // the factorised form keeps fewer values live (the cut's derivatives and the
@@ -596,6 +604,9 @@ namespace symx
                     hess = hess_c;
                     pm.factorised = true;
                 }
+                else {
+                    contraction_inputs.clear();
+                }
             }
         }
         std::vector<KernelSpec>& specs = pm.specs;
@@ -603,6 +614,7 @@ namespace symx
         specs[0] = KernelSpec{ base + "_P", KernelKind::P, { expression }, n };
         specs[1] = KernelSpec{ base + "_P__dP_du", KernelKind::PG, collect_scalars({ { expression }, grad.values() }), n };
         specs[2] = KernelSpec{ base + "_P__dP_du__d2P_du2", KernelKind::PGH, collect_scalars({ { expression }, grad.values(), hess.values() }), n };
+        specs[2].f32_from = contraction_inputs;
         if (P.has_condition) specs.push_back(KernelSpec{ base + "_C", KernelKind::C, { condition }, n });
 
         DeviceGlobal::PotentialInfo& info = pm.info;

@@ -379,8 +379,10 @@ namespace symx
     // rule, used when it takes fewer operations than the flat derivatives); on by default.
     void set_chain_rule(bool enabled);
     bool get_chain_rule();
+    // contraction_inputs: the nodes the Hessian contraction reads (second derivatives of the
+    // cut quantities and the nonzero coefficients dL/dx).
     bool chain_rule_derivatives(Graph& g, const Potential& P, const Scalar& expression, const std::vector<Scalar>& dofs, DiffCache& dc,
-                                Vector& grad, Matrix& hess);
+                                Vector& grad, Matrix& hess, std::vector<int32_t>* contraction_inputs = nullptr);
     PotentialModule generate_potential_module(GlobalPotential& G, const Potential& P);
 
     // ---- views (View.h) ----------------------------------------------------------------

@@ -97,6 +97,16 @@ struct symx_hip_adjoint_params {
 static_assert(sizeof(symx_hip_adjoint_params) == 1040, "symx_hip_adjoint_params layout");
 __device__ __forceinline__ double symx_ln(double x) { return (x <= 0.0) ? -__builtin_inf() : log(x); }
 __device__ __forceinline__ double symx_log10(double x) { return (x <= 0.0) ? -__builtin_inf() : log10(x); }
+__device__ __forceinline__ float symx_lnf(float x) { return (x <= 0.0f) ? -__builtin_inff() : logf(x); }
+__device__ __forceinline__ float symx_log10f(float x) { return (x <= 0.0f) ? -__builtin_inff() : log10f(x); }
+__device__ __forceinline__ float symx_powif(float x, int n)
+{
+    const bool neg = n < 0;
+    unsigned int m = neg ? (unsigned int)(-n) : (unsigned int)n;
+    float r = 1.0f, b = x;
+    while (m) { if (m & 1u) r *= b; b *= b; m >>= 1; }
+    return neg ? 1.0f / r : r;
+}
 __device__ __forceinline__ double symx_powi(double x, int n)
 {
     const bool neg = n < 0;
@@ -207,20 +217,35 @@ __device__ __forceinline__ void symx_put_f32(double* st, int j, double v)
 {
     reinterpret_cast<float*>(st)[(threadIdx.x & 63) * 33 + j] = (float)v;
 }
+__device__ __forceinline__ void symx_put_f32(double* st, int j, float v)
+{
+    reinterpret_cast<float*>(st)[(threadIdx.x & 63) * 33 + j] = v;
+}
 // Lane l stores value t * 64 + l of the chunk, i.e. row r = q0 + t*64/CW (+1 when the
-// column wraps) and column j of it; rows at or past n_valid are not stored when GUARD.
+// column wraps) and column j of it; rows at or past n_valid are not stored when GUARD. When
+// CW divides 64 every store instruction covers 64/CW whole rows and the offsets advance by
+// compile-time steps.
 template <int W, int CW, int SS, bool GUARD, typename T>
 __device__ __forceinline__ void symx_flush_store(T* base, const T* st, int n_valid)
 {
     const int lane = threadIdx.x & 63;
     const int q0 = lane / CW, j0 = lane - q0 * CW;
+    if constexpr (64 % CW == 0) {
+        constexpr int RPT = 64 / CW;
+        const int go = q0 * W + j0, lo = q0 * SS + j0;
 #pragma unroll
-    for (int t = 0; t < CW; t++) {
-        const int a = (t * 64) / CW, b = (t * 64) % CW;
-        const int jj = j0 + b;
-        const bool wrap = jj >= CW;
-        const int r = q0 + a + (wrap ? 1 : 0), j = wrap ? jj - CW : jj;
-        if (!GUARD || r < n_valid) base[r * W + j] = st[r * SS + j];
+        for (int t = 0; t < CW; t++)
+            if (!GUARD || q0 + RPT * t < n_valid) base[go + RPT * W * t] = st[lo + RPT * SS * t];
+    }
+    else {
+#pragma unroll
+        for (int t = 0; t < CW; t++) {
+            const int a = (t * 64) / CW, b = (t * 64) % CW;
+            const int jj = j0 + b;
+            const bool wrap = jj >= CW;
+            const int r = q0 + a + (wrap ? 1 : 0), j = wrap ? jj - CW : jj;
+            if (!GUARD || r < n_valid) base[r * W + j] = st[r * SS + j];
+        }
     }
 }
 // e0 is the wave's first element (wave-uniform); rows of inactive elements are zeroed in

@@ -2,11 +2,13 @@
 
 The reference stores the block matrix as SYMX_HESS_STORAGE_FLOAT (ElementHessians.cpp:
 236-249 casts each element block when it is added); here the cast moves to the element
-Hessians themselves. The element kernels still compute in FP64 and round once, so the FP32
-values must be the FP64 kernel's values rounded to float (to one float ulp, allowing for a
-different FMA contraction between the two kernels); projection reads them, works in FP64
-and rounds its result once; assembly sums them in FP64. Tolerances follow from one or two
-float roundings (2^-24 = 6e-8 relative)."""
+Hessians themselves. Energies and gradients stay FP64. Where the element kernel
+differentiates through the linear cut of the energy (the deformation gradient of a Tet4
+element), the material law's second derivatives and the coefficients dF/dx are computed in
+FP64 and rounded once, and the contraction H = C^T A C runs in FP32 (nine products per
+entry); otherwise the FP64 values are rounded once. Projection reads the floats, works in
+FP64 and rounds its result once; assembly sums them in FP64. Tolerances follow from a few
+float roundings (2^-24 = 6e-8 relative) of the largest terms."""
 import numpy as np
 import pytest
 
@@ -43,11 +45,19 @@ def test_f32_element_hessians_are_the_fp64_ones_rounded(gpu, oracle):
     # energies and gradients stay FP64
     assert E32 == E64 and np.array_equal(g32, g64)
     assert np.array_equal(H32, H32.astype(np.float32).astype(np.float64))  # stored as floats
-    r = H64.astype(np.float32).astype(np.float64)
-    assert np.all(np.abs(H32 - r) <= F32_ULP * np.abs(r))
+    # a few float roundings of the largest term (measured 1.3e-7 of max|H|)
+    scale64 = np.abs(H64).max(axis=1)
+    err = (np.abs(H32 - H64).max(axis=1) / scale64).max()
+    assert err <= 1e-6, err
     He = oracle.element_hessians(S)
     scale = np.abs(He).max(axis=(1, 2))
-    assert (np.abs(unpack_ub(H32, 4) - He).max(axis=(1, 2)) / scale).max() <= 1e-7
+    err_o = (np.abs(unpack_ub(H32, 4) - He).max(axis=(1, 2)) / scale).max()
+    assert err_o <= 1e-6, err_o
+    # translation invariance survives the FP32 contraction at the storage's resolution: the
+    # block-row sums stay below the projection's threshold for the translation shortcut
+    # (1e-6 max|H|; measured 1.2e-7)
+    rows = np.abs(unpack_ub(H32, 4).reshape(-1, 12, 4, 3).sum(axis=2)).max(axis=(1, 2)) / scale
+    assert rows.max() <= 1e-6, rows.max()
 
 
 @pytest.mark.parametrize("noise,seed", [(0.3 / 3, 1), (0.0, 1), (0.12, 2)])
