@@ -1726,6 +1726,133 @@ namespace symx_hip
         if (threadIdx.x == 0) pb[w] = s;
     }
 
+    // The same SpMV for small, cache-resident systems, whose workgroups own fewer rows than
+    // threads (a 49k-DoF system gives 64 rows to each of 256 workgroups of 512 threads): LPR
+    // consecutive lanes share a row, lane g summing the row's blocks j = g, g + LPR, ..., so
+    // a row's chain of dependent loads (column id, then the gathered vector) is LPR times
+    // shorter; the lanes' partial products are then summed across the LPR lanes in a fixed
+    // order (xor butterfly), and the row's first lane stores the result. The host takes this
+    // form when every row of a workgroup's chunk gets its LPR lanes in one pass.
+    template <typename TS, typename TV, typename CT, int NTS, int LPR, bool PRE>
+    __global__ __launch_bounds__(NTS) void k_pcg_spmv_split(int32_t n_rows, int32_t k, const int64_t* __restrict__ slice_ptr,
+                                                           const int32_t* __restrict__ row_at, const CT* __restrict__ col,
+                                                           const TS* __restrict__ val, const TV* __restrict__ z,
+                                                           const TV* __restrict__ p_old, TV* __restrict__ p_new, TV* __restrict__ q,
+                                                           PcgState* st, const double* __restrict__ pa_prev,
+                                                           const double* __restrict__ pa_prev2, double* __restrict__ pb)
+    {
+        static_assert(LPR >= 2 && LPR <= 64 && (LPR & (LPR - 1)) == 0, "lanes per row: a power of two");
+        // blocks of each lane loaded before the prologue (its partial sums and the state are
+        // a round trip of their own; nothing loaded here depends on them)
+        constexpr int NPRE = 2;
+        __shared__ double lds[3 * NTS / 64];
+        const int64_t G = gridDim.x;
+        const int64_t w = xcd_remap();
+        int64_t r0, r1;
+        row_chunk(n_rows, w, G, r0, r1);
+        const int sub = threadIdx.x & (LPR - 1);
+        const int64_t pos = r0 + threadIdx.x / LPR;
+        const bool live = pos < r1;  // every lane of a row agrees
+        int64_t row = 0, b0 = 0;
+        int wd = 0, lane = 0;
+        TS v[NPRE][9];
+        TV xa[NPRE][3], xb[NPRE][3];
+        if (live) {
+            row = row_at ? row_at[pos] : pos;
+            b0 = slice_ptr[pos >> 6];
+            wd = (int)((slice_ptr[(pos >> 6) + 1] - b0) >> 6);
+            lane = (int)(pos & 63);
+#pragma unroll
+            for (int u = 0; u < NPRE; u++) {
+                const int j = sub + u * LPR;
+                if (j < wd) {
+                    const int64_t c = 3 * block_col(col[b0 + lane + (int64_t)j * 64], row);
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        if constexpr (PRE) xa[u][i] = p_new[c + i];
+                        else {
+                            xa[u][i] = z[c + i];
+                            xb[u][i] = p_old[c + i];
+                        }
+                    }
+                    load_block<TS, false>(val + b0 * 9 + (int64_t)j * 576, lane, v[u]);
+                }
+            }
+        }
+        if (st->done) return;
+        double beta = 0.0;
+        if constexpr (!PRE) {
+            if (pcg_prologue<NTS>(k, st, pa_prev, pa_prev2, lds, beta)) return;
+        }
+        const TV tb = (TV)beta;
+        double dot = 0.0;
+        TV a0 = (TV)0, a1 = (TV)0, a2 = (TV)0;
+        auto add_block = [&](const TS (&b)[9], TV x0, TV x1, TV x2) {
+            a0 += (TV)b[0] * x0 + (TV)b[1] * x1 + (TV)b[2] * x2;
+            a1 += (TV)b[3] * x0 + (TV)b[4] * x1 + (TV)b[5] * x2;
+            a2 += (TV)b[6] * x0 + (TV)b[7] * x1 + (TV)b[8] * x2;
+        };
+        if (live) {
+#pragma unroll
+            for (int u = 0; u < NPRE; u++) {
+                if (sub + u * LPR < wd) {
+                    if constexpr (PRE) add_block(v[u], xa[u][0], xa[u][1], xa[u][2]);
+                    else add_block(v[u], xa[u][0] + tb * xb[u][0], xa[u][1] + tb * xb[u][1], xa[u][2] + tb * xb[u][2]);
+                }
+            }
+            const CT* cp = col + b0 + lane;
+            const TS* vp = val + b0 * 9;
+#pragma unroll 2
+            for (int j = sub + NPRE * LPR; j < wd; j += LPR) {
+                const int64_t c = 3 * block_col(cp[j * 64], row);
+                TV x0, x1, x2;
+                if constexpr (PRE) {
+                    x0 = p_new[c + 0];
+                    x1 = p_new[c + 1];
+                    x2 = p_new[c + 2];
+                }
+                else {
+                    x0 = z[c + 0] + tb * p_old[c + 0];
+                    x1 = z[c + 1] + tb * p_old[c + 1];
+                    x2 = z[c + 2] + tb * p_old[c + 2];
+                }
+                TS b[9];
+                load_block<TS, false>(vp + (int64_t)j * 576, lane, b);
+                add_block(b, x0, x1, x2);
+            }
+        }
+        // every lane takes part in the butterfly (rows of a wave are whole: LPR divides 64)
+#pragma unroll
+        for (int m = LPR / 2; m >= 1; m >>= 1) {
+            a0 += __shfl_xor(a0, m, 64);
+            a1 += __shfl_xor(a1, m, 64);
+            a2 += __shfl_xor(a2, m, 64);
+        }
+        if (live && sub == 0) {
+            const int64_t kk = 3 * row;
+            TV p0, p1, p2;
+            if constexpr (PRE) {
+                p0 = p_new[kk + 0];
+                p1 = p_new[kk + 1];
+                p2 = p_new[kk + 2];
+            }
+            else {
+                p0 = z[kk + 0] + tb * p_old[kk + 0];
+                p1 = z[kk + 1] + tb * p_old[kk + 1];
+                p2 = z[kk + 2] + tb * p_old[kk + 2];
+                p_new[kk + 0] = p0;
+                p_new[kk + 1] = p1;
+                p_new[kk + 2] = p2;
+            }
+            q[kk + 0] = a0;
+            q[kk + 1] = a1;
+            q[kk + 2] = a2;
+            dot = (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
+        }
+        const double s = block_sum<NTS>(dot, lds);
+        if (threadIdx.x == 0) pb[w] = s;
+    }
+
     // Rows loaded by each thread of k_pcg_update before its partial sums (two rows per
     // thread cover a 1024-row chunk of a 512-thread workgroup).
     constexpr int kUpdateAhead = 2;
@@ -1736,7 +1863,6 @@ namespace symx_hip
                                                        double* __restrict__ pa_next, const double* __restrict__ pb)
     {
         __shared__ double lds[2 * NTS / 64];
-        if (st->done) return;
         const int64_t G = gridDim.x;
         const int64_t w = xcd_remap();
         int64_t r0, r1;
@@ -1756,6 +1882,9 @@ namespace symx_hip
                 load_dinv(dinv, n_rows, row, da[u]);
             }
         }
+        // the state is read after the rows are requested (one round trip less; the loads
+        // read valid memory whether or not the solve is done)
+        if (st->done) return;
         double sums[2];
         sum_partials_k<NTS, 2>({ pb, pa_cur + 1 }, { 1, 2 }, G, lds, sums);
         const double pAp = sums[0];
@@ -1850,6 +1979,37 @@ namespace symx_hip
             k_pcg_spmv<TS, TV, STREAM, CT, kSpmvThreads, false><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, col, val, z,
                                                                                           p_old, p_new, q, st, pa_prev, pa_prev2, pb);
     }
+    // Lanes per row of the split SpMV for a cache-resident system: the rows of a workgroup's
+    // chunk (row_chunk) times LPR must fit its threads in one pass; 1: the one-lane form.
+    inline int spmv_lanes_per_row(int32_t n_rows, int G)
+    {
+        const int64_t per = ((((int64_t)n_rows + G - 1) / G) + 63) / 64 * 64;
+        int lpr = 1;
+        while (lpr < 8 && per * (2 * lpr) <= kSpmvThreads) lpr *= 2;
+        return lpr;
+    }
+    template <typename TS, typename TV, typename CT, int LPR>
+    void launch_spmv_split(hipStream_t s, const SellPattern& P, const CT* col, bool pre, int G, int32_t k, const TS* val, const TV* z,
+                           const TV* p_old, TV* p_new, TV* q, PcgState* st, const double* pa_prev, const double* pa_prev2, double* pb)
+    {
+        if (pre)
+            k_pcg_spmv_split<TS, TV, CT, kSpmvThreads, LPR, true><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, col, val, z,
+                                                                                            p_old, p_new, q, st, pa_prev, pa_prev2, pb);
+        else
+            k_pcg_spmv_split<TS, TV, CT, kSpmvThreads, LPR, false><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, P.d_slice_ptr, P.d_row_at, col, val, z,
+                                                                                             p_old, p_new, q, st, pa_prev, pa_prev2, pb);
+    }
+    template <typename TS, typename TV, typename CT>
+    void launch_spmv_small(hipStream_t s, const SellPattern& P, const CT* col, bool pre, int G, int lpr, int32_t k, const TS* val,
+                           const TV* z, const TV* p_old, TV* p_new, TV* q, PcgState* st, const double* pa_prev, const double* pa_prev2,
+                           double* pb)
+    {
+        switch (lpr) {
+        case 2: launch_spmv_split<TS, TV, CT, 2>(s, P, col, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb); break;
+        case 4: launch_spmv_split<TS, TV, CT, 4>(s, P, col, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb); break;
+        default: launch_spmv_split<TS, TV, CT, 8>(s, P, col, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb); break;
+        }
+    }
     template <typename TS, typename TV>
     void launch_spmv(hipStream_t s, const SellPattern& P, int G, int32_t k, const TS* val, const TV* z, const TV* p_old, TV* p_new,
                      TV* q, PcgState* st, const double* pa_prev, const double* pa_prev2, double* pb)
@@ -1865,7 +2025,12 @@ namespace symx_hip
             else
                 k_pcg_direction<TV, kSpmvThreads, false><<<G, kSpmvThreads, 0, s>>>(P.n_rows, k, z, p_old, p_new, st, pa_prev, pa_prev2);
         }
-        if (P.d_dcol) {
+        const int lpr = stream ? 1 : spmv_lanes_per_row(P.n_rows, G);
+        if (lpr > 1) {
+            if (P.d_dcol) launch_spmv_small<TS, TV>(s, P, P.d_dcol, pre, G, lpr, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
+            else launch_spmv_small<TS, TV>(s, P, P.d_col, pre, G, lpr, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
+        }
+        else if (P.d_dcol) {
             if (stream) launch_spmv_c<TS, TV, true>(s, P, P.d_dcol, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
             else launch_spmv_c<TS, TV, false>(s, P, P.d_dcol, pre, G, k, val, z, p_old, p_new, q, st, pa_prev, pa_prev2, pb);
         }
