@@ -63,8 +63,9 @@ enum {
  * elements (fixed order); G is [n_elem][n_dof]; H holds, per element, the
  * upper 3x3 blocks (a <= b, row-major order of (a,b)) with 9 row-major values
  * each, as doubles, or as floats for the kernels emitted for FP32 Hessian
- * storage (name suffix "_f32"). Inactive elements (active[e] == 0) write zeros
- * and have energy 0.
+ * storage (name suffix "_f32"), in records of h_stride values (the potential's
+ * symx_hip_potential_desc; pad values are written as zeros). Inactive elements
+ * (active[e] == 0) write zeros and have energy 0.
  * e_mode selects what happens to the element energies E_e:
  *   SYMX_HIP_EMODE_SUM    E_partial gets the sum of E_e;
  *   SYMX_HIP_EMODE_STORE  the same, and E_elem[e] = E_e for every e < n_elem;
@@ -152,6 +153,11 @@ typedef struct symx_hip_potential_desc {
      * 0 or 256 for one element per thread (generated element kernels), 4 for one element
      * per wavefront (structured FEM kernels). The C kernel always runs one per thread. */
     int32_t elems_per_group;
+    /* Values per element Hessian record on the device: 0 for n_ub * 9 (upper 3x3 blocks,
+     * contiguous). Larger (a multiple of 3) when the PGH kernels pad each record, e.g. 96
+     * for a 4-node element's 90 values, so that every 128-byte chunk the kernel stores is a
+     * whole aligned line; the pad stays out of projection and assembly. */
+    int32_t h_stride;
 } symx_hip_potential_desc;
 
 int symx_hip_problem_add_potential(symx_hip_problem prob, const symx_hip_potential_desc* desc, int32_t* pot_id);
@@ -195,7 +201,7 @@ int symx_hip_problem_gradient_dot(symx_hip_problem prob, int32_t vec, double* ou
 /* Number of element Hessians produced by the last PGH evaluation. */
 int symx_hip_problem_n_element_hessians(symx_hip_problem prob, int64_t* n);
 /* Copy element outputs of potential pot_id to host (tests): G [n_elem][n_dof],
- * H [n_elem][n_ub*9]; either pointer may be NULL. */
+ * H [n_elem][n_ub*9] (without the records' pad); either pointer may be NULL. */
 int symx_hip_problem_element_outputs(symx_hip_problem prob, int32_t pot_id, double* G, double* H);
 
 /* Project every element Hessian to PD (project_to_PD.cpp:13-82 via

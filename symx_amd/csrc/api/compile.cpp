@@ -298,7 +298,7 @@ namespace symx
         void emit_chunked_kernel(std::ostringstream& os, const Graph& g, const std::vector<DataMap>& maps, const SummationMap& sum,
                                  const KernelSpec& ks, bool h32)
         {
-            const int n = ks.n_dofs, nb = n / 3, n_ub = nb * (nb + 1) / 2, nh = n_ub * 9;
+            const int n = ks.n_dofs, nb = n / 3, n_ub = nb * (nb + 1) / 2, nh = n_ub * 9, hs = hessian_record_stride(nh);
             const bool pgh = ks.kind == KernelKind::PGH;
             std::vector<int32_t> roots;
             for (const Scalar& s : ks.outputs) roots.push_back(s.id);
@@ -323,8 +323,9 @@ namespace symx
                         for (int i = 0; i < 3; i++)
                             for (int j = 0; j < 3; j++) hv.push_back(ks.outputs[1 + n + (3 * a + i) * n + 3 * b + j].id);
                 const int cw = h32 ? 32 : 16;
-                for (int c0 = 0; c0 < nh; c0 += cw) {
-                    Chunk c{ true, c0, std::min(cw, nh - c0), {} };
+                hv.resize(hs, g.zero());  // the record's pad (zeros)
+                for (int c0 = 0; c0 < hs; c0 += cw) {
+                    Chunk c{ true, c0, std::min(cw, hs - c0), {} };
                     c.node.assign(hv.begin() + c0, hv.begin() + c0 + c.cw);
                     chunks.push_back(c);
                 }
@@ -370,7 +371,7 @@ namespace symx
             const std::string hdst = h32 ? "(float*)p.H" : "p.H";
             os << "    if (!__syncthreads_or(act)) {\n";
             os << "        symx_zero_rows<" << n << ">(p.G, e0, nv);\n";
-            if (pgh) os << "        symx_zero_rows<" << nh << ">(" << hdst << ", e0, nv);\n";
+            if (pgh) os << "        symx_zero_rows<" << hs << ">(" << hdst << ", e0, nv);\n";
             os << "        symx_store_energy(0.0, p, e);\n        return;\n    }\n";
             os << "    double* const st = symx_stage_wave();\n";
             emit_gather(os, g, maps, use, sum, {}, {}, "ec");
@@ -431,7 +432,7 @@ namespace symx
                     os << "    " << put << "(st, " << k << ", " << (in_f32[id] ? "f" + std::to_string(id) : ref(g, id)) << ");\n";
                 }
                 if (ci == 0) os << "    const double e_out = act ? " << ref(g, ks.outputs[0].id) << " : 0.0;\n";
-                if (c.hess) os << "    symx_flush_rows<" << nh << ", " << c.c0 << ", " << c.cw << ">(" << hdst << ", st, e0, nv, act);\n";
+                if (c.hess) os << "    symx_flush_rows<" << hs << ", " << c.c0 << ", " << c.cw << ">(" << hdst << ", st, e0, nv, act);\n";
                 else os << "    symx_flush_rows<" << n << ", " << c.c0 << ", " << c.cw << ">(p.G, st, e0, nv, act);\n";
             }
             os << "    symx_store_energy(e_out, p, e);\n}\n";
@@ -777,6 +778,8 @@ namespace symx
         os << "extern \"C\" void get_sha256(char* out) { std::memcpy(out, \"" << sha256_hex << "\", 64); }\n";
         return os.str();
     }
+
+    int32_t hessian_record_stride(int32_t n_values) { return n_values > 16 ? (n_values + 31) / 32 * 32 : n_values; }
 
     int64_t count_operations(Graph& g, const std::vector<Scalar>& outputs)
     {

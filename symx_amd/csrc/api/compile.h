@@ -314,6 +314,12 @@ namespace symx
     std::string emit_host_function(const std::string& name, Graph& g, const std::vector<Scalar>& outputs, int32_t n_inputs,
                                    const std::string& sha256_hex);
 
+    // Values per element Hessian record written by the single-point (chunked) PGH kernels
+    // for a Hessian of n_values upper-block values: records longer than one 128-byte chunk
+    // of doubles are padded to a multiple of 32 values, so that every chunk the kernels
+    // store (32 floats or 16 doubles) is one aligned 128-byte line.
+    int32_t hessian_record_stride(int32_t n_values);
+
     // Number of SSA operations that a kernel with these outputs executes (for reports).
     int64_t count_operations(Graph& g, const std::vector<Scalar>& outputs);
 }

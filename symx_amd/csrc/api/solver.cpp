@@ -653,6 +653,8 @@ namespace symx
             info.ops_adjoint = count_operations(*g, ks.outputs);
             specs.push_back(ks);
         }
+        // single-point PG/PGH kernels store their outputs chunk by chunk (emit_chunked_kernel)
+        if (mws.get_summation().n_iterations <= 1) pm.h_stride = hessian_record_stride((n / 3) * (n / 3 + 1) / 2 * 9);
         pm.source = emit_hip_module(base, *g, maps, mws.get_summation(), specs, symx_hip_eval_prelude());
         info.ops_P = count_operations(*g, specs[0].outputs);
         info.ops_PG = count_operations(*g, specs[1].outputs);
@@ -732,6 +734,7 @@ namespace symx
             d.n_maps = (int32_t)mdesc.size();
             d.maps = mdesc.data();
             d.elems_per_group = pm.elems_per_group;
+            d.h_stride = pm.h_stride;
             int32_t pid = -1;
             check(symx_hip_problem_add_potential(prob, &d, &pid), hip, ("adding potential " + P.name).c_str());
             infos.push_back(info);

@@ -367,6 +367,7 @@ namespace symx
         std::vector<int32_t> map_dof_set;    // DoF set per data map (-1: not a DoF)
         bool structured = false;             // structured FEM kernels (see FemStructure)
         bool factorised = false;             // derivatives through the linear cut (chain_rule_derivatives)
+        int32_t h_stride = 0;                // values per element Hessian record (0: n_ub * 9)
         FemStructure fem;
         int32_t elems_per_group = 256;
     };

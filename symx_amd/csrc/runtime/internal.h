@@ -130,7 +130,8 @@ namespace symx_hip
         std::vector<DevMap> maps;
         uint8_t* d_active = nullptr;
         int64_t g_base = 0;                 // first gradient block (3 doubles)
-        int64_t h_base = 0;                 // first Hessian block (9 doubles)
+        int32_t h_stride = 0;               // values per element Hessian record (>= n_ub * 9)
+        int64_t h_off = 0;                  // first value of the potential's Hessian records
         int64_t part_base = 0;              // first energy partial
         int64_t e_base = 0;                 // first element in problem-wide element numbering
         int32_t n_groups = 0;               // workgroups of the element kernel
@@ -187,7 +188,7 @@ struct symx_hip_problem_s
     uint32_t* d_mirror = nullptr; // [n_assemble] slot that receives the transposed block, ~0u for none; null: no mirroring
     int64_t n_assemble = 0;
     uint32_t* d_coff = nullptr;   // [n_slots + 1] contribution lists in that order
-    uint32_t* d_clist = nullptr;  // Hessian block id * 2 + transpose flag
+    uint32_t* d_clist = nullptr;  // (offset of the Hessian block's first value / 3) * 2 + transpose flag
     double* d_val64 = nullptr;
     float* d_val32 = nullptr;
     double* d_dinv64 = nullptr;
@@ -229,9 +230,9 @@ namespace symx_hip
                                double eta, unsigned long long* out);
     // select (may be NULL): only elements with select[e] != 0 are projected.
     // flags, list: per-element scratch (n_mats entries); n_list: one device counter.
-    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, int64_t rs, double eps, int32_t mirror, uint8_t* flags,
                           uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count);
-    int launch_project_ub(hipStream_t s, float* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+    int launch_project_ub(hipStream_t s, float* H, int64_t n_mats, int32_t nb, int64_t rs, double eps, int32_t mirror, uint8_t* flags,
                           uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count);
     int launch_project_dense(hipStream_t s, double* mats, int64_t n_mats, int32_t n, double eps, int32_t mirror, uint8_t* changed);
     // n entries of aslot/coff; mirror (may be NULL): slot that also receives the transposed block

@@ -512,7 +512,7 @@ namespace symx_hip
     // record per instruction, instead of 64 cache lines when each lane reads its own record).
     // Loads cover the records below nv (nv >= 1); stores only those with use[rec] != 0.
     template <int W, int SS, typename T>
-    __device__ __forceinline__ void r4_stage_in(const T* __restrict__ base, int nv, double* stage, int lane)
+    __device__ __forceinline__ void r4_stage_in(const T* __restrict__ base, int64_t rs, int nv, double* stage, int lane)
     {
         T v[W];
         if constexpr (sizeof(T) == 4) {
@@ -523,14 +523,14 @@ namespace symx_hip
 #pragma unroll
             for (int s = 0; s < W; s++) {
                 const int k = lane + 64 * s, rec = k / W, q = k - rec * W;
-                v[s] = base[(int64_t)(rec < nv ? rec : nv - 1) * 90 + q];
+                v[s] = base[(int64_t)(rec < nv ? rec : nv - 1) * rs + q];
             }
         }
         else {
 #pragma unroll
             for (int s = 0; s < W; s++) {
                 const int k = lane + 64 * s, rec = k / W, q = k - rec * W;
-                v[s] = rec < nv ? base[(int64_t)rec * 90 + q] : T(0);
+                v[s] = rec < nv ? base[(int64_t)rec * rs + q] : T(0);
             }
         }
 #pragma unroll
@@ -540,12 +540,12 @@ namespace symx_hip
         }
     }
     template <int W, int SS, typename T>
-    __device__ __forceinline__ void r4_stage_out(T* __restrict__ base, const int* use, const double* stage, int lane)
+    __device__ __forceinline__ void r4_stage_out(T* __restrict__ base, int64_t rs, const int* use, const double* stage, int lane)
     {
 #pragma unroll 4
         for (int k = lane; k < 64 * W; k += 64) {
             const int rec = k / W, q = k - rec * W;
-            if (use[rec]) base[(int64_t)rec * 90 + q] = (T)stage[rec * SS + q];
+            if (use[rec]) base[(int64_t)rec * rs + q] = (T)stage[rec * SS + q];
         }
     }
 
@@ -562,7 +562,7 @@ namespace symx_hip
     // written through LDS 16 values at a time with coalesced transfers: M, the block-row
     // sums and the scale accumulate chunk by chunk, and the update is added chunk by chunk.
     template <typename TH>
-    __global__ __launch_bounds__(64, 2) void k_project_ub4_reduced(TH* __restrict__ H, int64_t n_mats, double eps, int mirror,
+    __global__ __launch_bounds__(64, 2) void k_project_ub4_reduced(TH* __restrict__ H, int64_t n_mats, int64_t rec_stride, double eps, int mirror,
                                                                unsigned long long* count, uint8_t* __restrict__ needs_full,
                                                                const uint8_t* __restrict__ select)
     {
@@ -576,7 +576,7 @@ namespace symx_hip
         const int64_t e0 = (int64_t)blockIdx.x * 64, e = e0 + lane;
         const bool active = e < n_mats && (select == nullptr || select[e]);
         const int nv = n_mats - e0 < 64 ? (int)(n_mats - e0) : 64;  // records in this wave
-        TH* const base = H + e0 * 90;
+        TH* const base = H + e0 * rec_stride;
         s_use[lane] = active ? 1 : 0;
         __syncthreads();
         bool ok = false;
@@ -587,7 +587,7 @@ namespace symx_hip
             static_for<0, 36>([&](auto Kc) { rs[Kc()] = 0.0; });
             static_for<0, NCH>([&](auto Cc) {
                 constexpr int c0 = CW * decltype(Cc)::value, W = 90 - c0 < CW ? 90 - c0 : CW;
-                r4_stage_in<W, SS>(base + c0, nv, s_lds, lane);
+                r4_stage_in<W, SS>(base + c0, rec_stride, nv, s_lds, lane);
                 __syncthreads();
                 static_for<0, W>([&](auto Kc) { r4_take_value<c0 + decltype(Kc)::value>(s_lds[lane * SS + Kc()], m, rs, scale); });
                 __syncthreads();
@@ -622,7 +622,7 @@ namespace symx_hip
             });
             static_for<0, NCH>([&](auto Cc) {
                 constexpr int c0 = CW * decltype(Cc)::value, W = 90 - c0 < CW ? 90 - c0 : CW;
-                r4_stage_in<W, SS>(base + c0, nv, s_lds, lane);
+                r4_stage_in<W, SS>(base + c0, rec_stride, nv, s_lds, lane);
                 __syncthreads();
                 if (ok) {
                     static_for<0, W>([&](auto Kc) {
@@ -637,7 +637,7 @@ namespace symx_hip
                     });
                 }
                 __syncthreads();
-                r4_stage_out<W, SS>(base + c0, s_use, s_lds, lane);
+                r4_stage_out<W, SS>(base + c0, rec_stride, s_use, s_lds, lane);
                 __syncthreads();
             });
         }
@@ -645,13 +645,13 @@ namespace symx_hip
     }
 
     template <int NB, typename T>
-    __global__ __launch_bounds__(64) void k_project_ub_full(T* __restrict__ H, int64_t n_mats, double eps, int mirror,
+    __global__ __launch_bounds__(64) void k_project_ub_full(T* __restrict__ H, int64_t n_mats, int64_t rs, double eps, int mirror,
                                                            unsigned long long* count, const uint8_t* __restrict__ only)
     {
         constexpr int NUB = NB * (NB + 1) / 2;
         const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
         bool changed = false;
-        if (e < n_mats && (only == nullptr || only[e])) changed = project_element_full<NB>(H + e * (NUB * 9), eps, mirror != 0);
+        if (e < n_mats && (only == nullptr || only[e])) changed = project_element_full<NB>(H + e * rs, eps, mirror != 0);
         count_changed(changed, count);
     }
 
@@ -1115,32 +1115,29 @@ namespace symx_hip
     // too_many (nullable): 1 for each matrix left to the full-size variant (KMAX < N), 0 for
     // every other one, selected or not.
     template <int N, bool DENSE, typename T, int KMAX = N, int PL = PLU>
-    __global__ __launch_bounds__(PNT) void k_project_large(T* __restrict__ H, int64_t n_mats, double eps, int mirror,
+    __global__ __launch_bounds__(PNT) void k_project_large(T* __restrict__ H, int64_t n_mats, int64_t rs, double eps, int mirror,
                                                           unsigned long long* count, const uint8_t* __restrict__ select,
                                                           uint8_t* __restrict__ changed_out, uint8_t* __restrict__ too_many)
     {
-        constexpr int NUB = (N / 3) * (N / 3 + 1) / 2;
         const int64_t e = blockIdx.x;
         if (e >= n_mats) return;
         bool left = false;
         if (select == nullptr || select[e])
-            left = project_large_one<N, DENSE, PNT, T, KMAX, PL>(DENSE ? H + e * (int64_t)(N * N) : H + e * (int64_t)(NUB * 9), e, eps, mirror,
-                                                                 count, changed_out);
+            left = project_large_one<N, DENSE, PNT, T, KMAX, PL>(H + e * rs, e, eps, mirror, count, changed_out);
         if (too_many != nullptr && threadIdx.x == 0) too_many[e] = left ? 1 : 0;
     }
 
     // The elements listed in list[0 .. *n_list) (upper-block storage), a fixed grid of
     // PT-thread workgroups striding over the list (one wave per 12x12 matrix).
     template <int N, int PT, typename T>
-    __global__ __launch_bounds__(PT) void k_project_large_list(T* __restrict__ H, const uint32_t* __restrict__ list,
+    __global__ __launch_bounds__(PT) void k_project_large_list(T* __restrict__ H, int64_t rs, const uint32_t* __restrict__ list,
                                                               const unsigned int* __restrict__ n_list, double eps, int mirror,
                                                               unsigned long long* count)
     {
-        constexpr int NUB = (N / 3) * (N / 3 + 1) / 2;
         const unsigned int n = *n_list;
         for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
             const int64_t e = list[i];
-            project_large_one<N, false, PT, T>(H + e * (int64_t)(NUB * 9), e, eps, mirror, count, nullptr);
+            project_large_one<N, false, PT, T>(H + e * rs, e, eps, mirror, count, nullptr);
             __syncthreads();  // LDS is reused by the next matrix
         }
     }
@@ -1156,49 +1153,50 @@ namespace symx_hip
     __global__ void k_zero_counter(unsigned int* c) { *c = 0u; }
 
     template <typename T>
-    int launch_project_ub_t(hipStream_t s, T* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+    int launch_project_ub_t(hipStream_t s, T* H, int64_t n_mats, int32_t nb, int64_t rs, double eps, int32_t mirror, uint8_t* flags,
                             uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count)
     {
         if (n_mats <= 0) return 0;
+        if (rs < (int64_t)nb * (nb + 1) / 2 * 9) return -1;
         const unsigned g = (unsigned)((n_mats + 63) / 64);
         switch (nb) {
-        case 1: k_project_ub_full<1, T><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
-        case 2: k_project_ub_full<2, T><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
-        case 3: k_project_ub_full<3, T><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, select); return 0;
+        case 1: k_project_ub_full<1, T><<<g, 64, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select); return 0;
+        case 2: k_project_ub_full<2, T><<<g, 64, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select); return 0;
+        case 3: k_project_ub_full<3, T><<<g, 64, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select); return 0;
         case 4:
             // reduced 9x9 path in registers; the few elements it hands back (more than three
             // eigenvalues below eps, or not translation invariant) are listed and projected by
             // the workgroup-per-matrix kernel
-            k_project_ub4_reduced<T><<<g, 64, 0, s>>>(H, n_mats, eps, mirror, d_count, flags, select);
+            k_project_ub4_reduced<T><<<g, 64, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, flags, select);
             k_zero_counter<<<1, 1, 0, s>>>(n_list);
             k_list_flagged<<<grid_for(n_mats), NT, 0, s>>>(flags, n_mats, list, n_list);
-            k_project_large_list<12, 64, T><<<4096, 64, 0, s>>>(H, list, n_list, eps, mirror, d_count);
+            k_project_large_list<12, 64, T><<<4096, 64, 0, s>>>(H, rs, list, n_list, eps, mirror, d_count);
             return 0;
-        case 8: k_project_large<24, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr, nullptr); return 0;
-        case 10: k_project_large<30, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr, nullptr); return 0;
+        case 8: k_project_large<24, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select, nullptr, nullptr); return 0;
+        case 10: k_project_large<30, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select, nullptr, nullptr); return 0;
         case 27:
             // 81x81: eigenvectors for at most 8 eigenvalues below eps and four inverse-iteration
             // lanes keep the LDS at 73 KB, two matrices per CU (the full-size variant takes
             // 143 KB, one per CU); Hex27 elements have at most 3 such eigenvalues as a rule, the
             // others are listed and projected by the full-size variant
-            k_project_large<81, false, T, 8, 4><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, eps, mirror, d_count, select, nullptr, flags);
+            k_project_large<81, false, T, 8, 4><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select, nullptr, flags);
             k_zero_counter<<<1, 1, 0, s>>>(n_list);
             k_list_flagged<<<grid_for(n_mats), NT, 0, s>>>(flags, n_mats, list, n_list);
-            k_project_large_list<81, PNT, T><<<1024, PNT, 0, s>>>(H, list, n_list, eps, mirror, d_count);
+            k_project_large_list<81, PNT, T><<<1024, PNT, 0, s>>>(H, rs, list, n_list, eps, mirror, d_count);
             return 0;
         default: return -1;
         }
     }
 
-    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+    int launch_project_ub(hipStream_t s, double* H, int64_t n_mats, int32_t nb, int64_t rs, double eps, int32_t mirror, uint8_t* flags,
                           uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count)
     {
-        return launch_project_ub_t(s, H, n_mats, nb, eps, mirror, flags, list, n_list, select, d_count);
+        return launch_project_ub_t(s, H, n_mats, nb, rs, eps, mirror, flags, list, n_list, select, d_count);
     }
-    int launch_project_ub(hipStream_t s, float* H, int64_t n_mats, int32_t nb, double eps, int32_t mirror, uint8_t* flags,
+    int launch_project_ub(hipStream_t s, float* H, int64_t n_mats, int32_t nb, int64_t rs, double eps, int32_t mirror, uint8_t* flags,
                           uint32_t* list, unsigned int* n_list, const uint8_t* select, unsigned long long* d_count)
     {
-        return launch_project_ub_t(s, H, n_mats, nb, eps, mirror, flags, list, n_list, select, d_count);
+        return launch_project_ub_t(s, H, n_mats, nb, rs, eps, mirror, flags, list, n_list, select, d_count);
     }
 
     // ---- progressive projection (NewtonsMethod.cpp:306-330, ElementHessians.cpp:79-165) ----
@@ -1328,9 +1326,9 @@ namespace symx_hip
         case 6: k_project_dense<6><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
         case 9: k_project_dense<9><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
         case 12: k_project_dense<12><<<g, 64, 0, s>>>(mats, n_mats, eps, mirror, changed); return 0;
-        case 24: k_project_large<24, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed, nullptr); return 0;
-        case 30: k_project_large<30, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed, nullptr); return 0;
-        case 81: k_project_large<81, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, eps, mirror, nullptr, nullptr, changed, nullptr); return 0;
+        case 24: k_project_large<24, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, 24 * 24, eps, mirror, nullptr, nullptr, changed, nullptr); return 0;
+        case 30: k_project_large<30, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, 30 * 30, eps, mirror, nullptr, nullptr, changed, nullptr); return 0;
+        case 81: k_project_large<81, true, double><<<(unsigned)n_mats, PNT, 0, s>>>(mats, n_mats, 81 * 81, eps, mirror, nullptr, nullptr, changed, nullptr); return 0;
         default: return -1;
         }
     }
@@ -1365,13 +1363,13 @@ namespace symx_hip
 #pragma unroll
             for (int u = 0; u < kSlotAhead; u++) code[u] = clist[k + u];
 #pragma unroll
-            for (int u = 0; u < kSlotAhead; u++) v[u] = H[(int64_t)(code[u] >> 1) * 9 + ((code[u] & 1u) ? qt : q)];
+            for (int u = 0; u < kSlotAhead; u++) v[u] = H[(int64_t)(code[u] >> 1) * 3 + ((code[u] & 1u) ? qt : q)];
 #pragma unroll
             for (int u = 0; u < kSlotAhead; u++) acc += (double)v[u];
         }
         for (; k < k1; k++) {
             const uint32_t code = clist[k];
-            acc += (double)H[(int64_t)(code >> 1) * 9 + ((code & 1u) ? qt : q)];
+            acc += (double)H[(int64_t)(code >> 1) * 3 + ((code & 1u) ? qt : q)];
         }
         val[sell_val_index<TS>(slot, q)] = (TS)acc;
         // the transposed slot sums the same element blocks transposed, in the same order

@@ -647,6 +647,9 @@ int symx_hip_problem_add_potential(symx_hip_problem p, const symx_hip_potential_
     if (pot.elems_per_group != SYMX_HIP_BLOCK && d->kernel_C) return fail(ctx, SYMX_HIP_ERR_ARG, "conditions need one element per thread");
     pot.nb = d->n_dof_blocks;
     pot.n_ub = pot.nb * (pot.nb + 1) / 2;
+    pot.h_stride = d->h_stride == 0 ? pot.n_ub * 9 : d->h_stride;
+    if (pot.h_stride < pot.n_ub * 9 || pot.h_stride % 3 != 0)
+        return fail(ctx, SYMX_HIP_ERR_ARG, "h_stride must be a multiple of 3 and at least 9 values per upper block");
     pot.dof_col.assign(d->dof_conn_col, d->dof_conn_col + pot.nb);
     pot.dof_off.assign(d->dof_set_offset, d->dof_set_offset + pot.nb);
     const int64_t ne = d->n_elem;
@@ -747,21 +750,21 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     if (p->pots.empty()) return fail(ctx, SYMX_HIP_ERR_ARG, "problem without potentials");
     const int32_t nb = p->nb;
 
-    int64_t g = 0, h = 0, parts = 0, elems = 0;
+    int64_t g = 0, h = 0, parts = 0, elems = 0;  // h: values of all Hessian records
     uint64_t n_contrib = 0;
     for (auto& pot : p->pots) {
         pot.g_base = g;
-        pot.h_base = h;
+        pot.h_off = h;
         pot.part_base = parts;
         pot.e_base = elems;
         elems += pot.n_elem;
         pot.n_groups = (pot.n_elem + pot.elems_per_group - 1) / pot.elems_per_group;
         g += (int64_t)pot.n_elem * pot.nb;
-        h += (int64_t)pot.n_elem * pot.n_ub;
+        h += (int64_t)pot.n_elem * pot.h_stride;
         parts += pot.n_groups;
         n_contrib += (uint64_t)pot.n_elem * pot.nb * pot.nb;
     }
-    if (h * 2 + 2 >= (int64_t)UINT32_MAX || g >= (int64_t)UINT32_MAX || n_contrib >= (uint64_t)UINT32_MAX)
+    if ((h / 3) * 2 + 2 >= (int64_t)UINT32_MAX || g >= (int64_t)UINT32_MAX || n_contrib >= (uint64_t)UINT32_MAX)
         return fail(ctx, SYMX_HIP_ERR_ARG, "problem too large for 32-bit assembly maps");
     p->total_g = g;
     p->total_h = h;
@@ -903,8 +906,8 @@ int symx_hip_problem_finalize(symx_hip_problem p)
                     for (int b = 0; b < pot.nb; b++) {
                         const int lo = std::min(a, b), hi = std::max(a, b);
                         const int ub = lo * pot.nb - (lo * (lo - 1)) / 2 + (hi - lo);
-                        const uint64_t blk = (uint64_t)pot.h_base + (uint64_t)e * pot.n_ub + ub;
-                        clist[cur[slot_of[k++]]++] = (uint32_t)(blk * 2 + (a > b ? 1u : 0u));
+                        const uint64_t off3 = ((uint64_t)pot.h_off + (uint64_t)e * pot.h_stride + 9u * ub) / 3;
+                        clist[cur[slot_of[k++]]++] = (uint32_t)(off3 * 2 + (a > b ? 1u : 0u));
                     }
             }
         }
@@ -1022,7 +1025,7 @@ int symx_hip_problem_finalize(symx_hip_problem p)
     // Upload.
     int rc = 0;
     rc = rc ? rc : dalloc(ctx, &p->d_G, (size_t)std::max<int64_t>(g * 3, 1));
-    rc = rc ? rc : dalloc(ctx, &p->d_H, (size_t)std::max<int64_t>(h * 9, 1));
+    rc = rc ? rc : dalloc(ctx, &p->d_H, (size_t)std::max<int64_t>(h, 1));
     rc = rc ? rc : dalloc(ctx, &p->d_Epart, (size_t)std::max<int64_t>(parts, 1));
     rc = rc ? rc : dalloc(ctx, &p->d_Eelem, (size_t)std::max<int64_t>(p->total_elem, 1));
     {
@@ -1134,7 +1137,7 @@ int symx_hip_problem_device_ptr(symx_hip_problem p, int32_t vec, double** dptr)
 // FP32 element-Hessian storage reuses the FP64 buffer (it needs half of it).
 static float* h32_of(symx_hip_problem p, const DevPotential& pot)
 {
-    return reinterpret_cast<float*>(p->d_H) + pot.h_base * 9;
+    return reinterpret_cast<float*>(p->d_H) + pot.h_off;
 }
 
 static void fill_params(symx_hip_problem p, DevPotential& pot, symx_hip_eval_params& prm)
@@ -1146,7 +1149,7 @@ static void fill_params(symx_hip_problem p, DevPotential& pot, symx_hip_eval_par
         prm.maps[m] = pot.maps[m].is_dof ? p->d_vec[SYMX_HIP_VEC_DOFS] + pot.maps[m].dof_offset : pot.maps[m].d_data;
     prm.E_partial = p->d_Epart + pot.part_base;
     prm.G = p->d_G + pot.g_base * 3;
-    prm.H = p->d_H + pot.h_base * 9;
+    prm.H = p->d_H + pot.h_off;
     prm.C_out = pot.d_active;
     prm.E_elem = p->d_Eelem + pot.e_base;
     prm.n_elem = pot.n_elem;
@@ -1282,12 +1285,21 @@ int symx_hip_problem_element_outputs(symx_hip_problem p, int32_t pot_id, double*
     // the context's stream is non-blocking: a plain hipMemcpy does not wait for its kernels
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (G) HIP_TRY(ctx, hipMemcpy(G, p->d_G + pot.g_base * 3, sizeof(double) * pot.n_elem * pot.nb * 3, hipMemcpyDeviceToHost));
-    if (H && !p->h_f32) HIP_TRY(ctx, hipMemcpy(H, p->d_H + pot.h_base * 9, sizeof(double) * pot.n_elem * pot.n_ub * 9, hipMemcpyDeviceToHost));
-    if (H && p->h_f32) {
-        const size_t n = (size_t)pot.n_elem * pot.n_ub * 9;
-        std::vector<float> f(n);
-        HIP_TRY(ctx, hipMemcpy(f.data(), h32_of(p, pot), sizeof(float) * n, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < n; i++) H[i] = (double)f[i];
+    if (H) {
+        // records of h_stride values on the device, n_ub * 9 here (the pad is dropped)
+        const size_t w = (size_t)pot.n_ub * 9, rs = (size_t)pot.h_stride, n = (size_t)pot.n_elem * rs;
+        std::vector<double> d;
+        std::vector<float> f;
+        if (p->h_f32) {
+            f.resize(n);
+            HIP_TRY(ctx, hipMemcpy(f.data(), h32_of(p, pot), sizeof(float) * n, hipMemcpyDeviceToHost));
+        }
+        else {
+            d.resize(n);
+            HIP_TRY(ctx, hipMemcpy(d.data(), p->d_H + pot.h_off, sizeof(double) * n, hipMemcpyDeviceToHost));
+        }
+        for (size_t e = 0; e < (size_t)pot.n_elem; e++)
+            for (size_t i = 0; i < w; i++) H[e * w + i] = p->h_f32 ? (double)f[e * rs + i] : d[e * rs + i];
     }
     return SYMX_HIP_OK;
 }
@@ -1303,9 +1315,9 @@ int symx_hip_problem_project(symx_hip_problem p, double eps, int32_t mirror, int
     for (auto& pot : p->pots) {
         // elements of a conditional potential that are inactive have no Hessian to project
         const uint8_t* only = pot.kC.empty() ? nullptr : pot.d_active;
-        const int bad = p->h_f32 ? launch_project_ub(ctx->stream, h32_of(p, pot), pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters,
+        const int bad = p->h_f32 ? launch_project_ub(ctx->stream, h32_of(p, pot), pot.n_elem, pot.nb, pot.h_stride, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters,
                                                      only, d_count)
-                                 : launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist,
+                                 : launch_project_ub(ctx->stream, p->d_H + pot.h_off, pot.n_elem, pot.nb, pot.h_stride, eps, mirror, p->d_flags, p->d_plist,
                                                      ctx->d_counters, only, d_count);
         if (bad)
             return fail(ctx, SYMX_HIP_ERR_ARG, "PD projection supports 1-4, 8, 10 and 27 DoF blocks per element");
@@ -1353,9 +1365,9 @@ int symx_hip_problem_project_selective(symx_hip_problem p, double eps, int32_t m
     launch_ppn_select(ctx->stream, p->d_goff, p->d_glist, p->d_grow, p->d_vec[SYMX_HIP_VEC_GRAD], p->nb, threshold, pr, p->d_projected, p->d_select, cnt + 2);
     for (auto& pot : p->pots) {
         if (p->h_f32)
-            launch_project_ub(ctx->stream, h32_of(p, pot), pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters, p->d_select + pot.e_base, cnt);
+            launch_project_ub(ctx->stream, h32_of(p, pot), pot.n_elem, pot.nb, pot.h_stride, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters, p->d_select + pot.e_base, cnt);
         else
-            launch_project_ub(ctx->stream, p->d_H + pot.h_base * 9, pot.n_elem, pot.nb, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters,
+            launch_project_ub(ctx->stream, p->d_H + pot.h_off, pot.n_elem, pot.nb, pot.h_stride, eps, mirror, p->d_flags, p->d_plist, ctx->d_counters,
                               p->d_select + pot.e_base, cnt);
     }
     launch_ppn_commit(ctx->stream, p->d_projected, p->d_select, p->total_elem, cnt + 1);
@@ -1882,13 +1894,13 @@ int symx_hip_problem_profile(symx_hip_problem p, int32_t phase, int32_t pot_id, 
     const SellPattern& S = p->sell;
     double* scratch = nullptr;
     // phase 1 projects a copy of the element Hessians in their storage precision
-    const size_t h_vals = (size_t)pot.n_elem * pot.n_ub * 9;
+    const size_t h_vals = (size_t)pot.n_elem * pot.h_stride;
     const size_t h_bytes = h_vals * (p->h_f32 ? sizeof(float) : sizeof(double));
-    const void* h_src = p->h_f32 ? (const void*)h32_of(p, pot) : (const void*)(p->d_H + pot.h_base * 9);
+    const void* h_src = p->h_f32 ? (const void*)h32_of(p, pot) : (const void*)(p->d_H + pot.h_off);
     auto project_scratch = [&]() {
-        return p->h_f32 ? launch_project_ub(ctx->stream, reinterpret_cast<float*>(scratch), pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist,
+        return p->h_f32 ? launch_project_ub(ctx->stream, reinterpret_cast<float*>(scratch), pot.n_elem, pot.nb, pot.h_stride, 1e-10, 0, p->d_flags, p->d_plist,
                                             ctx->d_counters, nullptr, nullptr)
-                        : launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, 1e-10, 0, p->d_flags, p->d_plist, ctx->d_counters, nullptr,
+                        : launch_project_ub(ctx->stream, scratch, pot.n_elem, pot.nb, pot.h_stride, 1e-10, 0, p->d_flags, p->d_plist, ctx->d_counters, nullptr,
                                             nullptr);
     };
     if (phase == 1) {
