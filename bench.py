@@ -31,10 +31,11 @@ FP64_PEAK_TFLOPS = 78.6
 # workload (101^3 block, mixed), each counter scaled by its reading for a 1 GiB device copy
 # (--calibration-copy-mib): FETCH x 2.0, WRITE x 1.0. The per-kernel table is in profiles/.
 MEASURED_TRAFFIC = {("block", "pcg_iteration", 1): 759.3e6}
-# what the mixed path computes in: element kernels and projection in FP64 arithmetic with
-# FP32 element-Hessian storage, FP32 block matrix and inner PCG, FP64 outer refinement,
+# what the mixed path computes in: element energies, gradients and material-law derivatives
+# in FP64, the element-Hessian contraction in FP32 (FP32 storage), projection in FP64
+# arithmetic on the FP32 records, FP32 block matrix and inner PCG, FP64 outer refinement,
 # FP64 energies, gradients and line search
-DTYPE_MIXED = "f32 element Hessians/BSR/PCG (f64 arithmetic in element and projection kernels) + f64 refinement"
+DTYPE_MIXED = "f32 element Hessians (f64 material-law derivatives, f32 contraction)/BSR/PCG, f64 projection arithmetic + f64 refinement"
 
 
 def block_params(cells):
