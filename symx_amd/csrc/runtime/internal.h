@@ -94,6 +94,8 @@ struct symx_hip_ctx_s
     symx_hip::PcgState* h_pcg = nullptr;  // two pinned copies: the PCG batch pipeline reads one while the other is in flight
     hipEvent_t pcg_ev[2] = { nullptr, nullptr };
     int32_t* d_loop_scratch = nullptr;  // device address of this device's loop scratch array
+    char* h_stage[2] = { nullptr, nullptr };  // pinned staging of large host transfers (allocated on first use)
+    hipEvent_t stage_ev[2] = { nullptr, nullptr };
 };
 
 struct symx_hip_module_s

@@ -10,6 +10,7 @@
 #include <numeric>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <sys/stat.h>
 #include <unistd.h>
 #include <vector>
@@ -470,6 +471,10 @@ void symx_hip_ctx_destroy(symx_hip_ctx ctx)
     if (ctx->h_pcg) (void)hipHostFree(ctx->h_pcg);
     for (auto& ev : ctx->pcg_ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (int b = 0; b < 2; b++) {
+        if (ctx->h_stage[b]) (void)hipHostFree(ctx->h_stage[b]);
+        if (ctx->stage_ev[b]) (void)hipEventDestroy(ctx->stage_ev[b]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -715,6 +720,100 @@ int symx_hip_problem_add_potential(symx_hip_problem p, const symx_hip_potential_
     return SYMX_HIP_OK;
 }
 
+// ---- host transfers of user arrays ---------------------------------------------------
+// User arrays live in pageable host memory, which the runtime copies through its own staging
+// with one host thread. Large transfers here go through two pinned staging buffers of
+// kStageBytes instead: each chunk is copied on the host by several threads while the DMA of
+// the previous chunk runs.
+namespace
+{
+    constexpr size_t kStageBytes = size_t(16) << 20;
+    constexpr size_t kStageMin = size_t(4) << 20;  // smaller transfers go straight through
+
+    void parallel_memcpy(void* dst, const void* src, size_t n)
+    {
+        const unsigned hw = std::max(1u, std::min(std::thread::hardware_concurrency(), 8u));
+        const size_t T = std::min<size_t>(hw, (n + (size_t(1) << 20) - 1) >> 20);  // >= 1 MiB per thread
+        if (T <= 1) {
+            std::memcpy(dst, src, n);
+            return;
+        }
+        const size_t per = (n + T - 1) / T;
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < T; t++) {
+            const size_t a = t * per;
+            if (a >= n) break;
+            const size_t b = std::min(n, a + per);
+            th.emplace_back([=]() { std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a); });
+        }
+        std::memcpy(dst, src, std::min(per, n));
+        for (auto& x : th) x.join();
+    }
+
+    int ensure_staging(symx_hip_ctx ctx)
+    {
+        for (int b = 0; b < 2; b++) {
+            if (!ctx->h_stage[b]) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_stage[b], kStageBytes, hipHostMallocDefault));
+            if (!ctx->stage_ev[b]) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->stage_ev[b], hipEventDisableTiming));
+        }
+        return SYMX_HIP_OK;
+    }
+
+    // Host -> device on the context's stream (after the work already on it); returns when
+    // the data is on the device.
+    int host_to_device(symx_hip_ctx ctx, void* d, const void* host, size_t bytes)
+    {
+        if (bytes < kStageMin) {
+            HIP_TRY(ctx, hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            return SYMX_HIP_OK;
+        }
+        int rc = ensure_staging(ctx);
+        if (rc) return rc;
+        bool used[2] = { false, false };
+        int b = 0;
+        for (size_t off = 0; off < bytes; off += kStageBytes, b ^= 1) {
+            const size_t n = std::min(kStageBytes, bytes - off);
+            if (used[b]) HIP_TRY(ctx, hipEventSynchronize(ctx->stage_ev[b]));  // its previous chunk has left
+            parallel_memcpy(ctx->h_stage[b], static_cast<const char*>(host) + off, n);
+            HIP_TRY(ctx, hipMemcpyAsync(static_cast<char*>(d) + off, ctx->h_stage[b], n, hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(ctx, hipEventRecord(ctx->stage_ev[b], ctx->stream));
+            used[b] = true;
+        }
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return SYMX_HIP_OK;
+    }
+
+    // Device -> host after the work already on the context's stream.
+    int device_to_host(symx_hip_ctx ctx, void* host, const void* d, size_t bytes)
+    {
+        if (bytes < kStageMin) {
+            HIP_TRY(ctx, hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            return SYMX_HIP_OK;
+        }
+        int rc = ensure_staging(ctx);
+        if (rc) return rc;
+        const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+        auto enqueue = [&](size_t i) -> int {
+            const size_t off = i * kStageBytes, n = std::min(kStageBytes, bytes - off);
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->h_stage[i & 1], static_cast<const char*>(d) + off, n, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipEventRecord(ctx->stage_ev[i & 1], ctx->stream));
+            return SYMX_HIP_OK;
+        };
+        rc = enqueue(0);
+        for (size_t i = 0; !rc && i < nch; i++) {
+            // chunk i + 1 goes to the buffer whose chunk i - 1 was copied out in the last round
+            if (i + 1 < nch) rc = enqueue(i + 1);
+            if (rc) break;
+            HIP_TRY(ctx, hipEventSynchronize(ctx->stage_ev[i & 1]));
+            const size_t off = i * kStageBytes;
+            parallel_memcpy(static_cast<char*>(host) + off, ctx->h_stage[i & 1], std::min(kStageBytes, bytes - off));
+        }
+        return rc;
+    }
+}
+
 int symx_hip_problem_set_map_data(symx_hip_problem p, int32_t pot_id, int32_t map_id, const double* host, int64_t n)
 {
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
@@ -733,9 +832,7 @@ int symx_hip_problem_set_map_data(symx_hip_problem p, int32_t pot_id, int32_t ma
         m.owner = true;
     }
     m.host = host;
-    HIP_TRY(ctx, hipMemcpyAsync(m.d_data, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return SYMX_HIP_OK;
+    return host_to_device(ctx, m.d_data, host, (size_t)n * sizeof(double));
 }
 
 // Topology: gradient scatter lists, block pattern (sliced, diagonal first) and
@@ -1107,9 +1204,7 @@ int symx_hip_problem_download(symx_hip_problem p, int32_t vec, double* host)
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
     double* d = vec_ptr(p, vec);
     if (!d || !host) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id or null pointer");
-    HIP_TRY(p->ctx, hipMemcpyAsync(host, d, sizeof(double) * p->n_dofs, hipMemcpyDeviceToHost, p->ctx->stream));
-    HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));
-    return SYMX_HIP_OK;
+    return device_to_host(p->ctx, host, d, sizeof(double) * (size_t)p->n_dofs);
 }
 int symx_hip_problem_upload(symx_hip_problem p, int32_t vec, const double* host)
 {
@@ -1117,9 +1212,7 @@ int symx_hip_problem_upload(symx_hip_problem p, int32_t vec, const double* host)
     double* d = vec_ptr(p, vec);
     if (!d || !host) return fail(p->ctx, SYMX_HIP_ERR_ARG, "bad vector id or null pointer");
     if (vec == SYMX_HIP_VEC_GRAD) p->grad_max_valid = false;
-    HIP_TRY(p->ctx, hipMemcpyAsync(d, host, sizeof(double) * p->n_dofs, hipMemcpyHostToDevice, p->ctx->stream));
-    HIP_TRY(p->ctx, hipStreamSynchronize(p->ctx->stream));
-    return SYMX_HIP_OK;
+    return host_to_device(p->ctx, d, host, sizeof(double) * (size_t)p->n_dofs);
 }
 int symx_hip_problem_device_ptr(symx_hip_problem p, int32_t vec, double** dptr)
 {
