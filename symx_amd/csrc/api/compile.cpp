@@ -478,13 +478,13 @@ namespace symx
                << (adj ? ", const symx_hip_adjoint_params a" : "") << ")\n{\n";
             os << "    const int e = symx_block() * 256 + threadIdx.x;\n";
             if (ks.kind != KernelKind::C && !adj) os << "    double e_out = 0.0;\n";
-            // single-point gradient/Hessian kernels collect their outputs in registers and
-            // store them through symx_store_rows (coalesced); summed kernels accumulate in place,
-            // except the FP32-storage twins, which sum the points in FP64 registers and round once
-            const bool staged = (ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH) && (!multi_q || h32);
+            // (single-point gradient/Hessian kernels take emit_chunked_kernel above.) Summed
+            // kernels accumulate their outputs in place, except the FP32-storage twins, which sum
+            // the points in FP64 registers, round once and store through symx_store_rows
+            const bool staged = (ks.kind == KernelKind::PG || ks.kind == KernelKind::PGH) && h32;
             if (staged) {
                 os << "    double sg[" << n << "] = {};\n";
-                if (ks.kind == KernelKind::PGH) os << "    " << (h32 && !multi_q ? "float" : "double") << " sh[" << n_ub * 9 << "] = {};\n";
+                if (ks.kind == KernelKind::PGH) os << "    double sh[" << n_ub * 9 << "] = {};\n";
             }
             os << "    if (e < p.n_elem) {\n";
             if (ks.kind != KernelKind::C) os << "        const bool act = (p.active == nullptr) || (p.active[e] != 0);\n";
@@ -568,8 +568,7 @@ namespace symx
                 os << "        const long long rem = (long long)p.n_elem - e0;\n";
                 os << "        const int nv = rem < 0 ? 0 : (rem > 64 ? 64 : (int)rem);\n";
                 os << "        symx_store_rows<" << n << ">(p.G, sg, e0, nv);\n";
-                if (ks.kind == KernelKind::PGH && h32) os << "        symx_store_rows_f32<" << n_ub * 9 << ">((float*)p.H, sh, e0, nv);\n";
-                if (ks.kind == KernelKind::PGH && !h32) os << "        symx_store_rows<" << n_ub * 9 << ">(p.H, sh, e0, nv);\n";
+                if (ks.kind == KernelKind::PGH) os << "        symx_store_rows_f32<" << n_ub * 9 << ">((float*)p.H, sh, e0, nv);\n";
                 os << "    }\n    symx_store_energy(e_out, p, e);\n}\n";
             }
             else {
