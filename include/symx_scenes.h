@@ -59,13 +59,14 @@ typedef struct symx_sphere_params {
     double step_filter;        /* eta of the plane step filter (0: off) */
 } symx_sphere_params;
 
-/* Hexahedral block (Hex8 or Hex27 cuboid over [0, size]) with a per-element Young's
- * modulus drawn log-uniformly in [youngs_min, youngs_max]; loads and boundary as the
- * Tet4 block; element energy written with fem_integrator / fem_jacobian. */
+/* Hexahedral block (Hex8 or Hex27 cuboid over [0, size]; with nodes_per_element = 10 the
+ * Tet10 block of the same cuboid) with a per-element Young's modulus drawn log-uniformly in
+ * [youngs_min, youngs_max]; loads and boundary as the Tet4 block; element energy written
+ * with fem_integrator / fem_jacobian. */
 typedef struct symx_hex_block_params {
     int32_t cells[3];
     double size[3];
-    int32_t nodes_per_element;  /* 8 or 27 */
+    int32_t nodes_per_element;  /* 8 (Hex8), 27 (Hex27) or 10 (Tet10: six per cell) */
     double youngs_min;
     double youngs_max;
     double poissons_ratio;

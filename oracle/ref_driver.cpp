@@ -15,11 +15,12 @@
 //       Adjoint dL/dlog_mu of the quasistatic cantilever at given positions, composed
 //       from the reference's Hessian assembly, DirectLLT solver and Compiled<double>.
 //   ref_driver hex_element <in.bin> <out.bin>
-//       E, gradient and Hessian of one Neo-Hookean Hex8 / Hex27 element per case, built with
+//       E, gradient and Hessian of one Neo-Hookean Hex8 / Hex27 / Tet10 element per case
+//       (8, 27 or 10 nodes), built with
 //       the reference's fem_integrator / fem_jacobian (fem_integrators.cpp:41-133) and
 //       evaluated by SecondOrderCompiledGlobal (unprojected element Hessian).
 //   ref_driver solve_hex <in.bin> <out.bin>
-//       One implicit step of a hexahedral block (Neo-Hookean with per-element mu via
+//       One implicit step of a hexahedral or Tet10 block (Neo-Hookean with per-element mu via
 //       fem_integrator, backward Euler inertia, penalty boundary) with NewtonsMethod.
 #include <symx>
 #include <Eigen/SparseCholesky>
@@ -361,12 +362,13 @@ static int run_adjoint(int argc, char** argv)
     return 0;
 }
 
-// Neo-Hookean hexahedron (nn = 8 or 27 nodes) through the reference's own isoparametric
-// integrator: sum over the element's quadrature points of psi(F) det(J_X) w.
+// Neo-Hookean hexahedron (nn = 8 or 27 nodes) or quadratic tetrahedron (nn = 10) through the
+// reference's own isoparametric integrator: sum over the element's quadrature points of
+// psi(F) det(J_X) w.
 static Scalar nh_hex_energy(MappedWorkspace<double>& mws, int nn, std::vector<Vector>& x, std::vector<Vector>& X, const Scalar& l,
                             const Scalar& m)
 {
-    const FEM_Element el = nn == 27 ? FEM_Element::Hex27 : FEM_Element::Hex8;
+    const FEM_Element el = nn == 27 ? FEM_Element::Hex27 : (nn == 10 ? FEM_Element::Tet10 : FEM_Element::Hex8);
     return fem_integrator(mws, el, [&](Scalar& w, Vector& xi) {
         Matrix Jx = fem_jacobian(el, x, xi);
         Matrix JX = fem_jacobian(el, X, xi);
@@ -382,7 +384,7 @@ static int run_hex_element(int argc, char** argv)
     int32_t hdr[2];
     rd(f, hdr, 2);
     const int nn = hdr[0], ncase = hdr[1];
-    if (nn != 8 && nn != 27) return 3;
+    if (nn != 8 && nn != 10 && nn != 27) return 3;
     const int n = 3 * nn;
     struct
     {
@@ -442,7 +444,7 @@ static int run_solve_hex(int argc, char** argv)
     rd(f, hdr, 9);
     const int nv = hdr[0], ne = hdr[1], nbc = hdr[2], has_inertia = hdr[3], threads = hdr[4], projection_mode = hdr[5], mirror = hdr[6],
               max_iter = hdr[7], nn = hdr[8];
-    if (nn != 8 && nn != 27) return 3;
+    if (nn != 8 && nn != 10 && nn != 27) return 3;
     double prm[11];
     rd(f, prm, 11);
     struct Data

@@ -154,12 +154,15 @@ ORACLE_API int oracle_nh_tet4(const double* x, const double* X, double lambda, d
     return 0;
 }
 
-/* ---- Neo-Hookean Hex8 / Hex27 elements (closed form, isoparametric) ------- */
-/* Node order, shape functions and quadrature of fem_integrators.h:84-150 and
- * fem_integrators.cpp:69-116: per axis the node sits at -1, 0 or +1 (0, 1, 2 below);
- * Hex8: 0.5 (1 -+ t); Hex27: t(t-1), 1 - t^2, t(t+1) scaled by 1/8, 1/4, 1/2, 1 for
+/* ---- Neo-Hookean Hex8 / Hex27 / Tet10 elements (closed form, isoparametric) */
+/* Node order, shape functions and quadrature of fem_integrators.h:56-150 and
+ * fem_integrators.cpp:60-116. Hexahedra: per axis the node sits at -1, 0 or +1 (0, 1, 2
+ * below); Hex8: 0.5 (1 -+ t); Hex27: t(t-1), 1 - t^2, t(t+1) scaled by 1/8, 1/4, 1/2, 1 for
  * nodes with 0, 1, 2, 3 mid-axis positions. Rules: 2x2x2 Gauss (weight 1) and the 27-point
- * rule with the reference's 10-digit constants, in the reference's order. */
+ * rule with the reference's 10-digit constants, in the reference's order. Tet10 (quadratic
+ * tetrahedron, barycentric L0 = 1 - u - v - w, L1 = u, L2 = v, L3 = w): vertices
+ * Li (2 Li - 1), then mid-edges 4 Li Lj in the order 01 12 02 03 13 23; the reference's
+ * 4-point rule with its 8-digit constants (weights 0.041666667, summing to 1/6 + 2e-9). */
 static const int8_t hex_nodes[27][3] = {
     { 0, 0, 0 }, { 2, 0, 0 }, { 2, 2, 0 }, { 0, 2, 0 }, { 0, 0, 2 }, { 2, 0, 2 }, { 2, 2, 2 }, { 0, 2, 2 },
     { 1, 0, 0 }, { 2, 1, 0 }, { 1, 2, 0 }, { 0, 1, 0 }, { 1, 0, 2 }, { 2, 1, 2 }, { 1, 2, 2 }, { 0, 1, 2 },
@@ -167,8 +170,14 @@ static const int8_t hex_nodes[27][3] = {
     { 0, 1, 1 }, { 2, 1, 1 }, { 1, 2, 1 }, { 1, 0, 1 }, { 1, 1, 0 }, { 1, 1, 2 },
     { 1, 1, 1 } };
 
-static int hex_rule(int nn, double (*q)[4])
+static int iso_rule(int nn, double (*q)[4])
 {
+    if (nn == 10) {
+        const double a = 0.58541020, b = 0.13819660, w = 0.041666667;
+        const double t[4][4] = { { w, a, b, b }, { w, b, a, b }, { w, b, b, a }, { w, b, b, b } };
+        memcpy(q, t, sizeof(t));
+        return 4;
+    }
     if (nn == 8) {
         const double p = 1.0 / sqrt(3.0);
         const double pts[8][3] = { { -p, -p, -p }, { p, -p, -p }, { p, p, -p }, { -p, p, -p }, { -p, -p, p }, { p, -p, p }, { p, p, p }, { -p, p, p } };
@@ -189,8 +198,20 @@ static int hex_rule(int nn, double (*q)[4])
 }
 
 /* dN_a / dxi at xi (dN [nn][3]) */
-static void hex_dshape(int nn, const double* xi, double* dN)
+static void iso_dshape(int nn, const double* xi, double* dN)
 {
+    if (nn == 10) {
+        const double L[4] = { 1.0 - xi[0] - xi[1] - xi[2], xi[0], xi[1], xi[2] };
+        const double dL[4][3] = { { -1.0, -1.0, -1.0 }, { 1.0, 0.0, 0.0 }, { 0.0, 1.0, 0.0 }, { 0.0, 0.0, 1.0 } };
+        const int edge[6][2] = { { 0, 1 }, { 1, 2 }, { 0, 2 }, { 0, 3 }, { 1, 3 }, { 2, 3 } };
+        for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 3; k++) dN[3 * i + k] = (4.0 * L[i] - 1.0) * dL[i][k];
+        for (int m = 0; m < 6; m++) {
+            const int i = edge[m][0], j = edge[m][1];
+            for (int k = 0; k < 3; k++) dN[3 * (4 + m) + k] = 4.0 * (dL[i][k] * L[j] + L[i] * dL[j][k]);
+        }
+        return;
+    }
     for (int a = 0; a < nn; a++) {
         double f[3], df[3];
         int n_mid = 0;
@@ -227,16 +248,16 @@ static void hex_dshape(int nn, const double* xi, double* dN)
  * dF_ij/dx_(a,i) = G_a[j] = sum_k dN_a[k] JX^-1[k][j]. x, X: [nn][3]; g: [3nn]; H: [3nn][3nn]. */
 ORACLE_API int oracle_nh_hex(int nn, const double* x, const double* X, double lambda, double mu, double* E, double* g, double* H)
 {
-    if (nn != 8 && nn != 27) return 2;
+    if (nn != 8 && nn != 10 && nn != 27) return 2;
     const int n = 3 * nn;
     double q[27][4];
-    const int nq = hex_rule(nn, q);
+    const int nq = iso_rule(nn, q);
     double dN[27 * 3], G[27 * 3];
     if (E) *E = 0.0;
     if (g) memset(g, 0, sizeof(double) * n);
     if (H) memset(H, 0, sizeof(double) * n * n);
     for (int iq = 0; iq < nq; iq++) {
-        hex_dshape(nn, &q[iq][1], dN);
+        iso_dshape(nn, &q[iq][1], dN);
         double Jx[9] = { 0 }, JX[9] = { 0 }, JXinv[9], F[9], Finv[9], FinvT[9];
         for (int a = 0; a < nn; a++)
             for (int i = 0; i < 3; i++)
@@ -299,7 +320,7 @@ ORACLE_API int oracle_nh_hex(int nn, const double* x, const double* X, double la
     return 0;
 }
 
-/* element energy of a scene element (Tet4 or hexahedron) */
+/* element energy of a scene element (Tet4, Tet10 or hexahedron) */
 static int scene_element(int npe, const double* xe, const double* Xe, double lam, double mu, double* E, double* g, double* H)
 {
     return npe == 4 ? oracle_nh_tet4(xe, Xe, lam, mu, E, g, H) : oracle_nh_hex(npe, xe, Xe, lam, mu, E, g, H);
@@ -538,7 +559,7 @@ typedef struct oracle_scene {
     int32_t pad_;
     const int32_t* contact_vertex;
     double kappa, dhat, step_eta;
-    /* element type of `tets`: 4 (Tet4, also when 0), 8 (Hex8) or 27 (Hex27) */
+    /* element type of `tets`: 4 (Tet4, also when 0), 8 (Hex8), 10 (Tet10) or 27 (Hex27) */
     int32_t nodes_per_element;
     int32_t pad2_;
 } oracle_scene;

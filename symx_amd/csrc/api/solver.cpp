@@ -1824,6 +1824,39 @@ namespace symx
         return m;
     }
 
+    FEMMesh generate_cuboid_Tet10_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& n, const std::array<double, 3>& origin)
+    {
+        FEMMesh m;
+        m.element_type = FEM_Element::Tet10;
+        m.connectivity_stride = 10;
+        const int N[3] = { 2 * n[0] + 1, 2 * n[1] + 1, 2 * n[2] + 1 };
+        const double d[3] = { size[0] / (2.0 * n[0]), size[1] / (2.0 * n[1]), size[2] / (2.0 * n[2]) };
+        for (int i = 0; i < N[0]; i++)
+            for (int j = 0; j < N[1]; j++)
+                for (int k = 0; k < N[2]; k++) m.vertices.push_back({ origin[0] + i * d[0], origin[1] + j * d[1], origin[2] + k * d[2] });
+        auto idx = [&](int i, int j, int k) { return (int32_t)(i * N[1] * N[2] + j * N[2] + k); };
+        // the six tetrahedra of the Tet4 block per cell (corners in the order of
+        // k_hex27_nodes[0..7]); each gets its vertices, then the mid-edges 01 12 02 03 13 23,
+        // the node order of the Tet10 shape functions
+        static const int split[6][4] = { { 0, 1, 6, 5 }, { 0, 1, 2, 6 }, { 0, 2, 3, 6 }, { 0, 5, 6, 4 }, { 0, 6, 3, 7 }, { 0, 6, 7, 4 } };
+        static const int edge[6][2] = { { 0, 1 }, { 1, 2 }, { 0, 2 }, { 0, 3 }, { 1, 3 }, { 2, 3 } };
+        m.connectivity.reserve((size_t)n[0] * n[1] * n[2] * 60);
+        for (int i = 0; i < n[0]; i++)
+            for (int j = 0; j < n[1]; j++)
+                for (int k = 0; k < n[2]; k++)
+                    for (const auto& t : split) {
+                        auto node = [&](int c0, int c1) {
+                            const int8_t* p = k_hex27_nodes[c0];
+                            const int8_t* q = k_hex27_nodes[c1];
+                            return idx(2 * i + (p[0] + q[0]) / 2, 2 * j + (p[1] + q[1]) / 2, 2 * k + (p[2] + q[2]) / 2);
+                        };
+                        for (int a = 0; a < 4; a++) m.connectivity.push_back(node(t[a], t[a]));
+                        for (const auto& e : edge) m.connectivity.push_back(node(t[e[0]], t[e[1]]));
+                    }
+        m.n_elements = (int32_t)(m.connectivity.size() / 10);
+        return m;
+    }
+
     FEMMesh generate_cuboid_Hex8_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& n, const std::array<double, 3>& origin)
     {
         FEMMesh m;
@@ -1861,5 +1894,16 @@ namespace symx
                         m.connectivity.push_back(idx(2 * i + k_hex27_nodes[c][0], 2 * j + k_hex27_nodes[c][1], 2 * k + k_hex27_nodes[c][2]));
         m.n_elements = (int32_t)(m.connectivity.size() / 27);
         return m;
+    }
+
+    FEMMesh generate_cuboid_mesh(const FEM_Element& e, const std::array<double, 3>& size, const std::array<int32_t, 3>& n, const std::array<double, 3>& origin)
+    {
+        switch (e) {
+        case FEM_Element::Tet4: return generate_cuboid_Tet4_mesh(size, n, origin);
+        case FEM_Element::Tet10: return generate_cuboid_Tet10_mesh(size, n, origin);
+        case FEM_Element::Hex8: return generate_cuboid_Hex8_mesh(size, n, origin);
+        case FEM_Element::Hex27: return generate_cuboid_Hex27_mesh(size, n, origin);
+        }
+        throw std::runtime_error("symx: generate_cuboid_mesh got an unknown element type");
     }
 }

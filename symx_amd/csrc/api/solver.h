@@ -687,6 +687,11 @@ namespace symx
     // 6-tetrahedra pattern of mesh_generation.cpp:78-83; vertices ordered i-major.
     FEMMesh generate_cuboid_Tet4_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
                                       const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
+    // Cuboid of Tet10 elements: nodes on a (2n+1)^3 lattice, each cell split into the six
+    // tetrahedra of the Tet4 mesh, element nodes v0..v3 then mid-edges 01 12 02 03 13 23
+    // (mesh_generation.cpp:93-159; no RCM renumbering).
+    FEMMesh generate_cuboid_Tet10_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
+                                       const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
     // Cuboid of Hex8 elements (mesh_generation.cpp:161-206 node order; no RCM renumbering).
     FEMMesh generate_cuboid_Hex8_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
                                       const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
@@ -694,4 +699,7 @@ namespace symx
     // order of mesh_generation.cpp:209-300; no RCM renumbering).
     FEMMesh generate_cuboid_Hex27_mesh(const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
                                        const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
+    // The generator of element type `element` (mesh_generation.cpp:307-326).
+    FEMMesh generate_cuboid_mesh(const FEM_Element& element, const std::array<double, 3>& size, const std::array<int32_t, 3>& elements_per_axis,
+                                 const std::array<double, 3>& origin = { 0.0, 0.0, 0.0 });
 }

@@ -406,23 +406,26 @@ static void build_sphere(symx_scene_s* s, const symx_sphere_params* p, const sym
     finish_scene(s, spec);
 }
 
-// Hexahedral block: Hex8 or Hex27 cuboid over [0, size], per-element Young's modulus
+// Hexahedral block: Hex8 or Hex27 cuboid over [0, size] (or, with 10 nodes per element, the
+// Tet10 block: six quadratic tetrahedra per cell), per-element Young's modulus
 // log-uniform in [youngs_min, youngs_max] (SplitMix64 stream seed ^ 0x9e37..., one draw
 // per element), node noise like the Tet4 block (stream `seed`, 3 draws per node, noise in
 // node spacings), masses lumped uniformly (rho V_e / nodes per element to each node).
 static void build_hex(symx_scene_s* s, const symx_hex_block_params* p, const symx_newton_params* np)
 {
-    if (p->nodes_per_element != 8 && p->nodes_per_element != 27) throw std::runtime_error("nodes_per_element must be 8 or 27");
+    if (p->nodes_per_element != 8 && p->nodes_per_element != 10 && p->nodes_per_element != 27)
+        throw std::runtime_error("nodes_per_element must be 8, 10 or 27");
     s->np = *np;
     s->npe = p->nodes_per_element;
     const int npe = s->npe;
-    const FEMMesh mesh = npe == 27 ? generate_cuboid_Hex27_mesh({ p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] })
-                                   : generate_cuboid_Hex8_mesh({ p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] });
+    // 10 nodes: the Tet10 block of the same cuboid (six quadratic tetrahedra per cell)
+    const FEM_Element type = npe == 27 ? FEM_Element::Hex27 : (npe == 10 ? FEM_Element::Tet10 : FEM_Element::Hex8);
+    const FEMMesh mesh = generate_cuboid_mesh(type, { p->size[0], p->size[1], p->size[2] }, { p->cells[0], p->cells[1], p->cells[2] });
     const int32_t nv = (int32_t)mesh.vertices.size(), ne = mesh.n_elements;
     s->X = mesh.vertices;
     s->tets = mesh.connectivity;
     s->x = s->X;
-    const int sub = npe == 27 ? 2 : 1;
+    const int sub = npe == 8 ? 1 : 2;
     const double h[3] = { p->size[0] / (sub * p->cells[0]), p->size[1] / (sub * p->cells[1]), p->size[2] / (sub * p->cells[2]) };
     SplitMix64 rng(p->seed);
     std::vector<uint8_t> fixed(nv, 0);
@@ -444,7 +447,7 @@ static void build_hex(symx_scene_s* s, const symx_hex_block_params* p, const sym
     s->E_elem.resize(ne);
     const double l0 = std::log(p->youngs_min), l1 = std::log(p->youngs_max);
     for (int32_t e = 0; e < ne; e++) s->E_elem[e] = std::exp(l0 + rng_e.uniform() * (l1 - l0));
-    const double vol = (p->size[0] / p->cells[0]) * (p->size[1] / p->cells[1]) * (p->size[2] / p->cells[2]);
+    const double vol = (p->size[0] / p->cells[0]) * (p->size[1] / p->cells[1]) * (p->size[2] / p->cells[2]) / (npe == 10 ? 6.0 : 1.0);
     s->mass.assign(nv, 0.0);
     for (int32_t e = 0; e < ne; e++)
         for (int a = 0; a < npe; a++) s->mass[s->tets[(size_t)npe * e + a]] += p->density * vol / npe;
@@ -461,8 +464,7 @@ static void build_hex(symx_scene_s* s, const symx_hex_block_params* p, const sym
     s->gravity = { p->gravity[0], p->gravity[1], p->gravity[2] };
     s->G = GlobalPotential::create();
     symx_scene_s* S = s;
-    const FEM_Element type = npe == 27 ? FEM_Element::Hex27 : FEM_Element::Hex8;
-    S->G->add_potential(npe == 27 ? "neohookean_hex27" : "neohookean_hex8", S->elems_enum, npe + 1,
+    S->G->add_potential(npe == 27 ? "neohookean_hex27" : (npe == 10 ? "neohookean_tet10" : "neohookean_hex8"), S->elems_enum, npe + 1,
                         [S, npe, type](MappedWorkspace<double>& mws, Element& elem) {
                             std::vector<Vector> xs = mws.make_vectors(S->x, elem.slice(1, npe + 1));
                             std::vector<Vector> Xs = mws.make_vectors(S->X, elem.slice(1, npe + 1));

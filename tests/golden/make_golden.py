@@ -21,6 +21,11 @@ build container:  python tests/golden/make_golden.py
                              (`python tests/golden/make_golden.py hex` writes these two)
   newton_ref_block6_llt.npz  the block6 case with LinearSolver::DirectLLT to 1e-8 relative
                              (`python tests/golden/make_golden.py llt`)
+  tet10_ref_elements.npz     E, gradient, Hessian of Neo-Hookean Tet10 elements from the
+                             reference's fem_integrator (ref_driver hex_element, 10 nodes)
+  tet10_ref_block2.npz       the reference NewtonsMethod (ProjectOnDemand) on a 2^3-cell Tet10
+                             block (48 elements) with per-element Young's moduli
+                             (`python tests/golden/make_golden.py tet10` writes these two)
 """
 import json
 import os
@@ -150,19 +155,51 @@ def hex_cases():
     print("hex elements", {k: v.shape for k, v in out.items()})
 
 
-def hex_newton_case(name, cells, mode=2, rel_tol=1e-6, threads=1):
+TET10_REF_NODES = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (0.5, 0, 0), (0.5, 0.5, 0), (0, 0.5, 0), (0, 0, 0.5), (0.5, 0, 0.5),
+                   (0, 0.5, 0.5)]
+
+
+def tet10_cases():
+    """Perturbed Tet10 elements: scaled, rotated-and-stretched and noisy unit tetrahedra
+    (vertices, then mid-edges 01 12 02 03 13 23), random materials."""
+    from ref_io import read_hex_element_output, write_hex_element_input
+
+    rng = np.random.default_rng(20261205)
+    base = np.array(TET10_REF_NODES, float)
+    cases = []
+    for i in range(6):
+        h = 10.0 ** rng.uniform(-2, 0)
+        X = base * h + rng.uniform(-0.03, 0.03, base.shape) * h
+        A = np.eye(3) + rng.uniform(-0.15, 0.15, (3, 3))
+        x = X @ A.T + rng.uniform(-0.02, 0.02, X.shape) * h
+        E_, nu = 10.0 ** rng.uniform(4, 6), rng.uniform(0.2, 0.45)
+        lam, mu = E_ * nu / ((1 + nu) * (1 - 2 * nu)), E_ / (2 * (1 + nu))
+        cases.append((lam, mu, x, X))
+    with tempfile.TemporaryDirectory() as d:
+        inp, outp = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        write_hex_element_input(inp, 10, cases)
+        subprocess.run([os.path.join(REF, "ref_driver"), "hex_element", inp, outp], check=True, env=ENV)
+        E, g, H = read_hex_element_output(outp, 10, len(cases))
+    out = dict(x=np.stack([c[2] for c in cases]), X=np.stack([c[3] for c in cases]), lam=np.array([c[0] for c in cases]),
+               mu=np.array([c[1] for c in cases]), E=E, g=g, H=H)
+    np.savez_compressed(os.path.join(HERE, "tet10_ref_elements.npz"), **out)
+    print("tet10 elements", {k: v.shape for k, v in out.items()})
+
+
+def hex_newton_case(name, cells, mode=2, rel_tol=1e-6, threads=1, nn=27):
     from ref_io import read_solve_output, write_solve_hex_input
     from scene_np import hex_block
     from symx_amd import NewtonParams
 
-    S = hex_block(cells=cells, nn=27)
+    S = hex_block(cells=cells, nn=nn)
     newton = NewtonParams(0, mode, 0, 100, 0.0, rel_tol, 1e-3, 1e-12, 10000, 1e-10, 0.0)
     with tempfile.TemporaryDirectory() as d:
         inp, outp = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         write_solve_hex_input(inp, S, newton, threads)
         subprocess.run([os.path.join(REF, "ref_driver"), "solve_hex", inp, outp], check=True, env=ENV)
         res = read_solve_output(outp, S.X.shape[0])
-    np.savez_compressed(os.path.join(HERE, f"hex27_ref_{name}.npz"), x=res["x"], cells=np.array(cells), rel_tol=rel_tol, result=res["result"],
+    prefix = "tet10" if nn == 10 else f"hex{nn}"
+    np.savez_compressed(os.path.join(HERE, f"{prefix}_ref_{name}.npz"), x=res["x"], cells=np.array(cells), rel_tol=rel_tol, result=res["result"],
                         newton_iterations=res["newton_iterations"], cg_iterations=res["cg_iterations"],
                         ls_bt_iterations=res["ls_bt_iterations"], seconds=res["seconds"], threads=threads, projection_mode=mode)
     print(name, {k: v for k, v in res.items() if k != "x"})
@@ -175,6 +212,10 @@ if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "hex":
         hex_cases()
         hex_newton_case("block2", (2, 2, 2))
+        sys.exit(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "tet10":
+        tet10_cases()
+        hex_newton_case("block2", (2, 2, 2), nn=10)
         sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "adjoint":
         adjoint_case("cantilever16", (15, 3, 3))

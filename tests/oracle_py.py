@@ -159,7 +159,7 @@ class Oracle:
         return (E, g) if with_grad else E
 
     def nh_hex(self, x, X, lam, mu):
-        """E, gradient, Hessian of a Hex8 / Hex27 Neo-Hookean element (nodes [nn, 3])."""
+        """E, gradient, Hessian of a Hex8 / Tet10 / Hex27 Neo-Hookean element (nodes [nn, 3])."""
         x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
         X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
         nn = x.shape[0]
@@ -168,7 +168,7 @@ class Oracle:
         H = np.zeros(9 * nn * nn)
         rc = self.lib.oracle_nh_hex(c_int(nn), _p(x), _p(X), c_double(lam), c_double(mu), ctypes.byref(E), _p(g), _p(H))
         if rc == 2:
-            raise ValueError("nn must be 8 or 27")
+            raise ValueError("nn must be 8, 10 or 27")
         return E.value, g, H.reshape(3 * nn, 3 * nn)
 
     def element_hessians(self, S: SceneData, x=None):

@@ -103,12 +103,41 @@ def hex_mesh(cells, size=(1.0, 1.0, 1.0), nn=27):
     return X, np.stack(cols, 1).astype(np.int32)
 
 
+# Tet10 element nodes in the units of HEX_NODES (0, 1, 2 along each axis): the vertices of
+# the unit tetrahedron, then the mid-edges 01 12 02 03 13 23
+TET10_NODES = [(0, 0, 0), (2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1)]
+TET10_SPLIT = [(0, 1, 6, 5), (0, 1, 2, 6), (0, 2, 3, 6), (0, 5, 6, 4), (0, 6, 3, 7), (0, 6, 7, 4)]
+TET10_EDGES = [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]
+
+
+def tet10_mesh(cells, size=(1.0, 1.0, 1.0)):
+    """Tet10 cuboid over [0, size] (mesh_generation.cpp:93-159 without the RCM renumbering):
+    nodes on the (2n+1)^3 lattice, each cell split into the six tetrahedra of the Tet4 block,
+    element nodes v0..v3 then the mid-edges 01 12 02 03 13 23."""
+    N = [2 * c + 1 for c in cells]
+    d = [size[i] / (2 * cells[i]) for i in range(3)]
+    I, J, K = np.meshgrid(np.arange(N[0]), np.arange(N[1]), np.arange(N[2]), indexing="ij")
+    X = np.stack([I.ravel() * d[0], J.ravel() * d[1], K.ravel() * d[2]], 1)
+    i, j, k = np.meshgrid(np.arange(cells[0]), np.arange(cells[1]), np.arange(cells[2]), indexing="ij")
+    base = np.stack([2 * i.ravel(), 2 * j.ravel(), 2 * k.ravel()], 1)
+    corner = np.array(HEX_NODES[:8])
+    lin = lambda p: p[:, 0] * N[1] * N[2] + p[:, 1] * N[2] + p[:, 2]
+    T = []
+    for t in TET10_SPLIT:
+        cols = [lin(base + corner[a]) for a in t]
+        cols += [lin(base + (corner[t[a]] + corner[t[b]]) // 2) for (a, b) in TET10_EDGES]
+        T.append(np.stack(cols, 1))
+    # cells in i-major order, the six tetrahedra of a cell consecutive
+    return X, np.stack(T, 1).reshape(-1, 10).astype(np.int32)
+
+
 def hex_block(cells=(2, 2, 2), size=(1.0, 1.0, 1.0), nn=27, E_min=1e4, E_max=1e6, nu=0.45, density=1000.0, gravity=(0, 0, -9.81),
               dt=0.01, noise=0.1, seed=3, k=1e8, fixed_axis=2):
-    """numpy restatement of symx_scene_hex_block_create's inputs (SceneData with mu_e)."""
-    X, T = hex_mesh(cells, size, nn)
+    """numpy restatement of symx_scene_hex_block_create's inputs (SceneData with mu_e);
+    nn = 10 builds the Tet10 block of the same cuboid."""
+    X, T = tet10_mesh(cells, size) if nn == 10 else hex_mesh(cells, size, nn)
     nv = X.shape[0]
-    sub = 2 if nn == 27 else 1
+    sub = 1 if nn == 8 else 2
     h = np.array([size[i] / (sub * cells[i]) for i in range(3)])
     fixed = np.abs(X[:, fixed_axis]) < 1e-12 if fixed_axis >= 0 else np.zeros(nv, bool)
     u = splitmix64_uniform(seed, 3 * nv).reshape(nv, 3)
@@ -116,7 +145,7 @@ def hex_block(cells=(2, 2, 2), size=(1.0, 1.0, 1.0), nn=27, E_min=1e4, E_max=1e6
     ne = T.shape[0]
     ue = splitmix64_uniform(seed ^ 0x9E3779B97F4A7C15, ne)
     E = np.exp(np.log(E_min) + ue * (np.log(E_max) - np.log(E_min)))
-    vol = np.prod([size[i] / cells[i] for i in range(3)])
+    vol = np.prod([size[i] / cells[i] for i in range(3)]) / (6.0 if nn == 10 else 1.0)
     mass = np.zeros(nv)
     np.add.at(mass, T.ravel(), density * vol / nn)
     mu_e = E / (2 * (1 + nu))

@@ -1,11 +1,12 @@
-"""Hexahedral elements (Hex8, Hex27): the structured FEM path (deferred isoparametric
-Jacobian, generated material derivatives, hand-written contraction), the LDS Jacobi PD
-projection of 24x24 / 81x81 element Hessians, and Newton on hexahedral blocks.
+"""Isoparametric elements (Hex8, Hex27 and the quadratic tetrahedron Tet10): the structured
+FEM path (deferred isoparametric Jacobian, generated material derivatives, hand-written
+contraction), the PD projection of 24x24 / 30x30 / 81x81 element Hessians, and Newton on
+hexahedral and Tet10 blocks.
 
-CPU tests pin the oracle's closed-form Hex8/Hex27 Neo-Hookean element (central
-differences, rest state, affine deformations) and check the structured decomposition
-on the host against the generic expansion and the oracle. GPU tests run the HIP path
-through the C ABI against the oracle.
+CPU tests pin the oracle's closed-form Neo-Hookean elements (central differences, rest
+state, affine deformations, the reference's own fem_integrator and NewtonsMethod outputs)
+and check the structured decomposition on the host against the generic expansion and the
+oracle. GPU tests run the HIP path through the C ABI against the oracle and the reference.
 """
 import os
 
@@ -13,7 +14,7 @@ import numpy as np
 import pytest
 
 import symx_amd
-from scene_np import HEX_NODES, hex_block
+from scene_np import HEX_NODES, TET10_NODES, hex_block
 
 
 def tol_rel(a, b):
@@ -21,11 +22,13 @@ def tol_rel(a, b):
 
 
 def ref_element(nn, scale=0.05):
+    if nn == 10:
+        return np.array(TET10_NODES, float) * scale  # a right tetrahedron with legs 2 * scale
     return np.array(HEX_NODES[:nn], float) * scale  # a cube of side 2 * scale
 
 
 # ---- CPU: oracle ---------------------------------------------------------------------------
-@pytest.mark.parametrize("nn", [8, 27])
+@pytest.mark.parametrize("nn", [8, 10, 27])
 def test_oracle_hex_matches_finite_differences(oracle, nn):
     rng = np.random.default_rng(nn)
     X = ref_element(nn)
@@ -49,7 +52,7 @@ def test_oracle_hex_matches_finite_differences(oracle, nn):
     assert np.abs(H - H.T).max() <= 1e-12 * np.abs(H).max()
 
 
-@pytest.mark.parametrize("nn", [8, 27])
+@pytest.mark.parametrize("nn", [8, 10, 27])
 def test_oracle_hex_affine_deformation_closed_form(oracle, nn):
     """x = A X: F = A at every quadrature point, so E = V psi(A), g sums to zero
     (translation invariance) and the rest state has E = 0, g = 0."""
@@ -59,17 +62,22 @@ def test_oracle_hex_affine_deformation_closed_form(oracle, nn):
     assert abs(E0) < 1e-12 and np.abs(g0).max() < 1e-9
     A = np.array([[1.1, 0.05, 0.0], [0.02, 0.95, 0.03], [0.0, -0.04, 1.02]])
     E, g, _ = oracle.nh_hex(X @ A.T, X, lam, mu)
-    V = 0.1 ** 3
+    V = 0.1 ** 3 / (6.0 if nn == 10 else 1.0)
     J = np.linalg.det(A)
     psi = 0.5 * mu * (np.sum(A * A) - 3) - mu * np.log(J) + 0.5 * lam * np.log(J) ** 2
-    # the truncated 10-digit quadrature weights sum to 8 only to about 1e-10
-    assert abs(E - V * psi) <= 1e-9 * abs(V * psi)
+    # the truncated 10-digit Hex27 weights sum to 8 only to about 1e-10; the Tet10 rule's
+    # four weights 0.041666667 sum to 1/6 (1 + 1.2e-8)
+    if nn == 10:
+        assert abs(E / (V * psi) - 4 * 0.041666667 * 6) <= 1e-12
+    else:
+        assert abs(E - V * psi) <= 1e-9 * abs(V * psi)
     assert np.abs(g.reshape(-1, 3).sum(axis=0)).max() <= 1e-9 * np.abs(g).max()
 
 
 # ---- CPU: structured decomposition on the host ---------------------------------------------
-def test_structured_hex8_matches_generic_expansion():
-    p = symx_amd.default_hex_params(cells=[2, 2, 2], nodes_per_element=8)
+@pytest.mark.parametrize("nn", [8, 10])
+def test_structured_hex8_matches_generic_expansion(nn):
+    p = symx_amd.default_hex_params(cells=[2, 2, 2], nodes_per_element=nn)
     for e in [0, 5]:
         E, g, H, st = symx_amd.host_hex_element(p, e, structured=True)
         E2, g2, H2, st2 = symx_amd.host_hex_element(p, e, structured=False)
@@ -78,7 +86,7 @@ def test_structured_hex8_matches_generic_expansion():
         assert tol_rel(g, g2) < 1e-13 and tol_rel(H, H2) < 1e-13
 
 
-@pytest.mark.parametrize("nn", [8, 27])
+@pytest.mark.parametrize("nn", [8, 10, 27])
 def test_structured_hex_matches_oracle(oracle, nn):
     p = symx_amd.default_hex_params(cells=[2, 2, 2], nodes_per_element=nn)
     S = hex_block(cells=(2, 2, 2), nn=nn, noise=p.noise, seed=p.seed)
@@ -91,7 +99,7 @@ def test_structured_hex_matches_oracle(oracle, nn):
         assert tol_rel(g, go) < 1e-12 and tol_rel(H, Ho) < 1e-12
 
 
-@pytest.mark.parametrize("nn", [8, 27])
+@pytest.mark.parametrize("nn", [8, 10, 27])
 def test_structured_hex_modules_compile_for_gfx950(tmp_path, nn):
     n, code, n_structured = symx_amd.host_hex_jit(symx_amd.default_hex_params(cells=[2, 2, 2], nodes_per_element=nn), str(tmp_path))
     assert n == 3 and code > 0 and n_structured == 1
@@ -119,7 +127,7 @@ def hex_scene(nn, cells=(2, 2, 2), precision=symx_amd.FP64, **kw):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nn", [8, 27])
+@pytest.mark.parametrize("nn", [8, 10, 27])
 def test_hex_scene_inputs_match_numpy_restatement(gpu, nn):
     s, p = hex_scene(nn, cells=(3, 2, 2))
     S = hex_block(cells=(3, 2, 2), nn=nn, noise=p.noise, seed=p.seed)
@@ -130,7 +138,7 @@ def test_hex_scene_inputs_match_numpy_restatement(gpu, nn):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nn", [8, 27])
+@pytest.mark.parametrize("nn", [8, 10, 27])
 def test_hex_element_outputs_match_oracle(gpu, oracle, nn):
     s, p = hex_scene(nn)
     S = hex_block(cells=(2, 2, 2), nn=nn, noise=p.noise, seed=p.seed)
@@ -146,7 +154,7 @@ def test_hex_element_outputs_match_oracle(gpu, oracle, nn):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nn", [8, 27])
+@pytest.mark.parametrize("nn", [8, 10, 27])
 @pytest.mark.parametrize("mirror", [False, True])
 def test_hex_projection_matches_oracle(gpu, oracle, nn, mirror):
     # a strongly perturbed state (every element indefinite, none inverted)
@@ -177,7 +185,7 @@ def test_hex_projection_matches_oracle(gpu, oracle, nn, mirror):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nn,mode", [(8, 1), (27, 1), (27, 2)])
+@pytest.mark.parametrize("nn,mode", [(8, 1), (10, 1), (10, 2), (27, 1), (27, 2)])
 def test_hex_newton_matches_oracle(gpu, oracle, nn, mode):
     s, p = hex_scene(nn)
     s.set_newton(symx_amd.default_newton_params(precision=symx_amd.FP64, projection_mode=mode))
@@ -243,6 +251,76 @@ def test_hex27_newton_matches_reference_solve(gpu, precision):
     # two solves stopped at a 1e-6 relative residual differ by the error that residual
     # leaves (5.7e-6 of the displacement measured for the mixed path); the FP64 path takes
     # the reference's iterates and agrees far closer
+    tol = 1e-6 if precision == symx_amd.FP64 else 1e-4
+    assert np.abs(s.get("x") - R["x"]).max() <= tol * disp
+    s.close()
+
+
+# ---- the reference's own Tet10 (tests/golden/tet10_*, written by oracle/_ref) ----------------
+def test_oracle_tet10_matches_reference_fem_integrator(oracle):
+    """E / gradient / Hessian of the reference's fem_integrator(FEM_Element::Tet10)
+    (fem_integrators.h:56-77 shape functions, fem_integrators.cpp:60-68 4-point rule with its
+    8-digit constants) evaluated by its SecondOrderCompiledGlobal, on rotated, stretched and
+    jittered quadratic tetrahedra of sizes 1e-2..1 with random materials: the oracle's closed
+    form agrees to FP64 round-off."""
+    G = np.load(os.path.join(GOLDEN, "tet10_ref_elements.npz"))
+    for i in range(G["E"].shape[0]):
+        E, g, H = oracle.nh_hex(G["x"][i], G["X"][i], G["lam"][i], G["mu"][i])
+        assert abs(E - G["E"][i]) <= 1e-12 * abs(G["E"][i]), i
+        assert tol_rel(g, G["g"][i]) < 1e-11, i
+        assert tol_rel(H, G["H"][i]) < 1e-11, i
+
+
+def test_tet10_mesh_geometry():
+    """The Tet10 cuboid mesh (restated in scene_np; the product's mesh equals it, checked
+    through the element outputs above and the scene inputs on the GPU): every mid-edge node
+    halves its edge, every element has positive volume, the elements fill the cuboid and
+    every lattice node is used."""
+    from scene_np import tet10_mesh
+
+    X, T = tet10_mesh((2, 3, 1), (1.0, 1.5, 0.5))
+    edges = [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]
+    for m, (a, b) in enumerate(edges):
+        assert np.allclose(X[T[:, 4 + m]], 0.5 * (X[T[:, a]] + X[T[:, b]]), atol=1e-15)
+    V = np.linalg.det(np.stack([X[T[:, k]] - X[T[:, 0]] for k in (1, 2, 3)], axis=1)) / 6.0
+    assert V.min() > 0 and abs(V.sum() - 0.75) <= 1e-14
+    assert len(np.unique(T)) == X.shape[0]
+
+
+def test_oracle_tet10_newton_matches_reference_solve(oracle):
+    """One implicit step of the 2^3-cell Tet10 block (48 elements, per-element Young's moduli,
+    ProjectOnDemand, 1e-6 relative residual) by the oracle against the reference's own
+    NewtonsMethod on the same inputs (tet10_ref_block2.npz): same outcome and Newton count,
+    CG count within one per Newton iteration, positions within 1e-6 of the displacement."""
+    from symx_amd import NewtonParams
+
+    R = np.load(os.path.join(GOLDEN, "tet10_ref_block2.npz"))
+    S = hex_block(cells=tuple(R["cells"]), nn=10)
+    newton = NewtonParams(0, int(R["projection_mode"]), 0, 100, 0.0, float(R["rel_tol"]), 1e-3, 1e-12, 10000, 1e-10, 0.0)
+    res = oracle.newton(S, newton)
+    assert res["result"] == int(R["result"]) == 0
+    assert res["newton_iterations"] == int(R["newton_iterations"])
+    assert abs(res["cg_iterations"] - int(R["cg_iterations"])) <= res["newton_iterations"]
+    disp = np.abs(R["x"] - S.X).max()
+    assert np.abs(res["x"] - R["x"]).max() <= 1e-6 * disp
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [symx_amd.FP64, symx_amd.MIXED])
+def test_tet10_newton_matches_reference_solve(gpu, precision):
+    """The HIP path on the 2^3-cell Tet10 block against the reference's NewtonsMethod
+    (tet10_ref_block2.npz), with the bounds of the Hex27 comparison above."""
+    R = np.load(os.path.join(GOLDEN, "tet10_ref_block2.npz"))
+    s, p = hex_scene(10, cells=tuple(R["cells"]))
+    s.set_newton(symx_amd.default_newton_params(precision=precision, projection_mode=int(R["projection_mode"])))
+    X = s.get("X")
+    st = s.solve()
+    assert st["result_name"] == "Successful" and int(R["result"]) == 0
+    if precision == symx_amd.FP64:
+        assert st["newton_iterations"] == int(R["newton_iterations"])
+    else:
+        assert abs(st["newton_iterations"] - int(R["newton_iterations"])) <= 1
+    disp = np.abs(R["x"] - X).max()
     tol = 1e-6 if precision == symx_amd.FP64 else 1e-4
     assert np.abs(s.get("x") - R["x"]).max() <= tol * disp
     s.close()
