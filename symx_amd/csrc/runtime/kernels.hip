@@ -744,26 +744,38 @@ namespace symx_hip
         c1 = n1;
     }
 
-    // One matrix h (element e) with the whole workgroup of PT threads (a multiple of 64);
-    // every early return is uniform. KMAX bounds the eigenvectors held in LDS and PL the
-    // lanes running inverse iteration at once: with more than KMAX eigenvalues below eps the
-    // matrix is left untouched and true is returned (the caller hands it to a variant with
-    // KMAX = N). A smaller LDS footprint lets a CU hold several matrices at once.
-    template <int N, bool DENSE, int PT, typename T, int KMAX = N, int PL = PLU>
-    __device__ bool project_large_one(T* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
-                                      uint8_t* __restrict__ changed_out)
+    // LDS of the phases after the tridiagonalisation (count, multisection, inverse iteration,
+    // back-transform, update)
+    template <int N, int PT, int KMAX, int PL>
+    struct ProjTail
     {
-        constexpr int NB = N / 3, LD = N | 1, NUB = NB * (NB + 1) / 2, NW = PT / 64, NPT = PT * PPTS;
-        static_assert(N <= 128, "one wave holds a vector in two registers per lane");
-        static_assert(KMAX >= 1 && KMAX <= N && PL >= 1 && PL <= 64, "eigenvector and solve lane bounds");
-        __shared__ double A[N * LD];
-        __shared__ double Y[KMAX * N];
-        __shared__ double dg[N], od[N], tau[N], lamv[N], pv[N], wv[N], bra[N], brb[N];
-        __shared__ double lu[3 * N * PL];
-        __shared__ int cbuf[NPT];
-        __shared__ double red[2 * NW];
-        __shared__ int busy[3];
+        double Y[KMAX * N];
+        double dg[N], od[N], tau[N], lamv[N], bra[N], brb[N];
+        double lu[3 * N * PL];
+        int cbuf[PT * PPTS];
+        int busy[3];
+    };
+
+    // Steps 3-7 of the large projection on the tridiagonal T = Q^T A Q in L.dg / L.od (and
+    // L.tau, written and synchronised by the caller). vref(i, k) is entry i >= k + 2 of the
+    // Householder vector of column k (v_{k+1} = 1, zero above). Every early return is uniform;
+    // true: more than KMAX eigenvalues below eps, the matrix is left untouched.
+    template <int N, bool DENSE, int PT, typename T, int KMAX, int PL, typename VREF>
+    __device__ bool project_tail(T* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
+                                 uint8_t* __restrict__ changed_out, bool ti, double scale, ProjTail<N, PT, KMAX, PL>& L, VREF vref)
+    {
+        constexpr int NB = N / 3, NUB = NB * (NB + 1) / 2, NW = PT / 64, NPT = PT * PPTS;
         const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+        double* Y = L.Y;
+        double* dg = L.dg;
+        double* od = L.od;
+        double* tau = L.tau;
+        double* lamv = L.lamv;
+        double* bra = L.bra;
+        double* brb = L.brb;
+        double* lu = L.lu;
+        int* cbuf = L.cbuf;
+        int* busy = L.busy;
         auto ubi = [](int a, int b) { return a * NB - (a * (a - 1)) / 2 + (b - a); };
         // element (r, c) of the upper-block storage, r <= c blockwise
         auto ub_index = [&](int t, int& r, int& c) {
@@ -776,121 +788,6 @@ namespace symx_hip
             r = 3 * a + ij / 3;
             c = 3 * (a + rem) + ij % 3;
         };
-        for (int t = tid; t < N * N; t += PT) {
-            const int r = t / N, c = t - r * N;
-            double v;
-            if (DENSE) v = 0.5 * ((double)h[r * N + c] + (double)h[c * N + r]);
-            else {
-                const int a = r / 3, i = r - 3 * a, b = c / 3, j = c - 3 * b;
-                v = a <= b ? (double)h[ubi(a, b) * 9 + 3 * i + j] : (double)h[ubi(b, a) * 9 + 3 * j + i];
-            }
-            A[r * LD + c] = v;
-        }
-        __syncthreads();
-        auto block_max = [&](double v) {
-            v = wave_max_all(v);
-            if (lane == 0) red[wid] = v;
-            __syncthreads();
-            double r = red[0];
-            for (int w = 1; w < NW; w++) r = fmax(r, red[w]);
-            __syncthreads();
-            return r;
-        };
-        // ---- 1. scale, translation invariance ---------------------------------------------
-        double amax = 0.0, rsum = 0.0;
-        for (int t = tid; t < N * N; t += PT) amax = fmax(amax, fabs(A[(t / N) * LD + t % N]));
-        if (!DENSE) {
-            for (int t = tid; t < N * 3; t += PT) {  // row r, component j: sum over blocks b of A[r][3b + j]
-                const int r = t / 3, j = t - 3 * r;
-                double acc = 0.0;
-                for (int b = 0; b < NB; b++) acc += A[r * LD + 3 * b + j];
-                rsum = fmax(rsum, fabs(acc));
-            }
-        }
-        const double scale = block_max(amax);
-        const bool ti = !DENSE && block_max(rsum) <= translation_tol<T>() * scale * NB && scale > 0.0;
-        if (!(scale > 0.0)) {
-            // zero matrix: every eigenvalue (0) is below eps -> eps I (clamp), 0 (mirror)
-            if (!mirror) {
-                for (int r = tid; r < N; r += PT) {
-                    if (DENSE) h[r * N + r] = eps;
-                    else h[ubi(r / 3, r / 3) * 9 + 4 * (r % 3)] = eps;
-                }
-            }
-            if (tid == 0) {
-                if (count != nullptr) atomicAdd(count, 1ull);
-                if (changed_out != nullptr) changed_out[e] = 1;
-            }
-            return false;
-        }
-        if (ti) {
-            const double sh = scale / NB;
-            for (int t = tid; t < N * N; t += PT) {
-                const int r = t / N, c = t - r * N;
-                if ((r % 3) == (c % 3)) A[r * LD + c] += sh;
-            }
-            __syncthreads();
-        }
-        // ---- 2. Householder tridiagonalisation --------------------------------------------
-        for (int k = 0; k < N - 2; k++) {
-            const int m = N - k - 1;
-            double s = 0.0;
-            for (int i = k + 2 + lane; i < N; i += 64) {
-                const double a = A[i * LD + k];
-                s += a * a;
-            }
-            s = wave_sum_all(s);
-            const double x0 = A[(k + 1) * LD + k];
-            double beta = x0, t = 0.0, invd = 0.0;
-            if (s != 0.0) {
-                beta = -copysign(sqrt(x0 * x0 + s), x0);
-                t = (beta - x0) / beta;
-                invd = 1.0 / (x0 - beta);
-            }
-            if (tid < m) {  // p = t A22 v, column access (A22 symmetric)
-                const int c = k + 1 + tid;
-                double a0 = 0.0, a1 = 0.0;
-                int j = 1;
-                for (; j + 1 < m; j += 2) {
-                    a0 += A[(k + 1 + j) * LD + c] * A[(k + 1 + j) * LD + k];
-                    a1 += A[(k + 2 + j) * LD + c] * A[(k + 2 + j) * LD + k];
-                }
-                if (j < m) a0 += A[(k + 1 + j) * LD + c] * A[(k + 1 + j) * LD + k];
-                pv[tid] = t * (A[(k + 1) * LD + c] + invd * (a0 + a1));
-                wv[tid] = tid == 0 ? 1.0 : A[c * LD + k] * invd;
-            }
-            __syncthreads();
-            double kk = 0.0;
-            for (int i = lane; i < m; i += 64) kk += pv[i] * wv[i];
-            kk = 0.5 * t * wave_sum_all(kk);
-            if (t != 0.0) {
-                const int step = PT / m;
-                if (tid < step * m) {
-                    const int j = tid % m;
-                    const double vj = wv[j], wj = pv[j] - kk * vj;
-                    for (int i = tid / m; i < m; i += step) {
-                        const double vi = wv[i], wi = pv[i] - kk * vi;
-                        A[(k + 1 + i) * LD + k + 1 + j] -= vi * wj + wi * vj;
-                    }
-                }
-            }
-            if (tid >= 1 && tid < m) A[(k + 1 + tid) * LD + k] = wv[tid];
-            if (tid == 0) {
-                dg[k] = A[k * LD + k];
-                od[k] = beta;
-                tau[k] = t;
-            }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            dg[N - 2] = A[(N - 2) * LD + N - 2];
-            od[N - 2] = A[(N - 1) * LD + N - 2];
-            dg[N - 1] = A[(N - 1) * LD + N - 1];
-            od[N - 1] = 0.0;
-            tau[N - 2] = 0.0;
-            tau[N - 1] = 0.0;
-        }
-        __syncthreads();
         // ---- 3. count below eps -----------------------------------------------------------
         double tn = 0.0, glo = 0.0, emax = 0.0;
         {
@@ -1072,10 +969,10 @@ namespace symx_hip
             for (int k = N - 3; k >= 0; k--) {
                 const double t = tau[k];
                 if (t == 0.0) continue;
-                // v: 0 above k + 1, 1 at k + 1, A[i][k] below
-                const double v0 = lane < k + 1 || lane >= N ? 0.0 : (lane == k + 1 ? 1.0 : A[lane * LD + k]);
+                // v: 0 above k + 1, 1 at k + 1, vref below
+                const double v0 = lane < k + 1 || lane >= N ? 0.0 : (lane == k + 1 ? 1.0 : vref(lane, k));
                 const int i1 = lane + 64;
-                const double v1 = i1 < k + 1 || i1 >= N ? 0.0 : (i1 == k + 1 ? 1.0 : A[i1 * LD + k]);
+                const double v1 = i1 < k + 1 || i1 >= N ? 0.0 : (i1 == k + 1 ? 1.0 : vref(i1, k));
                 const double d = t * wave_sum_all(v0 * y0 + v1 * y1);
                 y0 -= d * v0;
                 y1 -= d * v1;
@@ -1110,6 +1007,399 @@ namespace symx_hip
             if (changed_out != nullptr) changed_out[e] = 1;
         }
         return false;
+    }
+
+
+    // One matrix h (element e) with the whole workgroup of PT threads (a multiple of 64);
+    // every early return is uniform. KMAX bounds the eigenvectors held in LDS and PL the
+    // lanes running inverse iteration at once: with more than KMAX eigenvalues below eps the
+    // matrix is left untouched and true is returned (the caller hands it to a variant with
+    // KMAX = N). A smaller LDS footprint lets a CU hold several matrices at once.
+    template <int N, bool DENSE, int PT, typename T, int KMAX = N, int PL = PLU>
+    __device__ bool project_large_one(T* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
+                                      uint8_t* __restrict__ changed_out)
+    {
+        constexpr int NB = N / 3, LD = N | 1, NW = PT / 64;
+        static_assert(N <= 128, "one wave holds a vector in two registers per lane");
+        static_assert(KMAX >= 1 && KMAX <= N && PL >= 1 && PL <= 64, "eigenvector and solve lane bounds");
+        __shared__ double A[N * LD];
+        __shared__ double pv[N], wv[N];
+        __shared__ double red[2 * NW];
+        __shared__ ProjTail<N, PT, KMAX, PL> L;
+        double* dg = L.dg;
+        double* od = L.od;
+        double* tau = L.tau;
+        const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+        auto ubi = [](int a, int b) { return a * NB - (a * (a - 1)) / 2 + (b - a); };
+        for (int t = tid; t < N * N; t += PT) {
+            const int r = t / N, c = t - r * N;
+            double v;
+            if (DENSE) v = 0.5 * ((double)h[r * N + c] + (double)h[c * N + r]);
+            else {
+                const int a = r / 3, i = r - 3 * a, b = c / 3, j = c - 3 * b;
+                v = a <= b ? (double)h[ubi(a, b) * 9 + 3 * i + j] : (double)h[ubi(b, a) * 9 + 3 * j + i];
+            }
+            A[r * LD + c] = v;
+        }
+        __syncthreads();
+        auto block_max = [&](double v) {
+            v = wave_max_all(v);
+            if (lane == 0) red[wid] = v;
+            __syncthreads();
+            double r = red[0];
+            for (int w = 1; w < NW; w++) r = fmax(r, red[w]);
+            __syncthreads();
+            return r;
+        };
+        // ---- 1. scale, translation invariance ---------------------------------------------
+        double amax = 0.0, rsum = 0.0;
+        for (int t = tid; t < N * N; t += PT) amax = fmax(amax, fabs(A[(t / N) * LD + t % N]));
+        if (!DENSE) {
+            for (int t = tid; t < N * 3; t += PT) {  // row r, component j: sum over blocks b of A[r][3b + j]
+                const int r = t / 3, j = t - 3 * r;
+                double acc = 0.0;
+                for (int b = 0; b < NB; b++) acc += A[r * LD + 3 * b + j];
+                rsum = fmax(rsum, fabs(acc));
+            }
+        }
+        const double scale = block_max(amax);
+        const bool ti = !DENSE && block_max(rsum) <= translation_tol<T>() * scale * NB && scale > 0.0;
+        if (!(scale > 0.0)) {
+            // zero matrix: every eigenvalue (0) is below eps -> eps I (clamp), 0 (mirror)
+            if (!mirror) {
+                for (int r = tid; r < N; r += PT) {
+                    if (DENSE) h[r * N + r] = eps;
+                    else h[ubi(r / 3, r / 3) * 9 + 4 * (r % 3)] = eps;
+                }
+            }
+            if (tid == 0) {
+                if (count != nullptr) atomicAdd(count, 1ull);
+                if (changed_out != nullptr) changed_out[e] = 1;
+            }
+            return false;
+        }
+        if (ti) {
+            const double sh = scale / NB;
+            for (int t = tid; t < N * N; t += PT) {
+                const int r = t / N, c = t - r * N;
+                if ((r % 3) == (c % 3)) A[r * LD + c] += sh;
+            }
+            __syncthreads();
+        }
+        // ---- 2. Householder tridiagonalisation --------------------------------------------
+        for (int k = 0; k < N - 2; k++) {
+            const int m = N - k - 1;
+            double s = 0.0;
+            for (int i = k + 2 + lane; i < N; i += 64) {
+                const double a = A[i * LD + k];
+                s += a * a;
+            }
+            s = wave_sum_all(s);
+            const double x0 = A[(k + 1) * LD + k];
+            double beta = x0, t = 0.0, invd = 0.0;
+            if (s != 0.0) {
+                beta = -copysign(sqrt(x0 * x0 + s), x0);
+                t = (beta - x0) / beta;
+                invd = 1.0 / (x0 - beta);
+            }
+            if (tid < m) {  // p = t A22 v, column access (A22 symmetric)
+                const int c = k + 1 + tid;
+                double a0 = 0.0, a1 = 0.0;
+                int j = 1;
+                for (; j + 1 < m; j += 2) {
+                    a0 += A[(k + 1 + j) * LD + c] * A[(k + 1 + j) * LD + k];
+                    a1 += A[(k + 2 + j) * LD + c] * A[(k + 2 + j) * LD + k];
+                }
+                if (j < m) a0 += A[(k + 1 + j) * LD + c] * A[(k + 1 + j) * LD + k];
+                pv[tid] = t * (A[(k + 1) * LD + c] + invd * (a0 + a1));
+                wv[tid] = tid == 0 ? 1.0 : A[c * LD + k] * invd;
+            }
+            __syncthreads();
+            double kk = 0.0;
+            for (int i = lane; i < m; i += 64) kk += pv[i] * wv[i];
+            kk = 0.5 * t * wave_sum_all(kk);
+            if (t != 0.0) {
+                const int step = PT / m;
+                if (tid < step * m) {
+                    const int j = tid % m;
+                    const double vj = wv[j], wj = pv[j] - kk * vj;
+                    for (int i = tid / m; i < m; i += step) {
+                        const double vi = wv[i], wi = pv[i] - kk * vi;
+                        A[(k + 1 + i) * LD + k + 1 + j] -= vi * wj + wi * vj;
+                    }
+                }
+            }
+            if (tid >= 1 && tid < m) A[(k + 1 + tid) * LD + k] = wv[tid];
+            if (tid == 0) {
+                dg[k] = A[k * LD + k];
+                od[k] = beta;
+                tau[k] = t;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            dg[N - 2] = A[(N - 2) * LD + N - 2];
+            od[N - 2] = A[(N - 1) * LD + N - 2];
+            dg[N - 1] = A[(N - 1) * LD + N - 1];
+            od[N - 1] = 0.0;
+            tau[N - 2] = 0.0;
+            tau[N - 1] = 0.0;
+        }
+        __syncthreads();
+        return project_tail<N, DENSE, PT, T, KMAX, PL>(h, e, eps, mirror, count, changed_out, ti, scale, L,
+                                                        [&](int i, int k) { return A[i * LD + k]; });
+    }
+
+    // Sum over the 16 lanes of a DPP row (row rotations by 8, 4, 2, 1): every lane of the row
+    // gets the same, bitwise equal sum (each step adds two equal pairs in swapped order). All
+    // lanes must be active.
+    __device__ inline double row16_sum(double v)
+    {
+        v += dpp_mov<0x128>(v);  // row_ror:8
+        v += dpp_mov<0x124>(v);  // row_ror:4
+        v += dpp_mov<0x122>(v);  // row_ror:2
+        v += dpp_mov<0x121>(v);  // row_ror:1
+        return v;
+    }
+
+    // project_large_one for the upper-block storage of large elements (Hex27: 81 x 81) with
+    // the matrix in registers during the tridiagonalisation instead of in LDS. Thread
+    // (ti, tj) of a 16 x 16 grid (ti = 4 wave + lane / 16, so the 16 threads of a DPP row share
+    // ti; tj = lane % 16) holds rows ti + 16 q and columns tj + 16 s (q, s < TQ) in a[q][s]:
+    //  - A22 v is a 36-FMA partial product per thread summed over its DPP row (no barrier);
+    //  - the rank-2 update touches only q, s >= (k + 1) / 16, the row and column groups that
+    //    still reach the active block, so the work shrinks with it;
+    //  - two barriers per column (p, then the next column), against three LDS round trips per
+    //    column entry in the LDS form; the Householder vectors go to a packed LDS array for
+    //    the back-transform (25 KB instead of the 52 KB matrix), three workgroups per CU.
+    // The arithmetic is that of project_large_one (dsytd2), summed in another order.
+    template <int N, typename T, int KMAX, int PL>
+    __device__ bool project_tiled_one(T* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
+                                      uint8_t* __restrict__ changed_out)
+    {
+        constexpr int PT = 256, NW = PT / 64, TQ = (N + 15) / 16, NB = N / 3;
+        constexpr int NV = (N - 1) * (N - 2) / 2;  // column k holds rows k + 2 .. N - 1
+        static_assert(N % 3 == 0 && N <= 128 && TQ * 16 >= N, "tile bounds");
+        __shared__ double V[NV];
+        __shared__ double xb[2][128];
+        __shared__ double pb[128];
+        __shared__ double red[2 * NW];
+        __shared__ ProjTail<N, PT, KMAX, PL> L;
+        const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+        const int ti = wid * 4 + (lane >> 4), tj = lane & 15;
+        auto ubi = [](int a, int b) { return a * NB - (a * (a - 1)) / 2 + (b - a); };
+        auto voff = [](int k) { return k * (N - 2) - (k * (k - 1)) / 2; };
+        double a[TQ][TQ];
+#pragma unroll
+        for (int q = 0; q < TQ; q++) {
+#pragma unroll
+            for (int s = 0; s < TQ; s++) {
+                const int r = ti + 16 * q, c = tj + 16 * s;
+                double v = 0.0;
+                if (r < N && c < N) {
+                    const int A_ = r / 3, i = r - 3 * A_, B_ = c / 3, j = c - 3 * B_;
+                    v = A_ <= B_ ? (double)h[ubi(A_, B_) * 9 + 3 * i + j] : (double)h[ubi(B_, A_) * 9 + 3 * j + i];
+                }
+                a[q][s] = v;
+            }
+        }
+        auto block_max = [&](double v) {
+            v = wave_max_all(v);
+            if (lane == 0) red[wid] = v;
+            __syncthreads();
+            double r = red[0];
+            for (int w = 1; w < NW; w++) r = fmax(r, red[w]);
+            __syncthreads();
+            return r;
+        };
+        // ---- 1. scale, translation invariance (block-row sums per component) ---------------
+        double amax = 0.0, rsum = 0.0;
+#pragma unroll
+        for (int q = 0; q < TQ; q++) {
+            double ps0 = 0.0, ps1 = 0.0, ps2 = 0.0;
+#pragma unroll
+            for (int s = 0; s < TQ; s++) {
+                const int c = tj + 16 * s, cm = c % 3;
+                amax = fmax(amax, fabs(a[q][s]));
+                ps0 += cm == 0 ? a[q][s] : 0.0;
+                ps1 += cm == 1 ? a[q][s] : 0.0;
+                ps2 += cm == 2 ? a[q][s] : 0.0;
+            }
+            ps0 = row16_sum(ps0);
+            ps1 = row16_sum(ps1);
+            ps2 = row16_sum(ps2);
+            if (ti + 16 * q < N) rsum = fmax(rsum, fmax(fabs(ps0), fmax(fabs(ps1), fabs(ps2))));
+        }
+        const double scale = block_max(amax);
+        const bool tinv = block_max(rsum) <= translation_tol<T>() * scale * NB && scale > 0.0;
+        if (!(scale > 0.0)) {
+            // zero matrix: every eigenvalue (0) is below eps -> eps I (clamp), 0 (mirror)
+            if (!mirror)
+                for (int r = tid; r < N; r += PT) h[ubi(r / 3, r / 3) * 9 + 4 * (r % 3)] = eps;
+            if (tid == 0) {
+                if (count != nullptr) atomicAdd(count, 1ull);
+                if (changed_out != nullptr) changed_out[e] = 1;
+            }
+            return false;
+        }
+        if (tinv) {
+            const double sh = scale / NB;
+#pragma unroll
+            for (int q = 0; q < TQ; q++)
+#pragma unroll
+                for (int s = 0; s < TQ; s++) {
+                    const int r = ti + 16 * q, c = tj + 16 * s;
+                    if (r < N && c < N && (r % 3) == (c % 3)) a[q][s] += sh;
+                }
+        }
+        // ---- 2. Householder tridiagonalisation in registers --------------------------------
+        // reg(q, s): a[q][s] for a wave-uniform s
+        auto col_of = [&](int q, int sc) {
+            double v = 0.0;
+#pragma unroll
+            for (int s = 0; s < TQ; s++)
+                if (s == sc) v = a[q][s];
+            return v;
+        };
+        // xb / pb entries are read without bounds tests (lane + 64 < 128) and masked by
+        // selects, so they start as zeros
+        for (int i = tid; i < 128; i += PT) {
+            xb[0][i] = 0.0;
+            xb[1][i] = 0.0;
+            pb[i] = 0.0;
+        }
+        __syncthreads();
+        if (tj == 0) {
+#pragma unroll
+            for (int q = 0; q < TQ; q++)
+                if (ti + 16 * q < N) xb[0][ti + 16 * q] = a[q][0];
+        }
+        __syncthreads();
+        // Column k of the reduction, for q0 = (k + 1) / 16 fixed at compile time: the groups
+        // q, s < q0 lie wholly above row k + 1 and drop out of every loop.
+        auto step = [&](auto q0c, int k) {
+            constexpr int Q0 = decltype(q0c)::value;
+            // every LDS read of the step's first half in one batch: column k (rows k + 1 ..
+            // N - 1) at the lanes (for the norm), the own rows and columns, and x_{k+1}
+            const double* x = xb[k & 1];
+            const double xl0 = x[lane], xl1 = x[lane + 64], x0 = x[k + 1];
+            double xr[TQ], xc[TQ];
+#pragma unroll
+            for (int q = Q0; q < TQ; q++) {
+                xr[q] = x[ti + 16 * q];
+                xc[q] = x[tj + 16 * q];
+            }
+            double ss = (lane >= k + 2 ? xl0 * xl0 : 0.0) + (lane + 64 >= k + 2 && lane + 64 < N ? xl1 * xl1 : 0.0);
+            ss = wave_sum_all(ss);
+            double beta = x0, t = 0.0, invd = 0.0;
+            if (ss != 0.0) {
+                beta = -copysign(sqrt(x0 * x0 + ss), x0);
+                t = (beta - x0) / beta;
+                invd = 1.0 / (x0 - beta);
+            }
+            // v: 0 up to k, 1 at k + 1, x invd below
+            auto vsel = [&](int i, double xi) { return i == k + 1 ? 1.0 : (i > k + 1 && i < N ? xi * invd : 0.0); };
+            if (wid == 0) {
+                if (lane >= k + 2 && lane < N) V[voff(k) + lane - k - 2] = xl0 * invd;
+                if (lane + 64 >= k + 2 && lane + 64 < N) V[voff(k) + lane + 64 - k - 2] = xl1 * invd;
+            }
+            if (t != 0.0) {
+                double vr[TQ], vc[TQ], p[TQ], pc[TQ];
+#pragma unroll
+                for (int q = Q0; q < TQ; q++) {
+                    vr[q] = vsel(ti + 16 * q, xr[q]);
+                    vc[q] = vsel(tj + 16 * q, xc[q]);
+                }
+                // p = t A22 v for the own rows
+#pragma unroll
+                for (int q = Q0; q < TQ; q++) {
+                    double acc = a[q][Q0] * vc[Q0];
+#pragma unroll
+                    for (int s = Q0 + 1; s < TQ; s++) acc += a[q][s] * vc[s];
+                    p[q] = t * row16_sum(acc);
+                }
+                if (tj == 0) {
+#pragma unroll
+                    for (int q = Q0; q < TQ; q++)
+                        if (ti + 16 * q < N) pb[ti + 16 * q] = p[q];
+                }
+                __syncthreads();
+                const double pl0 = pb[lane], pl1 = pb[lane + 64];
+#pragma unroll
+                for (int q = Q0; q < TQ; q++) pc[q] = pb[tj + 16 * q];
+                const double vl0 = vsel(lane, xl0), vl1 = vsel(lane + 64, xl1);
+                double kk = (lane > k ? pl0 * vl0 : 0.0) + (lane + 64 > k && lane + 64 < N ? pl1 * vl1 : 0.0);
+                kk = 0.5 * t * wave_sum_all(kk);
+#pragma unroll
+                for (int q = Q0; q < TQ; q++) {
+                    const int r = ti + 16 * q, c = tj + 16 * q;
+                    p[q] = r > k && r < N ? p[q] - kk * vr[q] : 0.0;    // w, own rows
+                    pc[q] = c > k && c < N ? pc[q] - kk * vc[q] : 0.0;  // w, own columns
+                }
+#pragma unroll
+                for (int q = Q0; q < TQ; q++)
+#pragma unroll
+                    for (int s = Q0; s < TQ; s++) a[q][s] -= vr[q] * pc[s] + p[q] * vc[s];
+            }
+            if (ti == (k & 15) && tj == (k & 15)) {
+                if constexpr (Q0 > 0) L.dg[k] = (k >> 4) == Q0 ? a[Q0][Q0] : a[Q0 - 1][Q0 - 1];
+                else L.dg[k] = a[0][0];
+            }
+            if (tid == 0) {
+                L.od[k] = beta;
+                L.tau[k] = t;
+            }
+            // column k + 1 (rows below it; in column group Q0) for the next step
+            if (tj == ((k + 1) & 15)) {
+#pragma unroll
+                for (int q = 0; q < TQ; q++)
+                    if (ti + 16 * q < N) xb[(k + 1) & 1][ti + 16 * q] = a[q][Q0];
+            }
+            __syncthreads();
+        };
+        static_for<0, TQ>([&](auto q0c) {
+            constexpr int Q0 = decltype(q0c)::value;
+            const int kb = Q0 == 0 ? 0 : 16 * Q0 - 1;
+            const int ke = 16 * Q0 + 15 < N - 2 ? 16 * Q0 + 15 : N - 2;
+            for (int k = kb; k < ke; k++) step(q0c, k);
+        });
+        {
+            // the last 2 x 2 block: (N-2, N-2), (N-1, N-2), (N-1, N-1)
+            const int r0 = N - 2, r1 = N - 1;
+            if (tj == (r0 & 15)) {
+#pragma unroll
+                for (int q = 0; q < TQ; q++) {
+                    const int r = ti + 16 * q;
+                    if (r == r0) L.dg[r0] = col_of(q, r0 >> 4);
+                    if (r == r1) L.od[r0] = col_of(q, r0 >> 4);
+                }
+            }
+            if (tj == (r1 & 15)) {
+#pragma unroll
+                for (int q = 0; q < TQ; q++)
+                    if (ti + 16 * q == r1) L.dg[r1] = col_of(q, r1 >> 4);
+            }
+            if (tid == 0) {
+                L.od[N - 1] = 0.0;
+                L.tau[N - 2] = 0.0;
+                L.tau[N - 1] = 0.0;
+            }
+        }
+        __syncthreads();
+        return project_tail<N, false, PT, T, KMAX, PL>(h, e, eps, mirror, count, changed_out, tinv, scale, L,
+                                                       [&](int i, int k) { return V[voff(k) + i - k - 2]; });
+    }
+
+    template <int N, typename T, int KMAX, int PL>
+    __global__ __launch_bounds__(PNT, 3) void k_project_tiled(T* __restrict__ H, int64_t n_mats, int64_t rs, double eps, int mirror,
+                                                             unsigned long long* count, const uint8_t* __restrict__ select,
+                                                             uint8_t* __restrict__ too_many)
+    {
+        const int64_t e = blockIdx.x;
+        if (e >= n_mats) return;
+        bool left = false;
+        if (select == nullptr || select[e]) left = project_tiled_one<N, T, KMAX, PL>(H + e * rs, e, eps, mirror, count, nullptr);
+        if (too_many != nullptr && threadIdx.x == 0) too_many[e] = left ? 1 : 0;
     }
 
     // too_many (nullable): 1 for each matrix left to the full-size variant (KMAX < N), 0 for
@@ -1179,7 +1469,7 @@ namespace symx_hip
             // lanes keep the LDS at 73 KB, two matrices per CU (the full-size variant takes
             // 143 KB, one per CU); Hex27 elements have at most 3 such eigenvalues as a rule, the
             // others are listed and projected by the full-size variant
-            k_project_large<81, false, T, 8, 4><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select, nullptr, flags);
+            k_project_tiled<81, T, 8, 4><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select, flags);
             k_zero_counter<<<1, 1, 0, s>>>(n_list);
             k_list_flagged<<<grid_for(n_mats), NT, 0, s>>>(flags, n_mats, list, n_list);
             k_project_large_list<81, PNT, T><<<1024, PNT, 0, s>>>(H, rs, list, n_list, eps, mirror, d_count);
