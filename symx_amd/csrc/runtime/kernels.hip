@@ -744,6 +744,22 @@ namespace symx_hip
         c1 = n1;
     }
 
+    // Number of eigenvalues of the tridiagonal (dg, od) below x (sturm2 with one point).
+    template <int N>
+    __device__ inline int sturm1(const double* dg, const double* od, double x, double pivmin)
+    {
+        double q = dg[0] - x;
+        if (fabs(q) < pivmin) q = -pivmin;
+        int n = q < 0.0;
+        for (int i = 1; i < N; i++) {
+            const double e = od[i - 1];
+            q = (dg[i] - x) - e * e * rcp_nr1(q);  // |q| >= pivmin keeps e2 / q finite
+            if (fabs(q) < pivmin) q = -pivmin;
+            n += q < 0.0;
+        }
+        return n;
+    }
+
     // LDS of the phases after the tridiagonalisation (count, multisection, inverse iteration,
     // back-transform, update)
     template <int N, int PT, int KMAX, int PL>
@@ -752,6 +768,8 @@ namespace symx_hip
         double Y[KMAX * N];
         double dg[N], od[N], tau[N], lamv[N], bra[N], brb[N];
         double lu[3 * N * PL];
+        double lm[N * PL];
+        uint8_t lsw[N * PL];
         int cbuf[PT * PPTS];
         int busy[3];
     };
@@ -842,20 +860,16 @@ namespace symx_hip
                 brb[j] = lo < NPT ? glo + w0 * (lo + 1) : cut;
             }
             __syncthreads();
-            const int P = min(64, NPT / cnt);
+            // later rounds: one point per thread, P per eigenvalue still above tol (the round's
+            // cost is the instructions of every wave holding a point, so the points go one to
+            // a lane: 192 points for three eigenvalues occupy three waves, not two at twice
+            // the length)
+            const int P = min(64, PT / cnt);
             for (int round = 0; round < 16; round++) {
-                int pts[PPTS];
-                double xs[PPTS];
-#pragma unroll
-                for (int q = 0; q < PPTS; q++) {
-                    const int p = tid * PPTS + q, j = p / P, l = p - j * P;
-                    pts[q] = j < cnt ? p : -1;
-                    xs[q] = j < cnt ? bra[j] + (brb[j] - bra[j]) * ((double)(l + 1) / (P + 1)) : 0.0;
-                }
-                if (pts[0] >= 0) {
-                    sturm2<N>(dg, od, xs[0], xs[1], pivmin, c0, c1);
-                    cbuf[pts[0]] = c0;
-                    if (pts[1] >= 0) cbuf[pts[1]] = c1;
+                {
+                    const int j = tid / P, l = tid - j * P;
+                    if (j < cnt && brb[j] - bra[j] > tol)
+                        cbuf[tid] = sturm1<N>(dg, od, bra[j] + (brb[j] - bra[j]) * ((double)(l + 1) / (P + 1)), pivmin);
                 }
                 if (tid == 0) busy[(round + 1) % 3] = 0;
                 __syncthreads();
@@ -892,6 +906,8 @@ namespace symx_hip
             double* ur = lu;
             double* u1 = lu + N * PL;
             double* u2 = lu + 2 * N * PL;
+            double* lm = L.lm;
+            uint8_t* lsw = L.lsw;
             const double ptol = ulp * tn, bigr = 1.0 / ptol;
             for (int t = tid; t < cnt * N; t += PT) {
                 const int j = t / N;
@@ -923,28 +939,45 @@ namespace symx_hip
             };
             for (int it = 0; it < 3; it++) {
                 if (tid < PL) {
+                    // with at most PL vectors each lane keeps its factors of T - lambda I (the
+                    // shift does not change) and later rounds replay the row interchanges and
+                    // multipliers of the forward pass instead of factoring again
+                    const bool replay = it > 0 && cnt <= PL;
                     for (int j = tid; j < cnt; j += PL) {
                         double* y = Y + j * N;
-                        const double lam = lamv[j];
-                        double a = dg[0] - lam, c = od[0], r0 = y[0];
-                        for (int k = 0; k < N - 1; k++) {
-                            const double b = od[k], an = dg[k + 1] - lam, cn = k + 1 < N - 1 ? od[k + 1] : 0.0;
-                            const double r1 = y[k + 1];
-                            // partial pivoting: rows k and k + 1 interchanged when |b| > |a|
-                            const bool sw = fabs(a) < fabs(b);
-                            const double piv = sw ? b : a, oth = sw ? a : b;
-                            const double inv = fabs(piv) >= 2.2250738585072014e-308 ? rcp_nr(piv) : 0.0;
-                            const double mult = oth * inv;  // |mult| <= 1; a = b = 0 gives 0
-                            u1[k * PL + tid] = sw ? an : c;
-                            u2[k * PL + tid] = sw ? cn : 0.0;
-                            y[k] = sw ? r1 : r0;
-                            r0 = sw ? r0 - mult * r1 : r1 - mult * r0;
-                            const double an2 = sw ? c - mult * an : an - mult * c;
-                            c = sw ? -mult * cn : cn;
-                            a = an2;
-                            ur[k * PL + tid] = fabs(piv) < ptol ? (piv < 0.0 ? -bigr : bigr) : inv;
+                        double r0 = y[0];
+                        if (replay) {
+                            for (int k = 0; k < N - 1; k++) {
+                                const double r1 = y[k + 1], mult = lm[k * PL + tid];
+                                const bool sw = lsw[k * PL + tid] != 0;
+                                y[k] = sw ? r1 : r0;
+                                r0 = sw ? r0 - mult * r1 : r1 - mult * r0;
+                            }
                         }
-                        ur[(N - 1) * PL + tid] = fabs(a) < ptol ? (a < 0.0 ? -bigr : bigr) : rcp_nr(a);
+                        else {
+                            const double lam = lamv[j];
+                            double a = dg[0] - lam, c = od[0];
+                            for (int k = 0; k < N - 1; k++) {
+                                const double b = od[k], an = dg[k + 1] - lam, cn = k + 1 < N - 1 ? od[k + 1] : 0.0;
+                                const double r1 = y[k + 1];
+                                // partial pivoting: rows k and k + 1 interchanged when |b| > |a|
+                                const bool sw = fabs(a) < fabs(b);
+                                const double piv = sw ? b : a, oth = sw ? a : b;
+                                const double inv = fabs(piv) >= 2.2250738585072014e-308 ? rcp_nr(piv) : 0.0;
+                                const double mult = oth * inv;  // |mult| <= 1; a = b = 0 gives 0
+                                lm[k * PL + tid] = mult;
+                                lsw[k * PL + tid] = sw ? 1 : 0;
+                                u1[k * PL + tid] = sw ? an : c;
+                                u2[k * PL + tid] = sw ? cn : 0.0;
+                                y[k] = sw ? r1 : r0;
+                                r0 = sw ? r0 - mult * r1 : r1 - mult * r0;
+                                const double an2 = sw ? c - mult * an : an - mult * c;
+                                c = sw ? -mult * cn : cn;
+                                a = an2;
+                                ur[k * PL + tid] = fabs(piv) < ptol ? (piv < 0.0 ? -bigr : bigr) : inv;
+                            }
+                            ur[(N - 1) * PL + tid] = fabs(a) < ptol ? (a < 0.0 ? -bigr : bigr) : rcp_nr(a);
+                        }
                         double y1 = r0 * ur[(N - 1) * PL + tid], y2 = 0.0;
                         y[N - 1] = y1;
                         for (int k = N - 2; k >= 0; k--) {  // back substitution, U has two superdiagonals
