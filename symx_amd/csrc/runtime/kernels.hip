@@ -689,8 +689,10 @@ namespace symx_hip
     template <int CTRL>
     __device__ __forceinline__ double dpp_mov(double v)
     {
-        const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+        // quad permutes and row rotations read a valid lane everywhere, so no "old" value is
+        // needed (update_dpp with old = 0 would cost two extra moves per call)
+        const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
         return __hiloint2double(hi, lo);
     }
     __device__ __forceinline__ double readlane_d(double v, int l)
@@ -1322,16 +1324,17 @@ namespace symx_hip
                 xr[q] = x[ti + 16 * q];
                 xc[q] = x[tj + 16 * q];
             }
-            double ss = (lane >= k + 2 ? xl0 * xl0 : 0.0) + (lane + 64 >= k + 2 && lane + 64 < N ? xl1 * xl1 : 0.0);
+            // (entries at N and beyond stay zero)
+            double ss = (lane >= k + 2 ? xl0 * xl0 : 0.0) + (lane + 64 >= k + 2 ? xl1 * xl1 : 0.0);
             ss = wave_sum_all(ss);
             double beta = x0, t = 0.0, invd = 0.0;
             if (ss != 0.0) {
                 beta = -copysign(sqrt(x0 * x0 + ss), x0);
-                t = (beta - x0) / beta;
-                invd = 1.0 / (x0 - beta);
+                t = (beta - x0) * rcp_nr(beta);
+                invd = rcp_nr(x0 - beta);
             }
             // v: 0 up to k, 1 at k + 1, x invd below
-            auto vsel = [&](int i, double xi) { return i == k + 1 ? 1.0 : (i > k + 1 && i < N ? xi * invd : 0.0); };
+            auto vsel = [&](int i, double xi) { return i == k + 1 ? 1.0 : (i > k + 1 ? xi * invd : 0.0); };
             if (wid == 0) {
                 if (lane >= k + 2 && lane < N) V[voff(k) + lane - k - 2] = xl0 * invd;
                 if (lane + 64 >= k + 2 && lane + 64 < N) V[voff(k) + lane + 64 - k - 2] = xl1 * invd;
@@ -1354,20 +1357,21 @@ namespace symx_hip
                 if (tj == 0) {
 #pragma unroll
                     for (int q = Q0; q < TQ; q++)
-                        if (ti + 16 * q < N) pb[ti + 16 * q] = p[q];
+                        if (q < TQ - 1 || ti + 16 * q < N) pb[ti + 16 * q] = p[q];
                 }
                 __syncthreads();
                 const double pl0 = pb[lane], pl1 = pb[lane + 64];
 #pragma unroll
                 for (int q = Q0; q < TQ; q++) pc[q] = pb[tj + 16 * q];
                 const double vl0 = vsel(lane, xl0), vl1 = vsel(lane + 64, xl1);
-                double kk = (lane > k ? pl0 * vl0 : 0.0) + (lane + 64 > k && lane + 64 < N ? pl1 * vl1 : 0.0);
+                double kk = (lane > k ? pl0 * vl0 : 0.0) + (lane + 64 > k ? pl1 * vl1 : 0.0);
                 kk = 0.5 * t * wave_sum_all(kk);
 #pragma unroll
                 for (int q = Q0; q < TQ; q++) {
                     const int r = ti + 16 * q, c = tj + 16 * q;
-                    p[q] = r > k && r < N ? p[q] - kk * vr[q] : 0.0;    // w, own rows
-                    pc[q] = c > k && c < N ? pc[q] - kk * vc[q] : 0.0;  // w, own columns
+                    // padded rows and columns (N and beyond) hold zeros throughout
+                    p[q] = r > k ? p[q] - kk * vr[q] : 0.0;    // w, own rows
+                    pc[q] = c > k ? pc[q] - kk * vc[q] : 0.0;  // w, own columns
                 }
 #pragma unroll
                 for (int q = Q0; q < TQ; q++)
@@ -1382,11 +1386,12 @@ namespace symx_hip
                 L.od[k] = beta;
                 L.tau[k] = t;
             }
-            // column k + 1 (rows below it; in column group Q0) for the next step
+            // column k + 1 (rows below it; in column group Q0, row groups q >= Q0) for the
+            // next step; only the last row group runs past N
             if (tj == ((k + 1) & 15)) {
 #pragma unroll
-                for (int q = 0; q < TQ; q++)
-                    if (ti + 16 * q < N) xb[(k + 1) & 1][ti + 16 * q] = a[q][Q0];
+                for (int q = Q0; q < TQ; q++)
+                    if (q < TQ - 1 || ti + 16 * q < N) xb[(k + 1) & 1][ti + 16 * q] = a[q][Q0];
             }
             __syncthreads();
         };
