@@ -1185,51 +1185,55 @@ namespace symx_hip
                                                         [&](int i, int k) { return A[i * LD + k]; });
     }
 
-    // Sum over the 16 lanes of a DPP row (row rotations by 8, 4, 2, 1): every lane of the row
-    // gets the same, bitwise equal sum (each step adds two equal pairs in swapped order). All
-    // lanes must be active.
-    __device__ inline double row16_sum(double v)
+    // Sum over the 8 lanes of a half DPP row (quad swaps, then the half-row mirror): every
+    // lane of the group gets the same, bitwise equal sum (each step adds two equal operands in
+    // swapped order). All lanes must be active.
+    __device__ inline double grp8_sum(double v)
     {
-        v += dpp_mov<0x128>(v);  // row_ror:8
-        v += dpp_mov<0x124>(v);  // row_ror:4
-        v += dpp_mov<0x122>(v);  // row_ror:2
-        v += dpp_mov<0x121>(v);  // row_ror:1
+        v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+        v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+        v += dpp_mov<0x141>(v);  // row_half_mirror
         return v;
     }
 
     // project_large_one for the upper-block storage of large elements (Hex27: 81 x 81) with
     // the matrix in registers during the tridiagonalisation instead of in LDS. Thread
-    // (ti, tj) of a 16 x 16 grid (ti = 4 wave + lane / 16, so the 16 threads of a DPP row share
-    // ti; tj = lane % 16) holds rows ti + 16 q and columns tj + 16 s (q, s < TQ) in a[q][s]:
-    //  - A22 v is a 36-FMA partial product per thread summed over its DPP row (no barrier);
-    //  - the rank-2 update touches only q, s >= (k + 1) / 16, the row and column groups that
-    //    still reach the active block, so the work shrinks with it;
+    // (ti, tj) of a 32 x 8 grid (tj = lane % 8, ti = 8 wave + lane / 8, so the 8 threads of a
+    // half DPP row share ti) holds rows ti + 32 q (q < TR) and columns tj + 8 s (s < TC) in
+    // a[q][s]:
+    //  - A22 v is a partial product per thread summed over its 8 lanes (no barrier);
+    //  - the step is specialised at compile time on the first column group s0 = (k + 1) / 8
+    //    that still reaches the active block (and the row group q0 = s0 / 4), so the products
+    //    and the rank-2 update shrink with the block;
     //  - two barriers per column (p, then the next column), against three LDS round trips per
     //    column entry in the LDS form; the Householder vectors go to a packed LDS array for
     //    the back-transform (25 KB instead of the 52 KB matrix), three workgroups per CU.
+    // The kernel is bound by VALU issue, hence the narrow row groups: the per-column fixed
+    // cost (two wave sums, the Householder scalars, selects) is paid by every wave, the
+    // group sums cost 3 DPP steps per own row.
     // The arithmetic is that of project_large_one (dsytd2), summed in another order.
     template <int N, typename T, int KMAX, int PL>
     __device__ bool project_tiled_one(T* __restrict__ h, int64_t e, double eps, int mirror, unsigned long long* count,
                                       uint8_t* __restrict__ changed_out)
     {
-        constexpr int PT = 256, NW = PT / 64, TQ = (N + 15) / 16, NB = N / 3;
+        constexpr int PT = 256, NW = PT / 64, TI = 32, TJ = 8, TR = (N + TI - 1) / TI, TC = (N + TJ - 1) / TJ, NB = N / 3;
         constexpr int NV = (N - 1) * (N - 2) / 2;  // column k holds rows k + 2 .. N - 1
-        static_assert(N % 3 == 0 && N <= 128 && TQ * 16 >= N, "tile bounds");
+        static_assert(N % 3 == 0 && N <= 128 && TR * TI >= N && TC * TJ >= N && TI == 4 * TJ, "tile bounds");
         __shared__ double V[NV];
         __shared__ double xb[2][128];
         __shared__ double pb[128];
         __shared__ double red[2 * NW];
         __shared__ ProjTail<N, PT, KMAX, PL> L;
         const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-        const int ti = wid * 4 + (lane >> 4), tj = lane & 15;
+        const int ti = wid * 8 + (lane >> 3), tj = lane & 7;
         auto ubi = [](int a, int b) { return a * NB - (a * (a - 1)) / 2 + (b - a); };
         auto voff = [](int k) { return k * (N - 2) - (k * (k - 1)) / 2; };
-        double a[TQ][TQ];
+        double a[TR][TC];
 #pragma unroll
-        for (int q = 0; q < TQ; q++) {
+        for (int q = 0; q < TR; q++) {
 #pragma unroll
-            for (int s = 0; s < TQ; s++) {
-                const int r = ti + 16 * q, c = tj + 16 * s;
+            for (int s = 0; s < TC; s++) {
+                const int r = ti + TI * q, c = tj + TJ * s;
                 double v = 0.0;
                 if (r < N && c < N) {
                     const int A_ = r / 3, i = r - 3 * A_, B_ = c / 3, j = c - 3 * B_;
@@ -1250,20 +1254,20 @@ namespace symx_hip
         // ---- 1. scale, translation invariance (block-row sums per component) ---------------
         double amax = 0.0, rsum = 0.0;
 #pragma unroll
-        for (int q = 0; q < TQ; q++) {
+        for (int q = 0; q < TR; q++) {
             double ps0 = 0.0, ps1 = 0.0, ps2 = 0.0;
 #pragma unroll
-            for (int s = 0; s < TQ; s++) {
-                const int c = tj + 16 * s, cm = c % 3;
+            for (int s = 0; s < TC; s++) {
+                const int cm = (tj + TJ * s) % 3;
                 amax = fmax(amax, fabs(a[q][s]));
                 ps0 += cm == 0 ? a[q][s] : 0.0;
                 ps1 += cm == 1 ? a[q][s] : 0.0;
                 ps2 += cm == 2 ? a[q][s] : 0.0;
             }
-            ps0 = row16_sum(ps0);
-            ps1 = row16_sum(ps1);
-            ps2 = row16_sum(ps2);
-            if (ti + 16 * q < N) rsum = fmax(rsum, fmax(fabs(ps0), fmax(fabs(ps1), fabs(ps2))));
+            ps0 = grp8_sum(ps0);
+            ps1 = grp8_sum(ps1);
+            ps2 = grp8_sum(ps2);
+            if (ti + TI * q < N) rsum = fmax(rsum, fmax(fabs(ps0), fmax(fabs(ps1), fabs(ps2))));
         }
         const double scale = block_max(amax);
         const bool tinv = block_max(rsum) <= translation_tol<T>() * scale * NB && scale > 0.0;
@@ -1280,24 +1284,26 @@ namespace symx_hip
         if (tinv) {
             const double sh = scale / NB;
 #pragma unroll
-            for (int q = 0; q < TQ; q++)
+            for (int q = 0; q < TR; q++)
 #pragma unroll
-                for (int s = 0; s < TQ; s++) {
-                    const int r = ti + 16 * q, c = tj + 16 * s;
+                for (int s = 0; s < TC; s++) {
+                    const int r = ti + TI * q, c = tj + TJ * s;
                     if (r < N && c < N && (r % 3) == (c % 3)) a[q][s] += sh;
                 }
         }
         // ---- 2. Householder tridiagonalisation in registers --------------------------------
-        // reg(q, s): a[q][s] for a wave-uniform s
-        auto col_of = [&](int q, int sc) {
+        // a[q][s] at run-time q, s (wave-uniform)
+        auto elem_of = [&](int qc, int sc) {
             double v = 0.0;
 #pragma unroll
-            for (int s = 0; s < TQ; s++)
-                if (s == sc) v = a[q][s];
+            for (int q = 0; q < TR; q++)
+#pragma unroll
+                for (int s = 0; s < TC; s++)
+                    if (q == qc && s == sc) v = a[q][s];
             return v;
         };
-        // xb / pb entries are read without bounds tests (lane + 64 < 128) and masked by
-        // selects, so they start as zeros
+        // xb / pb entries are read without bounds tests (lane + 64 < 128); entries at N and
+        // beyond stay zero, those above the active block are masked by selects
         for (int i = tid; i < 128; i += PT) {
             xb[0][i] = 0.0;
             xb[1][i] = 0.0;
@@ -1306,25 +1312,24 @@ namespace symx_hip
         __syncthreads();
         if (tj == 0) {
 #pragma unroll
-            for (int q = 0; q < TQ; q++)
-                if (ti + 16 * q < N) xb[0][ti + 16 * q] = a[q][0];
+            for (int q = 0; q < TR; q++)
+                if (ti + TI * q < N) xb[0][ti + TI * q] = a[q][0];
         }
         __syncthreads();
-        // Column k of the reduction, for q0 = (k + 1) / 16 fixed at compile time: the groups
-        // q, s < q0 lie wholly above row k + 1 and drop out of every loop.
-        auto step = [&](auto q0c, int k) {
-            constexpr int Q0 = decltype(q0c)::value;
+        // Column k of the reduction, for s0 = (k + 1) / TJ fixed at compile time: the column
+        // groups s < s0 and the row groups q < q0 = s0 / 4 lie wholly above row k + 1 and drop
+        // out of every loop.
+        auto step = [&](auto s0c, int k) {
+            constexpr int S0 = decltype(s0c)::value, Q0 = S0 * TJ / TI;
             // every LDS read of the step's first half in one batch: column k (rows k + 1 ..
             // N - 1) at the lanes (for the norm), the own rows and columns, and x_{k+1}
             const double* x = xb[k & 1];
             const double xl0 = x[lane], xl1 = x[lane + 64], x0 = x[k + 1];
-            double xr[TQ], xc[TQ];
+            double xr[TR], xc[TC];
 #pragma unroll
-            for (int q = Q0; q < TQ; q++) {
-                xr[q] = x[ti + 16 * q];
-                xc[q] = x[tj + 16 * q];
-            }
-            // (entries at N and beyond stay zero)
+            for (int q = Q0; q < TR; q++) xr[q] = x[ti + TI * q];
+#pragma unroll
+            for (int s = S0; s < TC; s++) xc[s] = x[tj + TJ * s];
             double ss = (lane >= k + 2 ? xl0 * xl0 : 0.0) + (lane + 64 >= k + 2 ? xl1 * xl1 : 0.0);
             ss = wave_sum_all(ss);
             double beta = x0, t = 0.0, invd = 0.0;
@@ -1340,83 +1345,76 @@ namespace symx_hip
                 if (lane + 64 >= k + 2 && lane + 64 < N) V[voff(k) + lane + 64 - k - 2] = xl1 * invd;
             }
             if (t != 0.0) {
-                double vr[TQ], vc[TQ], p[TQ], pc[TQ];
+                double vr[TR], vc[TC], p[TR], pc[TC];
 #pragma unroll
-                for (int q = Q0; q < TQ; q++) {
-                    vr[q] = vsel(ti + 16 * q, xr[q]);
-                    vc[q] = vsel(tj + 16 * q, xc[q]);
-                }
+                for (int q = Q0; q < TR; q++) vr[q] = vsel(ti + TI * q, xr[q]);
+#pragma unroll
+                for (int s = S0; s < TC; s++) vc[s] = vsel(tj + TJ * s, xc[s]);
                 // p = t A22 v for the own rows
 #pragma unroll
-                for (int q = Q0; q < TQ; q++) {
-                    double acc = a[q][Q0] * vc[Q0];
+                for (int q = Q0; q < TR; q++) {
+                    double acc = a[q][S0] * vc[S0];
 #pragma unroll
-                    for (int s = Q0 + 1; s < TQ; s++) acc += a[q][s] * vc[s];
-                    p[q] = t * row16_sum(acc);
+                    for (int s = S0 + 1; s < TC; s++) acc += a[q][s] * vc[s];
+                    p[q] = t * grp8_sum(acc);
                 }
                 if (tj == 0) {
 #pragma unroll
-                    for (int q = Q0; q < TQ; q++)
-                        if (q < TQ - 1 || ti + 16 * q < N) pb[ti + 16 * q] = p[q];
+                    for (int q = Q0; q < TR; q++)
+                        if (q < TR - 1 || ti + TI * q < N) pb[ti + TI * q] = p[q];
                 }
                 __syncthreads();
                 const double pl0 = pb[lane], pl1 = pb[lane + 64];
 #pragma unroll
-                for (int q = Q0; q < TQ; q++) pc[q] = pb[tj + 16 * q];
+                for (int s = S0; s < TC; s++) pc[s] = pb[tj + TJ * s];
                 const double vl0 = vsel(lane, xl0), vl1 = vsel(lane + 64, xl1);
                 double kk = (lane > k ? pl0 * vl0 : 0.0) + (lane + 64 > k ? pl1 * vl1 : 0.0);
                 kk = 0.5 * t * wave_sum_all(kk);
+                // w = p - kk v (padded rows and columns, N and beyond, hold zeros throughout)
 #pragma unroll
-                for (int q = Q0; q < TQ; q++) {
-                    const int r = ti + 16 * q, c = tj + 16 * q;
-                    // padded rows and columns (N and beyond) hold zeros throughout
-                    p[q] = r > k ? p[q] - kk * vr[q] : 0.0;    // w, own rows
-                    pc[q] = c > k ? pc[q] - kk * vc[q] : 0.0;  // w, own columns
-                }
+                for (int q = Q0; q < TR; q++) p[q] = ti + TI * q > k ? p[q] - kk * vr[q] : 0.0;
 #pragma unroll
-                for (int q = Q0; q < TQ; q++)
+                for (int s = S0; s < TC; s++) pc[s] = tj + TJ * s > k ? pc[s] - kk * vc[s] : 0.0;
 #pragma unroll
-                    for (int s = Q0; s < TQ; s++) a[q][s] -= vr[q] * pc[s] + p[q] * vc[s];
+                for (int q = Q0; q < TR; q++)
+#pragma unroll
+                    for (int s = S0; s < TC; s++) a[q][s] -= vr[q] * pc[s] + p[q] * vc[s];
             }
-            if (ti == (k & 15) && tj == (k & 15)) {
-                if constexpr (Q0 > 0) L.dg[k] = (k >> 4) == Q0 ? a[Q0][Q0] : a[Q0 - 1][Q0 - 1];
-                else L.dg[k] = a[0][0];
+            // diagonal entry k: column group k / TJ is S0, or S0 - 1 for the phase's first k
+            if (ti == (k & (TI - 1)) && tj == (k & (TJ - 1))) {
+                if constexpr (S0 > 0) {
+                    constexpr int KF = S0 * TJ - 1;  // the phase's first column
+                    L.dg[k] = k == KF ? a[KF / TI][KF / TJ] : a[Q0][S0];
+                }
+                else {
+                    L.dg[k] = a[0][0];
+                }
             }
             if (tid == 0) {
                 L.od[k] = beta;
                 L.tau[k] = t;
             }
-            // column k + 1 (rows below it; in column group Q0, row groups q >= Q0) for the
+            // column k + 1 (rows below it; in column group S0, row groups q >= Q0) for the
             // next step; only the last row group runs past N
-            if (tj == ((k + 1) & 15)) {
+            if (tj == ((k + 1) & (TJ - 1))) {
 #pragma unroll
-                for (int q = Q0; q < TQ; q++)
-                    if (q < TQ - 1 || ti + 16 * q < N) xb[(k + 1) & 1][ti + 16 * q] = a[q][Q0];
+                for (int q = Q0; q < TR; q++)
+                    if (q < TR - 1 || ti + TI * q < N) xb[(k + 1) & 1][ti + TI * q] = a[q][S0];
             }
             __syncthreads();
         };
-        static_for<0, TQ>([&](auto q0c) {
-            constexpr int Q0 = decltype(q0c)::value;
-            const int kb = Q0 == 0 ? 0 : 16 * Q0 - 1;
-            const int ke = 16 * Q0 + 15 < N - 2 ? 16 * Q0 + 15 : N - 2;
-            for (int k = kb; k < ke; k++) step(q0c, k);
+        static_for<0, TC>([&](auto s0c) {
+            constexpr int S0 = decltype(s0c)::value;
+            const int kb = S0 == 0 ? 0 : TJ * S0 - 1;
+            const int ke = TJ * S0 + TJ - 1 < N - 2 ? TJ * S0 + TJ - 1 : N - 2;
+            for (int k = kb; k < ke; k++) step(s0c, k);
         });
         {
             // the last 2 x 2 block: (N-2, N-2), (N-1, N-2), (N-1, N-1)
             const int r0 = N - 2, r1 = N - 1;
-            if (tj == (r0 & 15)) {
-#pragma unroll
-                for (int q = 0; q < TQ; q++) {
-                    const int r = ti + 16 * q;
-                    if (r == r0) L.dg[r0] = col_of(q, r0 >> 4);
-                    if (r == r1) L.od[r0] = col_of(q, r0 >> 4);
-                }
-            }
-            if (tj == (r1 & 15)) {
-#pragma unroll
-                for (int q = 0; q < TQ; q++)
-                    if (ti + 16 * q == r1) L.dg[r1] = col_of(q, r1 >> 4);
-            }
+            if (ti == (r0 % TI) && tj == (r0 % TJ)) L.dg[r0] = elem_of(r0 / TI, r0 / TJ);
+            if (ti == (r1 % TI) && tj == (r0 % TJ)) L.od[r0] = elem_of(r1 / TI, r0 / TJ);
+            if (ti == (r1 % TI) && tj == (r1 % TJ)) L.dg[r1] = elem_of(r1 / TI, r1 / TJ);
             if (tid == 0) {
                 L.od[N - 1] = 0.0;
                 L.tau[N - 2] = 0.0;
