@@ -842,36 +842,60 @@ namespace symx_hip
         }
         if (cnt > KMAX) return true;
         // ---- 4. multisection for the cnt lowest eigenvalues ------------------------------
+        // A round costs the instructions of every wave that holds a point. Up to 8 eigenvalues
+        // (the rule for element Hessians) take all points of a round in wave 0, one per lane
+        // (64 / cnt per eigenvalue, at least 3.2 bits a round): fewer wave-rounds than more
+        // points over more waves. Larger counts take two points per thread over the workgroup.
         {
-            const double w0 = (cut - glo) / (NPT + 1);
-            const int p0 = tid * PPTS;
-            int c0, c1;
-            sturm2<N>(dg, od, glo + w0 * (p0 + 1), glo + w0 * (p0 + 2), pivmin, c0, c1);
-            cbuf[p0] = c0;
-            cbuf[p0 + 1] = c1;
+            const bool one_wave = cnt <= 8;
+            const int NP0 = one_wave ? 64 : NPT;
+            const double w0 = (cut - glo) / (NP0 + 1);
+            if (one_wave) {
+                if (tid < 64) cbuf[tid] = sturm1<N>(dg, od, glo + w0 * (tid + 1), pivmin);
+            }
+            else {
+                const int p0 = tid * PPTS;
+                int c0, c1;
+                sturm2<N>(dg, od, glo + w0 * (p0 + 1), glo + w0 * (p0 + 2), pivmin, c0, c1);
+                cbuf[p0] = c0;
+                cbuf[p0 + 1] = c1;
+            }
             if (tid == 0) busy[0] = 0;
             __syncthreads();
             for (int j = tid; j < cnt; j += PT) {  // first point with more than j eigenvalues below it
-                int lo = 0, hi = NPT;
+                int lo = 0, hi = NP0;
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
                     if (cbuf[mid] > j) hi = mid;
                     else lo = mid + 1;
                 }
                 bra[j] = lo > 0 ? glo + w0 * lo : glo;
-                brb[j] = lo < NPT ? glo + w0 * (lo + 1) : cut;
+                brb[j] = lo < NP0 ? glo + w0 * (lo + 1) : cut;
             }
             __syncthreads();
-            // later rounds: one point per thread, P per eigenvalue still above tol (the round's
-            // cost is the instructions of every wave holding a point, so the points go one to
-            // a lane: 192 points for three eigenvalues occupy three waves, not two at twice
-            // the length)
-            const int P = min(64, PT / cnt);
-            for (int round = 0; round < 16; round++) {
-                {
+            // later rounds: P points per eigenvalue still above tol
+            const int P = min(64, (one_wave ? 64 : NPT) / cnt);
+            for (int round = 0; round < 20; round++) {
+                if (one_wave) {
                     const int j = tid / P, l = tid - j * P;
-                    if (j < cnt && brb[j] - bra[j] > tol)
+                    if (tid < 64 && j < cnt && brb[j] - bra[j] > tol)
                         cbuf[tid] = sturm1<N>(dg, od, bra[j] + (brb[j] - bra[j]) * ((double)(l + 1) / (P + 1)), pivmin);
+                }
+                else {
+                    int pts[PPTS];
+                    double xs[PPTS];
+#pragma unroll
+                    for (int q = 0; q < PPTS; q++) {
+                        const int p = tid * PPTS + q, j = p / P, l = p - j * P;
+                        pts[q] = j < cnt ? p : -1;
+                        xs[q] = j < cnt ? bra[j] + (brb[j] - bra[j]) * ((double)(l + 1) / (P + 1)) : 0.0;
+                    }
+                    if (pts[0] >= 0) {
+                        int c0, c1;
+                        sturm2<N>(dg, od, xs[0], xs[1], pivmin, c0, c1);
+                        cbuf[pts[0]] = c0;
+                        if (pts[1] >= 0) cbuf[pts[1]] = c1;
+                    }
                 }
                 if (tid == 0) busy[(round + 1) % 3] = 0;
                 __syncthreads();
