@@ -58,6 +58,13 @@ namespace symx_hip
     // Position = row unless sorting the rows by length inside windows of kSellSigma
     // rows removes enough padding (SELL-C-sigma); then row_at[pos] names the row.
     constexpr int32_t kSellSigma = 1024;
+    // SpMV and PCG-update workgroup size. The grid (pcg_grid) is a fixed set of reduction
+    // workgroups, each owning a contiguous chunk of positions; wave v of a workgroup takes
+    // slices v, v + 8, ... of its chunk. 512 threads keep 8 waves per SIMD in flight; a
+    // streaming read reaches 7.3 TB/s with 8 waves per SIMD and 5.5 TB/s with 4.
+    constexpr int kSpmvThreads = 512;
+    // Workgroups of the PCG kernels for n_rows block rows.
+    int pcg_grid(int32_t n_rows);
     struct SellPattern
     {
         int32_t n_rows = 0;

@@ -96,10 +96,7 @@ namespace symx_hip
         if constexpr (sizeof(CT) == 2) return row + (int64_t)stored;
         else return (int64_t)stored;
     }
-    // SpMV and PCG-update workgroup size. The grid is the fixed set of at most MAX_GROUPS reduction
-    // workgroups; 512 threads keep 8 waves per SIMD in flight (the kernel needs ~56
-    // VGPRs). A streaming read reaches 7.3 TB/s with 8 waves per SIMD and 5.5 TB/s with 4.
-    constexpr int kSpmvThreads = 512;
+    // SpMV and PCG-update workgroup size: kSpmvThreads (internal.h).
     // Mean blocks per row from which the search direction gets a pass of its own.
     constexpr int kDirectionPassMeanRow = 32;
     // Matrices above this many value bytes are streamed (the MI355X infinity cache is 256 MiB).

@@ -839,6 +839,52 @@ int symx_hip_problem_set_map_data(symx_hip_problem p, int32_t pot_id, int32_t ma
 // slot-sorted Hessian contribution lists. All orders are stable in
 // (potential, element, local block row, local block col), which fixes the
 // summation order of every output entry.
+// Rows sorted by length give slices of very different widths (27-node hexahedra: 27 to 125
+// blocks), and the SpMV's fixed schedule (wave v of a workgroup takes slices v, v + 8, ... of
+// its chunk) then loads some waves with two wide slices while others hold one narrow one:
+// the kernel lasts as long as its most loaded wave (2.3x the mean on a 40^3 Hex27 cube).
+// Within each workgroup chunk the whole slices are reordered so that, taken largest first,
+// each goes to the least loaded wave that still has a slot, ties to waves with fewer slots
+// (1.7x the mean there). Only the order of slices inside a chunk changes.
+static void balance_slices(std::vector<int32_t>& order, const std::vector<int32_t>& row_len, int32_t nb)
+{
+    const int G = pcg_grid(nb);
+    const int64_t per = (((int64_t)nb + G - 1) / G + 63) / 64 * 64;
+    const int nw = kSpmvThreads / 64;
+    const int64_t full = nb / 64;  // slices wholly inside the matrix (a last partial one stays last)
+    std::vector<int32_t> width(full);
+    for (int64_t sl = 0; sl < full; sl++) {
+        int32_t w = 0;
+        for (int64_t q = 64 * sl; q < 64 * sl + 64; q++) w = std::max(w, row_len[order[q]]);
+        width[sl] = w;
+    }
+    std::vector<int32_t> tmp;
+    for (int64_t c0 = 0; c0 < nb; c0 += per) {
+        const int64_t s0 = c0 / 64, s1 = std::min<int64_t>((c0 + per) / 64, full);
+        const int64_t m = s1 - s0;
+        if (m <= 1) continue;
+        std::vector<int64_t> by_width(m);
+        for (int64_t i = 0; i < m; i++) by_width[i] = s0 + i;
+        std::stable_sort(by_width.begin(), by_width.end(), [&](int64_t a, int64_t b) { return width[a] > width[b]; });
+        std::vector<int64_t> load(nw, 0), next(nw), assign(m, -1);
+        for (int v = 0; v < nw; v++) next[v] = v;  // next free slot of wave v: v, v + nw, ...
+        auto slots_left = [&](int v) { return next[v] < m ? (m - 1 - next[v]) / nw + 1 : 0; };
+        for (int64_t sl : by_width) {
+            int best = -1;
+            for (int v = 0; v < nw; v++) {
+                if (next[v] >= m) continue;
+                if (best < 0 || load[v] < load[best] || (load[v] == load[best] && slots_left(v) < slots_left(best))) best = v;
+            }
+            assign[next[best]] = sl;
+            next[best] += nw;
+            load[best] += width[sl];
+        }
+        tmp.assign(order.begin() + 64 * s0, order.begin() + 64 * s1);
+        for (int64_t i = 0; i < m; i++)
+            std::copy(tmp.begin() + 64 * (assign[i] - s0), tmp.begin() + 64 * (assign[i] - s0) + 64, order.begin() + 64 * (s0 + i));
+    }
+}
+
 int symx_hip_problem_finalize(symx_hip_problem p)
 {
     if (check_problem(p)) return SYMX_HIP_ERR_ARG;
@@ -945,6 +991,7 @@ int symx_hip_problem_finalize(symx_hip_problem p)
             std::stable_sort(order.begin() + w0, order.begin() + std::min(nb, w0 + kSellSigma),
                              [&](int32_t a, int32_t b) { return row_len[a] > row_len[b]; });
         if (20 * slots_of(order.data()) <= 19 * slots_of(nullptr)) {
+            balance_slices(order, row_len, nb);
             S.h_pos.assign(nb, 0);
             for (int32_t q = 0; q < nb; q++) S.h_pos[order[q]] = q;
             S.h_row_at = std::move(order);
