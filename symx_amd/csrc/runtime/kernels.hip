@@ -1522,10 +1522,11 @@ namespace symx_hip
         case 8: k_project_large<24, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select, nullptr, nullptr); return 0;
         case 10: k_project_large<30, false, T><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select, nullptr, nullptr); return 0;
         case 27:
-            // 81x81: eigenvectors for at most 8 eigenvalues below eps and four inverse-iteration
-            // lanes keep the LDS at 73 KB, two matrices per CU (the full-size variant takes
-            // 143 KB, one per CU); Hex27 elements have at most 3 such eigenvalues as a rule, the
-            // others are listed and projected by the full-size variant
+            // 81x81: the matrix in registers during the tridiagonalisation, eigenvectors for at
+            // most 8 eigenvalues below eps and four inverse-iteration lanes keep the LDS at 50 KB,
+            // three matrices per CU (the full-size LDS variant takes 155 KB, one per CU); Hex27
+            // elements have at most 3 such eigenvalues as a rule, the others are listed and
+            // projected by the full-size variant
             k_project_tiled<81, T, 8, 4><<<(unsigned)n_mats, PNT, 0, s>>>(H, n_mats, rs, eps, mirror, d_count, select, flags);
             k_zero_counter<<<1, 1, 0, s>>>(n_list);
             k_list_flagged<<<grid_for(n_mats), NT, 0, s>>>(flags, n_mats, list, n_list);
