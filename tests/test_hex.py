@@ -185,6 +185,34 @@ def test_hex_projection_matches_oracle(gpu, oracle, nn, mirror):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("noise,squash", [(0.05, 1.0), (0.08, 1.0), (0.15, 0.8), (0.15, 0.5)])
+def test_hex27_projection_of_compressed_states_matches_oracle(gpu, oracle, noise, squash):
+    """81x81 projection over a range of negative-eigenvalue counts: node noise 0.05 and 0.08
+    give 0 to 10 negative eigenvalues per element (the register-tiled kernel's multisection,
+    inverse iteration and rank-k update; above 8 the element goes to the full-size list
+    kernel), the compressed states (z scaled by 0.8 and 0.5) 15 to 26 (all to the list
+    kernel after the tiled tridiagonalisation and count). Every element matches the oracle's
+    Jacobi projection (FP64 storage, clamp)."""
+    s, p = hex_scene(27, noise=noise)
+    S = hex_block(cells=(2, 2, 2), nn=27, noise=noise, seed=p.seed)
+    x = S.x.copy()
+    x[:, 2] *= squash
+    s.set_dofs(x)
+    S.x[...] = x
+    ne = S.tets.shape[0]
+    s.eval(symx_amd.EVAL_PGH)
+    s.project(1e-10, False)
+    _, Hub = s.element_outputs(0, ne, 27)
+    H = upper_blocks_to_full(Hub, 27)
+    H0 = oracle.element_hessians(S)
+    Ho, _ = oracle.project(H0, 1e-10, False)
+    counts = [int((np.linalg.eigvalsh(H0[e]) < -1e-13 * np.abs(H0[e]).max()).sum()) for e in range(ne)]
+    assert max(counts) > 0, counts
+    for e in range(ne):
+        assert tol_rel(H[e], Ho[e]) < 1e-9, (e, counts[e])
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("nn,mode", [(8, 1), (10, 1), (10, 2), (10, 3), (27, 1), (27, 2)])
 def test_hex_newton_matches_oracle(gpu, oracle, nn, mode):
     s, p = hex_scene(nn)
