@@ -778,7 +778,7 @@ namespace symx
         return os.str();
     }
 
-    int32_t hessian_record_stride(int32_t n_values) { return n_values > 16 ? (n_values + 31) / 32 * 32 : n_values; }
+    int32_t hessian_record_stride(int32_t n_values) { return n_values > 16 ? (n_values + 95) / 96 * 96 : n_values; }
 
     int64_t count_operations(Graph& g, const std::vector<Scalar>& outputs)
     {

@@ -316,8 +316,10 @@ namespace symx
 
     // Values per element Hessian record written by the single-point (chunked) PGH kernels
     // for a Hessian of n_values upper-block values: records longer than one 128-byte chunk
-    // of doubles are padded to a multiple of 32 values, so that every chunk the kernels
-    // store (32 floats or 16 doubles) is one aligned 128-byte line.
+    // of doubles are padded to a multiple of 96 values: a multiple of 32, so that every chunk
+    // the kernels store (32 floats or 16 doubles) is one aligned 128-byte line, and of 3, as the
+    // runtime addresses records in 3-value units (2 and 3 blocks: 27 and 54 values -> 96; 4 blocks:
+    // 90 -> 96).
     int32_t hessian_record_stride(int32_t n_values);
 
     // Number of SSA operations that a kernel with these outputs executes (for reports).

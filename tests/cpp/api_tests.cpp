@@ -12,6 +12,7 @@
 //                        and scalar DoF containers)
 //   loop_triangle_area   tests/test_loops.cpp:58-233 (CompiledInLoop against closed-form
 //                        triangle areas: summation, activity mask, element counts)
+//   empty_potential      a potential with no elements beside a populated one (same solve)
 // Each case prints "PASS <name>" or "FAIL <name>: <reason>"; the exit status is the
 // number of failed cases.
 #include <array>
@@ -266,6 +267,40 @@ namespace
         require(G2->get_total_n_dofs() == 1, "scalar DoF set");
     }
 
+    // A potential added with an empty connectivity (no elements) contributes nothing: the solve
+    // takes the same Newton and CG iterations to the same DoFs as without it.
+    void empty_potential()
+    {
+        const std::vector<std::array<double, 3>> t = { { 0.5, -0.25, 1.0 }, { 2.0, 0.0, -1.0 }, { -3.0, 0.125, 7.0 } };
+        auto solve = [&](bool with_empty, std::vector<std::array<double, 3>>& x) {
+            x = { { 0.0, 0.0, 0.0 }, { 1.0, 1.0, 1.0 }, { 2.0, 2.0, 2.0 } };
+            spGlobalPotential G = GlobalPotential::create();
+            std::vector<std::array<int32_t, 2>> pairs = { { 0, 1 }, { 1, 2 } }, none;
+            auto energy = [&](MappedWorkspace<double>& mws, Element& el) {
+                Vector a = mws.make_vector(x, el[0]), b = mws.make_vector(x, el[1]);
+                Vector ta = mws.make_vector(t, el[0]), tb = mws.make_vector(t, el[1]);
+                return 0.5 * (a - ta).squared_norm() + 0.5 * (b - tb).squared_norm() + ((a - b) - (ta - tb)).squared_norm().powN(2);
+            };
+            G->add_potential("pairs", pairs, energy);
+            if (with_empty) G->add_potential("none", none, energy);
+            G->add_dof(x);
+            spContext context = quiet_context();
+            NewtonsMethod newton(G, context);
+            newton.settings.residual_tolerance_abs = 1e-10;
+            const SolverReturn ret = newton.solve();
+            require(ret == SolverReturn::Successful, std::string("solver returned ") + to_string(ret));
+            return std::make_pair(newton.get_last_solve_stats().newton_iterations, newton.get_last_solve_stats().cg_iterations);
+        };
+        std::vector<std::array<double, 3>> x0, x1;
+        const auto it0 = solve(false, x0), it1 = solve(true, x1);
+        require(it0 == it1, "same Newton and CG iteration counts with an empty potential");
+        for (int i = 0; i < 3; i++)
+            for (int d = 0; d < 3; d++) {
+                require(x0[i][d] == x1[i][d], "same DoFs with an empty potential");
+                require(std::abs(x1[i][d] - t[i][d]) <= 1e-6, "DoFs reach the targets");
+            }
+    }
+
     void compiled_ops()
     {
         Workspace ws;
@@ -476,6 +511,7 @@ int main(int argc, char** argv)
         { "compiled_ops", compiled_ops },
         { "loop_triangle_area", loop_triangle_area },
         { "add_dof_overloads", add_dof_overloads },
+        { "empty_potential", empty_potential },
     };
     int failed = 0;
     for (const auto& [name, f] : cases) {
